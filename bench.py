@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 grammar codec on a seeded synthetic pangenome.
+
+One step = grammar-compress the whole batch of traversals (delta + 8 rounds) and
+decompress it again (rule expansion + inverse delta), with the traversals already
+resident in HBM when the timed region starts. ``value`` = input-GFA bytes moved through
+compress and decompress per second: gfa_bytes / (t_compress + t_decompress).
+
+Default workload: ``plain_500k`` - 500,000-site backbone, 64 P-lines, 2-allele bubbles at
+every 10th site, 2% reversed steps, 1% deletions of 1-5 sites, 0.1% tandem duplications of
+50-500 steps (about 39.6M int32 steps, 368 MB of GFA text); rounds=8, threshold=2,
+delta_round=1.
+
+Prints ONE JSON line. ``roofline`` is the dominant kernel group of the timed region
+(HIP events on the codec's own stream); ``cpu_baseline`` is the reference CPU path
+compiled from source (oracle/_ref, when present) or the repo's CPU oracle, timed on this
+host's cores on the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+# Algorithmic bytes per unit of each timed kernel group:
+# expand: 4 E + 4 N + 8 R; the compress kernels per round r: pair_insert 4 L_r,
+# cand_tiles 4 L_r, emit 4 L_r + 4 L_{r+1}, remap 8 L_{r+1}; inverse_delta 8 N.
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(cfg, tr, gfa_bytes: int, threads: int) -> dict:
+    """Reference CPU path (oracle/_ref/gfaz_ref) on the same workload when it was built;
+    otherwise the repo's CPU oracle (codec stage only)."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle
+    from gfaz_amd import synth
+    if oracle.ref_available():
+        with tempfile.TemporaryDirectory(prefix="gfaz_bench_") as tmp:
+            gfa = os.path.join(tmp, "in.gfa")
+            gz = os.path.join(tmp, "in.gfaz")
+            synth.write_gfa(cfg, tr, gfa)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            run = lambda *a: subprocess.run([oracle.REF_BIN, *a], check=True, env=env,
+                                            capture_output=True, text=True).stdout
+            c = json.loads(run("time-compress", gfa, "8", "2", "1", str(threads), "1"))
+            run("compress", gfa, gz, "8", "2", "1", str(threads))
+            d = json.loads(run("time-decompress", gz, str(threads), "1"))
+            gfaz_bytes = os.path.getsize(gz)
+        total = c["seconds"] + d["seconds"]
+        return {"value": gfa_bytes / total / 1e9, "unit": "GB/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"whole workload {cfg.name}: gfaz compress + decompress end to end "
+                          "(parse, grammar, zstd, GFA write to /dev/null)",
+                "compress_s": c["seconds"], "decompress_s": d["seconds"],
+                "compress_gbps": gfa_bytes / c["seconds"] / 1e9,
+                "decompress_gbps": gfa_bytes / d["seconds"] / 1e9,
+                "gfaz_bytes": gfaz_bytes, "ratio": gfa_bytes / gfaz_bytes}
+    t0 = time.perf_counter()
+    r = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=threads)
+    t1 = time.perf_counter()
+    oracle.decompress(r["encoded"], r["encoded_lengths"], r["rules_first"], r["rules_second"],
+                      r["start_id"], 1, tr.n_steps, threads=threads)
+    t2 = time.perf_counter()
+    return {"value": gfa_bytes / (t2 - t0) / 1e9, "unit": "GB/s", "cores": threads,
+            "kind": "port", "sample": f"whole workload {cfg.name}: traversal codec only",
+            "compress_s": t1 - t0, "decompress_s": t2 - t1}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plain_500k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verify-oracle", action="store_true",
+                    help="also compare the encoded grammar with the CPU oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: every GPU box here has a single MI355X "
+              f"(asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    sys.path.insert(0, ROOT)
+    from gfaz_amd import codec, synth
+
+    cfg = synth.CONFIGS[args.workload]
+    log(f"[bench] generating {cfg.name}")
+    tr = synth.generate(cfg)
+    gfa_bytes = synth.gfa_bytes(cfg, tr)
+    log(f"[bench] steps={tr.n_steps} traversals={tr.lengths.size} gfa_bytes={gfa_bytes}")
+
+    ctx = codec.Context(0)
+    d_nodes = codec.DeviceArray.from_host(ctx, tr.nodes)
+    d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
+
+    def step():
+        t0 = time.perf_counter()
+        comp = codec.DeviceCompressed(ctx, d_nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+        ctx.sync()
+        t1 = time.perf_counter()
+        rb = comp.rulebook()
+        codec.decompress_traversals(ctx, rb, comp.encoded_view(), comp.encoded_lengths,
+                                    tr.lengths, 1, out=d_out)
+        ctx.sync()
+        t2 = time.perf_counter()
+        rb.close()
+        return t1 - t0, t2 - t1, comp
+
+    for i in range(args.warmup):
+        log(f"[bench] warmup {i}")
+        step()[2].close()
+
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    tc = td = 0.0
+    last = None
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        c, d, comp = step()
+        tc += c
+        td += d
+        if last is not None:
+            last.close()
+        last = comp
+        log(f"[bench] step {i}: compress {c*1e3:.2f} ms decompress {d*1e3:.2f} ms")
+    ctx.sync()
+    t_total = time.perf_counter() - t_start
+    ctx.set_profiling(False)
+    groups = ["pair_insert", "cand_tiles", "tile_scan", "emit", "collect_used", "radix_sort",
+              "assign_ids", "remap", "delta", "rule_sizes", "expand", "inverse_delta"]
+    kstats = {g: ctx.kernel_stat(g) for g in groups}
+
+    # Correctness of what was timed: exact round trip, and grammar counts.
+    got = d_out.to_host()
+    if not np.array_equal(got, tr.nodes):
+        log("[bench] ROUND TRIP MISMATCH")
+        return 1
+    host = last.to_host()
+    if args.verify_oracle:
+        from oracle import oracle
+        ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=cpu_threads())
+        same = (np.array_equal(ref["encoded"], host.encoded)
+                and np.array_equal(ref["rules_first"], host.rules_first)
+                and np.array_equal(ref["rules_second"], host.rules_second))
+        if not same:
+            log("[bench] ORACLE MISMATCH")
+            return 1
+    last.close()
+
+    N, E, R = tr.n_steps, host.n_encoded, host.n_rules
+    # Roofline of the dominant kernel group.
+    dom = max(kstats, key=lambda g: kstats[g]["total_ms"])
+    ks = kstats[dom]
+    exp = kstats["expand"]
+    exp_bytes = 4.0 * E + 4.0 * N + 8.0 * R
+    roofline = None
+    if ks["launches"]:
+        if dom == "expand":
+            per_launch = exp_bytes
+        else:
+            per_launch = ks["bytes"] / ks["launches"] if ks["bytes"] else None
+        avg_ms = ks["total_ms"] / ks["launches"]
+        if per_launch:
+            ach = per_launch / (avg_ms * 1e-3) / 1e9
+            roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "avg_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch}
+    expand_roof = None
+    if exp["launches"]:
+        avg = exp["total_ms"] / exp["launches"]
+        a = exp_bytes / (avg * 1e-3) / 1e9
+        expand_roof = {"kernel": "expand", "achieved": a, "frac": a / HBM_PEAK_GBS, "avg_ms": avg}
+
+    steps = args.steps
+    ms_step = (tc + td) / steps * 1e3
+    value = gfa_bytes / ((tc + td) / steps) / 1e9
+    out = {
+        "metric": "Compress/decompress GFA GB/s at matched ratio, %HBM peak; extract p50/p99 ms",
+        "value": value, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg.name, "backbone_sites": cfg.backbone,
+                   "traversals": int(tr.lengths.size), "steps": N, "gfa_bytes": gfa_bytes,
+                   "rounds": 8, "threshold": 2, "delta_round": 1},
+        "compress_gbps": gfa_bytes / (tc / steps) / 1e9,
+        "decompress_gbps": gfa_bytes / (td / steps) / 1e9,
+        "compress_ms": tc / steps * 1e3, "decompress_ms": td / steps * 1e3,
+        "encoded_symbols": E, "rules": R, "rounds_run": host.rounds_run,
+        "wall_s_timed_region": t_total,
+        "roofline": roofline, "expand_roofline": expand_roof,
+        "kernel_ms_per_step": {g: kstats[g]["total_ms"] / steps for g in groups
+                               if kstats[g]["launches"]},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(cfg, tr, gfa_bytes, cpu_threads())
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

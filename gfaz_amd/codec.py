@@ -1,0 +1,386 @@
+"""Python host side of the gfx950 grammar codec, over the C ABI of include/gfaz_amd.h.
+
+Names follow the reference's operator interface for this path:
+
+* ``compress_traversals``  - the traversal grammar of ``run_path_compression_gpu``
+  (include/gpu/compression/compression_workflow_gpu.hpp:75-78) with the reference CPU
+  compressor's exact output (src/workflows/compression_workflow.cpp:235-279,548-700).
+* ``Rulebook`` / ``decompress_traversals`` - ``prepare_gpu_traversal_rulebook`` and
+  ``decompress_gpu_traversal_materialized``
+  (src/gpu/decompression/decompression_primitives_gpu.cu:18-39,
+  src/gpu/decompression/traversal_decode_gpu.cu:76-120).
+
+Errors raise ``RuntimeError`` carrying the library's message, as the reference's
+pybind module does. There is no CPU fallback: without the built HIP library, or
+without a GPU, these calls fail.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+GFZ_INPUT_DEVICE = 1
+GFZ_OUTPUT_DEVICE = 2
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+
+
+class _CompressOut(ctypes.Structure):
+    _fields_ = [
+        ("encoded", ctypes.POINTER(ctypes.c_int32)),
+        ("n_encoded", _u64),
+        ("encoded_lengths", ctypes.POINTER(ctypes.c_uint32)),
+        ("n_trav", _u64),
+        ("rules_first", ctypes.POINTER(ctypes.c_int32)),
+        ("rules_second", ctypes.POINTER(ctypes.c_int32)),
+        ("n_rules", _u64),
+        ("start_id", _u32),
+        ("end_id", _u32),
+        ("rounds_run", ctypes.c_int32),
+        ("flags", _u32),
+        ("round_rules", _u64 * 32),
+        ("round_lengths", _u64 * 32),
+        ("internal", _vp),
+    ]
+
+
+_SIGNATURES = {
+    "gfz_last_error": ([], ctypes.c_char_p),
+    "gfz_version": ([], ctypes.c_char_p),
+    "gfz_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_ctx_destroy": ([_vp], None),
+    "gfz_ctx_sync": ([_vp], ctypes.c_int),
+    "gfz_ctx_set_memory_limit": ([_vp, _u64], ctypes.c_int),
+    "gfz_ctx_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gfz_ctx_reset_stats": ([_vp], ctypes.c_int),
+    "gfz_ctx_kernel_stat": ([_vp, ctypes.c_char_p, ctypes.POINTER(_u64),
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+                            ctypes.c_int),
+    "gfz_ctx_num_cus": ([_vp], ctypes.c_int),
+    "gfz_dev_alloc": ([_vp, _u64, ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_dev_free": ([_vp, _vp], ctypes.c_int),
+    "gfz_memcpy_h2d": ([_vp, _vp, _vp, _u64], ctypes.c_int),
+    "gfz_memcpy_d2h": ([_vp, _vp, _vp, _u64], ctypes.c_int),
+    "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, _u32, ctypes.POINTER(_CompressOut)], ctypes.c_int),
+    "gfz_compress_free": ([ctypes.POINTER(_CompressOut)], None),
+    "gfz_rulebook_load": ([_vp, _vp, _vp, _u64, _u32, _u32, ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_rulebook_free": ([_vp], None),
+    "gfz_rulebook_rule_size": ([_vp, _u64, ctypes.POINTER(_u32)], ctypes.c_int),
+    "gfz_decoded_size": ([_vp, _vp, _vp, _u64, _vp, _u64, _u32, ctypes.POINTER(_u64)],
+                         ctypes.c_int),
+    "gfz_decompress_traversals": ([_vp, _vp, _vp, _u64, _vp, _vp, _u64, ctypes.c_int, _vp, _u64,
+                                   _u32, ctypes.POINTER(_u64), _vp, ctypes.POINTER(ctypes.c_int)],
+                                  ctypes.c_int),
+}
+
+
+def lib() -> ctypes.CDLL:
+    L = _native.codec_lib()
+    if not getattr(L, "_gfz_typed", False):
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        L._gfz_typed = True
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().gfz_last_error()
+        raise RuntimeError((msg.decode() if msg else "") or f"gfaz_amd error {rc}")
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+class DeviceArray:
+    """A pool-backed device buffer of a numpy dtype (no torch needed)."""
+
+    def __init__(self, ctx: "Context", n: int, dtype):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = _vp()
+        _check(lib().gfz_dev_alloc(ctx.handle, max(1, self.n) * self.dtype.itemsize, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        if a.size:
+            _check(lib().gfz_memcpy_h2d(ctx.handle, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=self.dtype)
+        if n:
+            _check(lib().gfz_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr is not None and self.ctx.handle:
+            lib().gfz_dev_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context: stream, caching device pool, optional kernel timing."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().gfz_ctx_create(device, ctypes.byref(h)))
+        self.handle = h.value
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gfz_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self) -> None:
+        _check(lib().gfz_ctx_sync(self.handle))
+
+    def set_memory_limit(self, nbytes: int) -> None:
+        _check(lib().gfz_ctx_set_memory_limit(self.handle, int(nbytes)))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().gfz_ctx_set_profiling(self.handle, int(bool(on))))
+
+    def reset_stats(self) -> None:
+        _check(lib().gfz_ctx_reset_stats(self.handle))
+
+    def kernel_stat(self, name: str) -> dict:
+        n, ms, b = _u64(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().gfz_ctx_kernel_stat(self.handle, name.encode(), ctypes.byref(n),
+                                         ctypes.byref(ms), ctypes.byref(b)))
+        return {"launches": int(n.value), "total_ms": float(ms.value), "bytes": float(b.value)}
+
+    def num_cus(self) -> int:
+        return int(lib().gfz_ctx_num_cus(self.handle))
+
+
+@dataclass
+class Compressed:
+    """Traversal grammar: what CompressedData's traversal fields hold before zstd."""
+
+    encoded: np.ndarray | None        # int32 (host) or None when kept on the device
+    encoded_lengths: np.ndarray       # uint32 per traversal
+    rules_first: np.ndarray | None    # int32 raw, rule-ID order
+    rules_second: np.ndarray | None
+    start_id: int
+    end_id: int
+    rounds_run: int
+    rules_per_round: list = field(default_factory=list)
+    lengths_per_round: list = field(default_factory=list)
+    n_encoded: int = 0
+    n_rules: int = 0
+
+
+def compress_traversals(ctx: Context, nodes, lengths, n_paths: int = 0, rounds: int = 8,
+                        threshold: int = 2, delta_round: int = 1,
+                        keep_on_device: bool = False) -> Compressed:
+    """Grammar-compress flat traversals. ``nodes`` is a host int32 array or a DeviceArray."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(nodes, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        nptr, nn = nodes.ptr, nodes.n
+    else:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        nptr, nn = nodes.ctypes.data, nodes.size
+    if keep_on_device:
+        flags |= GFZ_OUTPUT_DEVICE
+    out = _CompressOut()
+    _check(lib().gfz_compress_traversals(ctx.handle, nptr, nn, lengths.ctypes.data, lengths.size,
+                                         n_paths, rounds, threshold, delta_round, flags,
+                                         ctypes.byref(out)))
+    try:
+        def arr(p, n, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+
+        host = not keep_on_device
+        res = Compressed(
+            encoded=arr(out.encoded, out.n_encoded, np.int32) if host else None,
+            encoded_lengths=arr(out.encoded_lengths, out.n_trav, np.uint32),
+            rules_first=arr(out.rules_first, out.n_rules, np.int32) if host else None,
+            rules_second=arr(out.rules_second, out.n_rules, np.int32) if host else None,
+            start_id=int(out.start_id), end_id=int(out.end_id), rounds_run=int(out.rounds_run),
+            rules_per_round=[int(out.round_rules[i]) for i in range(min(32, out.rounds_run))],
+            lengths_per_round=[int(out.round_lengths[i]) for i in range(min(32, out.rounds_run))],
+            n_encoded=int(out.n_encoded), n_rules=int(out.n_rules))
+    finally:
+        lib().gfz_compress_free(ctypes.byref(out))
+    return res
+
+
+class Rulebook:
+    """Device-resident rulebook: rule columns (raw, rule-ID order) and the ID window."""
+
+    def __init__(self, ctx: Context, rules_first, rules_second, min_rule_id: int):
+        self.ctx = ctx
+        rf = np.ascontiguousarray(rules_first, dtype=np.int32)
+        rs = np.ascontiguousarray(rules_second, dtype=np.int32)
+        if rf.size != rs.size:
+            raise ValueError("rule columns differ in length")
+        h = _vp()
+        _check(lib().gfz_rulebook_load(ctx.handle, rf.ctypes.data, rs.ctypes.data, rf.size,
+                                       int(min_rule_id), 0, ctypes.byref(h)))
+        self.handle = h.value
+        self.n_rules = int(rf.size)
+        self.min_rule_id = int(min_rule_id)
+
+    def rule_size(self, index: int) -> int:
+        s = _u32()
+        _check(lib().gfz_rulebook_rule_size(self.handle, index, ctypes.byref(s)))
+        return int(s.value)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gfz_rulebook_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decompress_traversals(ctx: Context, rb: Rulebook, encoded, encoded_lengths,
+                          orig_lengths=None, delta_round: int = 1, out: DeviceArray | None = None):
+    """Expand + inverse delta. Returns (decoded int32 host array or DeviceArray, lengths,
+    length_mismatch)."""
+    enc_len = np.ascontiguousarray(encoded_lengths, dtype=np.uint32)
+    orig = None if orig_lengths is None else np.ascontiguousarray(orig_lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(encoded, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        eptr, en = encoded.ptr, encoded.n
+    else:
+        encoded = np.ascontiguousarray(encoded, dtype=np.int32)
+        eptr, en = encoded.ctypes.data, encoded.size
+    n_out = _u64()
+    _check(lib().gfz_decoded_size(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
+                                  enc_len.size, flags, ctypes.byref(n_out)))
+    total = int(n_out.value)
+    lens = np.empty(max(1, enc_len.size), dtype=np.uint32)
+    mism = ctypes.c_int()
+    if out is not None:
+        flags |= GFZ_OUTPUT_DEVICE
+        if out.n < total:
+            raise ValueError(f"device output holds {out.n} < {total}")
+        optr, cap = out.ptr, out.n
+        result = out
+    else:
+        result = np.empty(max(1, total), dtype=np.int32)
+        optr, cap = result.ctypes.data, total
+    _check(lib().gfz_decompress_traversals(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
+                                           _ptr(orig), enc_len.size, delta_round, optr, cap, flags,
+                                           ctypes.byref(n_out), lens.ctypes.data,
+                                           ctypes.byref(mism)))
+    if out is None:
+        result = result[:total]
+    return result, lens[: enc_len.size], bool(mism.value)
+
+
+class DeviceCompressed:
+    """Compression result kept in HBM (GFZ_OUTPUT_DEVICE): encoded stream and rule
+    columns stay on the device until ``close``; lengths and counts are on the host."""
+
+    def __init__(self, ctx: Context, nodes, lengths, n_paths: int = 0, rounds: int = 8,
+                 threshold: int = 2, delta_round: int = 1):
+        self.ctx = ctx
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+        flags = GFZ_OUTPUT_DEVICE
+        if isinstance(nodes, DeviceArray):
+            flags |= GFZ_INPUT_DEVICE
+            nptr, nn = nodes.ptr, nodes.n
+        else:
+            nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+            nptr, nn = nodes.ctypes.data, nodes.size
+        self._out = _CompressOut()
+        _check(lib().gfz_compress_traversals(ctx.handle, nptr, nn, self.lengths.ctypes.data,
+                                             self.lengths.size, n_paths, rounds, threshold,
+                                             delta_round, flags, ctypes.byref(self._out)))
+        o = self._out
+        self.n_encoded = int(o.n_encoded)
+        self.n_rules = int(o.n_rules)
+        self.start_id = int(o.start_id)
+        self.end_id = int(o.end_id)
+        self.rounds_run = int(o.rounds_run)
+        self.rules_per_round = [int(o.round_rules[i]) for i in range(min(32, o.rounds_run))]
+        self.lengths_per_round = [int(o.round_lengths[i]) for i in range(min(32, o.rounds_run))]
+        self.encoded_lengths = np.ctypeslib.as_array(o.encoded_lengths, shape=(max(1, o.n_trav),))[
+            : o.n_trav].copy()
+
+    @property
+    def encoded_ptr(self) -> int:
+        return ctypes.cast(self._out.encoded, _vp).value
+
+    def rulebook(self) -> "Rulebook":
+        rb = Rulebook.__new__(Rulebook)
+        rb.ctx = self.ctx
+        h = _vp()
+        _check(lib().gfz_rulebook_load(self.ctx.handle, ctypes.cast(self._out.rules_first, _vp),
+                                       ctypes.cast(self._out.rules_second, _vp), self.n_rules,
+                                       self.start_id, GFZ_INPUT_DEVICE, ctypes.byref(h)))
+        rb.handle = h.value
+        rb.n_rules = self.n_rules
+        rb.min_rule_id = self.start_id
+        return rb
+
+    def encoded_view(self) -> DeviceArray:
+        """Non-owning DeviceArray over the encoded stream."""
+        d = DeviceArray.__new__(DeviceArray)
+        d.ctx, d.n, d.dtype, d.ptr = self.ctx, self.n_encoded, np.dtype(np.int32), self.encoded_ptr
+        d.free = lambda: None
+        return d
+
+    def to_host(self) -> Compressed:
+        o = self._out
+        enc = np.empty(self.n_encoded, np.int32)
+        rf = np.empty(self.n_rules, np.int32)
+        rs = np.empty(self.n_rules, np.int32)
+        for dst, src in ((enc, o.encoded), (rf, o.rules_first), (rs, o.rules_second)):
+            if dst.size:
+                _check(lib().gfz_memcpy_d2h(self.ctx.handle, dst.ctypes.data,
+                                            ctypes.cast(src, _vp), dst.nbytes))
+        return Compressed(enc, self.encoded_lengths.copy(), rf, rs, self.start_id, self.end_id,
+                          self.rounds_run, list(self.rules_per_round),
+                          list(self.lengths_per_round), self.n_encoded, self.n_rules)
+
+    def close(self) -> None:
+        if self._out is not None:
+            lib().gfz_compress_free(ctypes.byref(self._out))
+            self._out = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

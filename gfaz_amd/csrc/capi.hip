@@ -1,0 +1,375 @@
+// C ABI of include/gfaz_amd.h over the gfz:: device codec. Nothing throws
+// across this boundary: every entry point catches, records the message for
+// gfz_last_error() and returns a status code.
+#include "gfaz_amd.h"
+
+#include "codec.hpp"
+#include "prims.hpp"
+
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+
+struct gfz_ctx {
+  std::unique_ptr<gfz::Ctx> ctx;
+};
+
+struct gfz_rulebook {
+  gfz_ctx *owner = nullptr;
+  gfz::Rulebook rb;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(const char *prefix, F &&f) {
+  try {
+    f();
+    return GFZ_OK;
+  } catch (const gfz::Error &e) {
+    return fail(e.code, std::string(prefix) + e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(GFZ_E_NOMEM, std::string(prefix) + "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(GFZ_E_INVALID, std::string(prefix) + e.what());
+  }
+}
+
+constexpr const char *kCompressPrefix = "GPU compression error: ";
+constexpr const char *kDecompressPrefix = "GPU decompression error: ";
+constexpr const char *kRuntimePrefix = "GPU runtime error: ";
+
+struct CompressInternal {
+  gfz::CompressResult res;
+};
+
+template <class T>
+T *host_copy(gfz::Ctx &ctx, const T *d, uint64_t n) {
+  T *h = static_cast<T *>(std::malloc(std::max<uint64_t>(n, 1) * sizeof(T)));
+  if (!h)
+    throw std::bad_alloc();
+  if (n) {
+    GFZ_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+  }
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gfz_last_error(void) { return g_last_error.c_str(); }
+const char *gfz_version(void) { return "gfaz_amd 0.1 (gfx950)"; }
+
+int gfz_ctx_create(int device, gfz_ctx **out) {
+  if (!out)
+    return fail(GFZ_E_INVALID, "gfz_ctx_create: null output");
+  *out = nullptr;
+  return guarded(kRuntimePrefix, [&] {
+    int n = 0;
+    GFZ_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+      throw gfz::Error(GFZ_E_INVALID, "device " + std::to_string(device) + " not present (" +
+                                          std::to_string(n) + " visible)");
+    gfz::DeviceGuard g(device);
+    auto c = std::make_unique<gfz_ctx>();
+    c->ctx = std::make_unique<gfz::Ctx>(device);
+    *out = c.release();
+  });
+}
+
+void gfz_ctx_destroy(gfz_ctx *ctx) { delete ctx; }
+
+int gfz_ctx_sync(gfz_ctx *ctx) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  return guarded(kRuntimePrefix, [&] {
+    gfz::DeviceGuard g(ctx->ctx->device());
+    ctx->ctx->sync();
+  });
+}
+
+int gfz_ctx_set_memory_limit(gfz_ctx *ctx, uint64_t bytes) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  ctx->ctx->pool().set_limit(bytes);
+  return GFZ_OK;
+}
+
+int gfz_ctx_set_profiling(gfz_ctx *ctx, int on) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  ctx->ctx->set_profiling(on != 0);
+  return GFZ_OK;
+}
+
+int gfz_ctx_reset_stats(gfz_ctx *ctx) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  ctx->ctx->reset_stats();
+  return GFZ_OK;
+}
+
+int gfz_ctx_kernel_stat(gfz_ctx *ctx, const char *name, uint64_t *launches, double *total_ms,
+                        double *bytes) {
+  if (!ctx || !name)
+    return fail(GFZ_E_INVALID, "null argument");
+  const auto &st = ctx->ctx->stats();
+  auto it = st.find(name);
+  if (launches)
+    *launches = it == st.end() ? 0 : it->second.launches;
+  if (total_ms)
+    *total_ms = it == st.end() ? 0 : it->second.total_ms;
+  if (bytes)
+    *bytes = it == st.end() ? 0 : it->second.bytes;
+  return GFZ_OK;
+}
+
+int gfz_ctx_num_cus(gfz_ctx *ctx) { return ctx ? ctx->ctx->num_cus() : 0; }
+
+int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out) {
+  if (!ctx || !out)
+    return fail(GFZ_E_INVALID, "null argument");
+  return guarded(kRuntimePrefix, [&] {
+    gfz::DeviceGuard g(ctx->ctx->device());
+    *out = ctx->ctx->pool().get(bytes ? bytes : 1);
+  });
+}
+
+int gfz_dev_free(gfz_ctx *ctx, void *ptr) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  ctx->ctx->pool().put(ptr);
+  return GFZ_OK;
+}
+
+int gfz_memcpy_h2d(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  return guarded(kRuntimePrefix, [&] {
+    gfz::DeviceGuard g(ctx->ctx->device());
+    if (bytes)
+      GFZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->ctx->stream()));
+    ctx->ctx->sync();
+  });
+}
+
+int gfz_memcpy_d2h(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  return guarded(kRuntimePrefix, [&] {
+    gfz::DeviceGuard g(ctx->ctx->device());
+    if (bytes)
+      GFZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->ctx->stream()));
+    ctx->ctx->sync();
+  });
+}
+
+int gfz_compress_traversals(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                            const uint32_t *lengths, uint64_t n_trav, uint32_t n_paths,
+                            int rounds, int threshold, int delta_round, uint32_t flags,
+                            gfz_compress_out *out) {
+  if (!ctx || !out || (n_trav && !lengths) || (n_nodes && !nodes))
+    return fail(GFZ_E_INVALID, std::string(kCompressPrefix) + "null argument");
+  if (n_paths > n_trav)
+    return fail(GFZ_E_INVALID, std::string(kCompressPrefix) + "n_paths exceeds n_trav");
+  std::memset(out, 0, sizeof(*out));
+  return guarded(kCompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_nodes = nodes;
+    if (!(flags & GFZ_INPUT_DEVICE)) {
+      staged = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_nodes, 1));
+      if (n_nodes)
+        GFZ_HIP(hipMemcpyAsync(staged.get(), nodes, n_nodes * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+      d_nodes = staged.get();
+    }
+    auto in = std::make_unique<CompressInternal>();
+    gfz::CompressParams p;
+    p.rounds = rounds;
+    p.threshold = threshold;
+    p.delta_round = delta_round;
+    gfz::compress_device(c, d_nodes, n_nodes, lengths, n_trav, p, in->res);
+    staged.reset();
+    gfz::CompressResult &r = in->res;
+    out->n_encoded = r.n_encoded;
+    out->n_trav = n_trav;
+    out->n_rules = r.n_rules;
+    out->start_id = r.start_id;
+    out->end_id = r.end_id;
+    out->rounds_run = (int32_t)r.rounds.size();
+    out->flags = flags;
+    for (size_t i = 0; i < r.rounds.size() && i < 32; ++i) {
+      out->round_rules[i] = r.rounds[i].rules;
+      out->round_lengths[i] = r.rounds[i].length_out;
+    }
+    out->encoded_lengths =
+        static_cast<uint32_t *>(std::malloc(std::max<uint64_t>(n_trav, 1) * sizeof(uint32_t)));
+    if (!out->encoded_lengths)
+      throw std::bad_alloc();
+    if (n_trav)
+      std::memcpy(out->encoded_lengths, r.encoded_lengths.data(), n_trav * sizeof(uint32_t));
+    if (flags & GFZ_OUTPUT_DEVICE) {
+      out->encoded = r.encoded.get();
+      out->rules_first = r.rules_first.get();
+      out->rules_second = r.rules_second.get();
+    } else {
+      out->encoded = host_copy(c, r.encoded.get(), r.n_encoded);
+      out->rules_first = host_copy(c, r.rules_first.get(), r.n_rules);
+      out->rules_second = host_copy(c, r.rules_second.get(), r.n_rules);
+      r.encoded.reset();
+      r.rules_first.reset();
+      r.rules_second.reset();
+    }
+    out->internal = in.release();
+  });
+}
+
+void gfz_compress_free(gfz_compress_out *out) {
+  if (!out)
+    return;
+  if (!(out->flags & GFZ_OUTPUT_DEVICE)) {
+    std::free(out->encoded);
+    std::free(out->rules_first);
+    std::free(out->rules_second);
+  }
+  std::free(out->encoded_lengths);
+  delete static_cast<CompressInternal *>(out->internal);
+  std::memset(out, 0, sizeof(*out));
+}
+
+int gfz_rulebook_load(gfz_ctx *ctx, const int32_t *rules_first, const int32_t *rules_second,
+                      uint64_t n_rules, uint32_t min_rule_id, uint32_t flags,
+                      gfz_rulebook **out) {
+  if (!ctx || !out || (n_rules && (!rules_first || !rules_second)))
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  *out = nullptr;
+  return guarded(kDecompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> f, s;
+    const int32_t *df = rules_first, *ds = rules_second;
+    if (!(flags & GFZ_INPUT_DEVICE)) {
+      f = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_rules, 1));
+      s = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_rules, 1));
+      if (n_rules) {
+        GFZ_HIP(hipMemcpyAsync(f.get(), rules_first, n_rules * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+        GFZ_HIP(hipMemcpyAsync(s.get(), rules_second, n_rules * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+      }
+      df = f.get();
+      ds = s.get();
+    }
+    auto rb = std::make_unique<gfz_rulebook>();
+    rb->owner = ctx;
+    gfz::rulebook_build(c, df, ds, n_rules, min_rule_id, rb->rb);
+    c.sync();
+    *out = rb.release();
+  });
+}
+
+void gfz_rulebook_free(gfz_rulebook *rb) { delete rb; }
+
+int gfz_rulebook_rule_size(gfz_rulebook *rb, uint64_t rule_index, uint32_t *size) {
+  if (!rb || !size)
+    return fail(GFZ_E_INVALID, "null argument");
+  if (rule_index >= rb->rb.n_rules)
+    return fail(GFZ_E_INVALID, "rule index out of range");
+  return guarded(kDecompressPrefix, [&] {
+    gfz::Ctx &c = *rb->owner->ctx;
+    gfz::DeviceGuard g(c.device());
+    GFZ_HIP(hipMemcpyAsync(size, rb->rb.size.get() + rule_index, 4, hipMemcpyDeviceToHost,
+                           c.stream()));
+    c.sync();
+  });
+}
+
+static void stage_encoded(gfz::Ctx &c, const int32_t *encoded, uint64_t n, uint32_t flags,
+                          gfz::DBuf<int32_t> &staged, const int32_t *&d_enc) {
+  d_enc = encoded;
+  if (!(flags & GFZ_INPUT_DEVICE)) {
+    staged = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n, 1));
+    if (n)
+      GFZ_HIP(hipMemcpyAsync(staged.get(), encoded, n * 4, hipMemcpyHostToDevice, c.stream()));
+    d_enc = staged.get();
+  }
+}
+
+int gfz_decoded_size(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                     uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                     uint32_t flags, uint64_t *n_out) {
+  if (!ctx || !rb || !n_out || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  return guarded(kDecompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_enc;
+    stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
+    gfz::DecodePlan plan;
+    gfz::DBuf<uint64_t> sym_off;
+    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, nullptr, n_trav, plan,
+                     sym_off);
+    *n_out = plan.n_out;
+  });
+}
+
+int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                              uint64_t n_encoded, const uint32_t *encoded_lengths,
+                              const uint32_t *orig_lengths, uint64_t n_trav, int delta_round,
+                              int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
+                              uint32_t *out_lengths, int *length_mismatch) {
+  if (!ctx || !rb || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  return guarded(kDecompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_enc;
+    stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
+    gfz::DecodePlan plan;
+    gfz::DBuf<uint64_t> sym_off;
+    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, orig_lengths, n_trav, plan,
+                     sym_off);
+    if (n_out)
+      *n_out = plan.n_out;
+    if (length_mismatch)
+      *length_mismatch = plan.length_mismatch ? 1 : 0;
+    if (out_lengths)
+      for (uint64_t t = 0; t < n_trav; ++t)
+        out_lengths[t] = (uint32_t)(plan.out_offsets[t + 1] - plan.out_offsets[t]);
+    if (plan.n_out > out_cap)
+      throw gfz::Error(GFZ_E_INVALID, "output capacity " + std::to_string(out_cap) +
+                                          " < decoded size " + std::to_string(plan.n_out));
+    if (plan.n_out && !out)
+      throw gfz::Error(GFZ_E_INVALID, "null output buffer");
+    const int dr = delta_round < 0 ? 0 : delta_round;
+    if (flags & GFZ_OUTPUT_DEVICE) {
+      gfz::decode_run(c, rb->rb, d_enc, n_encoded, encoded_lengths, n_trav, dr, plan,
+                      sym_off.get(), out);
+      c.sync();
+    } else {
+      gfz::DBuf<int32_t> d_out(c.pool(), std::max<uint64_t>(plan.n_out, 1));
+      gfz::decode_run(c, rb->rb, d_enc, n_encoded, encoded_lengths, n_trav, dr, plan,
+                      sym_off.get(), d_out.get());
+      if (plan.n_out)
+        GFZ_HIP(hipMemcpyAsync(out, d_out.get(), plan.n_out * 4, hipMemcpyDeviceToHost,
+                               c.stream()));
+      c.sync();
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Host-side entry points of the grammar codec on one device.
+#pragma once
+
+#include "ctx.hpp"
+
+#include <vector>
+
+namespace gfz {
+
+struct CompressParams {
+  int rounds = 8;
+  int threshold = 2;  // count >= 2 whatever the value (the reference's two-set scheme)
+  int delta_round = 1;
+};
+
+struct RoundInfo {
+  uint64_t length_in = 0;
+  uint64_t length_out = 0;
+  uint64_t distinct_pairs = 0;
+  uint64_t rules = 0;
+  uint64_t table_slots = 0;
+  int table_retries = 0;
+};
+
+// Result of compressing one batch of traversals. Device arrays are owned by
+// the Ctx pool (DBuf); host vectors are filled on request.
+struct CompressResult {
+  DBuf<int32_t> encoded;       // flat encoded traversals (device)
+  uint64_t n_encoded = 0;
+  std::vector<uint32_t> encoded_lengths;  // one per input traversal (host)
+  DBuf<int32_t> rules_first;   // raw, rule-ID order (device)
+  DBuf<int32_t> rules_second;
+  uint64_t n_rules = 0;
+  uint32_t start_id = 0;
+  uint32_t end_id = 0;
+  std::vector<RoundInfo> rounds;
+};
+
+// nodes: device pointer to the flat traversals (paths first, then walks);
+// lengths: host array, one per traversal.
+void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t n_nodes,
+                     const uint32_t *lengths, uint64_t n_trav, const CompressParams &p,
+                     CompressResult &out);
+
+// Device-resident rulebook for expansion: per rule {first, second, |first|, |second|}
+// (expanded sizes of the two children) plus the rule-ID window.
+struct RuleRec {
+  int32_t a, b;
+  uint32_t sa, sb;
+};
+
+struct Rulebook {
+  DBuf<RuleRec> rec;
+  DBuf<uint32_t> size;  // expanded size of each rule
+  uint64_t n_rules = 0;
+  uint32_t min_id = 0;
+  uint32_t depth = 0;
+};
+
+// first/second: device pointers (raw rule columns, rule-ID order).
+void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, uint64_t n_rules,
+                    uint32_t min_id, Rulebook &rb);
+
+struct DecodePlan {
+  uint64_t n_out = 0;                 // total decoded symbols
+  std::vector<uint64_t> out_offsets;  // per traversal (n_trav + 1), host
+  bool length_mismatch = false;       // decoded lengths differ from the stored originals
+};
+
+// Expand the encoded traversals and undo delta_round prefix sums.
+// d_enc: device; enc_lengths / orig_lengths: host (orig may be null).
+// d_out must hold the decoded total; call decode_size first to learn it.
+void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                 const uint32_t *enc_lengths, const uint32_t *orig_lengths, uint64_t n_trav,
+                 DecodePlan &plan, DBuf<uint64_t> &sym_off);
+void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                const uint32_t *enc_lengths, uint64_t n_trav, int delta_round,
+                const DecodePlan &plan, const uint64_t *d_sym_off, int32_t *d_out);
+
+// Segmented inclusive prefix sum (int32, wrapping) of x[0..n) in place, with
+// segments starting at the set bits of `heads`.
+void segmented_prefix_sum(Ctx &ctx, int32_t *x, uint64_t n, const uint32_t *heads);
+
+}  // namespace gfz

@@ -1,0 +1,648 @@
+// Grammar compression of GFA traversals on gfx950.
+//
+// Restates, with the same output bit for bit, the reference CPU pipeline:
+//   delta transform + max |symbol|  (src/workflows/compression_workflow.cpp:235-265,
+//                                    src/codec/codec.cpp:45-76)
+//   per round (compression_workflow.cpp:567-690):
+//     repeated canonical pairs      (src/grammar/rule_generator.cpp:148-260)
+//     greedy non-overlapping replace (src/grammar/path_encoder.cpp:29-83)
+//     used rules sorted by signed key -> IDs next+rank, remap
+//                                   (compression_workflow.cpp:601-680,
+//                                    src/workflows/compression_utils.cpp:201-221)
+//
+// Layout in HBM: one flat int32 stream of all traversals (paths, then walks), a
+// head bitmap marking each non-empty traversal's first element, a 64-bit
+// element index everywhere. Work is split into fixed tiles of the flat stream,
+// independent of traversal lengths; the greedy pick recurrence crosses tiles
+// through a one-bit carry (bits.hpp).
+#include "bits.hpp"
+#include "codec.hpp"
+#include "prims.hpp"
+
+#include <algorithm>
+
+namespace gfz {
+
+namespace {
+
+constexpr uint32_t kInsItems = 8;                   // pair-insert tile: 2048 elements
+constexpr uint32_t kInsTile = kBlock * kInsItems;
+constexpr uint32_t kLdsSlots = 4096;                // LDS pair table (>= 2x tile)
+constexpr uint32_t kRtItems = 16;                   // cand/emit tile: 4096 elements
+constexpr uint32_t kRtTile = kBlock * kRtItems;     // = 64 words of 64 candidates
+constexpr uint32_t kRtWords = kRtTile / 64;
+static_assert(kRtWords == kWave, "one candidate word per lane of one wave");
+constexpr uint32_t kTileScanThreads = 1024;
+
+__device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
+  return (heads[i >> 5] >> (i & 31)) & 1u;
+}
+
+struct DevTable {
+  unsigned long long *keys;  // kEmptyKey = free
+  uint32_t *val;             // 0 seen once, 1 repeated; later the final rule ID
+  uint8_t *used;
+  uint64_t mask;
+  uint32_t *distinct;
+  uint32_t *overflow;
+  uint32_t limit;
+};
+
+__device__ inline int64_t table_find(const unsigned long long *keys, uint64_t mask, uint64_t k) {
+  uint64_t s = key_hash(k) & mask;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    const uint64_t c = keys[s];
+    if (c == k)
+      return (int64_t)s;
+    if (c == kEmptyKey)
+      return -1;
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
+__device__ inline void table_insert(const DevTable &t, uint64_t k, uint32_t cnt) {
+  if (__hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    return;
+  uint64_t s = key_hash(k) & t.mask;
+  for (uint64_t p = 0; p <= t.mask; ++p) {
+    const unsigned long long old = atomicCAS(&t.keys[s], (unsigned long long)kEmptyKey,
+                                             (unsigned long long)k);
+    if (old == kEmptyKey) {
+      if (atomicAdd(t.distinct, 1u) + 1u > t.limit)
+        atomicExch(t.overflow, 1u);
+      if (cnt >= 2)
+        t.val[s] = 1u;
+      return;
+    }
+    if (old == k) {
+      t.val[s] = 1u;
+      return;
+    }
+    s = (s + 1) & t.mask;
+  }
+  atomicExch(t.overflow, 1u);
+}
+
+// ---- delta (codec.cpp:45-76) ------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in,
+                                                  int32_t *__restrict__ out,
+                                                  const uint32_t *__restrict__ heads, uint64_t n,
+                                                  int transform, uint32_t *max_abs) {
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    int32_t v = in[i];
+    if (transform) {
+      if (!head_at(heads, i))
+        v = (int32_t)((uint32_t)v - (uint32_t)in[i - 1]);
+      out[i] = v;
+    }
+    const uint32_t a = uabs(v);
+    m = a > m ? a : m;
+  }
+  m = wave_reduce_max(m);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m)
+    atomicMax(max_abs, m);
+}
+
+// ---- K1: repeated-pair set ----------------------------------------------------
+// Each block folds its tile's canonical pairs into an LDS table (hot keys such as
+// the backbone's (+1,+1) collapse to one entry with a count), then flushes every
+// distinct key once into the HBM table.
+__global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restrict__ x,
+                                                        const uint32_t *__restrict__ heads,
+                                                        uint64_t L, DevTable t) {
+  __shared__ unsigned long long skeys[kLdsSlots];
+  __shared__ uint32_t scnt[kLdsSlots];
+  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
+    skeys[s] = kEmptyKey;
+    scnt[s] = 0;
+  }
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
+#pragma unroll 2
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    if (i + 1 < L && !head_at(heads, i + 1)) {
+      const unsigned long long k = (unsigned long long)canon2(pack2(x[i], x[i + 1]));
+      uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
+      for (uint32_t p = 0; p < kLdsSlots; ++p) {
+        const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kEmptyKey, k);
+        if (old == kEmptyKey || old == k) {
+          atomicAdd(&scnt[s], 1u);
+          break;
+        }
+        s = (s + 1) & (kLdsSlots - 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
+    const unsigned long long k = skeys[s];
+    if (k != kEmptyKey)
+      table_insert(t, k, scnt[s]);
+  }
+}
+
+// Inclusive scan of transfer functions across the 64 lanes of a wave.
+__device__ inline uint32_t wave_fn_inclusive(uint32_t f) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t g = __shfl_up(f, d);
+    if (lane >= (uint32_t)d)
+      f = fn_then(g, f);
+  }
+  return f;
+}
+
+// ---- K2: candidate bits and each tile's transfer function --------------------
+__global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
+                                                 const uint32_t *__restrict__ heads, uint64_t L,
+                                                 const unsigned long long *__restrict__ keys,
+                                                 const uint32_t *__restrict__ val, uint64_t mask,
+                                                 uint64_t *__restrict__ cand, uint64_t n_words,
+                                                 uint32_t *__restrict__ tile_fn) {
+  __shared__ uint64_t wbits[kRtWords];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
+  for (uint32_t j = 0; j < kRtItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    bool c = false;
+    if (i + 1 < L && !head_at(heads, i + 1)) {
+      const uint64_t k = (uint64_t)canon2(pack2(x[i], x[i + 1]));
+      const int64_t s = table_find(keys, mask, k);
+      c = s >= 0 && val[s] != 0;
+    }
+    const uint64_t m = __ballot(c);
+    if (lane == 0) {
+      const uint32_t w = j * (kBlock / kWave) + wid;
+      wbits[w] = m;
+      const uint64_t gw = (base >> 6) + w;
+      if (gw < n_words)
+        cand[gw] = m;
+    }
+  }
+  __syncthreads();
+  if (wid == 0) {
+    const uint32_t f = wave_fn_inclusive(word_fn(wbits[lane]));
+    if (lane == kWave - 1)
+      tile_fn[blockIdx.x] = f;
+  }
+}
+
+// ---- K2b: carry bit and output offset of every tile --------------------------
+// tile_info[t] = (survivors before tile t) << 1 | pick[-1] of tile t;
+// *total = survivors overall (the next round's length).
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
+    const uint32_t *__restrict__ tile_fn, uint64_t n_tiles, uint64_t L,
+    uint64_t *__restrict__ tile_info, uint64_t *__restrict__ total) {
+  __shared__ FnWide wsum[kTileScanThreads / kWave];
+  __shared__ FnWide carry_in[kTileScanThreads / kWave];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t chunk = (n_tiles + kTileScanThreads - 1) / kTileScanThreads;
+  const uint64_t begin = (uint64_t)threadIdx.x * chunk;
+  const uint64_t end = begin + chunk < n_tiles ? begin + chunk : n_tiles;
+  auto widen = [&](uint64_t t) {
+    const uint32_t f = tile_fn[t];
+    const uint64_t len = (t + 1) * kRtTile <= L ? kRtTile : L - t * kRtTile;
+    FnWide w;
+    w.o0 = fn_out(f, 0);
+    w.o1 = fn_out(f, 1);
+    w.s0 = len - fn_cnt(f, 0) + w.o0;
+    w.s1 = len - 1 - fn_cnt(f, 1) + w.o1;
+    return w;
+  };
+  FnWide mine = fnw_identity();
+  for (uint64_t t = begin; t < end; ++t)
+    mine = fnw_then(mine, widen(t));
+  // Inclusive scan across the wave, then across waves.
+  FnWide inc = mine;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    FnWide g;
+    g.s0 = __shfl_up(inc.s0, d);
+    g.s1 = __shfl_up(inc.s1, d);
+    g.o0 = __shfl_up(inc.o0, d);
+    g.o1 = __shfl_up(inc.o1, d);
+    if (lane >= (uint32_t)d)
+      inc = fnw_then(g, inc);
+  }
+  if (lane == kWave - 1)
+    wsum[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    FnWide run = fnw_identity();
+    for (uint32_t w = 0; w < kTileScanThreads / kWave; ++w) {
+      carry_in[w] = run;
+      run = fnw_then(run, wsum[w]);
+    }
+    *total = run.s0;
+  }
+  __syncthreads();
+  FnWide ex;
+  ex.s0 = __shfl_up(inc.s0, 1);
+  ex.s1 = __shfl_up(inc.s1, 1);
+  ex.o0 = __shfl_up(inc.o0, 1);
+  ex.o1 = __shfl_up(inc.o1, 1);
+  if (lane == 0)
+    ex = fnw_identity();
+  const FnWide pre = fnw_then(carry_in[wid], ex);
+  uint32_t p = pre.o0;
+  uint64_t off = pre.s0;
+  for (uint64_t t = begin; t < end; ++t) {
+    tile_info[t] = (off << 1) | p;
+    const FnWide w = widen(t);
+    off += p ? w.s1 : w.s0;
+    p = p ? w.o1 : w.o0;
+  }
+}
+
+// ---- K3: emit the next round's stream ---------------------------------------
+// Survivors are elements whose predecessor was not picked; a picked element
+// emits +/-(next + slot) (provisional, remapped after the sort). Heads always
+// survive and carry their bit to the output position.
+__global__ __launch_bounds__(kBlock) void k_emit(
+    const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
+    const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
+    const unsigned long long *__restrict__ keys, uint64_t mask, uint8_t *__restrict__ used,
+    uint32_t next, int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
+    uint32_t *__restrict__ err) {
+  __shared__ uint64_t spicks[kRtWords];
+  __shared__ uint64_t ssurv[kRtWords];
+  __shared__ uint32_t soff[kRtWords];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
+  const uint64_t info = tile_info[blockIdx.x];
+  const uint32_t p_tile = (uint32_t)(info & 1);
+  const uint64_t off_tile = info >> 1;
+  if (wid == 0) {
+    const uint64_t gw = (base >> 6) + lane;
+    const uint64_t c = gw < n_words ? cand[gw] : 0;
+    const uint32_t inc = wave_fn_inclusive(word_fn(c));
+    uint32_t ex = __shfl_up(inc, 1);
+    if (lane == 0)
+      ex = fn_identity();
+    const uint32_t p_in = fn_out(ex, p_tile);
+    const uint64_t picks = word_picks(c, p_in);
+    const uint64_t lo = base + (uint64_t)lane * 64;
+    const uint64_t nvalid = lo >= L ? 0 : (L - lo >= 64 ? 64 : L - lo);
+    const uint64_t vmask = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
+    const uint64_t surv = ~((picks << 1) | (uint64_t)p_in) & vmask;
+    const uint32_t sc = (uint32_t)__popcll(surv);
+    uint32_t sx = sc;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t g = __shfl_up(sx, d);
+      if (lane >= (uint32_t)d)
+        sx += g;
+    }
+    spicks[lane] = picks;
+    ssurv[lane] = surv;
+    soff[lane] = sx - sc;
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < kRtItems; ++j) {
+    const uint32_t w = j * (kBlock / kWave) + wid;
+    const uint64_t i = base + (uint64_t)w * 64 + lane;
+    const uint64_t sv = ssurv[w];
+    if (i < L && ((sv >> lane) & 1)) {
+      const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(sv & lanemask_lt());
+      int32_t v = x[i];
+      if ((spicks[w] >> lane) & 1) {
+        const int64_t k = pack2(v, x[i + 1]);
+        const int64_t c = canon2(k);
+        const int64_t s = table_find(keys, mask, (uint64_t)c);
+        if (s < 0) {
+          atomicExch(err, 1u);
+        } else {
+          used[s] = 1;
+          v = (int32_t)(next + (uint32_t)s);
+          if (k != c)
+            v = -v;
+        }
+      }
+      y[pos] = v;
+      if (head_at(heads, i))
+        atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
+    }
+  }
+}
+
+// ---- K4: keys of the used rules (order irrelevant: they are sorted next) -----
+__global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long long *__restrict__ keys,
+                                                         const uint8_t *__restrict__ used,
+                                                         uint64_t cap, uint64_t *__restrict__ out,
+                                                         unsigned long long *count) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint64_t s0 = (uint64_t)blockIdx.x * kBlock; s0 < cap; s0 += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t s = s0 + threadIdx.x;
+    const bool u = s < cap && used[s];
+    const uint64_t m = __ballot(u);
+    if (!m)
+      continue;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    unsigned long long b = 0;
+    if (lane == leader)
+      b = atomicAdd(count, (unsigned long long)__popcll(m));
+    b = __shfl(b, (int)leader);
+    if (u)
+      out[b + (uint64_t)__popcll(m & lanemask_lt())] = keys[s];
+  }
+}
+
+// ---- K6: final IDs next + rank, rule columns ----------------------------------
+__global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
+                                                   const unsigned long long *__restrict__ keys,
+                                                   uint64_t mask, uint32_t *__restrict__ val,
+                                                   uint32_t next, int32_t *__restrict__ rf,
+                                                   int32_t *__restrict__ rs,
+                                                   uint32_t *__restrict__ err) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t k = sorted[r];
+    const int64_t s = table_find(keys, mask, k);
+    if (s < 0) {
+      atomicExch(err, 2u);
+      continue;
+    }
+    val[s] = next + (uint32_t)r;
+    rf[r] = key_first((int64_t)k);
+    rs[r] = key_second((int64_t)k);
+  }
+}
+
+// ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
+__global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
+                                                  const uint32_t *__restrict__ val,
+                                                  uint32_t next) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const int32_t v = y[i];
+    const uint32_t a = uabs(v);
+    if (a >= next) {
+      const int32_t f = (int32_t)val[a - next];
+      y[i] = v > 0 ? f : -f;
+    }
+  }
+}
+
+uint32_t stride_grid(Ctx &ctx, uint64_t n) {
+  const uint64_t want = ceil_div(n, kBlock * 4);
+  const uint64_t cap = (uint64_t)ctx.num_cus() * 8;
+  return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+template <class T>
+T read_scalar(Ctx &ctx, const T *d) {
+  T v{};
+  GFZ_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, ctx.stream()));
+  ctx.sync();
+  return v;
+}
+
+}  // namespace
+
+void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_t *lengths,
+                     uint64_t T, const CompressParams &p, CompressResult &out) {
+  if (p.rounds < 0)
+    throw Error(kErrInvalid, "rounds must be >= 0");
+  const int delta_round = p.delta_round < 0 ? 0 : p.delta_round;  // clamp, as the reference
+  out = CompressResult();
+  out.encoded_lengths.assign(T, 0);
+  std::vector<uint64_t> starts;
+  std::vector<uint64_t> ne_index;
+  uint64_t total = 0;
+  for (uint64_t t = 0; t < T; ++t) {
+    if (lengths[t]) {
+      starts.push_back(total);
+      ne_index.push_back(t);
+    }
+    total += lengths[t];
+  }
+  if (total != N)
+    throw Error(kErrInvalid, "sum of traversal lengths (" + std::to_string(total) +
+                                 ") != number of nodes (" + std::to_string(N) + ")");
+  Pool &pool = ctx.pool();
+  const uint64_t words32 = ceil_div(std::max<uint64_t>(N, 1), 32);
+  DBuf<uint32_t> heads(pool, words32), heads_next(pool, words32);
+  fill_u32(ctx, heads.get(), 0, words32);
+  if (!starts.empty()) {
+    DBuf<uint64_t> d_starts(pool, starts.size());
+    GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
+                           hipMemcpyHostToDevice, ctx.stream()));
+    set_head_bits(ctx, d_starts.get(), starts.size(), heads.get(), N);
+    ctx.sync();  // d_starts is released at scope end
+  }
+
+  // Delta transform and the rule-ID floor.
+  DBuf<int32_t> xa(pool, std::max<uint64_t>(N, 1)), xb(pool, std::max<uint64_t>(N, 1));
+  DBuf<uint32_t> d_max(pool, 1);
+  fill_u32(ctx, d_max.get(), 0, 1);
+  const int32_t *cur = d_nodes;
+  int32_t *cur_buf = nullptr;
+  if (N) {
+    if (delta_round == 0) {
+      ctx.timed("delta", [&] {
+        hipLaunchKernelGGL(k_delta, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(),
+                           d_nodes, (int32_t *)nullptr, heads.get(), N, 0, d_max.get());
+      });
+      GFZ_CHECK_LAUNCH();
+    }
+    for (int r = 0; r < delta_round; ++r) {
+      int32_t *dst = (r % 2 == 0) ? xa.get() : xb.get();
+      ctx.timed("delta", [&] {
+        hipLaunchKernelGGL(k_delta, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(),
+                           cur, dst, heads.get(), N, 1, d_max.get());
+      });
+      GFZ_CHECK_LAUNCH();
+      cur = dst;
+      cur_buf = dst;
+    }
+  }
+  const uint32_t max_abs = read_scalar(ctx, d_max.get());
+  if (max_abs == UINT32_MAX)
+    throw Error(kErrOverflow, "delta values too large for rule ID assignment");
+  uint32_t next = max_abs + 1;
+  out.start_id = next;
+
+  uint64_t L = N;
+  uint64_t rules_cap = 0;
+  DBuf<int32_t> rf, rs;
+  DBuf<uint32_t> d_err(pool, 1);
+  fill_u32(ctx, d_err.get(), 0, 1);
+  uint64_t prev_distinct = 0;
+
+  for (int round = 0; round < p.rounds && L >= 2; ++round) {
+    RoundInfo info;
+    info.length_in = L;
+    int32_t *dst = (cur_buf == xa.get()) ? xb.get() : xa.get();
+    const uint64_t n_tiles = ceil_div(L, kRtTile);
+    const uint64_t n_words = ceil_div(L, 64);
+
+    // K1 with capacity retry: the table holds at most cap/2 distinct keys.
+    uint64_t cap = next_pow2(std::max<uint64_t>(
+        uint64_t(1) << 22, round == 0 ? L / 8 : std::min<uint64_t>(L, 4 * prev_distinct)));
+    DBuf<unsigned long long> tkeys;
+    DBuf<uint32_t> tval;
+    DBuf<uint32_t> counters(pool, 2);  // distinct, overflow
+    uint32_t distinct = 0;
+    for (;;) {
+      if ((uint64_t)next + cap >= (uint64_t)INT32_MAX)
+        throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
+                                      ", table " + std::to_string(cap) + ")");
+      tkeys = DBuf<unsigned long long>(pool, cap);
+      tval = DBuf<uint32_t>(pool, cap);
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(tkeys.get()), kEmptyKey, cap);
+      fill_u32(ctx, tval.get(), 0, cap);
+      fill_u32(ctx, counters.get(), 0, 2);
+      DevTable t{tkeys.get(), tval.get(), nullptr, cap - 1, counters.get(), counters.get() + 1,
+                 (uint32_t)std::min<uint64_t>(cap / 2, UINT32_MAX)};
+      ctx.timed("pair_insert", [&] {
+        hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)ceil_div(L, kInsTile)), dim3(kBlock), 0,
+                           ctx.stream(), cur, heads.get(), L, t);
+      });
+      GFZ_CHECK_LAUNCH();
+      ctx.add_bytes("pair_insert", 4.0 * L);
+      uint32_t hc[2];
+      GFZ_HIP(hipMemcpyAsync(hc, counters.get(), sizeof(hc), hipMemcpyDeviceToHost,
+                             ctx.stream()));
+      ctx.sync();
+      if (!hc[1]) {
+        distinct = hc[0];
+        break;
+      }
+      cap *= 4;
+      info.table_retries += 1;
+    }
+    info.distinct_pairs = distinct;
+    info.table_slots = cap;
+    prev_distinct = distinct;
+
+    // K2 + K2b: candidates, tile carries, next length.
+    DBuf<uint64_t> cand(pool, n_words);
+    DBuf<uint32_t> tile_fn(pool, n_tiles);
+    DBuf<uint64_t> tile_info(pool, n_tiles);
+    DBuf<uint64_t> d_total(pool, 1);
+    ctx.timed("cand_tiles", [&] {
+      hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
+                         heads.get(), L, tkeys.get(), tval.get(), cap - 1, cand.get(), n_words,
+                         tile_fn.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    ctx.add_bytes("cand_tiles", 4.0 * L);
+    ctx.timed("tile_scan", [&] {
+      hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
+                         tile_fn.get(), n_tiles, L, tile_info.get(), d_total.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    const uint64_t L_next = read_scalar(ctx, d_total.get());
+    if (L_next >= L)
+      break;  // no repeated pair: the reference stops here (compression_workflow.cpp:576)
+
+    // K3: emit.
+    DBuf<uint8_t> used(pool, cap);
+    fill_u8(ctx, used.get(), 0, cap);
+    fill_u32(ctx, heads_next.get(), 0, ceil_div(L_next, 32));
+    ctx.timed("emit", [&] {
+      hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
+                         heads.get(), L, cand.get(), n_words, tile_info.get(), tkeys.get(),
+                         cap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    ctx.add_bytes("emit", 4.0 * L + 4.0 * L_next);
+    cand.reset();
+    tile_fn.reset();
+    tile_info.reset();
+
+    // K4: used keys.
+    DBuf<uint64_t> ukeys(pool, distinct ? distinct : 1), utmp(pool, distinct ? distinct : 1);
+    DBuf<unsigned long long> ucount(pool, 1);
+    fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
+    ctx.timed("collect_used", [&] {
+      hipLaunchKernelGGL(k_collect_used, dim3(stride_grid(ctx, cap)), dim3(kBlock), 0,
+                         ctx.stream(), tkeys.get(), used.get(), cap, ukeys.get(), ucount.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    const uint64_t R = read_scalar(ctx, ucount.get());
+    if (read_scalar(ctx, d_err.get()))
+      throw Error(kErrCorrupt, "pair table lost a key (internal error)");
+    if (R == 0)
+      break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
+
+    // K5: sort by signed key; K6: IDs and rule columns.
+    sort_i64(ctx, ukeys.get(), utmp.get(), R);
+    if (out.n_rules + R > rules_cap) {
+      const uint64_t ncap = std::max<uint64_t>(2 * rules_cap, out.n_rules + R);
+      DBuf<int32_t> nf(pool, ncap), ns(pool, ncap);
+      if (out.n_rules) {
+        GFZ_HIP(hipMemcpyAsync(nf.get(), rf.get(), out.n_rules * 4, hipMemcpyDeviceToDevice,
+                               ctx.stream()));
+        GFZ_HIP(hipMemcpyAsync(ns.get(), rs.get(), out.n_rules * 4, hipMemcpyDeviceToDevice,
+                               ctx.stream()));
+      }
+      ctx.sync();
+      rf = std::move(nf);
+      rs = std::move(ns);
+      rules_cap = ncap;
+    }
+    ctx.timed("assign_ids", [&] {
+      hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                         ukeys.get(), R, tkeys.get(), cap - 1, tval.get(), next,
+                         rf.get() + out.n_rules, rs.get() + out.n_rules, d_err.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    // K7: remap provisional IDs.
+    ctx.timed("remap", [&] {
+      hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
+                         dst, L_next, tval.get(), next);
+    });
+    GFZ_CHECK_LAUNCH();
+    ctx.add_bytes("remap", 8.0 * L_next);
+    if (read_scalar(ctx, d_err.get()))
+      throw Error(kErrCorrupt, "sorted key missing from the pair table (internal error)");
+
+    next += (uint32_t)R;
+    out.n_rules += R;
+    info.length_out = L_next;
+    info.rules = R;
+    out.rounds.push_back(info);
+    L = L_next;
+    cur = dst;
+    cur_buf = dst;
+    heads.swap(heads_next);
+  }
+  out.end_id = next;
+
+  // Encoded lengths from the head positions of the final stream.
+  if (!ne_index.empty()) {
+    DBuf<uint64_t> pos(pool, ne_index.size());
+    const uint64_t cnt = bitmap_positions(ctx, heads.get(), L, pos.get(), ne_index.size());
+    if (cnt != ne_index.size())
+      throw Error(kErrCorrupt, "traversal heads lost during encoding (internal error)");
+    std::vector<uint64_t> h(cnt);
+    GFZ_HIP(hipMemcpyAsync(h.data(), pos.get(), cnt * 8, hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    for (uint64_t k = 0; k < cnt; ++k) {
+      const uint64_t end = k + 1 < cnt ? h[k + 1] : L;
+      out.encoded_lengths[ne_index[k]] = (uint32_t)(end - h[k]);
+    }
+  }
+  out.n_encoded = L;
+  if (cur_buf == xa.get()) {
+    out.encoded = std::move(xa);
+  } else if (cur_buf == xb.get()) {
+    out.encoded = std::move(xb);
+  } else {
+    out.encoded = DBuf<int32_t>(pool, std::max<uint64_t>(L, 1));
+    if (L)
+      GFZ_HIP(hipMemcpyAsync(out.encoded.get(), d_nodes, L * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+  }
+  out.rules_first = std::move(rf);
+  out.rules_second = std::move(rs);
+  ctx.sync();
+}
+
+}  // namespace gfz

@@ -1,0 +1,140 @@
+#include "ctx.hpp"
+
+#include <algorithm>
+
+namespace gfz {
+
+Pool::~Pool() { trim(); }
+
+void *Pool::get(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  std::lock_guard<std::mutex> lk(mu_);
+  // Smallest cached block that fits and wastes at most half of itself.
+  auto it = free_.lower_bound(bytes);
+  if (it != free_.end() && it->first <= 2 * bytes) {
+    void *p = it->second;
+    cached_ -= it->first;
+    live_ += it->first;
+    free_.erase(it);
+    return p;
+  }
+  if (limit_ && live_ + cached_ + bytes > limit_) {
+    for (auto &kv : free_)
+      (void)hipFree(kv.second), size_of_.erase(kv.second);
+    free_.clear();
+    cached_ = 0;
+    if (live_ + bytes > limit_)
+      throw Error(kErrOutOfMemory, "device pool limit reached (" + std::to_string(live_ + bytes) +
+                                       " > " + std::to_string(limit_) + " bytes)");
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    for (auto &kv : free_)
+      (void)hipFree(kv.second), size_of_.erase(kv.second);
+    free_.clear();
+    cached_ = 0;
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(kErrOutOfMemory, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  size_of_[p] = bytes;
+  live_ += bytes;
+  return p;
+}
+
+void Pool::put(void *p) {
+  if (!p)
+    return;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = size_of_.find(p);
+  if (it == size_of_.end())
+    return;
+  live_ -= it->second;
+  cached_ += it->second;
+  free_.emplace(it->second, p);
+}
+
+void Pool::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto &kv : free_) {
+    (void)hipFree(kv.second);
+    size_of_.erase(kv.second);
+  }
+  free_.clear();
+  cached_ = 0;
+}
+
+Ctx::Ctx(int device) : device_(device) {
+  DeviceGuard g(device);
+  GFZ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    num_cus_ = std::max(1, prop.multiProcessorCount);
+}
+
+Ctx::~Ctx() {
+  (void)hipSetDevice(device_);
+  if (stream_)
+    (void)hipStreamSynchronize(stream_);
+  for (auto &p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : free_events_)
+    (void)hipEventDestroy(e);
+  if (pinned_)
+    (void)hipHostFree(pinned_);
+  pool_.trim();
+  if (stream_)
+    (void)hipStreamDestroy(stream_);
+}
+
+hipEvent_t Ctx::event() {
+  if (!free_events_.empty()) {
+    hipEvent_t e = free_events_.back();
+    free_events_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  GFZ_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::sync() {
+  GFZ_HIP(hipStreamSynchronize(stream_));
+  collect();
+}
+
+void Ctx::collect() {
+  for (auto &p : pending_) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto &s = stats_[p.name];
+      s.launches += 1;
+      s.total_ms += ms;
+    }
+    free_events_.push_back(p.a);
+    free_events_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+void *Ctx::pinned(size_t bytes) {
+  if (bytes <= pinned_bytes_)
+    return pinned_;
+  if (pinned_) {
+    GFZ_HIP(hipStreamSynchronize(stream_));
+    GFZ_HIP(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    pinned_bytes_ = 0;
+  }
+  GFZ_HIP(hipHostMalloc(&pinned_, bytes, hipHostMallocDefault));
+  pinned_bytes_ = bytes;
+  return pinned_;
+}
+
+}  // namespace gfz

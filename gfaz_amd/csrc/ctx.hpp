@@ -1,0 +1,157 @@
+// Device context: one per (device, caller). Owns a stream, a caching pool of
+// device buffers (workspaces are reused across calls instead of hipMalloc'd per
+// stage) and optional per-kernel HIP-event timing on that stream.
+#pragma once
+
+#include "common.hpp"
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace gfz {
+
+class Pool {
+ public:
+  Pool() = default;
+  ~Pool();
+  void *get(size_t bytes);
+  void put(void *p);
+  void trim();
+  size_t cached_bytes() const { return cached_; }
+  size_t live_bytes() const { return live_; }
+  // Optional cap on live + cached bytes (0 = none); get() throws kErrOutOfMemory
+  // past it after trimming the cache.
+  void set_limit(size_t bytes) { limit_ = bytes; }
+
+ private:
+  std::multimap<size_t, void *> free_;
+  std::unordered_map<void *, size_t> size_of_;
+  size_t cached_ = 0, live_ = 0, limit_ = 0;
+  std::mutex mu_;
+};
+
+// RAII device buffer from a Pool.
+template <class T>
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(Pool &pool, size_t n) : pool_(&pool), n_(n) {
+    ptr_ = static_cast<T *>(pool.get(std::max<size_t>(1, n) * sizeof(T)));
+  }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept { swap(o); }
+  DBuf &operator=(DBuf &&o) noexcept {
+    reset();
+    swap(o);
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() {
+    if (ptr_ && pool_)
+      pool_->put(ptr_);
+    ptr_ = nullptr;
+    n_ = 0;
+  }
+  void swap(DBuf &o) noexcept {
+    std::swap(pool_, o.pool_);
+    std::swap(ptr_, o.ptr_);
+    std::swap(n_, o.n_);
+  }
+  T *get() const { return ptr_; }
+  size_t size() const { return n_; }
+  // Hand ownership to the caller (returned to the pool by Pool::put later).
+  T *release() {
+    T *p = ptr_;
+    ptr_ = nullptr;
+    n_ = 0;
+    return p;
+  }
+
+ private:
+  Pool *pool_ = nullptr;
+  T *ptr_ = nullptr;
+  size_t n_ = 0;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double total_ms = 0;
+  double bytes = 0;  // algorithmic bytes accumulated by the caller
+};
+
+class Ctx {
+ public:
+  explicit Ctx(int device);
+  ~Ctx();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  Pool &pool() { return pool_; }
+  int num_cus() const { return num_cus_; }
+
+  // Per-kernel timing: when enabled, launches wrapped in timed() record a
+  // start/stop event pair on the context stream; collect() accumulates the
+  // elapsed times after the stream has been synchronised.
+  void set_profiling(bool on) { profiling_ = on; }
+  bool profiling() const { return profiling_; }
+  template <class F>
+  void timed(const char *name, F &&launch) {
+    if (!profiling_) {
+      launch();
+      return;
+    }
+    hipEvent_t a = event(), b = event();
+    GFZ_HIP(hipEventRecord(a, stream_));
+    launch();
+    GFZ_HIP(hipEventRecord(b, stream_));
+    pending_.push_back({name, a, b});
+  }
+  void sync();  // hipStreamSynchronize + collect()
+  void collect();
+  void reset_stats() { stats_.clear(); }
+  const std::map<std::string, KernelStat> &stats() const { return stats_; }
+  void add_bytes(const char *name, double bytes) {
+    if (profiling_)
+      stats_[name].bytes += bytes;
+  }
+
+  // Pinned host staging buffer of at least `bytes`, reused across calls.
+  void *pinned(size_t bytes);
+
+ private:
+  hipEvent_t event();
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  int device_;
+  int num_cus_ = 256;
+  hipStream_t stream_ = nullptr;
+  Pool pool_;
+  bool profiling_ = false;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> free_events_;
+  std::map<std::string, KernelStat> stats_;
+  void *pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
+};
+
+// RAII: make `ctx`'s device current for the duration of a call.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    GFZ_HIP(hipGetDevice(&prev));
+    if (prev != dev)
+      GFZ_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev)
+      (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace gfz

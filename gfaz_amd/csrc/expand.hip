@@ -1,0 +1,442 @@
+// Rule expansion and inverse delta on gfx950 (decompress / extract).
+//
+// Restates src/workflows/decompression_workflow.cpp:70-174 (recursive
+// expand_rule, then delta_round inverse prefix sums per traversal):
+//   expand(+r) = expand(first[r]) ++ expand(second[r])
+//   expand(-r) = expand(-second[r]) ++ expand(-first[r])
+//   a symbol is a rule iff min_id <= |s| < min_id + R.
+// Instead of recursing per encoded symbol, every output position finds its
+// encoded symbol (binary search over the symbol offsets staged in LDS) and
+// descends the rule tree by expanded sizes, so one lane writes one output
+// element and a wave's stores are consecutive.
+#include "codec.hpp"
+#include "prims.hpp"
+
+#include <algorithm>
+
+namespace gfz {
+
+namespace {
+
+constexpr uint32_t kExItems = 16;
+constexpr uint32_t kExTile = kBlock * kExItems;  // output elements per block
+constexpr uint32_t kSsItems = 16;
+constexpr uint32_t kSsTile = kBlock * kSsItems;
+constexpr uint32_t kSsScanThreads = 1024;
+
+__device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
+  const uint32_t a = uabs(v);
+  return a >= min_id && (uint64_t)(a - min_id) < R;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict__ f,
+                                                      const int32_t *__restrict__ s, uint64_t R,
+                                                      uint32_t min_id, RuleRec *__restrict__ rec,
+                                                      uint32_t *__restrict__ size) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * kBlock) {
+    const int32_t a = f[r], b = s[r];
+    RuleRec q;
+    q.a = a;
+    q.b = b;
+    q.sa = is_rule(a, min_id, R) ? 0u : 1u;
+    q.sb = is_rule(b, min_id, R) ? 0u : 1u;
+    rec[r] = q;
+    size[r] = (q.sa && q.sb) ? 2u : 0u;
+  }
+}
+
+// One bottom-up pass: a rule whose children both have known sizes gets its
+// size; depth d rules resolve in pass d.
+__global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
+                                                      uint32_t *__restrict__ size, uint64_t R,
+                                                      uint32_t min_id, uint32_t *changed,
+                                                      uint32_t *err) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * kBlock) {
+    if (size[r])
+      continue;
+    RuleRec q = rec[r];
+    const uint32_t sa = q.sa ? q.sa : __hip_atomic_load(&size[uabs(q.a) - min_id],
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t sb = q.sb ? q.sb : __hip_atomic_load(&size[uabs(q.b) - min_id],
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (sa && sb) {
+      const uint32_t sum = sa + sb;
+      if (sum < sa) {
+        atomicExch(err, 1u);
+        continue;
+      }
+      q.sa = sa;
+      q.sb = sb;
+      rec[r] = q;
+      __hip_atomic_store(&size[r], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicExch(changed, 1u);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rule_unresolved(const uint32_t *__restrict__ size,
+                                                            uint64_t R, uint32_t *err) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * kBlock)
+    if (!size[r])
+      atomicExch(err, 2u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sym_sizes(const int32_t *__restrict__ enc, uint64_t n,
+                                                      const uint32_t *__restrict__ size, uint64_t R,
+                                                      uint32_t min_id, uint32_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const int32_t v = enc[i];
+    out[i] = is_rule(v, min_id, R) ? size[uabs(v) - min_id] : 1u;
+  }
+}
+
+__global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *__restrict__ idx,
+                             uint64_t n, uint64_t *__restrict__ dst) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+__device__ inline uint64_t upper_bound_u64(const uint64_t *a, uint64_t n, uint64_t v) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Each block owns kExTile consecutive output positions.
+__global__ __launch_bounds__(kBlock) void k_expand(const int32_t *__restrict__ enc,
+                                                   const uint64_t *__restrict__ sym_off,
+                                                   uint64_t n_enc, uint64_t n_out,
+                                                   const RuleRec *__restrict__ rec, uint64_t R,
+                                                   uint32_t min_id, int32_t *__restrict__ out) {
+  __shared__ uint64_t soff[kExTile];
+  __shared__ int32_t ssym[kExTile];
+  __shared__ uint64_t srange[2];
+  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
+  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+  if (threadIdx.x == 0) {
+    srange[0] = upper_bound_u64(sym_off, n_enc + 1, o0) - 1;
+    srange[1] = upper_bound_u64(sym_off, n_enc + 1, o_end - 1) - 1;
+  }
+  __syncthreads();
+  const uint64_t k0 = srange[0];
+  const uint32_t nsym = (uint32_t)(srange[1] - k0 + 1);
+  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
+    soff[s] = sym_off[k0 + s];
+    ssym[s] = enc[k0 + s];
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < kExItems; ++j) {
+    const uint64_t o = o0 + (uint64_t)j * kBlock + threadIdx.x;
+    if (o >= o_end)
+      break;
+    uint32_t lo = 0, hi = nsym - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (soff[mid] <= o)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    uint64_t q = o - soff[lo];
+    int32_t v = ssym[lo];
+    while (is_rule(v, min_id, R)) {
+      const RuleRec c = rec[uabs(v) - min_id];
+      int32_t left, right;
+      uint32_t ls;
+      if (v > 0) {
+        left = c.a;
+        right = c.b;
+        ls = c.sa;
+      } else {
+        left = -c.b;
+        right = -c.a;
+        ls = c.sb;
+      }
+      if (q < ls) {
+        v = left;
+      } else {
+        q -= ls;
+        v = right;
+      }
+    }
+    out[o] = v;
+  }
+}
+
+// ---- segmented inclusive prefix sum -----------------------------------------
+// Value (sum, flag): flag = a segment head was seen. (a) then (b) =
+// (b.flag ? b.sum : a.sum + b.sum, a.flag | b.flag).
+struct Seg {
+  uint32_t sum;
+  uint32_t flag;
+};
+__device__ inline Seg seg_then(Seg a, Seg b) {
+  return Seg{b.flag ? b.sum : a.sum + b.sum, a.flag | b.flag};
+}
+
+template <uint32_t NT>
+__device__ inline Seg block_seg_exclusive(Seg v, Seg *lds, Seg &total) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  Seg x = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    Seg g;
+    g.sum = __shfl_up(x.sum, d);
+    g.flag = __shfl_up(x.flag, d);
+    if (lane >= (uint32_t)d)
+      x = seg_then(g, x);
+  }
+  if (lane == kWave - 1)
+    lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Seg run{0, 0};
+    for (uint32_t w = 0; w < NT / kWave; ++w) {
+      const Seg t = lds[w];
+      lds[w] = run;
+      run = seg_then(run, t);
+    }
+    lds[NT / kWave] = run;
+  }
+  __syncthreads();
+  Seg ex;
+  ex.sum = __shfl_up(x.sum, 1);
+  ex.flag = __shfl_up(x.flag, 1);
+  if (lane == 0)
+    ex = Seg{0, 0};
+  const Seg res = seg_then(lds[wid], ex);
+  total = lds[NT / kWave];
+  __syncthreads();
+  return res;
+}
+
+__device__ inline bool bit_at(const uint32_t *bits, uint64_t i) {
+  return (bits[i >> 5] >> (i & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_reduce(const int32_t *__restrict__ x, uint64_t n,
+                                                       const uint32_t *__restrict__ heads,
+                                                       Seg *__restrict__ agg) {
+  __shared__ Seg lds[kBlock / kWave + 1];
+  const uint64_t base = (uint64_t)blockIdx.x * kSsTile + (uint64_t)threadIdx.x * kSsItems;
+  Seg a{0, 0};
+#pragma unroll
+  for (uint32_t j = 0; j < kSsItems; ++j) {
+    const uint64_t i = base + j;
+    if (i < n)
+      a = seg_then(a, Seg{(uint32_t)x[i], bit_at(heads, i) ? 1u : 0u});
+  }
+  Seg total;
+  block_seg_exclusive<kBlock>(a, lds, total);
+  if (threadIdx.x == 0)
+    agg[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kSsScanThreads) void k_seg_tiles(Seg *agg, uint64_t nb) {
+  __shared__ Seg lds[kSsScanThreads / kWave + 1];
+  const uint64_t chunk = (nb + kSsScanThreads - 1) / kSsScanThreads;
+  const uint64_t begin = (uint64_t)threadIdx.x * chunk;
+  const uint64_t end = begin + chunk < nb ? begin + chunk : nb;
+  Seg s{0, 0};
+  for (uint64_t i = begin; i < end; ++i)
+    s = seg_then(s, agg[i]);
+  Seg total;
+  Seg run = block_seg_exclusive<kSsScanThreads>(s, lds, total);
+  for (uint64_t i = begin; i < end; ++i) {
+    const Seg t = agg[i];
+    agg[i] = run;
+    run = seg_then(run, t);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_apply(int32_t *__restrict__ x, uint64_t n,
+                                                      const uint32_t *__restrict__ heads,
+                                                      const Seg *__restrict__ carry) {
+  __shared__ Seg lds[kBlock / kWave + 1];
+  const uint64_t base = (uint64_t)blockIdx.x * kSsTile + (uint64_t)threadIdx.x * kSsItems;
+  uint32_t v[kSsItems];
+  uint32_t hb = 0;
+  Seg a{0, 0};
+#pragma unroll
+  for (uint32_t j = 0; j < kSsItems; ++j) {
+    const uint64_t i = base + j;
+    v[j] = i < n ? (uint32_t)x[i] : 0u;
+    const uint32_t h = (i < n && bit_at(heads, i)) ? 1u : 0u;
+    hb |= h << j;
+    a = seg_then(a, Seg{v[j], h});
+  }
+  Seg total;
+  Seg run = seg_then(carry[blockIdx.x], block_seg_exclusive<kBlock>(a, lds, total));
+#pragma unroll
+  for (uint32_t j = 0; j < kSsItems; ++j) {
+    const uint64_t i = base + j;
+    run = seg_then(run, Seg{v[j], (hb >> j) & 1u});
+    if (i < n)
+      x[i] = (int32_t)run.sum;
+  }
+}
+
+uint32_t stride_grid(Ctx &ctx, uint64_t n) {
+  const uint64_t want = ceil_div(n, kBlock * 4);
+  const uint64_t cap = (uint64_t)ctx.num_cus() * 8;
+  return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+}  // namespace
+
+void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, uint64_t R,
+                    uint32_t min_id, Rulebook &rb) {
+  rb = Rulebook();
+  rb.n_rules = R;
+  rb.min_id = min_id;
+  if ((uint64_t)min_id + R > (uint64_t)INT32_MAX + 1)
+    throw Error(kErrCorrupt, "rule-ID window exceeds the int32 symbol range");
+  rb.rec = DBuf<RuleRec>(ctx.pool(), std::max<uint64_t>(R, 1));
+  rb.size = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
+  if (!R)
+    return;
+  DBuf<uint32_t> flags(ctx.pool(), 2);  // changed, err
+  hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get());
+  GFZ_CHECK_LAUNCH();
+  uint32_t depth = 1;
+  for (uint64_t pass = 0; pass <= R; ++pass) {
+    fill_u32(ctx, flags.get(), 0, 2);
+    ctx.timed("rule_sizes", [&] {
+      hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                         rb.rec.get(), rb.size.get(), R, min_id, flags.get(), flags.get() + 1);
+    });
+    GFZ_CHECK_LAUNCH();
+    uint32_t h[2];
+    GFZ_HIP(hipMemcpyAsync(h, flags.get(), sizeof(h), hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+    if (h[1])
+      throw Error(kErrCorrupt, "rule expansion size exceeds 2^32 symbols");
+    if (!h[0])
+      break;
+    ++depth;
+  }
+  hipLaunchKernelGGL(k_rule_unresolved, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                     rb.size.get(), R, flags.get() + 1);
+  GFZ_CHECK_LAUNCH();
+  uint32_t e = 0;
+  GFZ_HIP(hipMemcpyAsync(&e, flags.get() + 1, 4, hipMemcpyDeviceToHost, ctx.stream()));
+  ctx.sync();
+  if (e)
+    throw Error(kErrCorrupt, "rulebook is cyclic: some rule never bottoms out in node IDs");
+  rb.depth = depth;
+}
+
+void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                 const uint32_t *enc_lengths, const uint32_t *orig_lengths, uint64_t n_trav,
+                 DecodePlan &plan, DBuf<uint64_t> &sym_off) {
+  plan = DecodePlan();
+  uint64_t total_enc = 0;
+  std::vector<uint64_t> enc_start(n_trav);
+  for (uint64_t t = 0; t < n_trav; ++t) {
+    enc_start[t] = total_enc;
+    total_enc += enc_lengths[t];
+  }
+  if (total_enc != n_enc)
+    throw Error(kErrCorrupt, "flattened sequence block is truncated (lengths sum " +
+                                 std::to_string(total_enc) + ", block holds " +
+                                 std::to_string(n_enc) + ")");
+  sym_off = DBuf<uint64_t>(ctx.pool(), n_enc + 1);
+  {
+    DBuf<uint32_t> sz(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    if (n_enc) {
+      hipLaunchKernelGGL(k_sym_sizes, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0,
+                         ctx.stream(), d_enc, n_enc, rb.size.get(), rb.n_rules, rb.min_id,
+                         sz.get());
+      GFZ_CHECK_LAUNCH();
+    }
+    exclusive_scan(ctx, sz.get(), sym_off.get(), n_enc);
+  }
+  // Decoded offset of each traversal = symbol offset at its first encoded symbol.
+  plan.out_offsets.assign(n_trav + 1, 0);
+  if (n_trav) {
+    DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
+    std::vector<uint64_t> h_idx(enc_start);
+    h_idx.push_back(n_enc);
+    GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx.data(), (n_trav + 1) * 8, hipMemcpyHostToDevice,
+                           ctx.stream()));
+    hipLaunchKernelGGL(k_gather_u64, dim3(stride_grid(ctx, n_trav + 1)), dim3(kBlock), 0,
+                       ctx.stream(), sym_off.get(), idx.get(), n_trav + 1, val.get());
+    GFZ_CHECK_LAUNCH();
+    GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
+                           hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+  } else {
+    GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), sym_off.get() + n_enc, 8,
+                           hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+  }
+  plan.n_out = plan.out_offsets[n_trav];
+  if (orig_lengths)
+    for (uint64_t t = 0; t < n_trav; ++t)
+      if (plan.out_offsets[t + 1] - plan.out_offsets[t] != orig_lengths[t])
+        plan.length_mismatch = true;
+}
+
+void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                const uint32_t *enc_lengths, uint64_t n_trav, int delta_round,
+                const DecodePlan &plan, const uint64_t *d_sym_off, int32_t *d_out) {
+  const uint64_t n_out = plan.n_out;
+  if (!n_out)
+    return;
+  const uint32_t nb = (uint32_t)ceil_div(n_out, kExTile);
+  ctx.timed("expand", [&] {
+    hipLaunchKernelGGL(k_expand, dim3(nb), dim3(kBlock), 0, ctx.stream(), d_enc, d_sym_off, n_enc,
+                       n_out, rb.rec.get(), rb.n_rules, rb.min_id, d_out);
+  });
+  GFZ_CHECK_LAUNCH();
+  ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
+  if (delta_round <= 0)
+    return;
+  // Heads at the decoded start of every non-empty traversal.
+  std::vector<uint64_t> starts;
+  for (uint64_t t = 0; t < n_trav; ++t)
+    if (enc_lengths[t] && plan.out_offsets[t + 1] > plan.out_offsets[t])
+      starts.push_back(plan.out_offsets[t]);
+  const uint64_t words = ceil_div(n_out, 32);
+  DBuf<uint32_t> heads(ctx.pool(), words);
+  fill_u32(ctx, heads.get(), 0, words);
+  if (!starts.empty()) {
+    DBuf<uint64_t> d_starts(ctx.pool(), starts.size());
+    GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
+                           hipMemcpyHostToDevice, ctx.stream()));
+    set_head_bits(ctx, d_starts.get(), starts.size(), heads.get(), n_out);
+  }
+  for (int r = 0; r < delta_round; ++r)
+    segmented_prefix_sum(ctx, d_out, n_out, heads.get());
+}
+
+void segmented_prefix_sum(Ctx &ctx, int32_t *x, uint64_t n, const uint32_t *heads) {
+  if (!n)
+    return;
+  const uint64_t nb = ceil_div(n, kSsTile);
+  DBuf<Seg> agg(ctx.pool(), nb);
+  ctx.timed("inverse_delta", [&] {
+    hipLaunchKernelGGL(k_seg_reduce, dim3((uint32_t)nb), dim3(kBlock), 0, ctx.stream(), x, n,
+                       heads, agg.get());
+    GFZ_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(kSsScanThreads), 0, ctx.stream(), agg.get(), nb);
+    GFZ_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_seg_apply, dim3((uint32_t)nb), dim3(kBlock), 0, ctx.stream(), x, n, heads,
+                       agg.get());
+    GFZ_CHECK_LAUNCH();
+  });
+}
+
+}  // namespace gfz

@@ -1,0 +1,73 @@
+// Device-wide primitives written for gfx950 (no hipCUB/rocPRIM on this path):
+// exclusive scans, LSD radix sort of signed 64-bit keys, fills, head bitmaps.
+#pragma once
+
+#include "ctx.hpp"
+
+namespace gfz {
+
+// Block-wide exclusive scan for NT threads; `lds` needs NT/kWave + 1 slots.
+// Returns this thread's exclusive prefix and the block total.
+template <uint32_t NT, typename T>
+__device__ inline T block_exclusive_scan(T v, T *lds, T &total) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  T x = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    T y = __shfl_up(x, d);
+    if (lane >= (uint32_t)d)
+      x += y;
+  }
+  if (lane == kWave - 1)
+    lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (uint32_t w = 0; w < NT / kWave; ++w) {
+      T t = lds[w];
+      lds[w] = run;
+      run += t;
+    }
+    lds[NT / kWave] = run;
+  }
+  __syncthreads();
+  T res = lds[wid] + x - v;
+  total = lds[NT / kWave];
+  __syncthreads();
+  return res;
+}
+
+template <typename T>
+__device__ inline T wave_reduce_max(T v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    T o = __shfl_xor(v, d);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// out[0..n) = exclusive prefix of in, out[n] = total (out holds n + 1).
+void exclusive_scan(Ctx &ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+void exclusive_scan(Ctx &ctx, const uint64_t *in, uint64_t *out, uint64_t n);
+
+// Sort n signed 64-bit keys ascending (as int64). `keys` is sorted in place
+// (tmp must hold n keys). Keys must be distinct or the order of equal keys is
+// irrelevant to the caller.
+void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n);
+
+void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
+void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);
+void fill_u8(Ctx &ctx, uint8_t *p, uint8_t v, uint64_t n);
+
+// Set bit off[t] of `heads` (pre-zeroed, ceil(n_pos/32) words) for every
+// t < n_off with off[t] < n_pos.
+void set_head_bits(Ctx &ctx, const uint64_t *off, uint64_t n_off, uint32_t *heads,
+                   uint64_t n_pos);
+
+// Positions of the set bits of `bits` (n_bits bits) in ascending order into
+// `pos` (at most pos_cap written); returns the full count (synchronises).
+uint64_t bitmap_positions(Ctx &ctx, const uint32_t *bits, uint64_t n_bits, uint64_t *pos,
+                          uint64_t pos_cap);
+
+}  // namespace gfz

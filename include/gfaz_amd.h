@@ -1,0 +1,142 @@
+/* gfaz_amd - MI355X (gfx950) grammar codec for GFA path/walk traversals.
+ *
+ * C ABI: plain pointers and sizes, integer status codes, no exceptions and no
+ * C++ or torch types across the boundary. Every entry point returns GFZ_OK (0)
+ * or a negative GFZ_E* code; gfz_last_error() gives the calling thread's last
+ * message (prefixed like the reference's exceptions, e.g. "GPU compression
+ * error: ...").
+ *
+ * Traversal layout everywhere: one flat int32 array of signed node IDs (sign =
+ * orientation), paths first then walks, plus one uint32 length per traversal -
+ * the reference's FlattenedPaths (include/gpu/core/gfa_graph_gpu.hpp:20-26).
+ *
+ * Ownership: the caller owns every input pointer, which stays read-only for the
+ * duration of the call. Buffers the library returns are released through the
+ * matching gfz_*_free call. Device memory belongs to a gfz_ctx (one per device
+ * and calling thread; calls on one ctx are serialised by the caller, calls on
+ * different ctxs are independent). */
+#ifndef GFAZ_AMD_H
+#define GFAZ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GFZ_OK = 0,
+  GFZ_E_INVALID = -1,   /* bad argument */
+  GFZ_E_HIP = -2,       /* HIP runtime error */
+  GFZ_E_NOMEM = -3,     /* device or host allocation failed / budget exceeded */
+  GFZ_E_OVERFLOW = -4,  /* ID space exhausted (codec.cpp overflow_error analogue) */
+  GFZ_E_CORRUPT = -5,   /* malformed rulebook or encoded stream */
+  GFZ_E_BUSY = -6       /* serving queue full or request not finished */
+};
+
+/* Flags for pointer arguments. */
+enum {
+  GFZ_INPUT_DEVICE = 1u,  /* node / encoded arrays are device pointers */
+  GFZ_OUTPUT_DEVICE = 2u  /* results stay in device memory */
+};
+
+typedef struct gfz_ctx gfz_ctx;
+typedef struct gfz_rulebook gfz_rulebook;
+
+const char *gfz_last_error(void);
+const char *gfz_version(void);
+
+/* One context per device: a HIP stream, a caching device pool and optional
+ * per-kernel event timing. */
+int gfz_ctx_create(int device, gfz_ctx **out);
+void gfz_ctx_destroy(gfz_ctx *ctx);
+int gfz_ctx_sync(gfz_ctx *ctx);
+/* Cap on device bytes the ctx may hold (0 = none): allocations beyond it fail
+ * with GFZ_E_NOMEM instead of reaching hipMalloc. */
+int gfz_ctx_set_memory_limit(gfz_ctx *ctx, uint64_t bytes);
+int gfz_ctx_set_profiling(gfz_ctx *ctx, int on);
+int gfz_ctx_reset_stats(gfz_ctx *ctx);
+/* Accumulated launches / milliseconds / algorithmic bytes of a timed kernel
+ * group ("expand", "pair_insert", "emit", ...); zeros if never timed. */
+int gfz_ctx_kernel_stat(gfz_ctx *ctx, const char *name, uint64_t *launches, double *total_ms,
+                        double *bytes);
+/* Number of CUs of the ctx's device. */
+int gfz_ctx_num_cus(gfz_ctx *ctx);
+
+/* Device memory helpers (pool-backed) for callers without their own allocator. */
+int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out);
+int gfz_dev_free(gfz_ctx *ctx, void *ptr);
+int gfz_memcpy_h2d(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int gfz_memcpy_d2h(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- compress ------------------------------------------------------------
+ * Replaces run_path_compression_gpu (include/gpu/compression/
+ * compression_workflow_gpu.hpp:75-78), whose result the reference turns into
+ * CompressedData; here the traversal grammar itself: the same encoded stream,
+ * encoded lengths, rule columns and rule-ID window that the reference CPU
+ * compressor produces (src/workflows/compression_workflow.cpp:235-279,548-700),
+ * honouring delta_round (the reference GPU path forces 1,
+ * src/gpu/compression/compression_workflow_gpu.cu:268). threshold is accepted
+ * for interface parity; as in the reference CPU path every pair seen twice
+ * becomes a rule (src/grammar/rule_generator.cpp:242-246). */
+typedef struct gfz_compress_out {
+  int32_t *encoded;           /* n_encoded symbols; host or device per flags */
+  uint64_t n_encoded;
+  uint32_t *encoded_lengths;  /* host, one per input traversal */
+  uint64_t n_trav;
+  int32_t *rules_first;       /* n_rules, raw (not delta-coded), rule-ID order */
+  int32_t *rules_second;      /* host or device per flags */
+  uint64_t n_rules;
+  uint32_t start_id;          /* LayerRuleRange{2, start_id, end_id, 0, 2*n_rules} */
+  uint32_t end_id;
+  int32_t rounds_run;
+  uint32_t flags;             /* copy of the call's flags */
+  uint64_t round_rules[32];   /* rules created per round */
+  uint64_t round_lengths[32]; /* stream length after each round */
+  void *internal;             /* owned device buffers */
+} gfz_compress_out;
+
+int gfz_compress_traversals(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                            const uint32_t *lengths, uint64_t n_trav, uint32_t n_paths,
+                            int rounds, int threshold, int delta_round, uint32_t flags,
+                            gfz_compress_out *out);
+void gfz_compress_free(gfz_compress_out *out);
+
+/* ---- decompress ----------------------------------------------------------
+ * gfz_rulebook_load replaces prepare_gpu_traversal_rulebook
+ * (src/gpu/decompression/decompression_primitives_gpu.cu:18-39): rule columns
+ * already prefix-summed (raw), min_rule_id = layer_rule_ranges[0].start_id
+ * (include/model/compressed_data.hpp:49-51). Expanded rule sizes are computed
+ * on the device; a cyclic or out-of-range rulebook fails with GFZ_E_CORRUPT. */
+int gfz_rulebook_load(gfz_ctx *ctx, const int32_t *rules_first, const int32_t *rules_second,
+                      uint64_t n_rules, uint32_t min_rule_id, uint32_t flags,
+                      gfz_rulebook **out);
+void gfz_rulebook_free(gfz_rulebook *rb);
+/* Expanded length of one rule (for tests and planning). */
+int gfz_rulebook_rule_size(gfz_rulebook *rb, uint64_t rule_index, uint32_t *size);
+
+/* Total decoded length of a batch of encoded traversals. */
+int gfz_decoded_size(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                     uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                     uint32_t flags, uint64_t *n_out);
+
+/* Replaces decompress_paths_gpu / decompress_gpu_traversal_materialized
+ * (include/gpu/decompression/decompression_workflow_gpu.hpp:32-33,
+ * include/gpu/decompression/traversal_decode_gpu.hpp:115-119): expand every
+ * traversal, then undo delta_round prefix sums per traversal. out holds
+ * out_cap int32 (host or device per flags); *n_out receives the decoded total
+ * and out_lengths (host, n_trav, may be NULL) the decoded per-traversal
+ * lengths. If orig_lengths is given and a decoded length differs, the call
+ * still succeeds and sets *length_mismatch (the reference only warns,
+ * src/workflows/decompression_workflow.cpp:144-150). */
+int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                              uint64_t n_encoded, const uint32_t *encoded_lengths,
+                              const uint32_t *orig_lengths, uint64_t n_trav, int delta_round,
+                              int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
+                              uint32_t *out_lengths, int *length_mismatch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFAZ_AMD_H */

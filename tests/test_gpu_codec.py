@@ -1,0 +1,157 @@
+"""GPU parity tests: the HIP codec through the C ABI against the reference's golden
+outputs and the CPU oracle, bit-exact (integer work, no tolerance)."""
+import sys
+
+import numpy as np
+import pytest
+
+import goldens
+from gfaz_amd import codec, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check_compress(ctx, nodes, lengths, rounds, delta, n_paths=0, threshold=2, ref=None):
+    say(f"  compress n={nodes.size} trav={lengths.size} rounds={rounds} delta={delta}")
+    got = codec.compress_traversals(ctx, nodes, lengths, n_paths, rounds, threshold, delta)
+    if ref is None:
+        ref = oracle.compress(nodes, lengths, rounds, threshold, delta, threads=8)
+    assert (got.start_id, got.end_id) == (ref["start_id"], ref["end_id"])
+    np.testing.assert_array_equal(got.encoded_lengths, ref["encoded_lengths"])
+    np.testing.assert_array_equal(got.encoded, ref["encoded"])
+    np.testing.assert_array_equal(got.rules_first, ref["rules_first"])
+    np.testing.assert_array_equal(got.rules_second, ref["rules_second"])
+    return got
+
+
+def check_decompress(ctx, nodes, lengths, comp, delta):
+    say(f"  decompress enc={comp['encoded'].size} rules={comp['rules_first'].size}")
+    rb = codec.Rulebook(ctx, comp["rules_first"], comp["rules_second"], comp["start_id"])
+    dec, lens, mism = codec.decompress_traversals(ctx, rb, comp["encoded"],
+                                                  comp["encoded_lengths"], lengths, delta)
+    rb.close()
+    assert not mism
+    np.testing.assert_array_equal(lens, lengths)
+    np.testing.assert_array_equal(dec, nodes)
+
+
+@pytest.mark.parametrize("name", goldens.names())
+def test_compress_matches_reference_golden(gpu_ctx, name):
+    say(f"golden compress {name}")
+    g = goldens.load(name)
+    nodes, lengths, n_paths = goldens.inputs(g)
+    m = g["meta"]
+    exp = goldens.expected(g)
+    check_compress(gpu_ctx, nodes, lengths, m["rounds"], m["delta_round"], n_paths,
+                   m["threshold"], ref=exp)
+
+
+@pytest.mark.parametrize("name", goldens.names())
+def test_decompress_reference_golden(gpu_ctx, name):
+    say(f"golden decompress {name}")
+    g = goldens.load(name)
+    nodes, lengths, _ = goldens.inputs(g)
+    exp = goldens.expected(g)
+    check_decompress(gpu_ctx, nodes, lengths, exp, exp["delta_round"])
+
+
+def _random_case(seed, n_trav, max_len, alphabet, zero_frac=0.1):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, max_len + 1, size=n_trav).astype(np.uint32)
+    lens[rng.random(n_trav) < zero_frac] = 0
+    n = int(lens.sum(dtype=np.uint64))
+    nodes = (rng.integers(1, alphabet + 1, size=n) * rng.choice([-1, 1], size=n)).astype(np.int32)
+    return nodes, lens
+
+
+@pytest.mark.parametrize("seed,n_trav,max_len,alphabet,rounds,delta", [
+    (1, 1, 5000, 2, 8, 0),       # dense candidate runs crossing tile boundaries
+    (2, 200, 3000, 3, 8, 1),
+    (3, 50, 20000, 5, 8, 2),
+    (4, 1000, 40, 2, 3, 0),      # many short traversals, empty ones, single steps
+    (5, 7, 100000, 4, 8, 1),
+    (6, 3, 1, 2, 8, 1),          # nothing to pair
+])
+def test_random_matches_oracle(gpu_ctx, seed, n_trav, max_len, alphabet, rounds, delta):
+    say(f"random seed={seed}")
+    nodes, lengths = _random_case(seed, n_trav, max_len, alphabet)
+    ref = oracle.compress(nodes, lengths, rounds, 2, delta, threads=8)
+    check_compress(gpu_ctx, nodes, lengths, rounds, delta, ref=ref)
+    check_decompress(gpu_ctx, nodes, lengths, ref, delta)
+
+
+def test_long_single_run_across_tiles(gpu_ctx):
+    """One traversal of a repeated node: every pair is the same key, so the greedy
+    parity must carry across thousands of tiles."""
+    say("long run")
+    for n in (4095, 4096, 4097, 8193, 300_001):
+        nodes = np.full(n, 7, dtype=np.int32)
+        lengths = np.array([n], dtype=np.uint32)
+        ref = check_compress(gpu_ctx, nodes, lengths, 8, 1)
+        comp = {"encoded": ref.encoded, "encoded_lengths": ref.encoded_lengths,
+                "rules_first": ref.rules_first, "rules_second": ref.rules_second,
+                "start_id": ref.start_id}
+        check_decompress(gpu_ctx, nodes, lengths, comp, 1)
+
+
+def test_empty_inputs(gpu_ctx):
+    say("empty")
+    for lengths in ([], [0], [0, 0, 0]):
+        lengths = np.array(lengths, dtype=np.uint32)
+        got = codec.compress_traversals(gpu_ctx, np.zeros(0, np.int32), lengths, 0, 8, 2, 1)
+        ref = oracle.compress(np.zeros(0, np.int32), lengths, 8, 2, 1)
+        assert got.encoded.size == 0 and got.n_rules == 0
+        assert (got.start_id, got.end_id) == (ref["start_id"], ref["end_id"]) == (1, 1)
+        np.testing.assert_array_equal(got.encoded_lengths, lengths)
+
+
+@pytest.mark.parametrize("cfg_name,backbone,haps", [("plain_500k", 100_000, 32),
+                                                    ("walks_2m", 200_000, 100)])
+def test_synthetic_scaled_matches_oracle(gpu_ctx, cfg_name, backbone, haps):
+    cfg = synth.CONFIGS[cfg_name].scaled(backbone=backbone, n_haps=haps, len_max=200_000)
+    say(f"synthetic {cfg_name} scaled backbone={backbone}")
+    tr = synth.generate(cfg)
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=8)
+    check_compress(gpu_ctx, tr.nodes, tr.lengths, 8, 1, tr.n_paths, ref=ref)
+    check_decompress(gpu_ctx, tr.nodes, tr.lengths, ref, 1)
+
+
+def test_device_resident_paths_and_determinism(gpu_ctx):
+    say("device resident")
+    tr = synth.generate(synth.CONFIGS["plain_20k"])
+    d_nodes = codec.DeviceArray.from_host(gpu_ctx, tr.nodes)
+    a = codec.compress_traversals(gpu_ctx, d_nodes, tr.lengths, 16, 8, 2, 1)
+    b = codec.compress_traversals(gpu_ctx, tr.nodes, tr.lengths, 16, 8, 2, 1)
+    np.testing.assert_array_equal(a.encoded, b.encoded)
+    np.testing.assert_array_equal(a.rules_first, b.rules_first)
+    rb = codec.Rulebook(gpu_ctx, a.rules_first, a.rules_second, a.start_id)
+    d_enc = codec.DeviceArray.from_host(gpu_ctx, a.encoded)
+    out = codec.DeviceArray(gpu_ctx, tr.n_steps, np.int32)
+    res, lens, mism = codec.decompress_traversals(gpu_ctx, rb, d_enc, a.encoded_lengths,
+                                                  tr.lengths, 1, out=out)
+    assert res is out and not mism
+    np.testing.assert_array_equal(out.to_host(), tr.nodes)
+
+
+def test_corrupt_rulebooks_fail_cleanly(gpu_ctx):
+    say("corrupt rulebooks")
+    # rule 10 = (11, 1), rule 11 = (10, 2): a cycle
+    with pytest.raises(RuntimeError, match="cyclic"):
+        codec.Rulebook(gpu_ctx, [11, 10], [1, 2], 10)
+    # decoded lengths that disagree with the stored originals only flag a mismatch
+    rb = codec.Rulebook(gpu_ctx, [1], [2], 10)
+    dec, lens, mism = codec.decompress_traversals(gpu_ctx, rb, np.array([10, 3], np.int32),
+                                                  np.array([2], np.uint32),
+                                                  np.array([2], np.uint32), 0)
+    assert mism and dec.tolist() == [1, 2, 3] and lens.tolist() == [3]
+    # rule sizes: (1,2) -> 2
+    assert rb.rule_size(0) == 2
+    with pytest.raises(RuntimeError, match="truncated"):
+        codec.decompress_traversals(gpu_ctx, rb, np.array([10], np.int32),
+                                    np.array([5], np.uint32))
