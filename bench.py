@@ -148,7 +148,8 @@ def main() -> int:
     t_total = time.perf_counter() - t_start
     ctx.set_profiling(False)
     groups = ["pair_insert", "cand_tiles", "tile_scan", "emit", "collect_used", "radix_sort",
-              "assign_ids", "remap", "delta", "rule_sizes", "expand", "inverse_delta"]
+              "assign_ids", "remap", "delta", "rule_sizes", "build_bodies", "expand",
+              "inverse_delta"]
     kstats = {g: ctx.kernel_stat(g) for g in groups}
 
     # Correctness of what was timed: exact round trip, and grammar counts.
@@ -157,6 +158,8 @@ def main() -> int:
         log("[bench] ROUND TRIP MISMATCH")
         return 1
     host = last.to_host()
+    round_info = {"lengths": last.lengths_per_round, "rules": last.rules_per_round,
+                  "distinct": last.distinct_per_round, "slots": last.slots_per_round}
     if args.verify_oracle:
         from oracle import oracle
         ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=cpu_threads())
@@ -207,6 +210,9 @@ def main() -> int:
         "decompress_gbps": gfa_bytes / (td / steps) / 1e9,
         "compress_ms": tc / steps * 1e3, "decompress_ms": td / steps * 1e3,
         "encoded_symbols": E, "rules": R, "rounds_run": host.rounds_run,
+        "round_stats": [{"length_out": l, "rules": r, "distinct_pairs": d, "table_slots": t}
+                        for l, r, d, t in zip(round_info["lengths"], round_info["rules"],
+                                              round_info["distinct"], round_info["slots"])],
         "wall_s_timed_region": t_total,
         "roofline": roofline, "expand_roofline": expand_roof,
         "kernel_ms_per_step": {g: kstats[g]["total_ms"] / steps for g in groups

@@ -46,6 +46,8 @@ class _CompressOut(ctypes.Structure):
         ("flags", _u32),
         ("round_rules", _u64 * 32),
         ("round_lengths", _u64 * 32),
+        ("round_distinct", _u64 * 32),
+        ("round_slots", _u64 * 32),
         ("internal", _vp),
     ]
 
@@ -73,6 +75,8 @@ _SIGNATURES = {
     "gfz_rulebook_load": ([_vp, _vp, _vp, _u64, _u32, _u32, ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_rulebook_free": ([_vp], None),
     "gfz_rulebook_rule_size": ([_vp, _u64, ctypes.POINTER(_u32)], ctypes.c_int),
+    "gfz_rulebook_info": ([_vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_decoded_size": ([_vp, _vp, _vp, _u64, _vp, _u64, _u32, ctypes.POINTER(_u64)],
                          ctypes.c_int),
     "gfz_decompress_traversals": ([_vp, _vp, _vp, _u64, _vp, _vp, _u64, ctypes.c_int, _vp, _u64,
@@ -254,6 +258,13 @@ class Rulebook:
         self.n_rules = int(rf.size)
         self.min_rule_id = int(min_rule_id)
 
+    def info(self) -> dict:
+        d, hb, be = _u32(), ctypes.c_int(), _u64()
+        _check(lib().gfz_rulebook_info(self.handle, ctypes.byref(d), ctypes.byref(hb),
+                                       ctypes.byref(be)))
+        return {"max_depth": int(d.value), "has_bodies": bool(hb.value),
+                "body_elems": int(be.value)}
+
     def rule_size(self, index: int) -> int:
         s = _u32()
         _check(lib().gfz_rulebook_rule_size(self.handle, index, ctypes.byref(s)))
@@ -335,6 +346,8 @@ class DeviceCompressed:
         self.rounds_run = int(o.rounds_run)
         self.rules_per_round = [int(o.round_rules[i]) for i in range(min(32, o.rounds_run))]
         self.lengths_per_round = [int(o.round_lengths[i]) for i in range(min(32, o.rounds_run))]
+        self.distinct_per_round = [int(o.round_distinct[i]) for i in range(min(32, o.rounds_run))]
+        self.slots_per_round = [int(o.round_slots[i]) for i in range(min(32, o.rounds_run))]
         self.encoded_lengths = np.ctypeslib.as_array(o.encoded_lengths, shape=(max(1, o.n_trav),))[
             : o.n_trav].copy()
 

@@ -213,6 +213,8 @@ int gfz_compress_traversals(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes
     for (size_t i = 0; i < r.rounds.size() && i < 32; ++i) {
       out->round_rules[i] = r.rounds[i].rules;
       out->round_lengths[i] = r.rounds[i].length_out;
+      out->round_distinct[i] = r.rounds[i].distinct_pairs;
+      out->round_slots[i] = r.rounds[i].table_slots;
     }
     out->encoded_lengths =
         static_cast<uint32_t *>(std::malloc(std::max<uint64_t>(n_trav, 1) * sizeof(uint32_t)));
@@ -294,6 +296,19 @@ int gfz_rulebook_rule_size(gfz_rulebook *rb, uint64_t rule_index, uint32_t *size
                            c.stream()));
     c.sync();
   });
+}
+
+int gfz_rulebook_info(gfz_rulebook *rb, uint32_t *max_depth, int *has_bodies,
+                      uint64_t *body_elems) {
+  if (!rb)
+    return fail(GFZ_E_INVALID, "null rulebook");
+  if (max_depth)
+    *max_depth = rb->rb.depth;
+  if (has_bodies)
+    *has_bodies = rb->rb.has_bodies ? 1 : 0;
+  if (body_elems)
+    *body_elems = rb->rb.body_elems;
+  return GFZ_OK;
 }
 
 static void stage_encoded(gfz::Ctx &c, const int32_t *encoded, uint64_t n, uint32_t flags,

@@ -51,10 +51,17 @@ struct RuleRec {
 
 struct Rulebook {
   DBuf<RuleRec> rec;
-  DBuf<uint32_t> size;  // expanded size of each rule
+  DBuf<uint32_t> size;      // expanded size of each rule
   uint64_t n_rules = 0;
   uint32_t min_id = 0;
-  uint32_t depth = 0;
+  uint32_t depth = 0;       // deepest rule
+  // Pre-expanded bodies: body of rule r at [body_off[r], body_off[r+1]). Built
+  // when the rule depths are non-decreasing in rule ID and the total fits the
+  // budget; otherwise expansion descends the rule tree per output position.
+  bool has_bodies = false;
+  DBuf<uint64_t> body_off;
+  DBuf<int32_t> bodies;
+  uint64_t body_elems = 0;
 };
 
 // first/second: device pointers (raw rule columns, rule-ID order).

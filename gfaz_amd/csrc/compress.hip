@@ -61,27 +61,27 @@ __device__ inline int64_t table_find(const unsigned long long *keys, uint64_t ma
   return -1;
 }
 
-__device__ inline void table_insert(const DevTable &t, uint64_t k, uint32_t cnt) {
-  if (__hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    return;
+// Insert k (seen `cnt` times in the caller's tile). Returns true when k was new
+// to the table; the caller aggregates those counts before touching the shared
+// distinct-key counter (one global atomic per block, not per key).
+__device__ inline bool table_insert(const DevTable &t, uint64_t k, uint32_t cnt) {
   uint64_t s = key_hash(k) & t.mask;
   for (uint64_t p = 0; p <= t.mask; ++p) {
     const unsigned long long old = atomicCAS(&t.keys[s], (unsigned long long)kEmptyKey,
                                              (unsigned long long)k);
     if (old == kEmptyKey) {
-      if (atomicAdd(t.distinct, 1u) + 1u > t.limit)
-        atomicExch(t.overflow, 1u);
       if (cnt >= 2)
         t.val[s] = 1u;
-      return;
+      return true;
     }
     if (old == k) {
       t.val[s] = 1u;
-      return;
+      return false;
     }
     s = (s + 1) & t.mask;
   }
   atomicExch(t.overflow, 1u);
+  return false;
 }
 
 // ---- delta (codec.cpp:45-76) ------------------------------------------------
@@ -107,14 +107,19 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
 }
 
 // ---- K1: repeated-pair set ----------------------------------------------------
-// Each block folds its tile's canonical pairs into an LDS table (hot keys such as
-// the backbone's (+1,+1) collapse to one entry with a count), then flushes every
-// distinct key once into the HBM table.
+// Three levels of aggregation before HBM: (1) within a wave, a run of lanes with
+// the same key (consecutive steps, e.g. the backbone's (+1,+1)) becomes one
+// insert carrying the run length; (2) each block folds its tile into an LDS
+// table with counts; (3) every distinct key of the tile is inserted once into
+// the HBM table, and the block adds its number of new keys to the global
+// counter with a single atomic.
 __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restrict__ x,
                                                         const uint32_t *__restrict__ heads,
                                                         uint64_t L, DevTable t) {
   __shared__ unsigned long long skeys[kLdsSlots];
   __shared__ uint32_t scnt[kLdsSlots];
+  __shared__ uint32_t snew[kBlock / kWave];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
     skeys[s] = kEmptyKey;
     scnt[s] = 0;
@@ -124,13 +129,23 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
 #pragma unroll 2
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    if (i + 1 < L && !head_at(heads, i + 1)) {
-      const unsigned long long k = (unsigned long long)canon2(pack2(x[i], x[i + 1]));
+    const bool valid = i + 1 < L && !head_at(heads, i + 1);
+    unsigned long long k = kEmptyKey;
+    if (valid)
+      k = (unsigned long long)canon2(pack2(x[i], x[i + 1]));
+    const unsigned long long prev = __shfl_up(k, 1);
+    const bool run_head = valid && (lane == 0 || prev != k);
+    const uint64_t vmask = __ballot(valid);
+    const uint64_t hmask = __ballot(run_head);
+    if (run_head) {
+      const uint64_t ends = hmask | ~vmask;
+      const uint64_t rest = lane == kWave - 1 ? 0ull : ends >> (lane + 1);
+      const uint32_t len = rest ? (uint32_t)__builtin_ctzll(rest) + 1u : (uint32_t)(kWave - lane);
       uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
       for (uint32_t p = 0; p < kLdsSlots; ++p) {
         const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kEmptyKey, k);
         if (old == kEmptyKey || old == k) {
-          atomicAdd(&scnt[s], 1u);
+          atomicAdd(&scnt[s], len);
           break;
         }
         s = (s + 1) & (kLdsSlots - 1);
@@ -138,10 +153,26 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
     }
   }
   __syncthreads();
-  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
-    const unsigned long long k = skeys[s];
-    if (k != kEmptyKey)
-      table_insert(t, k, scnt[s]);
+  uint32_t fresh = 0;
+  if (!__hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
+      const unsigned long long k = skeys[s];
+      if (k != kEmptyKey && table_insert(t, k, scnt[s]))
+        ++fresh;
+    }
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1)
+    fresh += __shfl_xor(fresh, d);
+  if (lane == 0)
+    snew[wid] = fresh;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    for (uint32_t w = 0; w < kBlock / kWave; ++w)
+      n += snew[w];
+    if (n && atomicAdd(t.distinct, n) + n > t.limit)
+      atomicExch(t.overflow, 1u);
   }
 }
 
@@ -331,24 +362,41 @@ __global__ __launch_bounds__(kBlock) void k_emit(
 }
 
 // ---- K4: keys of the used rules (order irrelevant: they are sorted next) -----
+// Each block owns kCollectTile consecutive slots: a block-wide scan of the used
+// flags gives every key its place, and one atomic per block claims the range.
+constexpr uint32_t kCollectItems = 16;
+constexpr uint32_t kCollectTile = kBlock * kCollectItems;
+
 __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long long *__restrict__ keys,
                                                          const uint8_t *__restrict__ used,
                                                          uint64_t cap, uint64_t *__restrict__ out,
                                                          unsigned long long *count) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  for (uint64_t s0 = (uint64_t)blockIdx.x * kBlock; s0 < cap; s0 += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t s = s0 + threadIdx.x;
-    const bool u = s < cap && used[s];
-    const uint64_t m = __ballot(u);
-    if (!m)
-      continue;
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-    unsigned long long b = 0;
-    if (lane == leader)
-      b = atomicAdd(count, (unsigned long long)__popcll(m));
-    b = __shfl(b, (int)leader);
-    if (u)
-      out[b + (uint64_t)__popcll(m & lanemask_lt())] = keys[s];
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  __shared__ unsigned long long sbase;
+  const uint64_t s0 = (uint64_t)blockIdx.x * kCollectTile + (uint64_t)threadIdx.x * kCollectItems;
+  uint32_t flags = 0;
+  if (s0 + kCollectItems <= cap) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(used + s0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (uint32_t q = 0; q < 16; ++q)
+      flags |= ((w[q >> 2] >> (8 * (q & 3))) & 0xFFu) ? (1u << q) : 0u;
+  } else {
+    for (uint32_t q = 0; q < kCollectItems; ++q)
+      if (s0 + q < cap && used[s0 + q])
+        flags |= 1u << q;
+  }
+  uint32_t total;
+  uint32_t off = block_exclusive_scan<kBlock>((uint32_t)__popc(flags), lds, total);
+  if (threadIdx.x == 0)
+    sbase = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
+  __syncthreads();
+  const uint64_t b = sbase + off;
+  uint32_t k = 0;
+  while (flags) {
+    const uint32_t q = (uint32_t)__ffs(flags) - 1;
+    out[b + k++] = keys[s0 + q];
+    flags &= flags - 1;
   }
 }
 
@@ -482,8 +530,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     const uint64_t n_words = ceil_div(L, 64);
 
     // K1 with capacity retry: the table holds at most cap/2 distinct keys.
+    // Distinct pairs grow slowly from round to round (each round's keys are
+    // pairs of the previous round's symbols); 4x the last count, at most 2L
+    // (there are fewer than L pairs), leaves the table at most half full.
     uint64_t cap = next_pow2(std::max<uint64_t>(
-        uint64_t(1) << 22, round == 0 ? L / 8 : std::min<uint64_t>(L, 4 * prev_distinct)));
+        uint64_t(1) << 23,
+        round == 0 ? L / 8 : std::min<uint64_t>(2 * L, 4 * prev_distinct)));
     DBuf<unsigned long long> tkeys;
     DBuf<uint32_t> tval;
     DBuf<uint32_t> counters(pool, 2);  // distinct, overflow
@@ -561,7 +613,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> ucount(pool, 1);
     fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
     ctx.timed("collect_used", [&] {
-      hipLaunchKernelGGL(k_collect_used, dim3(stride_grid(ctx, cap)), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(cap, kCollectTile)), dim3(kBlock), 0,
                          ctx.stream(), tkeys.get(), used.get(), cap, ukeys.get(), ucount.get());
     });
     GFZ_CHECK_LAUNCH();

@@ -5,14 +5,17 @@
 //   expand(+r) = expand(first[r]) ++ expand(second[r])
 //   expand(-r) = expand(-second[r]) ++ expand(-first[r])
 //   a symbol is a rule iff min_id <= |s| < min_id + R.
-// Instead of recursing per encoded symbol, every output position finds its
-// encoded symbol (binary search over the symbol offsets staged in LDS) and
-// descends the rule tree by expanded sizes, so one lane writes one output
-// element and a wave's stores are consecutive.
+// Instead of recursing per encoded symbol, the rulebook is pre-expanded once
+// (bodies, built level by level on the device) and every output position finds
+// its encoded symbol (a rank over a start bitmap staged in LDS) and reads one
+// body element, so one lane writes one output element and a wave's stores are
+// consecutive. A rulebook whose bodies do not fit the budget is expanded by
+// descending the rule tree by expanded sizes instead.
 #include "codec.hpp"
 #include "prims.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace gfz {
 
@@ -32,7 +35,8 @@ __device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
 __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict__ f,
                                                       const int32_t *__restrict__ s, uint64_t R,
                                                       uint32_t min_id, RuleRec *__restrict__ rec,
-                                                      uint32_t *__restrict__ size) {
+                                                      uint32_t *__restrict__ size,
+                                                      uint32_t *__restrict__ depth) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
     const int32_t a = f[r], b = s[r];
@@ -43,44 +47,66 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
     q.sb = is_rule(b, min_id, R) ? 0u : 1u;
     rec[r] = q;
     size[r] = (q.sa && q.sb) ? 2u : 0u;
+    depth[r] = (q.sa && q.sb) ? 1u : 0u;
   }
 }
 
-// One bottom-up pass: a rule whose children both have known sizes gets its
-// size; depth d rules resolve in pass d.
+// One bottom-up pass. In pass k a rule resolves when each child is a node ID or a
+// rule resolved before this pass (depth <= k + 1); it then gets depth k + 2 and
+// its size. Children resolved during the same pass are not used, so every depth
+// is exact whatever the thread order. `changed` is raised once per wave.
 __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
-                                                      uint32_t *__restrict__ size, uint64_t R,
-                                                      uint32_t min_id, uint32_t *changed,
-                                                      uint32_t *err) {
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
-       r += (uint64_t)gridDim.x * kBlock) {
-    if (size[r])
-      continue;
-    RuleRec q = rec[r];
-    const uint32_t sa = q.sa ? q.sa : __hip_atomic_load(&size[uabs(q.a) - min_id],
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t sb = q.sb ? q.sb : __hip_atomic_load(&size[uabs(q.b) - min_id],
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (sa && sb) {
-      const uint32_t sum = sa + sb;
-      if (sum < sa) {
-        atomicExch(err, 1u);
-        continue;
+                                                      uint32_t *__restrict__ size,
+                                                      uint32_t *__restrict__ depth, uint64_t R,
+                                                      uint32_t min_id, uint32_t pass,
+                                                      uint32_t *changed, uint32_t *err) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kBlock; r0 < R; r0 += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t r = r0 + threadIdx.x;
+    bool resolved = false;
+    if (r < R && !depth[r]) {
+      RuleRec q = rec[r];
+      bool ok = true;
+      uint32_t sa = q.sa, sb = q.sb;
+      if (!sa) {
+        const uint64_t c = uabs(q.a) - min_id;
+        const uint32_t dc = depth[c];
+        ok = ok && dc != 0 && dc <= pass + 1;
+        if (ok)
+          sa = size[c];
       }
-      q.sa = sa;
-      q.sb = sb;
-      rec[r] = q;
-      __hip_atomic_store(&size[r], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      atomicExch(changed, 1u);
+      if (!sb) {
+        const uint64_t c = uabs(q.b) - min_id;
+        const uint32_t dc = depth[c];
+        ok = ok && dc != 0 && dc <= pass + 1;
+        if (ok)
+          sb = size[c];
+      }
+      if (ok) {
+        const uint32_t sum = sa + sb;
+        if (sum < sa) {
+          atomicExch(err, 1u);
+        } else {
+          q.sa = sa;
+          q.sb = sb;
+          rec[r] = q;
+          size[r] = sum;
+          depth[r] = pass + 2;
+          resolved = true;
+        }
+      }
     }
+    const uint64_t m = __ballot(resolved);
+    if (m && lane == (uint32_t)__ffsll((unsigned long long)m) - 1)
+      *changed = 1u;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_rule_unresolved(const uint32_t *__restrict__ size,
+__global__ __launch_bounds__(kBlock) void k_rule_unresolved(const uint32_t *__restrict__ depth,
                                                             uint64_t R, uint32_t *err) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock)
-    if (!size[r])
+    if (!depth[r])
       atomicExch(err, 2u);
 }
 
@@ -101,76 +127,154 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
     dst[i] = src[idx[i]];
 }
 
-__device__ inline uint64_t upper_bound_u64(const uint64_t *a, uint64_t n, uint64_t v) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] <= v)
-      lo = mid + 1;
-    else
-      hi = mid;
+// tile_first[t] = index of the symbol covering output position base + t*kExTile
+// (symbols are disjoint, non-empty intervals, so each tile start has one writer).
+__global__ void k_tile_first(const uint64_t *__restrict__ sym_off, uint64_t n_sym, uint64_t base,
+                             uint64_t n_tiles, uint64_t *__restrict__ tile_first) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_sym;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = sym_off[k] - base, hi = sym_off[k + 1] - base;
+    for (uint64_t t = (lo + kExTile - 1) / kExTile; t < n_tiles && t * kExTile < hi; ++t)
+      tile_first[t] = k;
   }
-  return lo;
 }
 
-// Each block owns kExTile consecutive output positions.
-__global__ __launch_bounds__(kBlock) void k_expand(const int32_t *__restrict__ enc,
-                                                   const uint64_t *__restrict__ sym_off,
-                                                   uint64_t n_enc, uint64_t n_out,
-                                                   const RuleRec *__restrict__ rec, uint64_t R,
-                                                   uint32_t min_id, int32_t *__restrict__ out) {
-  __shared__ uint64_t soff[kExTile];
+// Output of one block: kExTile consecutive positions [base + o0, base + o_end).
+// The symbols overlapping the tile are staged in LDS with a bitmap of their start
+// positions; a position's symbol is the rank of the set bits at or before it.
+// With kBodies, a rule symbol reads its pre-expanded body (reversed and negated
+// for a reverse-oriented symbol); otherwise it descends the rule tree by sizes.
+template <bool kBodies>
+__global__ __launch_bounds__(kBlock) void k_expand(
+    const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
+    uint64_t base, uint64_t n_out, const uint64_t *__restrict__ tile_first,
+    const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
+    const uint64_t *__restrict__ body_off, const int32_t *__restrict__ bodies, uint64_t R,
+    uint32_t min_id, int32_t *__restrict__ out) {
+  __shared__ uint32_t srel[kExTile];
   __shared__ int32_t ssym[kExTile];
-  __shared__ uint64_t srange[2];
+  __shared__ unsigned long long sbits[kExTile / 64];
+  __shared__ uint32_t srank[kExTile / 64];
+  __shared__ uint64_t sfirst_off;
+  __shared__ uint32_t snsym;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
   const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
   const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
-  if (threadIdx.x == 0) {
-    srange[0] = upper_bound_u64(sym_off, n_enc + 1, o0) - 1;
-    srange[1] = upper_bound_u64(sym_off, n_enc + 1, o_end - 1) - 1;
+  if (tid < kExTile / 64)
+    sbits[tid] = 0;
+  if (tid == 0) {
+    const uint64_t k0 = tile_first[blockIdx.x];
+    uint64_t k_last = n_sym - 1;
+    if (o_end < n_out) {
+      const uint64_t tf = tile_first[blockIdx.x + 1];
+      k_last = (sym_off[tf] - base == o_end) ? tf - 1 : tf;
+    }
+    snsym = (uint32_t)(k_last - k0 + 1);
+    sfirst_off = sym_off[k0] - base;
   }
   __syncthreads();
-  const uint64_t k0 = srange[0];
-  const uint32_t nsym = (uint32_t)(srange[1] - k0 + 1);
-  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
-    soff[s] = sym_off[k0 + s];
-    ssym[s] = enc[k0 + s];
+  const uint64_t k0 = tile_first[blockIdx.x];
+  const uint32_t nsym = snsym;
+  for (uint32_t s = tid; s < nsym; s += kBlock) {
+    const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - base - o0);
+    srel[s] = rel;
+    ssym[s] = syms[k0 + s];
+    atomicOr(&sbits[rel >> 6], 1ull << (rel & 63));
   }
   __syncthreads();
+  if (tid < kWave) {
+    const uint32_t c = (uint32_t)__popcll(sbits[lane]);
+    uint32_t x = c;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t g = __shfl_up(x, d);
+      if (lane >= (uint32_t)d)
+        x += g;
+    }
+    srank[lane] = x - c;
+  }
+  __syncthreads();
+  const uint64_t first_off = sfirst_off;
+#pragma unroll 4
   for (uint32_t j = 0; j < kExItems; ++j) {
-    const uint64_t o = o0 + (uint64_t)j * kBlock + threadIdx.x;
+    const uint32_t p = j * kBlock + tid;
+    const uint64_t o = o0 + p;
     if (o >= o_end)
       break;
-    uint32_t lo = 0, hi = nsym - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (soff[mid] <= o)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    uint64_t q = o - soff[lo];
-    int32_t v = ssym[lo];
-    while (is_rule(v, min_id, R)) {
-      const RuleRec c = rec[uabs(v) - min_id];
-      int32_t left, right;
-      uint32_t ls;
-      if (v > 0) {
-        left = c.a;
-        right = c.b;
-        ls = c.sa;
+    const uint32_t w = p >> 6, b = p & 63;
+    const uint64_t upto = (2ull << b) - 1ull;
+    const uint32_t s = srank[w] + (uint32_t)__popcll(sbits[w] & upto) - 1u;
+    uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - srel[s]);
+    int32_t v = ssym[s];
+    if (is_rule(v, min_id, R)) {
+      if (kBodies) {
+        const uint32_t r = uabs(v) - min_id;
+        const uint64_t src = v > 0 ? body_off[r] + q : body_off[r] + (rsize[r] - 1 - q);
+        const int32_t x = bodies[src];
+        v = v > 0 ? x : -x;
       } else {
-        left = -c.b;
-        right = -c.a;
-        ls = c.sb;
-      }
-      if (q < ls) {
-        v = left;
-      } else {
-        q -= ls;
-        v = right;
+        do {
+          const RuleRec c = rec[uabs(v) - min_id];
+          int32_t left, right;
+          uint32_t ls;
+          if (v > 0) {
+            left = c.a;
+            right = c.b;
+            ls = c.sa;
+          } else {
+            left = -c.b;
+            right = -c.a;
+            ls = c.sb;
+          }
+          if (q < ls) {
+            v = left;
+          } else {
+            q -= ls;
+            v = right;
+          }
+        } while (is_rule(v, min_id, R));
       }
     }
-    out[o] = v;
+    out[base + o] = v;
+  }
+}
+
+// Pair stream of the rulebook: symbols (first[r], second[r]) at body offsets
+// (body_off[r], body_off[r] + |first[r]|), so that expanding it one level over
+// already-built bodies writes the bodies of the next depth.
+__global__ void k_pair_stream(const RuleRec *__restrict__ rec, const uint64_t *__restrict__ body_off,
+                              uint64_t R, int32_t *__restrict__ ps, uint64_t *__restrict__ ps_off) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const RuleRec c = rec[r];
+    ps[2 * r] = c.a;
+    ps[2 * r + 1] = c.b;
+    ps_off[2 * r] = body_off[r];
+    ps_off[2 * r + 1] = body_off[r] + c.sa;
+    if (r == R - 1)
+      ps_off[2 * R] = body_off[R];
+  }
+}
+
+// Level boundaries when depth is non-decreasing in rule ID (true for every
+// rulebook the reference CPU compressor writes: a rule's depth is the round that
+// created it, and rounds hand out increasing IDs). bound[d] = first rule of
+// depth d; *bad = 1 if depth ever decreases.
+__global__ void k_depth_bounds(const uint32_t *__restrict__ depth, uint64_t R, uint32_t max_depth,
+                               uint64_t *__restrict__ bound, uint32_t *bad) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = depth[r];
+    if (r == 0) {
+      for (uint32_t e = 1; e <= d && e <= max_depth; ++e)
+        bound[e] = 0;
+    } else {
+      const uint32_t p = depth[r - 1];
+      if (d < p)
+        atomicExch(bad, 1u);
+      for (uint32_t e = p + 1; e <= d && e <= max_depth; ++e)
+        bound[e] = r;
+    }
   }
 }
 
@@ -306,16 +410,18 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   rb.size = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
   if (!R)
     return;
+  DBuf<uint32_t> depth(ctx.pool(), R);
   DBuf<uint32_t> flags(ctx.pool(), 2);  // changed, err
   hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get());
+                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get(), depth.get());
   GFZ_CHECK_LAUNCH();
-  uint32_t depth = 1;
+  uint32_t max_depth = 1;
   for (uint64_t pass = 0; pass <= R; ++pass) {
     fill_u32(ctx, flags.get(), 0, 2);
     ctx.timed("rule_sizes", [&] {
       hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         rb.rec.get(), rb.size.get(), R, min_id, flags.get(), flags.get() + 1);
+                         rb.rec.get(), rb.size.get(), depth.get(), R, min_id, (uint32_t)pass,
+                         flags.get(), flags.get() + 1);
     });
     GFZ_CHECK_LAUNCH();
     uint32_t h[2];
@@ -325,17 +431,84 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
       throw Error(kErrCorrupt, "rule expansion size exceeds 2^32 symbols");
     if (!h[0])
       break;
-    ++depth;
+    max_depth = (uint32_t)pass + 2;
   }
   hipLaunchKernelGGL(k_rule_unresolved, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     rb.size.get(), R, flags.get() + 1);
+                     depth.get(), R, flags.get() + 1);
   GFZ_CHECK_LAUNCH();
   uint32_t e = 0;
   GFZ_HIP(hipMemcpyAsync(&e, flags.get() + 1, 4, hipMemcpyDeviceToHost, ctx.stream()));
   ctx.sync();
   if (e)
     throw Error(kErrCorrupt, "rulebook is cyclic: some rule never bottoms out in node IDs");
-  rb.depth = depth;
+  rb.depth = max_depth;
+
+  // Level ranges (contiguous when depth is non-decreasing in rule ID).
+  std::vector<uint64_t> bound(max_depth + 2, R);
+  {
+    DBuf<uint64_t> d_bound(ctx.pool(), max_depth + 2);
+    fill_u64(ctx, d_bound.get(), R, max_depth + 2);
+    fill_u32(ctx, flags.get(), 0, 1);
+    hipLaunchKernelGGL(k_depth_bounds, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                       depth.get(), R, max_depth, d_bound.get(), flags.get());
+    GFZ_CHECK_LAUNCH();
+    GFZ_HIP(hipMemcpyAsync(bound.data(), d_bound.get(), bound.size() * 8, hipMemcpyDeviceToHost,
+                           ctx.stream()));
+    GFZ_HIP(hipMemcpyAsync(&e, flags.get(), 4, hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
+  }
+  if (e)
+    return;  // depth not monotone in rule ID: expansion descends the tree instead
+  depth.reset();
+
+  // Bodies, one depth level at a time (children are always shallower).
+  rb.body_off = DBuf<uint64_t>(ctx.pool(), R + 1);
+  exclusive_scan(ctx, rb.size.get(), rb.body_off.get(), R);
+  std::vector<uint64_t> lvl_off(max_depth + 2);
+  {
+    DBuf<uint64_t> d_lvl(ctx.pool(), max_depth + 2), d_idx(ctx.pool(), max_depth + 2);
+    GFZ_HIP(hipMemcpyAsync(d_idx.get(), bound.data(), bound.size() * 8, hipMemcpyHostToDevice,
+                           ctx.stream()));
+    hipLaunchKernelGGL(k_gather_u64, dim3(1), dim3(kBlock), 0, ctx.stream(), rb.body_off.get(),
+                       d_idx.get(), max_depth + 2, d_lvl.get());
+    GFZ_CHECK_LAUNCH();
+    GFZ_HIP(hipMemcpyAsync(lvl_off.data(), d_lvl.get(), lvl_off.size() * 8, hipMemcpyDeviceToHost,
+                           ctx.stream()));
+    ctx.sync();
+  }
+  const uint64_t total = lvl_off[max_depth + 1];
+  uint64_t budget = uint64_t(1) << 30;  // elements (4 GiB)
+  if (const char *env = std::getenv("GFAZ_BODY_BUDGET_MB"))
+    budget = std::strtoull(env, nullptr, 10) * (uint64_t(1) << 20) / 4;
+  if (total > budget)
+    return;
+  rb.bodies = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(total, 1));
+  rb.body_elems = total;
+  DBuf<int32_t> ps(ctx.pool(), 2 * R);
+  DBuf<uint64_t> ps_off(ctx.pool(), 2 * R + 1);
+  hipLaunchKernelGGL(k_pair_stream, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                     rb.rec.get(), rb.body_off.get(), R, ps.get(), ps_off.get());
+  GFZ_CHECK_LAUNCH();
+  for (uint32_t d = 1; d <= max_depth; ++d) {
+    const uint64_t r0 = bound[d], r1 = bound[d + 1];
+    if (r0 >= r1)
+      continue;
+    const uint64_t lo = lvl_off[d], hi = lvl_off[d + 1];
+    const uint64_t n_tiles = ceil_div(hi - lo, kExTile);
+    DBuf<uint64_t> tf(ctx.pool(), n_tiles);
+    ctx.timed("build_bodies", [&] {
+      hipLaunchKernelGGL(k_tile_first, dim3(stride_grid(ctx, 2 * (r1 - r0))), dim3(kBlock), 0,
+                         ctx.stream(), ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, n_tiles, tf.get());
+      GFZ_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         ps.get() + 2 * r0, ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, hi - lo,
+                         tf.get(), rb.rec.get(), rb.size.get(), rb.body_off.get(), rb.bodies.get(),
+                         R, min_id, rb.bodies.get());
+      GFZ_CHECK_LAUNCH();
+    });
+  }
+  ctx.sync();
+  rb.has_bodies = true;
 }
 
 void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
@@ -395,10 +568,21 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const uint64_t n_out = plan.n_out;
   if (!n_out)
     return;
-  const uint32_t nb = (uint32_t)ceil_div(n_out, kExTile);
+  const uint64_t n_tiles = ceil_div(n_out, kExTile);
+  DBuf<uint64_t> tf(ctx.pool(), n_tiles);
+  hipLaunchKernelGGL(k_tile_first, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                     d_sym_off, n_enc, 0, n_tiles, tf.get());
+  GFZ_CHECK_LAUNCH();
   ctx.timed("expand", [&] {
-    hipLaunchKernelGGL(k_expand, dim3(nb), dim3(kBlock), 0, ctx.stream(), d_enc, d_sym_off, n_enc,
-                       n_out, rb.rec.get(), rb.n_rules, rb.min_id, d_out);
+    if (rb.has_bodies)
+      hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, d_sym_off, n_enc, 0, n_out, tf.get(), rb.rec.get(), rb.size.get(),
+                         rb.body_off.get(), rb.bodies.get(), rb.n_rules, rb.min_id, d_out);
+    else
+      hipLaunchKernelGGL(k_expand<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, d_sym_off, n_enc, 0, n_out, tf.get(), rb.rec.get(), rb.size.get(),
+                         (const uint64_t *)nullptr, (const int32_t *)nullptr, rb.n_rules,
+                         rb.min_id, d_out);
   });
   GFZ_CHECK_LAUNCH();
   ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);

@@ -94,6 +94,8 @@ typedef struct gfz_compress_out {
   uint32_t flags;             /* copy of the call's flags */
   uint64_t round_rules[32];   /* rules created per round */
   uint64_t round_lengths[32]; /* stream length after each round */
+  uint64_t round_distinct[32];/* distinct canonical pairs seen per round */
+  uint64_t round_slots[32];   /* pair-table slots used per round (after retries) */
   void *internal;             /* owned device buffers */
 } gfz_compress_out;
 
@@ -115,6 +117,10 @@ int gfz_rulebook_load(gfz_ctx *ctx, const int32_t *rules_first, const int32_t *r
 void gfz_rulebook_free(gfz_rulebook *rb);
 /* Expanded length of one rule (for tests and planning). */
 int gfz_rulebook_rule_size(gfz_rulebook *rb, uint64_t rule_index, uint32_t *size);
+/* Deepest rule, and whether the rule bodies were pre-expanded in HBM (and how
+ * many int32 they take); without bodies, expansion descends the rule tree. */
+int gfz_rulebook_info(gfz_rulebook *rb, uint32_t *max_depth, int *has_bodies,
+                      uint64_t *body_elems);
 
 /* Total decoded length of a batch of encoded traversals. */
 int gfz_decoded_size(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,

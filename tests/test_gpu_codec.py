@@ -155,3 +155,36 @@ def test_corrupt_rulebooks_fail_cleanly(gpu_ctx):
     with pytest.raises(RuntimeError, match="truncated"):
         codec.decompress_traversals(gpu_ctx, rb, np.array([10], np.int32),
                                     np.array([5], np.uint32))
+
+
+def test_bodies_and_descent_paths_agree(gpu_ctx, monkeypatch):
+    """Expansion through pre-expanded bodies (default) and by descending the rule
+    tree (bodies disabled by a zero budget) both reproduce the input."""
+    g = goldens.load("plain_20k_r8_t2_d1")
+    nodes, lengths, _ = goldens.inputs(g)
+    exp = goldens.expected(g)
+    rb = codec.Rulebook(gpu_ctx, exp["rules_first"], exp["rules_second"], exp["start_id"])
+    info = rb.info()
+    say(f"bodies info {info}")
+    assert info["has_bodies"] and info["max_depth"] == 7
+    rb.close()
+    monkeypatch.setenv("GFAZ_BODY_BUDGET_MB", "0")
+    rb = codec.Rulebook(gpu_ctx, exp["rules_first"], exp["rules_second"], exp["start_id"])
+    assert not rb.info()["has_bodies"]
+    dec, _, mism = codec.decompress_traversals(gpu_ctx, rb, exp["encoded"], exp["encoded_lengths"],
+                                               lengths, 1)
+    rb.close()
+    assert not mism
+    np.testing.assert_array_equal(dec, nodes)
+
+
+def test_non_topological_rulebook_uses_descent(gpu_ctx):
+    """A child with a higher ID than its parent (legal for the recursive CPU
+    decoder) makes depth non-monotone in rule ID: expansion still succeeds."""
+    say("non-topological rulebook")
+    rb = codec.Rulebook(gpu_ctx, [11, 1], [1, 2], 10)  # 10 = (11, 1), 11 = (1, 2)
+    assert not rb.info()["has_bodies"]
+    dec, lens, _ = codec.decompress_traversals(gpu_ctx, rb, np.array([10, -10, 3], np.int32),
+                                               np.array([3], np.uint32), None, 0)
+    assert dec.tolist() == [1, 2, 1, -1, -2, -1, 3]
+    rb.close()
