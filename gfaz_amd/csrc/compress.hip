@@ -64,8 +64,22 @@ __device__ inline int64_t table_find(const unsigned long long *keys, uint64_t ma
 // Insert k (seen `cnt` times in the caller's tile). Returns true when k was new
 // to the table; the caller aggregates those counts before touching the shared
 // distinct-key counter (one global atomic per block, not per key).
+// A plain probe runs first: keys never change once set, so finding k that way
+// is final, and the hot keys every tile shares (the backbone's (+1,+1) and its
+// bubble neighbours) are resolved from cache without a device-scope atomic.
 __device__ inline bool table_insert(const DevTable &t, uint64_t k, uint32_t cnt) {
   uint64_t s = key_hash(k) & t.mask;
+  for (uint64_t p = 0; p <= t.mask; ++p) {
+    const uint64_t c = t.keys[s];
+    if (c == k) {
+      if (!t.val[s])
+        t.val[s] = 1u;
+      return false;
+    }
+    if (c == kEmptyKey)
+      break;
+    s = (s + 1) & t.mask;
+  }
   for (uint64_t p = 0; p <= t.mask; ++p) {
     const unsigned long long old = atomicCAS(&t.keys[s], (unsigned long long)kEmptyKey,
                                              (unsigned long long)k);
@@ -82,6 +96,43 @@ __device__ inline bool table_insert(const DevTable &t, uint64_t k, uint32_t cnt)
   }
   atomicExch(t.overflow, 1u);
   return false;
+}
+
+// Probe up to N keys at once: the loads of all lanes' keys are issued together
+// so the dependent chain per element is one or two rounds of independent loads
+// rather than N serial probes. slot[j] = -1 when absent or invalid.
+template <int N>
+__device__ inline void table_find_batch(const unsigned long long *__restrict__ keys,
+                                        uint64_t mask, const uint64_t (&k)[N],
+                                        const bool (&valid)[N], int64_t (&slot)[N]) {
+  uint64_t s[N];
+  uint32_t pending = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    s[j] = key_hash(k[j]) & mask;
+    slot[j] = -1;
+    if (valid[j])
+      pending |= 1u << j;
+  }
+  for (uint64_t p = 0; pending && p <= mask; ++p) {
+    uint64_t c[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      c[j] = (pending >> j) & 1 ? keys[s[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (!((pending >> j) & 1))
+        continue;
+      if (c[j] == k[j]) {
+        slot[j] = (int64_t)s[j];
+        pending &= ~(1u << j);
+      } else if (c[j] == kEmptyKey) {
+        pending &= ~(1u << j);
+      } else {
+        s[j] = (s[j] + 1) & mask;
+      }
+    }
+  }
 }
 
 // ---- delta (codec.cpp:45-76) ------------------------------------------------
@@ -126,16 +177,20 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   }
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
-#pragma unroll 2
+  unsigned long long kv[kInsItems];
+  bool valid[kInsItems];
+#pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    const bool valid = i + 1 < L && !head_at(heads, i + 1);
-    unsigned long long k = kEmptyKey;
-    if (valid)
-      k = (unsigned long long)canon2(pack2(x[i], x[i + 1]));
+    valid[j] = i + 1 < L && !head_at(heads, i + 1);
+    kv[j] = valid[j] ? (unsigned long long)canon2(pack2(x[i], x[i + 1]))
+                     : (unsigned long long)kEmptyKey;
+  }
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const unsigned long long k = kv[j];
     const unsigned long long prev = __shfl_up(k, 1);
-    const bool run_head = valid && (lane == 0 || prev != k);
-    const uint64_t vmask = __ballot(valid);
+    const bool run_head = valid[j] && (lane == 0 || prev != k);
+    const uint64_t vmask = __ballot(valid[j]);
     const uint64_t hmask = __ballot(run_head);
     if (run_head) {
       const uint64_t ends = hmask | ~vmask;
@@ -189,6 +244,8 @@ __device__ inline uint32_t wave_fn_inclusive(uint32_t f) {
 }
 
 // ---- K2: candidate bits and each tile's transfer function --------------------
+constexpr int kProbeBatch = 8;
+
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
                                                  const unsigned long long *__restrict__ keys,
@@ -198,21 +255,31 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
   __shared__ uint64_t wbits[kRtWords];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
-  for (uint32_t j = 0; j < kRtItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    bool c = false;
-    if (i + 1 < L && !head_at(heads, i + 1)) {
-      const uint64_t k = (uint64_t)canon2(pack2(x[i], x[i + 1]));
-      const int64_t s = table_find(keys, mask, k);
-      c = s >= 0 && val[s] != 0;
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kProbeBatch) {
+    uint64_t k[kProbeBatch];
+    bool valid[kProbeBatch];
+#pragma unroll
+    for (int b = 0; b < kProbeBatch; ++b) {
+      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
+      valid[b] = i + 1 < L && !head_at(heads, i + 1);
+      k[b] = valid[b] ? (uint64_t)canon2(pack2(x[i], x[i + 1])) : 0;
     }
-    const uint64_t m = __ballot(c);
-    if (lane == 0) {
-      const uint32_t w = j * (kBlock / kWave) + wid;
-      wbits[w] = m;
-      const uint64_t gw = (base >> 6) + w;
-      if (gw < n_words)
-        cand[gw] = m;
+    int64_t slot[kProbeBatch];
+    table_find_batch<kProbeBatch>(keys, mask, k, valid, slot);
+    uint32_t v[kProbeBatch];
+#pragma unroll
+    for (int b = 0; b < kProbeBatch; ++b)
+      v[b] = slot[b] >= 0 ? val[slot[b]] : 0u;
+#pragma unroll
+    for (int b = 0; b < kProbeBatch; ++b) {
+      const uint64_t m = __ballot(v[b] != 0);
+      if (lane == 0) {
+        const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+        wbits[w] = m;
+        const uint64_t gw = (base >> 6) + w;
+        if (gw < n_words)
+          cand[gw] = m;
+      }
     }
   }
   __syncthreads();
@@ -334,29 +401,56 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     soff[lane] = sx - sc;
   }
   __syncthreads();
-  for (uint32_t j = 0; j < kRtItems; ++j) {
-    const uint32_t w = j * (kBlock / kWave) + wid;
-    const uint64_t i = base + (uint64_t)w * 64 + lane;
-    const uint64_t sv = ssurv[w];
-    if (i < L && ((sv >> lane) & 1)) {
-      const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(sv & lanemask_lt());
-      int32_t v = x[i];
-      if ((spicks[w] >> lane) & 1) {
-        const int64_t k = pack2(v, x[i + 1]);
-        const int64_t c = canon2(k);
-        const int64_t s = table_find(keys, mask, (uint64_t)c);
-        if (s < 0) {
-          atomicExch(err, 1u);
-        } else {
-          used[s] = 1;
-          v = (int32_t)(next + (uint32_t)s);
-          if (k != c)
-            v = -v;
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kProbeBatch) {
+    int32_t v[kProbeBatch];
+    uint64_t kk[kProbeBatch];
+    bool pk[kProbeBatch], neg[kProbeBatch], sv[kProbeBatch], hd[kProbeBatch];
+    uint64_t pos[kProbeBatch];
+#pragma unroll
+    for (int b = 0; b < kProbeBatch; ++b) {
+      const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+      const uint64_t i = base + (uint64_t)w * 64 + lane;
+      const uint64_t svm = ssurv[w];
+      sv[b] = i < L && ((svm >> lane) & 1);
+      pk[b] = false;
+      neg[b] = false;
+      kk[b] = 0;
+      v[b] = 0;
+      pos[b] = 0;
+      hd[b] = false;
+      if (sv[b]) {
+        pos[b] = off_tile + soff[w] + (uint64_t)__popcll(svm & lanemask_lt());
+        v[b] = x[i];
+        hd[b] = head_at(heads, i);
+        if ((spicks[w] >> lane) & 1) {
+          const int64_t k = pack2(v[b], x[i + 1]);
+          const int64_t c = canon2(k);
+          kk[b] = (uint64_t)c;
+          neg[b] = k != c;
+          pk[b] = true;
         }
       }
-      y[pos] = v;
-      if (head_at(heads, i))
-        atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
+    }
+    int64_t slot[kProbeBatch];
+    table_find_batch<kProbeBatch>(keys, mask, kk, pk, slot);
+#pragma unroll
+    for (int b = 0; b < kProbeBatch; ++b) {
+      if (!sv[b])
+        continue;
+      int32_t o = v[b];
+      if (pk[b]) {
+        if (slot[b] < 0) {
+          atomicExch(err, 1u);
+        } else {
+          used[slot[b]] = 1;
+          o = (int32_t)(next + (uint32_t)slot[b]);
+          if (neg[b])
+            o = -o;
+        }
+      }
+      y[pos[b]] = o;
+      if (hd[b])
+        atomicOr(&heads_out[pos[b] >> 5], 1u << (pos[b] & 31));
     }
   }
 }
@@ -367,9 +461,31 @@ __global__ __launch_bounds__(kBlock) void k_emit(
 constexpr uint32_t kCollectItems = 16;
 constexpr uint32_t kCollectTile = kBlock * kCollectItems;
 
+// Keys are written compacted when span > 0: every symbol of the round has
+// |v| < next = span / 2, so (a + next) * span + (b >= 0 ? b : b + span) keeps the
+// signed packed-key order (a signed, then b as unsigned 32 bits) in fewer bits
+// and the radix sort needs fewer passes.
+__device__ inline uint64_t compact_key(uint64_t k, uint64_t span) {
+  if (!span)
+    return k;
+  const int64_t a = key_first((int64_t)k), b = key_second((int64_t)k);
+  const int64_t half = (int64_t)(span / 2);
+  return (uint64_t)(a + half) * span + (uint64_t)(b >= 0 ? b : b + (int64_t)span);
+}
+__device__ inline uint64_t expand_key(uint64_t c, uint64_t span) {
+  if (!span)
+    return c;
+  const int64_t half = (int64_t)(span / 2);
+  const int64_t a = (int64_t)(c / span) - half;
+  const int64_t bq = (int64_t)(c % span);
+  const int64_t b = bq < half ? bq : bq - (int64_t)span;
+  return (uint64_t)pack2((int32_t)a, (int32_t)b);
+}
+
 __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long long *__restrict__ keys,
                                                          const uint8_t *__restrict__ used,
-                                                         uint64_t cap, uint64_t *__restrict__ out,
+                                                         uint64_t cap, uint64_t span,
+                                                         uint64_t *__restrict__ out,
                                                          unsigned long long *count) {
   __shared__ uint32_t lds[kBlock / kWave + 1];
   __shared__ unsigned long long sbase;
@@ -395,13 +511,14 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
   uint32_t k = 0;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    out[b + k++] = keys[s0 + q];
+    out[b + k++] = compact_key(keys[s0 + q], span);
     flags &= flags - 1;
   }
 }
 
 // ---- K6: final IDs next + rank, rule columns ----------------------------------
 __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
+                                                   uint64_t span,
                                                    const unsigned long long *__restrict__ keys,
                                                    uint64_t mask, uint32_t *__restrict__ val,
                                                    uint32_t next, int32_t *__restrict__ rf,
@@ -409,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ 
                                                    uint32_t *__restrict__ err) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t k = sorted[r];
+    const uint64_t k = expand_key(sorted[r], span);
     const int64_t s = table_find(keys, mask, k);
     if (s < 0) {
       atomicExch(err, 2u);
@@ -608,13 +725,15 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     tile_fn.reset();
     tile_info.reset();
 
-    // K4: used keys.
+    // K4: used keys (compacted for the sort when the span allows it).
+    const uint64_t span = (uint64_t)next < (uint64_t(1) << 30) ? 2 * (uint64_t)next : 0;
     DBuf<uint64_t> ukeys(pool, distinct ? distinct : 1), utmp(pool, distinct ? distinct : 1);
     DBuf<unsigned long long> ucount(pool, 1);
     fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
     ctx.timed("collect_used", [&] {
       hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(cap, kCollectTile)), dim3(kBlock), 0,
-                         ctx.stream(), tkeys.get(), used.get(), cap, ukeys.get(), ucount.get());
+                         ctx.stream(), tkeys.get(), used.get(), cap, span, ukeys.get(),
+                         ucount.get());
     });
     GFZ_CHECK_LAUNCH();
     const uint64_t R = read_scalar(ctx, ucount.get());
@@ -641,7 +760,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     }
     ctx.timed("assign_ids", [&] {
       hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         ukeys.get(), R, tkeys.get(), cap - 1, tval.get(), next,
+                         ukeys.get(), R, span, tkeys.get(), cap - 1, tval.get(), next,
                          rf.get() + out.n_rules, rs.get() + out.n_rules, d_err.get());
     });
     GFZ_CHECK_LAUNCH();

@@ -329,6 +329,201 @@ __device__ inline bool bit_at(const uint32_t *bits, uint64_t i) {
   return (bits[i >> 5] >> (i & 31)) & 1u;
 }
 
+// ---- expansion fused with the first inverse delta ----------------------------
+// One pass over the output: each block expands a tile (positions staged as in
+// k_expand), runs the segmented prefix sum over the tile in LDS, and chains the
+// carry across tiles with a decoupled look-back (tiles taken in order from an
+// atomic ticket, so every tile a block waits on belongs to a block already
+// running). A status word per tile packs (state, head flag, 32-bit sum).
+constexpr uint64_t kStNotReady = 0, kStAggregate = 1ull << 62, kStInclusive = 2ull << 62;
+constexpr uint32_t kPad = kExTile + kExTile / 16;  // LDS transpose with a 17-word stride
+
+__device__ inline uint32_t pad_idx(uint32_t p) { return p + (p >> 4); }
+
+__device__ inline uint64_t st_pack(uint64_t state, Seg v) {
+  return state | ((uint64_t)(v.flag & 1) << 32) | v.sum;
+}
+
+template <bool kBodies>
+__global__ __launch_bounds__(kBlock) void k_expand_delta(
+    const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
+    uint64_t n_out, const uint64_t *__restrict__ tile_first, const RuleRec *__restrict__ rec,
+    const uint32_t *__restrict__ rsize, const uint64_t *__restrict__ body_off,
+    const int32_t *__restrict__ bodies, uint64_t R, uint32_t min_id,
+    const uint32_t *__restrict__ heads, unsigned long long *status, uint32_t *ticket,
+    uint32_t *err, int32_t *__restrict__ out) {
+  // srel/ssym stage the tile's symbols; once every lane holds its expanded
+  // values in registers the same LDS is reused as the transpose buffer sval.
+  __shared__ uint32_t sbuf[2 * kExTile];
+  static_assert(kPad <= 2 * kExTile, "transpose buffer fits the staging area");
+  uint32_t *const srel = sbuf;
+  int32_t *const ssym = reinterpret_cast<int32_t *>(sbuf + kExTile);
+  uint32_t *const sval = sbuf;
+  __shared__ unsigned long long sbits[kExTile / 64];
+  __shared__ uint32_t srank[kExTile / 64];
+  __shared__ Seg slds[kBlock / kWave + 1];
+  __shared__ uint64_t sfirst_off;
+  __shared__ uint32_t snsym, stile;
+  __shared__ Seg scarry;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
+  if (tid == 0)
+    stile = atomicAdd(ticket, 1u);
+  if (tid < kExTile / 64)
+    sbits[tid] = 0;
+  __syncthreads();
+  const uint32_t tile = stile;
+  const uint64_t o0 = (uint64_t)tile * kExTile;
+  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+  if (tid == 0) {
+    const uint64_t k0 = tile_first[tile];
+    uint64_t k_last = n_sym - 1;
+    if (o_end < n_out) {
+      const uint64_t tf = tile_first[tile + 1];
+      k_last = (sym_off[tf] == o_end) ? tf - 1 : tf;
+    }
+    snsym = (uint32_t)(k_last - k0 + 1);
+    sfirst_off = sym_off[k0];
+  }
+  __syncthreads();
+  const uint64_t k0 = tile_first[tile];
+  const uint32_t nsym = snsym;
+  for (uint32_t s = tid; s < nsym; s += kBlock) {
+    const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - o0);
+    srel[s] = rel;
+    ssym[s] = syms[k0 + s];
+    atomicOr(&sbits[rel >> 6], 1ull << (rel & 63));
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    const uint32_t c = (uint32_t)__popcll(sbits[lane]);
+    uint32_t x = c;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t g = __shfl_up(x, d);
+      if (lane >= (uint32_t)d)
+        x += g;
+    }
+    srank[lane] = x - c;
+  }
+  __syncthreads();
+  const uint64_t first_off = sfirst_off;
+  // Expand in the striped layout (consecutive lanes, consecutive positions).
+  int32_t ev[kExItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kExItems; ++j) {
+    const uint32_t p = j * kBlock + tid;
+    const uint64_t o = o0 + p;
+    int32_t v = 0;
+    if (o < o_end) {
+      const uint32_t w = p >> 6, b = p & 63;
+      const uint64_t upto = (2ull << b) - 1ull;
+      const uint32_t s = srank[w] + (uint32_t)__popcll(sbits[w] & upto) - 1u;
+      uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - srel[s]);
+      v = ssym[s];
+      if (is_rule(v, min_id, R)) {
+        if (kBodies) {
+          const uint32_t r = uabs(v) - min_id;
+          const uint64_t src = v > 0 ? body_off[r] + q : body_off[r] + (rsize[r] - 1 - q);
+          const int32_t x = bodies[src];
+          v = v > 0 ? x : -x;
+        } else {
+          do {
+            const RuleRec c = rec[uabs(v) - min_id];
+            int32_t left, right;
+            uint32_t ls;
+            if (v > 0) {
+              left = c.a;
+              right = c.b;
+              ls = c.sa;
+            } else {
+              left = -c.b;
+              right = -c.a;
+              ls = c.sb;
+            }
+            if (q < ls) {
+              v = left;
+            } else {
+              q -= ls;
+              v = right;
+            }
+          } while (is_rule(v, min_id, R));
+        }
+      }
+    }
+    ev[j] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kExItems; ++j)
+    sval[pad_idx(j * kBlock + tid)] = (uint32_t)ev[j];
+  __syncthreads();
+  // Blocked layout for the scan: thread t owns positions [16t, 16t + 16).
+  const uint32_t pb = tid * kExItems;
+  uint32_t hbits = 0;
+  if (o0 + pb < o_end) {
+    const uint64_t g = o0 + pb;  // 16-aligned, so the 16 bits sit in one word
+    hbits = (heads[g >> 5] >> (g & 31)) & 0xFFFFu;
+  }
+  uint32_t vals[kExItems];
+  Seg a{0, 0};
+#pragma unroll
+  for (uint32_t i = 0; i < kExItems; ++i) {
+    vals[i] = sval[pad_idx(pb + i)];
+    a = seg_then(a, Seg{vals[i], (hbits >> i) & 1u});
+  }
+  Seg total;
+  const Seg pre = block_seg_exclusive<kBlock>(a, slds, total);
+  if (tid == 0) {
+    Seg carry{0, 0};
+    if (tile == 0) {
+      __hip_atomic_store(&status[0], st_pack(kStInclusive, total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&status[tile],
+                         st_pack(total.flag ? kStInclusive : kStAggregate, total),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t pred = (int64_t)tile - 1;
+      uint32_t spins = 0;
+      while (pred >= 0) {
+        const uint64_t st = __hip_atomic_load(&status[pred], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t state = st & (3ull << 62);
+        if (state == kStNotReady) {
+          if (++spins > (1u << 26)) {
+            atomicExch(err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const Seg pv{(uint32_t)st, (uint32_t)(st >> 32) & 1u};
+        carry = seg_then(pv, carry);
+        if (state == kStInclusive || pv.flag)
+          break;
+        --pred;
+      }
+      if (!total.flag)
+        __hip_atomic_store(&status[tile], st_pack(kStInclusive, seg_then(carry, total)),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    scarry = carry;
+  }
+  __syncthreads();
+  Seg run = seg_then(scarry, pre);
+#pragma unroll
+  for (uint32_t i = 0; i < kExItems; ++i) {
+    run = seg_then(run, Seg{vals[i], (hbits >> i) & 1u});
+    sval[pad_idx(pb + i)] = run.sum;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (uint32_t j = 0; j < kExItems; ++j) {
+    const uint32_t p = j * kBlock + tid;
+    if (o0 + p < o_end)
+      out[o0 + p] = (int32_t)sval[pad_idx(p)];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_seg_reduce(const int32_t *__restrict__ x, uint64_t n,
                                                        const uint32_t *__restrict__ heads,
                                                        Seg *__restrict__ agg) {
@@ -569,25 +764,29 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   if (!n_out)
     return;
   const uint64_t n_tiles = ceil_div(n_out, kExTile);
+  if (n_tiles > UINT32_MAX)
+    throw Error(kErrInvalid, "decoded batch too large for one launch");
   DBuf<uint64_t> tf(ctx.pool(), n_tiles);
   hipLaunchKernelGGL(k_tile_first, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
                      d_sym_off, n_enc, 0, n_tiles, tf.get());
   GFZ_CHECK_LAUNCH();
-  ctx.timed("expand", [&] {
-    if (rb.has_bodies)
-      hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, d_sym_off, n_enc, 0, n_out, tf.get(), rb.rec.get(), rb.size.get(),
-                         rb.body_off.get(), rb.bodies.get(), rb.n_rules, rb.min_id, d_out);
-    else
-      hipLaunchKernelGGL(k_expand<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, d_sym_off, n_enc, 0, n_out, tf.get(), rb.rec.get(), rb.size.get(),
-                         (const uint64_t *)nullptr, (const int32_t *)nullptr, rb.n_rules,
-                         rb.min_id, d_out);
-  });
-  GFZ_CHECK_LAUNCH();
-  ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
-  if (delta_round <= 0)
+  const uint64_t *bo = rb.has_bodies ? rb.body_off.get() : nullptr;
+  const int32_t *bd = rb.has_bodies ? rb.bodies.get() : nullptr;
+  if (delta_round <= 0) {
+    ctx.timed("expand", [&] {
+      if (rb.has_bodies)
+        hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                           ctx.stream(), d_enc, d_sym_off, n_enc, 0, n_out, tf.get(),
+                           rb.rec.get(), rb.size.get(), bo, bd, rb.n_rules, rb.min_id, d_out);
+      else
+        hipLaunchKernelGGL(k_expand<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                           ctx.stream(), d_enc, d_sym_off, n_enc, 0, n_out, tf.get(),
+                           rb.rec.get(), rb.size.get(), bo, bd, rb.n_rules, rb.min_id, d_out);
+    });
+    GFZ_CHECK_LAUNCH();
+    ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
     return;
+  }
   // Heads at the decoded start of every non-empty traversal.
   std::vector<uint64_t> starts;
   for (uint64_t t = 0; t < n_trav; ++t)
@@ -596,14 +795,39 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const uint64_t words = ceil_div(n_out, 32);
   DBuf<uint32_t> heads(ctx.pool(), words);
   fill_u32(ctx, heads.get(), 0, words);
+  DBuf<uint64_t> d_starts;
   if (!starts.empty()) {
-    DBuf<uint64_t> d_starts(ctx.pool(), starts.size());
+    d_starts = DBuf<uint64_t>(ctx.pool(), starts.size());
     GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
                            hipMemcpyHostToDevice, ctx.stream()));
     set_head_bits(ctx, d_starts.get(), starts.size(), heads.get(), n_out);
   }
-  for (int r = 0; r < delta_round; ++r)
+  // First inverse delta fused into the expansion; any further rounds separately.
+  DBuf<unsigned long long> status(ctx.pool(), n_tiles);
+  DBuf<uint32_t> ctl(ctx.pool(), 2);  // ticket, err
+  fill_u64(ctx, reinterpret_cast<uint64_t *>(status.get()), 0, n_tiles);
+  fill_u32(ctx, ctl.get(), 0, 2);
+  ctx.timed("expand", [&] {
+    if (rb.has_bodies)
+      hipLaunchKernelGGL(k_expand_delta<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                         ctx.stream(), d_enc, d_sym_off, n_enc, n_out, tf.get(), rb.rec.get(),
+                         rb.size.get(), bo, bd, rb.n_rules, rb.min_id, heads.get(), status.get(),
+                         ctl.get(), ctl.get() + 1, d_out);
+    else
+      hipLaunchKernelGGL(k_expand_delta<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                         ctx.stream(), d_enc, d_sym_off, n_enc, n_out, tf.get(), rb.rec.get(),
+                         rb.size.get(), bo, bd, rb.n_rules, rb.min_id, heads.get(), status.get(),
+                         ctl.get(), ctl.get() + 1, d_out);
+  });
+  GFZ_CHECK_LAUNCH();
+  ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
+  for (int r = 1; r < delta_round; ++r)
     segmented_prefix_sum(ctx, d_out, n_out, heads.get());
+  uint32_t e = 0;
+  GFZ_HIP(hipMemcpyAsync(&e, ctl.get() + 1, 4, hipMemcpyDeviceToHost, ctx.stream()));
+  ctx.sync();
+  if (e)
+    throw Error(kErrHip, "expansion look-back did not complete (internal error)");
 }
 
 void segmented_prefix_sum(Ctx &ctx, int32_t *x, uint64_t n, const uint32_t *heads) {

@@ -97,6 +97,7 @@ void exclusive_scan_impl(Ctx &ctx, const T *in, uint64_t *out, uint64_t n) {
 constexpr uint32_t kSortItems = 16;
 constexpr uint32_t kSortTile = kBlock * kSortItems;
 constexpr uint64_t kSign = 0x8000000000000000ull;
+constexpr uint32_t kInlineOffsetBlocks = 128;
 
 __global__ __launch_bounds__(kBlock) void k_or_diff(const uint64_t *__restrict__ keys, uint64_t n,
                                                     unsigned long long *acc) {
@@ -131,16 +132,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t *__restric
 
 // Stable scatter: keys keep their input order within a digit. Ranks inside a
 // wave come from eight ballots (lanes whose digit agrees on every bit).
+// With offs == nullptr the block derives its digit offsets from the per-block
+// histograms itself (cheap while nb is small), saving the three scan launches.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t *__restrict__ in,
                                                           uint64_t *__restrict__ out, uint64_t n,
                                                           uint32_t shift,
                                                           const uint64_t *__restrict__ offs,
+                                                          const uint32_t *__restrict__ hist,
                                                           uint32_t nb) {
   __shared__ uint64_t base_off[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[kBlock / kWave][256];
+  __shared__ uint64_t lds[kBlock / kWave + 1];
   const uint32_t tid = threadIdx.x, wid = tid / kWave;
-  base_off[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
+  if (offs) {
+    base_off[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
+  } else {
+    uint64_t tot = 0, pre = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t h = hist[(uint64_t)tid * nb + b];
+      tot += h;
+      if (b < blockIdx.x)
+        pre += h;
+    }
+    uint64_t total;
+    base_off[tid] = block_exclusive_scan<kBlock>(tot, lds, total) + pre;
+  }
   run[tid] = 0;
   const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
   for (uint32_t j = 0; j < kSortItems; ++j) {
@@ -337,9 +354,12 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
       hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, ctx.stream(), src, n, shift,
                          hist.get(), nb);
       GFZ_CHECK_LAUNCH();
-      exclusive_scan(ctx, hist.get(), offs.get(), (uint64_t)256 * nb);
+      const bool inline_offsets = nb <= kInlineOffsetBlocks;
+      if (!inline_offsets)
+        exclusive_scan(ctx, hist.get(), offs.get(), (uint64_t)256 * nb);
       hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, ctx.stream(), src, dst, n,
-                         shift, offs.get(), nb);
+                         shift, inline_offsets ? (const uint64_t *)nullptr : offs.get(),
+                         hist.get(), nb);
       GFZ_CHECK_LAUNCH();
     });
     std::swap(src, dst);
