@@ -38,20 +38,27 @@ __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
   return (heads[i >> 5] >> (i & 31)) & 1u;
 }
 
+// HBM pair table: 16-byte slots {key, val} so that one probe fetches a key and
+// its state in a single cache line. val: 0 = seen once, 1 = repeated; after the
+// sort it holds the final rule ID of a used key. `used` is a byte per slot.
+struct alignas(16) Slot {
+  unsigned long long key;  // kEmptyKey = free
+  uint32_t val;
+  uint32_t pad;
+};
+
 struct DevTable {
-  unsigned long long *keys;  // kEmptyKey = free
-  uint32_t *val;             // 0 seen once, 1 repeated; later the final rule ID
-  uint8_t *used;
+  Slot *slots;
   uint64_t mask;
   uint32_t *distinct;
   uint32_t *overflow;
   uint32_t limit;
 };
 
-__device__ inline int64_t table_find(const unsigned long long *keys, uint64_t mask, uint64_t k) {
+__device__ inline int64_t table_find(const Slot *slots, uint64_t mask, uint64_t k) {
   uint64_t s = key_hash(k) & mask;
   for (uint64_t p = 0; p <= mask; ++p) {
-    const uint64_t c = keys[s];
+    const uint64_t c = slots[s].key;
     if (c == k)
       return (int64_t)s;
     if (c == kEmptyKey)
@@ -61,77 +68,120 @@ __device__ inline int64_t table_find(const unsigned long long *keys, uint64_t ma
   return -1;
 }
 
-// Insert k (seen `cnt` times in the caller's tile). Returns true when k was new
-// to the table; the caller aggregates those counts before touching the shared
-// distinct-key counter (one global atomic per block, not per key).
-// A plain probe runs first: keys never change once set, so finding k that way
-// is final, and the hot keys every tile shares (the backbone's (+1,+1) and its
-// bubble neighbours) are resolved from cache without a device-scope atomic.
-__device__ inline bool table_insert(const DevTable &t, uint64_t k, uint32_t cnt) {
-  uint64_t s = key_hash(k) & t.mask;
-  for (uint64_t p = 0; p <= t.mask; ++p) {
-    const uint64_t c = t.keys[s];
-    if (c == k) {
-      if (!t.val[s])
-        t.val[s] = 1u;
-      return false;
-    }
-    if (c == kEmptyKey)
-      break;
-    s = (s + 1) & t.mask;
-  }
-  for (uint64_t p = 0; p <= t.mask; ++p) {
-    const unsigned long long old = atomicCAS(&t.keys[s], (unsigned long long)kEmptyKey,
-                                             (unsigned long long)k);
-    if (old == kEmptyKey) {
-      if (cnt >= 2)
-        t.val[s] = 1u;
-      return true;
-    }
-    if (old == k) {
-      t.val[s] = 1u;
-      return false;
-    }
-    s = (s + 1) & t.mask;
-  }
-  atomicExch(t.overflow, 1u);
-  return false;
-}
-
-// Probe up to N keys at once: the loads of all lanes' keys are issued together
-// so the dependent chain per element is one or two rounds of independent loads
-// rather than N serial probes. slot[j] = -1 when absent or invalid.
+// Probe N keys at once: every round of the loop issues one load per pending key
+// before looking at any result, so the dependent chain per key is one or two
+// rounds of independent loads. slot[j] = -1 when absent or not valid.
 template <int N>
-__device__ inline void table_find_batch(const unsigned long long *__restrict__ keys,
-                                        uint64_t mask, const uint64_t (&k)[N],
-                                        const bool (&valid)[N], int64_t (&slot)[N]) {
+__device__ inline void table_find_batch(const Slot *__restrict__ slots, uint64_t mask,
+                                        const uint64_t (&k)[N], const bool (&valid)[N],
+                                        int64_t (&slot)[N], uint32_t (&val)[N]) {
   uint64_t s[N];
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     s[j] = key_hash(k[j]) & mask;
     slot[j] = -1;
+    val[j] = 0;
     if (valid[j])
       pending |= 1u << j;
   }
   for (uint64_t p = 0; pending && p <= mask; ++p) {
-    uint64_t c[N];
+    Slot c[N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      c[j] = (pending >> j) & 1 ? keys[s[j]] : 0;
+      if ((pending >> j) & 1)
+        c[j] = slots[s[j]];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((pending >> j) & 1))
         continue;
-      if (c[j] == k[j]) {
+      if (c[j].key == k[j]) {
         slot[j] = (int64_t)s[j];
+        val[j] = c[j].val;
         pending &= ~(1u << j);
-      } else if (c[j] == kEmptyKey) {
+      } else if (c[j].key == kEmptyKey) {
         pending &= ~(1u << j);
       } else {
         s[j] = (s[j] + 1) & mask;
       }
     }
+  }
+}
+
+// Insert up to N keys (bit j of `have`), key j seen cnt[j] times in the caller's
+// tile; returns how many keys were new to the table. A plain probe runs first:
+// a key never changes once set, so finding it that way is final, and the hot
+// keys every tile shares (the backbone's (+1,+1) and its bubble neighbours) are
+// resolved from cache without a device-scope atomic. Keys not found claim an
+// empty slot with a CAS. All pending keys issue their memory operation in the
+// same round.
+template <int N>
+__device__ inline uint32_t table_insert_batch(const DevTable &t, const uint64_t (&k)[N],
+                                              const uint32_t (&cnt)[N], uint32_t have) {
+  uint64_t s[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    s[j] = key_hash(k[j]) & t.mask;
+  uint32_t probe = have, cas = 0, fresh = 0;
+  for (uint64_t p = 0; probe && p <= t.mask; ++p) {
+    Slot c[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if ((probe >> j) & 1)
+        c[j] = t.slots[s[j]];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (!((probe >> j) & 1))
+        continue;
+      if (c[j].key == k[j]) {
+        if (!c[j].val)
+          t.slots[s[j]].val = 1u;
+        probe &= ~(1u << j);
+      } else if (c[j].key == kEmptyKey) {
+        probe &= ~(1u << j);
+        cas |= 1u << j;
+      } else {
+        s[j] = (s[j] + 1) & t.mask;
+      }
+    }
+  }
+  for (uint64_t p = 0; cas && p <= t.mask; ++p) {
+    unsigned long long old[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if ((cas >> j) & 1)
+        old[j] = atomicCAS(&t.slots[s[j]].key, (unsigned long long)kEmptyKey,
+                           (unsigned long long)k[j]);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (!((cas >> j) & 1))
+        continue;
+      if (old[j] == kEmptyKey) {
+        ++fresh;
+        if (cnt[j] >= 2)
+          t.slots[s[j]].val = 1u;
+        cas &= ~(1u << j);
+      } else if (old[j] == k[j]) {
+        t.slots[s[j]].val = 1u;
+        cas &= ~(1u << j);
+      } else {
+        s[j] = (s[j] + 1) & t.mask;
+      }
+    }
+  }
+  if (probe | cas)
+    atomicExch(t.overflow, 1u);
+  return fresh;
+}
+
+__global__ void k_fill_slots(Slot *slots, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    Slot e;
+    e.key = kEmptyKey;
+    e.val = 0;
+    e.pad = 0;
+    slots[i] = e;
   }
 }
 
@@ -210,10 +260,23 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   __syncthreads();
   uint32_t fresh = 0;
   if (!__hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
-      const unsigned long long k = skeys[s];
-      if (k != kEmptyKey && table_insert(t, k, scnt[s]))
-        ++fresh;
+    constexpr int kPer = kLdsSlots / kBlock;
+    constexpr int kHalf = kPer / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint64_t fk[kHalf];
+      uint32_t fc[kHalf];
+      uint32_t have = 0;
+#pragma unroll
+      for (int i = 0; i < kHalf; ++i) {
+        const uint32_t sl = threadIdx.x + (uint32_t)(h * kHalf + i) * kBlock;
+        fk[i] = skeys[sl];
+        fc[i] = scnt[sl];
+        if (fk[i] != kEmptyKey)
+          have |= 1u << i;
+      }
+      if (have)
+        fresh += table_insert_batch<kHalf>(t, fk, fc, have);
     }
   }
 #pragma unroll
@@ -245,11 +308,11 @@ __device__ inline uint32_t wave_fn_inclusive(uint32_t f) {
 
 // ---- K2: candidate bits and each tile's transfer function --------------------
 constexpr int kProbeBatch = 8;
+constexpr int kEmitBatch = 4;
 
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
-                                                 const unsigned long long *__restrict__ keys,
-                                                 const uint32_t *__restrict__ val, uint64_t mask,
+                                                 const Slot *__restrict__ slots, uint64_t mask,
                                                  uint64_t *__restrict__ cand, uint64_t n_words,
                                                  uint32_t *__restrict__ tile_fn) {
   __shared__ uint64_t wbits[kRtWords];
@@ -265,11 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
       k[b] = valid[b] ? (uint64_t)canon2(pack2(x[i], x[i + 1])) : 0;
     }
     int64_t slot[kProbeBatch];
-    table_find_batch<kProbeBatch>(keys, mask, k, valid, slot);
     uint32_t v[kProbeBatch];
-#pragma unroll
-    for (int b = 0; b < kProbeBatch; ++b)
-      v[b] = slot[b] >= 0 ? val[slot[b]] : 0u;
+    table_find_batch<kProbeBatch>(slots, mask, k, valid, slot, v);
 #pragma unroll
     for (int b = 0; b < kProbeBatch; ++b) {
       const uint64_t m = __ballot(v[b] != 0);
@@ -364,7 +424,7 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    const unsigned long long *__restrict__ keys, uint64_t mask, uint8_t *__restrict__ used,
+    const Slot *__restrict__ slots, uint64_t mask, uint8_t *__restrict__ used,
     uint32_t next, int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
     uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
@@ -401,13 +461,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     soff[lane] = sx - sc;
   }
   __syncthreads();
-  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kProbeBatch) {
-    int32_t v[kProbeBatch];
-    uint64_t kk[kProbeBatch];
-    bool pk[kProbeBatch], neg[kProbeBatch], sv[kProbeBatch], hd[kProbeBatch];
-    uint64_t pos[kProbeBatch];
+  constexpr int kEb = kEmitBatch;
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kEb) {
+    int32_t v[kEb];
+    uint64_t kk[kEb];
+    bool pk[kEb], neg[kEb], sv[kEb], hd[kEb];
+    uint64_t pos[kEb];
 #pragma unroll
-    for (int b = 0; b < kProbeBatch; ++b) {
+    for (int b = 0; b < kEb; ++b) {
       const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
       const uint64_t i = base + (uint64_t)w * 64 + lane;
       const uint64_t svm = ssurv[w];
@@ -431,10 +492,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(
         }
       }
     }
-    int64_t slot[kProbeBatch];
-    table_find_batch<kProbeBatch>(keys, mask, kk, pk, slot);
+    int64_t slot[kEb];
+    uint32_t sval[kEb];
+    table_find_batch<kEb>(slots, mask, kk, pk, slot, sval);
 #pragma unroll
-    for (int b = 0; b < kProbeBatch; ++b) {
+    for (int b = 0; b < kEb; ++b) {
       if (!sv[b])
         continue;
       int32_t o = v[b];
@@ -482,7 +544,7 @@ __device__ inline uint64_t expand_key(uint64_t c, uint64_t span) {
   return (uint64_t)pack2((int32_t)a, (int32_t)b);
 }
 
-__global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long long *__restrict__ keys,
+__global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict__ slots,
                                                          const uint8_t *__restrict__ used,
                                                          uint64_t cap, uint64_t span,
                                                          uint64_t *__restrict__ out,
@@ -511,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
   uint32_t k = 0;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    out[b + k++] = compact_key(keys[s0 + q], span);
+    out[b + k++] = compact_key(slots[s0 + q].key, span);
     flags &= flags - 1;
   }
 }
@@ -519,20 +581,19 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
 // ---- K6: final IDs next + rank, rule columns ----------------------------------
 __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
                                                    uint64_t span,
-                                                   const unsigned long long *__restrict__ keys,
-                                                   uint64_t mask, uint32_t *__restrict__ val,
+                                                   Slot *__restrict__ slots, uint64_t mask,
                                                    uint32_t next, int32_t *__restrict__ rf,
                                                    int32_t *__restrict__ rs,
                                                    uint32_t *__restrict__ err) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
     const uint64_t k = expand_key(sorted[r], span);
-    const int64_t s = table_find(keys, mask, k);
+    const int64_t s = table_find(slots, mask, k);
     if (s < 0) {
       atomicExch(err, 2u);
       continue;
     }
-    val[s] = next + (uint32_t)r;
+    slots[s].val = next + (uint32_t)r;
     rf[r] = key_first((int64_t)k);
     rs[r] = key_second((int64_t)k);
   }
@@ -540,14 +601,14 @@ __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ 
 
 // ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
 __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
-                                                  const uint32_t *__restrict__ val,
+                                                  const Slot *__restrict__ slots,
                                                   uint32_t next) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock) {
     const int32_t v = y[i];
     const uint32_t a = uabs(v);
     if (a >= next) {
-      const int32_t f = (int32_t)val[a - next];
+      const int32_t f = (int32_t)slots[a - next].val;
       y[i] = v > 0 ? f : -f;
     }
   }
@@ -653,20 +714,19 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     uint64_t cap = next_pow2(std::max<uint64_t>(
         uint64_t(1) << 23,
         round == 0 ? L / 8 : std::min<uint64_t>(2 * L, 4 * prev_distinct)));
-    DBuf<unsigned long long> tkeys;
-    DBuf<uint32_t> tval;
+    DBuf<Slot> tslots;
     DBuf<uint32_t> counters(pool, 2);  // distinct, overflow
     uint32_t distinct = 0;
     for (;;) {
       if ((uint64_t)next + cap >= (uint64_t)INT32_MAX)
         throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
                                       ", table " + std::to_string(cap) + ")");
-      tkeys = DBuf<unsigned long long>(pool, cap);
-      tval = DBuf<uint32_t>(pool, cap);
-      fill_u64(ctx, reinterpret_cast<uint64_t *>(tkeys.get()), kEmptyKey, cap);
-      fill_u32(ctx, tval.get(), 0, cap);
+      tslots = DBuf<Slot>(pool, cap);
+      hipLaunchKernelGGL(k_fill_slots, dim3(stride_grid(ctx, cap)), dim3(kBlock), 0, ctx.stream(),
+                         tslots.get(), cap);
+      GFZ_CHECK_LAUNCH();
       fill_u32(ctx, counters.get(), 0, 2);
-      DevTable t{tkeys.get(), tval.get(), nullptr, cap - 1, counters.get(), counters.get() + 1,
+      DevTable t{tslots.get(), cap - 1, counters.get(), counters.get() + 1,
                  (uint32_t)std::min<uint64_t>(cap / 2, UINT32_MAX)};
       ctx.timed("pair_insert", [&] {
         hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)ceil_div(L, kInsTile)), dim3(kBlock), 0,
@@ -696,7 +756,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint64_t> d_total(pool, 1);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, tkeys.get(), tval.get(), cap - 1, cand.get(), n_words,
+                         heads.get(), L, tslots.get(), cap - 1, cand.get(), n_words,
                          tile_fn.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -716,7 +776,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     fill_u32(ctx, heads_next.get(), 0, ceil_div(L_next, 32));
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), n_words, tile_info.get(), tkeys.get(),
+                         heads.get(), L, cand.get(), n_words, tile_info.get(), tslots.get(),
                          cap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -732,7 +792,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
     ctx.timed("collect_used", [&] {
       hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(cap, kCollectTile)), dim3(kBlock), 0,
-                         ctx.stream(), tkeys.get(), used.get(), cap, span, ukeys.get(),
+                         ctx.stream(), tslots.get(), used.get(), cap, span, ukeys.get(),
                          ucount.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -760,14 +820,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     }
     ctx.timed("assign_ids", [&] {
       hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         ukeys.get(), R, span, tkeys.get(), cap - 1, tval.get(), next,
+                         ukeys.get(), R, span, tslots.get(), cap - 1, next,
                          rf.get() + out.n_rules, rs.get() + out.n_rules, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     // K7: remap provisional IDs.
     ctx.timed("remap", [&] {
       hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
-                         dst, L_next, tval.get(), next);
+                         dst, L_next, tslots.get(), next);
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);

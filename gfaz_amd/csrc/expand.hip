@@ -473,40 +473,65 @@ __global__ __launch_bounds__(kBlock) void k_expand_delta(
   }
   Seg total;
   const Seg pre = block_seg_exclusive<kBlock>(a, slds, total);
-  if (tid == 0) {
-    Seg carry{0, 0};
+  // Look-back by wave 0 over windows of 64 predecessors: lane l reads the status
+  // of tile (base - l). The nearest predecessor that is inclusive, or whose own
+  // aggregate already contains a traversal start, ends the walk; everything
+  // nearer must have published its aggregate, else the window is re-read.
+  if (tid < kWave) {
     if (tile == 0) {
-      __hip_atomic_store(&status[0], st_pack(kStInclusive, total), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) {
+        __hip_atomic_store(&status[0], st_pack(kStInclusive, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        scarry = Seg{0, 0};
+      }
     } else {
-      __hip_atomic_store(&status[tile],
-                         st_pack(total.flag ? kStInclusive : kStAggregate, total),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t pred = (int64_t)tile - 1;
+      if (lane == 0)
+        __hip_atomic_store(&status[tile],
+                           st_pack(total.flag ? kStInclusive : kStAggregate, total),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      Seg carry{0, 0};
+      int64_t wbase = (int64_t)tile - 1;
       uint32_t spins = 0;
-      while (pred >= 0) {
-        const uint64_t st = __hip_atomic_load(&status[pred], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+      for (;;) {
+        const int64_t pt = wbase - (int64_t)lane;
+        uint64_t st = kStInclusive;  // before tile 0: the identity, inclusive
+        if (pt >= 0)
+          st = __hip_atomic_load(&status[pt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t state = st & (3ull << 62);
-        if (state == kStNotReady) {
-          if (++spins > (1u << 26)) {
-            atomicExch(err, 1u);
+        const uint32_t fl = (uint32_t)(st >> 32) & 1u;
+        const bool stop = state == kStInclusive || (state == kStAggregate && fl);
+        const uint64_t stop_mask = __ballot(stop);
+        const uint64_t nr_mask = __ballot(state == kStNotReady);
+        const uint32_t L = stop_mask ? (uint32_t)__builtin_ctzll(stop_mask) : (uint32_t)kWave;
+        const uint64_t need = L >= (uint32_t)kWave ? ~0ull : ((1ull << L) - 1ull);
+        if (nr_mask & need) {
+          if (++spins > (1u << 24)) {
+            if (lane == 0)
+              atomicExch(err, 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        const Seg pv{(uint32_t)st, (uint32_t)(st >> 32) & 1u};
-        carry = seg_then(pv, carry);
-        if (state == kStInclusive || pv.flag)
+        const uint32_t top = L >= (uint32_t)kWave ? (uint32_t)kWave - 1 : L;
+        Seg acc{0, 0};
+        for (int l = (int)top; l >= 0; --l) {
+          const uint32_t ps = __shfl((uint32_t)st, l);
+          const uint32_t pf = __shfl(fl, l);
+          acc = seg_then(acc, Seg{ps, pf});
+        }
+        carry = seg_then(acc, carry);
+        if (L < (uint32_t)kWave)
           break;
-        --pred;
+        wbase -= kWave;
       }
-      if (!total.flag)
-        __hip_atomic_store(&status[tile], st_pack(kStInclusive, seg_then(carry, total)),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) {
+        if (!total.flag)
+          __hip_atomic_store(&status[tile], st_pack(kStInclusive, seg_then(carry, total)),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        scarry = carry;
+      }
     }
-    scarry = carry;
   }
   __syncthreads();
   Seg run = seg_then(scarry, pre);

@@ -97,7 +97,7 @@ void exclusive_scan_impl(Ctx &ctx, const T *in, uint64_t *out, uint64_t n) {
 constexpr uint32_t kSortItems = 16;
 constexpr uint32_t kSortTile = kBlock * kSortItems;
 constexpr uint64_t kSign = 0x8000000000000000ull;
-constexpr uint32_t kInlineOffsetBlocks = 128;
+constexpr uint32_t kInlineOffsetBlocks = 16;
 
 __global__ __launch_bounds__(kBlock) void k_or_diff(const uint64_t *__restrict__ keys, uint64_t n,
                                                     unsigned long long *acc) {
