@@ -95,6 +95,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-oracle", action="store_true",
                     help="also compare the encoded grammar with the CPU oracle")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="for hardware-counter runs: one 1 GiB calibration copy, then one "
+                         "compress and one decompress of the workload; prints a JSON summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every GPU box here has a single MI355X "
@@ -113,6 +116,27 @@ def main() -> int:
     ctx = codec.Context(0)
     d_nodes = codec.DeviceArray.from_host(ctx, tr.nodes)
     d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
+
+    if args.pmc_pass:
+        nbytes = 1 << 30
+        a = codec.DeviceArray(ctx, nbytes // 4, np.int32)
+        b = codec.DeviceArray(ctx, nbytes // 4, np.int32)
+        copy_ms = ctx.calibration_copy(b, a, nbytes)
+        a.free()
+        b.free()
+        comp = codec.DeviceCompressed(ctx, d_nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+        rb = comp.rulebook()
+        codec.decompress_traversals(ctx, rb, comp.encoded_view(), comp.encoded_lengths,
+                                    tr.lengths, 1, out=d_out)
+        ctx.sync()
+        ok = bool(np.array_equal(d_out.to_host(), tr.nodes))
+        print(json.dumps({"pmc_pass": cfg.name, "copy_bytes": nbytes, "copy_ms": copy_ms,
+                          "steps": tr.n_steps, "encoded": comp.n_encoded, "rules": comp.n_rules,
+                          "round_trip_ok": ok}), flush=True)
+        rb.close()
+        comp.close()
+        ctx.close()
+        return 0 if ok else 1
 
     def step():
         t0 = time.perf_counter()

@@ -69,6 +69,8 @@ _SIGNATURES = {
     "gfz_dev_free": ([_vp, _vp], ctypes.c_int),
     "gfz_memcpy_h2d": ([_vp, _vp, _vp, _u64], ctypes.c_int),
     "gfz_memcpy_d2h": ([_vp, _vp, _vp, _u64], ctypes.c_int),
+    "gfz_calibration_copy": ([_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double)],
+                             ctypes.c_int),
     "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, _u32, ctypes.POINTER(_CompressOut)], ctypes.c_int),
     "gfz_compress_free": ([ctypes.POINTER(_CompressOut)], None),
@@ -184,6 +186,13 @@ class Context:
 
     def num_cus(self) -> int:
         return int(lib().gfz_ctx_num_cus(self.handle))
+
+    def calibration_copy(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int) -> float:
+        """Plain device copy of nbytes; returns its duration in ms."""
+        ms = ctypes.c_double()
+        _check(lib().gfz_calibration_copy(self.handle, dst.ptr, src.ptr, int(nbytes),
+                                          ctypes.byref(ms)))
+        return float(ms.value)
 
 
 @dataclass

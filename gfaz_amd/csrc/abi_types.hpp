@@ -1,0 +1,18 @@
+// Definitions behind the opaque handles of include/gfaz_amd.h.
+#pragma once
+
+#include "gfaz_amd.h"
+
+#include "codec.hpp"
+#include "ctx.hpp"
+
+#include <memory>
+
+struct gfz_ctx {
+  std::unique_ptr<gfz::Ctx> ctx;
+};
+
+struct gfz_rulebook {
+  gfz_ctx *owner = nullptr;
+  gfz::Rulebook rb;
+};

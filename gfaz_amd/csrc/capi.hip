@@ -11,14 +11,7 @@
 #include <memory>
 #include <string>
 
-struct gfz_ctx {
-  std::unique_ptr<gfz::Ctx> ctx;
-};
-
-struct gfz_rulebook {
-  gfz_ctx *owner = nullptr;
-  gfz::Rulebook rb;
-};
+#include "abi_types.hpp"
 
 namespace {
 

@@ -220,7 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   __shared__ unsigned long long skeys[kLdsSlots];
   __shared__ uint32_t scnt[kLdsSlots];
   __shared__ uint32_t snew[kBlock / kWave];
+  __shared__ uint32_t sover;
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  // One coherent read of the shared overflow flag per block (not per thread).
+  if (threadIdx.x == 0)
+    sover = __hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
     skeys[s] = kEmptyKey;
     scnt[s] = 0;
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   }
   __syncthreads();
   uint32_t fresh = 0;
-  if (!__hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+  if (!sover) {
     constexpr int kPer = kLdsSlots / kBlock;
     constexpr int kHalf = kPer / 2;
 #pragma unroll

@@ -127,15 +127,30 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
     dst[i] = src[idx[i]];
 }
 
-// tile_first[t] = index of the symbol covering output position base + t*kExTile
-// (symbols are disjoint, non-empty intervals, so each tile start has one writer).
-__global__ void k_tile_first(const uint64_t *__restrict__ sym_off, uint64_t n_sym, uint64_t base,
-                             uint64_t n_tiles, uint64_t *__restrict__ tile_first) {
+// Per output tile: the symbols covering its first and last positions and where
+// the first one starts (relative to base). Symbols are disjoint, non-empty
+// intervals, so every field has exactly one writer.
+struct TileDesc {
+  uint64_t k0;
+  uint64_t first_off;
+  uint64_t k_last;
+  uint64_t pad;
+};
+
+__global__ void k_tile_desc(const uint64_t *__restrict__ sym_off, uint64_t n_sym, uint64_t base,
+                            uint64_t n_out, uint64_t n_tiles, TileDesc *__restrict__ desc) {
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_sym;
        k += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t lo = sym_off[k] - base, hi = sym_off[k + 1] - base;
-    for (uint64_t t = (lo + kExTile - 1) / kExTile; t < n_tiles && t * kExTile < hi; ++t)
-      tile_first[t] = k;
+    for (uint64_t t = lo / kExTile; t < n_tiles && t * kExTile < hi; ++t) {
+      if (t * kExTile >= lo) {
+        desc[t].k0 = k;
+        desc[t].first_off = lo;
+      }
+      const uint64_t end = ((t + 1) * kExTile < n_out ? (t + 1) * kExTile : n_out) - 1;
+      if (end >= lo && end < hi)
+        desc[t].k_last = k;
+    }
   }
 }
 
@@ -147,7 +162,7 @@ __global__ void k_tile_first(const uint64_t *__restrict__ sym_off, uint64_t n_sy
 template <bool kBodies>
 __global__ __launch_bounds__(kBlock) void k_expand(
     const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
-    uint64_t base, uint64_t n_out, const uint64_t *__restrict__ tile_first,
+    uint64_t base, uint64_t n_out, const TileDesc *__restrict__ tdesc,
     const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
     const uint64_t *__restrict__ body_off, const int32_t *__restrict__ bodies, uint64_t R,
     uint32_t min_id, int32_t *__restrict__ out) {
@@ -155,26 +170,16 @@ __global__ __launch_bounds__(kBlock) void k_expand(
   __shared__ int32_t ssym[kExTile];
   __shared__ unsigned long long sbits[kExTile / 64];
   __shared__ uint32_t srank[kExTile / 64];
-  __shared__ uint64_t sfirst_off;
-  __shared__ uint32_t snsym;
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
   const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
   const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
   if (tid < kExTile / 64)
     sbits[tid] = 0;
-  if (tid == 0) {
-    const uint64_t k0 = tile_first[blockIdx.x];
-    uint64_t k_last = n_sym - 1;
-    if (o_end < n_out) {
-      const uint64_t tf = tile_first[blockIdx.x + 1];
-      k_last = (sym_off[tf] - base == o_end) ? tf - 1 : tf;
-    }
-    snsym = (uint32_t)(k_last - k0 + 1);
-    sfirst_off = sym_off[k0] - base;
-  }
+  const TileDesc td = tdesc[blockIdx.x];
+  const uint64_t k0 = td.k0;
+  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
+  (void)n_sym;
   __syncthreads();
-  const uint64_t k0 = tile_first[blockIdx.x];
-  const uint32_t nsym = snsym;
   for (uint32_t s = tid; s < nsym; s += kBlock) {
     const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - base - o0);
     srel[s] = rel;
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(
     srank[lane] = x - c;
   }
   __syncthreads();
-  const uint64_t first_off = sfirst_off;
+  const uint64_t first_off = td.first_off;
 #pragma unroll 4
   for (uint32_t j = 0; j < kExItems; ++j) {
     const uint32_t p = j * kBlock + tid;
@@ -347,7 +352,7 @@ __device__ inline uint64_t st_pack(uint64_t state, Seg v) {
 template <bool kBodies>
 __global__ __launch_bounds__(kBlock) void k_expand_delta(
     const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
-    uint64_t n_out, const uint64_t *__restrict__ tile_first, const RuleRec *__restrict__ rec,
+    uint64_t n_out, const TileDesc *__restrict__ tdesc, const RuleRec *__restrict__ rec,
     const uint32_t *__restrict__ rsize, const uint64_t *__restrict__ body_off,
     const int32_t *__restrict__ bodies, uint64_t R, uint32_t min_id,
     const uint32_t *__restrict__ heads, unsigned long long *status, uint32_t *ticket,
@@ -362,8 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_delta(
   __shared__ unsigned long long sbits[kExTile / 64];
   __shared__ uint32_t srank[kExTile / 64];
   __shared__ Seg slds[kBlock / kWave + 1];
-  __shared__ uint64_t sfirst_off;
-  __shared__ uint32_t snsym, stile;
+  __shared__ uint32_t stile;
   __shared__ Seg scarry;
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
   if (tid == 0)
@@ -374,19 +378,10 @@ __global__ __launch_bounds__(kBlock) void k_expand_delta(
   const uint32_t tile = stile;
   const uint64_t o0 = (uint64_t)tile * kExTile;
   const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
-  if (tid == 0) {
-    const uint64_t k0 = tile_first[tile];
-    uint64_t k_last = n_sym - 1;
-    if (o_end < n_out) {
-      const uint64_t tf = tile_first[tile + 1];
-      k_last = (sym_off[tf] == o_end) ? tf - 1 : tf;
-    }
-    snsym = (uint32_t)(k_last - k0 + 1);
-    sfirst_off = sym_off[k0];
-  }
-  __syncthreads();
-  const uint64_t k0 = tile_first[tile];
-  const uint32_t nsym = snsym;
+  const TileDesc td = tdesc[tile];
+  const uint64_t k0 = td.k0;
+  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
+  (void)n_sym;
   for (uint32_t s = tid; s < nsym; s += kBlock) {
     const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - o0);
     srel[s] = rel;
@@ -406,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_delta(
     srank[lane] = x - c;
   }
   __syncthreads();
-  const uint64_t first_off = sfirst_off;
+  const uint64_t first_off = td.first_off;
   // Expand in the striped layout (consecutive lanes, consecutive positions).
   int32_t ev[kExItems];
 #pragma unroll
@@ -715,10 +710,11 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
       continue;
     const uint64_t lo = lvl_off[d], hi = lvl_off[d + 1];
     const uint64_t n_tiles = ceil_div(hi - lo, kExTile);
-    DBuf<uint64_t> tf(ctx.pool(), n_tiles);
+    DBuf<TileDesc> tf(ctx.pool(), n_tiles);
     ctx.timed("build_bodies", [&] {
-      hipLaunchKernelGGL(k_tile_first, dim3(stride_grid(ctx, 2 * (r1 - r0))), dim3(kBlock), 0,
-                         ctx.stream(), ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, n_tiles, tf.get());
+      hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, 2 * (r1 - r0))), dim3(kBlock), 0,
+                         ctx.stream(), ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, hi - lo, n_tiles,
+                         tf.get());
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
                          ps.get() + 2 * r0, ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, hi - lo,
@@ -791,9 +787,9 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const uint64_t n_tiles = ceil_div(n_out, kExTile);
   if (n_tiles > UINT32_MAX)
     throw Error(kErrInvalid, "decoded batch too large for one launch");
-  DBuf<uint64_t> tf(ctx.pool(), n_tiles);
-  hipLaunchKernelGGL(k_tile_first, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                     d_sym_off, n_enc, 0, n_tiles, tf.get());
+  DBuf<TileDesc> tf(ctx.pool(), n_tiles);
+  hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                     d_sym_off, n_enc, 0, n_out, n_tiles, tf.get());
   GFZ_CHECK_LAUNCH();
   const uint64_t *bo = rb.has_bodies ? rb.body_off.get() : nullptr;
   const int32_t *bd = rb.has_bodies ? rb.bodies.get() : nullptr;

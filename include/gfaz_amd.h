@@ -142,6 +142,12 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
                               int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
                               uint32_t *out_lengths, int *length_mismatch);
 
+/* ---- measurement support -------------------------------------------------
+ * No counterpart in the reference: a plain copy of `bytes` (a multiple of 16)
+ * between device buffers, timed with HIP events on the ctx stream. Profiles use
+ * it to scale the HBM traffic counters read for the codec kernels. */
+int gfz_calibration_copy(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
