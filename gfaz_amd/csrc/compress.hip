@@ -38,21 +38,48 @@ __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
   return (heads[i >> 5] >> (i & 31)) & 1u;
 }
 
-// HBM pair table: 16-byte slots {key, val} so that one probe fetches a key and
-// its state in a single cache line. val: 0 = seen once, 1 = repeated; after the
-// sort it holds the final rule ID of a used key. `used` is a byte per slot.
-struct alignas(16) Slot {
-  unsigned long long key;  // kEmptyKey = free
-  uint32_t val;
-  uint32_t pad;
+// Keys are compacted before they reach any table: every symbol of a round has
+// |v| < next = span / 2, so ck = (a + next) * span + (b >= 0 ? b : b + span)
+// keeps the signed packed-key order of (a, b) (a signed, then b as unsigned 32
+// bits, include/grammar/packed_2mer.hpp:15-17 and the int64 sort of
+// src/workflows/compression_workflow.cpp:648-655) in fewer than 62 bits. That
+// leaves the top bit free for the "repeated" mark of the counting table, and
+// all-ones free as the empty slot.
+constexpr uint64_t kFree = ~0ull;
+constexpr uint64_t kRep = 1ull << 63;
+
+__device__ inline uint64_t compact_key(int64_t k, uint64_t span) {
+  const int64_t a = key_first(k), b = key_second(k);
+  const int64_t half = (int64_t)(span / 2);
+  return (uint64_t)(a + half) * span + (uint64_t)(b >= 0 ? b : b + (int64_t)span);
+}
+__device__ inline int64_t expand_key(uint64_t c, uint64_t span) {
+  const int64_t half = (int64_t)(span / 2);
+  const int64_t a = (int64_t)(c / span) - half;
+  const int64_t bq = (int64_t)(c % span);
+  const int64_t b = bq < half ? bq : bq - (int64_t)span;
+  return pack2((int32_t)a, (int32_t)b);
+}
+__device__ inline uint64_t pair_key(int32_t a, int32_t b, uint64_t span) {
+  return compact_key(canon2(pack2(a, b)), span);
+}
+
+// Counting table (one 8-byte word per slot): key | kRep once the key was seen
+// twice. Its only job is to find the repeated keys of a round.
+struct CountTable {
+  unsigned long long *slots;
+  uint64_t mask;
+  uint32_t *counters;  // [0] distinct keys, [1] repeated keys, [2] overflow
+  uint32_t limit;
 };
 
-struct DevTable {
-  Slot *slots;
-  uint64_t mask;
-  uint32_t *distinct;
-  uint32_t *overflow;
-  uint32_t limit;
+// Rule table (16-byte slots): the repeated keys only, dense enough to stay in
+// cache in late rounds. val = final rule ID after the sort; the slot index is
+// the provisional rule ID within the round.
+struct alignas(16) Slot {
+  unsigned long long key;  // kFree = empty
+  uint32_t val;
+  uint32_t pad;
 };
 
 __device__ inline int64_t table_find(const Slot *slots, uint64_t mask, uint64_t k) {
@@ -61,7 +88,7 @@ __device__ inline int64_t table_find(const Slot *slots, uint64_t mask, uint64_t 
     const uint64_t c = slots[s].key;
     if (c == k)
       return (int64_t)s;
-    if (c == kEmptyKey)
+    if (c == kFree)
       return -1;
     s = (s + 1) & mask;
   }
@@ -69,37 +96,33 @@ __device__ inline int64_t table_find(const Slot *slots, uint64_t mask, uint64_t 
 }
 
 // Probe N keys at once: every round of the loop issues one load per pending key
-// before looking at any result, so the dependent chain per key is one or two
-// rounds of independent loads. slot[j] = -1 when absent or not valid.
+// before looking at any result. slot[j] = -1 when absent or not valid.
 template <int N>
 __device__ inline void table_find_batch(const Slot *__restrict__ slots, uint64_t mask,
                                         const uint64_t (&k)[N], const bool (&valid)[N],
-                                        int64_t (&slot)[N], uint32_t (&val)[N]) {
+                                        int64_t (&slot)[N]) {
   uint64_t s[N];
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     s[j] = key_hash(k[j]) & mask;
     slot[j] = -1;
-    val[j] = 0;
     if (valid[j])
       pending |= 1u << j;
   }
   for (uint64_t p = 0; pending && p <= mask; ++p) {
-    Slot c[N];
+    uint64_t c[N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      if ((pending >> j) & 1)
-        c[j] = slots[s[j]];
+      c[j] = (pending >> j) & 1 ? slots[s[j]].key : 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((pending >> j) & 1))
         continue;
-      if (c[j].key == k[j]) {
+      if (c[j] == k[j]) {
         slot[j] = (int64_t)s[j];
-        val[j] = c[j].val;
         pending &= ~(1u << j);
-      } else if (c[j].key == kEmptyKey) {
+      } else if (c[j] == kFree) {
         pending &= ~(1u << j);
       } else {
         s[j] = (s[j] + 1) & mask;
@@ -108,36 +131,37 @@ __device__ inline void table_find_batch(const Slot *__restrict__ slots, uint64_t
   }
 }
 
-// Insert up to N keys (bit j of `have`), key j seen cnt[j] times in the caller's
-// tile; returns how many keys were new to the table. A plain probe runs first:
-// a key never changes once set, so finding it that way is final, and the hot
-// keys every tile shares (the backbone's (+1,+1) and its bubble neighbours) are
-// resolved from cache without a device-scope atomic. Keys not found claim an
-// empty slot with a CAS. All pending keys issue their memory operation in the
-// same round.
+// Count up to N keys (bit j of `have`), key j seen cnt[j] times in the caller's
+// tile. A plain probe runs first: a key never leaves its slot, so finding it is
+// final, and the hot keys every tile shares (the backbone's (+1,+1) and its
+// bubble neighbours) already carry kRep and cost no atomic at all. Missing keys
+// claim an empty slot with a CAS. Every pending key issues its memory operation
+// in the same round. fresh/reps count keys new to the table and keys this call
+// turned repeated (each key is turned repeated exactly once, so the block sums
+// give exact totals).
 template <int N>
-__device__ inline uint32_t table_insert_batch(const DevTable &t, const uint64_t (&k)[N],
-                                              const uint32_t (&cnt)[N], uint32_t have) {
+__device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&k)[N],
+                                          const uint32_t (&cnt)[N], uint32_t have,
+                                          uint32_t &fresh, uint32_t &reps) {
   uint64_t s[N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
     s[j] = key_hash(k[j]) & t.mask;
-  uint32_t probe = have, cas = 0, fresh = 0;
+  uint32_t probe = have, cas = 0, mark = 0;
   for (uint64_t p = 0; probe && p <= t.mask; ++p) {
-    Slot c[N];
+    uint64_t c[N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      if ((probe >> j) & 1)
-        c[j] = t.slots[s[j]];
+      c[j] = (probe >> j) & 1 ? t.slots[s[j]] : 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((probe >> j) & 1))
         continue;
-      if (c[j].key == k[j]) {
-        if (!c[j].val)
-          t.slots[s[j]].val = 1u;
+      if ((c[j] & ~kRep) == k[j]) {
+        if (!(c[j] & kRep))
+          mark |= 1u << j;
         probe &= ~(1u << j);
-      } else if (c[j].key == kEmptyKey) {
+      } else if (c[j] == kFree) {
         probe &= ~(1u << j);
         cas |= 1u << j;
       } else {
@@ -150,35 +174,46 @@ __device__ inline uint32_t table_insert_batch(const DevTable &t, const uint64_t 
 #pragma unroll
     for (int j = 0; j < N; ++j)
       if ((cas >> j) & 1)
-        old[j] = atomicCAS(&t.slots[s[j]].key, (unsigned long long)kEmptyKey,
-                           (unsigned long long)k[j]);
+        old[j] = atomicCAS(&t.slots[s[j]], (unsigned long long)kFree,
+                           (unsigned long long)(cnt[j] >= 2 ? (k[j] | kRep) : k[j]));
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((cas >> j) & 1))
         continue;
-      if (old[j] == kEmptyKey) {
+      if (old[j] == kFree) {
         ++fresh;
         if (cnt[j] >= 2)
-          t.slots[s[j]].val = 1u;
+          ++reps;
         cas &= ~(1u << j);
-      } else if (old[j] == k[j]) {
-        t.slots[s[j]].val = 1u;
+      } else if ((old[j] & ~kRep) == k[j]) {
+        if (!(old[j] & kRep))
+          mark |= 1u << j;
         cas &= ~(1u << j);
       } else {
         s[j] = (s[j] + 1) & t.mask;
       }
     }
   }
+  if (mark) {
+    unsigned long long old[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if ((mark >> j) & 1)
+        old[j] = atomicOr(&t.slots[s[j]], (unsigned long long)kRep);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (((mark >> j) & 1) && !(old[j] & kRep))
+        ++reps;
+  }
   if (probe | cas)
-    atomicExch(t.overflow, 1u);
-  return fresh;
+    atomicExch(&t.counters[2], 1u);
 }
 
 __global__ void k_fill_slots(Slot *slots, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     Slot e;
-    e.key = kEmptyKey;
+    e.key = kFree;
     e.val = 0;
     e.pad = 0;
     slots[i] = e;
@@ -208,61 +243,79 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
 }
 
 // ---- K1: repeated-pair set ----------------------------------------------------
-// Three levels of aggregation before HBM: (1) within a wave, a run of lanes with
-// the same key (consecutive steps, e.g. the backbone's (+1,+1)) becomes one
-// insert carrying the run length; (2) each block folds its tile into an LDS
-// table with counts; (3) every distinct key of the tile is inserted once into
-// the HBM table, and the block adds its number of new keys to the global
-// counter with a single atomic.
+// Three levels of aggregation before HBM: (1) within a wave, the few most common
+// keys (the backbone's (+1,+1) dominates early rounds) are matched across all
+// lanes with ballots, and the rest collapse into runs of equal consecutive keys;
+// (2) each block folds its tile into an LDS table with counts; (3) every distinct
+// key of the tile is counted once in the HBM table, and the block adds its new
+// and newly repeated keys to the global counters with one atomic each.
+constexpr int kWaveMatch = 3;
+
+__device__ inline void lds_add(unsigned long long *skeys, uint32_t *scnt, uint64_t k, uint32_t n) {
+  uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
+  for (uint32_t p = 0; p < kLdsSlots; ++p) {
+    const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kFree,
+                                             (unsigned long long)k);
+    if (old == kFree || old == k) {
+      atomicAdd(&scnt[s], n);
+      return;
+    }
+    s = (s + 1) & (kLdsSlots - 1);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restrict__ x,
                                                         const uint32_t *__restrict__ heads,
-                                                        uint64_t L, DevTable t) {
+                                                        uint64_t L, uint64_t span, CountTable t) {
   __shared__ unsigned long long skeys[kLdsSlots];
   __shared__ uint32_t scnt[kLdsSlots];
-  __shared__ uint32_t snew[kBlock / kWave];
+  __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
   __shared__ uint32_t sover;
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   // One coherent read of the shared overflow flag per block (not per thread).
   if (threadIdx.x == 0)
-    sover = __hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sover = __hip_atomic_load(&t.counters[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
-    skeys[s] = kEmptyKey;
+    skeys[s] = kFree;
     scnt[s] = 0;
   }
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
-  unsigned long long kv[kInsItems];
+  uint64_t kv[kInsItems];
   bool valid[kInsItems];
 #pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
     valid[j] = i + 1 < L && !head_at(heads, i + 1);
-    kv[j] = valid[j] ? (unsigned long long)canon2(pack2(x[i], x[i + 1]))
-                     : (unsigned long long)kEmptyKey;
+    kv[j] = valid[j] ? pair_key(x[i], x[i + 1], span) : kFree;
   }
   for (uint32_t j = 0; j < kInsItems; ++j) {
-    const unsigned long long k = kv[j];
-    const unsigned long long prev = __shfl_up(k, 1);
-    const bool run_head = valid[j] && (lane == 0 || prev != k);
-    const uint64_t vmask = __ballot(valid[j]);
+    const uint64_t k = kv[j];
+    uint64_t rem = __ballot(valid[j]);
+    // (1a) match the most common keys across the whole wave
+    for (int it = 0; it < kWaveMatch && rem; ++it) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(rem);
+      const uint64_t lk = __shfl(k, (int)leader);
+      const uint64_t same = __ballot(k == lk) & rem;
+      if (lane == leader)
+        lds_add(skeys, scnt, lk, (uint32_t)__popcll(same));
+      rem &= ~same;
+    }
+    // (1b) the rest: runs of equal consecutive keys
+    const bool mine = (rem >> lane) & 1;
+    const uint64_t prev = __shfl_up(k, 1);
+    const bool prev_mine = lane > 0 && ((rem >> (lane - 1)) & 1);
+    const bool run_head = mine && !(prev_mine && prev == k);
     const uint64_t hmask = __ballot(run_head);
     if (run_head) {
-      const uint64_t ends = hmask | ~vmask;
-      const uint64_t rest = lane == kWave - 1 ? 0ull : ends >> (lane + 1);
-      const uint32_t len = rest ? (uint32_t)__builtin_ctzll(rest) + 1u : (uint32_t)(kWave - lane);
-      uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
-      for (uint32_t p = 0; p < kLdsSlots; ++p) {
-        const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kEmptyKey, k);
-        if (old == kEmptyKey || old == k) {
-          atomicAdd(&scnt[s], len);
-          break;
-        }
-        s = (s + 1) & (kLdsSlots - 1);
-      }
+      const uint64_t ends = hmask | ~rem;
+      const uint64_t after = lane == kWave - 1 ? 0ull : ends >> (lane + 1);
+      const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u : (uint32_t)(kWave - lane);
+      lds_add(skeys, scnt, k, len);
     }
   }
   __syncthreads();
-  uint32_t fresh = 0;
+  uint32_t fresh = 0, reps = 0;
   if (!sover) {
     constexpr int kPer = kLdsSlots / kBlock;
     constexpr int kHalf = kPer / 2;
@@ -276,26 +329,71 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
         const uint32_t sl = threadIdx.x + (uint32_t)(h * kHalf + i) * kBlock;
         fk[i] = skeys[sl];
         fc[i] = scnt[sl];
-        if (fk[i] != kEmptyKey)
+        if (fk[i] != kFree)
           have |= 1u << i;
       }
       if (have)
-        fresh += table_insert_batch<kHalf>(t, fk, fc, have);
+        count_insert_batch<kHalf>(t, fk, fc, have, fresh, reps);
     }
   }
 #pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1)
+  for (int d = kWave / 2; d > 0; d >>= 1) {
     fresh += __shfl_xor(fresh, d);
-  if (lane == 0)
-    snew[wid] = fresh;
+    reps += __shfl_xor(reps, d);
+  }
+  if (lane == 0) {
+    sfresh[wid] = fresh;
+    sreps[wid] = reps;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t n = 0;
-    for (uint32_t w = 0; w < kBlock / kWave; ++w)
-      n += snew[w];
-    if (n && atomicAdd(t.distinct, n) + n > t.limit)
-      atomicExch(t.overflow, 1u);
+    uint32_t n = 0, r = 0;
+    for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+      n += sfresh[w];
+      r += sreps[w];
+    }
+    if (n && atomicAdd(&t.counters[0], n) + n > t.limit)
+      atomicExch(&t.counters[2], 1u);
+    if (r)
+      atomicAdd(&t.counters[1], r);
   }
+}
+
+// ---- K1b: the repeated keys, moved into the rule table -------------------------
+constexpr uint32_t kExtractItems = 16;
+
+__global__ __launch_bounds__(kBlock) void k_extract_repeated(
+    const unsigned long long *__restrict__ count_slots, uint64_t cap, Slot *__restrict__ rt,
+    uint64_t rmask, uint32_t *__restrict__ err) {
+  const uint64_t s0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kExtractItems;
+  uint64_t k[kExtractItems], s[kExtractItems];
+  uint32_t pending = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kExtractItems; ++i) {
+    const uint64_t v = s0 + i < cap ? count_slots[s0 + i] : kFree;
+    k[i] = v & ~kRep;
+    if (v != kFree && (v & kRep))
+      pending |= 1u << i;
+    s[i] = key_hash(k[i]) & rmask;
+  }
+  for (uint64_t p = 0; pending && p <= rmask; ++p) {
+    unsigned long long old[kExtractItems];
+#pragma unroll
+    for (uint32_t i = 0; i < kExtractItems; ++i)
+      if ((pending >> i) & 1)
+        old[i] = atomicCAS(&rt[s[i]].key, (unsigned long long)kFree, (unsigned long long)k[i]);
+#pragma unroll
+    for (uint32_t i = 0; i < kExtractItems; ++i) {
+      if (!((pending >> i) & 1))
+        continue;
+      if (old[i] == kFree)
+        pending &= ~(1u << i);
+      else
+        s[i] = (s[i] + 1) & rmask;
+    }
+  }
+  if (pending)
+    atomicExch(err, 3u);
 }
 
 // Inclusive scan of transfer functions across the 64 lanes of a wave.
@@ -311,14 +409,15 @@ __device__ inline uint32_t wave_fn_inclusive(uint32_t f) {
 }
 
 // ---- K2: candidate bits and each tile's transfer function --------------------
+// A pair is a candidate iff its key is in the rule table (the repeated keys).
 constexpr int kProbeBatch = 8;
 constexpr int kEmitBatch = 4;
 
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
-                                                 const Slot *__restrict__ slots, uint64_t mask,
-                                                 uint64_t *__restrict__ cand, uint64_t n_words,
-                                                 uint32_t *__restrict__ tile_fn) {
+                                                 uint64_t span, const Slot *__restrict__ rt,
+                                                 uint64_t rmask, uint64_t *__restrict__ cand,
+                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn) {
   __shared__ uint64_t wbits[kRtWords];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
@@ -329,14 +428,13 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
     for (int b = 0; b < kProbeBatch; ++b) {
       const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
       valid[b] = i + 1 < L && !head_at(heads, i + 1);
-      k[b] = valid[b] ? (uint64_t)canon2(pack2(x[i], x[i + 1])) : 0;
+      k[b] = valid[b] ? pair_key(x[i], x[i + 1], span) : 0;
     }
     int64_t slot[kProbeBatch];
-    uint32_t v[kProbeBatch];
-    table_find_batch<kProbeBatch>(slots, mask, k, valid, slot, v);
+    table_find_batch<kProbeBatch>(rt, rmask, k, valid, slot);
 #pragma unroll
     for (int b = 0; b < kProbeBatch; ++b) {
-      const uint64_t m = __ballot(v[b] != 0);
+      const uint64_t m = __ballot(slot[b] >= 0);
       if (lane == 0) {
         const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
         wbits[w] = m;
@@ -428,7 +526,7 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    const Slot *__restrict__ slots, uint64_t mask, uint8_t *__restrict__ used,
+    uint64_t span, const Slot *__restrict__ rt, uint64_t rmask, uint8_t *__restrict__ used,
     uint32_t next, int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
     uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
@@ -490,15 +588,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(
         if ((spicks[w] >> lane) & 1) {
           const int64_t k = pack2(v[b], x[i + 1]);
           const int64_t c = canon2(k);
-          kk[b] = (uint64_t)c;
+          kk[b] = compact_key(c, span);
           neg[b] = k != c;
           pk[b] = true;
         }
       }
     }
     int64_t slot[kEb];
-    uint32_t sval[kEb];
-    table_find_batch<kEb>(slots, mask, kk, pk, slot, sval);
+    table_find_batch<kEb>(rt, rmask, kk, pk, slot);
 #pragma unroll
     for (int b = 0; b < kEb; ++b) {
       if (!sv[b])
@@ -522,36 +619,16 @@ __global__ __launch_bounds__(kBlock) void k_emit(
 }
 
 // ---- K4: keys of the used rules (order irrelevant: they are sorted next) -----
-// Each block owns kCollectTile consecutive slots: a block-wide scan of the used
-// flags gives every key its place, and one atomic per block claims the range.
+// Each block owns kCollectTile consecutive rule-table slots: a block-wide scan of
+// the used flags gives every key its place, and one atomic per block claims the
+// range. Keys are already compacted, so their unsigned order is the signed
+// packed-key order the reference sorts by.
 constexpr uint32_t kCollectItems = 16;
 constexpr uint32_t kCollectTile = kBlock * kCollectItems;
 
-// Keys are written compacted when span > 0: every symbol of the round has
-// |v| < next = span / 2, so (a + next) * span + (b >= 0 ? b : b + span) keeps the
-// signed packed-key order (a signed, then b as unsigned 32 bits) in fewer bits
-// and the radix sort needs fewer passes.
-__device__ inline uint64_t compact_key(uint64_t k, uint64_t span) {
-  if (!span)
-    return k;
-  const int64_t a = key_first((int64_t)k), b = key_second((int64_t)k);
-  const int64_t half = (int64_t)(span / 2);
-  return (uint64_t)(a + half) * span + (uint64_t)(b >= 0 ? b : b + (int64_t)span);
-}
-__device__ inline uint64_t expand_key(uint64_t c, uint64_t span) {
-  if (!span)
-    return c;
-  const int64_t half = (int64_t)(span / 2);
-  const int64_t a = (int64_t)(c / span) - half;
-  const int64_t bq = (int64_t)(c % span);
-  const int64_t b = bq < half ? bq : bq - (int64_t)span;
-  return (uint64_t)pack2((int32_t)a, (int32_t)b);
-}
-
-__global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict__ slots,
+__global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict__ rt,
                                                          const uint8_t *__restrict__ used,
-                                                         uint64_t cap, uint64_t span,
-                                                         uint64_t *__restrict__ out,
+                                                         uint64_t cap, uint64_t *__restrict__ out,
                                                          unsigned long long *count) {
   __shared__ uint32_t lds[kBlock / kWave + 1];
   __shared__ unsigned long long sbase;
@@ -577,42 +654,42 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict_
   uint32_t k = 0;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    out[b + k++] = compact_key(slots[s0 + q].key, span);
+    out[b + k++] = rt[s0 + q].key;
     flags &= flags - 1;
   }
 }
 
 // ---- K6: final IDs next + rank, rule columns ----------------------------------
 __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
-                                                   uint64_t span,
-                                                   Slot *__restrict__ slots, uint64_t mask,
-                                                   uint32_t next, int32_t *__restrict__ rf,
+                                                   uint64_t span, Slot *__restrict__ rt,
+                                                   uint64_t rmask, uint32_t next,
+                                                   int32_t *__restrict__ rf,
                                                    int32_t *__restrict__ rs,
                                                    uint32_t *__restrict__ err) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t k = expand_key(sorted[r], span);
-    const int64_t s = table_find(slots, mask, k);
+    const uint64_t ck = sorted[r];
+    const int64_t s = table_find(rt, rmask, ck);
     if (s < 0) {
       atomicExch(err, 2u);
       continue;
     }
-    slots[s].val = next + (uint32_t)r;
-    rf[r] = key_first((int64_t)k);
-    rs[r] = key_second((int64_t)k);
+    rt[s].val = next + (uint32_t)r;
+    const int64_t k = expand_key(ck, span);
+    rf[r] = key_first(k);
+    rs[r] = key_second(k);
   }
 }
 
 // ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
 __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
-                                                  const Slot *__restrict__ slots,
-                                                  uint32_t next) {
+                                                  const Slot *__restrict__ rt, uint32_t next) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock) {
     const int32_t v = y[i];
     const uint32_t a = uabs(v);
     if (a >= next) {
-      const int32_t f = (int32_t)slots[a - next].val;
+      const int32_t f = (int32_t)rt[a - next].val;
       y[i] = v > 0 ? f : -f;
     }
   }
@@ -710,40 +787,41 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     int32_t *dst = (cur_buf == xa.get()) ? xb.get() : xa.get();
     const uint64_t n_tiles = ceil_div(L, kRtTile);
     const uint64_t n_words = ceil_div(L, 64);
+    // Compacted keys need every |symbol| < next < 2^30 (keys below 2^62).
+    if (next >= (1u << 30))
+      throw Error(kErrOverflow, "symbol space too large for 62-bit pair keys (next id " +
+                                    std::to_string(next) + ")");
+    const uint64_t span = 2 * (uint64_t)next;
 
-    // K1 with capacity retry: the table holds at most cap/2 distinct keys.
-    // Distinct pairs grow slowly from round to round (each round's keys are
+    // K1 with capacity retry: the counting table holds at most cap/2 distinct
+    // keys. Distinct pairs grow slowly from round to round (each round's keys are
     // pairs of the previous round's symbols); 4x the last count, at most 2L
     // (there are fewer than L pairs), leaves the table at most half full.
     uint64_t cap = next_pow2(std::max<uint64_t>(
-        uint64_t(1) << 23,
+        uint64_t(1) << 22,
         round == 0 ? L / 8 : std::min<uint64_t>(2 * L, 4 * prev_distinct)));
-    DBuf<Slot> tslots;
-    DBuf<uint32_t> counters(pool, 2);  // distinct, overflow
-    uint32_t distinct = 0;
+    DBuf<unsigned long long> ctab;
+    DBuf<uint32_t> counters(pool, 3);  // distinct, repeated, overflow
+    uint32_t distinct = 0, repeated = 0;
     for (;;) {
-      if ((uint64_t)next + cap >= (uint64_t)INT32_MAX)
-        throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
-                                      ", table " + std::to_string(cap) + ")");
-      tslots = DBuf<Slot>(pool, cap);
-      hipLaunchKernelGGL(k_fill_slots, dim3(stride_grid(ctx, cap)), dim3(kBlock), 0, ctx.stream(),
-                         tslots.get(), cap);
-      GFZ_CHECK_LAUNCH();
-      fill_u32(ctx, counters.get(), 0, 2);
-      DevTable t{tslots.get(), cap - 1, counters.get(), counters.get() + 1,
-                 (uint32_t)std::min<uint64_t>(cap / 2, UINT32_MAX)};
+      ctab = DBuf<unsigned long long>(pool, cap);
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(ctab.get()), kFree, cap);
+      fill_u32(ctx, counters.get(), 0, 3);
+      CountTable t{ctab.get(), cap - 1, counters.get(),
+                   (uint32_t)std::min<uint64_t>(cap / 2, UINT32_MAX)};
       ctx.timed("pair_insert", [&] {
         hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)ceil_div(L, kInsTile)), dim3(kBlock), 0,
-                           ctx.stream(), cur, heads.get(), L, t);
+                           ctx.stream(), cur, heads.get(), L, span, t);
       });
       GFZ_CHECK_LAUNCH();
       ctx.add_bytes("pair_insert", 4.0 * L);
-      uint32_t hc[2];
+      uint32_t hc[3];
       GFZ_HIP(hipMemcpyAsync(hc, counters.get(), sizeof(hc), hipMemcpyDeviceToHost,
                              ctx.stream()));
       ctx.sync();
-      if (!hc[1]) {
+      if (!hc[2]) {
         distinct = hc[0];
+        repeated = hc[1];
         break;
       }
       cap *= 4;
@@ -752,6 +830,26 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     info.distinct_pairs = distinct;
     info.table_slots = cap;
     prev_distinct = distinct;
+    if (repeated == 0)
+      break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
+
+    // K1b: rule table of the repeated keys (load <= 1/2).
+    const uint64_t rcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)repeated));
+    if ((uint64_t)next + rcap >= (uint64_t)INT32_MAX)
+      throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
+                                    ", rule table " + std::to_string(rcap) + ")");
+    DBuf<Slot> rt(pool, rcap);
+    hipLaunchKernelGGL(k_fill_slots, dim3(stride_grid(ctx, rcap)), dim3(kBlock), 0, ctx.stream(),
+                       rt.get(), rcap);
+    GFZ_CHECK_LAUNCH();
+    ctx.timed("extract_repeated", [&] {
+      hipLaunchKernelGGL(k_extract_repeated,
+                         dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
+                         dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt.get(), rcap - 1,
+                         d_err.get());
+    });
+    GFZ_CHECK_LAUNCH();
+    ctab.reset();
 
     // K2 + K2b: candidates, tile carries, next length.
     DBuf<uint64_t> cand(pool, n_words);
@@ -760,7 +858,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint64_t> d_total(pool, 1);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, tslots.get(), cap - 1, cand.get(), n_words,
+                         heads.get(), L, span, rt.get(), rcap - 1, cand.get(), n_words,
                          tile_fn.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -772,16 +870,16 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
     const uint64_t L_next = read_scalar(ctx, d_total.get());
     if (L_next >= L)
-      break;  // no repeated pair: the reference stops here (compression_workflow.cpp:576)
+      throw Error(kErrCorrupt, "repeated pairs found but none picked (internal error)");
 
     // K3: emit.
-    DBuf<uint8_t> used(pool, cap);
-    fill_u8(ctx, used.get(), 0, cap);
+    DBuf<uint8_t> used(pool, rcap);
+    fill_u8(ctx, used.get(), 0, rcap);
     fill_u32(ctx, heads_next.get(), 0, ceil_div(L_next, 32));
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), n_words, tile_info.get(), tslots.get(),
-                         cap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
+                         heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt.get(),
+                         rcap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("emit", 4.0 * L + 4.0 * L_next);
@@ -789,14 +887,13 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     tile_fn.reset();
     tile_info.reset();
 
-    // K4: used keys (compacted for the sort when the span allows it).
-    const uint64_t span = (uint64_t)next < (uint64_t(1) << 30) ? 2 * (uint64_t)next : 0;
-    DBuf<uint64_t> ukeys(pool, distinct ? distinct : 1), utmp(pool, distinct ? distinct : 1);
+    // K4: used keys.
+    DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);
     DBuf<unsigned long long> ucount(pool, 1);
     fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
     ctx.timed("collect_used", [&] {
-      hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(cap, kCollectTile)), dim3(kBlock), 0,
-                         ctx.stream(), tslots.get(), used.get(), cap, span, ukeys.get(),
+      hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(rcap, kCollectTile)),
+                         dim3(kBlock), 0, ctx.stream(), rt.get(), used.get(), rcap, ukeys.get(),
                          ucount.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -806,7 +903,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     if (R == 0)
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
-    // K5: sort by signed key; K6: IDs and rule columns.
+    // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
     sort_i64(ctx, ukeys.get(), utmp.get(), R);
     if (out.n_rules + R > rules_cap) {
       const uint64_t ncap = std::max<uint64_t>(2 * rules_cap, out.n_rules + R);
@@ -817,26 +914,25 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         GFZ_HIP(hipMemcpyAsync(ns.get(), rs.get(), out.n_rules * 4, hipMemcpyDeviceToDevice,
                                ctx.stream()));
       }
-      ctx.sync();
       rf = std::move(nf);
       rs = std::move(ns);
       rules_cap = ncap;
     }
     ctx.timed("assign_ids", [&] {
       hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         ukeys.get(), R, span, tslots.get(), cap - 1, next,
-                         rf.get() + out.n_rules, rs.get() + out.n_rules, d_err.get());
+                         ukeys.get(), R, span, rt.get(), rcap - 1, next, rf.get() + out.n_rules,
+                         rs.get() + out.n_rules, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     // K7: remap provisional IDs.
     ctx.timed("remap", [&] {
       hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
-                         dst, L_next, tslots.get(), next);
+                         dst, L_next, rt.get(), next);
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);
     if (read_scalar(ctx, d_err.get()))
-      throw Error(kErrCorrupt, "sorted key missing from the pair table (internal error)");
+      throw Error(kErrCorrupt, "sorted key missing from the rule table (internal error)");
 
     next += (uint32_t)R;
     out.n_rules += R;
