@@ -300,7 +300,7 @@ int gfz_rulebook_info(gfz_rulebook *rb, uint32_t *max_depth, int *has_bodies,
   if (has_bodies)
     *has_bodies = rb->rb.has_bodies ? 1 : 0;
   if (body_elems)
-    *body_elems = rb->rb.body_elems;
+    *body_elems = rb->rb.z_elems;
   return GFZ_OK;
 }
 

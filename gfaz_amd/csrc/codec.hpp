@@ -52,16 +52,17 @@ struct RuleRec {
 struct Rulebook {
   DBuf<RuleRec> rec;
   DBuf<uint32_t> size;      // expanded size of each rule
+  DBuf<uint32_t> sum;       // sum of each rule's expansion (mod 2^32)
   uint64_t n_rules = 0;
   uint32_t min_id = 0;
   uint32_t depth = 0;       // deepest rule
-  // Pre-expanded bodies: body of rule r at [body_off[r], body_off[r+1]). Built
-  // when the rule depths are non-decreasing in rule ID and the total fits the
-  // budget; otherwise expansion descends the rule tree per output position.
+  // Prefix-summed bodies with a leading zero: rule r at [zoff[r], zoff[r+1]),
+  // size[r] + 1 entries. Built when the rule depths are non-decreasing in rule
+  // ID and the total fits the budget; otherwise expansion descends the tree.
   bool has_bodies = false;
-  DBuf<uint64_t> body_off;
-  DBuf<int32_t> bodies;
-  uint64_t body_elems = 0;
+  DBuf<uint64_t> zoff;
+  DBuf<int32_t> z;
+  uint64_t z_elems = 0;
 };
 
 // first/second: device pointers (raw rule columns, rule-ID order).
@@ -71,6 +72,8 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
 struct DecodePlan {
   uint64_t n_out = 0;                 // total decoded symbols
   std::vector<uint64_t> out_offsets;  // per traversal (n_trav + 1), host
+  std::vector<uint64_t> enc_starts;   // encoded start of each non-empty traversal
+  DBuf<int32_t> sym_sum;              // sum of each encoded symbol's expansion
   bool length_mismatch = false;       // decoded lengths differ from the stored originals
 };
 

@@ -66,11 +66,17 @@ __device__ inline uint64_t pair_key(int32_t a, int32_t b, uint64_t span) {
 
 // Counting table (one 8-byte word per slot): key | kRep once the key was seen
 // twice. Its only job is to find the repeated keys of a round.
+// Inserts give up after kMaxProbe slots and raise `overflow` (the host then
+// retries with a larger table), so no block needs a shared running count and
+// every probe loop is short. Per-block key counts go to `block_counts` with
+// plain stores and are summed afterwards.
+constexpr uint32_t kMaxProbe = 256;
+
 struct CountTable {
   unsigned long long *slots;
   uint64_t mask;
-  uint32_t *counters;  // [0] distinct keys, [1] repeated keys, [2] overflow
-  uint32_t limit;
+  uint32_t *overflow;
+  uint2 *block_counts;  // (new keys, newly repeated keys) per block
 };
 
 // Rule table (16-byte slots): the repeated keys only, dense enough to stay in
@@ -148,7 +154,7 @@ __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&
   for (int j = 0; j < N; ++j)
     s[j] = key_hash(k[j]) & t.mask;
   uint32_t probe = have, cas = 0, mark = 0;
-  for (uint64_t p = 0; probe && p <= t.mask; ++p) {
+  for (uint32_t p = 0; probe && p < kMaxProbe; ++p) {
     uint64_t c[N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
@@ -169,7 +175,7 @@ __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&
       }
     }
   }
-  for (uint64_t p = 0; cas && p <= t.mask; ++p) {
+  for (uint32_t p = 0; cas && p < kMaxProbe; ++p) {
     unsigned long long old[N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
@@ -206,7 +212,7 @@ __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&
         ++reps;
   }
   if (probe | cas)
-    atomicExch(&t.counters[2], 1u);
+    atomicExch(t.overflow, 1u);
 }
 
 __global__ void k_fill_slots(Slot *slots, uint64_t n) {
@@ -270,11 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   __shared__ unsigned long long skeys[kLdsSlots];
   __shared__ uint32_t scnt[kLdsSlots];
   __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
-  __shared__ uint32_t sover;
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  // One coherent read of the shared overflow flag per block (not per thread).
-  if (threadIdx.x == 0)
-    sover = __hip_atomic_load(&t.counters[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
     skeys[s] = kFree;
     scnt[s] = 0;
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   }
   __syncthreads();
   uint32_t fresh = 0, reps = 0;
-  if (!sover) {
+  {
     constexpr int kPer = kLdsSlots / kBlock;
     constexpr int kHalf = kPer / 2;
 #pragma unroll
@@ -352,10 +354,38 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
       n += sfresh[w];
       r += sreps[w];
     }
-    if (n && atomicAdd(&t.counters[0], n) + n > t.limit)
-      atomicExch(&t.counters[2], 1u);
-    if (r)
-      atomicAdd(&t.counters[1], r);
+    t.block_counts[blockIdx.x] = make_uint2(n, r);
+  }
+}
+
+// Sum of the per-block counts: out[0] = distinct keys, out[1] = repeated keys.
+__global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c, uint64_t n,
+                                                     unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long s0[1024 / kWave], s1[1024 / kWave];
+  unsigned long long a = 0, b = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += 1024) {
+    const uint2 v = c[i];
+    a += v.x;
+    b += v.y;
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    a += __shfl_xor(a, d);
+    b += __shfl_xor(b, d);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s0[threadIdx.x / kWave] = a;
+    s1[threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (uint32_t w = 0; w < 1024 / kWave; ++w) {
+      x += s0[w];
+      y += s1[w];
+    }
+    out[0] = x;
+    out[1] = y;
   }
 }
 
@@ -376,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_extract_repeated(
       pending |= 1u << i;
     s[i] = key_hash(k[i]) & rmask;
   }
-  for (uint64_t p = 0; pending && p <= rmask; ++p) {
+  for (uint32_t p = 0; pending && p < kMaxProbe; ++p) {
     unsigned long long old[kExtractItems];
 #pragma unroll
     for (uint32_t i = 0; i < kExtractItems; ++i)
@@ -801,25 +831,29 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         uint64_t(1) << 22,
         round == 0 ? L / 8 : std::min<uint64_t>(2 * L, 4 * prev_distinct)));
     DBuf<unsigned long long> ctab;
-    DBuf<uint32_t> counters(pool, 3);  // distinct, repeated, overflow
-    uint32_t distinct = 0, repeated = 0;
+    const uint64_t n_ins = ceil_div(L, kInsTile);
+    DBuf<uint2> bcounts(pool, n_ins);
+    DBuf<unsigned long long> sums(pool, 3);  // distinct, repeated, overflow flag
+    uint64_t distinct = 0, repeated = 0;
     for (;;) {
       ctab = DBuf<unsigned long long>(pool, cap);
       fill_u64(ctx, reinterpret_cast<uint64_t *>(ctab.get()), kFree, cap);
-      fill_u32(ctx, counters.get(), 0, 3);
-      CountTable t{ctab.get(), cap - 1, counters.get(),
-                   (uint32_t)std::min<uint64_t>(cap / 2, UINT32_MAX)};
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 3);
+      CountTable t{ctab.get(), cap - 1, reinterpret_cast<uint32_t *>(sums.get() + 2),
+                   bcounts.get()};
       ctx.timed("pair_insert", [&] {
-        hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)ceil_div(L, kInsTile)), dim3(kBlock), 0,
-                           ctx.stream(), cur, heads.get(), L, span, t);
+        hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)n_ins), dim3(kBlock), 0, ctx.stream(),
+                           cur, heads.get(), L, span, t);
       });
       GFZ_CHECK_LAUNCH();
       ctx.add_bytes("pair_insert", 4.0 * L);
-      uint32_t hc[3];
-      GFZ_HIP(hipMemcpyAsync(hc, counters.get(), sizeof(hc), hipMemcpyDeviceToHost,
-                             ctx.stream()));
+      hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), n_ins,
+                         sums.get());
+      GFZ_CHECK_LAUNCH();
+      unsigned long long hc[3];
+      GFZ_HIP(hipMemcpyAsync(hc, sums.get(), sizeof(hc), hipMemcpyDeviceToHost, ctx.stream()));
       ctx.sync();
-      if (!hc[2]) {
+      if (!hc[2] && hc[0] <= cap / 2) {
         distinct = hc[0];
         repeated = hc[1];
         break;

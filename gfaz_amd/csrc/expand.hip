@@ -5,12 +5,27 @@
 //   expand(+r) = expand(first[r]) ++ expand(second[r])
 //   expand(-r) = expand(-second[r]) ++ expand(-first[r])
 //   a symbol is a rule iff min_id <= |s| < min_id + R.
-// Instead of recursing per encoded symbol, the rulebook is pre-expanded once
-// (bodies, built level by level on the device) and every output position finds
-// its encoded symbol (a rank over a start bitmap staged in LDS) and reads one
-// body element, so one lane writes one output element and a wave's stores are
-// consecutive. A rulebook whose bodies do not fit the budget is expanded by
-// descending the rule tree by expanded sizes instead.
+//
+// Prefix-summed bodies (Z). For every rule the rulebook keeps, in HBM, the
+// running sums of its expansion with a leading zero: Z_r[0] = 0,
+// Z_r[u + 1] = body_r[0] + ... + body_r[u]. With sum_r = Z_r[size_r]:
+//   prefix of expand(+r) up to q  =          Z_r[q + 1]
+//   prefix of expand(-r) up to q  = -sum_r + Z_r[size_r - 1 - q]
+// so after one inverse delta the value at position q of encoded symbol s is
+//   (sum of the traversal's earlier symbols) + that prefix:
+// one gather per output element and no carry chain between output tiles (the
+// symbol-level sums are scanned over the encoded stream, which is much
+// shorter). Without an inverse delta a body element is a difference of two
+// neighbouring Z entries. Every output position finds its encoded symbol by a
+// rank over a bitmap of symbol starts staged in LDS, so one lane writes one
+// output element and a wave's stores are consecutive.
+//
+// Z is built level by level on the device: a rule's Z is its first child's
+// prefixes followed by its second child's shifted by the first child's sum,
+// which is the same gather over a three-symbol stream per rule (0, first,
+// second). Rulebooks whose depth is not non-decreasing in rule ID, or whose Z
+// exceeds the budget, are expanded by descending the rule tree by expanded
+// sizes, then inverse-delta'd by segmented scans.
 #include "codec.hpp"
 #include "prims.hpp"
 
@@ -32,10 +47,12 @@ __device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
   return a >= min_id && (uint64_t)(a - min_id) < R;
 }
 
+// ---- rule sizes, sums and depths --------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict__ f,
                                                       const int32_t *__restrict__ s, uint64_t R,
                                                       uint32_t min_id, RuleRec *__restrict__ rec,
                                                       uint32_t *__restrict__ size,
+                                                      uint32_t *__restrict__ rsum,
                                                       uint32_t *__restrict__ depth) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
@@ -46,17 +63,29 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
     q.sa = is_rule(a, min_id, R) ? 0u : 1u;
     q.sb = is_rule(b, min_id, R) ? 0u : 1u;
     rec[r] = q;
-    size[r] = (q.sa && q.sb) ? 2u : 0u;
-    depth[r] = (q.sa && q.sb) ? 1u : 0u;
+    const bool leaf = q.sa && q.sb;
+    size[r] = leaf ? 2u : 0u;
+    rsum[r] = leaf ? (uint32_t)a + (uint32_t)b : 0u;
+    depth[r] = leaf ? 1u : 0u;
   }
 }
 
+// Signed sum (mod 2^32) of a child symbol's expansion.
+__device__ inline uint32_t child_sum(int32_t c, bool raw, const uint32_t *rsum, uint32_t min_id) {
+  if (raw)
+    return (uint32_t)c;
+  const uint32_t v = rsum[uabs(c) - min_id];
+  return c > 0 ? v : (uint32_t)0 - v;
+}
+
 // One bottom-up pass. In pass k a rule resolves when each child is a node ID or a
-// rule resolved before this pass (depth <= k + 1); it then gets depth k + 2 and
-// its size. Children resolved during the same pass are not used, so every depth
-// is exact whatever the thread order. `changed` is raised once per wave.
+// rule resolved before this pass (depth <= k + 1); it then gets depth k + 2, its
+// size and its sum. Children resolved during the same pass are not used, so
+// every depth is exact whatever the thread order. `changed` is raised once per
+// wave.
 __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
                                                       uint32_t *__restrict__ size,
+                                                      uint32_t *__restrict__ rsum,
                                                       uint32_t *__restrict__ depth, uint64_t R,
                                                       uint32_t min_id, uint32_t pass,
                                                       uint32_t *changed, uint32_t *err) {
@@ -66,31 +95,31 @@ __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
     bool resolved = false;
     if (r < R && !depth[r]) {
       RuleRec q = rec[r];
+      const bool ra = q.sa != 0, rb = q.sb != 0;  // node-ID children
       bool ok = true;
       uint32_t sa = q.sa, sb = q.sb;
-      if (!sa) {
-        const uint64_t c = uabs(q.a) - min_id;
-        const uint32_t dc = depth[c];
+      if (!ra) {
+        const uint32_t dc = depth[uabs(q.a) - min_id];
         ok = ok && dc != 0 && dc <= pass + 1;
         if (ok)
-          sa = size[c];
+          sa = size[uabs(q.a) - min_id];
       }
-      if (!sb) {
-        const uint64_t c = uabs(q.b) - min_id;
-        const uint32_t dc = depth[c];
+      if (!rb) {
+        const uint32_t dc = depth[uabs(q.b) - min_id];
         ok = ok && dc != 0 && dc <= pass + 1;
         if (ok)
-          sb = size[c];
+          sb = size[uabs(q.b) - min_id];
       }
       if (ok) {
-        const uint32_t sum = sa + sb;
-        if (sum < sa) {
+        const uint32_t total = sa + sb;
+        if (total < sa) {
           atomicExch(err, 1u);
         } else {
+          rsum[r] = child_sum(q.a, ra, rsum, min_id) + child_sum(q.b, rb, rsum, min_id);
           q.sa = sa;
           q.sb = sb;
           rec[r] = q;
-          size[r] = sum;
+          size[r] = total;
           depth[r] = pass + 2;
           resolved = true;
         }
@@ -110,13 +139,25 @@ __global__ __launch_bounds__(kBlock) void k_rule_unresolved(const uint32_t *__re
       atomicExch(err, 2u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sym_sizes(const int32_t *__restrict__ enc, uint64_t n,
-                                                      const uint32_t *__restrict__ size, uint64_t R,
-                                                      uint32_t min_id, uint32_t *__restrict__ out) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock) {
-    const int32_t v = enc[i];
-    out[i] = is_rule(v, min_id, R) ? size[uabs(v) - min_id] : 1u;
+// Level boundaries when depth is non-decreasing in rule ID (true for every
+// rulebook the reference CPU compressor writes: a rule's depth is the round that
+// created it, and rounds hand out increasing IDs). bound[d] = first rule of
+// depth d; *bad = 1 if depth ever decreases.
+__global__ void k_depth_bounds(const uint32_t *__restrict__ depth, uint64_t R, uint32_t max_depth,
+                               uint64_t *__restrict__ bound, uint32_t *bad) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = depth[r];
+    if (r == 0) {
+      for (uint32_t e = 1; e <= d && e <= max_depth; ++e)
+        bound[e] = 0;
+    } else {
+      const uint32_t p = depth[r - 1];
+      if (d < p)
+        atomicExch(bad, 1u);
+      for (uint32_t e = p + 1; e <= d && e <= max_depth; ++e)
+        bound[e] = r;
+    }
   }
 }
 
@@ -127,6 +168,114 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
     dst[i] = src[idx[i]];
 }
 
+__global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] + 1u;
+}
+
+// ---- symbols of an expansion ------------------------------------------------------
+// What one symbol writes at local position q:
+//   flags bit0 clear:     base                   (node ID, or a Z's leading zero)
+//   bit0 set, bit1 clear: base + Z[zidx0 + q]    (rule in forward orientation)
+//   bit0 set, bit1 set:   base + Z[zidx0 - q]    (rule in reverse orientation)
+// With kDiff (no inverse delta wanted) a rule gives Z[j] - Z[j -/+ 1] instead.
+struct alignas(16) SymInfo {
+  uint32_t base;
+  uint32_t flags;
+  uint64_t zidx0;
+};
+
+__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint32_t *rsize,
+                                   const uint32_t *rsum, const uint64_t *zoff, uint32_t min_id) {
+  const uint32_t r = uabs(v) - min_id;
+  SymInfo s;
+  if (v > 0) {
+    s.base = pre;
+    s.flags = 1u;
+    s.zidx0 = zoff[r] + 1;
+  } else {
+    s.base = delta ? pre - rsum[r] : pre;
+    s.flags = 3u;
+    s.zidx0 = zoff[r] + rsize[r] - 1;
+  }
+  return s;
+}
+
+// Per encoded symbol: expanded size, and the signed sum of its expansion.
+__global__ __launch_bounds__(kBlock) void k_sym_plan(const int32_t *__restrict__ enc, uint64_t n,
+                                                     const uint32_t *__restrict__ rsize,
+                                                     const uint32_t *__restrict__ rsum, uint64_t R,
+                                                     uint32_t min_id, uint32_t *__restrict__ sz,
+                                                     int32_t *__restrict__ ssum) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const int32_t v = enc[i];
+    if (is_rule(v, min_id, R)) {
+      const uint32_t r = uabs(v) - min_id;
+      sz[i] = rsize[r];
+      ssum[i] = (int32_t)(v > 0 ? rsum[r] : (uint32_t)0 - rsum[r]);
+    } else {
+      sz[i] = 1u;
+      ssum[i] = v;
+    }
+  }
+}
+
+// Symbol records of the encoded stream. incl = inclusive segmented scan of the
+// symbol sums (traversal starts reset it), so incl - own sum is the sum of the
+// traversal's earlier symbols.
+__global__ __launch_bounds__(kBlock) void k_sym_info(const int32_t *__restrict__ enc, uint64_t n,
+                                                     const int32_t *__restrict__ incl,
+                                                     const int32_t *__restrict__ ssum, bool delta,
+                                                     const uint32_t *__restrict__ rsize,
+                                                     const uint32_t *__restrict__ rsum,
+                                                     const uint64_t *__restrict__ zoff, uint64_t R,
+                                                     uint32_t min_id, SymInfo *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const int32_t v = enc[i];
+    const uint32_t pre = delta ? (uint32_t)incl[i] - (uint32_t)ssum[i] : 0u;
+    SymInfo s;
+    if (is_rule(v, min_id, R)) {
+      s = rule_sym(v, pre, delta, rsize, rsum, zoff, min_id);
+    } else {
+      s.base = pre + (uint32_t)v;
+      s.flags = 0;
+      s.zidx0 = 0;
+    }
+    out[i] = s;
+  }
+}
+
+// Z-building stream of rules [r0, r1): three symbols per rule - the leading
+// zero, the first child, the second child shifted by the first child's sum.
+// off[] gets the Z position of each symbol and, after the last, the level's end.
+__global__ void k_level_stream(const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
+                               const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff,
+                               uint64_t r0, uint64_t r1, uint32_t min_id, uint64_t R,
+                               SymInfo *__restrict__ syms, uint64_t *__restrict__ off) {
+  for (uint64_t r = r0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < r1;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const RuleRec c = rec[r];
+    const uint64_t k = 3 * (r - r0);
+    const uint64_t z0 = zoff[r];
+    const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
+    syms[k] = SymInfo{0u, 0u, 0ull};
+    off[k] = z0;
+    const uint32_t suma = child_sum(c.a, ra, rsum, min_id);
+    syms[k + 1] = ra ? SymInfo{(uint32_t)c.a, 0u, 0ull}
+                     : rule_sym(c.a, 0u, true, rsize, rsum, zoff, min_id);
+    off[k + 1] = z0 + 1;
+    syms[k + 2] = rb ? SymInfo{suma + (uint32_t)c.b, 0u, 0ull}
+                     : rule_sym(c.b, suma, true, rsize, rsum, zoff, min_id);
+    off[k + 2] = z0 + 1 + c.sa;
+    if (r == r1 - 1)
+      off[k + 3] = zoff[r + 1];
+  }
+}
+
+// ---- tiles ------------------------------------------------------------------------
 // Per output tile: the symbols covering its first and last positions and where
 // the first one starts (relative to base). Symbols are disjoint, non-empty
 // intervals, so every field has exactly one writer.
@@ -154,41 +303,28 @@ __global__ void k_tile_desc(const uint64_t *__restrict__ sym_off, uint64_t n_sym
   }
 }
 
-// Output of one block: kExTile consecutive positions [base + o0, base + o_end).
-// The symbols overlapping the tile are staged in LDS with a bitmap of their start
-// positions; a position's symbol is the rank of the set bits at or before it.
-// With kBodies, a rule symbol reads its pre-expanded body (reversed and negated
-// for a reverse-oriented symbol); otherwise it descends the rule tree by sizes.
-template <bool kBodies>
-__global__ __launch_bounds__(kBlock) void k_expand(
-    const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
-    uint64_t base, uint64_t n_out, const TileDesc *__restrict__ tdesc,
-    const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
-    const uint64_t *__restrict__ body_off, const int32_t *__restrict__ bodies, uint64_t R,
-    uint32_t min_id, int32_t *__restrict__ out) {
-  __shared__ uint32_t srel[kExTile];
-  __shared__ int32_t ssym[kExTile];
-  __shared__ unsigned long long sbits[kExTile / 64];
-  __shared__ uint32_t srank[kExTile / 64];
+// Symbols overlapping one tile: start offsets (16-bit, relative to the tile) and
+// a bitmap of starts with per-word ranks.
+struct TileStage {
+  uint16_t rel[kExTile];
+  unsigned long long bits[kExTile / 64];
+  uint32_t rank[kExTile / 64];
+};
+
+__device__ inline void stage_tile(TileStage &st, const uint64_t *__restrict__ sym_off, uint64_t k0,
+                                  uint32_t nsym, uint64_t base, uint64_t o0) {
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
-  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
-  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
   if (tid < kExTile / 64)
-    sbits[tid] = 0;
-  const TileDesc td = tdesc[blockIdx.x];
-  const uint64_t k0 = td.k0;
-  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
-  (void)n_sym;
+    st.bits[tid] = 0;
   __syncthreads();
   for (uint32_t s = tid; s < nsym; s += kBlock) {
     const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - base - o0);
-    srel[s] = rel;
-    ssym[s] = syms[k0 + s];
-    atomicOr(&sbits[rel >> 6], 1ull << (rel & 63));
+    st.rel[s] = (uint16_t)rel;
+    atomicOr(&st.bits[rel >> 6], 1ull << (rel & 63));
   }
   __syncthreads();
   if (tid < kWave) {
-    const uint32_t c = (uint32_t)__popcll(sbits[lane]);
+    const uint32_t c = (uint32_t)__popcll(st.bits[lane]);
     uint32_t x = c;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
@@ -196,90 +332,96 @@ __global__ __launch_bounds__(kBlock) void k_expand(
       if (lane >= (uint32_t)d)
         x += g;
     }
-    srank[lane] = x - c;
+    st.rank[lane] = x - c;
   }
   __syncthreads();
+}
+
+// Local index of the symbol covering tile position p.
+__device__ inline uint32_t tile_symbol(const TileStage &st, uint32_t p) {
+  const uint32_t w = p >> 6, b = p & 63;
+  const unsigned long long upto = b == 63 ? ~0ull : (2ull << b) - 1ull;
+  return st.rank[w] + (uint32_t)__popcll(st.bits[w] & upto) - 1u;
+}
+
+// Expansion through Z (see the file comment). out[base + o] for o in
+// [0, n_out); Z and out may be the same buffer when the ranges are disjoint.
+template <bool kDiff>
+__global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__ syms,
+                                                     const uint64_t *__restrict__ sym_off,
+                                                     uint64_t base, uint64_t n_out,
+                                                     const TileDesc *__restrict__ tdesc,
+                                                     const int32_t *Z, int32_t *out) {
+  __shared__ TileStage st;
+  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
+  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+  const TileDesc td = tdesc[blockIdx.x];
+  const uint64_t k0 = td.k0;
+  stage_tile(st, sym_off, k0, (uint32_t)(td.k_last - k0 + 1), base, o0);
   const uint64_t first_off = td.first_off;
 #pragma unroll 4
   for (uint32_t j = 0; j < kExItems; ++j) {
-    const uint32_t p = j * kBlock + tid;
+    const uint32_t p = j * kBlock + threadIdx.x;
     const uint64_t o = o0 + p;
     if (o >= o_end)
       break;
-    const uint32_t w = p >> 6, b = p & 63;
-    const uint64_t upto = (2ull << b) - 1ull;
-    const uint32_t s = srank[w] + (uint32_t)__popcll(sbits[w] & upto) - 1u;
-    uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - srel[s]);
-    int32_t v = ssym[s];
-    if (is_rule(v, min_id, R)) {
-      if (kBodies) {
-        const uint32_t r = uabs(v) - min_id;
-        const uint64_t src = v > 0 ? body_off[r] + q : body_off[r] + (rsize[r] - 1 - q);
-        const int32_t x = bodies[src];
-        v = v > 0 ? x : -x;
+    const uint32_t s = tile_symbol(st, p);
+    const uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - st.rel[s]);
+    const SymInfo si = syms[k0 + s];
+    uint32_t v = si.base;
+    if (si.flags & 1u) {
+      const bool rev = si.flags & 2u;
+      const uint64_t jz = rev ? si.zidx0 - q : si.zidx0 + q;
+      uint32_t z = (uint32_t)Z[jz];
+      if (kDiff)
+        z -= (uint32_t)Z[rev ? jz + 1 : jz - 1];
+      v += z;
+    }
+    out[base + o] = (int32_t)v;
+  }
+}
+
+// Expansion by descending the rule tree (no Z): node IDs, no inverse delta.
+__global__ __launch_bounds__(kBlock) void k_expand_descent(
+    const int32_t *__restrict__ enc, const uint64_t *__restrict__ sym_off, uint64_t n_out,
+    const TileDesc *__restrict__ tdesc, const RuleRec *__restrict__ rec, uint64_t R,
+    uint32_t min_id, int32_t *__restrict__ out) {
+  __shared__ TileStage st;
+  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
+  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+  const TileDesc td = tdesc[blockIdx.x];
+  const uint64_t k0 = td.k0;
+  stage_tile(st, sym_off, k0, (uint32_t)(td.k_last - k0 + 1), 0, o0);
+  const uint64_t first_off = td.first_off;
+  for (uint32_t j = 0; j < kExItems; ++j) {
+    const uint32_t p = j * kBlock + threadIdx.x;
+    const uint64_t o = o0 + p;
+    if (o >= o_end)
+      break;
+    const uint32_t s = tile_symbol(st, p);
+    uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - st.rel[s]);
+    int32_t v = enc[k0 + s];
+    while (is_rule(v, min_id, R)) {
+      const RuleRec c = rec[uabs(v) - min_id];
+      int32_t left, right;
+      uint32_t ls;
+      if (v > 0) {
+        left = c.a;
+        right = c.b;
+        ls = c.sa;
       } else {
-        do {
-          const RuleRec c = rec[uabs(v) - min_id];
-          int32_t left, right;
-          uint32_t ls;
-          if (v > 0) {
-            left = c.a;
-            right = c.b;
-            ls = c.sa;
-          } else {
-            left = -c.b;
-            right = -c.a;
-            ls = c.sb;
-          }
-          if (q < ls) {
-            v = left;
-          } else {
-            q -= ls;
-            v = right;
-          }
-        } while (is_rule(v, min_id, R));
+        left = -c.b;
+        right = -c.a;
+        ls = c.sb;
+      }
+      if (q < ls) {
+        v = left;
+      } else {
+        q -= ls;
+        v = right;
       }
     }
-    out[base + o] = v;
-  }
-}
-
-// Pair stream of the rulebook: symbols (first[r], second[r]) at body offsets
-// (body_off[r], body_off[r] + |first[r]|), so that expanding it one level over
-// already-built bodies writes the bodies of the next depth.
-__global__ void k_pair_stream(const RuleRec *__restrict__ rec, const uint64_t *__restrict__ body_off,
-                              uint64_t R, int32_t *__restrict__ ps, uint64_t *__restrict__ ps_off) {
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
-       r += (uint64_t)gridDim.x * blockDim.x) {
-    const RuleRec c = rec[r];
-    ps[2 * r] = c.a;
-    ps[2 * r + 1] = c.b;
-    ps_off[2 * r] = body_off[r];
-    ps_off[2 * r + 1] = body_off[r] + c.sa;
-    if (r == R - 1)
-      ps_off[2 * R] = body_off[R];
-  }
-}
-
-// Level boundaries when depth is non-decreasing in rule ID (true for every
-// rulebook the reference CPU compressor writes: a rule's depth is the round that
-// created it, and rounds hand out increasing IDs). bound[d] = first rule of
-// depth d; *bad = 1 if depth ever decreases.
-__global__ void k_depth_bounds(const uint32_t *__restrict__ depth, uint64_t R, uint32_t max_depth,
-                               uint64_t *__restrict__ bound, uint32_t *bad) {
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
-       r += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t d = depth[r];
-    if (r == 0) {
-      for (uint32_t e = 1; e <= d && e <= max_depth; ++e)
-        bound[e] = 0;
-    } else {
-      const uint32_t p = depth[r - 1];
-      if (d < p)
-        atomicExch(bad, 1u);
-      for (uint32_t e = p + 1; e <= d && e <= max_depth; ++e)
-        bound[e] = r;
-    }
+    out[o] = v;
   }
 }
 
@@ -332,216 +474,6 @@ __device__ inline Seg block_seg_exclusive(Seg v, Seg *lds, Seg &total) {
 
 __device__ inline bool bit_at(const uint32_t *bits, uint64_t i) {
   return (bits[i >> 5] >> (i & 31)) & 1u;
-}
-
-// ---- expansion fused with the first inverse delta ----------------------------
-// One pass over the output: each block expands a tile (positions staged as in
-// k_expand), runs the segmented prefix sum over the tile in LDS, and chains the
-// carry across tiles with a decoupled look-back (tiles taken in order from an
-// atomic ticket, so every tile a block waits on belongs to a block already
-// running). A status word per tile packs (state, head flag, 32-bit sum).
-constexpr uint64_t kStNotReady = 0, kStAggregate = 1ull << 62, kStInclusive = 2ull << 62;
-constexpr uint32_t kPad = kExTile + kExTile / 16;  // LDS transpose with a 17-word stride
-
-__device__ inline uint32_t pad_idx(uint32_t p) { return p + (p >> 4); }
-
-__device__ inline uint64_t st_pack(uint64_t state, Seg v) {
-  return state | ((uint64_t)(v.flag & 1) << 32) | v.sum;
-}
-
-template <bool kBodies>
-__global__ __launch_bounds__(kBlock) void k_expand_delta(
-    const int32_t *__restrict__ syms, const uint64_t *__restrict__ sym_off, uint64_t n_sym,
-    uint64_t n_out, const TileDesc *__restrict__ tdesc, const RuleRec *__restrict__ rec,
-    const uint32_t *__restrict__ rsize, const uint64_t *__restrict__ body_off,
-    const int32_t *__restrict__ bodies, uint64_t R, uint32_t min_id,
-    const uint32_t *__restrict__ heads, unsigned long long *status, uint32_t *ticket,
-    uint32_t *err, int32_t *__restrict__ out) {
-  // srel/ssym stage the tile's symbols; once every lane holds its expanded
-  // values in registers the same LDS is reused as the transpose buffer sval.
-  __shared__ uint32_t sbuf[2 * kExTile];
-  static_assert(kPad <= 2 * kExTile, "transpose buffer fits the staging area");
-  uint32_t *const srel = sbuf;
-  int32_t *const ssym = reinterpret_cast<int32_t *>(sbuf + kExTile);
-  uint32_t *const sval = sbuf;
-  __shared__ unsigned long long sbits[kExTile / 64];
-  __shared__ uint32_t srank[kExTile / 64];
-  __shared__ Seg slds[kBlock / kWave + 1];
-  __shared__ uint32_t stile;
-  __shared__ Seg scarry;
-  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
-  if (tid == 0)
-    stile = atomicAdd(ticket, 1u);
-  if (tid < kExTile / 64)
-    sbits[tid] = 0;
-  __syncthreads();
-  const uint32_t tile = stile;
-  const uint64_t o0 = (uint64_t)tile * kExTile;
-  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
-  const TileDesc td = tdesc[tile];
-  const uint64_t k0 = td.k0;
-  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
-  (void)n_sym;
-  for (uint32_t s = tid; s < nsym; s += kBlock) {
-    const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - o0);
-    srel[s] = rel;
-    ssym[s] = syms[k0 + s];
-    atomicOr(&sbits[rel >> 6], 1ull << (rel & 63));
-  }
-  __syncthreads();
-  if (tid < kWave) {
-    const uint32_t c = (uint32_t)__popcll(sbits[lane]);
-    uint32_t x = c;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t g = __shfl_up(x, d);
-      if (lane >= (uint32_t)d)
-        x += g;
-    }
-    srank[lane] = x - c;
-  }
-  __syncthreads();
-  const uint64_t first_off = td.first_off;
-  // Expand in the striped layout (consecutive lanes, consecutive positions).
-  int32_t ev[kExItems];
-#pragma unroll
-  for (uint32_t j = 0; j < kExItems; ++j) {
-    const uint32_t p = j * kBlock + tid;
-    const uint64_t o = o0 + p;
-    int32_t v = 0;
-    if (o < o_end) {
-      const uint32_t w = p >> 6, b = p & 63;
-      const uint64_t upto = (2ull << b) - 1ull;
-      const uint32_t s = srank[w] + (uint32_t)__popcll(sbits[w] & upto) - 1u;
-      uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - srel[s]);
-      v = ssym[s];
-      if (is_rule(v, min_id, R)) {
-        if (kBodies) {
-          const uint32_t r = uabs(v) - min_id;
-          const uint64_t src = v > 0 ? body_off[r] + q : body_off[r] + (rsize[r] - 1 - q);
-          const int32_t x = bodies[src];
-          v = v > 0 ? x : -x;
-        } else {
-          do {
-            const RuleRec c = rec[uabs(v) - min_id];
-            int32_t left, right;
-            uint32_t ls;
-            if (v > 0) {
-              left = c.a;
-              right = c.b;
-              ls = c.sa;
-            } else {
-              left = -c.b;
-              right = -c.a;
-              ls = c.sb;
-            }
-            if (q < ls) {
-              v = left;
-            } else {
-              q -= ls;
-              v = right;
-            }
-          } while (is_rule(v, min_id, R));
-        }
-      }
-    }
-    ev[j] = v;
-  }
-  __syncthreads();
-#pragma unroll
-  for (uint32_t j = 0; j < kExItems; ++j)
-    sval[pad_idx(j * kBlock + tid)] = (uint32_t)ev[j];
-  __syncthreads();
-  // Blocked layout for the scan: thread t owns positions [16t, 16t + 16).
-  const uint32_t pb = tid * kExItems;
-  uint32_t hbits = 0;
-  if (o0 + pb < o_end) {
-    const uint64_t g = o0 + pb;  // 16-aligned, so the 16 bits sit in one word
-    hbits = (heads[g >> 5] >> (g & 31)) & 0xFFFFu;
-  }
-  uint32_t vals[kExItems];
-  Seg a{0, 0};
-#pragma unroll
-  for (uint32_t i = 0; i < kExItems; ++i) {
-    vals[i] = sval[pad_idx(pb + i)];
-    a = seg_then(a, Seg{vals[i], (hbits >> i) & 1u});
-  }
-  Seg total;
-  const Seg pre = block_seg_exclusive<kBlock>(a, slds, total);
-  // Look-back by wave 0 over windows of 64 predecessors: lane l reads the status
-  // of tile (base - l). The nearest predecessor that is inclusive, or whose own
-  // aggregate already contains a traversal start, ends the walk; everything
-  // nearer must have published its aggregate, else the window is re-read.
-  if (tid < kWave) {
-    if (tile == 0) {
-      if (lane == 0) {
-        __hip_atomic_store(&status[0], st_pack(kStInclusive, total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        scarry = Seg{0, 0};
-      }
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&status[tile],
-                           st_pack(total.flag ? kStInclusive : kStAggregate, total),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      Seg carry{0, 0};
-      int64_t wbase = (int64_t)tile - 1;
-      uint32_t spins = 0;
-      for (;;) {
-        const int64_t pt = wbase - (int64_t)lane;
-        uint64_t st = kStInclusive;  // before tile 0: the identity, inclusive
-        if (pt >= 0)
-          st = __hip_atomic_load(&status[pt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t state = st & (3ull << 62);
-        const uint32_t fl = (uint32_t)(st >> 32) & 1u;
-        const bool stop = state == kStInclusive || (state == kStAggregate && fl);
-        const uint64_t stop_mask = __ballot(stop);
-        const uint64_t nr_mask = __ballot(state == kStNotReady);
-        const uint32_t L = stop_mask ? (uint32_t)__builtin_ctzll(stop_mask) : (uint32_t)kWave;
-        const uint64_t need = L >= (uint32_t)kWave ? ~0ull : ((1ull << L) - 1ull);
-        if (nr_mask & need) {
-          if (++spins > (1u << 24)) {
-            if (lane == 0)
-              atomicExch(err, 1u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        const uint32_t top = L >= (uint32_t)kWave ? (uint32_t)kWave - 1 : L;
-        Seg acc{0, 0};
-        for (int l = (int)top; l >= 0; --l) {
-          const uint32_t ps = __shfl((uint32_t)st, l);
-          const uint32_t pf = __shfl(fl, l);
-          acc = seg_then(acc, Seg{ps, pf});
-        }
-        carry = seg_then(acc, carry);
-        if (L < (uint32_t)kWave)
-          break;
-        wbase -= kWave;
-      }
-      if (lane == 0) {
-        if (!total.flag)
-          __hip_atomic_store(&status[tile], st_pack(kStInclusive, seg_then(carry, total)),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        scarry = carry;
-      }
-    }
-  }
-  __syncthreads();
-  Seg run = seg_then(scarry, pre);
-#pragma unroll
-  for (uint32_t i = 0; i < kExItems; ++i) {
-    run = seg_then(run, Seg{vals[i], (hbits >> i) & 1u});
-    sval[pad_idx(pb + i)] = run.sum;
-  }
-  __syncthreads();
-#pragma unroll 4
-  for (uint32_t j = 0; j < kExItems; ++j) {
-    const uint32_t p = j * kBlock + tid;
-    if (o0 + p < o_end)
-      out[o0 + p] = (int32_t)sval[pad_idx(p)];
-  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_reduce(const int32_t *__restrict__ x, uint64_t n,
@@ -623,20 +555,22 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     throw Error(kErrCorrupt, "rule-ID window exceeds the int32 symbol range");
   rb.rec = DBuf<RuleRec>(ctx.pool(), std::max<uint64_t>(R, 1));
   rb.size = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
+  rb.sum = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
   if (!R)
     return;
   DBuf<uint32_t> depth(ctx.pool(), R);
   DBuf<uint32_t> flags(ctx.pool(), 2);  // changed, err
   hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get(), depth.get());
+                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get(), rb.sum.get(),
+                     depth.get());
   GFZ_CHECK_LAUNCH();
   uint32_t max_depth = 1;
   for (uint64_t pass = 0; pass <= R; ++pass) {
     fill_u32(ctx, flags.get(), 0, 2);
     ctx.timed("rule_sizes", [&] {
       hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         rb.rec.get(), rb.size.get(), depth.get(), R, min_id, (uint32_t)pass,
-                         flags.get(), flags.get() + 1);
+                         rb.rec.get(), rb.size.get(), rb.sum.get(), depth.get(), R, min_id,
+                         (uint32_t)pass, flags.get(), flags.get() + 1);
     });
     GFZ_CHECK_LAUNCH();
     uint32_t h[2];
@@ -676,15 +610,22 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     return;  // depth not monotone in rule ID: expansion descends the tree instead
   depth.reset();
 
-  // Bodies, one depth level at a time (children are always shallower).
-  rb.body_off = DBuf<uint64_t>(ctx.pool(), R + 1);
-  exclusive_scan(ctx, rb.size.get(), rb.body_off.get(), R);
+  // Z offsets: size + 1 entries per rule, levels contiguous.
+  rb.zoff = DBuf<uint64_t>(ctx.pool(), R + 1);
+  {
+    DBuf<uint32_t> szp1(ctx.pool(), R);
+    hipLaunchKernelGGL(k_plus_one, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                       rb.size.get(), R, szp1.get());
+    GFZ_CHECK_LAUNCH();
+    exclusive_scan(ctx, szp1.get(), rb.zoff.get(), R);
+    ctx.sync();
+  }
   std::vector<uint64_t> lvl_off(max_depth + 2);
   {
     DBuf<uint64_t> d_lvl(ctx.pool(), max_depth + 2), d_idx(ctx.pool(), max_depth + 2);
     GFZ_HIP(hipMemcpyAsync(d_idx.get(), bound.data(), bound.size() * 8, hipMemcpyHostToDevice,
                            ctx.stream()));
-    hipLaunchKernelGGL(k_gather_u64, dim3(1), dim3(kBlock), 0, ctx.stream(), rb.body_off.get(),
+    hipLaunchKernelGGL(k_gather_u64, dim3(1), dim3(kBlock), 0, ctx.stream(), rb.zoff.get(),
                        d_idx.get(), max_depth + 2, d_lvl.get());
     GFZ_CHECK_LAUNCH();
     GFZ_HIP(hipMemcpyAsync(lvl_off.data(), d_lvl.get(), lvl_off.size() * 8, hipMemcpyDeviceToHost,
@@ -695,31 +636,36 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   uint64_t budget = uint64_t(1) << 30;  // elements (4 GiB)
   if (const char *env = std::getenv("GFAZ_BODY_BUDGET_MB"))
     budget = std::strtoull(env, nullptr, 10) * (uint64_t(1) << 20) / 4;
-  if (total > budget)
+  if (total > budget) {
+    rb.zoff.reset();
     return;
-  rb.bodies = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(total, 1));
-  rb.body_elems = total;
-  DBuf<int32_t> ps(ctx.pool(), 2 * R);
-  DBuf<uint64_t> ps_off(ctx.pool(), 2 * R + 1);
-  hipLaunchKernelGGL(k_pair_stream, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     rb.rec.get(), rb.body_off.get(), R, ps.get(), ps_off.get());
-  GFZ_CHECK_LAUNCH();
+  }
+  rb.z = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(total, 1));
+  rb.z_elems = total;
+  // One level at a time: children are always at shallower, already-built levels.
   for (uint32_t d = 1; d <= max_depth; ++d) {
     const uint64_t r0 = bound[d], r1 = bound[d + 1];
     if (r0 >= r1)
       continue;
     const uint64_t lo = lvl_off[d], hi = lvl_off[d + 1];
+    const uint64_t n_sym = 3 * (r1 - r0);
     const uint64_t n_tiles = ceil_div(hi - lo, kExTile);
-    DBuf<TileDesc> tf(ctx.pool(), n_tiles);
+    if (n_tiles > UINT32_MAX)
+      throw Error(kErrInvalid, "rule level too large for one launch");
+    DBuf<SymInfo> syms(ctx.pool(), n_sym);
+    DBuf<uint64_t> off(ctx.pool(), n_sym + 1);
+    DBuf<TileDesc> tdesc(ctx.pool(), n_tiles);
     ctx.timed("build_bodies", [&] {
-      hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, 2 * (r1 - r0))), dim3(kBlock), 0,
-                         ctx.stream(), ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, hi - lo, n_tiles,
-                         tf.get());
+      hipLaunchKernelGGL(k_level_stream, dim3(stride_grid(ctx, r1 - r0)), dim3(kBlock), 0,
+                         ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), rb.zoff.get(),
+                         r0, r1, min_id, R, syms.get(), off.get());
       GFZ_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
-                         ps.get() + 2 * r0, ps_off.get() + 2 * r0, 2 * (r1 - r0), lo, hi - lo,
-                         tf.get(), rb.rec.get(), rb.size.get(), rb.body_off.get(), rb.bodies.get(),
-                         R, min_id, rb.bodies.get());
+      hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_sym)), dim3(kBlock), 0,
+                         ctx.stream(), off.get(), n_sym, lo, hi - lo, n_tiles, tdesc.get());
+      GFZ_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_expand_z<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                         ctx.stream(), syms.get(), off.get(), lo, hi - lo, tdesc.get(),
+                         rb.z.get(), rb.z.get());
       GFZ_CHECK_LAUNCH();
     });
   }
@@ -742,22 +688,24 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
                                  std::to_string(total_enc) + ", block holds " +
                                  std::to_string(n_enc) + ")");
   sym_off = DBuf<uint64_t>(ctx.pool(), n_enc + 1);
+  plan.sym_sum = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
   {
     DBuf<uint32_t> sz(ctx.pool(), std::max<uint64_t>(n_enc, 1));
     if (n_enc) {
-      hipLaunchKernelGGL(k_sym_sizes, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0,
-                         ctx.stream(), d_enc, n_enc, rb.size.get(), rb.n_rules, rb.min_id,
-                         sz.get());
+      hipLaunchKernelGGL(k_sym_plan, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0,
+                         ctx.stream(), d_enc, n_enc, rb.size.get(), rb.sum.get(), rb.n_rules,
+                         rb.min_id, sz.get(), plan.sym_sum.get());
       GFZ_CHECK_LAUNCH();
     }
     exclusive_scan(ctx, sz.get(), sym_off.get(), n_enc);
+    ctx.sync();
   }
   // Decoded offset of each traversal = symbol offset at its first encoded symbol.
   plan.out_offsets.assign(n_trav + 1, 0);
-  if (n_trav) {
-    DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
+  {
     std::vector<uint64_t> h_idx(enc_start);
     h_idx.push_back(n_enc);
+    DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
     GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx.data(), (n_trav + 1) * 8, hipMemcpyHostToDevice,
                            ctx.stream()));
     hipLaunchKernelGGL(k_gather_u64, dim3(stride_grid(ctx, n_trav + 1)), dim3(kBlock), 0,
@@ -766,12 +714,11 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
     GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
                            hipMemcpyDeviceToHost, ctx.stream()));
     ctx.sync();
-  } else {
-    GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), sym_off.get() + n_enc, 8,
-                           hipMemcpyDeviceToHost, ctx.stream()));
-    ctx.sync();
   }
   plan.n_out = plan.out_offsets[n_trav];
+  for (uint64_t t = 0; t < n_trav; ++t)
+    if (enc_lengths[t])
+      plan.enc_starts.push_back(enc_start[t]);
   if (orig_lengths)
     for (uint64_t t = 0; t < n_trav; ++t)
       if (plan.out_offsets[t + 1] - plan.out_offsets[t] != orig_lengths[t])
@@ -787,68 +734,78 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const uint64_t n_tiles = ceil_div(n_out, kExTile);
   if (n_tiles > UINT32_MAX)
     throw Error(kErrInvalid, "decoded batch too large for one launch");
-  DBuf<TileDesc> tf(ctx.pool(), n_tiles);
+  DBuf<TileDesc> tdesc(ctx.pool(), n_tiles);
   hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                     d_sym_off, n_enc, 0, n_out, n_tiles, tf.get());
+                     d_sym_off, n_enc, 0, n_out, n_tiles, tdesc.get());
   GFZ_CHECK_LAUNCH();
-  const uint64_t *bo = rb.has_bodies ? rb.body_off.get() : nullptr;
-  const int32_t *bd = rb.has_bodies ? rb.bodies.get() : nullptr;
-  if (delta_round <= 0) {
+  const bool delta = delta_round > 0;
+  int scans_left = delta_round;
+  if (rb.has_bodies) {
+    // Symbol records; with an inverse delta to undo, the first one is folded in
+    // through the traversal-level sums of the encoded symbols.
+    DBuf<SymInfo> syms(ctx.pool(), n_enc);
+    DBuf<int32_t> incl(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    if (delta) {
+      const uint64_t w = ceil_div(n_enc, 32);
+      DBuf<uint32_t> eheads(ctx.pool(), w);
+      fill_u32(ctx, eheads.get(), 0, w);
+      DBuf<uint64_t> d_st(ctx.pool(), std::max<size_t>(plan.enc_starts.size(), 1));
+      if (!plan.enc_starts.empty()) {
+        GFZ_HIP(hipMemcpyAsync(d_st.get(), plan.enc_starts.data(), plan.enc_starts.size() * 8,
+                               hipMemcpyHostToDevice, ctx.stream()));
+        set_head_bits(ctx, d_st.get(), plan.enc_starts.size(), eheads.get(), n_enc);
+      }
+      GFZ_HIP(hipMemcpyAsync(incl.get(), plan.sym_sum.get(), n_enc * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+      segmented_prefix_sum(ctx, incl.get(), n_enc, eheads.get());
+      scans_left -= 1;
+    }
+    hipLaunchKernelGGL(k_sym_info, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                       d_enc, n_enc, incl.get(), plan.sym_sum.get(), delta, rb.size.get(),
+                       rb.sum.get(), rb.zoff.get(), rb.n_rules, rb.min_id, syms.get());
+    GFZ_CHECK_LAUNCH();
     ctx.timed("expand", [&] {
-      if (rb.has_bodies)
-        hipLaunchKernelGGL(k_expand<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                           ctx.stream(), d_enc, d_sym_off, n_enc, 0, n_out, tf.get(),
-                           rb.rec.get(), rb.size.get(), bo, bd, rb.n_rules, rb.min_id, d_out);
+      if (delta)
+        hipLaunchKernelGGL(k_expand_z<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                           ctx.stream(), syms.get(), d_sym_off, 0, n_out, tdesc.get(),
+                           rb.z.get(), d_out);
       else
-        hipLaunchKernelGGL(k_expand<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                           ctx.stream(), d_enc, d_sym_off, n_enc, 0, n_out, tf.get(),
-                           rb.rec.get(), rb.size.get(), bo, bd, rb.n_rules, rb.min_id, d_out);
+        hipLaunchKernelGGL(k_expand_z<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                           ctx.stream(), syms.get(), d_sym_off, 0, n_out, tdesc.get(),
+                           rb.z.get(), d_out);
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
+  } else {
+    ctx.timed("expand", [&] {
+      hipLaunchKernelGGL(k_expand_descent, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, d_sym_off, n_out, tdesc.get(), rb.rec.get(), rb.n_rules,
+                         rb.min_id, d_out);
+    });
+    GFZ_CHECK_LAUNCH();
+    ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
+  }
+  if (scans_left <= 0) {
+    ctx.sync();
     return;
   }
-  // Heads at the decoded start of every non-empty traversal.
+  // Remaining inverse deltas: segmented scans over the decoded stream.
   std::vector<uint64_t> starts;
   for (uint64_t t = 0; t < n_trav; ++t)
-    if (enc_lengths[t] && plan.out_offsets[t + 1] > plan.out_offsets[t])
+    if (plan.out_offsets[t + 1] > plan.out_offsets[t])
       starts.push_back(plan.out_offsets[t]);
   const uint64_t words = ceil_div(n_out, 32);
   DBuf<uint32_t> heads(ctx.pool(), words);
   fill_u32(ctx, heads.get(), 0, words);
-  DBuf<uint64_t> d_starts;
+  DBuf<uint64_t> d_starts(ctx.pool(), std::max<size_t>(starts.size(), 1));
   if (!starts.empty()) {
-    d_starts = DBuf<uint64_t>(ctx.pool(), starts.size());
     GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
                            hipMemcpyHostToDevice, ctx.stream()));
     set_head_bits(ctx, d_starts.get(), starts.size(), heads.get(), n_out);
   }
-  // First inverse delta fused into the expansion; any further rounds separately.
-  DBuf<unsigned long long> status(ctx.pool(), n_tiles);
-  DBuf<uint32_t> ctl(ctx.pool(), 2);  // ticket, err
-  fill_u64(ctx, reinterpret_cast<uint64_t *>(status.get()), 0, n_tiles);
-  fill_u32(ctx, ctl.get(), 0, 2);
-  ctx.timed("expand", [&] {
-    if (rb.has_bodies)
-      hipLaunchKernelGGL(k_expand_delta<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                         ctx.stream(), d_enc, d_sym_off, n_enc, n_out, tf.get(), rb.rec.get(),
-                         rb.size.get(), bo, bd, rb.n_rules, rb.min_id, heads.get(), status.get(),
-                         ctl.get(), ctl.get() + 1, d_out);
-    else
-      hipLaunchKernelGGL(k_expand_delta<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                         ctx.stream(), d_enc, d_sym_off, n_enc, n_out, tf.get(), rb.rec.get(),
-                         rb.size.get(), bo, bd, rb.n_rules, rb.min_id, heads.get(), status.get(),
-                         ctl.get(), ctl.get() + 1, d_out);
-  });
-  GFZ_CHECK_LAUNCH();
-  ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
-  for (int r = 1; r < delta_round; ++r)
+  for (int r = 0; r < scans_left; ++r)
     segmented_prefix_sum(ctx, d_out, n_out, heads.get());
-  uint32_t e = 0;
-  GFZ_HIP(hipMemcpyAsync(&e, ctl.get() + 1, 4, hipMemcpyDeviceToHost, ctx.stream()));
   ctx.sync();
-  if (e)
-    throw Error(kErrHip, "expansion look-back did not complete (internal error)");
 }
 
 void segmented_prefix_sum(Ctx &ctx, int32_t *x, uint64_t n, const uint32_t *heads) {
