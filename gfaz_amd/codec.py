@@ -65,6 +65,9 @@ _SIGNATURES = {
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                             ctypes.c_int),
     "gfz_ctx_num_cus": ([_vp], ctypes.c_int),
+    "gfz_ctx_set_autotune": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gfz_ctx_tuned": ([_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "gfz_ctx_set_tuned": ([_vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "gfz_dev_alloc": ([_vp, _u64, ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_dev_free": ([_vp, _vp], ctypes.c_int),
     "gfz_memcpy_h2d": ([_vp, _vp, _vp, _u64], ctypes.c_int),
@@ -186,6 +189,17 @@ class Context:
 
     def num_cus(self) -> int:
         return int(lib().gfz_ctx_num_cus(self.handle))
+
+    def set_autotune(self, on: bool) -> None:
+        _check(lib().gfz_ctx_set_autotune(self.handle, int(bool(on))))
+
+    def tuned(self, key: str) -> int:
+        v = ctypes.c_int()
+        _check(lib().gfz_ctx_tuned(self.handle, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_tuned(self, key: str, value: int) -> None:
+        _check(lib().gfz_ctx_set_tuned(self.handle, key.encode(), int(value)))
 
     def calibration_copy(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int) -> float:
         """Plain device copy of nbytes; returns its duration in ms."""

@@ -129,6 +129,27 @@ int gfz_ctx_kernel_stat(gfz_ctx *ctx, const char *name, uint64_t *launches, doub
 
 int gfz_ctx_num_cus(gfz_ctx *ctx) { return ctx ? ctx->ctx->num_cus() : 0; }
 
+int gfz_ctx_set_autotune(gfz_ctx *ctx, int on) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  ctx->ctx->set_autotune(on != 0);
+  return GFZ_OK;
+}
+
+int gfz_ctx_tuned(gfz_ctx *ctx, const char *key, int *value) {
+  if (!ctx || !key || !value)
+    return fail(GFZ_E_INVALID, "null argument");
+  *value = ctx->ctx->tuned(key);
+  return GFZ_OK;
+}
+
+int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value) {
+  if (!ctx || !key)
+    return fail(GFZ_E_INVALID, "null argument");
+  ctx->ctx->set_tuned(key, value);
+  return GFZ_OK;
+}
+
 int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out) {
   if (!ctx || !out)
     return fail(GFZ_E_INVALID, "null argument");

@@ -61,8 +61,9 @@ struct Rulebook {
   // ID and the total fits the budget; otherwise expansion descends the tree.
   bool has_bodies = false;
   DBuf<uint64_t> zoff;
-  DBuf<int32_t> z;
+  DBuf<int32_t> z;          // + two trailing zeros that node-ID symbols point at
   uint64_t z_elems = 0;
+  uint32_t z_raw = 0;       // index of the second trailing zero
 };
 
 // first/second: device pointers (raw rule columns, rule-ID order).

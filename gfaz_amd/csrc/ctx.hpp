@@ -121,6 +121,33 @@ class Ctx {
   // Pinned host staging buffer of at least `bytes`, reused across calls.
   void *pinned(size_t bytes);
 
+  // Autotuned launch choices (tile sizes and the like), keyed by kernel and
+  // size class. When autotuning is on, the first call that needs an untuned
+  // key times every candidate on its own input and keeps the fastest; all
+  // candidates compute the same result.
+  void set_autotune(bool on) { autotune_ = on; }
+  bool autotune() const { return autotune_; }
+  int tuned(const std::string &key) const {
+    auto it = tuned_.find(key);
+    return it == tuned_.end() ? -1 : it->second;
+  }
+  void set_tuned(const std::string &key, int v) { tuned_[key] = v; }
+  const std::map<std::string, int> &tuned_all() const { return tuned_; }
+  // Elapsed milliseconds of `launch` on the context stream (synchronises).
+  template <class F>
+  float time_ms(F &&launch) {
+    hipEvent_t a = event(), b = event();
+    GFZ_HIP(hipEventRecord(a, stream_));
+    launch();
+    GFZ_HIP(hipEventRecord(b, stream_));
+    GFZ_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    GFZ_HIP(hipEventElapsedTime(&ms, a, b));
+    free_events_.push_back(a);
+    free_events_.push_back(b);
+    return ms;
+  }
+
  private:
   hipEvent_t event();
   struct Pending {
@@ -135,6 +162,8 @@ class Ctx {
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> free_events_;
   std::map<std::string, KernelStat> stats_;
+  bool autotune_ = true;
+  std::map<std::string, int> tuned_;
   void *pinned_ = nullptr;
   size_t pinned_bytes_ = 0;
 };

@@ -36,8 +36,8 @@ namespace gfz {
 
 namespace {
 
-constexpr uint32_t kExItems = 16;
-constexpr uint32_t kExTile = kBlock * kExItems;  // output elements per block
+constexpr uint32_t kExItems = 16;                 // output elements per thread
+constexpr uint32_t kExTile = kBlock * kExItems;   // output elements per block
 constexpr uint32_t kSsItems = 16;
 constexpr uint32_t kSsTile = kBlock * kSsItems;
 constexpr uint32_t kSsScanThreads = 1024;
@@ -175,16 +175,17 @@ __global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t
 }
 
 // ---- symbols of an expansion ------------------------------------------------------
-// What one symbol writes at local position q:
-//   flags bit0 clear:     base                   (node ID, or a Z's leading zero)
-//   bit0 set, bit1 clear: base + Z[zidx0 + q]    (rule in forward orientation)
-//   bit0 set, bit1 set:   base + Z[zidx0 - q]    (rule in reverse orientation)
-// With kDiff (no inverse delta wanted) a rule gives Z[j] - Z[j -/+ 1] instead.
-struct alignas(16) SymInfo {
+// What one symbol writes at its local position q: base + Z[zidx0 + q], or
+// base + Z[zidx0 - q] when bit 31 of zz (reverse orientation) is set, with
+// zidx0 = zz & 0x7fffffff. A node ID (or the leading zero of a Z being built)
+// points at the two reserved zero entries at the end of Z, so every symbol
+// takes the same path. With kDiff (no inverse delta wanted) the value is
+// Z[j] - Z[j -/+ 1] instead, a body element.
+struct alignas(8) SymInfo {
   uint32_t base;
-  uint32_t flags;
-  uint64_t zidx0;
+  uint32_t zz;
 };
+constexpr uint32_t kRevBit = 0x80000000u;
 
 __device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint32_t *rsize,
                                    const uint32_t *rsum, const uint64_t *zoff, uint32_t min_id) {
@@ -192,12 +193,10 @@ __device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const ui
   SymInfo s;
   if (v > 0) {
     s.base = pre;
-    s.flags = 1u;
-    s.zidx0 = zoff[r] + 1;
+    s.zz = (uint32_t)(zoff[r] + 1);
   } else {
     s.base = delta ? pre - rsum[r] : pre;
-    s.flags = 3u;
-    s.zidx0 = zoff[r] + rsize[r] - 1;
+    s.zz = (uint32_t)(zoff[r] + rsize[r] - 1) | kRevBit;
   }
   return s;
 }
@@ -231,7 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_info(const int32_t *__restrict__
                                                      const uint32_t *__restrict__ rsize,
                                                      const uint32_t *__restrict__ rsum,
                                                      const uint64_t *__restrict__ zoff, uint64_t R,
-                                                     uint32_t min_id, SymInfo *__restrict__ out) {
+                                                     uint32_t min_id, uint32_t zraw,
+                                                     SymInfo *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock) {
     const int32_t v = enc[i];
@@ -241,8 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_info(const int32_t *__restrict__
       s = rule_sym(v, pre, delta, rsize, rsum, zoff, min_id);
     } else {
       s.base = pre + (uint32_t)v;
-      s.flags = 0;
-      s.zidx0 = 0;
+      s.zz = zraw;
     }
     out[i] = s;
   }
@@ -254,20 +253,21 @@ __global__ __launch_bounds__(kBlock) void k_sym_info(const int32_t *__restrict__
 __global__ void k_level_stream(const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
                                const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff,
                                uint64_t r0, uint64_t r1, uint32_t min_id, uint64_t R,
-                               SymInfo *__restrict__ syms, uint64_t *__restrict__ off) {
+                               uint32_t zraw, SymInfo *__restrict__ syms,
+                               uint64_t *__restrict__ off) {
   for (uint64_t r = r0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < r1;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const RuleRec c = rec[r];
     const uint64_t k = 3 * (r - r0);
     const uint64_t z0 = zoff[r];
     const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
-    syms[k] = SymInfo{0u, 0u, 0ull};
+    syms[k] = SymInfo{0u, zraw};
     off[k] = z0;
     const uint32_t suma = child_sum(c.a, ra, rsum, min_id);
-    syms[k + 1] = ra ? SymInfo{(uint32_t)c.a, 0u, 0ull}
+    syms[k + 1] = ra ? SymInfo{(uint32_t)c.a, zraw}
                      : rule_sym(c.a, 0u, true, rsize, rsum, zoff, min_id);
     off[k + 1] = z0 + 1;
-    syms[k + 2] = rb ? SymInfo{suma + (uint32_t)c.b, 0u, 0ull}
+    syms[k + 2] = rb ? SymInfo{suma + (uint32_t)c.b, zraw}
                      : rule_sym(c.b, suma, true, rsize, rsum, zoff, min_id);
     off[k + 2] = z0 + 1 + c.sa;
     if (r == r1 - 1)
@@ -287,44 +287,42 @@ struct TileDesc {
 };
 
 __global__ void k_tile_desc(const uint64_t *__restrict__ sym_off, uint64_t n_sym, uint64_t base,
-                            uint64_t n_out, uint64_t n_tiles, TileDesc *__restrict__ desc) {
+                            uint64_t n_out, uint32_t tile, uint64_t n_tiles,
+                            TileDesc *__restrict__ desc) {
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_sym;
        k += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t lo = sym_off[k] - base, hi = sym_off[k + 1] - base;
-    for (uint64_t t = lo / kExTile; t < n_tiles && t * kExTile < hi; ++t) {
-      if (t * kExTile >= lo) {
+    for (uint64_t t = lo / tile; t < n_tiles && t * tile < hi; ++t) {
+      if (t * tile >= lo) {
         desc[t].k0 = k;
         desc[t].first_off = lo;
       }
-      const uint64_t end = ((t + 1) * kExTile < n_out ? (t + 1) * kExTile : n_out) - 1;
+      const uint64_t end = ((t + 1) * tile < n_out ? (t + 1) * tile : n_out) - 1;
       if (end >= lo && end < hi)
         desc[t].k_last = k;
     }
   }
 }
 
-// Symbols overlapping one tile: start offsets (16-bit, relative to the tile) and
-// a bitmap of starts with per-word ranks.
-struct TileStage {
-  uint16_t rel[kExTile];
-  unsigned long long bits[kExTile / 64];
-  uint32_t rank[kExTile / 64];
-};
+// Bitmap of symbol starts in a tile, with per-64-bit-word ranks, so the symbol
+// covering a position is rank + popcount.
+template <uint32_t kTile>
+struct StartMap {
+  unsigned long long bits[kTile / 64];
+  uint32_t rank[kTile / 64];
 
-__device__ inline void stage_tile(TileStage &st, const uint64_t *__restrict__ sym_off, uint64_t k0,
-                                  uint32_t nsym, uint64_t base, uint64_t o0) {
-  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1);
-  if (tid < kExTile / 64)
-    st.bits[tid] = 0;
-  __syncthreads();
-  for (uint32_t s = tid; s < nsym; s += kBlock) {
-    const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - base - o0);
-    st.rel[s] = (uint16_t)rel;
-    atomicOr(&st.bits[rel >> 6], 1ull << (rel & 63));
+  __device__ void clear() {
+    for (uint32_t w = threadIdx.x; w < kTile / 64; w += kBlock)
+      bits[w] = 0;
   }
-  __syncthreads();
-  if (tid < kWave) {
-    const uint32_t c = (uint32_t)__popcll(st.bits[lane]);
+  __device__ void mark(uint32_t rel) { atomicOr(&bits[rel >> 6], 1ull << (rel & 63)); }
+  // Call after a barrier that follows the last mark(); ends with a barrier.
+  __device__ void rank_words() {
+    static_assert(kTile / 64 <= kBlock, "one thread per bitmap word");
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    constexpr uint32_t kWords = kTile / 64;
+    __shared__ uint32_t wsum[kBlock / kWave];
+    uint32_t c = threadIdx.x < kWords ? (uint32_t)__popcll(bits[threadIdx.x]) : 0u;
     uint32_t x = c;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
@@ -332,74 +330,124 @@ __device__ inline void stage_tile(TileStage &st, const uint64_t *__restrict__ sy
       if (lane >= (uint32_t)d)
         x += g;
     }
-    st.rank[lane] = x - c;
+    if (lane == kWave - 1)
+      wsum[wid] = x;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < wid; ++w)
+      carry += wsum[w];
+    if (threadIdx.x < kWords)
+      rank[threadIdx.x] = carry + x - c;
+    __syncthreads();
   }
-  __syncthreads();
-}
+  __device__ uint32_t symbol(uint32_t p) const {
+    const uint32_t w = p >> 6, b = p & 63;
+    const unsigned long long upto = b == 63 ? ~0ull : (2ull << b) - 1ull;
+    return rank[w] + (uint32_t)__popcll(bits[w] & upto) - 1u;
+  }
+};
 
-// Local index of the symbol covering tile position p.
-__device__ inline uint32_t tile_symbol(const TileStage &st, uint32_t p) {
-  const uint32_t w = p >> 6, b = p & 63;
-  const unsigned long long upto = b == 63 ? ~0ull : (2ull << b) - 1ull;
-  return st.rank[w] + (uint32_t)__popcll(st.bits[w] & upto) - 1u;
-}
-
-// Expansion through Z (see the file comment). out[base + o] for o in
-// [0, n_out); Z and out may be the same buffer when the ranges are disjoint.
-template <bool kDiff>
+// Expansion through Z (see the file comment): out[base + o] for o in [0, n_out).
+// Each tile stages, per overlapping symbol, its base and the Z index of tile
+// position 0 (zstart, so position p reads Z[zstart +/- p]); then every thread
+// issues all of its Z gathers before its stores. Z and out may be one buffer
+// when the ranges read and written are disjoint.
+template <uint32_t kItems, bool kDiff>
 __global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__ syms,
                                                      const uint64_t *__restrict__ sym_off,
                                                      uint64_t base, uint64_t n_out,
                                                      const TileDesc *__restrict__ tdesc,
-                                                     const int32_t *Z, int32_t *out) {
-  __shared__ TileStage st;
-  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
-  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+                                                     const int32_t *Z, uint32_t zraw,
+                                                     int32_t *out) {
+  constexpr uint32_t kTile = kBlock * kItems;
+  __shared__ StartMap<kTile> map;
+  __shared__ uint32_t sbase[kTile];
+  __shared__ uint32_t sstart[kTile];
+  __shared__ uint32_t srev[kTile / 32];
+  const uint64_t o0 = (uint64_t)blockIdx.x * kTile;
+  const uint32_t n_here = (uint32_t)(o0 + kTile < n_out ? kTile : n_out - o0);
   const TileDesc td = tdesc[blockIdx.x];
   const uint64_t k0 = td.k0;
-  stage_tile(st, sym_off, k0, (uint32_t)(td.k_last - k0 + 1), base, o0);
-  const uint64_t first_off = td.first_off;
-#pragma unroll 4
-  for (uint32_t j = 0; j < kExItems; ++j) {
-    const uint32_t p = j * kBlock + threadIdx.x;
-    const uint64_t o = o0 + p;
-    if (o >= o_end)
-      break;
-    const uint32_t s = tile_symbol(st, p);
-    const uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - st.rel[s]);
+  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
+  map.clear();
+  for (uint32_t w = threadIdx.x; w < kTile / 32; w += kBlock)
+    srev[w] = 0;
+  __syncthreads();
+  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
+    // Start of symbol s relative to the tile (<= 0 for the first one).
+    const uint32_t rel = s == 0 ? (uint32_t)(td.first_off - o0)
+                                : (uint32_t)(sym_off[k0 + s] - base - o0);
     const SymInfo si = syms[k0 + s];
-    uint32_t v = si.base;
-    if (si.flags & 1u) {
-      const bool rev = si.flags & 2u;
-      const uint64_t jz = rev ? si.zidx0 - q : si.zidx0 + q;
-      uint32_t z = (uint32_t)Z[jz];
-      if (kDiff)
-        z -= (uint32_t)Z[rev ? jz + 1 : jz - 1];
-      v += z;
-    }
-    out[base + o] = (int32_t)v;
+    const bool rev = si.zz & kRevBit;
+    const uint32_t zi = si.zz & ~kRevBit;
+    sbase[s] = si.base;
+    sstart[s] = rev ? zi + rel : zi - rel;
+    if (rev)
+      atomicOr(&srev[s >> 5], 1u << (s & 31));
+    map.mark(s == 0 ? 0u : rel);
+  }
+  __syncthreads();
+  map.rank_words();
+  uint32_t jz[kItems], vb[kItems];
+  uint32_t revm = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kItems; ++j) {
+    const uint32_t p = j * kBlock + threadIdx.x;
+    const uint32_t s = map.symbol(p < n_here ? p : n_here - 1);
+    const uint32_t rv = (srev[s >> 5] >> (s & 31)) & 1u;
+    revm |= rv << j;
+    jz[j] = p < n_here ? (rv ? sstart[s] - p : sstart[s] + p) : zraw;
+    vb[j] = sbase[s];
+  }
+  uint32_t z[kItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kItems; ++j)
+    z[j] = (uint32_t)Z[jz[j]];
+  if (kDiff) {
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j)
+      z[j] -= (uint32_t)Z[((revm >> j) & 1u) ? jz[j] + 1 : jz[j] - 1];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kItems; ++j) {
+    const uint32_t p = j * kBlock + threadIdx.x;
+    if (p < n_here)
+      out[base + o0 + p] = (int32_t)(vb[j] + z[j]);
   }
 }
 
 // Expansion by descending the rule tree (no Z): node IDs, no inverse delta.
+constexpr uint32_t kDescItems = 16;
+constexpr uint32_t kDescTile = kBlock * kDescItems;
+
 __global__ __launch_bounds__(kBlock) void k_expand_descent(
     const int32_t *__restrict__ enc, const uint64_t *__restrict__ sym_off, uint64_t n_out,
     const TileDesc *__restrict__ tdesc, const RuleRec *__restrict__ rec, uint64_t R,
     uint32_t min_id, int32_t *__restrict__ out) {
-  __shared__ TileStage st;
-  const uint64_t o0 = (uint64_t)blockIdx.x * kExTile;
-  const uint64_t o_end = o0 + kExTile < n_out ? o0 + kExTile : n_out;
+  __shared__ StartMap<kDescTile> map;
+  __shared__ uint16_t srel[kDescTile];
+  const uint64_t o0 = (uint64_t)blockIdx.x * kDescTile;
+  const uint64_t o_end = o0 + kDescTile < n_out ? o0 + kDescTile : n_out;
   const TileDesc td = tdesc[blockIdx.x];
   const uint64_t k0 = td.k0;
-  stage_tile(st, sym_off, k0, (uint32_t)(td.k_last - k0 + 1), 0, o0);
+  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
+  map.clear();
+  __syncthreads();
+  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
+    const uint32_t rel = s == 0 ? 0u : (uint32_t)(sym_off[k0 + s] - o0);
+    srel[s] = (uint16_t)rel;
+    map.mark(rel);
+  }
+  __syncthreads();
+  map.rank_words();
   const uint64_t first_off = td.first_off;
-  for (uint32_t j = 0; j < kExItems; ++j) {
+  for (uint32_t j = 0; j < kDescItems; ++j) {
     const uint32_t p = j * kBlock + threadIdx.x;
     const uint64_t o = o0 + p;
     if (o >= o_end)
       break;
-    const uint32_t s = tile_symbol(st, p);
-    uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - st.rel[s]);
+    const uint32_t s = map.symbol(p);
+    uint64_t q = s == 0 ? o - first_off : (uint64_t)(p - srel[s]);
     int32_t v = enc[k0 + s];
     while (is_rule(v, min_id, R)) {
       const RuleRec c = rec[uabs(v) - min_id];
@@ -544,6 +592,71 @@ uint32_t stride_grid(Ctx &ctx, uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
+// Tile-size candidates of the Z expansion (output elements per thread).
+constexpr uint32_t kZItems[] = {8, 16, 32};
+
+template <uint32_t kItems>
+void launch_z(Ctx &ctx, bool diff, const SymInfo *syms, const uint64_t *off, uint64_t base,
+              uint64_t n_out, const TileDesc *td, uint64_t n_tiles, const int32_t *Z,
+              uint32_t zraw, int32_t *out) {
+  if (diff)
+    hipLaunchKernelGGL((k_expand_z<kItems, true>), dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                       ctx.stream(), syms, off, base, n_out, td, Z, zraw, out);
+  else
+    hipLaunchKernelGGL((k_expand_z<kItems, false>), dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                       ctx.stream(), syms, off, base, n_out, td, Z, zraw, out);
+  GFZ_CHECK_LAUNCH();
+}
+
+// out[base + o] for o in [0, n_out) from the symbols syms[0..n_sym) whose
+// starts (absolute, offset by base) are off[0..n_sym] - through Z.
+void expand_through_z(Ctx &ctx, const char *group, bool diff, const SymInfo *syms,
+                      const uint64_t *off, uint64_t n_sym, uint64_t base, uint64_t n_out,
+                      const int32_t *Z, uint32_t zraw, int32_t *out) {
+  if (!n_out)
+    return;
+  auto run = [&](uint32_t items, bool timed) {
+    const uint64_t tile = (uint64_t)kBlock * items;
+    const uint64_t n_tiles = ceil_div(n_out, tile);
+    if (n_tiles > UINT32_MAX)
+      throw Error(kErrInvalid, "expansion too large for one launch");
+    DBuf<TileDesc> td(ctx.pool(), n_tiles);
+    hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_sym)), dim3(kBlock), 0, ctx.stream(),
+                       off, n_sym, base, n_out, (uint32_t)tile, n_tiles, td.get());
+    GFZ_CHECK_LAUNCH();
+    auto go = [&] {
+      switch (items) {
+        case 8: launch_z<8>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
+        case 32: launch_z<32>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
+        default: launch_z<16>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
+      }
+    };
+    if (timed)
+      return ctx.time_ms(go);
+    ctx.timed(group, go);
+    return 0.0f;
+  };
+  // Size classes of 4x; small expansions keep the default.
+  uint32_t cls = 0;
+  for (uint64_t n = n_out; n >= 4; n >>= 2)
+    ++cls;
+  const std::string key = std::string("expand_items/") + std::to_string(cls);
+  int items = ctx.tuned(key);
+  if (items < 0 && ctx.autotune() && n_out >= (uint64_t(1) << 20)) {
+    float best = 0;
+    for (uint32_t c : kZItems) {
+      run(c, true);  // warm
+      const float ms = run(c, true);
+      if (items < 0 || ms < best) {
+        best = ms;
+        items = (int)c;
+      }
+    }
+    ctx.set_tuned(key, items);
+  }
+  run(items < 0 ? kExItems : (uint32_t)items, false);
+}
+
 }  // namespace
 
 void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, uint64_t R,
@@ -636,12 +749,15 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   uint64_t budget = uint64_t(1) << 30;  // elements (4 GiB)
   if (const char *env = std::getenv("GFAZ_BODY_BUDGET_MB"))
     budget = std::strtoull(env, nullptr, 10) * (uint64_t(1) << 20) / 4;
+  budget = std::min<uint64_t>(budget, (uint64_t(1) << 31) - 2);  // 31-bit Z indices
   if (total > budget) {
     rb.zoff.reset();
     return;
   }
-  rb.z = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(total, 1));
+  rb.z = DBuf<int32_t>(ctx.pool(), total + 2);
   rb.z_elems = total;
+  rb.z_raw = (uint32_t)(total + 1);
+  fill_u32(ctx, reinterpret_cast<uint32_t *>(rb.z.get()) + total, 0, 2);
   // One level at a time: children are always at shallower, already-built levels.
   for (uint32_t d = 1; d <= max_depth; ++d) {
     const uint64_t r0 = bound[d], r1 = bound[d + 1];
@@ -649,25 +765,16 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
       continue;
     const uint64_t lo = lvl_off[d], hi = lvl_off[d + 1];
     const uint64_t n_sym = 3 * (r1 - r0);
-    const uint64_t n_tiles = ceil_div(hi - lo, kExTile);
-    if (n_tiles > UINT32_MAX)
-      throw Error(kErrInvalid, "rule level too large for one launch");
     DBuf<SymInfo> syms(ctx.pool(), n_sym);
     DBuf<uint64_t> off(ctx.pool(), n_sym + 1);
-    DBuf<TileDesc> tdesc(ctx.pool(), n_tiles);
     ctx.timed("build_bodies", [&] {
       hipLaunchKernelGGL(k_level_stream, dim3(stride_grid(ctx, r1 - r0)), dim3(kBlock), 0,
                          ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), rb.zoff.get(),
-                         r0, r1, min_id, R, syms.get(), off.get());
-      GFZ_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_sym)), dim3(kBlock), 0,
-                         ctx.stream(), off.get(), n_sym, lo, hi - lo, n_tiles, tdesc.get());
-      GFZ_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_expand_z<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                         ctx.stream(), syms.get(), off.get(), lo, hi - lo, tdesc.get(),
-                         rb.z.get(), rb.z.get());
-      GFZ_CHECK_LAUNCH();
+                         r0, r1, min_id, R, rb.z_raw, syms.get(), off.get());
     });
+    GFZ_CHECK_LAUNCH();
+    expand_through_z(ctx, "build_bodies", false, syms.get(), off.get(), n_sym, lo, hi - lo,
+                     rb.z.get(), rb.z_raw, rb.z.get());
   }
   ctx.sync();
   rb.has_bodies = true;
@@ -731,13 +838,6 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const uint64_t n_out = plan.n_out;
   if (!n_out)
     return;
-  const uint64_t n_tiles = ceil_div(n_out, kExTile);
-  if (n_tiles > UINT32_MAX)
-    throw Error(kErrInvalid, "decoded batch too large for one launch");
-  DBuf<TileDesc> tdesc(ctx.pool(), n_tiles);
-  hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                     d_sym_off, n_enc, 0, n_out, n_tiles, tdesc.get());
-  GFZ_CHECK_LAUNCH();
   const bool delta = delta_round > 0;
   int scans_left = delta_round;
   if (rb.has_bodies) {
@@ -762,21 +862,19 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
     }
     hipLaunchKernelGGL(k_sym_info, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
                        d_enc, n_enc, incl.get(), plan.sym_sum.get(), delta, rb.size.get(),
-                       rb.sum.get(), rb.zoff.get(), rb.n_rules, rb.min_id, syms.get());
+                       rb.sum.get(), rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, syms.get());
     GFZ_CHECK_LAUNCH();
-    ctx.timed("expand", [&] {
-      if (delta)
-        hipLaunchKernelGGL(k_expand_z<false>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                           ctx.stream(), syms.get(), d_sym_off, 0, n_out, tdesc.get(),
-                           rb.z.get(), d_out);
-      else
-        hipLaunchKernelGGL(k_expand_z<true>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                           ctx.stream(), syms.get(), d_sym_off, 0, n_out, tdesc.get(),
-                           rb.z.get(), d_out);
-    });
-    GFZ_CHECK_LAUNCH();
+    expand_through_z(ctx, "expand", !delta, syms.get(), d_sym_off, n_enc, 0, n_out, rb.z.get(),
+                     rb.z_raw, d_out);
     ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
   } else {
+    const uint64_t n_tiles = ceil_div(n_out, kDescTile);
+    if (n_tiles > UINT32_MAX)
+      throw Error(kErrInvalid, "decoded batch too large for one launch");
+    DBuf<TileDesc> tdesc(ctx.pool(), n_tiles);
+    hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                       d_sym_off, n_enc, 0, n_out, kDescTile, n_tiles, tdesc.get());
+    GFZ_CHECK_LAUNCH();
     ctx.timed("expand", [&] {
       hipLaunchKernelGGL(k_expand_descent, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
                          d_enc, d_sym_off, n_out, tdesc.get(), rb.rec.get(), rb.n_rules,

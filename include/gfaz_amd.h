@@ -63,6 +63,16 @@ int gfz_ctx_kernel_stat(gfz_ctx *ctx, const char *name, uint64_t *launches, doub
                         double *bytes);
 /* Number of CUs of the ctx's device. */
 int gfz_ctx_num_cus(gfz_ctx *ctx);
+/* Launch-shape autotuning (no counterpart in the reference, which hard-codes
+ * its chunk sizes, e.g. src/gpu/decompression/path_decompression_gpu_rolling.cu).
+ * With autotuning on (the default), the first call that meets an untuned
+ * kernel/size class times each candidate tile size on its own input and keeps
+ * the fastest for the ctx; every candidate gives the same result. Keys look
+ * like "expand_items/<size class>"; gfz_ctx_tuned gives -1 for an untuned key,
+ * gfz_ctx_set_tuned pins a choice (e.g. one saved from an earlier process). */
+int gfz_ctx_set_autotune(gfz_ctx *ctx, int on);
+int gfz_ctx_tuned(gfz_ctx *ctx, const char *key, int *value);
+int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value);
 
 /* Device memory helpers (pool-backed) for callers without their own allocator. */
 int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out);

@@ -188,3 +188,32 @@ def test_non_topological_rulebook_uses_descent(gpu_ctx):
                                                np.array([3], np.uint32), None, 0)
     assert dec.tolist() == [1, 2, 1, -1, -2, -1, 3]
     rb.close()
+
+
+def _size_class(n):
+    c = 0
+    while n >= 4:
+        n >>= 2
+        c += 1
+    return c
+
+
+@pytest.mark.parametrize("delta", [0, 1, 2])
+def test_every_expansion_tile_size_agrees(gpu_ctx, delta):
+    """Each tile-size candidate of the autotuner decodes bit-exactly (the
+    autotuner may pick any of them)."""
+    say(f"tile sizes delta={delta}")
+    tr = synth.generate(synth.CONFIGS["plain_20k"])
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, delta, threads=8)
+    rb = codec.Rulebook(gpu_ctx, ref["rules_first"], ref["rules_second"], ref["start_id"])
+    gpu_ctx.set_autotune(False)
+    try:
+        for items in (8, 16, 32):
+            gpu_ctx.set_tuned(f"expand_items/{_size_class(tr.n_steps)}", items)
+            dec, _, mism = codec.decompress_traversals(gpu_ctx, rb, ref["encoded"],
+                                                       ref["encoded_lengths"], tr.lengths, delta)
+            assert not mism
+            np.testing.assert_array_equal(dec, tr.nodes)
+    finally:
+        gpu_ctx.set_autotune(True)
+        rb.close()
