@@ -319,24 +319,35 @@ def decompress_traversals(ctx: Context, rb: Rulebook, encoded, encoded_lengths,
         encoded = np.ascontiguousarray(encoded, dtype=np.int32)
         eptr, en = encoded.ctypes.data, encoded.size
     n_out = _u64()
-    _check(lib().gfz_decoded_size(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
-                                  enc_len.size, flags, ctypes.byref(n_out)))
-    total = int(n_out.value)
     lens = np.empty(max(1, enc_len.size), dtype=np.uint32)
     mism = ctypes.c_int()
     if out is not None:
         flags |= GFZ_OUTPUT_DEVICE
-        if out.n < total:
-            raise ValueError(f"device output holds {out.n} < {total}")
-        optr, cap = out.ptr, out.n
-        result = out
+        result, optr, cap = out, out.ptr, out.n
     else:
+        # The stored original lengths give the decoded size without a planning
+        # pass; a disagreeing rulebook is caught by the library's capacity check.
+        if orig is not None:
+            total = int(orig.sum(dtype=np.uint64))
+        else:
+            _check(lib().gfz_decoded_size(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
+                                          enc_len.size, flags, ctypes.byref(n_out)))
+            total = int(n_out.value)
         result = np.empty(max(1, total), dtype=np.int32)
         optr, cap = result.ctypes.data, total
-    _check(lib().gfz_decompress_traversals(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
-                                           _ptr(orig), enc_len.size, delta_round, optr, cap, flags,
-                                           ctypes.byref(n_out), lens.ctypes.data,
-                                           ctypes.byref(mism)))
+    args = (ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data, _ptr(orig), enc_len.size,
+            delta_round)
+    rc = lib().gfz_decompress_traversals(*args, optr, cap, flags, ctypes.byref(n_out),
+                                         lens.ctypes.data, ctypes.byref(mism))
+    if rc != 0 and int(n_out.value) > cap:
+        if out is not None:
+            raise ValueError(f"device output holds {out.n} < {int(n_out.value)}")
+        result = np.empty(max(1, int(n_out.value)), dtype=np.int32)
+        optr, cap = result.ctypes.data, int(n_out.value)
+        rc = lib().gfz_decompress_traversals(*args, optr, cap, flags, ctypes.byref(n_out),
+                                             lens.ctypes.data, ctypes.byref(mism))
+    _check(rc)
+    total = int(n_out.value)
     if out is None:
         result = result[:total]
     return result, lens[: enc_len.size], bool(mism.value)

@@ -19,6 +19,8 @@
 #include "codec.hpp"
 #include "prims.hpp"
 
+#include <cstring>
+
 #include <algorithm>
 
 namespace gfz {
@@ -659,9 +661,11 @@ constexpr uint32_t kCollectTile = kBlock * kCollectItems;
 __global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict__ rt,
                                                          const uint8_t *__restrict__ used,
                                                          uint64_t cap, uint64_t *__restrict__ out,
-                                                         unsigned long long *count) {
+                                                         unsigned long long *scal) {
+  // scal[1] = keys collected, scal[2] = OR of the keys, scal[3] = AND of the keys.
   __shared__ uint32_t lds[kBlock / kWave + 1];
   __shared__ unsigned long long sbase;
+  __shared__ unsigned long long sor[kBlock / kWave], sand[kBlock / kWave];
   const uint64_t s0 = (uint64_t)blockIdx.x * kCollectTile + (uint64_t)threadIdx.x * kCollectItems;
   uint32_t flags = 0;
   if (s0 + kCollectItems <= cap) {
@@ -678,14 +682,39 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict_
   uint32_t total;
   uint32_t off = block_exclusive_scan<kBlock>((uint32_t)__popc(flags), lds, total);
   if (threadIdx.x == 0)
-    sbase = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
+    sbase = total ? atomicAdd(&scal[1], (unsigned long long)total) : 0ull;
   __syncthreads();
   const uint64_t b = sbase + off;
   uint32_t k = 0;
+  uint64_t kor = 0, kand = ~0ull;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    out[b + k++] = rt[s0 + q].key;
+    const uint64_t key = rt[s0 + q].key;
+    out[b + k++] = key;
+    kor |= key;
+    kand &= key;
     flags &= flags - 1;
+  }
+  if (!total)
+    return;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    kor |= __shfl_xor(kor, d);
+    kand &= __shfl_xor(kand, d);
+  }
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) {
+    sor[wid] = kor;
+    sand[wid] = kand;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < kBlock / kWave; ++w) {
+      kor |= sor[w];
+      kand &= sand[w];
+    }
+    atomicOr(&scal[2], (unsigned long long)kor);
+    atomicAnd(&scal[3], (unsigned long long)kand);
   }
 }
 
@@ -733,9 +762,11 @@ uint32_t stride_grid(Ctx &ctx, uint64_t n) {
 
 template <class T>
 T read_scalar(Ctx &ctx, const T *d) {
-  T v{};
-  GFZ_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, ctx.stream()));
+  static_assert(sizeof(T) <= 8, "scalar readback");
+  GFZ_HIP(hipMemcpyAsync(ctx.scalars(), d, sizeof(T), hipMemcpyDeviceToHost, ctx.stream()));
   ctx.sync();
+  T v;
+  std::memcpy(&v, ctx.scalars(), sizeof(T));
   return v;
 }
 
@@ -807,8 +838,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   uint64_t L = N;
   uint64_t rules_cap = 0;
   DBuf<int32_t> rf, rs;
-  DBuf<uint32_t> d_err(pool, 1);
-  fill_u32(ctx, d_err.get(), 0, 1);
+  DBuf<uint32_t> d_err(pool, 2);
+  fill_u32(ctx, d_err.get(), 0, 2);
   uint64_t prev_distinct = 0;
 
   for (int round = 0; round < p.rounds && L >= 2; ++round) {
@@ -850,9 +881,9 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), n_ins,
                          sums.get());
       GFZ_CHECK_LAUNCH();
-      unsigned long long hc[3];
-      GFZ_HIP(hipMemcpyAsync(hc, sums.get(), sizeof(hc), hipMemcpyDeviceToHost, ctx.stream()));
+      ctx.stage_scalars(sums.get(), 3);
       ctx.sync();
+      const uint64_t *hc = ctx.scalars();
       if (!hc[2] && hc[0] <= cap / 2) {
         distinct = hc[0];
         repeated = hc[1];
@@ -885,11 +916,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
     ctab.reset();
 
-    // K2 + K2b: candidates, tile carries, next length.
+    // K2 + K2b: candidates, tile carries, next length. Round scalars (read back
+    // once, after K4): [0] next length, [1] used keys, [2] OR and [3] AND of them.
     DBuf<uint64_t> cand(pool, n_words);
     DBuf<uint32_t> tile_fn(pool, n_tiles);
     DBuf<uint64_t> tile_info(pool, n_tiles);
-    DBuf<uint64_t> d_total(pool, 1);
+    DBuf<unsigned long long> scal(pool, 4);
+    fill_u64(ctx, reinterpret_cast<uint64_t *>(scal.get()), 0, 3);
+    fill_u64(ctx, reinterpret_cast<uint64_t *>(scal.get()) + 3, ~0ull, 1);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, span, rt.get(), rcap - 1, cand.get(), n_words,
@@ -899,46 +933,48 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     ctx.add_bytes("cand_tiles", 4.0 * L);
     ctx.timed("tile_scan", [&] {
       hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
-                         tile_fn.get(), n_tiles, L, tile_info.get(), d_total.get());
+                         tile_fn.get(), n_tiles, L, tile_info.get(),
+                         reinterpret_cast<uint64_t *>(scal.get()));
     });
     GFZ_CHECK_LAUNCH();
-    const uint64_t L_next = read_scalar(ctx, d_total.get());
-    if (L_next >= L)
-      throw Error(kErrCorrupt, "repeated pairs found but none picked (internal error)");
 
-    // K3: emit.
+    // K3: emit (the next length is at most L; its head bits are cleared up to L).
     DBuf<uint8_t> used(pool, rcap);
     fill_u8(ctx, used.get(), 0, rcap);
-    fill_u32(ctx, heads_next.get(), 0, ceil_div(L_next, 32));
+    fill_u32(ctx, heads_next.get(), 0, ceil_div(L, 32));
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt.get(),
                          rcap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
-    ctx.add_bytes("emit", 4.0 * L + 4.0 * L_next);
-    cand.reset();
-    tile_fn.reset();
-    tile_info.reset();
 
     // K4: used keys.
     DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);
-    DBuf<unsigned long long> ucount(pool, 1);
-    fill_u64(ctx, reinterpret_cast<uint64_t *>(ucount.get()), 0, 1);
     ctx.timed("collect_used", [&] {
       hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(rcap, kCollectTile)),
                          dim3(kBlock), 0, ctx.stream(), rt.get(), used.get(), rcap, ukeys.get(),
-                         ucount.get());
+                         scal.get());
     });
     GFZ_CHECK_LAUNCH();
-    const uint64_t R = read_scalar(ctx, ucount.get());
-    if (read_scalar(ctx, d_err.get()))
+    cand.reset();
+    tile_fn.reset();
+    tile_info.reset();
+    ctx.stage_scalars(scal.get(), 4);
+    ctx.stage_scalars(d_err.get(), 1, 4);
+    ctx.sync();
+    const uint64_t *hs = ctx.scalars();
+    const uint64_t L_next = hs[0], R = hs[1], varying = hs[2] & ~hs[3];
+    if ((uint32_t)hs[4])
       throw Error(kErrCorrupt, "pair table lost a key (internal error)");
+    if (L_next >= L)
+      throw Error(kErrCorrupt, "repeated pairs found but none picked (internal error)");
+    ctx.add_bytes("emit", 4.0 * L + 4.0 * L_next);
     if (R == 0)
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
-    sort_i64(ctx, ukeys.get(), utmp.get(), R);
+    sort_i64_varying(ctx, ukeys.get(), utmp.get(), R, varying);
     if (out.n_rules + R > rules_cap) {
       const uint64_t ncap = std::max<uint64_t>(2 * rules_cap, out.n_rules + R);
       DBuf<int32_t> nf(pool, ncap), ns(pool, ncap);
@@ -958,15 +994,13 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                          rs.get() + out.n_rules, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
-    // K7: remap provisional IDs.
+    // K7: remap provisional IDs (errors of K6 surface at the next readback).
     ctx.timed("remap", [&] {
       hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
                          dst, L_next, rt.get(), next);
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);
-    if (read_scalar(ctx, d_err.get()))
-      throw Error(kErrCorrupt, "sorted key missing from the rule table (internal error)");
 
     next += (uint32_t)R;
     out.n_rules += R;
@@ -979,6 +1013,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     heads.swap(heads_next);
   }
   out.end_id = next;
+  ctx.stage_scalars(d_err.get(), 1);
+  ctx.sync();
+  if ((uint32_t)ctx.scalars()[0])
+    throw Error(kErrCorrupt, "sorted key missing from the rule table (internal error)");
 
   // Encoded lengths from the head positions of the final stream.
   if (!ne_index.empty()) {

@@ -88,6 +88,8 @@ Ctx::~Ctx() {
     (void)hipEventDestroy(e);
   if (pinned_)
     (void)hipHostFree(pinned_);
+  if (scal_)
+    (void)hipHostFree(scal_);
   pool_.trim();
   if (stream_)
     (void)hipStreamDestroy(stream_);
@@ -121,6 +123,15 @@ void Ctx::collect() {
     free_events_.push_back(p.b);
   }
   pending_.clear();
+}
+
+uint64_t *Ctx::scalars() {
+  if (!scal_) {
+    void *p = nullptr;
+    GFZ_HIP(hipHostMalloc(&p, 512 * 8, hipHostMallocDefault));
+    scal_ = static_cast<uint64_t *>(p);
+  }
+  return scal_;
 }
 
 void *Ctx::pinned(size_t bytes) {

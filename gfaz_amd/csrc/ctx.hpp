@@ -120,6 +120,12 @@ class Ctx {
 
   // Pinned host staging buffer of at least `bytes`, reused across calls.
   void *pinned(size_t bytes);
+  // Small pinned area (512 x 8 bytes) for device-to-host scalar readbacks.
+  uint64_t *scalars();
+  // Copy n 64-bit device words into scalars()[at..) (no sync).
+  void stage_scalars(const void *dev, size_t n, size_t at = 0) {
+    GFZ_HIP(hipMemcpyAsync(scalars() + at, dev, n * 8, hipMemcpyDeviceToHost, stream_));
+  }
 
   // Autotuned launch choices (tile sizes and the like), keyed by kernel and
   // size class. When autotuning is on, the first call that needs an untuned
@@ -165,6 +171,7 @@ class Ctx {
   bool autotune_ = true;
   std::map<std::string, int> tuned_;
   void *pinned_ = nullptr;
+  uint64_t *scal_ = nullptr;
   size_t pinned_bytes_ = 0;
 };
 

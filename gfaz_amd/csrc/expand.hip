@@ -672,79 +672,88 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   if (!R)
     return;
   DBuf<uint32_t> depth(ctx.pool(), R);
-  DBuf<uint32_t> flags(ctx.pool(), 2);  // changed, err
+  // Passes run in batches without a host round trip: a pass that resolves
+  // nothing leaves nothing for the later passes of its batch either.
+  constexpr uint32_t kPassBatch = 8;
+  DBuf<uint32_t> flags(ctx.pool(), kPassBatch + 4);  // changed per pass, err, bad, pad
+  uint32_t *err = flags.get() + kPassBatch, *bad = flags.get() + kPassBatch + 1;
   hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
                      d_first, d_second, R, min_id, rb.rec.get(), rb.size.get(), rb.sum.get(),
                      depth.get());
   GFZ_CHECK_LAUNCH();
   uint32_t max_depth = 1;
-  for (uint64_t pass = 0; pass <= R; ++pass) {
-    fill_u32(ctx, flags.get(), 0, 2);
+  bool converged = false;
+  for (uint64_t pass0 = 0; !converged && pass0 <= R; pass0 += kPassBatch) {
+    fill_u32(ctx, flags.get(), 0, kPassBatch + 4);
     ctx.timed("rule_sizes", [&] {
-      hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         rb.rec.get(), rb.size.get(), rb.sum.get(), depth.get(), R, min_id,
-                         (uint32_t)pass, flags.get(), flags.get() + 1);
+      for (uint32_t k = 0; k < kPassBatch; ++k)
+        hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0,
+                           ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), depth.get(),
+                           R, min_id, (uint32_t)(pass0 + k), flags.get() + k, err);
     });
     GFZ_CHECK_LAUNCH();
-    uint32_t h[2];
-    GFZ_HIP(hipMemcpyAsync(h, flags.get(), sizeof(h), hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.stage_scalars(flags.get(), (kPassBatch + 4) / 2);
     ctx.sync();
-    if (h[1])
+    const uint32_t *h = reinterpret_cast<const uint32_t *>(ctx.scalars());
+    if (h[kPassBatch])
       throw Error(kErrCorrupt, "rule expansion size exceeds 2^32 symbols");
-    if (!h[0])
-      break;
-    max_depth = (uint32_t)pass + 2;
+    for (uint32_t k = 0; k < kPassBatch; ++k) {
+      if (!h[k]) {
+        converged = true;
+        break;
+      }
+      max_depth = (uint32_t)(pass0 + k) + 2;
+    }
   }
-  hipLaunchKernelGGL(k_rule_unresolved, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     depth.get(), R, flags.get() + 1);
-  GFZ_CHECK_LAUNCH();
-  uint32_t e = 0;
-  GFZ_HIP(hipMemcpyAsync(&e, flags.get() + 1, 4, hipMemcpyDeviceToHost, ctx.stream()));
-  ctx.sync();
-  if (e)
-    throw Error(kErrCorrupt, "rulebook is cyclic: some rule never bottoms out in node IDs");
   rb.depth = max_depth;
 
-  // Level ranges (contiguous when depth is non-decreasing in rule ID).
-  std::vector<uint64_t> bound(max_depth + 2, R);
-  {
-    DBuf<uint64_t> d_bound(ctx.pool(), max_depth + 2);
-    fill_u64(ctx, d_bound.get(), R, max_depth + 2);
-    fill_u32(ctx, flags.get(), 0, 1);
-    hipLaunchKernelGGL(k_depth_bounds, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                       depth.get(), R, max_depth, d_bound.get(), flags.get());
-    GFZ_CHECK_LAUNCH();
-    GFZ_HIP(hipMemcpyAsync(bound.data(), d_bound.get(), bound.size() * 8, hipMemcpyDeviceToHost,
-                           ctx.stream()));
-    GFZ_HIP(hipMemcpyAsync(&e, flags.get(), 4, hipMemcpyDeviceToHost, ctx.stream()));
-    ctx.sync();
-  }
-  if (e)
-    return;  // depth not monotone in rule ID: expansion descends the tree instead
-  depth.reset();
-
-  // Z offsets: size + 1 entries per rule, levels contiguous.
+  // Unresolved rules (a cycle), level boundaries, Z offsets (size + 1 entries
+  // per rule) and the Z offset of each level, read back together.
+  std::vector<uint64_t> bound(max_depth + 2, R), lvl_off(max_depth + 2);
   rb.zoff = DBuf<uint64_t>(ctx.pool(), R + 1);
   {
+    fill_u32(ctx, flags.get(), 0, kPassBatch + 4);
+    hipLaunchKernelGGL(k_rule_unresolved, dim3(stride_grid(ctx, R)), dim3(kBlock), 0,
+                       ctx.stream(), depth.get(), R, err);
+    GFZ_CHECK_LAUNCH();
+    DBuf<uint64_t> d_bound(ctx.pool(), max_depth + 2), d_lvl(ctx.pool(), max_depth + 2);
+    fill_u64(ctx, d_bound.get(), R, max_depth + 2);
+    hipLaunchKernelGGL(k_depth_bounds, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                       depth.get(), R, max_depth, d_bound.get(), bad);
+    GFZ_CHECK_LAUNCH();
     DBuf<uint32_t> szp1(ctx.pool(), R);
     hipLaunchKernelGGL(k_plus_one, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
                        rb.size.get(), R, szp1.get());
     GFZ_CHECK_LAUNCH();
     exclusive_scan(ctx, szp1.get(), rb.zoff.get(), R);
-    ctx.sync();
-  }
-  std::vector<uint64_t> lvl_off(max_depth + 2);
-  {
-    DBuf<uint64_t> d_lvl(ctx.pool(), max_depth + 2), d_idx(ctx.pool(), max_depth + 2);
-    GFZ_HIP(hipMemcpyAsync(d_idx.get(), bound.data(), bound.size() * 8, hipMemcpyHostToDevice,
-                           ctx.stream()));
     hipLaunchKernelGGL(k_gather_u64, dim3(1), dim3(kBlock), 0, ctx.stream(), rb.zoff.get(),
-                       d_idx.get(), max_depth + 2, d_lvl.get());
+                       d_bound.get(), max_depth + 2, d_lvl.get());
     GFZ_CHECK_LAUNCH();
-    GFZ_HIP(hipMemcpyAsync(lvl_off.data(), d_lvl.get(), lvl_off.size() * 8, hipMemcpyDeviceToHost,
-                           ctx.stream()));
-    ctx.sync();
+    const size_t nb = max_depth + 2;
+    ctx.stage_scalars(flags.get(), (kPassBatch + 4) / 2);
+    if (2 * nb + (kPassBatch + 4) / 2 <= 512) {
+      ctx.stage_scalars(d_bound.get(), nb, (kPassBatch + 4) / 2);
+      ctx.stage_scalars(d_lvl.get(), nb, (kPassBatch + 4) / 2 + nb);
+      ctx.sync();
+      const uint64_t *hs = ctx.scalars() + (kPassBatch + 4) / 2;
+      std::copy(hs, hs + nb, bound.begin());
+      std::copy(hs + nb, hs + 2 * nb, lvl_off.begin());
+    } else {  // deep rulebook: pageable copies
+      GFZ_HIP(hipMemcpyAsync(bound.data(), d_bound.get(), nb * 8, hipMemcpyDeviceToHost,
+                             ctx.stream()));
+      GFZ_HIP(hipMemcpyAsync(lvl_off.data(), d_lvl.get(), nb * 8, hipMemcpyDeviceToHost,
+                             ctx.stream()));
+      ctx.sync();
+    }
+    const uint32_t *hf = reinterpret_cast<const uint32_t *>(ctx.scalars());
+    if (hf[kPassBatch])
+      throw Error(kErrCorrupt, "rulebook is cyclic: some rule never bottoms out in node IDs");
+    if (hf[kPassBatch + 1]) {
+      rb.zoff.reset();
+      return;  // depth not monotone in rule ID: expansion descends the tree instead
+    }
   }
+  depth.reset();
   const uint64_t total = lvl_off[max_depth + 1];
   uint64_t budget = uint64_t(1) << 30;  // elements (4 GiB)
   if (const char *env = std::getenv("GFAZ_BODY_BUDGET_MB"))

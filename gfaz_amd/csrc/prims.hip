@@ -337,11 +337,13 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
   hipLaunchKernelGGL(k_or_diff, dim3(grid_for(ctx, n, kBlock * 8)), dim3(kBlock), 0,
                      ctx.stream(), keys, n, acc.get());
   GFZ_CHECK_LAUNCH();
-  unsigned long long varying = 0;
-  GFZ_HIP(hipMemcpyAsync(&varying, acc.get(), sizeof(varying), hipMemcpyDeviceToHost,
-                         ctx.stream()));
+  ctx.stage_scalars(acc.get(), 1);
   ctx.sync();
-  if (!varying)
+  sort_i64_varying(ctx, keys, tmp, n, ctx.scalars()[0]);
+}
+
+void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
+  if (n < 2 || !varying)
     return;
   const uint32_t nb = (uint32_t)ceil_div(n, kSortTile);
   DBuf<uint32_t> hist(ctx.pool(), (uint64_t)256 * nb);

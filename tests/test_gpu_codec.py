@@ -217,3 +217,40 @@ def test_every_expansion_tile_size_agrees(gpu_ctx, delta):
     finally:
         gpu_ctx.set_autotune(True)
         rb.close()
+
+
+def _expand_py(sym, first, second, min_id):
+    """Recursive definition of rule expansion (decompression_workflow.cpp:70-106)."""
+    r = abs(sym) - min_id
+    if not (0 <= r < len(first)):
+        return [sym]
+    if sym > 0:
+        return _expand_py(first[r], first, second, min_id) + _expand_py(second[r], first, second, min_id)
+    return _expand_py(-second[r], first, second, min_id) + _expand_py(-first[r], first, second, min_id)
+
+
+def test_deep_chain_rulebook(gpu_ctx):
+    """A 300-level chain (rule i = (rule i-1, node)): more levels than one scalar
+    readback holds, both orientations, with and without an inverse delta."""
+    say("deep chain rulebook")
+    import sys as _sys
+    _sys.setrecursionlimit(10000)
+    min_id, depth = 1000, 300
+    first = [5] + [min_id + i - 1 for i in range(1, depth)]
+    second = [7] + [(-1) ** i * (i % 13 + 1) for i in range(1, depth)]
+    rb = codec.Rulebook(gpu_ctx, first, second, min_id)
+    info = rb.info()
+    assert info["max_depth"] == depth and info["has_bodies"]
+    top = min_id + depth - 1
+    enc = np.array([3, top, -top, min_id + 7, -(min_id + 100)], np.int32)
+    want = []
+    for s in enc.tolist():
+        want += _expand_py(s, first, second, min_id)
+    for delta in (0, 1):
+        dec, lens, _ = codec.decompress_traversals(gpu_ctx, rb, enc, np.array([enc.size], np.uint32),
+                                                   None, delta)
+        exp = np.array(want, np.int64)
+        if delta:
+            exp = np.cumsum(exp)
+        np.testing.assert_array_equal(dec.astype(np.int64), (exp + 2**31) % 2**32 - 2**31)
+    rb.close()
