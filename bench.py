@@ -86,6 +86,24 @@ def cpu_baseline(cfg, tr, gfa_bytes: int, threads: int) -> dict:
             "compress_s": t1 - t0, "decompress_s": t2 - t1}
 
 
+# Timed kernel groups and the kernel whose hardware counters stand for them.
+GROUP_KERNEL = {"pair_insert": "pair_insert", "cand_tiles": "cand", "emit": "emit",
+                "collect_used": "collect_used", "remap": "remap", "delta": "delta",
+                "radix_sort": "radix_scatter", "expand": "expand_z"}
+
+
+def measured_traffic(workload: str, group: str):
+    """HBM bytes per launch of a kernel group from the committed counter summary
+    (profiles/traffic_<workload>.md: FETCH_SIZE and WRITE_SIZE scaled by a plain
+    copy), or None when no summary covers it."""
+    from gfaz_amd.profile_summary import load_traffic
+    t = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{workload}.md"))
+    k = GROUP_KERNEL.get(group)
+    if not t or k not in t["kernels"]:
+        return None
+    return t["kernels"][k]["bytes_per_dispatch"]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +229,8 @@ def main() -> int:
         if per_launch:
             ach = per_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "traffic": measured_traffic(cfg.name, dom),
                         "avg_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch}
     expand_roof = None
     if exp["launches"]:
@@ -220,8 +239,8 @@ def main() -> int:
         expand_roof = {"kernel": "expand", "achieved": a, "frac": a / HBM_PEAK_GBS, "avg_ms": avg}
 
     steps = args.steps
-    ms_step = (tc + td) / steps * 1e3
-    value = gfa_bytes / ((tc + td) / steps) / 1e9
+    ms_step = t_total / steps * 1e3
+    value = gfa_bytes / (t_total / steps) / 1e9
     out = {
         "metric": "Compress/decompress GFA GB/s at matched ratio, %HBM peak; extract p50/p99 ms",
         "value": value, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
