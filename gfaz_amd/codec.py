@@ -31,6 +31,11 @@ _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 
 
+class _Block(ctypes.Structure):
+    _fields_ = [("original_size", _u64), ("payload", ctypes.POINTER(ctypes.c_uint8)),
+                ("payload_size", _u64)]
+
+
 class _CompressOut(ctypes.Structure):
     _fields_ = [
         ("encoded", ctypes.POINTER(ctypes.c_int32)),
@@ -72,6 +77,12 @@ _SIGNATURES = {
     "gfz_dev_free": ([_vp, _vp], ctypes.c_int),
     "gfz_memcpy_h2d": ([_vp, _vp, _vp, _u64], ctypes.c_int),
     "gfz_memcpy_d2h": ([_vp, _vp, _vp, _u64], ctypes.c_int),
+    "gfz_frame_int32": ([_vp, _u64, ctypes.c_int, _vp], ctypes.c_int),
+    "gfz_frame_rule_columns": ([_vp, _vp, _u64, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "gfz_unframe_int32": ([_vp, _u64, _u64, _vp, _u64], ctypes.c_int),
+    "gfz_unframe_rule_columns": ([_vp, _u64, _u64, _vp, _u64, _u64, _vp, _vp, _u64],
+                                 ctypes.c_int),
+    "gfz_block_free": ([_vp], None),
     "gfz_calibration_copy": ([_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double)],
                              ctypes.c_int),
     "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
@@ -431,3 +442,50 @@ class DeviceCompressed:
             self.close()
         except Exception:
             pass
+
+
+# ---- traversal-column framing (host; gfz_frame_* / gfz_unframe_*) -------------
+
+def _take_block(b: _Block) -> tuple[int, bytes]:
+    data = ctypes.string_at(b.payload, b.payload_size) if b.payload_size else b""
+    lib().gfz_block_free(ctypes.byref(b))
+    return int(b.original_size), data
+
+
+def frame_int32(values, threads: int = 0) -> tuple[int, bytes]:
+    """(original_size, zstd payload) of an int32 column, as zstd_compress_int32_vector."""
+    a = np.ascontiguousarray(values, dtype=np.int32)
+    b = _Block()
+    _check(lib().gfz_frame_int32(a.ctypes.data, a.size, int(threads), ctypes.byref(b)))
+    return _take_block(b)
+
+
+def frame_rule_columns(first, second, threads: int = 0):
+    """Raw rule columns -> ((orig, payload), (orig, payload)) as process_rules + zstd."""
+    f = np.ascontiguousarray(first, dtype=np.int32)
+    s_ = np.ascontiguousarray(second, dtype=np.int32)
+    if f.size != s_.size:
+        raise ValueError("rule columns differ in length")
+    bf, bs = _Block(), _Block()
+    _check(lib().gfz_frame_rule_columns(f.ctypes.data, s_.ctypes.data, f.size, int(threads),
+                                        ctypes.byref(bf), ctypes.byref(bs)))
+    return _take_block(bf), _take_block(bs)
+
+
+def unframe_int32(payload: bytes, original_size: int) -> np.ndarray:
+    out = np.empty(max(1, original_size // 4), dtype=np.int32)
+    buf = ctypes.create_string_buffer(payload, len(payload)) if payload else None
+    _check(lib().gfz_unframe_int32(buf, len(payload), int(original_size), out.ctypes.data,
+                                   out.size))
+    return out[: original_size // 4]
+
+
+def unframe_rule_columns(first: tuple[int, bytes], second: tuple[int, bytes], n_rules: int):
+    f = np.empty(max(1, n_rules), dtype=np.int32)
+    s_ = np.empty(max(1, n_rules), dtype=np.int32)
+    pf = ctypes.create_string_buffer(first[1], len(first[1])) if first[1] else None
+    ps = ctypes.create_string_buffer(second[1], len(second[1])) if second[1] else None
+    _check(lib().gfz_unframe_rule_columns(pf, len(first[1]), int(first[0]), ps, len(second[1]),
+                                          int(second[0]), f.ctypes.data, s_.ctypes.data,
+                                          int(n_rules)))
+    return f[:n_rules], s_[:n_rules]

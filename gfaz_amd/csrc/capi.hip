@@ -11,30 +11,17 @@
 #include <memory>
 #include <string>
 
+#include "abi_error.hpp"
 #include "abi_types.hpp"
+
+namespace gfz {
+thread_local std::string g_last_error;
+}  // namespace gfz
 
 namespace {
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg) {
-  g_last_error = msg;
-  return code;
-}
-
-template <class F>
-int guarded(const char *prefix, F &&f) {
-  try {
-    f();
-    return GFZ_OK;
-  } catch (const gfz::Error &e) {
-    return fail(e.code, std::string(prefix) + e.what());
-  } catch (const std::bad_alloc &) {
-    return fail(GFZ_E_NOMEM, std::string(prefix) + "host allocation failed");
-  } catch (const std::exception &e) {
-    return fail(GFZ_E_INVALID, std::string(prefix) + e.what());
-  }
-}
+using gfz::fail;
+using gfz::guarded;
 
 constexpr const char *kCompressPrefix = "GPU compression error: ";
 constexpr const char *kDecompressPrefix = "GPU decompression error: ";
@@ -60,7 +47,7 @@ T *host_copy(gfz::Ctx &ctx, const T *d, uint64_t n) {
 
 extern "C" {
 
-const char *gfz_last_error(void) { return g_last_error.c_str(); }
+const char *gfz_last_error(void) { return gfz::g_last_error.c_str(); }
 const char *gfz_version(void) { return "gfaz_amd 0.1 (gfx950)"; }
 
 int gfz_ctx_create(int device, gfz_ctx **out) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "error.hpp"
+
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -12,22 +14,6 @@ namespace gfz {
 
 constexpr int kWave = 64;
 constexpr uint32_t kBlock = 256;  // 4 waves
-
-// Error type carried up to the C ABI, which turns it into a status code.
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-enum Status : int {
-  kOk = 0,
-  kErrInvalid = -1,
-  kErrHip = -2,
-  kErrOutOfMemory = -3,
-  kErrOverflow = -4,
-  kErrCorrupt = -5,
-  kErrBusy = -6,
-};
 
 #define GFZ_HIP(expr)                                                                    \
   do {                                                                                   \

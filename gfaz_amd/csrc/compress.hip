@@ -229,24 +229,67 @@ __global__ void k_fill_slots(Slot *slots, uint64_t n) {
 }
 
 // ---- delta (codec.cpp:45-76) ------------------------------------------------
+// x[i] -= x[i-1] inside each traversal (heads keep their value), and the max of
+// |value| over the result. Each thread takes 8 consecutive elements per step
+// (two 16-byte loads and stores); one atomic per block for the max.
+__device__ inline uint32_t block_max(uint32_t m) {
+  __shared__ uint32_t wmax[kBlock / kWave];
+  m = wave_reduce_max(m);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    wmax[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (uint32_t w = 1; w < kBlock / kWave; ++w)
+      m = wmax[w] > m ? wmax[w] : m;
+  return m;
+}
+
 __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in,
                                                   int32_t *__restrict__ out,
                                                   const uint32_t *__restrict__ heads, uint64_t n,
                                                   int transform, uint32_t *max_abs) {
   uint32_t m = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock) {
+  const uint64_t n8 = n / 8;
+  for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < n8;
+       c += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i0 = c * 8;
+    const int4 a = *reinterpret_cast<const int4 *>(in + i0);
+    const int4 b = *reinterpret_cast<const int4 *>(in + i0 + 4);
+    int32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    if (transform) {
+      const uint32_t hb = (heads[i0 >> 5] >> (i0 & 31)) & 0xFFu;  // i0 % 8 == 0
+      const int32_t prev = (hb & 1u) ? 0 : in[i0 - 1];
+      int32_t d[8];
+      d[0] = (int32_t)((uint32_t)v[0] - (uint32_t)prev);
+#pragma unroll
+      for (int j = 1; j < 8; ++j)
+        d[j] = ((hb >> j) & 1u) ? v[j] : (int32_t)((uint32_t)v[j] - (uint32_t)v[j - 1]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        v[j] = d[j];
+      *reinterpret_cast<int4 *>(out + i0) = make_int4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<int4 *>(out + i0 + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t u = uabs(v[j]);
+      m = u > m ? u : m;
+    }
+  }
+  // Tail (n % 8 elements), one block.
+  if (blockIdx.x == 0 && threadIdx.x < n - n8 * 8) {
+    const uint64_t i = n8 * 8 + threadIdx.x;
     int32_t v = in[i];
     if (transform) {
       if (!head_at(heads, i))
         v = (int32_t)((uint32_t)v - (uint32_t)in[i - 1]);
       out[i] = v;
     }
-    const uint32_t a = uabs(v);
-    m = a > m ? a : m;
+    const uint32_t u = uabs(v);
+    m = u > m ? u : m;
   }
-  m = wave_reduce_max(m);
-  if ((threadIdx.x & (kWave - 1)) == 0 && m)
+  m = block_max(m);
+  if (threadIdx.x == 0 && m)
     atomicMax(max_abs, m);
 }
 
@@ -811,18 +854,26 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   const int32_t *cur = d_nodes;
   int32_t *cur_buf = nullptr;
   if (N) {
+    // The kernel reads 16-byte vectors: a caller's pointer that is not 16-byte
+    // aligned is first copied into an aligned buffer.
+    if (reinterpret_cast<uintptr_t>(d_nodes) % 16) {
+      GFZ_HIP(hipMemcpyAsync(xb.get(), d_nodes, N * 4, hipMemcpyDeviceToDevice, ctx.stream()));
+      cur = xb.get();
+    }
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(ceil_div(N, (uint64_t)kBlock * 8), (uint64_t)ctx.num_cus() * 8));
     if (delta_round == 0) {
       ctx.timed("delta", [&] {
-        hipLaunchKernelGGL(k_delta, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(),
-                           d_nodes, (int32_t *)nullptr, heads.get(), N, 0, d_max.get());
+        hipLaunchKernelGGL(k_delta, dim3(grid), dim3(kBlock), 0, ctx.stream(), cur,
+                           (int32_t *)nullptr, heads.get(), N, 0, d_max.get());
       });
       GFZ_CHECK_LAUNCH();
     }
     for (int r = 0; r < delta_round; ++r) {
-      int32_t *dst = (r % 2 == 0) ? xa.get() : xb.get();
+      int32_t *dst = (cur == xa.get()) ? xb.get() : xa.get();
       ctx.timed("delta", [&] {
-        hipLaunchKernelGGL(k_delta, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(),
-                           cur, dst, heads.get(), N, 1, d_max.get());
+        hipLaunchKernelGGL(k_delta, dim3(grid), dim3(kBlock), 0, ctx.stream(), cur, dst,
+                           heads.get(), N, 1, d_max.get());
       });
       GFZ_CHECK_LAUNCH();
       cur = dst;

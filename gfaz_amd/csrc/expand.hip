@@ -36,8 +36,7 @@ namespace gfz {
 
 namespace {
 
-constexpr uint32_t kExItems = 16;                 // output elements per thread
-constexpr uint32_t kExTile = kBlock * kExItems;   // output elements per block
+constexpr uint32_t kExItems = 16;  // default output elements per thread
 constexpr uint32_t kSsItems = 16;
 constexpr uint32_t kSsTile = kBlock * kSsItems;
 constexpr uint32_t kSsScanThreads = 1024;

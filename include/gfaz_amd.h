@@ -152,6 +152,39 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
                               int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
                               uint32_t *out_lengths, int *length_mismatch);
 
+/* ---- traversal-column framing (host) -----------------------------------
+ * The steps the reference CPU workflow applies around the grammar before
+ * serialisation, producing the same bytes, so a caller can fill CompressedData
+ * (include/model/compressed_data.hpp:43-66) from gfz_compress_traversals:
+ *   rules_first_zstd / rules_second_zstd = gfz_frame_rule_columns: process_rules
+ *     (src/workflows/compression_utils.cpp:165-193, one LayerRuleRange), i.e. each
+ *     raw column delta-encoded over the whole column (src/codec/codec.cpp:79-86),
+ *     then zstd_compress_int32_vector;
+ *   paths_zstd / walks_zstd = gfz_frame_int32 over the flattened encoded paths,
+ *     resp. walks (compression_utils.cpp:38-54; codec.cpp:180-217).
+ * zstd runs at level 9 (GFA_COMPRESSION_ZSTD_LEVEL overrides, 1..22) with
+ * nbWorkers = GFA_COMPRESSION_ZSTD_WORKERS if > 0, else `threads` (<= 0: the
+ * OpenMP thread count), exactly as src/codec/codec.cpp:100-177.
+ * The inverses replace zstd_decompress_int32_vector (codec.cpp:224-244) and
+ * delta_decode_int32 (codec.cpp:88-95); where the reference prints a zstd error
+ * and returns an empty column, these return GFZ_E_CORRUPT. */
+typedef struct gfz_block {
+  uint64_t original_size;  /* bytes before zstd (ZstdCompressedBlock::original_size) */
+  uint8_t *payload;        /* zstd frame; library-owned, release with gfz_block_free */
+  uint64_t payload_size;
+} gfz_block;
+
+int gfz_frame_int32(const int32_t *data, uint64_t n, int threads, gfz_block *out);
+int gfz_frame_rule_columns(const int32_t *first, const int32_t *second, uint64_t n_rules,
+                           int threads, gfz_block *out_first, gfz_block *out_second);
+int gfz_unframe_int32(const uint8_t *payload, uint64_t payload_size, uint64_t original_size,
+                      int32_t *out, uint64_t out_cap);
+int gfz_unframe_rule_columns(const uint8_t *first_payload, uint64_t first_payload_size,
+                             uint64_t first_original_size, const uint8_t *second_payload,
+                             uint64_t second_payload_size, uint64_t second_original_size,
+                             int32_t *first, int32_t *second, uint64_t n_rules);
+void gfz_block_free(gfz_block *b);
+
 /* ---- measurement support -------------------------------------------------
  * No counterpart in the reference: a plain copy of `bytes` (a multiple of 16)
  * between device buffers, timed with HIP events on the ctx stream. Profiles use
