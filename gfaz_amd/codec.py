@@ -83,6 +83,13 @@ _SIGNATURES = {
     "gfz_unframe_rule_columns": ([_vp, _u64, _u64, _vp, _u64, _u64, _vp, _vp, _u64],
                                  ctypes.c_int),
     "gfz_block_free": ([_vp], None),
+    "gfz_resident_load": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_uint32,
+                           ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_resident_free": ([_vp], None),
+    "gfz_resident_info": ([_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)], ctypes.c_int),
+    "gfz_resident_lengths": ([_vp, _vp], ctypes.c_int),
+    "gfz_extract": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, _vp,
+                     ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_calibration_copy": ([_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double)],
                              ctypes.c_int),
     "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
@@ -489,3 +496,63 @@ def unframe_rule_columns(first: tuple[int, bytes], second: tuple[int, bytes], n_
                                           int(second[0]), f.ctypes.data, s_.ctypes.data,
                                           int(n_rules)))
     return f[:n_rules], s_[:n_rules]
+
+
+# ---- resident decoding / extract ----------------------------------------------
+
+class Resident:
+    """Encoded traversals kept decodable in HBM against a loaded Rulebook
+    (gfz_resident_load); ``extract`` decodes any subset in one launch."""
+
+    def __init__(self, ctx: Context, rb: "Rulebook", encoded, encoded_lengths,
+                 delta_round: int = 1):
+        self.ctx, self.rb = ctx, rb
+        enc_len = np.ascontiguousarray(encoded_lengths, dtype=np.uint32)
+        flags = 0
+        if isinstance(encoded, DeviceArray):
+            flags |= GFZ_INPUT_DEVICE
+            eptr, en = encoded.ptr, encoded.n
+        else:
+            encoded = np.ascontiguousarray(encoded, dtype=np.int32)
+            eptr, en = encoded.ctypes.data, encoded.size
+        h = _vp()
+        _check(lib().gfz_resident_load(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
+                                       enc_len.size, int(delta_round), flags, ctypes.byref(h)))
+        self.handle = h.value
+        nt, nd = _u64(), _u64()
+        _check(lib().gfz_resident_info(self.handle, ctypes.byref(nt), ctypes.byref(nd)))
+        self.n_trav, self.n_decoded = int(nt.value), int(nd.value)
+
+    def lengths(self) -> np.ndarray:
+        out = np.empty(max(1, self.n_trav), dtype=np.uint32)
+        _check(lib().gfz_resident_lengths(self.handle, out.ctypes.data))
+        return out[: self.n_trav]
+
+    def extract(self, ids, out: DeviceArray | None = None):
+        """Decode traversals ``ids`` back to back: (values, offsets[n_ids + 1])."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        offs = np.empty(ids.size + 1, dtype=np.uint64)
+        n = _u64()
+        if out is not None:
+            _check(lib().gfz_extract(self.ctx.handle, self.handle, ids.ctypes.data, ids.size,
+                                     out.ptr, out.n, GFZ_OUTPUT_DEVICE, offs.ctypes.data,
+                                     ctypes.byref(n)))
+            return out, offs
+        lens = self.lengths()
+        total = int(lens[ids.astype(np.int64)].sum(dtype=np.uint64)) if ids.size and \
+            int(ids.max()) < self.n_trav else 0
+        res = np.empty(max(1, total), dtype=np.int32)
+        _check(lib().gfz_extract(self.ctx.handle, self.handle, ids.ctypes.data, ids.size,
+                                 res.ctypes.data, total, 0, offs.ctypes.data, ctypes.byref(n)))
+        return res[: int(n.value)], offs
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().gfz_resident_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

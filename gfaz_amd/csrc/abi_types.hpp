@@ -5,6 +5,7 @@
 
 #include "codec.hpp"
 #include "ctx.hpp"
+#include "resident.hpp"
 
 #include <memory>
 
@@ -15,4 +16,10 @@ struct gfz_ctx {
 struct gfz_rulebook {
   gfz_ctx *owner = nullptr;
   gfz::Rulebook rb;
+};
+
+struct gfz_resident {
+  gfz_ctx *owner = nullptr;
+  const gfz_rulebook *rulebook = nullptr;
+  gfz::Resident res;
 };

@@ -26,6 +26,7 @@ using gfz::guarded;
 constexpr const char *kCompressPrefix = "GPU compression error: ";
 constexpr const char *kDecompressPrefix = "GPU decompression error: ";
 constexpr const char *kRuntimePrefix = "GPU runtime error: ";
+constexpr const char *kExtractPrefix = "Extraction workflow error: ";
 
 struct CompressInternal {
   gfz::CompressResult res;
@@ -383,6 +384,116 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
       if (plan.n_out)
         GFZ_HIP(hipMemcpyAsync(out, d_out.get(), plan.n_out * 4, hipMemcpyDeviceToHost,
                                c.stream()));
+      c.sync();
+    }
+  });
+}
+
+// ---- resident decoding / extract -------------------------------------------
+
+int gfz_resident_load(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                      uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                      int delta_round, uint32_t flags, gfz_resident **out) {
+  if (!ctx || !rb || !out || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  if (rb->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "rulebook belongs to another ctx");
+  *out = nullptr;
+  return guarded(kExtractPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_enc;
+    stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
+    auto r = std::make_unique<gfz_resident>();
+    r->owner = ctx;
+    r->rulebook = rb;
+    gfz::resident_load(c, rb->rb, d_enc, n_encoded, encoded_lengths, n_trav, delta_round, r->res);
+    *out = r.release();
+  });
+}
+
+void gfz_resident_free(gfz_resident *r) {
+  if (!r)
+    return;
+  gfz::DeviceGuard g(r->owner->ctx->device());
+  (void)hipStreamSynchronize(r->owner->ctx->stream());
+  delete r;
+}
+
+int gfz_resident_info(const gfz_resident *r, uint64_t *n_trav, uint64_t *n_decoded) {
+  if (!r)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null resident");
+  if (n_trav)
+    *n_trav = r->res.n_trav;
+  if (n_decoded)
+    *n_decoded = r->res.n_out;
+  return GFZ_OK;
+}
+
+int gfz_resident_lengths(const gfz_resident *r, uint32_t *decoded_lengths) {
+  if (!r || !decoded_lengths)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  for (uint64_t t = 0; t < r->res.n_trav; ++t)
+    decoded_lengths[t] = (uint32_t)r->res.dec_len[t];
+  return GFZ_OK;
+}
+
+int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64_t n_ids,
+                int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *offsets,
+                uint64_t *n_out) {
+  if (!ctx || !r || (n_ids && !ids))
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  if (r->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "resident state belongs to another ctx");
+  return guarded(kExtractPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n_ids; ++i) {
+      if (ids[i] >= r->res.n_trav)
+        throw gfz::Error(GFZ_E_INVALID, "traversal index " + std::to_string(ids[i]) +
+                                            " out of range (" + std::to_string(r->res.n_trav) +
+                                            " stored)");
+      total += r->res.dec_len[ids[i]];
+    }
+    if (n_out)
+      *n_out = total;
+    if (total > out_cap)
+      throw gfz::Error(GFZ_E_INVALID, "output capacity " + std::to_string(out_cap) +
+                                          " < extracted size " + std::to_string(total));
+    if (total && !out)
+      throw gfz::Error(GFZ_E_INVALID, "null output buffer");
+    gfz::DBuf<int32_t> tmp;
+    int32_t *d_out = out;
+    if (!(flags & GFZ_OUTPUT_DEVICE)) {
+      tmp = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(total, 1));
+      d_out = tmp.get();
+    }
+    // Tile size: autotuned per size class, like the batch decoder.
+    uint32_t cls = 0;
+    for (uint64_t n = total; n >= 4; n >>= 2)
+      ++cls;
+    const std::string key = "extract_items/" + std::to_string(cls);
+    int items = c.tuned(key);
+    if (items < 0 && c.autotune() && total >= (uint64_t(1) << 20)) {
+      float best = 0;
+      for (uint32_t cand : {8u, 16u, 32u}) {
+        gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);  // warm
+        const float ms = c.time_ms([&] {
+          gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);
+        });
+        if (items < 0 || ms < best) {
+          best = ms;
+          items = (int)cand;
+        }
+      }
+      c.set_tuned(key, items);
+    }
+    gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets,
+                          items < 0 ? 16u : (uint32_t)items);
+    if (!(flags & GFZ_OUTPUT_DEVICE) && total) {
+      GFZ_HIP(hipMemcpyAsync(out, d_out, total * 4, hipMemcpyDeviceToHost, c.stream()));
       c.sync();
     }
   });

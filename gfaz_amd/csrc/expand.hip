@@ -28,6 +28,7 @@
 // sizes, then inverse-delta'd by segmented scans.
 #include "codec.hpp"
 #include "prims.hpp"
+#include "zexpand.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -36,15 +37,12 @@ namespace gfz {
 
 namespace {
 
+using namespace zx;
+
 constexpr uint32_t kExItems = 16;  // default output elements per thread
 constexpr uint32_t kSsItems = 16;
 constexpr uint32_t kSsTile = kBlock * kSsItems;
 constexpr uint32_t kSsScanThreads = 1024;
-
-__device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
-  const uint32_t a = uabs(v);
-  return a >= min_id && (uint64_t)(a - min_id) < R;
-}
 
 // ---- rule sizes, sums and depths --------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict__ f,
@@ -173,33 +171,6 @@ __global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t
     out[i] = in[i] + 1u;
 }
 
-// ---- symbols of an expansion ------------------------------------------------------
-// What one symbol writes at its local position q: base + Z[zidx0 + q], or
-// base + Z[zidx0 - q] when bit 31 of zz (reverse orientation) is set, with
-// zidx0 = zz & 0x7fffffff. A node ID (or the leading zero of a Z being built)
-// points at the two reserved zero entries at the end of Z, so every symbol
-// takes the same path. With kDiff (no inverse delta wanted) the value is
-// Z[j] - Z[j -/+ 1] instead, a body element.
-struct alignas(8) SymInfo {
-  uint32_t base;
-  uint32_t zz;
-};
-constexpr uint32_t kRevBit = 0x80000000u;
-
-__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint32_t *rsize,
-                                   const uint32_t *rsum, const uint64_t *zoff, uint32_t min_id) {
-  const uint32_t r = uabs(v) - min_id;
-  SymInfo s;
-  if (v > 0) {
-    s.base = pre;
-    s.zz = (uint32_t)(zoff[r] + 1);
-  } else {
-    s.base = delta ? pre - rsum[r] : pre;
-    s.zz = (uint32_t)(zoff[r] + rsize[r] - 1) | kRevBit;
-  }
-  return s;
-}
-
 // Per encoded symbol: expanded size, and the signed sum of its expansion.
 __global__ __launch_bounds__(kBlock) void k_sym_plan(const int32_t *__restrict__ enc, uint64_t n,
                                                      const uint32_t *__restrict__ rsize,
@@ -303,54 +274,9 @@ __global__ void k_tile_desc(const uint64_t *__restrict__ sym_off, uint64_t n_sym
   }
 }
 
-// Bitmap of symbol starts in a tile, with per-64-bit-word ranks, so the symbol
-// covering a position is rank + popcount.
-template <uint32_t kTile>
-struct StartMap {
-  unsigned long long bits[kTile / 64];
-  uint32_t rank[kTile / 64];
-
-  __device__ void clear() {
-    for (uint32_t w = threadIdx.x; w < kTile / 64; w += kBlock)
-      bits[w] = 0;
-  }
-  __device__ void mark(uint32_t rel) { atomicOr(&bits[rel >> 6], 1ull << (rel & 63)); }
-  // Call after a barrier that follows the last mark(); ends with a barrier.
-  __device__ void rank_words() {
-    static_assert(kTile / 64 <= kBlock, "one thread per bitmap word");
-    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    constexpr uint32_t kWords = kTile / 64;
-    __shared__ uint32_t wsum[kBlock / kWave];
-    uint32_t c = threadIdx.x < kWords ? (uint32_t)__popcll(bits[threadIdx.x]) : 0u;
-    uint32_t x = c;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t g = __shfl_up(x, d);
-      if (lane >= (uint32_t)d)
-        x += g;
-    }
-    if (lane == kWave - 1)
-      wsum[wid] = x;
-    __syncthreads();
-    uint32_t carry = 0;
-    for (uint32_t w = 0; w < wid; ++w)
-      carry += wsum[w];
-    if (threadIdx.x < kWords)
-      rank[threadIdx.x] = carry + x - c;
-    __syncthreads();
-  }
-  __device__ uint32_t symbol(uint32_t p) const {
-    const uint32_t w = p >> 6, b = p & 63;
-    const unsigned long long upto = b == 63 ? ~0ull : (2ull << b) - 1ull;
-    return rank[w] + (uint32_t)__popcll(bits[w] & upto) - 1u;
-  }
-};
-
-// Expansion through Z (see the file comment): out[base + o] for o in [0, n_out).
-// Each tile stages, per overlapping symbol, its base and the Z index of tile
-// position 0 (zstart, so position p reads Z[zstart +/- p]); then every thread
-// issues all of its Z gathers before its stores. Z and out may be one buffer
-// when the ranges read and written are disjoint.
+// Expansion through Z (see the file comment): out[base + o] for o in [0, n_out),
+// one tile per block, symbols found through the tile descriptors. Z and out may
+// be one buffer when the ranges read and written are disjoint.
 template <uint32_t kItems, bool kDiff>
 __global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__ syms,
                                                      const uint64_t *__restrict__ sym_off,
@@ -359,60 +285,12 @@ __global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__
                                                      const int32_t *Z, uint32_t zraw,
                                                      int32_t *out) {
   constexpr uint32_t kTile = kBlock * kItems;
-  __shared__ StartMap<kTile> map;
-  __shared__ uint32_t sbase[kTile];
-  __shared__ uint32_t sstart[kTile];
-  __shared__ uint32_t srev[kTile / 32];
+  __shared__ ZTileSmem<kItems> sm;
   const uint64_t o0 = (uint64_t)blockIdx.x * kTile;
   const uint32_t n_here = (uint32_t)(o0 + kTile < n_out ? kTile : n_out - o0);
   const TileDesc td = tdesc[blockIdx.x];
-  const uint64_t k0 = td.k0;
-  const uint32_t nsym = (uint32_t)(td.k_last - k0 + 1);
-  map.clear();
-  for (uint32_t w = threadIdx.x; w < kTile / 32; w += kBlock)
-    srev[w] = 0;
-  __syncthreads();
-  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
-    // Start of symbol s relative to the tile (<= 0 for the first one).
-    const uint32_t rel = s == 0 ? (uint32_t)(td.first_off - o0)
-                                : (uint32_t)(sym_off[k0 + s] - base - o0);
-    const SymInfo si = syms[k0 + s];
-    const bool rev = si.zz & kRevBit;
-    const uint32_t zi = si.zz & ~kRevBit;
-    sbase[s] = si.base;
-    sstart[s] = rev ? zi + rel : zi - rel;
-    if (rev)
-      atomicOr(&srev[s >> 5], 1u << (s & 31));
-    map.mark(s == 0 ? 0u : rel);
-  }
-  __syncthreads();
-  map.rank_words();
-  uint32_t jz[kItems], vb[kItems];
-  uint32_t revm = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < kItems; ++j) {
-    const uint32_t p = j * kBlock + threadIdx.x;
-    const uint32_t s = map.symbol(p < n_here ? p : n_here - 1);
-    const uint32_t rv = (srev[s >> 5] >> (s & 31)) & 1u;
-    revm |= rv << j;
-    jz[j] = p < n_here ? (rv ? sstart[s] - p : sstart[s] + p) : zraw;
-    vb[j] = sbase[s];
-  }
-  uint32_t z[kItems];
-#pragma unroll
-  for (uint32_t j = 0; j < kItems; ++j)
-    z[j] = (uint32_t)Z[jz[j]];
-  if (kDiff) {
-#pragma unroll
-    for (uint32_t j = 0; j < kItems; ++j)
-      z[j] -= (uint32_t)Z[((revm >> j) & 1u) ? jz[j] + 1 : jz[j] - 1];
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kItems; ++j) {
-    const uint32_t p = j * kBlock + threadIdx.x;
-    if (p < n_here)
-      out[base + o0 + p] = (int32_t)(vb[j] + z[j]);
-  }
+  expand_tile<kItems, kDiff>(sm, syms, sym_off, td.k0, (uint32_t)(td.k_last - td.k0 + 1),
+                             base + o0, n_here, Z, zraw, out + base + o0);
 }
 
 // Expansion by descending the rule tree (no Z): node IDs, no inverse delta.
@@ -668,8 +546,16 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   rb.rec = DBuf<RuleRec>(ctx.pool(), std::max<uint64_t>(R, 1));
   rb.size = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
   rb.sum = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
-  if (!R)
+  if (!R) {  // no rules: Z is just the two zeros node IDs point at
+    rb.zoff = DBuf<uint64_t>(ctx.pool(), 1);
+    fill_u64(ctx, rb.zoff.get(), 0, 1);
+    rb.z = DBuf<int32_t>(ctx.pool(), 2);
+    fill_u32(ctx, reinterpret_cast<uint32_t *>(rb.z.get()), 0, 2);
+    rb.z_raw = 1;
+    rb.has_bodies = true;
+    ctx.sync();
     return;
+  }
   DBuf<uint32_t> depth(ctx.pool(), R);
   // Passes run in batches without a host round trip: a pass that resolves
   // nothing leaves nothing for the later passes of its batch either.

@@ -1,0 +1,321 @@
+// Resident decoder: a rulebook with its Z bodies plus encoded traversals kept in
+// HBM, decoded per request - any subset of traversals in one persistent,
+// work-stealing launch. The reference decodes extract requests on the host
+// (src/workflows/extraction_workflow.cpp:106-133,206-424, which zstd-decodes the
+// whole column per call); here the per-symbol state is computed once at load:
+//   sym_off[k]  decoded start of encoded symbol k (whole stream, exclusive scan);
+//   syms[k]     base and Z start of symbol k, with the traversal's first inverse
+//               delta folded in (see expand.hip);
+//   grid[c]     the symbol covering decoded position c * kGrid (coarse index).
+// A request is a list of segments (one per traversal); segment tiles are numbered
+// consecutively and blocks take the next tile from an atomic counter. A block
+// finds its segment and the symbols covering its tile with wave-wide 64-way
+// searches (tile prefix, then the grid cell, then sym_off), then expands the tile
+// exactly as the batch decoder does.
+#include "resident.hpp"
+
+#include "prims.hpp"
+#include "zexpand.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace gfz {
+
+namespace {
+
+using namespace zx;
+
+constexpr uint64_t kGrid = 4096;  // decoded positions per grid cell
+
+// Largest k in [lo, hi) with a[k] <= target, given a non-decreasing and
+// a[lo] <= target. Called by all 64 lanes of one wave; every lane returns it.
+__device__ inline uint64_t wave_last_le(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi,
+                                        uint64_t target) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + kWave - 1) / kWave;
+    const uint64_t k = lo + lane * step;
+    const bool ok = k < hi && a[k] <= target;
+    const uint64_t m = __ballot(ok);  // lane 0 always holds
+    const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
+    lo = lo + (uint64_t)last * step;
+    hi = hi < lo + step ? hi : lo + step;
+  }
+  return lo;
+}
+
+__global__ void k_grid(const uint64_t *__restrict__ sym_off, uint64_t n_sym,
+                       uint64_t *__restrict__ grid) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_sym;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = sym_off[k], hi = sym_off[k + 1];
+    for (uint64_t c = (lo + kGrid - 1) / kGrid; c * kGrid < hi; ++c)
+      grid[c] = k;
+  }
+}
+
+template <uint32_t kItems, bool kDiff>
+__global__ __launch_bounds__(kBlock) void k_expand_batch(
+    const SegDesc *__restrict__ segs, uint64_t n_segs, const uint64_t *__restrict__ tile_prefix,
+    uint64_t total_tiles, const SymInfo *__restrict__ syms, const uint64_t *__restrict__ sym_off,
+    const uint64_t *__restrict__ grid, uint64_t n_cells, const int32_t *Z, uint32_t zraw,
+    int32_t *out, unsigned long long *counter) {
+  constexpr uint32_t kTile = kBlock * kItems;
+  __shared__ ZTileSmem<kItems> sm;
+  __shared__ unsigned long long s_tile;
+  __shared__ uint64_t s_k0, s_kl, s_o0, s_dst;
+  __shared__ uint32_t s_n;
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_tile = atomicAdd(counter, 1ull);
+    __syncthreads();
+    const uint64_t t = s_tile;
+    if (t >= total_tiles)
+      break;  // uniform across the block
+    if (threadIdx.x < kWave) {
+      const uint64_t seg = wave_last_le(tile_prefix, 0, n_segs, t);
+      const SegDesc sd = segs[seg];
+      const uint64_t lt = t - tile_prefix[seg];
+      const uint64_t o0 = sd.dec_begin + lt * kTile;
+      const uint64_t seg_end = sd.dec_begin + sd.len;
+      const uint64_t end = o0 + kTile < seg_end ? o0 + kTile : seg_end;
+      const uint64_t c0 = o0 / kGrid, c1 = (end - 1) / kGrid + 1;
+      const uint64_t g0 = grid[c0], g1 = (c1 < n_cells ? grid[c1] : sd.k_hi - 1) + 1;
+      const uint64_t lo = sd.k_lo > g0 ? sd.k_lo : g0;
+      const uint64_t hi = sd.k_hi < g1 ? sd.k_hi : g1;
+      const uint64_t k0 = wave_last_le(sym_off, lo, hi, o0);
+      const uint64_t kl = wave_last_le(sym_off, k0, hi, end - 1);
+      if (threadIdx.x == 0) {
+        s_k0 = k0;
+        s_kl = kl;
+        s_o0 = o0;
+        s_n = (uint32_t)(end - o0);
+        s_dst = sd.dst + lt * kTile;
+      }
+    }
+    __syncthreads();
+    expand_tile<kItems, kDiff>(sm, syms, sym_off, s_k0, (uint32_t)(s_kl - s_k0 + 1), s_o0, s_n,
+                               Z, zraw, out + s_dst);
+  }
+}
+
+__global__ void k_sym_plan_r(const int32_t *__restrict__ enc, uint64_t n,
+                             const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
+                             uint64_t R, uint32_t min_id, uint32_t *__restrict__ sz,
+                             int32_t *__restrict__ ssum) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t v = enc[i];
+    if (is_rule(v, min_id, R)) {
+      const uint32_t r = uabs(v) - min_id;
+      sz[i] = rsize[r];
+      ssum[i] = (int32_t)(v > 0 ? rsum[r] : (uint32_t)0 - rsum[r]);
+    } else {
+      sz[i] = 1u;
+      ssum[i] = v;
+    }
+  }
+}
+
+__global__ void k_sym_info_r(const int32_t *__restrict__ enc, uint64_t n,
+                             const int32_t *__restrict__ incl, const int32_t *__restrict__ ssum,
+                             bool delta, const uint32_t *__restrict__ rsize,
+                             const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff,
+                             uint64_t R, uint32_t min_id, uint32_t zraw,
+                             SymInfo *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int32_t v = enc[i];
+    const uint32_t pre = delta ? (uint32_t)incl[i] - (uint32_t)ssum[i] : 0u;
+    out[i] = is_rule(v, min_id, R) ? rule_sym(v, pre, delta, rsize, rsum, zoff, min_id)
+                                   : SymInfo{pre + (uint32_t)v, zraw};
+  }
+}
+
+uint32_t grid_for(Ctx &ctx, uint64_t n) {
+  return (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(n, (uint64_t)kBlock * 4), (uint64_t)ctx.num_cus() * 8));
+}
+
+template <uint32_t kItems>
+void launch_batch(Ctx &ctx, bool diff, const Resident &res, const SegDesc *segs, uint64_t n_segs,
+                  const uint64_t *tile_prefix, uint64_t total_tiles, int32_t *out,
+                  unsigned long long *counter, uint32_t blocks) {
+  const Rulebook &rb = *res.rb;
+  if (diff)
+    hipLaunchKernelGGL((k_expand_batch<kItems, true>), dim3(blocks), dim3(kBlock), 0,
+                       ctx.stream(), segs, n_segs, tile_prefix, total_tiles, res.syms.get(),
+                       res.sym_off.get(), res.grid.get(), res.n_cells, rb.z.get(), rb.z_raw, out,
+                       counter);
+  else
+    hipLaunchKernelGGL((k_expand_batch<kItems, false>), dim3(blocks), dim3(kBlock), 0,
+                       ctx.stream(), segs, n_segs, tile_prefix, total_tiles, res.syms.get(),
+                       res.sym_off.get(), res.grid.get(), res.n_cells, rb.z.get(), rb.z_raw, out,
+                       counter);
+  GFZ_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                   const uint32_t *enc_lengths, uint64_t n_trav, int delta_round, Resident &res) {
+  if (!rb.has_bodies && rb.n_rules)
+    throw Error(kErrInvalid,
+                "resident decoding needs the rule bodies in HBM (this rulebook is expanded by "
+                "descent: its depths are not monotone in rule ID or its bodies exceed the budget)");
+  res = Resident();
+  res.rb = &rb;
+  res.delta_round = delta_round < 0 ? 0 : delta_round;
+  res.n_trav = n_trav;
+  res.n_enc = n_enc;
+  res.enc_start.resize(n_trav);
+  res.enc_len.assign(enc_lengths, enc_lengths + n_trav);
+  uint64_t total = 0;
+  std::vector<uint64_t> starts;
+  for (uint64_t t = 0; t < n_trav; ++t) {
+    res.enc_start[t] = total;
+    if (enc_lengths[t])
+      starts.push_back(total);
+    total += enc_lengths[t];
+  }
+  if (total != n_enc)
+    throw Error(kErrCorrupt, "flattened sequence block is truncated (lengths sum " +
+                                 std::to_string(total) + ", block holds " +
+                                 std::to_string(n_enc) + ")");
+  res.sym_off = DBuf<uint64_t>(ctx.pool(), n_enc + 1);
+  res.syms = DBuf<SymInfo>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+  const bool delta = res.delta_round > 0;
+  {
+    DBuf<uint32_t> sz(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    DBuf<int32_t> ssum(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    DBuf<int32_t> incl(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    if (n_enc) {
+      hipLaunchKernelGGL(k_sym_plan_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, n_enc, rb.size.get(), rb.sum.get(), rb.n_rules, rb.min_id,
+                         sz.get(), ssum.get());
+      GFZ_CHECK_LAUNCH();
+    }
+    exclusive_scan(ctx, sz.get(), res.sym_off.get(), n_enc);
+    if (delta && n_enc) {
+      const uint64_t w = ceil_div(n_enc, 32);
+      DBuf<uint32_t> heads(ctx.pool(), w);
+      fill_u32(ctx, heads.get(), 0, w);
+      DBuf<uint64_t> d_st(ctx.pool(), std::max<size_t>(starts.size(), 1));
+      if (!starts.empty()) {
+        GFZ_HIP(hipMemcpyAsync(d_st.get(), starts.data(), starts.size() * 8,
+                               hipMemcpyHostToDevice, ctx.stream()));
+        set_head_bits(ctx, d_st.get(), starts.size(), heads.get(), n_enc);
+      }
+      GFZ_HIP(hipMemcpyAsync(incl.get(), ssum.get(), n_enc * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+      segmented_prefix_sum(ctx, incl.get(), n_enc, heads.get());
+      ctx.sync();  // `starts` (pageable source) and the scratch buffers go out of scope
+    }
+    if (n_enc) {
+      hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, n_enc, incl.get(), ssum.get(), delta, rb.size.get(), rb.sum.get(),
+                         rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get());
+      GFZ_CHECK_LAUNCH();
+    }
+    ctx.sync();
+  }
+  // Decoded starts and lengths per traversal (host), and the total.
+  std::vector<uint64_t> h_off(n_enc + 1);
+  GFZ_HIP(hipMemcpyAsync(h_off.data(), res.sym_off.get(), (n_enc + 1) * 8, hipMemcpyDeviceToHost,
+                         ctx.stream()));
+  ctx.sync();
+  res.n_out = h_off[n_enc];
+  res.dec_start.resize(n_trav);
+  res.dec_len.resize(n_trav);
+  for (uint64_t t = 0; t < n_trav; ++t) {
+    res.dec_start[t] = h_off[res.enc_start[t]];
+    res.dec_len[t] = h_off[res.enc_start[t] + res.enc_len[t]] - res.dec_start[t];
+  }
+  // Coarse index; the last entry is a sentinel for ranges that end the stream.
+  res.n_cells = ceil_div(std::max<uint64_t>(res.n_out, 1), kGrid) + 1;
+  res.grid = DBuf<uint64_t>(ctx.pool(), res.n_cells);
+  fill_u64(ctx, res.grid.get(), n_enc ? n_enc - 1 : 0, res.n_cells);
+  if (n_enc) {
+    hipLaunchKernelGGL(k_grid, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
+                       res.sym_off.get(), n_enc, res.grid.get());
+    GFZ_CHECK_LAUNCH();
+  }
+  ctx.sync();
+}
+
+uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, uint64_t n_ids,
+                          int32_t *d_out, uint64_t out_cap, uint64_t *offsets,
+                          uint32_t items) {
+  std::vector<SegDesc> segs;
+  std::vector<uint64_t> tile_prefix{0}, seg_dst;
+  const uint64_t tile = (uint64_t)kBlock * items;
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n_ids; ++i) {
+    const uint64_t t = ids[i];
+    if (t >= res.n_trav)
+      throw Error(kErrInvalid, "traversal index " + std::to_string(t) + " out of range (" +
+                                   std::to_string(res.n_trav) + " stored)");
+    if (offsets)
+      offsets[i] = total;
+    const uint64_t len = res.dec_len[t];
+    if (len) {
+      SegDesc sd;
+      sd.dec_begin = res.dec_start[t];
+      sd.len = len;
+      sd.dst = total;
+      sd.k_lo = res.enc_start[t];
+      sd.k_hi = res.enc_start[t] + res.enc_len[t];
+      segs.push_back(sd);
+      tile_prefix.push_back(tile_prefix.back() + ceil_div(len, tile));
+      seg_dst.push_back(total);
+    }
+    total += len;
+  }
+  if (offsets)
+    offsets[n_ids] = total;
+  if (total > out_cap)
+    throw Error(kErrInvalid, "output capacity " + std::to_string(out_cap) + " < extracted size " +
+                                 std::to_string(total));
+  if (!total)
+    return 0;
+  const uint64_t n_segs = segs.size(), total_tiles = tile_prefix.back();
+  // One staging upload: descriptors, tile prefix, counter.
+  const size_t seg_bytes = n_segs * sizeof(SegDesc);
+  const size_t pre_bytes = (n_segs + 1) * 8;
+  DBuf<uint8_t> meta(ctx.pool(), seg_bytes + pre_bytes + 8);
+  std::vector<uint8_t> h(seg_bytes + pre_bytes + 8, 0);
+  std::memcpy(h.data(), segs.data(), seg_bytes);
+  std::memcpy(h.data() + seg_bytes, tile_prefix.data(), pre_bytes);
+  GFZ_HIP(hipMemcpyAsync(meta.get(), h.data(), h.size(), hipMemcpyHostToDevice, ctx.stream()));
+  const SegDesc *d_segs = reinterpret_cast<const SegDesc *>(meta.get());
+  const uint64_t *d_pre = reinterpret_cast<const uint64_t *>(meta.get() + seg_bytes);
+  auto *counter = reinterpret_cast<unsigned long long *>(meta.get() + seg_bytes + pre_bytes);
+  const uint32_t per_cu = items >= 32 ? 2 : 4;
+  const uint32_t blocks =
+      (uint32_t)std::min<uint64_t>(total_tiles, (uint64_t)ctx.num_cus() * per_cu);
+  const bool diff = res.delta_round == 0;
+  ctx.timed("extract", [&] {
+    switch (items) {
+      case 8: launch_batch<8>(ctx, diff, res, d_segs, n_segs, d_pre, total_tiles, d_out, counter, blocks); break;
+      case 32: launch_batch<32>(ctx, diff, res, d_segs, n_segs, d_pre, total_tiles, d_out, counter, blocks); break;
+      default: launch_batch<16>(ctx, diff, res, d_segs, n_segs, d_pre, total_tiles, d_out, counter, blocks); break;
+    }
+  });
+  ctx.add_bytes("extract", 4.0 * total);
+  if (res.delta_round >= 2) {
+    const uint64_t words = ceil_div(total, 32);
+    DBuf<uint32_t> heads(ctx.pool(), words);
+    fill_u32(ctx, heads.get(), 0, words);
+    DBuf<uint64_t> d_st(ctx.pool(), seg_dst.size());
+    GFZ_HIP(hipMemcpyAsync(d_st.get(), seg_dst.data(), seg_dst.size() * 8, hipMemcpyHostToDevice,
+                           ctx.stream()));
+    set_head_bits(ctx, d_st.get(), seg_dst.size(), heads.get(), total);
+    for (int r = 1; r < res.delta_round; ++r)
+      segmented_prefix_sum(ctx, d_out, total, heads.get());
+  }
+  ctx.sync();  // host staging vectors go out of scope
+  return total;
+}
+
+}  // namespace gfz

@@ -1,0 +1,47 @@
+// Resident decoder state (rulebook bodies + encoded traversals in HBM) and the
+// batched, work-stealing extraction over any subset of traversals.
+#pragma once
+
+#include "codec.hpp"
+#include "zexpand.hpp"
+
+#include <vector>
+
+namespace gfz {
+
+// One traversal of a request: decoded range [dec_begin, dec_begin + len) of the
+// resident stream, written at out[dst ..), its symbols in [k_lo, k_hi).
+struct SegDesc {
+  uint64_t dec_begin;
+  uint64_t len;
+  uint64_t dst;
+  uint64_t k_lo;
+  uint64_t k_hi;
+};
+
+struct Resident {
+  const Rulebook *rb = nullptr;  // not owned; must outlive the resident state
+  int delta_round = 1;
+  uint64_t n_trav = 0, n_enc = 0, n_out = 0;
+  DBuf<zx::SymInfo> syms;    // per encoded symbol
+  DBuf<uint64_t> sym_off;    // n_enc + 1 decoded starts
+  DBuf<uint64_t> grid;       // symbol covering each kGrid-th decoded position
+  uint64_t n_cells = 0;
+  std::vector<uint64_t> enc_start, dec_start, dec_len;
+  std::vector<uint32_t> enc_len;
+};
+
+// d_enc: device; enc_lengths: host. The encoded stream may be freed afterwards.
+void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
+                   const uint32_t *enc_lengths, uint64_t n_trav, int delta_round,
+                   Resident &res);
+
+// Decode traversals ids[0..n_ids) back to back into d_out (device, out_cap
+// int32); offsets (host, n_ids + 1, may be null) receives where each starts.
+// Returns the total; synchronises the ctx stream. items = output elements per
+// thread of the expansion tile (8, 16 or 32).
+uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, uint64_t n_ids,
+                          int32_t *d_out, uint64_t out_cap, uint64_t *offsets,
+                          uint32_t items = 16);
+
+}  // namespace gfz

@@ -152,6 +152,34 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
                               int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
                               uint32_t *out_lengths, int *length_mismatch);
 
+/* ---- resident decoding: extract -------------------------------------------
+ * gfz_resident_load keeps a rulebook's encoded traversals decodable in HBM: the
+ * per-symbol offsets and expansion state are computed once, so a request decodes
+ * any subset of traversals in one launch. gfz_extract replaces the decode part of
+ * extract_path_lines_by_name / extract_walk_lines(_by_name)
+ * (src/workflows/extraction_workflow.cpp:106-133,206-424: decode_sequence on the
+ * selected traversals after a zstd decode of the whole column on every call); the
+ * name or tuple -> index lookup and the P/W line text stay with the caller.
+ * Traversal indices count paths first, then walks (the flattened order). The
+ * rulebook must outlive the resident state and needs its bodies in HBM
+ * (gfz_rulebook_info has_bodies; otherwise GFZ_E_INVALID). Errors carry the
+ * reference's "Extraction workflow error: " prefix (extraction_workflow.cpp:15). */
+typedef struct gfz_resident gfz_resident;
+
+int gfz_resident_load(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                      uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                      int delta_round, uint32_t flags, gfz_resident **out);
+void gfz_resident_free(gfz_resident *r);
+int gfz_resident_info(const gfz_resident *r, uint64_t *n_trav, uint64_t *n_decoded);
+/* Decoded length of every stored traversal (n_trav uint32). */
+int gfz_resident_lengths(const gfz_resident *r, uint32_t *decoded_lengths);
+/* Decode traversals ids[0..n_ids) back to back into out (host, or device with
+ * GFZ_OUTPUT_DEVICE; out_cap int32). offsets (host, n_ids + 1, may be NULL)
+ * receives each one's start; *n_out the total. Repeated ids are decoded again. */
+int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64_t n_ids,
+                int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *offsets,
+                uint64_t *n_out);
+
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before
  * serialisation, producing the same bytes, so a caller can fill CompressedData
