@@ -36,6 +36,17 @@ class _Block(ctypes.Structure):
                 ("payload_size", _u64)]
 
 
+class _Layer(ctypes.Structure):
+    _fields_ = [("start_id", ctypes.c_uint32), ("end_id", ctypes.c_uint32),
+                ("row_offset", _u64)]
+
+
+class _EncodeOut(ctypes.Structure):
+    _fields_ = [("encoded", ctypes.POINTER(ctypes.c_int32)), ("n_encoded", _u64),
+                ("encoded_lengths", ctypes.POINTER(ctypes.c_uint32)), ("n_trav", _u64),
+                ("flags", ctypes.c_uint32), ("internal", _vp)]
+
+
 class _CompressOut(ctypes.Structure):
     _fields_ = [
         ("encoded", ctypes.POINTER(ctypes.c_int32)),
@@ -90,6 +101,9 @@ _SIGNATURES = {
     "gfz_resident_lengths": ([_vp, _vp], ctypes.c_int),
     "gfz_extract": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, _vp,
                      ctypes.POINTER(_u64)], ctypes.c_int),
+    "gfz_encode_with_rules": ([_vp, _vp, _u64, _vp, _u64, _vp, _vp, _u64, _vp, _u64, ctypes.c_int,
+                               ctypes.c_uint32, _vp], ctypes.c_int),
+    "gfz_encode_free": ([_vp], None),
     "gfz_calibration_copy": ([_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double)],
                              ctypes.c_int),
     "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
@@ -556,3 +570,31 @@ class Resident:
             self.close()
         except Exception:
             pass
+
+
+# ---- add-haplotypes encoding ------------------------------------------------------
+
+def encode_with_rules(ctx: Context, nodes, lengths, rules_first, rules_second, layers,
+                      delta_round: int = 1):
+    """Encode new traversals with stored rules (gfz_encode_with_rules).
+    layers: [(start_id, end_id, row_offset)]. Returns (encoded, encoded_lengths)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    rf = np.ascontiguousarray(rules_first, dtype=np.int32)
+    rs = np.ascontiguousarray(rules_second, dtype=np.int32)
+    if rf.size != rs.size:
+        raise ValueError("rule columns differ in length")
+    arr = (_Layer * max(1, len(layers)))(*[_Layer(int(a), int(b), int(c)) for a, b, c in layers])
+    out = _EncodeOut()
+    _check(lib().gfz_encode_with_rules(ctx.handle, nodes.ctypes.data, nodes.size,
+                                       lengths.ctypes.data, lengths.size, rf.ctypes.data,
+                                       rs.ctypes.data, rf.size, arr, len(layers),
+                                       int(delta_round), 0, ctypes.byref(out)))
+    try:
+        enc = (np.ctypeslib.as_array(out.encoded, shape=(out.n_encoded,)).copy()
+               if out.n_encoded else np.zeros(0, np.int32))
+        lens = (np.ctypeslib.as_array(out.encoded_lengths, shape=(out.n_trav,)).copy()
+                if out.n_trav else np.zeros(0, np.uint32))
+    finally:
+        lib().gfz_encode_free(ctypes.byref(out))
+    return enc, lens

@@ -4,6 +4,7 @@
 #include "gfaz_amd.h"
 
 #include "codec.hpp"
+#include "encode.hpp"
 #include "prims.hpp"
 
 #include <cstdlib>
@@ -387,6 +388,78 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
       c.sync();
     }
   });
+}
+
+// ---- add-haplotypes encoding ---------------------------------------------------
+
+namespace {
+constexpr const char *kAddPrefix = "Add-haplotypes workflow error: ";
+struct EncodeInternal {
+  gfz::EncodeResult res;
+};
+}  // namespace
+
+int gfz_encode_with_rules(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                          const uint32_t *lengths, uint64_t n_trav, const int32_t *rules_first,
+                          const int32_t *rules_second, uint64_t n_rules, const gfz_layer *layers,
+                          uint64_t n_layers, int delta_round, uint32_t flags,
+                          gfz_encode_out *out) {
+  if (!ctx || !out || (n_trav && !lengths) || (n_nodes && !nodes) ||
+      (n_rules && (!rules_first || !rules_second)) || (n_layers && !layers))
+    return fail(GFZ_E_INVALID, std::string(kAddPrefix) + "null argument");
+  std::memset(out, 0, sizeof(*out));
+  return guarded(kAddPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> sn, sf, ss;
+    const int32_t *dn = nodes, *df = rules_first, *ds = rules_second;
+    if (!(flags & GFZ_INPUT_DEVICE)) {
+      sn = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_nodes, 1));
+      sf = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_rules, 1));
+      ss = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_rules, 1));
+      if (n_nodes)
+        GFZ_HIP(hipMemcpyAsync(sn.get(), nodes, n_nodes * 4, hipMemcpyHostToDevice, c.stream()));
+      if (n_rules) {
+        GFZ_HIP(hipMemcpyAsync(sf.get(), rules_first, n_rules * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+        GFZ_HIP(hipMemcpyAsync(ss.get(), rules_second, n_rules * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+      }
+      dn = sn.get();
+      df = sf.get();
+      ds = ss.get();
+    }
+    std::vector<gfz::EncodeLayer> ls(n_layers);
+    for (uint64_t l = 0; l < n_layers; ++l)
+      ls[l] = gfz::EncodeLayer{layers[l].start_id, layers[l].end_id, layers[l].row_offset};
+    auto in = std::make_unique<EncodeInternal>();
+    gfz::encode_with_rules(c, dn, n_nodes, lengths, n_trav, df, ds, n_rules, ls.data(), n_layers,
+                           delta_round < 0 ? 0 : delta_round, in->res);
+    out->n_encoded = in->res.n_encoded;
+    out->n_trav = n_trav;
+    out->flags = flags;
+    out->encoded_lengths = static_cast<uint32_t *>(std::malloc(std::max<uint64_t>(n_trav, 1) * 4));
+    if (!out->encoded_lengths)
+      throw std::bad_alloc();
+    std::memcpy(out->encoded_lengths, in->res.lengths.data(), n_trav * 4);
+    if (flags & GFZ_OUTPUT_DEVICE) {
+      out->encoded = in->res.encoded.get();
+      out->internal = in.release();
+    } else {
+      out->encoded = host_copy(c, in->res.encoded.get(), in->res.n_encoded);
+    }
+  });
+}
+
+void gfz_encode_free(gfz_encode_out *out) {
+  if (!out)
+    return;
+  std::free(out->encoded_lengths);
+  if (out->internal)
+    delete static_cast<EncodeInternal *>(out->internal);
+  else
+    std::free(out->encoded);
+  std::memset(out, 0, sizeof(*out));
 }
 
 // ---- resident decoding / extract -------------------------------------------

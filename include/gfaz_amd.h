@@ -180,6 +180,39 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
                 int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *offsets,
                 uint64_t *n_out);
 
+/* ---- add-haplotypes: encoding with a stored rulebook ----------------------
+ * Replaces the encode step of add_haplotypes (src/workflows/
+ * add_haplotypes_workflow.cpp:336-379,447-465 with src/grammar/path_encoder.cpp:
+ * 13-84): delta_round transforms, the collision check (largest |value| must stay
+ * below layers[0].start_id when layers exist, else GFZ_E_OVERFLOW with the
+ * reference's message), then per layer the shift-reduce encoder over a table
+ * pack(first, second) -> start_id + i of the layer's rows as stored (first row
+ * wins on duplicate keys). Layer rows: row_offset = flattened_offset / k of the
+ * LayerRuleRange (k = 0: row 0). Rows past the columns: GFZ_E_CORRUPT ("rule
+ * column is truncated"). nodes and the rule columns are host or device arrays
+ * per GFZ_INPUT_DEVICE; the encoded stream is returned per GFZ_OUTPUT_DEVICE. */
+typedef struct gfz_layer {
+  uint32_t start_id;
+  uint32_t end_id;
+  uint64_t row_offset;
+} gfz_layer;
+
+typedef struct gfz_encode_out {
+  int32_t *encoded;           /* n_encoded symbols; host or device per flags */
+  uint64_t n_encoded;
+  uint32_t *encoded_lengths;  /* host, one per input traversal */
+  uint64_t n_trav;
+  uint32_t flags;
+  void *internal;
+} gfz_encode_out;
+
+int gfz_encode_with_rules(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                          const uint32_t *lengths, uint64_t n_trav, const int32_t *rules_first,
+                          const int32_t *rules_second, uint64_t n_rules, const gfz_layer *layers,
+                          uint64_t n_layers, int delta_round, uint32_t flags,
+                          gfz_encode_out *out);
+void gfz_encode_free(gfz_encode_out *out);
+
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before
  * serialisation, producing the same bytes, so a caller can fill CompressedData
