@@ -244,11 +244,11 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
   ctx.sync();
 }
 
-uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, uint64_t n_ids,
-                          int32_t *d_out, uint64_t out_cap, uint64_t *offsets,
-                          uint32_t items) {
-  std::vector<SegDesc> segs;
-  std::vector<uint64_t> tile_prefix{0}, seg_dst;
+uint64_t extract_plan(const Resident &res, const uint64_t *ids, uint64_t n_ids, uint32_t items,
+                      ExtractPlan &plan, uint64_t *offsets) {
+  plan = ExtractPlan();
+  plan.items = items;
+  plan.tile_prefix.push_back(0);
   const uint64_t tile = (uint64_t)kBlock * items;
   uint64_t total = 0;
   for (uint64_t i = 0; i < n_ids; ++i) {
@@ -260,37 +260,41 @@ uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, ui
       offsets[i] = total;
     const uint64_t len = res.dec_len[t];
     if (len) {
-      SegDesc sd;
-      sd.dec_begin = res.dec_start[t];
-      sd.len = len;
-      sd.dst = total;
-      sd.k_lo = res.enc_start[t];
-      sd.k_hi = res.enc_start[t] + res.enc_len[t];
-      segs.push_back(sd);
-      tile_prefix.push_back(tile_prefix.back() + ceil_div(len, tile));
-      seg_dst.push_back(total);
+      plan.segs.push_back(SegDesc{res.dec_start[t], len, total, res.enc_start[t],
+                                  res.enc_start[t] + res.enc_len[t]});
+      plan.tile_prefix.push_back(plan.tile_prefix.back() + ceil_div(len, tile));
+      plan.seg_dst.push_back(total);
     }
     total += len;
   }
   if (offsets)
     offsets[n_ids] = total;
-  if (total > out_cap)
-    throw Error(kErrInvalid, "output capacity " + std::to_string(out_cap) + " < extracted size " +
-                                 std::to_string(total));
-  if (!total)
-    return 0;
-  const uint64_t n_segs = segs.size(), total_tiles = tile_prefix.back();
-  // One staging upload: descriptors, tile prefix, counter.
-  const size_t seg_bytes = n_segs * sizeof(SegDesc);
-  const size_t pre_bytes = (n_segs + 1) * 8;
-  DBuf<uint8_t> meta(ctx.pool(), seg_bytes + pre_bytes + 8);
-  std::vector<uint8_t> h(seg_bytes + pre_bytes + 8, 0);
-  std::memcpy(h.data(), segs.data(), seg_bytes);
-  std::memcpy(h.data() + seg_bytes, tile_prefix.data(), pre_bytes);
-  GFZ_HIP(hipMemcpyAsync(meta.get(), h.data(), h.size(), hipMemcpyHostToDevice, ctx.stream()));
-  const SegDesc *d_segs = reinterpret_cast<const SegDesc *>(meta.get());
-  const uint64_t *d_pre = reinterpret_cast<const uint64_t *>(meta.get() + seg_bytes);
-  auto *counter = reinterpret_cast<unsigned long long *>(meta.get() + seg_bytes + pre_bytes);
+  plan.total = total;
+  return total;
+}
+
+size_t extract_meta_bytes(const ExtractPlan &plan) {
+  return plan.segs.size() * sizeof(SegDesc) + plan.tile_prefix.size() * 8 + 8 +
+         plan.seg_dst.size() * 8;
+}
+
+void extract_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, uint8_t *h_stage,
+                     uint8_t *d_meta, int32_t *d_out) {
+  if (!plan.total)
+    return;
+  const uint64_t n_segs = plan.segs.size(), total_tiles = plan.tile_prefix.back();
+  const size_t seg_bytes = n_segs * sizeof(SegDesc), pre_bytes = (n_segs + 1) * 8;
+  const size_t dst_off = seg_bytes + pre_bytes + 8;
+  std::memcpy(h_stage, plan.segs.data(), seg_bytes);
+  std::memcpy(h_stage + seg_bytes, plan.tile_prefix.data(), pre_bytes);
+  std::memset(h_stage + seg_bytes + pre_bytes, 0, 8);  // work-stealing counter
+  std::memcpy(h_stage + dst_off, plan.seg_dst.data(), plan.seg_dst.size() * 8);
+  GFZ_HIP(hipMemcpyAsync(d_meta, h_stage, extract_meta_bytes(plan), hipMemcpyHostToDevice,
+                         ctx.stream()));
+  const SegDesc *d_segs = reinterpret_cast<const SegDesc *>(d_meta);
+  const uint64_t *d_pre = reinterpret_cast<const uint64_t *>(d_meta + seg_bytes);
+  auto *counter = reinterpret_cast<unsigned long long *>(d_meta + seg_bytes + pre_bytes);
+  const uint32_t items = plan.items;
   const uint32_t per_cu = items >= 32 ? 2 : 4;
   const uint32_t blocks =
       (uint32_t)std::min<uint64_t>(total_tiles, (uint64_t)ctx.num_cus() * per_cu);
@@ -302,20 +306,133 @@ uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, ui
       default: launch_batch<16>(ctx, diff, res, d_segs, n_segs, d_pre, total_tiles, d_out, counter, blocks); break;
     }
   });
-  ctx.add_bytes("extract", 4.0 * total);
+  ctx.add_bytes("extract", 4.0 * plan.total);
   if (res.delta_round >= 2) {
-    const uint64_t words = ceil_div(total, 32);
-    DBuf<uint32_t> heads(ctx.pool(), words);
+    // Further inverse deltas: segmented scans restarted at every traversal.
+    const uint64_t words = ceil_div(plan.total, 32);
+    DBuf<uint32_t> heads(ctx.pool(), words);  // stream-ordered reuse: no sync needed
     fill_u32(ctx, heads.get(), 0, words);
-    DBuf<uint64_t> d_st(ctx.pool(), seg_dst.size());
-    GFZ_HIP(hipMemcpyAsync(d_st.get(), seg_dst.data(), seg_dst.size() * 8, hipMemcpyHostToDevice,
-                           ctx.stream()));
-    set_head_bits(ctx, d_st.get(), seg_dst.size(), heads.get(), total);
+    set_head_bits(ctx, reinterpret_cast<const uint64_t *>(d_meta + dst_off), plan.seg_dst.size(),
+                  heads.get(), plan.total);
     for (int r = 1; r < res.delta_round; ++r)
-      segmented_prefix_sum(ctx, d_out, total, heads.get());
+      segmented_prefix_sum(ctx, d_out, plan.total, heads.get());
   }
-  ctx.sync();  // host staging vectors go out of scope
+}
+
+uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, uint64_t n_ids,
+                          int32_t *d_out, uint64_t out_cap, uint64_t *offsets, uint32_t items) {
+  ExtractPlan plan;
+  const uint64_t total = extract_plan(res, ids, n_ids, items, plan, offsets);
+  if (total > out_cap)
+    throw Error(kErrInvalid, "output capacity " + std::to_string(out_cap) + " < extracted size " +
+                                 std::to_string(total));
+  if (!total)
+    return 0;
+  const size_t bytes = extract_meta_bytes(plan);
+  DBuf<uint8_t> meta(ctx.pool(), bytes);
+  std::vector<uint8_t> stage(bytes);
+  extract_enqueue(ctx, res, plan, stage.data(), meta.get(), d_out);
+  ctx.sync();  // the pageable staging vector goes out of scope
   return total;
+}
+
+namespace {
+
+__global__ void k_add_offset(const uint64_t *__restrict__ in, uint64_t n, uint64_t add,
+                             uint64_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] + add;
+}
+
+template <class T>
+void grow(Ctx &ctx, DBuf<T> &buf, uint64_t used, uint64_t need) {
+  if (need <= buf.size())
+    return;
+  DBuf<T> nb(ctx.pool(), std::max<uint64_t>(need, 2 * (uint64_t)buf.size()));
+  if (used)
+    GFZ_HIP(hipMemcpyAsync(nb.get(), buf.get(), used * sizeof(T), hipMemcpyDeviceToDevice,
+                           ctx.stream()));
+  buf = std::move(nb);
+}
+
+}  // namespace
+
+void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_new,
+                     const uint32_t *enc_lengths, uint64_t n_trav_new) {
+  const Rulebook &rb = *res.rb;
+  uint64_t total = 0;
+  std::vector<uint64_t> starts;
+  for (uint64_t t = 0; t < n_trav_new; ++t) {
+    if (enc_lengths[t])
+      starts.push_back(total);
+    total += enc_lengths[t];
+  }
+  if (total != n_new)
+    throw Error(kErrInvalid, "appended lengths sum " + std::to_string(total) + " != " +
+                                 std::to_string(n_new) + " symbols");
+  const uint64_t e0 = res.n_enc, out0 = res.n_out;
+  grow(ctx, res.syms, e0, e0 + n_new);
+  grow(ctx, res.sym_off, e0 + 1, e0 + n_new + 1);
+  std::vector<uint64_t> h_off(n_new + 1, 0);
+  if (n_new) {
+    DBuf<uint32_t> sz(ctx.pool(), n_new);
+    DBuf<int32_t> ssum(ctx.pool(), n_new), incl(ctx.pool(), n_new);
+    DBuf<uint64_t> off(ctx.pool(), n_new + 1);
+    hipLaunchKernelGGL(k_sym_plan_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
+                       d_enc, n_new, rb.size.get(), rb.sum.get(), rb.n_rules, rb.min_id, sz.get(),
+                       ssum.get());
+    GFZ_CHECK_LAUNCH();
+    exclusive_scan(ctx, sz.get(), off.get(), n_new);
+    hipLaunchKernelGGL(k_add_offset, dim3(grid_for(ctx, n_new + 1)), dim3(kBlock), 0,
+                       ctx.stream(), off.get(), n_new + 1, out0, res.sym_off.get() + e0);
+    GFZ_CHECK_LAUNCH();
+    const bool delta = res.delta_round > 0;
+    if (delta) {
+      const uint64_t w = ceil_div(n_new, 32);
+      DBuf<uint32_t> heads(ctx.pool(), w);
+      fill_u32(ctx, heads.get(), 0, w);
+      DBuf<uint64_t> d_st(ctx.pool(), std::max<size_t>(starts.size(), 1));
+      GFZ_HIP(hipMemcpyAsync(d_st.get(), starts.data(), starts.size() * 8, hipMemcpyHostToDevice,
+                             ctx.stream()));
+      set_head_bits(ctx, d_st.get(), starts.size(), heads.get(), n_new);
+      GFZ_HIP(hipMemcpyAsync(incl.get(), ssum.get(), n_new * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+      segmented_prefix_sum(ctx, incl.get(), n_new, heads.get());
+      ctx.sync();  // `starts` is pageable
+    }
+    hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
+                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.size.get(), rb.sum.get(),
+                       rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
+    GFZ_CHECK_LAUNCH();
+    GFZ_HIP(hipMemcpyAsync(h_off.data(), off.get(), (n_new + 1) * 8, hipMemcpyDeviceToHost,
+                           ctx.stream()));
+    ctx.sync();
+  }
+  // Host bookkeeping of the new traversals.
+  uint64_t k = 0;
+  for (uint64_t t = 0; t < n_trav_new; ++t) {
+    res.enc_start.push_back(e0 + k);
+    res.enc_len.push_back(enc_lengths[t]);
+    res.dec_start.push_back(out0 + h_off[k]);
+    res.dec_len.push_back(h_off[k + enc_lengths[t]] - h_off[k]);
+    k += enc_lengths[t];
+  }
+  res.n_trav += n_trav_new;
+  res.n_enc = e0 + n_new;
+  res.n_out = out0 + h_off[n_new];
+  // Grid: cells from the old sentinel on are set again (sentinel, then the new symbols).
+  const uint64_t old_cells = res.n_cells;
+  res.n_cells = ceil_div(std::max<uint64_t>(res.n_out, 1), kGrid) + 1;
+  grow(ctx, res.grid, old_cells, res.n_cells);
+  const uint64_t first = old_cells ? old_cells - 1 : 0;
+  fill_u64(ctx, res.grid.get() + first, res.n_enc ? res.n_enc - 1 : 0, res.n_cells - first);
+  if (n_new) {
+    hipLaunchKernelGGL(k_grid, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
+                       res.sym_off.get() + e0, n_new, res.grid.get());
+    GFZ_CHECK_LAUNCH();
+  }
+  ctx.sync();
 }
 
 }  // namespace gfz

@@ -36,6 +36,32 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
                    const uint32_t *enc_lengths, uint64_t n_trav, int delta_round,
                    Resident &res);
 
+// Host plan of one extraction: segments, tile numbering, where each starts.
+struct ExtractPlan {
+  uint32_t items = 16;
+  std::vector<SegDesc> segs;
+  std::vector<uint64_t> tile_prefix;  // segs + 1
+  std::vector<uint64_t> seg_dst;
+  uint64_t total = 0;
+};
+
+// Plans ids[0..n_ids) back to back (offsets: host, n_ids + 1, may be null);
+// returns the total. Throws on an index out of range.
+uint64_t extract_plan(const Resident &res, const uint64_t *ids, uint64_t n_ids, uint32_t items,
+                      ExtractPlan &plan, uint64_t *offsets);
+size_t extract_meta_bytes(const ExtractPlan &plan);
+// Enqueues the extraction on the ctx stream without waiting: the plan is
+// staged through h_stage (host, extract_meta_bytes; pinned for a truly
+// asynchronous copy, and untouched until the stream has passed the copy) into
+// d_meta (device, same size); results go to d_out[0..plan.total).
+void extract_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, uint8_t *h_stage,
+                     uint8_t *d_meta, int32_t *d_out);
+
+// Appends encoded traversals (d_enc device, enc_lengths host) to the resident
+// state; they get the next traversal indices. Synchronises the ctx stream.
+void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_new,
+                     const uint32_t *enc_lengths, uint64_t n_trav_new);
+
 // Decode traversals ids[0..n_ids) back to back into d_out (device, out_cap
 // int32); offsets (host, n_ids + 1, may be null) receives where each starts.
 // Returns the total; synchronises the ctx stream. items = output elements per

@@ -213,6 +213,66 @@ int gfz_encode_with_rules(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
                           gfz_encode_out *out);
 void gfz_encode_free(gfz_encode_out *out);
 
+/* ---- serving ----------------------------------------------------------------
+ * No counterpart in the reference, where extract and add-haplotypes are host
+ * workflows per call (extraction_workflow.cpp:206-424,
+ * add_haplotypes_workflow.cpp:336-592). A server keeps one rulebook and its
+ * encoded traversals resident in HBM and serves requests as they arrive: a
+ * worker thread admits queued requests in arrival order while their output fits
+ * one of two output slots (half of hbm_budget_bytes each), applies the batch's
+ * add requests (GPU encode with the stored rules, appended to the resident
+ * store; new traversals get the next indices), decodes all of the batch's
+ * extract requests in one persistent launch, and copies results to pinned host
+ * memory on a second stream while the next batch decodes. A request whose
+ * output exceeds one slot fails with GFZ_E_NOMEM; a full queue (max_queue > 0)
+ * refuses submissions with GFZ_E_BUSY. Submit, poll and wait are thread-safe.
+ * Latency = submission to result available on the host. */
+typedef struct gfz_server gfz_server;
+
+typedef struct gfz_server_config {
+  int device;
+  uint64_t hbm_budget_bytes;    /* device bytes for request outputs in flight */
+  uint32_t max_batch_requests;  /* 0: 256 */
+  uint32_t max_queue;           /* 0: unbounded */
+} gfz_server_config;
+
+typedef struct gfz_result {
+  int status;                 /* GFZ_OK or the request's error */
+  int is_add;
+  double latency_ms;
+  int32_t *values;            /* extract: decoded traversals back to back */
+  uint64_t n_values;
+  uint64_t *offsets;          /* extract: n_items + 1 starts */
+  uint64_t n_items;           /* extract: ids; add: traversals added */
+  uint64_t first_new_id;      /* add: index of the first added traversal */
+  uint32_t *encoded_lengths;  /* add: encoded length of each added traversal */
+} gfz_result;
+
+typedef struct gfz_server_stats {
+  uint64_t completed, rejected, batches, bytes_out, n_traversals;
+  double p50_ms, p99_ms, mean_ms;
+  double elapsed_s;           /* first submission to last completion */
+  double gbps;                /* decoded bytes delivered / elapsed_s */
+} gfz_serve_stats;
+
+int gfz_server_create(const gfz_server_config *cfg, const int32_t *rules_first,
+                      const int32_t *rules_second, uint64_t n_rules, uint32_t min_rule_id,
+                      const gfz_layer *layers, uint64_t n_layers, const int32_t *encoded,
+                      uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                      int delta_round, gfz_server **out);
+/* Finishes every queued request, then stops. */
+void gfz_server_destroy(gfz_server *s);
+int gfz_submit_extract(gfz_server *s, const uint64_t *ids, uint64_t n_ids, uint64_t *ticket);
+int gfz_submit_add(gfz_server *s, const int32_t *nodes, uint64_t n_nodes,
+                   const uint32_t *lengths, uint64_t n_trav, uint64_t *ticket);
+int gfz_poll(gfz_server *s, uint64_t ticket, int *done);
+/* Blocks until the request is done; returns its status (result fields valid
+ * on GFZ_OK; release with gfz_result_free). A ticket can be waited once. */
+int gfz_wait(gfz_server *s, uint64_t ticket, gfz_result *out);
+void gfz_result_free(gfz_result *r);
+int gfz_server_stats(gfz_server *s, gfz_serve_stats *out);
+int gfz_server_reset_stats(gfz_server *s);
+
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before
  * serialisation, producing the same bytes, so a caller can fill CompressedData

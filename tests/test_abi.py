@@ -59,3 +59,12 @@ def test_synthetic_generator_is_deterministic():
     a, b = synth.generate(cfg), synth.generate(cfg)
     assert np.array_equal(a.nodes, b.nodes) and np.array_equal(a.lengths, b.lengths)
     assert a.n_steps > 3000 * 4 * 0.9
+
+
+def test_serving_is_disabled_without_opt_in(monkeypatch):
+    """The serving loop refuses to start unless GFAZ_ENABLE_SERVING=1 (no GPU touched)."""
+    monkeypatch.delenv("GFAZ_ENABLE_SERVING", raising=False)
+    from gfaz_amd import serving
+    with pytest.raises(RuntimeError, match="serving loop is disabled"):
+        serving.Server(np.zeros(0, np.int32), np.zeros(0, np.int32), 1,
+                       np.zeros(0, np.int32), np.zeros(0, np.uint32), 1, hbm_budget_bytes=1 << 20)
