@@ -104,6 +104,8 @@ _SIGNATURES = {
     "gfz_encode_with_rules": ([_vp, _vp, _u64, _vp, _u64, _vp, _vp, _u64, _vp, _u64, ctypes.c_int,
                                ctypes.c_uint32, _vp], ctypes.c_int),
     "gfz_encode_free": ([_vp], None),
+    "gfz_decompress_stream": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, _u64,
+                               ctypes.c_uint32, _vp, _vp], ctypes.c_int),
     "gfz_calibration_copy": ([_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double)],
                              ctypes.c_int),
     "gfz_compress_traversals": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
@@ -598,3 +600,52 @@ def encode_with_rules(ctx: Context, nodes, lengths, rules_first, rules_second, l
     finally:
         lib().gfz_encode_free(ctypes.byref(out))
     return enc, lens
+
+
+# ---- streaming decompress -----------------------------------------------------------
+
+class _Chunk(ctypes.Structure):
+    _fields_ = [("values", ctypes.POINTER(ctypes.c_int32)), ("n_values", _u64),
+                ("trav_begin", _u64), ("trav_end", _u64), ("expanded_begin", _u64),
+                ("lengths", ctypes.POINTER(ctypes.c_uint32))]
+
+
+_CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(_Chunk), ctypes.c_void_p)
+
+
+def decompress_stream(ctx: Context, rb: "Rulebook", encoded, encoded_lengths,
+                      delta_round: int = 1, chunk_bytes: int = 0, consumer=None) -> None:
+    """Decode every traversal in chunks of whole traversals (gfz_decompress_stream).
+    consumer(values, trav_begin, trav_end, expanded_begin, lengths) gets views of pinned
+    memory valid during the call; a truthy return value stops the stream."""
+    enc_len = np.ascontiguousarray(encoded_lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(encoded, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        eptr, en = encoded.ptr, encoded.n
+    else:
+        encoded = np.ascontiguousarray(encoded, dtype=np.int32)
+        eptr, en = encoded.ctypes.data, encoded.size
+    errors = []
+
+    def cb(ptr, _user):
+        c = ptr.contents
+        try:
+            vals = (np.ctypeslib.as_array(c.values, shape=(c.n_values,)) if c.n_values
+                    else np.zeros(0, np.int32))
+            n_t = c.trav_end - c.trav_begin
+            lens = (np.ctypeslib.as_array(c.lengths, shape=(n_t,)) if n_t
+                    else np.zeros(0, np.uint32))
+            return 1 if consumer(vals, int(c.trav_begin), int(c.trav_end),
+                                 int(c.expanded_begin), lens) else 0
+        except Exception as e:  # surfaced after the stream stops
+            errors.append(e)
+            return 1
+
+    fn = _CHUNK_FN(cb)
+    rc = lib().gfz_decompress_stream(ctx.handle, rb.handle, eptr, en, enc_len.ctypes.data,
+                                     enc_len.size, int(delta_round), int(chunk_bytes), flags,
+                                     ctypes.cast(fn, _vp), None)
+    if errors:
+        raise errors[0]
+    _check(rc)

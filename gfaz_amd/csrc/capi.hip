@@ -390,6 +390,33 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
   });
 }
 
+// ---- streaming decompress ------------------------------------------------------
+
+int gfz_decompress_stream(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                          uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                          int delta_round, uint64_t chunk_bytes, uint32_t flags,
+                          gfz_chunk_consumer consumer, void *user) {
+  if (!ctx || !rb || !consumer || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  if (rb->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "rulebook belongs to another ctx");
+  return guarded(kDecompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_enc;
+    stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
+    gfz::Resident res;
+    gfz::resident_load(c, rb->rb, d_enc, n_encoded, encoded_lengths, n_trav, delta_round, res);
+    gfz::decompress_stream(c, res, chunk_bytes ? chunk_bytes : (uint64_t(1) << 30),
+                           [&](const gfz::StreamChunk &k) {
+                             gfz_chunk ch{k.values, k.n_values, k.trav_begin, k.trav_end,
+                                          k.expanded_begin, k.lengths};
+                             return consumer(&ch, user) == 0;
+                           });
+  });
+}
+
 // ---- add-haplotypes encoding ---------------------------------------------------
 
 namespace {

@@ -436,3 +436,96 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
 }
 
 }  // namespace gfz
+
+namespace gfz {
+
+void decompress_stream(Ctx &ctx, const Resident &res, uint64_t chunk_bytes,
+                       const std::function<bool(const StreamChunk &)> &consume) {
+  // Chunks of whole traversals, at most chunk_bytes of output each unless a
+  // single traversal is larger (then it is a chunk of its own).
+  struct Range {
+    uint64_t b, e, total;
+  };
+  std::vector<Range> chunks;
+  const uint64_t cap_elems = std::max<uint64_t>(chunk_bytes / 4, 1);
+  uint64_t biggest = 0;
+  for (uint64_t t = 0; t < res.n_trav;) {
+    uint64_t e = t, total = 0;
+    while (e < res.n_trav && (e == t || total + res.dec_len[e] <= cap_elems))
+      total += res.dec_len[e++];
+    chunks.push_back({t, e, total});
+    biggest = std::max(biggest, total);
+    t = e;
+  }
+  if (chunks.empty())
+    return;
+  const uint64_t buf_elems = std::max<uint64_t>(biggest, 1);
+  // Two device outputs and two pinned host buffers, used alternately.
+  DBuf<int32_t> d_out[2] = {DBuf<int32_t>(ctx.pool(), buf_elems),
+                            DBuf<int32_t>(ctx.pool(), buf_elems)};
+  int32_t *h_buf[2] = {nullptr, nullptr};
+  hipEvent_t ready[2] = {nullptr, nullptr};
+  struct Cleanup {
+    int32_t **h;
+    hipEvent_t *ev;
+    hipStream_t s;
+    ~Cleanup() {
+      (void)hipStreamSynchronize(s);
+      for (int i = 0; i < 2; ++i) {
+        if (h[i])
+          (void)hipHostFree(h[i]);
+        if (ev[i])
+          (void)hipEventDestroy(ev[i]);
+      }
+    }
+  } cleanup{h_buf, ready, ctx.stream()};
+  for (int i = 0; i < 2; ++i) {
+    void *p = nullptr;
+    GFZ_HIP(hipHostMalloc(&p, buf_elems * 4, hipHostMallocDefault));
+    h_buf[i] = static_cast<int32_t *>(p);
+    GFZ_HIP(hipEventCreateWithFlags(&ready[i], hipEventDisableTiming));
+  }
+  std::vector<uint64_t> ids;
+  std::vector<uint8_t> stage[2];  // reused only after its chunk's event was waited on
+  uint64_t expanded = 0;
+  std::vector<uint64_t> expanded_at(chunks.size());
+  auto enqueue = [&](size_t k) {
+    const Range &r = chunks[k];
+    ids.resize(r.e - r.b);
+    for (uint64_t t = r.b; t < r.e; ++t)
+      ids[t - r.b] = t;
+    ExtractPlan plan;
+    const int tuned = ctx.tuned("stream_items");
+    extract_plan(res, ids.data(), ids.size(), tuned > 0 ? (uint32_t)tuned : 16u, plan, nullptr);
+    if (plan.total) {
+      const size_t mb = extract_meta_bytes(plan);
+      stage[k & 1].resize(mb);
+      DBuf<uint8_t> d_meta(ctx.pool(), mb);  // stream-ordered reuse after return
+      extract_enqueue(ctx, res, plan, stage[k & 1].data(), d_meta.get(), d_out[k & 1].get());
+      GFZ_HIP(hipMemcpyAsync(h_buf[k & 1], d_out[k & 1].get(), plan.total * 4,
+                             hipMemcpyDeviceToHost, ctx.stream()));
+    }
+    GFZ_HIP(hipEventRecord(ready[k & 1], ctx.stream()));
+    expanded_at[k] = expanded;
+    expanded += r.total;
+  };
+  std::vector<uint32_t> lens;
+  auto deliver = [&](size_t k) {
+    GFZ_HIP(hipEventSynchronize(ready[k & 1]));
+    const Range &r = chunks[k];
+    lens.resize(r.e - r.b);
+    for (uint64_t t = r.b; t < r.e; ++t)
+      lens[t - r.b] = (uint32_t)res.dec_len[t];
+    StreamChunk c{h_buf[k & 1], r.total, r.b, r.e, expanded_at[k], lens.data()};
+    if (!consume(c))
+      throw Error(kErrInvalid, "stream stopped by the consumer at chunk " + std::to_string(k));
+  };
+  enqueue(0);
+  for (size_t k = 1; k < chunks.size(); ++k) {
+    enqueue(k);      // the device decodes chunk k ...
+    deliver(k - 1);  // ... while the consumer reads chunk k - 1
+  }
+  deliver(chunks.size() - 1);
+}
+
+}  // namespace gfz

@@ -5,6 +5,7 @@
 #include "codec.hpp"
 #include "zexpand.hpp"
 
+#include <functional>
 #include <vector>
 
 namespace gfz {
@@ -61,6 +62,23 @@ void extract_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, uin
 // state; they get the next traversal indices. Synchronises the ctx stream.
 void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_new,
                      const uint32_t *enc_lengths, uint64_t n_trav_new);
+
+// One chunk of a streamed decode: whole traversals [trav_begin, trav_end),
+// decoded back to back in pinned host memory valid during the callback.
+struct StreamChunk {
+  const int32_t *values;
+  uint64_t n_values;
+  uint64_t trav_begin, trav_end;
+  uint64_t expanded_begin;  // decoded offset of the chunk in the whole output
+  const uint32_t *lengths;  // decoded length per traversal of the chunk
+};
+
+// Decodes every traversal in chunks of at most chunk_bytes of output (a larger
+// traversal is a chunk of its own) through two alternating device and pinned
+// host buffers, calling consume in order; consume for chunk k runs while the
+// device decodes chunk k + 1. A false return stops the stream (Error).
+void decompress_stream(Ctx &ctx, const Resident &res, uint64_t chunk_bytes,
+                       const std::function<bool(const StreamChunk &)> &consume);
 
 // Decode traversals ids[0..n_ids) back to back into d_out (device, out_cap
 // int32); offsets (host, n_ids + 1, may be null) receives where each starts.

@@ -180,6 +180,31 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
                 int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *offsets,
                 uint64_t *n_out);
 
+/* ---- streaming decompress ---------------------------------------------------
+ * Replaces decompress_gpu_traversal_rolling_direct_writer / the rolling stream
+ * (src/gpu/decompression/traversal_decode_gpu.cu:122-156,
+ * include/gpu/decompression/path_decompression_gpu_rolling.hpp:26-44,74-81):
+ * every traversal is decoded in chunks of whole traversals of at most
+ * chunk_bytes of output (a larger traversal is a chunk of its own; 0 = 1 GiB,
+ * the reference's kDefaultRollingOutputChunkBytes) into two alternating pinned
+ * host buffers; the consumer is called in order on the calling thread, and runs
+ * for chunk k while the GPU decodes chunk k + 1. A nonzero return from the
+ * consumer stops the stream (GFZ_E_INVALID). Needs the rulebook bodies in HBM. */
+typedef struct gfz_chunk {
+  const int32_t *values;      /* pinned host memory, valid during the call */
+  uint64_t n_values;
+  uint64_t trav_begin;        /* traversals [trav_begin, trav_end) */
+  uint64_t trav_end;
+  uint64_t expanded_begin;    /* decoded offset of the chunk in the whole output */
+  const uint32_t *lengths;    /* decoded length of each traversal of the chunk */
+} gfz_chunk;
+typedef int (*gfz_chunk_consumer)(const gfz_chunk *chunk, void *user);
+
+int gfz_decompress_stream(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encoded,
+                          uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                          int delta_round, uint64_t chunk_bytes, uint32_t flags,
+                          gfz_chunk_consumer consumer, void *user);
+
 /* ---- add-haplotypes: encoding with a stored rulebook ----------------------
  * Replaces the encode step of add_haplotypes (src/workflows/
  * add_haplotypes_workflow.cpp:336-379,447-465 with src/grammar/path_encoder.cpp:
