@@ -31,7 +31,7 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
-# Algorithmic bytes per unit of each timed kernel group (see DESIGN.md):
+# Algorithmic bytes per unit of each timed kernel group:
 # expand: 4 E + 4 N + 8 R; the compress kernels per round r: pair_insert 4 L_r,
 # cand_tiles 4 L_r, emit 4 L_r + 4 L_{r+1}, remap 8 L_{r+1}; inverse_delta 8 N.
 
@@ -86,6 +86,138 @@ def cpu_baseline(cfg, tr, gfa_bytes: int, threads: int) -> dict:
             "compress_s": t1 - t0, "decompress_s": t2 - t1}
 
 
+def _compress_timed(ctx, codec, tr, d_nodes, reps=2):
+    """Best-of-reps whole-graph compress (device resident); returns (seconds, comp)."""
+    best, comp = None, None
+    for _ in range(reps):
+        if comp is not None:
+            comp.close()
+        ctx.sync()
+        t0 = time.perf_counter()
+        comp = codec.DeviceCompressed(ctx, d_nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best, comp
+
+
+def run_batching(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
+    """Decompress of every traversal: one persistent work-stealing launch for all of them
+    (gfz_extract with every index) against one launch per traversal (gfz_extract per
+    index), both from the same resident state into HBM. Also the whole-batch decoder."""
+    from gfaz_amd import codec
+    log("[bench] compress")
+    tc, comp = _compress_timed(ctx, codec, tr, d_nodes)
+    rb = comp.rulebook()
+    res = codec.Resident(ctx, rb, comp.encoded_view(), comp.encoded_lengths, 1)
+    n = tr.lengths.size
+    d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
+    ids = np.arange(n, dtype=np.uint64)
+    res.extract(ids, out=d_out)  # warm (and autotune)
+    ctx.sync()
+    ok = bool(np.array_equal(d_out.to_host(), tr.nodes))
+
+    def timed(fn, reps):
+        best = None
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_batched = timed(lambda: res.extract(ids, out=d_out), max(3, args.steps))
+    log("[bench] per-traversal launches")
+
+    def per_path():
+        for t in range(n):
+            res.extract(ids[t:t + 1], out=d_out)
+
+    per_path()  # warm
+    t_per_path = timed(per_path, 2)
+    t_whole = timed(lambda: codec.decompress_traversals(ctx, rb, comp.encoded_view(),
+                                                        comp.encoded_lengths, tr.lengths, 1,
+                                                        out=d_out), max(3, args.steps))
+    res.close()
+    rb.close()
+    comp.close()
+    out = {
+        "metric": "decompress GFA GB/s: batched persistent launch vs per-traversal launches",
+        "value": gfa_bytes / t_batched / 1e9, "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg.name, "traversals": int(n), "steps": int(tr.n_steps),
+                   "gfa_bytes": gfa_bytes, "rounds": 8, "threshold": 2, "delta_round": 1},
+        "round_trip_ok": ok,
+        "compress_s": tc, "compress_gbps": gfa_bytes / tc / 1e9,
+        "batched_ms": t_batched * 1e3, "per_traversal_ms": t_per_path * 1e3,
+        "batched_over_per_traversal": t_per_path / t_batched,
+        "whole_batch_decoder_ms": t_whole * 1e3,
+        "whole_batch_decoder_gbps": gfa_bytes / t_whole / 1e9,
+        "encoded_symbols": comp.n_encoded, "rules": comp.n_rules,
+        "tuned": {k: ctx.tuned(k) for k in [f"extract_items/{c}" for c in range(8, 16)]
+                  if ctx.tuned(k) > 0},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
+def run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
+    """Whole-graph compress, then decompress streamed to pinned host memory in 1 GiB
+    chunks of whole traversals (the consumer overlaps the next chunk's decode)."""
+    from gfaz_amd import codec
+    log("[bench] compress")
+    tc, comp = _compress_timed(ctx, codec, tr, d_nodes, reps=1)
+    log(f"[bench] compress {tc:.3f} s, rulebook")
+    t0 = time.perf_counter()
+    rb = comp.rulebook()
+    t_rb = time.perf_counter() - t0
+    offs = np.concatenate([[0], np.cumsum(tr.lengths, dtype=np.int64)])
+    state = {"ok": True, "chunks": 0}
+
+    def verify(values, b, e, expanded_begin, lengths):
+        state["chunks"] += 1
+        seg = tr.nodes[offs[b]:offs[e]]
+        state["ok"] = bool(state["ok"] and np.array_equal(values, seg)
+                           and int(expanded_begin) == int(offs[b]))
+        return False
+
+    def nothing(values, b, e, expanded_begin, lengths):
+        return False
+
+    log("[bench] stream (verify)")
+    codec.decompress_stream(ctx, rb, comp.encoded_view(), comp.encoded_lengths, 1, 1 << 30,
+                            verify)
+    log("[bench] stream (timed)")
+    ctx.sync()
+    t0 = time.perf_counter()
+    codec.decompress_stream(ctx, rb, comp.encoded_view(), comp.encoded_lengths, 1, 1 << 30,
+                            nothing)
+    td = time.perf_counter() - t0
+    info = rb.info()
+    out = {
+        "metric": "compress and streamed-decompress GFA GB/s (decoded into pinned host memory)",
+        "value": gfa_bytes / (tc + t_rb + td) / 1e9, "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg.name, "traversals": int(tr.lengths.size),
+                   "steps": int(tr.n_steps), "gfa_bytes": gfa_bytes, "rounds": 8,
+                   "threshold": 2, "delta_round": 1, "chunk_bytes": 1 << 30},
+        "round_trip_ok": state["ok"], "chunks": state["chunks"],
+        "compress_s": tc, "compress_gbps": gfa_bytes / tc / 1e9,
+        "rulebook_s": t_rb, "stream_decompress_s": td,
+        "stream_decompress_gbps": gfa_bytes / (t_rb + td) / 1e9,
+        "stream_int32_gbps": 4.0 * tr.n_steps / td / 1e9,
+        "encoded_symbols": comp.n_encoded, "rules": comp.n_rules, "rulebook": info,
+    }
+    rb.close()
+    comp.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if state["ok"] else 1
+
+
 # Timed kernel groups and the kernel whose hardware counters stand for them.
 GROUP_KERNEL = {"pair_insert": "pair_insert", "cand_tiles": "cand", "emit": "emit",
                 "collect_used": "collect_used", "remap": "remap", "delta": "delta",
@@ -113,6 +245,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-oracle", action="store_true",
                     help="also compare the encoded grammar with the CPU oracle")
+    ap.add_argument("--mode", default="roundtrip", choices=["roundtrip", "batching", "stream"],
+                    help="roundtrip (default): compress + decompress steps; batching: one "
+                         "persistent launch over all traversals vs one launch per traversal "
+                         "(walks_2m); stream: compress, then decompress streamed through two "
+                         "pinned host buffers (plain_10m)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for hardware-counter runs: one 1 GiB calibration copy, then one "
                          "compress and one decompress of the workload; prints a JSON summary")
@@ -125,6 +262,8 @@ def main() -> int:
     sys.path.insert(0, ROOT)
     from gfaz_amd import codec, synth
 
+    if args.mode != "roundtrip" and args.workload == "plain_500k":
+        args.workload = {"batching": "walks_2m", "stream": "plain_10m"}[args.mode]
     cfg = synth.CONFIGS[args.workload]
     log(f"[bench] generating {cfg.name}")
     tr = synth.generate(cfg)
@@ -133,6 +272,10 @@ def main() -> int:
 
     ctx = codec.Context(0)
     d_nodes = codec.DeviceArray.from_host(ctx, tr.nodes)
+    if args.mode == "batching":
+        return run_batching(args, ctx, cfg, tr, gfa_bytes, d_nodes)
+    if args.mode == "stream":
+        return run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
 
     if args.pmc_pass:
