@@ -79,7 +79,12 @@ struct CountTable {
   uint64_t mask;
   uint32_t *overflow;
   uint2 *block_counts;  // (new keys, newly repeated keys) per block
+  // Keys that became repeated, kStageCap per block (block b at b * kStageCap);
+  // a block with more raises *stage_overflow and the table is scanned instead.
+  unsigned long long *stage;
+  uint32_t *stage_overflow;
 };
+constexpr uint32_t kStageCap = 512;
 
 // Rule table (16-byte slots): the repeated keys only, dense enough to stay in
 // cache in late rounds. val = final rule ID after the sort; the slot index is
@@ -150,7 +155,7 @@ __device__ inline void table_find_batch(const Slot *__restrict__ slots, uint64_t
 template <int N>
 __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&k)[N],
                                           const uint32_t (&cnt)[N], uint32_t have,
-                                          uint32_t &fresh, uint32_t &reps) {
+                                          uint32_t &fresh, uint32_t &reps, uint32_t &newrep) {
   uint64_t s[N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
@@ -190,8 +195,10 @@ __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&
         continue;
       if (old[j] == kFree) {
         ++fresh;
-        if (cnt[j] >= 2)
+        if (cnt[j] >= 2) {
           ++reps;
+          newrep |= 1u << j;
+        }
         cas &= ~(1u << j);
       } else if ((old[j] & ~kRep) == k[j]) {
         if (!(old[j] & kRep))
@@ -210,11 +217,28 @@ __device__ inline void count_insert_batch(const CountTable &t, const uint64_t (&
         old[j] = atomicOr(&t.slots[s[j]], (unsigned long long)kRep);
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      if (((mark >> j) & 1) && !(old[j] & kRep))
+      if (((mark >> j) & 1) && !(old[j] & kRep)) {
         ++reps;
+        newrep |= 1u << j;
+      }
   }
   if (probe | cas)
     atomicExch(t.overflow, 1u);
+}
+
+// Appends the keys of the items in `mask` to the block's staging list.
+template <int N>
+__device__ inline void stage_keys(const CountTable &t, uint32_t *s_n, const uint64_t (&k)[N],
+                                  uint32_t mask) {
+  while (mask) {
+    const int j = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const uint32_t pos = atomicAdd(s_n, 1u);
+    if (pos < kStageCap)
+      t.stage[(uint64_t)blockIdx.x * kStageCap + pos] = k[j];
+    else
+      atomicExch(t.stage_overflow, 1u);
+  }
 }
 
 __global__ void k_fill_slots(Slot *slots, uint64_t n) {
@@ -302,13 +326,18 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
 // and newly repeated keys to the global counters with one atomic each.
 constexpr int kWaveMatch = 3;
 
-__device__ inline void lds_add(unsigned long long *skeys, uint32_t *scnt, uint64_t k, uint32_t n) {
+// LDS slot = key | kRep once the tile has seen the key twice (kRep is free:
+// compact keys are below 2^62), so the table is 8 bytes per slot.
+__device__ inline void lds_add(unsigned long long *skeys, uint64_t k, uint32_t n) {
   uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
+  const unsigned long long v = n >= 2 ? (k | kRep) : k;
   for (uint32_t p = 0; p < kLdsSlots; ++p) {
-    const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kFree,
-                                             (unsigned long long)k);
-    if (old == kFree || old == k) {
-      atomicAdd(&scnt[s], n);
+    const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kFree, v);
+    if (old == kFree)
+      return;
+    if ((old & ~kRep) == k) {
+      if (!(old & kRep))
+        atomicOr(&skeys[s], (unsigned long long)kRep);
       return;
     }
     s = (s + 1) & (kLdsSlots - 1);
@@ -319,13 +348,13 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
                                                         const uint32_t *__restrict__ heads,
                                                         uint64_t L, uint64_t span, CountTable t) {
   __shared__ unsigned long long skeys[kLdsSlots];
-  __shared__ uint32_t scnt[kLdsSlots];
   __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
+  __shared__ uint32_t s_stage_n;
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock) {
+  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock)
     skeys[s] = kFree;
-    scnt[s] = 0;
-  }
+  if (threadIdx.x == 0)
+    s_stage_n = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
   uint64_t kv[kInsItems];
@@ -345,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
       const uint64_t lk = __shfl(k, (int)leader);
       const uint64_t same = __ballot(k == lk) & rem;
       if (lane == leader)
-        lds_add(skeys, scnt, lk, (uint32_t)__popcll(same));
+        lds_add(skeys, lk, (uint32_t)__popcll(same));
       rem &= ~same;
     }
     // (1b) the rest: runs of equal consecutive keys
@@ -358,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
       const uint64_t ends = hmask | ~rem;
       const uint64_t after = lane == kWave - 1 ? 0ull : ends >> (lane + 1);
       const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u : (uint32_t)(kWave - lane);
-      lds_add(skeys, scnt, k, len);
+      lds_add(skeys, k, len);
     }
   }
   __syncthreads();
@@ -374,14 +403,69 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
 #pragma unroll
       for (int i = 0; i < kHalf; ++i) {
         const uint32_t sl = threadIdx.x + (uint32_t)(h * kHalf + i) * kBlock;
-        fk[i] = skeys[sl];
-        fc[i] = scnt[sl];
-        if (fk[i] != kFree)
+        const uint64_t v = skeys[sl];
+        fk[i] = v & ~kRep;
+        fc[i] = (v & kRep) ? 2u : 1u;
+        if (v != kFree)
           have |= 1u << i;
       }
-      if (have)
-        count_insert_batch<kHalf>(t, fk, fc, have, fresh, reps);
+      if (have) {
+        uint32_t newrep = 0;
+        count_insert_batch<kHalf>(t, fk, fc, have, fresh, reps, newrep);
+        stage_keys<kHalf>(t, &s_stage_n, fk, newrep);
+      }
     }
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    fresh += __shfl_xor(fresh, d);
+    reps += __shfl_xor(reps, d);
+  }
+  if (lane == 0) {
+    sfresh[wid] = fresh;
+    sreps[wid] = reps;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0, r = 0;
+    for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+      n += sfresh[w];
+      r += sreps[w];
+    }
+    t.block_counts[blockIdx.x] = make_uint2(n, r);
+  }
+}
+
+// Rounds where nearly every pair is distinct gain nothing from the tile's LDS
+// table: each thread inserts its pairs directly, all probe chains in flight.
+__global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__restrict__ x,
+                                                               const uint32_t *__restrict__ heads,
+                                                               uint64_t L, uint64_t span,
+                                                               CountTable t) {
+  __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
+  __shared__ uint32_t s_stage_n;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0)
+    s_stage_n = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
+  uint64_t kv[kInsItems];
+  uint32_t cnt[kInsItems];
+  uint32_t have = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    const bool v = i + 1 < L && !head_at(heads, i + 1);
+    kv[j] = v ? pair_key(x[i], x[i + 1], span) : kFree;
+    cnt[j] = 1;
+    if (v)
+      have |= 1u << j;
+  }
+  uint32_t fresh = 0, reps = 0;
+  if (have) {
+    uint32_t newrep = 0;
+    count_insert_batch<(int)kInsItems>(t, kv, cnt, have, fresh, reps, newrep);
+    stage_keys<(int)kInsItems>(t, &s_stage_n, kv, newrep);
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
@@ -435,6 +519,32 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c
 }
 
 // ---- K1b: the repeated keys, moved into the rule table -------------------------
+// From the per-block staging lists: one wave per block list.
+__global__ __launch_bounds__(kBlock) void k_extract_staged(
+    const unsigned long long *__restrict__ stage, const uint2 *__restrict__ block_counts,
+    uint64_t n_blocks, Slot *__restrict__ rt, uint64_t rmask, uint32_t *__restrict__ err) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t b = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (b >= n_blocks)
+    return;
+  const uint32_t n = block_counts[b].y;
+  for (uint32_t j = lane; j < n && j < kStageCap; j += kWave) {
+    const uint64_t k = stage[b * kStageCap + j];
+    uint64_t s = key_hash(k) & rmask;
+    bool placed = false;
+    for (uint32_t p = 0; p < kMaxProbe && !placed; ++p) {
+      const unsigned long long old =
+          atomicCAS(&rt[s].key, (unsigned long long)kFree, (unsigned long long)k);
+      if (old == kFree)
+        placed = true;
+      else
+        s = (s + 1) & rmask;
+    }
+    if (!placed)
+      atomicExch(err, 3u);
+  }
+}
+
 constexpr uint32_t kExtractItems = 16;
 
 __global__ __launch_bounds__(kBlock) void k_extract_repeated(
@@ -891,7 +1001,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   DBuf<int32_t> rf, rs;
   DBuf<uint32_t> d_err(pool, 2);
   fill_u32(ctx, d_err.get(), 0, 2);
-  uint64_t prev_distinct = 0;
+  uint64_t prev_distinct = 0, prev_len = 0;
 
   for (int round = 0; round < p.rounds && L >= 2; ++round) {
     RoundInfo info;
@@ -915,26 +1025,36 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> ctab;
     const uint64_t n_ins = ceil_div(L, kInsTile);
     DBuf<uint2> bcounts(pool, n_ins);
-    DBuf<unsigned long long> sums(pool, 3);  // distinct, repeated, overflow flag
+    DBuf<unsigned long long> stage(pool, n_ins * kStageCap);
+    DBuf<unsigned long long> sums(pool, 4);  // distinct, repeated, overflow, stage overflow
     uint64_t distinct = 0, repeated = 0;
+    // Once most pairs of the last round were distinct, tile aggregation only adds
+    // latency (distinct pairs only grow from round to round).
+    const bool direct = round > 0 && 2 * prev_distinct > prev_len;
+    bool stage_overflow = false;
     for (;;) {
       ctab = DBuf<unsigned long long>(pool, cap);
       fill_u64(ctx, reinterpret_cast<uint64_t *>(ctab.get()), kFree, cap);
-      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 3);
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 4);
       CountTable t{ctab.get(), cap - 1, reinterpret_cast<uint32_t *>(sums.get() + 2),
-                   bcounts.get()};
+                   bcounts.get(), stage.get(), reinterpret_cast<uint32_t *>(sums.get() + 3)};
       ctx.timed("pair_insert", [&] {
-        hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)n_ins), dim3(kBlock), 0, ctx.stream(),
-                           cur, heads.get(), L, span, t);
+        if (direct)
+          hipLaunchKernelGGL(k_pair_insert_direct, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                             ctx.stream(), cur, heads.get(), L, span, t);
+        else
+          hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)n_ins), dim3(kBlock), 0, ctx.stream(),
+                             cur, heads.get(), L, span, t);
       });
       GFZ_CHECK_LAUNCH();
       ctx.add_bytes("pair_insert", 4.0 * L);
       hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), n_ins,
                          sums.get());
       GFZ_CHECK_LAUNCH();
-      ctx.stage_scalars(sums.get(), 3);
+      ctx.stage_scalars(sums.get(), 4);
       ctx.sync();
       const uint64_t *hc = ctx.scalars();
+      stage_overflow = hc[3] != 0;
       if (!hc[2] && hc[0] <= cap / 2) {
         distinct = hc[0];
         repeated = hc[1];
@@ -946,6 +1066,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     info.distinct_pairs = distinct;
     info.table_slots = cap;
     prev_distinct = distinct;
+    prev_len = L;
     if (repeated == 0)
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
@@ -959,12 +1080,18 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                        rt.get(), rcap);
     GFZ_CHECK_LAUNCH();
     ctx.timed("extract_repeated", [&] {
-      hipLaunchKernelGGL(k_extract_repeated,
-                         dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
-                         dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt.get(), rcap - 1,
-                         d_err.get());
+      if (stage_overflow)  // some block staged too many: scan the whole table
+        hipLaunchKernelGGL(k_extract_repeated,
+                           dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
+                           dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt.get(), rcap - 1,
+                           d_err.get());
+      else
+        hipLaunchKernelGGL(k_extract_staged, dim3((uint32_t)ceil_div(n_ins, kBlock / kWave)),
+                           dim3(kBlock), 0, ctx.stream(), stage.get(), bcounts.get(), n_ins,
+                           rt.get(), rcap - 1, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
+    stage.reset();
     ctab.reset();
 
     // K2 + K2b: candidates, tile carries, next length. Round scalars (read back

@@ -640,7 +640,16 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   }
   depth.reset();
   const uint64_t total = lvl_off[max_depth + 1];
-  uint64_t budget = uint64_t(1) << 30;  // elements (4 GiB)
+  // Default budget: a quarter of the free device memory, at most 2^31 - 2
+  // elements (31-bit Z indices; 8 GiB).
+  uint64_t budget = uint64_t(1) << 30;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      budget = std::max<uint64_t>(budget, (uint64_t)free_b / 4 / 4);
+    else
+      (void)hipGetLastError();
+  }
   if (const char *env = std::getenv("GFAZ_BODY_BUDGET_MB"))
     budget = std::strtoull(env, nullptr, 10) * (uint64_t(1) << 20) / 4;
   budget = std::min<uint64_t>(budget, (uint64_t(1) << 31) - 2);  // 31-bit Z indices

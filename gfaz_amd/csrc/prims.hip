@@ -94,8 +94,12 @@ void exclusive_scan_impl(Ctx &ctx, const T *in, uint64_t *out, uint64_t n) {
 }
 
 // ---- radix sort -------------------------------------------------------------
-constexpr uint32_t kSortItems = 16;
-constexpr uint32_t kSortTile = kBlock * kSortItems;
+// Keys per thread. Sorts of at most kInlineOffsetBlocks tiles let each block
+// sum the per-block histograms itself (2 launches per digit instead of 5).
+// Larger tiles (64 keys per thread) to reach that threshold were measured
+// slower: too few blocks and four barriers per key step.
+constexpr uint32_t kSortItemsLarge = 16;
+constexpr uint32_t kSortItemsSmall = 16;
 constexpr uint64_t kSign = 0x8000000000000000ull;
 constexpr uint32_t kInlineOffsetBlocks = 16;
 
@@ -113,9 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_or_diff(const uint64_t *__restrict__
     atomicOr(acc, (unsigned long long)x);
 }
 
+template <uint32_t kSortItems>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                        uint64_t n, uint32_t shift,
                                                        uint32_t *__restrict__ hist, uint32_t nb) {
+  constexpr uint32_t kSortTile = kBlock * kSortItems;
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -134,12 +140,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t *__restric
 // wave come from eight ballots (lanes whose digit agrees on every bit).
 // With offs == nullptr the block derives its digit offsets from the per-block
 // histograms itself (cheap while nb is small), saving the three scan launches.
+template <uint32_t kSortItems>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t *__restrict__ in,
                                                           uint64_t *__restrict__ out, uint64_t n,
                                                           uint32_t shift,
                                                           const uint64_t *__restrict__ offs,
                                                           const uint32_t *__restrict__ hist,
                                                           uint32_t nb) {
+  constexpr uint32_t kSortTile = kBlock * kSortItems;
   __shared__ uint64_t base_off[256];
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[kBlock / kWave][256];
@@ -342,9 +350,9 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
   sort_i64_varying(ctx, keys, tmp, n, ctx.scalars()[0]);
 }
 
-void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
-  if (n < 2 || !varying)
-    return;
+template <uint32_t kSortItems>
+void radix_passes(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
+  constexpr uint32_t kSortTile = kBlock * kSortItems;
   const uint32_t nb = (uint32_t)ceil_div(n, kSortTile);
   DBuf<uint32_t> hist(ctx.pool(), (uint64_t)256 * nb);
   DBuf<uint64_t> offs(ctx.pool(), (uint64_t)256 * nb + 1);
@@ -353,15 +361,15 @@ void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint6
     if (((varying >> shift) & 255) == 0)
       continue;
     ctx.timed("radix_sort", [&] {
-      hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, ctx.stream(), src, n, shift,
-                         hist.get(), nb);
+      hipLaunchKernelGGL(k_radix_hist<kSortItems>, dim3(nb), dim3(kBlock), 0, ctx.stream(), src,
+                         n, shift, hist.get(), nb);
       GFZ_CHECK_LAUNCH();
       const bool inline_offsets = nb <= kInlineOffsetBlocks;
       if (!inline_offsets)
         exclusive_scan(ctx, hist.get(), offs.get(), (uint64_t)256 * nb);
-      hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, ctx.stream(), src, dst, n,
-                         shift, inline_offsets ? (const uint64_t *)nullptr : offs.get(),
-                         hist.get(), nb);
+      hipLaunchKernelGGL(k_radix_scatter<kSortItems>, dim3(nb), dim3(kBlock), 0, ctx.stream(),
+                         src, dst, n, shift,
+                         inline_offsets ? (const uint64_t *)nullptr : offs.get(), hist.get(), nb);
       GFZ_CHECK_LAUNCH();
     });
     std::swap(src, dst);
@@ -369,6 +377,15 @@ void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint6
   if (src != keys)
     GFZ_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                            ctx.stream()));
+}
+
+void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
+  if (n < 2 || !varying)
+    return;
+  if (n <= (uint64_t)kInlineOffsetBlocks * kBlock * kSortItemsSmall)
+    radix_passes<kSortItemsSmall>(ctx, keys, tmp, n, varying);
+  else
+    radix_passes<kSortItemsLarge>(ctx, keys, tmp, n, varying);
 }
 
 }  // namespace gfz
