@@ -1152,7 +1152,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
-    sort_i64_varying(ctx, ukeys.get(), utmp.get(), R, varying);
+    sort_i64_varying(ctx, ukeys.get(), utmp.get(), R, varying, d_err.get());
     if (out.n_rules + R > rules_cap) {
       const uint64_t ncap = std::max<uint64_t>(2 * rules_cap, out.n_rules + R);
       DBuf<int32_t> nf(pool, ncap), ns(pool, ncap);
@@ -1193,8 +1193,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   out.end_id = next;
   ctx.stage_scalars(d_err.get(), 1);
   ctx.sync();
-  if ((uint32_t)ctx.scalars()[0])
-    throw Error(kErrCorrupt, "sorted key missing from the rule table (internal error)");
+  if (const uint32_t code = (uint32_t)ctx.scalars()[0])
+    throw Error(kErrCorrupt, "grammar induction failed an internal check (code " +
+                                 std::to_string(code) + ": 1 sorted key missing from the rule "
+                                 "table, 3 rule table full, 4 sort look-back timed out)");
 
   // Encoded lengths from the head positions of the final stream.
   if (!ne_index.empty()) {

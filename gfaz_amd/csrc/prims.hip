@@ -94,14 +94,7 @@ void exclusive_scan_impl(Ctx &ctx, const T *in, uint64_t *out, uint64_t n) {
 }
 
 // ---- radix sort -------------------------------------------------------------
-// Keys per thread. Sorts of at most kInlineOffsetBlocks tiles let each block
-// sum the per-block histograms itself (2 launches per digit instead of 5).
-// Larger tiles (64 keys per thread) to reach that threshold were measured
-// slower: too few blocks and four barriers per key step.
-constexpr uint32_t kSortItemsLarge = 16;
-constexpr uint32_t kSortItemsSmall = 16;
-constexpr uint64_t kSign = 0x8000000000000000ull;
-constexpr uint32_t kInlineOffsetBlocks = 16;
+constexpr uint64_t kSign = 0x8000000000000000ull;  // signed order as unsigned
 
 __global__ __launch_bounds__(kBlock) void k_or_diff(const uint64_t *__restrict__ keys, uint64_t n,
                                                     unsigned long long *acc) {
@@ -115,95 +108,6 @@ __global__ __launch_bounds__(kBlock) void k_or_diff(const uint64_t *__restrict__
     x |= __shfl_xor(x, d);
   if ((threadIdx.x & (kWave - 1)) == 0 && x)
     atomicOr(acc, (unsigned long long)x);
-}
-
-template <uint32_t kSortItems>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t *__restrict__ keys,
-                                                       uint64_t n, uint32_t shift,
-                                                       uint32_t *__restrict__ hist, uint32_t nb) {
-  constexpr uint32_t kSortTile = kBlock * kSortItems;
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-#pragma unroll 4
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    if (i < n)
-      atomicAdd(&h[((keys[i] ^ kSign) >> shift) & 255], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable scatter: keys keep their input order within a digit. Ranks inside a
-// wave come from eight ballots (lanes whose digit agrees on every bit).
-// With offs == nullptr the block derives its digit offsets from the per-block
-// histograms itself (cheap while nb is small), saving the three scan launches.
-template <uint32_t kSortItems>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t *__restrict__ in,
-                                                          uint64_t *__restrict__ out, uint64_t n,
-                                                          uint32_t shift,
-                                                          const uint64_t *__restrict__ offs,
-                                                          const uint32_t *__restrict__ hist,
-                                                          uint32_t nb) {
-  constexpr uint32_t kSortTile = kBlock * kSortItems;
-  __shared__ uint64_t base_off[256];
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kBlock / kWave][256];
-  __shared__ uint64_t lds[kBlock / kWave + 1];
-  const uint32_t tid = threadIdx.x, wid = tid / kWave;
-  if (offs) {
-    base_off[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
-  } else {
-    uint64_t tot = 0, pre = 0;
-    for (uint32_t b = 0; b < nb; ++b) {
-      const uint32_t h = hist[(uint64_t)tid * nb + b];
-      tot += h;
-      if (b < blockIdx.x)
-        pre += h;
-    }
-    uint64_t total;
-    base_off[tid] = block_exclusive_scan<kBlock>(tot, lds, total) + pre;
-  }
-  run[tid] = 0;
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  for (uint32_t j = 0; j < kSortItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + tid;
-    const bool valid = i < n;
-    const uint64_t k = valid ? in[i] : 0;
-    const uint32_t d = (uint32_t)(((k ^ kSign) >> shift) & 255);
-#pragma unroll
-    for (uint32_t w = 0; w < kBlock / kWave; ++w)
-      wcnt[w][tid] = 0;
-    __syncthreads();
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (uint32_t b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    const uint64_t below = peers & lanemask_lt();
-    const uint32_t rank = (uint32_t)__popcll(below);
-    if (valid && below == 0)
-      wcnt[wid][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t r = run[tid];
-#pragma unroll
-      for (uint32_t w = 0; w < kBlock / kWave; ++w) {
-        const uint32_t t = wcnt[w][tid];
-        wcnt[w][tid] = r;
-        r += t;
-      }
-      run[tid] = r;
-    }
-    __syncthreads();
-    if (valid)
-      out[base_off[d] + wcnt[wid][d] + rank] = k;
-    __syncthreads();
-  }
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
@@ -347,45 +251,212 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
   GFZ_CHECK_LAUNCH();
   ctx.stage_scalars(acc.get(), 1);
   ctx.sync();
-  sort_i64_varying(ctx, keys, tmp, n, ctx.scalars()[0]);
+  const uint64_t varying = ctx.scalars()[0];
+  DBuf<uint32_t> err(ctx.pool(), 2);
+  fill_u32(ctx, err.get(), 0, 2);
+  sort_i64_varying(ctx, keys, tmp, n, varying, err.get());
+  ctx.stage_scalars(err.get(), 1);
+  ctx.sync();
+  if ((uint32_t)ctx.scalars()[0])
+    throw Error(kErrInvalid, "radix sort look-back did not complete (internal error)");
 }
 
-template <uint32_t kSortItems>
-void radix_passes(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
-  constexpr uint32_t kSortTile = kBlock * kSortItems;
-  const uint32_t nb = (uint32_t)ceil_div(n, kSortTile);
-  DBuf<uint32_t> hist(ctx.pool(), (uint64_t)256 * nb);
-  DBuf<uint64_t> offs(ctx.pool(), (uint64_t)256 * nb + 1);
-  uint64_t *src = keys, *dst = tmp;
-  for (uint32_t shift = 0; shift < 64; shift += 8) {
-    if (((varying >> shift) & 255) == 0)
-      continue;
-    ctx.timed("radix_sort", [&] {
-      hipLaunchKernelGGL(k_radix_hist<kSortItems>, dim3(nb), dim3(kBlock), 0, ctx.stream(), src,
-                         n, shift, hist.get(), nb);
-      GFZ_CHECK_LAUNCH();
-      const bool inline_offsets = nb <= kInlineOffsetBlocks;
-      if (!inline_offsets)
-        exclusive_scan(ctx, hist.get(), offs.get(), (uint64_t)256 * nb);
-      hipLaunchKernelGGL(k_radix_scatter<kSortItems>, dim3(nb), dim3(kBlock), 0, ctx.stream(),
-                         src, dst, n, shift,
-                         inline_offsets ? (const uint64_t *)nullptr : offs.get(), hist.get(), nb);
-      GFZ_CHECK_LAUNCH();
-    });
-    std::swap(src, dst);
+// ---- one-sweep LSD radix sort -------------------------------------------------
+// One histogram kernel for every digit, then one kernel per digit: each block
+// takes the next tile by ticket, ranks its keys stably (item, then wave, then lane),
+// publishes its per-digit counts, and finds the counts of all earlier tiles by
+// decoupled look-back on per-(tile, digit) status words:
+//   bits 63..62 = 1: this tile's own count, 2: inclusive prefix up to it.
+// A block only waits on tiles with smaller tickets, which belong to blocks
+// that already run and publish before they wait, so the waits end; each is
+// still bounded (kLookbackSpin polls) and raises an error flag if exceeded.
+// A status word carries its own count, so relaxed agent-scope atomics suffice;
+// acquire/release at agent scope would add L2 write-back and invalidation
+// (one L2 per XCD) to every publish and poll.
+constexpr uint32_t kOsItems = 16;
+constexpr uint32_t kOsTile = kBlock * kOsItems;
+constexpr uint64_t kStFlagAgg = 1ull << 62, kStFlagPre = 2ull << 62,
+                   kStCount = (1ull << 62) - 1;
+constexpr uint32_t kLookbackSpin = 1u << 22;
+constexpr uint32_t kWindow = 16;
+
+__global__ __launch_bounds__(kBlock) void k_sort_hist_all(const uint64_t *__restrict__ keys,
+                                                          uint64_t n, uint64_t varying,
+                                                          unsigned long long *__restrict__ ghist) {
+  __shared__ uint32_t h[8][256];
+  for (uint32_t i = threadIdx.x; i < 8 * 256; i += kBlock)
+    (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kOsTile;
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    if (i >= n)
+      break;
+    const uint64_t k = keys[i] ^ kSign;
+#pragma unroll
+    for (uint32_t p = 0; p < 8; ++p)
+      if ((varying >> (8 * p)) & 255)
+        atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
   }
-  if (src != keys)
-    GFZ_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                           ctx.stream()));
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 8 * 256; i += kBlock) {
+    const uint32_t c = (&h[0][0])[i];
+    if (c)
+      atomicAdd(&ghist[i], (unsigned long long)c);
+  }
 }
 
-void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying) {
+__global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t *__restrict__ in,
+                                                     uint64_t *__restrict__ out, uint64_t n,
+                                                     uint32_t shift,
+                                                     const unsigned long long *__restrict__ ghist,
+                                                     unsigned long long *status,
+                                                     unsigned long long *ticket, uint32_t *err) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[kBlock / kWave][256];
+  __shared__ uint64_t base_off[256];
+  __shared__ uint64_t lds[kBlock / kWave + 1];
+  __shared__ unsigned long long s_tile;
+  const uint32_t tid = threadIdx.x, wid = tid / kWave;
+  if (tid == 0)
+    s_tile = atomicAdd(ticket, 1ull);
+  run[tid] = 0;
+  __syncthreads();
+  const uint64_t t = s_tile;
+  const uint64_t base = t * kOsTile;
+  uint64_t key[kOsItems];
+  uint32_t rank[kOsItems];
+  // Phase A: stable local ranks (item j, then wave, then lane) and tile counts.
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + tid;
+    const bool valid = i < n;
+    const uint64_t k = valid ? in[i] : 0;
+    key[j] = k;
+    const uint32_t d = (uint32_t)(((k ^ kSign) >> shift) & 255);
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / kWave; ++w)
+      wcnt[w][tid] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint64_t below = peers & lanemask_lt();
+    if (valid && below == 0)
+      wcnt[wid][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t r = run[tid];
+#pragma unroll
+      for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+        const uint32_t c = wcnt[w][tid];
+        wcnt[w][tid] = r;
+        r += c;
+      }
+      run[tid] = r;
+    }
+    __syncthreads();
+    rank[j] = valid ? wcnt[wid][d] + (uint32_t)__popcll(below) : 0;
+    __syncthreads();
+  }
+  // Phase B: digit tid - global start, then this tile's place among earlier tiles.
+  uint64_t total;
+  const uint64_t gstart = block_exclusive_scan<kBlock>((uint64_t)ghist[tid], lds, total);
+  const uint64_t mine = run[tid];
+  unsigned long long *st = status + t * 256 + tid;
+  uint64_t excl = 0;
+  if (t == 0) {
+    __hip_atomic_store(st, kStFlagPre | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(st, kStFlagAgg | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Windowed look-back: kWindow predecessors per round trip, nearest first.
+    int64_t k = (int64_t)t - 1;  // tile 0 always publishes a prefix: k never passes it
+    uint32_t spins = 0;
+    for (;;) {
+      uint64_t v[kWindow];
+#pragma unroll
+      for (uint32_t w = 0; w < kWindow; ++w)
+        v[w] = k - (int64_t)w >= 0
+                   ? __hip_atomic_load(status + (uint64_t)(k - (int64_t)w) * 256 + tid,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : kStFlagPre;  // below tile 0: nothing more to add
+      bool done = false;
+      uint32_t used = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < kWindow; ++w) {
+        if (done || used < w)
+          continue;
+        if (v[w] & kStFlagPre) {
+          excl += v[w] & kStCount;
+          done = true;
+        } else if (v[w] & kStFlagAgg) {
+          excl += v[w] & kStCount;
+          used = w + 1;
+        }
+      }
+      if (done)
+        break;
+      k -= used;  // resume at the first tile that has not published yet
+      if (used == 0) {
+        if (++spins > kLookbackSpin) {
+          atomicExch(err, 4u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __hip_atomic_store(st, kStFlagPre | (excl + mine), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  base_off[tid] = gstart + excl;
+  __syncthreads();
+  // Phase C: scatter.
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + tid;
+    if (i < n) {
+      const uint32_t d = (uint32_t)(((key[j] ^ kSign) >> shift) & 255);
+      out[base_off[d] + rank[j]] = key[j];
+    }
+  }
+}
+
+void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying,
+                      uint32_t *err) {
   if (n < 2 || !varying)
     return;
-  if (n <= (uint64_t)kInlineOffsetBlocks * kBlock * kSortItemsSmall)
-    radix_passes<kSortItemsSmall>(ctx, keys, tmp, n, varying);
-  else
-    radix_passes<kSortItemsLarge>(ctx, keys, tmp, n, varying);
+  uint32_t passes = 0;
+  for (uint32_t p = 0; p < 8; ++p)
+    passes += ((varying >> (8 * p)) & 255) ? 1 : 0;
+  const uint64_t tiles = ceil_div(n, kOsTile);
+  // ghist (8 x 256) | tickets (8) | status (passes x tiles x 256), cleared by one fill.
+  const uint64_t words = 8 * 256 + 8 + (uint64_t)passes * tiles * 256;
+  DBuf<unsigned long long> scratch(ctx.pool(), words);
+  unsigned long long *ghist = scratch.get(), *tickets = ghist + 8 * 256,
+                     *status = tickets + 8;
+  fill_u64(ctx, reinterpret_cast<uint64_t *>(scratch.get()), 0, words);
+  ctx.timed("radix_sort", [&] {
+    hipLaunchKernelGGL(k_sort_hist_all, dim3((uint32_t)tiles), dim3(kBlock), 0, ctx.stream(),
+                       keys, n, varying, ghist);
+    GFZ_CHECK_LAUNCH();
+    uint64_t *src = keys, *dst = tmp;
+    uint32_t q = 0;
+    for (uint32_t p = 0; p < 8; ++p) {
+      if (((varying >> (8 * p)) & 255) == 0)
+        continue;
+      hipLaunchKernelGGL(k_onesweep, dim3((uint32_t)tiles), dim3(kBlock), 0, ctx.stream(), src,
+                         dst, n, 8 * p, ghist + 256 * p, status + (uint64_t)q * tiles * 256,
+                         tickets + p, err);
+      GFZ_CHECK_LAUNCH();
+      std::swap(src, dst);
+      ++q;
+    }
+    if (src != keys)
+      GFZ_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+  });
 }
 
 }  // namespace gfz

@@ -57,7 +57,9 @@ void exclusive_scan(Ctx &ctx, const uint64_t *in, uint64_t *out, uint64_t n);
 void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n);
 // Same, when the OR of (key XOR first key) over all keys is already known:
 // bytes that never vary are skipped.
-void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying);
+// `err` (device) is set non-zero if a look-back wait hit its bound.
+void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying,
+                      uint32_t *err);
 
 void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);
