@@ -37,10 +37,9 @@ def stats_markdown(csv_path: str, cmd: str, top: int = 25) -> str:
     agg = collections.OrderedDict()
     for r in rows:
         name = short_name(r["Name"])
-        head = r["Name"].split("(", 1)[0]
-        if "<" in head:  # template arguments of the kernel itself, not of its parameters
-            targs = head.split("<", 1)[1].rsplit(">", 1)[0].replace("u", "").replace(" ", "")
-            name = f"{name}<{targs}>"
+        m = re.search(r"k_\w+<([^()]*)>\(", r["Name"])  # the kernel's own template arguments
+        if m:
+            name = f"{name}<{m.group(1).replace('u', '').replace(' ', '')}>"
         a = agg.setdefault(name, [0, 0.0, float("inf"), 0.0])
         a[0] += int(r["Calls"])
         a[1] += float(r["TotalDurationNs"])
