@@ -487,6 +487,142 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__
   }
 }
 
+// ---- K1 for rounds of mostly distinct pairs: partitioned counting -----------
+// (1) every pair key goes to one of B hash buckets (fixed capacity each); (2)
+// one block per bucket counts its keys in a 128 KB LDS table and compacts the
+// repeated ones in place. Reads and writes are sequential per bucket instead
+// of one random table access per pair. Any bucket or table overflow raises a
+// flag; the host then counts the round with the global table instead.
+constexpr uint32_t kBktSlots = 16384;   // LDS table of one bucket
+constexpr uint32_t kBktProbe = 512;
+constexpr uint32_t kBktThreads = 1024;
+constexpr uint32_t kMaxBuckets = 4096;
+
+struct Buckets {
+  unsigned long long *entries;  // bucket b at [b * cap, b * cap + cap)
+  uint64_t cap;
+  uint32_t mask;                // B - 1
+  unsigned int *fill;           // keys written per bucket (may exceed cap: overflow)
+  uint32_t *overflow;
+};
+
+__device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
+  return (uint32_t)(key_hash(k) >> 40) & mask;  // independent of the low probe bits
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__restrict__ x,
+                                                           const uint32_t *__restrict__ heads,
+                                                           uint64_t L, uint64_t span, Buckets bk) {
+  __shared__ uint32_t hist[kMaxBuckets];
+  const uint32_t nb = bk.mask + 1;
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+    hist[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  uint64_t kv[kInsItems];
+  uint32_t loc[kInsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    const bool v = i + 1 < L && !head_at(heads, i + 1);
+    uint64_t k = v ? pair_key(x[i], x[i + 1], span) : kFree;
+    // The wave's most common key (its first lane's) is written once; if the wave
+    // holds it twice it is already known repeated (kRep set in the entry).
+    const uint64_t valid = __ballot(v);
+    if (valid) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(valid);
+      const uint64_t lk = __shfl(k, (int)leader);
+      const uint64_t same = __ballot(v && k == lk);
+      if (__popcll(same) >= 2) {
+        if (lane == leader)
+          k = lk | kRep;
+        else if ((same >> lane) & 1)
+          k = kFree;
+      }
+    }
+    kv[j] = k;
+    loc[j] = k != kFree ? atomicAdd(&hist[bucket_of(k & ~kRep, bk.mask)], 1u) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
+    const uint32_t c = hist[b];
+    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;  // this block's start in bucket b
+  }
+  __syncthreads();
+  bool over = false;
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    if (kv[j] == kFree)
+      continue;
+    const uint32_t b = bucket_of(kv[j] & ~kRep, bk.mask);
+    const uint64_t pos = (uint64_t)hist[b] + loc[j];
+    if (pos < bk.cap)
+      bk.entries[(uint64_t)b * bk.cap + pos] = kv[j];
+    else
+      over = true;
+  }
+  // One flag write per block: a full bucket must not turn every later key into
+  // an atomic on the same address.
+  if (__syncthreads_or(over) && threadIdx.x == 0)
+    atomicExch(bk.overflow, 1u);
+}
+
+__global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
+                                                              uint2 *__restrict__ block_counts) {
+  __shared__ unsigned long long tab[kBktSlots];
+  __shared__ uint32_t lds[kBktThreads / kWave + 1];
+  constexpr uint32_t kPer = kBktSlots / kBktThreads;
+  for (uint32_t i = threadIdx.x; i < kBktSlots; i += kBktThreads)
+    tab[i] = kFree;
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint64_t n = bk.fill[b] < bk.cap ? bk.fill[b] : bk.cap;
+  unsigned long long *ent = bk.entries + (uint64_t)b * bk.cap;
+  bool fail = false;
+  for (uint64_t e = threadIdx.x; e < n; e += kBktThreads) {
+    const uint64_t ev = ent[e];  // key, with kRep if already known repeated
+    const uint64_t k = ev & ~kRep;
+    uint32_t sl = (uint32_t)key_hash(k) & (kBktSlots - 1);
+    bool done = false;
+    for (uint32_t p = 0; p < kBktProbe && !done; ++p) {
+      const unsigned long long old = atomicCAS(&tab[sl], (unsigned long long)kFree,
+                                               (unsigned long long)ev);
+      if (old == kFree) {
+        done = true;
+      } else if ((old & ~kRep) == k) {
+        if (!(old & kRep))
+          atomicOr(&tab[sl], (unsigned long long)kRep);
+        done = true;
+      } else {
+        sl = (sl + 1) & (kBktSlots - 1);
+      }
+    }
+    fail = fail || !done;
+  }
+  if (__syncthreads_or(fail) && threadIdx.x == 0)
+    atomicExch(bk.overflow, 1u);
+  // Counts, and the repeated keys compacted to the front of the bucket.
+  uint32_t d = 0, r = 0;
+  unsigned long long v[kPer];
+#pragma unroll
+  for (uint32_t i = 0; i < kPer; ++i) {
+    v[i] = tab[threadIdx.x * kPer + i];
+    d += v[i] != kFree;
+    r += v[i] != kFree && (v[i] & kRep);
+  }
+  uint32_t total_r;
+  uint32_t off = block_exclusive_scan<kBktThreads>(r, lds, total_r);
+#pragma unroll
+  for (uint32_t i = 0; i < kPer; ++i)
+    if (v[i] != kFree && (v[i] & kRep))
+      ent[off++] = v[i] & ~kRep;
+  uint32_t total_d;
+  block_exclusive_scan<kBktThreads>(d, lds, total_d);
+  if (threadIdx.x == 0)
+    block_counts[b] = make_uint2(total_d, total_r);
+}
+
 // Sum of the per-block counts: out[0] = distinct keys, out[1] = repeated keys.
 __global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c, uint64_t n,
                                                      unsigned long long *__restrict__ out) {
@@ -521,15 +657,16 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c
 // ---- K1b: the repeated keys, moved into the rule table -------------------------
 // From the per-block staging lists: one wave per block list.
 __global__ __launch_bounds__(kBlock) void k_extract_staged(
-    const unsigned long long *__restrict__ stage, const uint2 *__restrict__ block_counts,
-    uint64_t n_blocks, Slot *__restrict__ rt, uint64_t rmask, uint32_t *__restrict__ err) {
+    const unsigned long long *__restrict__ stage, uint64_t stride,
+    const uint2 *__restrict__ block_counts, uint64_t n_blocks, Slot *__restrict__ rt,
+    uint64_t rmask, uint32_t *__restrict__ err) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t b = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (b >= n_blocks)
     return;
   const uint32_t n = block_counts[b].y;
-  for (uint32_t j = lane; j < n && j < kStageCap; j += kWave) {
-    const uint64_t k = stage[b * kStageCap + j];
+  for (uint64_t j = lane; j < n && j < stride; j += kWave) {
+    const uint64_t k = stage[b * stride + j];
     uint64_t s = key_hash(k) & rmask;
     bool placed = false;
     for (uint32_t p = 0; p < kMaxProbe && !placed; ++p) {
@@ -1024,15 +1161,59 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         round == 0 ? L / 8 : std::min<uint64_t>(2 * L, 4 * prev_distinct)));
     DBuf<unsigned long long> ctab;
     const uint64_t n_ins = ceil_div(L, kInsTile);
-    DBuf<uint2> bcounts(pool, n_ins);
-    DBuf<unsigned long long> stage(pool, n_ins * kStageCap);
+    DBuf<uint2> bcounts(pool, std::max<uint64_t>(n_ins, kMaxBuckets));
+    DBuf<unsigned long long> stage;
     DBuf<unsigned long long> sums(pool, 4);  // distinct, repeated, overflow, stage overflow
     uint64_t distinct = 0, repeated = 0;
-    // Once most pairs of the last round were distinct, tile aggregation only adds
-    // latency (distinct pairs only grow from round to round).
-    const bool direct = round > 0 && 2 * prev_distinct > prev_len;
+    // Once most pairs of the last round were distinct, count in hash buckets
+    // (distinct pairs only grow from round to round; earlier rounds hold keys
+    // hot enough to fill a bucket); the table counts the round if a bucket
+    // overflows anyway.
+    const bool mostly_distinct = round > 0 && 2 * prev_distinct > prev_len;
+    const bool direct = round > 0 && 2 * prev_distinct > prev_len;  // table fallback mode
     bool stage_overflow = false;
-    for (;;) {
+    uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
+    bool counted = false;
+    if (mostly_distinct) {
+      const uint64_t est = std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
+      const uint64_t nbk = std::min<uint64_t>(
+          kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
+      const uint64_t bcap = ceil_div(L, nbk) + ceil_div(L, nbk) / 4 + 2048;
+      stage = DBuf<unsigned long long>(pool, nbk * bcap);
+      DBuf<unsigned int> fill(pool, nbk);
+      fill_u32(ctx, fill.get(), 0, nbk);
+      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 4);
+      Buckets bk{stage.get(), bcap, (uint32_t)(nbk - 1), fill.get(),
+                 reinterpret_cast<uint32_t *>(sums.get() + 2)};
+      ctx.timed("pair_insert", [&] {
+        hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                           ctx.stream(), cur, heads.get(), L, span, bk);
+        GFZ_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
+                           ctx.stream(), bk, bcounts.get());
+        GFZ_CHECK_LAUNCH();
+      });
+      ctx.add_bytes("pair_insert", 4.0 * L);
+      hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), nbk,
+                         sums.get());
+      GFZ_CHECK_LAUNCH();
+      ctx.stage_scalars(sums.get(), 4);
+      ctx.sync();
+      const uint64_t *hc = ctx.scalars();
+      if (!hc[2]) {
+        distinct = hc[0];
+        repeated = hc[1];
+        counted = true;
+        stage_stride = bcap;
+        stage_blocks = nbk;
+        cap = nbk * kBktSlots;
+      } else {
+        info.table_retries += 1;  // a bucket overflowed: count with the table instead
+      }
+    }
+    if (!counted)
+      stage = DBuf<unsigned long long>(pool, n_ins * kStageCap);
+    for (; !counted;) {
       ctab = DBuf<unsigned long long>(pool, cap);
       fill_u64(ctx, reinterpret_cast<uint64_t *>(ctab.get()), kFree, cap);
       fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 4);
@@ -1058,6 +1239,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       if (!hc[2] && hc[0] <= cap / 2) {
         distinct = hc[0];
         repeated = hc[1];
+        counted = true;
         break;
       }
       cap *= 4;
@@ -1086,9 +1268,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                            dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt.get(), rcap - 1,
                            d_err.get());
       else
-        hipLaunchKernelGGL(k_extract_staged, dim3((uint32_t)ceil_div(n_ins, kBlock / kWave)),
-                           dim3(kBlock), 0, ctx.stream(), stage.get(), bcounts.get(), n_ins,
-                           rt.get(), rcap - 1, d_err.get());
+        hipLaunchKernelGGL(k_extract_staged,
+                           dim3((uint32_t)ceil_div(stage_blocks, kBlock / kWave)), dim3(kBlock),
+                           0, ctx.stream(), stage.get(), stage_stride, bcounts.get(),
+                           stage_blocks, rt.get(), rcap - 1, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     stage.reset();
