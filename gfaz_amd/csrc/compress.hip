@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
 // (2) each block folds its tile into an LDS table with counts; (3) every distinct
 // key of the tile is counted once in the HBM table, and the block adds its new
 // and newly repeated keys to the global counters with one atomic each.
-constexpr int kWaveMatch = 3;
+constexpr int kDict = 4;
 
 // LDS slot = key | kRep once the tile has seen the key twice (kRep is free:
 // compact keys are below 2^62), so the table is 8 bytes per slot.
@@ -344,17 +344,16 @@ __device__ inline void lds_add(unsigned long long *skeys, uint64_t k, uint32_t n
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restrict__ x,
-                                                        const uint32_t *__restrict__ heads,
-                                                        uint64_t L, uint64_t span, CountTable t) {
+__global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__restrict__ x,
+                                                           const uint32_t *__restrict__ heads,
+                                                           uint64_t L, uint64_t span,
+                                                           unsigned long long *__restrict__ out,
+                                                           uint32_t *__restrict__ out_n) {
   __shared__ unsigned long long skeys[kLdsSlots];
-  __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
-  __shared__ uint32_t s_stage_n;
-  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
   for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock)
     skeys[s] = kFree;
-  if (threadIdx.x == 0)
-    s_stage_n = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
   uint64_t kv[kInsItems];
@@ -365,16 +364,27 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
     valid[j] = i + 1 < L && !head_at(heads, i + 1);
     kv[j] = valid[j] ? pair_key(x[i], x[i + 1], span) : kFree;
   }
+  // A wave-wide dictionary of up to kDict keys with counts (wave-uniform values)
+  // absorbs the hot keys without LDS atomics; it is flushed once at the end.
+  uint64_t dkey[kDict];
+  uint32_t dcnt[kDict];
+#pragma unroll
+  for (int d = 0; d < kDict; ++d) {
+    dkey[d] = kFree;
+    dcnt[d] = 0;
+  }
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const uint64_t k = kv[j];
     uint64_t rem = __ballot(valid[j]);
-    // (1a) match the most common keys across the whole wave
-    for (int it = 0; it < kWaveMatch && rem; ++it) {
-      const uint32_t leader = (uint32_t)__builtin_ctzll(rem);
-      const uint64_t lk = __shfl(k, (int)leader);
-      const uint64_t same = __ballot(k == lk) & rem;
-      if (lane == leader)
-        lds_add(skeys, lk, (uint32_t)__popcll(same));
+    // (1a) keys already in the dictionary; a free entry takes the first other key
+#pragma unroll
+    for (int d = 0; d < kDict; ++d) {
+      if (!rem)
+        break;
+      if (dkey[d] == kFree)
+        dkey[d] = __shfl(k, (int)__builtin_ctzll(rem));
+      const uint64_t same = __ballot(k == dkey[d]) & rem;
+      dcnt[d] += (uint32_t)__popcll(same);
       rem &= ~same;
     }
     // (1b) the rest: runs of equal consecutive keys
@@ -390,31 +400,68 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
       lds_add(skeys, k, len);
     }
   }
+  if (lane < kDict) {
+    uint64_t fk = kFree;
+    uint32_t fc = 0;
+#pragma unroll
+    for (int d = 0; d < kDict; ++d)
+      if ((int)lane == d) {
+        fk = dkey[d];
+        fc = dcnt[d];
+      }
+    if (fc)
+      lds_add(skeys, fk, fc);
+  }
   __syncthreads();
+  // The tile's distinct keys (kRep: seen twice in the tile), compacted into the
+  // block's region of the staging buffer.
+  constexpr int kPer = kLdsSlots / kBlock;
+  unsigned long long v[kPer];
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    v[i] = skeys[threadIdx.x * kPer + i];
+    c += v[i] != kFree;
+  }
+  uint32_t total;
+  uint32_t off = block_exclusive_scan<kBlock>(c, lds, total);
+  unsigned long long *dst = out + (uint64_t)blockIdx.x * kInsTile;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i)
+    if (v[i] != kFree)
+      dst[off++] = v[i];
+  if (threadIdx.x == 0)
+    out_n[blockIdx.x] = total;
+}
+
+// Global inserts of the aggregated tiles: all of a thread's probe chains in flight.
+__global__ __launch_bounds__(kBlock) void k_pair_insert_staged(
+    const unsigned long long *__restrict__ in, const uint32_t *__restrict__ in_n, CountTable t) {
+  __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
+  __shared__ uint32_t s_stage_n;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0)
+    s_stage_n = 0;
+  __syncthreads();
+  const uint32_t n = in_n[blockIdx.x];
+  const unsigned long long *src = in + (uint64_t)blockIdx.x * kInsTile;
+  uint64_t kv[kInsItems];
+  uint32_t cnt[kInsItems];
+  uint32_t have = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const uint32_t e = j * kBlock + threadIdx.x;
+    const uint64_t v = e < n ? src[e] : kFree;
+    kv[j] = v & ~kRep;
+    cnt[j] = (v & kRep) ? 2u : 1u;
+    if (e < n)
+      have |= 1u << j;
+  }
   uint32_t fresh = 0, reps = 0;
-  {
-    constexpr int kPer = kLdsSlots / kBlock;
-    constexpr int kHalf = kPer / 2;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      uint64_t fk[kHalf];
-      uint32_t fc[kHalf];
-      uint32_t have = 0;
-#pragma unroll
-      for (int i = 0; i < kHalf; ++i) {
-        const uint32_t sl = threadIdx.x + (uint32_t)(h * kHalf + i) * kBlock;
-        const uint64_t v = skeys[sl];
-        fk[i] = v & ~kRep;
-        fc[i] = (v & kRep) ? 2u : 1u;
-        if (v != kFree)
-          have |= 1u << i;
-      }
-      if (have) {
-        uint32_t newrep = 0;
-        count_insert_batch<kHalf>(t, fk, fc, have, fresh, reps, newrep);
-        stage_keys<kHalf>(t, &s_stage_n, fk, newrep);
-      }
-    }
+  if (have) {
+    uint32_t newrep = 0;
+    count_insert_batch<(int)kInsItems>(t, kv, cnt, have, fresh, reps, newrep);
+    stage_keys<(int)kInsItems>(t, &s_stage_n, kv, newrep);
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
@@ -427,12 +474,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert(const int32_t *__restric
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t n = 0, r = 0;
+    uint32_t nf = 0, r = 0;
     for (uint32_t w = 0; w < kBlock / kWave; ++w) {
-      n += sfresh[w];
+      nf += sfresh[w];
       r += sreps[w];
     }
-    t.block_counts[blockIdx.x] = make_uint2(n, r);
+    t.block_counts[blockIdx.x] = make_uint2(nf, r);
   }
 }
 
@@ -564,6 +611,47 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
   }
   // One flag write per block: a full bucket must not turn every later key into
   // an atomic on the same address.
+  if (__syncthreads_or(over) && threadIdx.x == 0)
+    atomicExch(bk.overflow, 1u);
+}
+
+// Same partitioning for aggregated tiles (entries already carry kRep).
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
+    const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
+    Buckets bk) {
+  __shared__ uint32_t hist[kMaxBuckets];
+  const uint32_t nb = bk.mask + 1;
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+    hist[i] = 0;
+  __syncthreads();
+  const uint32_t n = tiles_n[blockIdx.x];
+  const unsigned long long *src = tiles + (uint64_t)blockIdx.x * kInsTile;
+  uint64_t kv[kInsItems];
+  uint32_t loc[kInsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    const uint32_t e = j * kBlock + threadIdx.x;
+    kv[j] = e < n ? src[e] : kFree;
+    loc[j] = e < n ? atomicAdd(&hist[bucket_of(kv[j] & ~kRep, bk.mask)], 1u) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
+    const uint32_t c = hist[b];
+    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;
+  }
+  __syncthreads();
+  bool over = false;
+#pragma unroll
+  for (uint32_t j = 0; j < kInsItems; ++j) {
+    if (kv[j] == kFree)
+      continue;
+    const uint32_t b = bucket_of(kv[j] & ~kRep, bk.mask);
+    const uint64_t pos = (uint64_t)hist[b] + loc[j];
+    if (pos < bk.cap)
+      bk.entries[(uint64_t)b * bk.cap + pos] = kv[j];
+    else
+      over = true;
+  }
   if (__syncthreads_or(over) && threadIdx.x == 0)
     atomicExch(bk.overflow, 1u);
 }
@@ -1170,11 +1258,17 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // hot enough to fill a bucket); the table counts the round if a bucket
     // overflows anyway.
     const bool mostly_distinct = round > 0 && 2 * prev_distinct > prev_len;
+    DBuf<unsigned long long> tiles;  // aggregated tiles (rounds with hot keys)
+    DBuf<uint32_t> tiles_n;
     const bool direct = round > 0 && 2 * prev_distinct > prev_len;  // table fallback mode
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
     bool counted = false;
-    if (mostly_distinct) {
+    // Bucket counting pays one global atomic per (tile, bucket): while tile
+    // aggregation leaves few distinct keys (under 5% of pairs in the last
+    // round), inserting the aggregated tiles into the table is cheaper.
+    const bool use_buckets = round > 0 && 20 * prev_distinct > prev_len;
+    if (use_buckets) {
       const uint64_t est = std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
       const uint64_t nbk = std::min<uint64_t>(
           kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
@@ -1186,8 +1280,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       Buckets bk{stage.get(), bcap, (uint32_t)(nbk - 1), fill.get(),
                  reinterpret_cast<uint32_t *>(sums.get() + 2)};
       ctx.timed("pair_insert", [&] {
-        hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                           ctx.stream(), cur, heads.get(), L, span, bk);
+        if (mostly_distinct) {
+          hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                             ctx.stream(), cur, heads.get(), L, span, bk);
+        } else {
+          // Hot keys: aggregate each tile first, so a key enters a bucket at most
+          // once per tile.
+          tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
+          tiles_n = DBuf<uint32_t>(pool, n_ins);
+          hipLaunchKernelGGL(k_pair_aggregate, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                             ctx.stream(), cur, heads.get(), L, span, tiles.get(),
+                             tiles_n.get());
+          GFZ_CHECK_LAUNCH();
+          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                             ctx.stream(), tiles.get(), tiles_n.get(), bk);
+        }
         GFZ_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
                            ctx.stream(), bk, bcounts.get());
@@ -1220,12 +1327,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       CountTable t{ctab.get(), cap - 1, reinterpret_cast<uint32_t *>(sums.get() + 2),
                    bcounts.get(), stage.get(), reinterpret_cast<uint32_t *>(sums.get() + 3)};
       ctx.timed("pair_insert", [&] {
-        if (direct)
+        if (direct) {
           hipLaunchKernelGGL(k_pair_insert_direct, dim3((uint32_t)n_ins), dim3(kBlock), 0,
                              ctx.stream(), cur, heads.get(), L, span, t);
-        else
-          hipLaunchKernelGGL(k_pair_insert, dim3((uint32_t)n_ins), dim3(kBlock), 0, ctx.stream(),
-                             cur, heads.get(), L, span, t);
+        } else {
+          if (!tiles.get()) {  // the aggregation does not depend on the table size
+            tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
+            tiles_n = DBuf<uint32_t>(pool, n_ins);
+            hipLaunchKernelGGL(k_pair_aggregate, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                               ctx.stream(), cur, heads.get(), L, span, tiles.get(),
+                               tiles_n.get());
+            GFZ_CHECK_LAUNCH();
+          }
+          hipLaunchKernelGGL(k_pair_insert_staged, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                             ctx.stream(), tiles.get(), tiles_n.get(), t);
+        }
       });
       GFZ_CHECK_LAUNCH();
       ctx.add_bytes("pair_insert", 4.0 * L);
