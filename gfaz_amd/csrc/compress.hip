@@ -86,62 +86,109 @@ struct CountTable {
 };
 constexpr uint32_t kStageCap = 512;
 
-// Rule table (16-byte slots): the repeated keys only, dense enough to stay in
-// cache in late rounds. val = final rule ID after the sort; the slot index is
-// the provisional rule ID within the round.
-struct alignas(16) Slot {
-  unsigned long long key;  // kFree = empty
-  uint32_t val;
-  uint32_t pad;
+// Rule table: the repeated keys of a round, kLine to a 64-byte line. A key sits
+// in the first line, from its hash on, that had a free slot when it was
+// inserted; keys never leave, so a lookup stops at the first line holding the
+// key or a free slot. At load <= 1/4 nearly every lookup reads one line (a
+// probe chain of single slots at load 1/2 reads 15-20 in the worst of a
+// wave's 512 lookups). The slot index is the key's provisional rule ID within
+// the round; vals[slot] becomes its final ID.
+constexpr uint32_t kLine = 8;
+struct RuleTable {
+  unsigned long long *keys;  // kFree = empty; 64-byte aligned
+  uint32_t *vals;
+  uint64_t lmask;            // lines - 1 (a power of two minus one)
 };
 
-__device__ inline int64_t table_find(const Slot *slots, uint64_t mask, uint64_t k) {
-  uint64_t s = key_hash(k) & mask;
-  for (uint64_t p = 0; p <= mask; ++p) {
-    const uint64_t c = slots[s].key;
-    if (c == k)
-      return (int64_t)s;
-    if (c == kFree)
-      return -1;
-    s = (s + 1) & mask;
+__device__ inline void load_line(const unsigned long long *keys, uint64_t line,
+                                 uint64_t (&w)[kLine]) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(keys + line * kLine);
+#pragma unroll
+  for (uint32_t q = 0; q < kLine / 2; ++q) {
+    const uint4 v = p[q];
+    w[2 * q] = (uint64_t)v.x | ((uint64_t)v.y << 32);
+    w[2 * q + 1] = (uint64_t)v.z | ((uint64_t)v.w << 32);
   }
-  return -1;
 }
 
-// Probe N keys at once: every round of the loop issues one load per pending key
-// before looking at any result. slot[j] = -1 when absent or not valid.
+// Looks up N keys at once: every round of the loop issues the line loads of all
+// pending keys before looking at any of them. slot[j] = -1 when absent or not
+// valid.
 template <int N>
-__device__ inline void table_find_batch(const Slot *__restrict__ slots, uint64_t mask,
-                                        const uint64_t (&k)[N], const bool (&valid)[N],
-                                        int64_t (&slot)[N]) {
-  uint64_t s[N];
+__device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[N],
+                                       const bool (&valid)[N], int64_t (&slot)[N]) {
+  uint64_t l[N];
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    s[j] = key_hash(k[j]) & mask;
+    l[j] = key_hash(k[j]) & rt.lmask;
     slot[j] = -1;
     if (valid[j])
       pending |= 1u << j;
   }
-  for (uint64_t p = 0; pending && p <= mask; ++p) {
-    uint64_t c[N];
+  for (uint64_t p = 0; pending && p <= rt.lmask; ++p) {
+    uint64_t w[N][kLine];
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      c[j] = (pending >> j) & 1 ? slots[s[j]].key : 0;
+      if ((pending >> j) & 1)
+        load_line(rt.keys, l[j], w[j]);
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((pending >> j) & 1))
         continue;
-      if (c[j] == k[j]) {
-        slot[j] = (int64_t)s[j];
+      int hit = -1;
+      bool has_free = false;
+#pragma unroll
+      for (uint32_t q = 0; q < kLine; ++q) {
+        hit = w[j][q] == k[j] ? (int)q : hit;
+        has_free |= w[j][q] == kFree;
+      }
+      if (hit >= 0) {
+        slot[j] = (int64_t)(l[j] * kLine + (uint32_t)hit);
         pending &= ~(1u << j);
-      } else if (c[j] == kFree) {
+      } else if (has_free) {
         pending &= ~(1u << j);
       } else {
-        s[j] = (s[j] + 1) & mask;
+        l[j] = (l[j] + 1) & rt.lmask;
       }
     }
   }
+}
+
+__device__ inline int64_t rule_find(const RuleTable &rt, uint64_t k) {
+  const uint64_t kk[1] = {k};
+  const bool v[1] = {true};
+  int64_t s[1];
+  rule_find_batch<1>(rt, kk, v, s);
+  return s[0];
+}
+
+// Inserts a key that is not in the table yet; false when kMaxProbe lines were
+// full. Which slots are taken is decided by the CAS results alone: a relaxed
+// read of the line may show a slot free that another block has just filled
+// (the CAS then fails and the next slot is tried), never the reverse, so
+// starting at the first slot seen free keeps every key in the first line that
+// had room.
+__device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
+  uint64_t l = key_hash(k) & rt.lmask;
+  for (uint32_t p = 0; p < kMaxProbe; ++p) {
+    unsigned long long *line = rt.keys + l * kLine;
+    uint64_t w[kLine];
+#pragma unroll
+    for (uint32_t q = 0; q < kLine; ++q)
+      w[q] = __hip_atomic_load(line + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t j = 0;
+    while (j < kLine && w[j] != kFree)
+      ++j;
+    for (; j < kLine; ++j) {
+      const unsigned long long old =
+          atomicCAS(line + j, (unsigned long long)kFree, (unsigned long long)k);
+      if (old == kFree || old == k)
+        return true;
+    }
+    l = (l + 1) & rt.lmask;
+  }
+  return false;
 }
 
 // Count up to N keys (bit j of `have`), key j seen cnt[j] times in the caller's
@@ -238,17 +285,6 @@ __device__ inline void stage_keys(const CountTable &t, uint32_t *s_n, const uint
       t.stage[(uint64_t)blockIdx.x * kStageCap + pos] = k[j];
     else
       atomicExch(t.stage_overflow, 1u);
-  }
-}
-
-__global__ void k_fill_slots(Slot *slots, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    Slot e;
-    e.key = kFree;
-    e.val = 0;
-    e.pad = 0;
-    slots[i] = e;
   }
 }
 
@@ -746,64 +782,30 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c
 // From the per-block staging lists: one wave per block list.
 __global__ __launch_bounds__(kBlock) void k_extract_staged(
     const unsigned long long *__restrict__ stage, uint64_t stride,
-    const uint2 *__restrict__ block_counts, uint64_t n_blocks, Slot *__restrict__ rt,
-    uint64_t rmask, uint32_t *__restrict__ err) {
+    const uint2 *__restrict__ block_counts, uint64_t n_blocks, RuleTable rt,
+    uint32_t *__restrict__ err) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t b = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (b >= n_blocks)
     return;
   const uint32_t n = block_counts[b].y;
-  for (uint64_t j = lane; j < n && j < stride; j += kWave) {
-    const uint64_t k = stage[b * stride + j];
-    uint64_t s = key_hash(k) & rmask;
-    bool placed = false;
-    for (uint32_t p = 0; p < kMaxProbe && !placed; ++p) {
-      const unsigned long long old =
-          atomicCAS(&rt[s].key, (unsigned long long)kFree, (unsigned long long)k);
-      if (old == kFree)
-        placed = true;
-      else
-        s = (s + 1) & rmask;
-    }
-    if (!placed)
+  for (uint64_t j = lane; j < n && j < stride; j += kWave)
+    if (!rule_insert(rt, stage[b * stride + j]))
       atomicExch(err, 3u);
-  }
 }
 
+// From the counting table itself (when some block staged too many keys).
 constexpr uint32_t kExtractItems = 16;
 
 __global__ __launch_bounds__(kBlock) void k_extract_repeated(
-    const unsigned long long *__restrict__ count_slots, uint64_t cap, Slot *__restrict__ rt,
-    uint64_t rmask, uint32_t *__restrict__ err) {
+    const unsigned long long *__restrict__ count_slots, uint64_t cap, RuleTable rt,
+    uint32_t *__restrict__ err) {
   const uint64_t s0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kExtractItems;
-  uint64_t k[kExtractItems], s[kExtractItems];
-  uint32_t pending = 0;
-#pragma unroll
-  for (uint32_t i = 0; i < kExtractItems; ++i) {
-    const uint64_t v = s0 + i < cap ? count_slots[s0 + i] : kFree;
-    k[i] = v & ~kRep;
-    if (v != kFree && (v & kRep))
-      pending |= 1u << i;
-    s[i] = key_hash(k[i]) & rmask;
+  for (uint32_t i = 0; i < kExtractItems && s0 + i < cap; ++i) {
+    const uint64_t v = count_slots[s0 + i];
+    if (v != kFree && (v & kRep) && !rule_insert(rt, v & ~kRep))
+      atomicExch(err, 3u);
   }
-  for (uint32_t p = 0; pending && p < kMaxProbe; ++p) {
-    unsigned long long old[kExtractItems];
-#pragma unroll
-    for (uint32_t i = 0; i < kExtractItems; ++i)
-      if ((pending >> i) & 1)
-        old[i] = atomicCAS(&rt[s[i]].key, (unsigned long long)kFree, (unsigned long long)k[i]);
-#pragma unroll
-    for (uint32_t i = 0; i < kExtractItems; ++i) {
-      if (!((pending >> i) & 1))
-        continue;
-      if (old[i] == kFree)
-        pending &= ~(1u << i);
-      else
-        s[i] = (s[i] + 1) & rmask;
-    }
-  }
-  if (pending)
-    atomicExch(err, 3u);
 }
 
 // Inclusive scan of transfer functions across the 64 lanes of a wave.
@@ -818,43 +820,204 @@ __device__ inline uint32_t wave_fn_inclusive(uint32_t f) {
   return f;
 }
 
-// ---- K2: candidate bits and each tile's transfer function --------------------
-// A pair is a candidate iff its key is in the rule table (the repeated keys).
-constexpr int kProbeBatch = 8;
-constexpr int kEmitBatch = 4;
+// ---- K1c: the round's hottest keys, cached in LDS by K2 and K3 ---------------
+// A few keys can make up most pairs of early rounds (the backbone's own pairs:
+// one key is three quarters of all pairs of the first round of a 64-haplotype
+// pangenome). Every wave then reads the same rule-table lines, and those reads
+// queue on one cache channel per die. One block samples 64 windows of 64
+// consecutive pairs spread over the stream, takes the keys seen at least
+// kHotMin times (at most 64), resolves their slots and publishes them as a
+// small open-addressing table that K2 and K3 copy into LDS. A key missing from
+// it is simply looked up in the rule table, so the sample only decides speed.
+constexpr uint32_t kHotWindows = 64, kHotWin = 64;   // 4096 sampled pairs
+constexpr uint32_t kHotSampleSlots = 8192;
+constexpr uint32_t kHotMin = 64;                     // 1.6% of the sample
+constexpr uint32_t kHotSlots = 128;                  // <= 64 keys: load <= 1/2
+constexpr uint32_t kHotThreads = 1024;
 
-__global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
-                                                 const uint32_t *__restrict__ heads, uint64_t L,
-                                                 uint64_t span, const Slot *__restrict__ rt,
-                                                 uint64_t rmask, uint64_t *__restrict__ cand,
-                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn) {
-  __shared__ uint64_t wbits[kRtWords];
-  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
-  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kProbeBatch) {
-    uint64_t k[kProbeBatch];
-    bool valid[kProbeBatch];
-#pragma unroll
-    for (int b = 0; b < kProbeBatch; ++b) {
-      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
-      valid[b] = i + 1 < L && !head_at(heads, i + 1);
-      k[b] = valid[b] ? pair_key(x[i], x[i + 1], span) : 0;
-    }
-    int64_t slot[kProbeBatch];
-    table_find_batch<kProbeBatch>(rt, rmask, k, valid, slot);
-#pragma unroll
-    for (int b = 0; b < kProbeBatch; ++b) {
-      const uint64_t m = __ballot(slot[b] >= 0);
-      if (lane == 0) {
-        const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
-        wbits[w] = m;
-        const uint64_t gw = (base >> 6) + w;
-        if (gw < n_words)
-          cand[gw] = m;
+struct alignas(16) HotEntry {
+  unsigned long long key;  // kFree = empty
+  uint32_t slot;
+  uint32_t pad;
+};
+
+__global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__restrict__ x,
+                                                          const uint32_t *__restrict__ heads,
+                                                          uint64_t L, uint64_t span, RuleTable rt,
+                                                          HotEntry *__restrict__ hot) {
+  __shared__ unsigned long long skey[kHotSampleSlots];
+  __shared__ uint32_t scnt[kHotSampleSlots];
+  __shared__ unsigned long long hkey[kHotSlots];
+  __shared__ uint32_t hslot[kHotSlots];
+  for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
+    skey[s] = kFree;
+    scnt[s] = 0;
+  }
+  for (uint32_t s = threadIdx.x; s < kHotSlots; s += kHotThreads) {
+    hkey[s] = kFree;
+    hslot[s] = 0;
+  }
+  __syncthreads();
+  const uint64_t room = L > kHotWin + 1 ? L - kHotWin - 1 : 0;
+  for (uint32_t q = threadIdx.x; q < kHotWindows * kHotWin; q += kHotThreads) {
+    const uint64_t i = (uint64_t)(q / kHotWin) * room / kHotWindows + q % kHotWin;
+    if (i + 1 >= L || head_at(heads, i + 1))
+      continue;
+    const unsigned long long k = pair_key(x[i], x[i + 1], span);
+    uint32_t sl = (uint32_t)key_hash(k) & (kHotSampleSlots - 1);
+    for (uint32_t p = 0; p < kHotSampleSlots; ++p) {
+      const unsigned long long old = atomicCAS(&skey[sl], (unsigned long long)kFree, k);
+      if (old == kFree || old == k) {
+        atomicAdd(&scnt[sl], 1u);
+        break;
       }
+      sl = (sl + 1) & (kHotSampleSlots - 1);
     }
   }
   __syncthreads();
+  for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
+    if (scnt[s] < kHotMin)
+      continue;
+    const unsigned long long k = skey[s];
+    const int64_t slot = rule_find(rt, k);
+    if (slot < 0)
+      continue;
+    uint32_t h = (uint32_t)key_hash(k) & (kHotSlots - 1);
+    for (uint32_t p = 0; p < kHotSlots; ++p) {
+      if (atomicCAS(&hkey[h], (unsigned long long)kFree, k) == kFree) {
+        hslot[h] = (uint32_t)slot;
+        break;
+      }
+      h = (h + 1) & (kHotSlots - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t s = threadIdx.x; s < kHotSlots; s += kHotThreads) {
+    HotEntry e;
+    e.key = hkey[s];
+    e.slot = hslot[s];
+    e.pad = 0;
+    hot[s] = e;
+  }
+}
+
+struct HotLds {
+  unsigned long long key[kHotSlots];
+  uint32_t slot[kHotSlots];
+};
+
+__device__ inline void hot_load(HotLds &h, const HotEntry *__restrict__ hot) {
+  for (uint32_t s = threadIdx.x; s < kHotSlots; s += blockDim.x) {
+    const HotEntry e = hot[s];
+    h.key[s] = e.key;
+    h.slot[s] = e.slot;
+  }
+}
+
+__device__ inline int64_t hot_find(const HotLds &h, uint64_t k) {
+  uint32_t s = (uint32_t)key_hash(k) & (kHotSlots - 1);
+  for (uint32_t p = 0; p < kHotSlots; ++p) {
+    const unsigned long long c = h.key[s];
+    if (c == k)
+      return (int64_t)h.slot[s];
+    if (c == kFree)
+      return -1;
+    s = (s + 1) & (kHotSlots - 1);
+  }
+  return -1;
+}
+
+// ---- K2: candidate bits and each tile's transfer function --------------------
+// A pair is a candidate iff its key is in the rule table (the repeated keys).
+// Per pass of kPassItems items a lane reads its pairs and resolves hot keys from
+// LDS; the other keys go to a per-wave LDS list, which the wave then looks up
+// kColdBatch keys per lane at a time, writing each result over its key. The
+// line reads of the lookups thus run once per wave and pass, over only the keys
+// that need them, and their registers are never live beside the stream's.
+constexpr uint32_t kPassItems = 8;
+constexpr uint32_t kColdCap = kPassItems * kWave;   // every item of a pass cold
+constexpr int kColdBatch = 2;
+
+// Looks up the wave's n listed keys; each entry becomes the key's slot, or
+// kFree when absent.
+__device__ inline void lookup_cold(const RuleTable &rt, unsigned long long *list, uint32_t n) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint32_t e0 = 0; e0 < n; e0 += kWave * kColdBatch) {
+    uint64_t kk[kColdBatch];
+    bool v[kColdBatch];
+#pragma unroll
+    for (int q = 0; q < kColdBatch; ++q) {
+      const uint32_t e = e0 + (uint32_t)q * kWave + lane;
+      v[q] = e < n;
+      kk[q] = v[q] ? list[e] : 0;
+    }
+    int64_t sl[kColdBatch];
+    rule_find_batch<kColdBatch>(rt, kk, v, sl);
+#pragma unroll
+    for (int q = 0; q < kColdBatch; ++q)
+      if (v[q])
+        list[e0 + (uint32_t)q * kWave + lane] = sl[q] >= 0 ? (unsigned long long)sl[q] : kFree;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
+                                                 const uint32_t *__restrict__ heads, uint64_t L,
+                                                 uint64_t span, RuleTable rt,
+                                                 const HotEntry *__restrict__ hot,
+                                                 uint64_t *__restrict__ cand,
+                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn) {
+  __shared__ uint64_t wbits[kRtWords];
+  __shared__ HotLds sh;
+  __shared__ unsigned long long cold[kBlock / kWave][kColdCap];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
+  const uint64_t below = lanemask_lt();
+  hot_load(sh, hot);
+  __syncthreads();
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
+    int32_t xa[kPassItems], xb[kPassItems];
+    bool valid[kPassItems];
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
+      valid[b] = i + 1 < L && !head_at(heads, i + 1);
+      xa[b] = valid[b] ? x[i] : 0;
+      xb[b] = valid[b] ? x[i + 1] : 0;
+    }
+    uint32_t hit = 0, n = 0;
+    uint64_t m[kPassItems];
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      const uint64_t k = pair_key(xa[b], xb[b], span);
+      const bool h = valid[b] && hot_find(sh, k) >= 0;
+      const bool c = valid[b] && !h;
+      hit |= h ? 1u << b : 0u;
+      m[b] = __ballot(c);
+      if (c)
+        cold[wid][n + (uint32_t)__popcll(m[b] & below)] = k;
+      n += (uint32_t)__popcll(m[b]);
+    }
+    __syncthreads();
+    lookup_cold(rt, cold[wid], n);
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      bool f = (hit >> b) & 1;
+      if ((m[b] >> lane) & 1)
+        f = cold[wid][pre + (uint32_t)__popcll(m[b] & below)] != kFree;
+      pre += (uint32_t)__popcll(m[b]);
+      const uint64_t word = __ballot(f);
+      if (lane == 0) {
+        const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+        wbits[w] = word;
+        const uint64_t gw = (base >> 6) + w;
+        if (gw < n_words)
+          cand[gw] = word;
+      }
+    }
+    __syncthreads();  // the list is refilled by the next pass
+  }
   if (wid == 0) {
     const uint32_t f = wave_fn_inclusive(word_fn(wbits[lane]));
     if (lane == kWave - 1)
@@ -936,17 +1099,21 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    uint64_t span, const Slot *__restrict__ rt, uint64_t rmask, uint8_t *__restrict__ used,
+    uint64_t span, RuleTable rt, const HotEntry *__restrict__ hot, uint8_t *__restrict__ used,
     uint32_t next, int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
     uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
+  __shared__ HotLds sh;
+  __shared__ unsigned long long cold[kBlock / kWave][kColdCap];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t below = lanemask_lt();
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const uint64_t info = tile_info[blockIdx.x];
   const uint32_t p_tile = (uint32_t)(info & 1);
   const uint64_t off_tile = info >> 1;
+  hot_load(sh, hot);
   if (wid == 0) {
     const uint64_t gw = (base >> 6) + lane;
     const uint64_t c = gw < n_words ? cand[gw] : 0;
@@ -973,58 +1140,67 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     soff[lane] = sx - sc;
   }
   __syncthreads();
-  constexpr int kEb = kEmitBatch;
-  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kEb) {
-    int32_t v[kEb];
-    uint64_t kk[kEb];
-    bool pk[kEb], neg[kEb], sv[kEb], hd[kEb];
-    uint64_t pos[kEb];
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
+    // Survivors that are not picked, and picks of hot keys, are written at once;
+    // picks of other keys wait for the wave's lookup pass.
+    uint64_t m[kPassItems];
+    uint32_t n = 0, neg = 0;
 #pragma unroll
-    for (int b = 0; b < kEb; ++b) {
+    for (uint32_t b = 0; b < kPassItems; ++b) {
       const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
       const uint64_t i = base + (uint64_t)w * 64 + lane;
       const uint64_t svm = ssurv[w];
-      sv[b] = i < L && ((svm >> lane) & 1);
-      pk[b] = false;
-      neg[b] = false;
-      kk[b] = 0;
-      v[b] = 0;
-      pos[b] = 0;
-      hd[b] = false;
-      if (sv[b]) {
-        pos[b] = off_tile + soff[w] + (uint64_t)__popcll(svm & lanemask_lt());
-        v[b] = x[i];
-        hd[b] = head_at(heads, i);
+      bool c = false;
+      uint64_t ck = 0;
+      if (i < L && ((svm >> lane) & 1)) {
+        const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(svm & below);
+        int32_t o = x[i];
         if ((spicks[w] >> lane) & 1) {
-          const int64_t k = pack2(v[b], x[i + 1]);
-          const int64_t c = canon2(k);
-          kk[b] = compact_key(c, span);
-          neg[b] = k != c;
-          pk[b] = true;
+          const int64_t k = pack2(o, x[i + 1]);
+          const int64_t cc = canon2(k);
+          ck = compact_key(cc, span);
+          const int64_t hs = hot_find(sh, ck);
+          if (hs >= 0) {
+            used[hs] = 1;
+            o = (int32_t)(next + (uint32_t)hs);
+            if (k != cc)
+              o = -o;
+          } else {
+            c = true;
+            neg |= k != cc ? 1u << b : 0u;
+          }
         }
+        if (!c)
+          y[pos] = o;
+        if (head_at(heads, i))
+          atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
       }
+      m[b] = __ballot(c);
+      if (c)
+        cold[wid][n + (uint32_t)__popcll(m[b] & below)] = ck;
+      n += (uint32_t)__popcll(m[b]);
     }
-    int64_t slot[kEb];
-    table_find_batch<kEb>(rt, rmask, kk, pk, slot);
+    __syncthreads();
+    lookup_cold(rt, cold[wid], n);
+    __syncthreads();
+    uint32_t pre = 0;
 #pragma unroll
-    for (int b = 0; b < kEb; ++b) {
-      if (!sv[b])
-        continue;
-      int32_t o = v[b];
-      if (pk[b]) {
-        if (slot[b] < 0) {
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      if ((m[b] >> lane) & 1) {
+        const unsigned long long sl = cold[wid][pre + (uint32_t)__popcll(m[b] & below)];
+        if (sl == kFree) {
           atomicExch(err, 1u);
         } else {
-          used[slot[b]] = 1;
-          o = (int32_t)(next + (uint32_t)slot[b]);
-          if (neg[b])
-            o = -o;
+          const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+          const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(ssurv[w] & below);
+          used[sl] = 1;
+          int32_t o = (int32_t)(next + (uint32_t)sl);
+          y[pos] = (neg >> b) & 1 ? -o : o;
         }
       }
-      y[pos[b]] = o;
-      if (hd[b])
-        atomicOr(&heads_out[pos[b] >> 5], 1u << (pos[b] & 31));
+      pre += (uint32_t)__popcll(m[b]);
     }
+    __syncthreads();  // the list is refilled by the next pass
   }
 }
 
@@ -1036,7 +1212,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(
 constexpr uint32_t kCollectItems = 16;
 constexpr uint32_t kCollectTile = kBlock * kCollectItems;
 
-__global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict__ rt,
+__global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long long *__restrict__ rkeys,
                                                          const uint8_t *__restrict__ used,
                                                          uint64_t cap, uint64_t *__restrict__ out,
                                                          unsigned long long *scal) {
@@ -1067,7 +1243,7 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict_
   uint64_t kor = 0, kand = ~0ull;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    const uint64_t key = rt[s0 + q].key;
+    const uint64_t key = rkeys[s0 + q];
     out[b + k++] = key;
     kor |= key;
     kand &= key;
@@ -1098,20 +1274,19 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const Slot *__restrict_
 
 // ---- K6: final IDs next + rank, rule columns ----------------------------------
 __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
-                                                   uint64_t span, Slot *__restrict__ rt,
-                                                   uint64_t rmask, uint32_t next,
+                                                   uint64_t span, RuleTable rt, uint32_t next,
                                                    int32_t *__restrict__ rf,
                                                    int32_t *__restrict__ rs,
                                                    uint32_t *__restrict__ err) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
     const uint64_t ck = sorted[r];
-    const int64_t s = table_find(rt, rmask, ck);
+    const int64_t s = rule_find(rt, ck);
     if (s < 0) {
       atomicExch(err, 2u);
       continue;
     }
-    rt[s].val = next + (uint32_t)r;
+    rt.vals[s] = next + (uint32_t)r;
     const int64_t k = expand_key(ck, span);
     rf[r] = key_first(k);
     rs[r] = key_second(k);
@@ -1120,13 +1295,13 @@ __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ 
 
 // ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
 __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
-                                                  const Slot *__restrict__ rt, uint32_t next) {
+                                                  const uint32_t *__restrict__ vals, uint32_t next) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock) {
     const int32_t v = y[i];
     const uint32_t a = uabs(v);
     if (a >= next) {
-      const int32_t f = (int32_t)rt[a - next].val;
+      const int32_t f = (int32_t)vals[a - next];
       y[i] = v > 0 ? f : -f;
     }
   }
@@ -1368,30 +1543,36 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     if (repeated == 0)
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
-    // K1b: rule table of the repeated keys (load <= 1/2).
-    const uint64_t rcap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)repeated));
+    // K1b: rule table of the repeated keys (load <= 1/4, whole 64-byte lines).
+    const uint64_t rcap = next_pow2(std::max<uint64_t>(1024, 4 * (uint64_t)repeated));
     if ((uint64_t)next + rcap >= (uint64_t)INT32_MAX)
       throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
                                     ", rule table " + std::to_string(rcap) + ")");
-    DBuf<Slot> rt(pool, rcap);
-    hipLaunchKernelGGL(k_fill_slots, dim3(stride_grid(ctx, rcap)), dim3(kBlock), 0, ctx.stream(),
-                       rt.get(), rcap);
-    GFZ_CHECK_LAUNCH();
+    DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
+    DBuf<uint32_t> rvals(pool, rcap);
+    fill_u64(ctx, reinterpret_cast<uint64_t *>(rkeys.get()), kFree, rcap);
+    const RuleTable rt{rkeys.get(), rvals.get(), rcap / kLine - 1};
     ctx.timed("extract_repeated", [&] {
       if (stage_overflow)  // some block staged too many: scan the whole table
         hipLaunchKernelGGL(k_extract_repeated,
                            dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
-                           dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt.get(), rcap - 1,
-                           d_err.get());
+                           dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt, d_err.get());
       else
         hipLaunchKernelGGL(k_extract_staged,
                            dim3((uint32_t)ceil_div(stage_blocks, kBlock / kWave)), dim3(kBlock),
                            0, ctx.stream(), stage.get(), stage_stride, bcounts.get(),
-                           stage_blocks, rt.get(), rcap - 1, d_err.get());
+                           stage_blocks, rt, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     stage.reset();
     ctab.reset();
+
+    DBuf<HotEntry> hot(pool, kHotSlots);
+    ctx.timed("cand_tiles", [&] {
+      hipLaunchKernelGGL(k_hot_sample, dim3(1), dim3(kHotThreads), 0, ctx.stream(), cur,
+                         heads.get(), L, span, rt, hot.get());
+    });
+    GFZ_CHECK_LAUNCH();
 
     // K2 + K2b: candidates, tile carries, next length. Round scalars (read back
     // once, after K4): [0] next length, [1] used keys, [2] OR and [3] AND of them.
@@ -1403,7 +1584,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     fill_u64(ctx, reinterpret_cast<uint64_t *>(scal.get()) + 3, ~0ull, 1);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, span, rt.get(), rcap - 1, cand.get(), n_words,
+                         heads.get(), L, span, rt, hot.get(), cand.get(), n_words,
                          tile_fn.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -1421,8 +1602,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     fill_u32(ctx, heads_next.get(), 0, ceil_div(L, 32));
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt.get(),
-                         rcap - 1, used.get(), next, dst, heads_next.get(), d_err.get());
+                         heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt,
+                         hot.get(), used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
 
@@ -1430,7 +1611,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);
     ctx.timed("collect_used", [&] {
       hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(rcap, kCollectTile)),
-                         dim3(kBlock), 0, ctx.stream(), rt.get(), used.get(), rcap, ukeys.get(),
+                         dim3(kBlock), 0, ctx.stream(), rkeys.get(), used.get(), rcap, ukeys.get(),
                          scal.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -1467,14 +1648,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     }
     ctx.timed("assign_ids", [&] {
       hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                         ukeys.get(), R, span, rt.get(), rcap - 1, next, rf.get() + out.n_rules,
+                         ukeys.get(), R, span, rt, next, rf.get() + out.n_rules,
                          rs.get() + out.n_rules, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     // K7: remap provisional IDs (errors of K6 surface at the next readback).
     ctx.timed("remap", [&] {
       hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
-                         dst, L_next, rt.get(), next);
+                         dst, L_next, rvals.get(), next);
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);

@@ -263,9 +263,9 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
 
 // ---- one-sweep LSD radix sort -------------------------------------------------
 // One histogram kernel for every digit, then one kernel per digit: each block
-// takes the next tile by ticket, ranks its keys stably (item, then wave, then lane),
-// publishes its per-digit counts, and finds the counts of all earlier tiles by
-// decoupled look-back on per-(tile, digit) status words:
+// takes the next tile by ticket, ranks its keys stably, publishes its per-digit
+// counts, and finds the counts of all earlier tiles by decoupled look-back on
+// per-(tile, digit) status words:
 //   bits 63..62 = 1: this tile's own count, 2: inclusive prefix up to it.
 // A block only waits on tiles with smaller tickets, which belong to blocks
 // that already run and publish before they wait, so the waits end; each is
@@ -273,6 +273,10 @@ void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n) {
 // A status word carries its own count, so relaxed agent-scope atomics suffice;
 // acquire/release at agent scope would add L2 write-back and invalidation
 // (one L2 per XCD) to every publish and poll.
+// Within a tile each wave owns kOsItems * 64 consecutive keys, loaded at once;
+// a key's rank among its wave's keys of the same digit comes from ballots over
+// the digit bits and one LDS add per digit group, so ranking needs no block
+// barrier, and one barrier turns the per-wave counts into wave offsets.
 constexpr uint32_t kOsItems = 16;
 constexpr uint32_t kOsTile = kBlock * kOsItems;
 constexpr uint64_t kStFlagAgg = 1ull << 62, kStFlagPre = 2ull << 62,
@@ -288,15 +292,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_hist_all(const uint64_t *__rest
     (&h[0][0])[i] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kOsTile;
+  uint64_t key[kOsItems];
+#pragma unroll
   for (uint32_t j = 0; j < kOsItems; ++j) {
     const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    if (i >= n)
+    key[j] = i < n ? keys[i] ^ kSign : 0;
+  }
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    if (base + (uint64_t)j * kBlock + threadIdx.x >= n)
       break;
-    const uint64_t k = keys[i] ^ kSign;
 #pragma unroll
     for (uint32_t p = 0; p < 8; ++p)
       if ((varying >> (8 * p)) & 255)
-        atomicAdd(&h[p][(k >> (8 * p)) & 255], 1u);
+        atomicAdd(&h[p][(key[j] >> (8 * p)) & 255], 1u);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < 8 * 256; i += kBlock) {
@@ -312,31 +320,32 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t *__restrict_
                                                      const unsigned long long *__restrict__ ghist,
                                                      unsigned long long *status,
                                                      unsigned long long *ticket, uint32_t *err) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kBlock / kWave][256];
+  __shared__ uint32_t wbin[kBlock / kWave][256];  // per-wave digit counts, then wave offsets
   __shared__ uint64_t base_off[256];
   __shared__ uint64_t lds[kBlock / kWave + 1];
   __shared__ unsigned long long s_tile;
-  const uint32_t tid = threadIdx.x, wid = tid / kWave;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   if (tid == 0)
     s_tile = atomicAdd(ticket, 1ull);
-  run[tid] = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / kWave; ++w)
+    wbin[w][tid] = 0;
   __syncthreads();
   const uint64_t t = s_tile;
-  const uint64_t base = t * kOsTile;
+  const uint64_t wbase = t * kOsTile + (uint64_t)wid * (kOsItems * kWave);
   uint64_t key[kOsItems];
-  uint32_t rank[kOsItems];
-  // Phase A: stable local ranks (item j, then wave, then lane) and tile counts.
-  for (uint32_t j = 0; j < kOsItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + tid;
-    const bool valid = i < n;
-    const uint64_t k = valid ? in[i] : 0;
-    key[j] = k;
-    const uint32_t d = (uint32_t)(((k ^ kSign) >> shift) & 255);
 #pragma unroll
-    for (uint32_t w = 0; w < kBlock / kWave; ++w)
-      wcnt[w][tid] = 0;
-    __syncthreads();
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = wbase + (uint64_t)j * kWave + lane;
+    key[j] = i < n ? in[i] : 0;
+  }
+  // Phase A: rank among the wave's keys of the same digit (item, then lane).
+  const uint64_t below = lanemask_lt();
+  uint32_t rank[kOsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const bool valid = wbase + (uint64_t)j * kWave + lane < n;
+    const uint32_t d = (uint32_t)(((key[j] ^ kSign) >> shift) & 255);
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (uint32_t b = 0; b < 8; ++b) {
@@ -344,28 +353,25 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t *__restrict_
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint64_t below = peers & lanemask_lt();
-    if (valid && below == 0)
-      wcnt[wid][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t r = run[tid];
+    const uint32_t leader = peers ? (uint32_t)__ffsll((long long)peers) - 1 : lane;
+    uint32_t old = 0;
+    if (valid && lane == leader)
+      old = atomicAdd(&wbin[wid][d], (uint32_t)__popcll(peers));
+    old = __shfl(old, (int)leader);
+    rank[j] = old + (uint32_t)__popcll(peers & below);
+  }
+  __syncthreads();
+  uint32_t r = 0;
 #pragma unroll
-      for (uint32_t w = 0; w < kBlock / kWave; ++w) {
-        const uint32_t c = wcnt[w][tid];
-        wcnt[w][tid] = r;
-        r += c;
-      }
-      run[tid] = r;
-    }
-    __syncthreads();
-    rank[j] = valid ? wcnt[wid][d] + (uint32_t)__popcll(below) : 0;
-    __syncthreads();
+  for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+    const uint32_t c = wbin[w][tid];
+    wbin[w][tid] = r;
+    r += c;
   }
   // Phase B: digit tid - global start, then this tile's place among earlier tiles.
   uint64_t total;
   const uint64_t gstart = block_exclusive_scan<kBlock>((uint64_t)ghist[tid], lds, total);
-  const uint64_t mine = run[tid];
+  const uint64_t mine = r;
   unsigned long long *st = status + t * 256 + tid;
   uint64_t excl = 0;
   if (t == 0) {
@@ -414,11 +420,11 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t *__restrict_
   base_off[tid] = gstart + excl;
   __syncthreads();
   // Phase C: scatter.
+#pragma unroll
   for (uint32_t j = 0; j < kOsItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + tid;
-    if (i < n) {
+    if (wbase + (uint64_t)j * kWave + lane < n) {
       const uint32_t d = (uint32_t)(((key[j] ^ kSign) >> shift) & 255);
-      out[base_off[d] + rank[j]] = key[j];
+      out[base_off[d] + wbin[wid][d] + rank[j]] = key[j];
     }
   }
 }
