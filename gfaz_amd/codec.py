@@ -76,6 +76,7 @@ _SIGNATURES = {
     "gfz_ctx_sync": ([_vp], ctypes.c_int),
     "gfz_ctx_set_memory_limit": ([_vp, _u64], ctypes.c_int),
     "gfz_ctx_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gfz_ctx_set_profile_groups": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "gfz_ctx_reset_stats": ([_vp], ctypes.c_int),
     "gfz_ctx_kernel_stat": ([_vp, ctypes.c_char_p, ctypes.POINTER(_u64),
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
@@ -209,7 +210,11 @@ class Context:
     def set_memory_limit(self, nbytes: int) -> None:
         _check(lib().gfz_ctx_set_memory_limit(self.handle, int(nbytes)))
 
-    def set_profiling(self, on: bool) -> None:
+    def set_profiling(self, on: bool, groups=None) -> None:
+        """Per-group kernel timing. `groups` limits it to those kernel groups
+        (None = all): each timed group drains the stream around its launches."""
+        _check(lib().gfz_ctx_set_profile_groups(self.handle,
+                                                ",".join(groups or ()).encode()))
         _check(lib().gfz_ctx_set_profiling(self.handle, int(bool(on))))
 
     def reset_stats(self) -> None:

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 
 #include "abi_error.hpp"
@@ -91,6 +92,26 @@ int gfz_ctx_set_profiling(gfz_ctx *ctx, int on) {
   if (!ctx)
     return fail(GFZ_E_INVALID, "null ctx");
   ctx->ctx->set_profiling(on != 0);
+  return GFZ_OK;
+}
+
+int gfz_ctx_set_profile_groups(gfz_ctx *ctx, const char *groups) {
+  if (!ctx)
+    return fail(GFZ_E_INVALID, "null ctx");
+  std::set<std::string> g;
+  std::string cur;
+  for (const char *c = groups ? groups : ""; ; ++c) {
+    if (*c == ',' || *c == 0) {
+      if (!cur.empty())
+        g.insert(cur);
+      cur.clear();
+      if (*c == 0)
+        break;
+    } else {
+      cur.push_back(*c);
+    }
+  }
+  ctx->ctx->set_profile_groups(std::move(g));
   return GFZ_OK;
 }
 

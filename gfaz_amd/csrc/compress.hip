@@ -19,7 +19,6 @@
 #include "codec.hpp"
 #include "prims.hpp"
 
-#include <cstring>
 
 #include <algorithm>
 
@@ -693,7 +692,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
 }
 
 __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
-                                                              uint2 *__restrict__ block_counts) {
+                                                              uint2 *__restrict__ block_counts,
+                                                              unsigned long long *__restrict__ sums) {
   __shared__ unsigned long long tab[kBktSlots];
   __shared__ uint32_t lds[kBktThreads / kWave + 1];
   constexpr uint32_t kPer = kBktSlots / kBktThreads;
@@ -743,8 +743,11 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
       ent[off++] = v[i] & ~kRep;
   uint32_t total_d;
   block_exclusive_scan<kBktThreads>(d, lds, total_d);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     block_counts[b] = make_uint2(total_d, total_r);
+    atomicAdd(&sums[0], (unsigned long long)total_d);
+    atomicAdd(&sums[1], (unsigned long long)total_r);
+  }
 }
 
 // Sum of the per-block counts: out[0] = distinct keys, out[1] = repeated keys.
@@ -1313,16 +1316,6 @@ uint32_t stride_grid(Ctx &ctx, uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
 }
 
-template <class T>
-T read_scalar(Ctx &ctx, const T *d) {
-  static_assert(sizeof(T) <= 8, "scalar readback");
-  GFZ_HIP(hipMemcpyAsync(ctx.scalars(), d, sizeof(T), hipMemcpyDeviceToHost, ctx.stream()));
-  ctx.sync();
-  T v;
-  std::memcpy(&v, ctx.scalars(), sizeof(T));
-  return v;
-}
-
 }  // namespace
 
 void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_t *lengths,
@@ -1359,8 +1352,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
 
   // Delta transform and the rule-ID floor.
   DBuf<int32_t> xa(pool, std::max<uint64_t>(N, 1)), xb(pool, std::max<uint64_t>(N, 1));
-  DBuf<uint32_t> d_max(pool, 1);
-  fill_u32(ctx, d_max.get(), 0, 1);
+  DBuf<uint32_t> d_max(pool, 2);  // [0] max |delta|; read back as one 64-bit word
+  fill_u32(ctx, d_max.get(), 0, 2);
   const int32_t *cur = d_nodes;
   int32_t *cur_buf = nullptr;
   if (N) {
@@ -1390,7 +1383,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       cur_buf = dst;
     }
   }
-  const uint32_t max_abs = read_scalar(ctx, d_max.get());
+  const uint32_t max_abs = (uint32_t)ctx.fetch({{d_max.get(), 1, 0}})[0];
   if (max_abs == UINT32_MAX)
     throw Error(kErrOverflow, "delta values too large for rule ID assignment");
   uint32_t next = max_abs + 1;
@@ -1450,8 +1443,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       const uint64_t bcap = ceil_div(L, nbk) + ceil_div(L, nbk) / 4 + 2048;
       stage = DBuf<unsigned long long>(pool, nbk * bcap);
       DBuf<unsigned int> fill(pool, nbk);
-      fill_u32(ctx, fill.get(), 0, nbk);
-      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 4);
+      fill_runs(ctx, {{fill.get(), nbk, 0, 4}, {sums.get(), 4, 0, 8}});
       Buckets bk{stage.get(), bcap, (uint32_t)(nbk - 1), fill.get(),
                  reinterpret_cast<uint32_t *>(sums.get() + 2)};
       ctx.timed("pair_insert", [&] {
@@ -1472,16 +1464,11 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         }
         GFZ_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
-                           ctx.stream(), bk, bcounts.get());
+                           ctx.stream(), bk, bcounts.get(), sums.get());
         GFZ_CHECK_LAUNCH();
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
-      hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), nbk,
-                         sums.get());
-      GFZ_CHECK_LAUNCH();
-      ctx.stage_scalars(sums.get(), 4);
-      ctx.sync();
-      const uint64_t *hc = ctx.scalars();
+      const uint64_t *hc = ctx.fetch({{sums.get(), 4, 0}});
       if (!hc[2]) {
         distinct = hc[0];
         repeated = hc[1];
@@ -1497,8 +1484,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       stage = DBuf<unsigned long long>(pool, n_ins * kStageCap);
     for (; !counted;) {
       ctab = DBuf<unsigned long long>(pool, cap);
-      fill_u64(ctx, reinterpret_cast<uint64_t *>(ctab.get()), kFree, cap);
-      fill_u64(ctx, reinterpret_cast<uint64_t *>(sums.get()), 0, 4);
+      fill_runs(ctx, {{ctab.get(), cap, kFree, 8}, {sums.get(), 4, 0, 8}});
       CountTable t{ctab.get(), cap - 1, reinterpret_cast<uint32_t *>(sums.get() + 2),
                    bcounts.get(), stage.get(), reinterpret_cast<uint32_t *>(sums.get() + 3)};
       ctx.timed("pair_insert", [&] {
@@ -1523,9 +1509,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, ctx.stream(), bcounts.get(), n_ins,
                          sums.get());
       GFZ_CHECK_LAUNCH();
-      ctx.stage_scalars(sums.get(), 4);
-      ctx.sync();
-      const uint64_t *hc = ctx.scalars();
+      const uint64_t *hc = ctx.fetch({{sums.get(), 4, 0}});
       stage_overflow = hc[3] != 0;
       if (!hc[2] && hc[0] <= cap / 2) {
         distinct = hc[0];
@@ -1550,8 +1534,18 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                                     ", rule table " + std::to_string(rcap) + ")");
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
-    fill_u64(ctx, reinterpret_cast<uint64_t *>(rkeys.get()), kFree, rcap);
     const RuleTable rt{rkeys.get(), rvals.get(), rcap / kLine - 1};
+    // Round scalars (read back once, after K4): [0] next length, [1] used keys,
+    // [2] OR and [3] AND of them. One launch clears everything the round needs
+    // cleared: the rule table, the scalars, the used flags and the next round's
+    // head bits (up to L: the next length is at most L).
+    DBuf<unsigned long long> scal(pool, 4);
+    DBuf<uint8_t> used(pool, rcap);
+    fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
+                    {scal.get(), 3, 0, 8},
+                    {scal.get() + 3, 1, ~0ull, 8},
+                    {used.get(), rcap, 0, 1},
+                    {heads_next.get(), ceil_div(L, 32), 0, 4}});
     ctx.timed("extract_repeated", [&] {
       if (stage_overflow)  // some block staged too many: scan the whole table
         hipLaunchKernelGGL(k_extract_repeated,
@@ -1574,14 +1568,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     });
     GFZ_CHECK_LAUNCH();
 
-    // K2 + K2b: candidates, tile carries, next length. Round scalars (read back
-    // once, after K4): [0] next length, [1] used keys, [2] OR and [3] AND of them.
+    // K2 + K2b: candidates, tile carries, next length.
     DBuf<uint64_t> cand(pool, n_words);
     DBuf<uint32_t> tile_fn(pool, n_tiles);
     DBuf<uint64_t> tile_info(pool, n_tiles);
-    DBuf<unsigned long long> scal(pool, 4);
-    fill_u64(ctx, reinterpret_cast<uint64_t *>(scal.get()), 0, 3);
-    fill_u64(ctx, reinterpret_cast<uint64_t *>(scal.get()) + 3, ~0ull, 1);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, span, rt, hot.get(), cand.get(), n_words,
@@ -1596,10 +1586,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     });
     GFZ_CHECK_LAUNCH();
 
-    // K3: emit (the next length is at most L; its head bits are cleared up to L).
-    DBuf<uint8_t> used(pool, rcap);
-    fill_u8(ctx, used.get(), 0, rcap);
-    fill_u32(ctx, heads_next.get(), 0, ceil_div(L, 32));
+    // K3: emit.
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt,
@@ -1618,10 +1605,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     cand.reset();
     tile_fn.reset();
     tile_info.reset();
-    ctx.stage_scalars(scal.get(), 4);
-    ctx.stage_scalars(d_err.get(), 1, 4);
-    ctx.sync();
-    const uint64_t *hs = ctx.scalars();
+    const uint64_t *hs = ctx.fetch({{scal.get(), 4, 0}, {d_err.get(), 1, 4}});
     const uint64_t L_next = hs[0], R = hs[1], varying = hs[2] & ~hs[3];
     if ((uint32_t)hs[4])
       throw Error(kErrCorrupt, "pair table lost a key (internal error)");
@@ -1634,7 +1618,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
     sort_i64_varying(ctx, ukeys.get(), utmp.get(), R, varying, d_err.get());
     if (out.n_rules + R > rules_cap) {
-      const uint64_t ncap = std::max<uint64_t>(2 * rules_cap, out.n_rules + R);
+      // Room for four times the first round's repeated keys up front: later
+      // rounds add fewer rules, so the columns rarely have to grow.
+      const uint64_t ncap = std::max<uint64_t>(std::max<uint64_t>(2 * rules_cap, out.n_rules + R),
+                                               4 * (uint64_t)repeated);
       DBuf<int32_t> nf(pool, ncap), ns(pool, ncap);
       if (out.n_rules) {
         GFZ_HIP(hipMemcpyAsync(nf.get(), rf.get(), out.n_rules * 4, hipMemcpyDeviceToDevice,

@@ -1,6 +1,9 @@
 #include "ctx.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <thread>
 
 namespace gfz {
 
@@ -90,6 +93,8 @@ Ctx::~Ctx() {
     (void)hipHostFree(pinned_);
   if (scal_)
     (void)hipHostFree(scal_);
+  if (mail_)
+    (void)hipHostFree(mail_);
   pool_.trim();
   if (stream_)
     (void)hipStreamDestroy(stream_);
@@ -123,6 +128,73 @@ void Ctx::collect() {
     free_events_.push_back(p.b);
   }
   pending_.clear();
+}
+
+namespace {
+constexpr uint32_t kMailWords = 512;  // the sequence number follows them
+constexpr uint32_t kMaxFetch = 4;
+struct FetchArgs {
+  const uint64_t *src[kMaxFetch];
+  uint32_t n[kMaxFetch];
+  uint32_t at[kMaxFetch];
+  uint32_t count;
+};
+
+// One wave: its stores, then a system-scope fence, then the sequence number.
+__global__ __launch_bounds__(kWave) void k_fetch(FetchArgs a, uint64_t *mail, uint64_t seq) {
+  for (uint32_t r = 0; r < a.count; ++r)
+    for (uint32_t i = threadIdx.x; i < a.n[r]; i += kWave)
+      mail[a.at[r] + i] = a.src[r][i];
+  __threadfence_system();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(mail + kMailWords, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+}  // namespace
+
+const uint64_t *Ctx::fetch(std::initializer_list<FetchRun> runs) {
+  if (!mail_) {
+    void *p = nullptr;
+    GFZ_HIP(hipHostMalloc(&p, (kMailWords + 1) * 8, hipHostMallocMapped | hipHostMallocCoherent));
+    mail_ = static_cast<uint64_t *>(p);
+    std::memset(mail_, 0, (kMailWords + 1) * 8);
+    void *d = nullptr;
+    GFZ_HIP(hipHostGetDevicePointer(&d, p, 0));
+    mail_dev_ = static_cast<uint64_t *>(d);
+  }
+  FetchArgs a{};
+  for (const FetchRun &r : runs) {
+    if (a.count == kMaxFetch || r.at + r.n > kMailWords)
+      throw Error(kErrInvalid, "fetch: too many runs or words for the mailbox");
+    a.src[a.count] = static_cast<const uint64_t *>(r.dev);
+    a.n[a.count] = r.n;
+    a.at[a.count] = r.at;
+    ++a.count;
+  }
+  const uint64_t seq = ++mail_seq_;
+  hipLaunchKernelGGL(k_fetch, dim3(1), dim3(kWave), 0, stream_, a, mail_dev_, seq);
+  GFZ_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t polls = 1;; ++polls) {
+    if (__atomic_load_n(mail_ + kMailWords, __ATOMIC_ACQUIRE) == seq)
+      break;
+    if (polls % 256 == 0) {
+      const hipError_t e = hipStreamQuery(stream_);
+      if (e == hipSuccess) {
+        if (__atomic_load_n(mail_ + kMailWords, __ATOMIC_ACQUIRE) == seq)
+          break;
+        throw Error(kErrHip, "readback mailbox not written by a completed stream");
+      }
+      if (e != hipErrorNotReady) {
+        (void)hipGetLastError();
+        throw Error(kErrHip, std::string("stream failed during a readback: ") +
+                                 hipGetErrorString(e));
+      }
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(500))
+        std::this_thread::yield();
+    }
+    __builtin_ia32_pause();
+  }
+  return mail_;
 }
 
 uint64_t *Ctx::scalars() {

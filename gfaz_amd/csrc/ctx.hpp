@@ -5,8 +5,10 @@
 
 #include "common.hpp"
 
+#include <initializer_list>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -95,11 +97,14 @@ class Ctx {
   // Per-kernel timing: when enabled, launches wrapped in timed() record a
   // start/stop event pair on the context stream; collect() accumulates the
   // elapsed times after the stream has been synchronised.
+  // An event pair drains the stream around its launches (several us per
+  // pair): set_profile_groups limits timing to the named groups (empty = all).
   void set_profiling(bool on) { profiling_ = on; }
   bool profiling() const { return profiling_; }
+  void set_profile_groups(std::set<std::string> groups) { only_ = std::move(groups); }
   template <class F>
   void timed(const char *name, F &&launch) {
-    if (!profiling_) {
+    if (!profiling_ || (!only_.empty() && !only_.count(name))) {
       launch();
       return;
     }
@@ -117,6 +122,18 @@ class Ctx {
     if (profiling_)
       stats_[name].bytes += bytes;
   }
+
+  // Device-to-host readback of a few 64-bit words without a stream
+  // synchronisation: one single-wave kernel copies up to 4 runs into a
+  // coherent pinned mailbox, then stores a sequence number after them, which
+  // the host polls (querying the stream for errors while it waits). Returns
+  // the mailbox: run r lands at words [at, at + n), at + n <= 512.
+  struct FetchRun {
+    const void *dev;
+    uint32_t n;
+    uint32_t at;
+  };
+  const uint64_t *fetch(std::initializer_list<FetchRun> runs);
 
   // Pinned host staging buffer of at least `bytes`, reused across calls.
   void *pinned(size_t bytes);
@@ -165,6 +182,10 @@ class Ctx {
   hipStream_t stream_ = nullptr;
   Pool pool_;
   bool profiling_ = false;
+  uint64_t *mail_ = nullptr;      // coherent pinned mailbox (host address)
+  uint64_t *mail_dev_ = nullptr;  // the same memory as the device sees it
+  uint64_t mail_seq_ = 0;
+  std::set<std::string> only_;
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> free_events_;
   std::map<std::string, KernelStat> stats_;

@@ -174,6 +174,27 @@ __global__ void k_bit_positions(const uint32_t *__restrict__ bits, uint64_t n_bi
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill_runs(FillRuns r) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (uint32_t q = 0; q < r.count; ++q) {
+    const FillRun f = r.run[q];
+    if (f.width == 8) {
+      uint64_t *p = static_cast<uint64_t *>(f.ptr);
+      for (uint64_t i = g; i < f.n; i += stride)
+        p[i] = f.value;
+    } else if (f.width == 4) {
+      uint32_t *p = static_cast<uint32_t *>(f.ptr);
+      for (uint64_t i = g; i < f.n; i += stride)
+        p[i] = (uint32_t)f.value;
+    } else {
+      uint8_t *p = static_cast<uint8_t *>(f.ptr);
+      for (uint64_t i = g; i < f.n; i += stride)
+        p[i] = (uint8_t)f.value;
+    }
+  }
+}
+
 uint32_t grid_for(Ctx &ctx, uint64_t n, uint32_t per_block) {
   const uint64_t want = ceil_div(n, per_block);
   const uint64_t cap = (uint64_t)ctx.num_cus() * 16;
@@ -208,6 +229,26 @@ void fill_u8(Ctx &ctx, uint8_t *p, uint8_t v, uint64_t n) {
     return;
   hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(ctx, n / 16 + 1, kBlock * 4)), dim3(kBlock), 0,
                      ctx.stream(), p, v, n);
+  GFZ_CHECK_LAUNCH();
+}
+
+void fill_runs(Ctx &ctx, std::initializer_list<FillRun> runs) {
+  FillRuns r{};
+  uint64_t most = 0;
+  for (const FillRun &f : runs) {
+    if (!f.n)
+      continue;
+    if (r.count == kMaxFillRuns)
+      throw Error(kErrInvalid, "fill_runs: more than " + std::to_string(kMaxFillRuns) + " runs");
+    if (f.width != 1 && f.width != 4 && f.width != 8)
+      throw Error(kErrInvalid, "fill_runs: element width must be 1, 4 or 8 bytes");
+    r.run[r.count++] = f;
+    most = std::max(most, f.n);
+  }
+  if (!r.count)
+    return;
+  hipLaunchKernelGGL(k_fill_runs, dim3(grid_for(ctx, most, kBlock * 4)), dim3(kBlock), 0,
+                     ctx.stream(), r);
   GFZ_CHECK_LAUNCH();
 }
 

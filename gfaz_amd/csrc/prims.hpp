@@ -4,6 +4,8 @@
 
 #include "ctx.hpp"
 
+#include <initializer_list>
+
 namespace gfz {
 
 // Block-wide exclusive scan for NT threads; `lds` needs NT/kWave + 1 slots.
@@ -64,6 +66,20 @@ void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint6
 void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);
 void fill_u8(Ctx &ctx, uint8_t *p, uint8_t v, uint64_t n);
+// Several fills in one launch: n elements of `width` bytes (1, 4 or 8) at ptr
+// set to the low bytes of value; at most kMaxFillRuns runs.
+struct FillRun {
+  void *ptr;
+  uint64_t n;
+  uint64_t value;
+  uint32_t width;
+};
+constexpr uint32_t kMaxFillRuns = 8;
+struct FillRuns {
+  FillRun run[kMaxFillRuns];
+  uint32_t count;
+};
+void fill_runs(Ctx &ctx, std::initializer_list<FillRun> runs);
 
 // Set bit off[t] of `heads` (pre-zeroed, ceil(n_pos/32) words) for every
 // t < n_off with off[t] < n_pos.

@@ -56,6 +56,11 @@ int gfz_ctx_sync(gfz_ctx *ctx);
  * with GFZ_E_NOMEM instead of reaching hipMalloc. */
 int gfz_ctx_set_memory_limit(gfz_ctx *ctx, uint64_t bytes);
 int gfz_ctx_set_profiling(gfz_ctx *ctx, int on);
+/* Limits profiling to the comma-separated kernel groups in `groups` (NULL or
+ * "" = every group). Each timed group records an event pair that drains the
+ * stream around its launches, so a throughput run times only the groups it
+ * reports. No counterpart in the reference (measurement support). */
+int gfz_ctx_set_profile_groups(gfz_ctx *ctx, const char *groups);
 int gfz_ctx_reset_stats(gfz_ctx *ctx);
 /* Accumulated launches / milliseconds / algorithmic bytes of a timed kernel
  * group ("expand", "pair_insert", "emit", ...); zeros if never timed. */
