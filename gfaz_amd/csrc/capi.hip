@@ -300,8 +300,10 @@ int gfz_rulebook_load(gfz_ctx *ctx, const int32_t *rules_first, const int32_t *r
     }
     auto rb = std::make_unique<gfz_rulebook>();
     rb->owner = ctx;
+    // The build's first readback follows the copies of host columns, so on
+    // return the caller's buffers are no longer read; the bodies finish on the
+    // ctx stream, which every use of the rulebook runs on.
     gfz::rulebook_build(c, df, ds, n_rules, min_rule_id, rb->rb);
-    c.sync();
     *out = rb.release();
   });
 }
@@ -372,6 +374,8 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
                               uint32_t *out_lengths, int *length_mismatch) {
   if (!ctx || !rb || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
     return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  if (rb->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "rulebook belongs to another ctx");
   return guarded(kDecompressPrefix, [&] {
     gfz::Ctx &c = *ctx->ctx;
     gfz::DeviceGuard g(c.device());

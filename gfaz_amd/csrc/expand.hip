@@ -31,6 +31,7 @@
 #include "zexpand.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <cstdlib>
 
 namespace gfz {
@@ -577,9 +578,8 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
                            R, min_id, (uint32_t)(pass0 + k), flags.get() + k, err);
     });
     GFZ_CHECK_LAUNCH();
-    ctx.stage_scalars(flags.get(), (kPassBatch + 4) / 2);
-    ctx.sync();
-    const uint32_t *h = reinterpret_cast<const uint32_t *>(ctx.scalars());
+    const uint32_t *h =
+        reinterpret_cast<const uint32_t *>(ctx.fetch({{flags.get(), (kPassBatch + 4) / 2, 0}}));
     if (h[kPassBatch])
       throw Error(kErrCorrupt, "rule expansion size exceeds 2^32 symbols");
     for (uint32_t k = 0; k < kPassBatch; ++k) {
@@ -615,22 +615,22 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
                        d_bound.get(), max_depth + 2, d_lvl.get());
     GFZ_CHECK_LAUNCH();
     const size_t nb = max_depth + 2;
-    ctx.stage_scalars(flags.get(), (kPassBatch + 4) / 2);
-    if (2 * nb + (kPassBatch + 4) / 2 <= 512) {
-      ctx.stage_scalars(d_bound.get(), nb, (kPassBatch + 4) / 2);
-      ctx.stage_scalars(d_lvl.get(), nb, (kPassBatch + 4) / 2 + nb);
-      ctx.sync();
-      const uint64_t *hs = ctx.scalars() + (kPassBatch + 4) / 2;
-      std::copy(hs, hs + nb, bound.begin());
-      std::copy(hs + nb, hs + 2 * nb, lvl_off.begin());
+    constexpr uint32_t kFlagWords = (kPassBatch + 4) / 2;
+    uint32_t hf[kPassBatch + 4];
+    if (2 * nb + kFlagWords <= 512) {
+      const uint64_t *m = ctx.fetch({{flags.get(), kFlagWords, 0},
+                                     {d_bound.get(), (uint32_t)nb, kFlagWords},
+                                     {d_lvl.get(), (uint32_t)nb, (uint32_t)(kFlagWords + nb)}});
+      std::memcpy(hf, m, sizeof(hf));
+      std::copy(m + kFlagWords, m + kFlagWords + nb, bound.begin());
+      std::copy(m + kFlagWords + nb, m + kFlagWords + 2 * nb, lvl_off.begin());
     } else {  // deep rulebook: pageable copies
       GFZ_HIP(hipMemcpyAsync(bound.data(), d_bound.get(), nb * 8, hipMemcpyDeviceToHost,
                              ctx.stream()));
       GFZ_HIP(hipMemcpyAsync(lvl_off.data(), d_lvl.get(), nb * 8, hipMemcpyDeviceToHost,
                              ctx.stream()));
-      ctx.sync();
+      std::memcpy(hf, ctx.fetch({{flags.get(), kFlagWords, 0}}), sizeof(hf));
     }
-    const uint32_t *hf = reinterpret_cast<const uint32_t *>(ctx.scalars());
     if (hf[kPassBatch])
       throw Error(kErrCorrupt, "rulebook is cyclic: some rule never bottoms out in node IDs");
     if (hf[kPassBatch + 1]) {
@@ -679,7 +679,9 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     expand_through_z(ctx, "build_bodies", false, syms.get(), off.get(), n_sym, lo, hi - lo,
                      rb.z.get(), rb.z_raw, rb.z.get());
   }
-  ctx.sync();
+  // No synchronisation: the bodies are built on the context stream, which every
+  // use of the rulebook runs on, and the rule columns were last read before the
+  // readbacks above returned.
   rb.has_bodies = true;
 }
 
@@ -708,22 +710,29 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
       GFZ_CHECK_LAUNCH();
     }
     exclusive_scan(ctx, sz.get(), sym_off.get(), n_enc);
-    ctx.sync();
   }
   // Decoded offset of each traversal = symbol offset at its first encoded symbol.
   plan.out_offsets.assign(n_trav + 1, 0);
   {
-    std::vector<uint64_t> h_idx(enc_start);
-    h_idx.push_back(n_enc);
+    // Indices staged in pinned memory (an asynchronous copy); the gathered
+    // offsets come back through the mailbox when they fit.
+    uint64_t *h_idx = static_cast<uint64_t *>(ctx.pinned((n_trav + 1) * 8));
+    std::copy(enc_start.begin(), enc_start.end(), h_idx);
+    h_idx[n_trav] = n_enc;
     DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
-    GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx.data(), (n_trav + 1) * 8, hipMemcpyHostToDevice,
+    GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx, (n_trav + 1) * 8, hipMemcpyHostToDevice,
                            ctx.stream()));
     hipLaunchKernelGGL(k_gather_u64, dim3(stride_grid(ctx, n_trav + 1)), dim3(kBlock), 0,
                        ctx.stream(), sym_off.get(), idx.get(), n_trav + 1, val.get());
     GFZ_CHECK_LAUNCH();
-    GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
-                           hipMemcpyDeviceToHost, ctx.stream()));
-    ctx.sync();
+    if (n_trav + 1 <= 512) {
+      const uint64_t *m = ctx.fetch({{val.get(), (uint32_t)(n_trav + 1), 0}});
+      std::copy(m, m + n_trav + 1, plan.out_offsets.begin());
+    } else {
+      GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
+                             hipMemcpyDeviceToHost, ctx.stream()));
+      ctx.sync();
+    }
   }
   plan.n_out = plan.out_offsets[n_trav];
   for (uint64_t t = 0; t < n_trav; ++t)
