@@ -163,19 +163,17 @@ __device__ inline int64_t rule_find(const RuleTable &rt, uint64_t k) {
 }
 
 // Inserts a key that is not in the table yet; false when kMaxProbe lines were
-// full. Which slots are taken is decided by the CAS results alone: a relaxed
-// read of the line may show a slot free that another block has just filled
-// (the CAS then fails and the next slot is tried), never the reverse, so
-// starting at the first slot seen free keeps every key in the first line that
-// had room.
+// full. Which slots are taken is decided by the CAS results alone: a plain read
+// of the line may show a slot free that another block has just filled (a line
+// cached before that block's CAS; the CAS then fails and the next slot is
+// tried), never the reverse, so starting at the first slot seen free keeps
+// every key in the first line that had room.
 __device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
   uint64_t l = key_hash(k) & rt.lmask;
   for (uint32_t p = 0; p < kMaxProbe; ++p) {
     unsigned long long *line = rt.keys + l * kLine;
     uint64_t w[kLine];
-#pragma unroll
-    for (uint32_t q = 0; q < kLine; ++q)
-      w[q] = __hip_atomic_load(line + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    load_line(rt.keys, l, w);
     uint32_t j = 0;
     while (j < kLine && w[j] != kFree)
       ++j;
@@ -782,17 +780,14 @@ __global__ __launch_bounds__(1024) void k_sum_counts(const uint2 *__restrict__ c
 }
 
 // ---- K1b: the repeated keys, moved into the rule table -------------------------
-// From the per-block staging lists: one wave per block list.
+// From the staging lists (one per counting block or bucket): one block per
+// list, so a bucket's few hundred keys take one or two inserts per thread.
 __global__ __launch_bounds__(kBlock) void k_extract_staged(
     const unsigned long long *__restrict__ stage, uint64_t stride,
-    const uint2 *__restrict__ block_counts, uint64_t n_blocks, RuleTable rt,
-    uint32_t *__restrict__ err) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint64_t b = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (b >= n_blocks)
-    return;
-  const uint32_t n = block_counts[b].y;
-  for (uint64_t j = lane; j < n && j < stride; j += kWave)
+    const uint2 *__restrict__ block_counts, RuleTable rt, uint32_t *__restrict__ err) {
+  const uint64_t b = blockIdx.x;
+  const uint64_t n = block_counts[b].y < stride ? block_counts[b].y : stride;
+  for (uint64_t j = threadIdx.x; j < n; j += kBlock)
     if (!rule_insert(rt, stage[b * stride + j]))
       atomicExch(err, 3u);
 }
@@ -844,14 +839,26 @@ struct alignas(16) HotEntry {
   uint32_t pad;
 };
 
+// The kHotTop most frequent sampled keys as canonical pairs, which K2 and K3
+// compare raw pairs against before any key arithmetic.
+constexpr uint32_t kHotTop = 4;
+struct alignas(16) HotPair {
+  int32_t a, b;  // canonical pair
+  uint32_t slot;
+  uint32_t set;  // 0 = empty entry
+};
+
 __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__restrict__ x,
                                                           const uint32_t *__restrict__ heads,
                                                           uint64_t L, uint64_t span, RuleTable rt,
-                                                          HotEntry *__restrict__ hot) {
+                                                          HotEntry *__restrict__ hot,
+                                                          HotPair *__restrict__ top) {
   __shared__ unsigned long long skey[kHotSampleSlots];
   __shared__ uint32_t scnt[kHotSampleSlots];
   __shared__ unsigned long long hkey[kHotSlots];
   __shared__ uint32_t hslot[kHotSlots];
+  __shared__ unsigned long long best;
+  __shared__ uint32_t chosen[kHotTop];
   for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
     skey[s] = kFree;
     scnt[s] = 0;
@@ -878,6 +885,40 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     }
   }
   __syncthreads();
+  // The kHotTop largest counts (ties to the larger sample slot), one block-wide
+  // maximum at a time.
+  for (uint32_t t = 0; t < kHotTop; ++t) {
+    if (threadIdx.x == 0)
+      best = 0;
+    __syncthreads();
+    unsigned long long mine = 0;
+    for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
+      bool taken = false;
+      for (uint32_t u = 0; u < t; ++u)
+        taken |= chosen[u] == s;
+      const unsigned long long c = ((unsigned long long)scnt[s] << 32) | s;
+      if (scnt[s] >= kHotMin && !taken && c > mine)
+        mine = c;
+    }
+    if (mine)
+      atomicMax(&best, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      chosen[t] = best ? (uint32_t)best : ~0u;
+    __syncthreads();
+  }
+  if (threadIdx.x < kHotTop) {
+    HotPair hp{0, 0, 0, 0};
+    const uint32_t s = chosen[threadIdx.x];
+    if (s != ~0u) {
+      const int64_t slot = rule_find(rt, skey[s]);
+      if (slot >= 0) {
+        const int64_t c = expand_key(skey[s], span);
+        hp = HotPair{key_first(c), key_second(c), (uint32_t)slot, 1};
+      }
+    }
+    top[threadIdx.x] = hp;
+  }
   for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
     if (scnt[s] < kHotMin)
       continue;
@@ -932,49 +973,93 @@ __device__ inline int64_t hot_find(const HotLds &h, uint64_t k) {
 
 // ---- K2: candidate bits and each tile's transfer function --------------------
 // A pair is a candidate iff its key is in the rule table (the repeated keys).
-// Per pass of kPassItems items a lane reads its pairs and resolves hot keys from
-// LDS; the other keys go to a per-wave LDS list, which the wave then looks up
-// kColdBatch keys per lane at a time, writing each result over its key. The
-// line reads of the lookups thus run once per wave and pass, over only the keys
-// that need them, and their registers are never live beside the stream's.
+// Per pass of kPassItems items a lane reads its pairs and compares them, raw,
+// with the kHotTop hottest canonical pairs (scalar registers); every other
+// pair goes to a per-wave LDS list. The wave then resolves its list densely:
+// key arithmetic, the LDS table of the other hot keys, and rule-table lookups
+// kColdBatch per lane at a time, each result written over its entry. The key
+// arithmetic and line reads thus run over only the pairs that need them (a
+// wave's lanes would otherwise all wait for the few that do), and the line
+// reads' registers are never live beside the stream's.
 constexpr uint32_t kPassItems = 8;
-constexpr uint32_t kColdCap = kPassItems * kWave;   // every item of a pass cold
+constexpr uint32_t kColdCap = kPassItems * kWave;   // every item of a pass listed
 constexpr int kColdBatch = 2;
 
-// Looks up the wave's n listed keys; each entry becomes the key's slot, or
-// kFree when absent.
-__device__ inline void lookup_cold(const RuleTable &rt, unsigned long long *list, uint32_t n) {
+// Raw pair (x, y) against the hot canonical pairs: the matching entry's index
+// and slot, or -1; rev = the pair appears as (-b, -a) of its canonical (a, b).
+__device__ inline int hot_pair(const HotPair (&hp)[kHotTop], int32_t x, int32_t y, bool &rev,
+                               uint32_t &slot) {
+  int hit = -1;
+  rev = false;
+  slot = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t) {
+    const bool fwd = x == hp[t].a && y == hp[t].b;
+    const bool bwd = x == -hp[t].b && y == -hp[t].a;
+    if (hp[t].set && (fwd || bwd)) {
+      hit = (int)t;
+      slot = hp[t].slot;
+      rev = !fwd;
+    }
+  }
+  return hit;
+}
+
+// Resolves the wave's n listed raw pairs: each entry becomes its key's rule
+// slot, or kFree when the key is not in the table.
+__device__ inline void resolve_list(const RuleTable &rt, const HotLds &sh, uint64_t span,
+                                    unsigned long long *list, uint32_t n) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   for (uint32_t e0 = 0; e0 < n; e0 += kWave * kColdBatch) {
     uint64_t kk[kColdBatch];
-    bool v[kColdBatch];
+    bool cold[kColdBatch];
+    int64_t hs[kColdBatch];
 #pragma unroll
     for (int q = 0; q < kColdBatch; ++q) {
       const uint32_t e = e0 + (uint32_t)q * kWave + lane;
-      v[q] = e < n;
-      kk[q] = v[q] ? list[e] : 0;
+      kk[q] = 0;
+      hs[q] = -1;
+      if (e < n) {
+        const int64_t raw = (int64_t)list[e];
+        kk[q] = pair_key(key_first(raw), key_second(raw), span);
+        hs[q] = hot_find(sh, kk[q]);
+      }
+      cold[q] = e < n && hs[q] < 0;
     }
     int64_t sl[kColdBatch];
-    rule_find_batch<kColdBatch>(rt, kk, v, sl);
+    rule_find_batch<kColdBatch>(rt, kk, cold, sl);
 #pragma unroll
-    for (int q = 0; q < kColdBatch; ++q)
-      if (v[q])
-        list[e0 + (uint32_t)q * kWave + lane] = sl[q] >= 0 ? (unsigned long long)sl[q] : kFree;
+    for (int q = 0; q < kColdBatch; ++q) {
+      const uint32_t e = e0 + (uint32_t)q * kWave + lane;
+      if (e < n) {
+        const int64_t r = hs[q] >= 0 ? hs[q] : sl[q];
+        list[e] = r >= 0 ? (unsigned long long)r : kFree;
+      }
+    }
   }
+}
+
+__device__ inline void hot_top_load(HotPair (&hp)[kHotTop], const HotPair *__restrict__ top) {
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t)
+    hp[t] = top[t];
 }
 
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
                                                  uint64_t span, RuleTable rt,
                                                  const HotEntry *__restrict__ hot,
+                                                 const HotPair *__restrict__ top,
                                                  uint64_t *__restrict__ cand,
                                                  uint64_t n_words, uint32_t *__restrict__ tile_fn) {
   __shared__ uint64_t wbits[kRtWords];
   __shared__ HotLds sh;
-  __shared__ unsigned long long cold[kBlock / kWave][kColdCap];
+  __shared__ unsigned long long list[kBlock / kWave][kColdCap];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const uint64_t below = lanemask_lt();
+  HotPair hp[kHotTop];
+  hot_top_load(hp, top);
   hot_load(sh, hot);
   __syncthreads();
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
@@ -991,24 +1076,25 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
     uint64_t m[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint64_t k = pair_key(xa[b], xb[b], span);
-      const bool h = valid[b] && hot_find(sh, k) >= 0;
+      bool rev;
+      uint32_t hs;
+      const bool h = valid[b] && hot_pair(hp, xa[b], xb[b], rev, hs) >= 0;
       const bool c = valid[b] && !h;
       hit |= h ? 1u << b : 0u;
       m[b] = __ballot(c);
       if (c)
-        cold[wid][n + (uint32_t)__popcll(m[b] & below)] = k;
+        list[wid][n + (uint32_t)__popcll(m[b] & below)] = (unsigned long long)pack2(xa[b], xb[b]);
       n += (uint32_t)__popcll(m[b]);
     }
     __syncthreads();
-    lookup_cold(rt, cold[wid], n);
+    resolve_list(rt, sh, span, list[wid], n);
     __syncthreads();
     uint32_t pre = 0;
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       bool f = (hit >> b) & 1;
       if ((m[b] >> lane) & 1)
-        f = cold[wid][pre + (uint32_t)__popcll(m[b] & below)] != kFree;
+        f = list[wid][pre + (uint32_t)__popcll(m[b] & below)] != kFree;
       pre += (uint32_t)__popcll(m[b]);
       const uint64_t word = __ballot(f);
       if (lane == 0) {
@@ -1102,20 +1188,22 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    uint64_t span, RuleTable rt, const HotEntry *__restrict__ hot, uint8_t *__restrict__ used,
-    uint32_t next, int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
-    uint32_t *__restrict__ err) {
+    uint64_t span, RuleTable rt, const HotEntry *__restrict__ hot,
+    const HotPair *__restrict__ top, uint8_t *__restrict__ used, uint32_t next,
+    int32_t *__restrict__ y, uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
   __shared__ HotLds sh;
-  __shared__ unsigned long long cold[kBlock / kWave][kColdCap];
+  __shared__ unsigned long long list[kBlock / kWave][kColdCap];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t below = lanemask_lt();
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const uint64_t info = tile_info[blockIdx.x];
   const uint32_t p_tile = (uint32_t)(info & 1);
   const uint64_t off_tile = info >> 1;
+  HotPair hp[kHotTop];
+  hot_top_load(hp, top);
   hot_load(sh, hot);
   if (wid == 0) {
     const uint64_t gw = (base >> 6) + lane;
@@ -1143,9 +1231,20 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     soff[lane] = sx - sc;
   }
   __syncthreads();
+  uint32_t hot_used = 0;  // hot entries this lane picked (their used flags, once per wave)
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
-    // Survivors that are not picked, and picks of hot keys, are written at once;
-    // picks of other keys wait for the wave's lookup pass.
+    // Survivors that are not picked, and picks of the hottest pairs, are written
+    // at once; other picks wait for the wave's list.
+    // Every load of the pass first (the stream is read whole anyway).
+    int32_t xa[kPassItems], xb[kPassItems];
+    uint32_t hw[kPassItems];
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      const uint64_t i = base + (uint64_t)((j0 + b) * (kBlock / kWave) + wid) * 64 + lane;
+      xa[b] = i < L ? x[i] : 0;
+      xb[b] = i + 1 < L ? x[i + 1] : 0;
+      hw[b] = i < L ? heads[i >> 5] : 0;
+    }
     uint64_t m[kPassItems];
     uint32_t n = 0, neg = 0;
 #pragma unroll
@@ -1154,50 +1253,51 @@ __global__ __launch_bounds__(kBlock) void k_emit(
       const uint64_t i = base + (uint64_t)w * 64 + lane;
       const uint64_t svm = ssurv[w];
       bool c = false;
-      uint64_t ck = 0;
+      int64_t raw = 0;
       if (i < L && ((svm >> lane) & 1)) {
         const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(svm & below);
-        int32_t o = x[i];
+        int32_t o = xa[b];
         if ((spicks[w] >> lane) & 1) {
-          const int64_t k = pack2(o, x[i + 1]);
-          const int64_t cc = canon2(k);
-          ck = compact_key(cc, span);
-          const int64_t hs = hot_find(sh, ck);
-          if (hs >= 0) {
-            used[hs] = 1;
-            o = (int32_t)(next + (uint32_t)hs);
-            if (k != cc)
+          const int32_t o2 = xb[b];
+          bool rev;
+          uint32_t hs;
+          const int t = hot_pair(hp, o, o2, rev, hs);
+          if (t >= 0) {
+            hot_used |= 1u << t;
+            o = (int32_t)(next + hs);
+            if (rev)
               o = -o;
           } else {
             c = true;
-            neg |= k != cc ? 1u << b : 0u;
+            raw = pack2(o, o2);
+            neg |= raw != canon2(raw) ? 1u << b : 0u;
           }
         }
         if (!c)
           y[pos] = o;
-        if (head_at(heads, i))
+        if ((hw[b] >> (i & 31)) & 1)
           atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
       }
       m[b] = __ballot(c);
       if (c)
-        cold[wid][n + (uint32_t)__popcll(m[b] & below)] = ck;
+        list[wid][n + (uint32_t)__popcll(m[b] & below)] = (unsigned long long)raw;
       n += (uint32_t)__popcll(m[b]);
     }
     __syncthreads();
-    lookup_cold(rt, cold[wid], n);
+    resolve_list(rt, sh, span, list[wid], n);
     __syncthreads();
     uint32_t pre = 0;
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       if ((m[b] >> lane) & 1) {
-        const unsigned long long sl = cold[wid][pre + (uint32_t)__popcll(m[b] & below)];
+        const unsigned long long sl = list[wid][pre + (uint32_t)__popcll(m[b] & below)];
         if (sl == kFree) {
           atomicExch(err, 1u);
         } else {
           const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
           const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(ssurv[w] & below);
           used[sl] = 1;
-          int32_t o = (int32_t)(next + (uint32_t)sl);
+          const int32_t o = (int32_t)(next + (uint32_t)sl);
           y[pos] = (neg >> b) & 1 ? -o : o;
         }
       }
@@ -1205,6 +1305,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     }
     __syncthreads();  // the list is refilled by the next pass
   }
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t)
+    if (__ballot((hot_used >> t) & 1) && lane == 0)
+      used[hp[t].slot] = 1;
 }
 
 // ---- K4: keys of the used rules (order irrelevant: they are sorted next) -----
@@ -1552,19 +1656,19 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                            dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
                            dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt, d_err.get());
       else
-        hipLaunchKernelGGL(k_extract_staged,
-                           dim3((uint32_t)ceil_div(stage_blocks, kBlock / kWave)), dim3(kBlock),
-                           0, ctx.stream(), stage.get(), stage_stride, bcounts.get(),
-                           stage_blocks, rt, d_err.get());
+        hipLaunchKernelGGL(k_extract_staged, dim3((uint32_t)stage_blocks), dim3(kBlock), 0,
+                           ctx.stream(), stage.get(), stage_stride, bcounts.get(), rt,
+                           d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     stage.reset();
     ctab.reset();
 
     DBuf<HotEntry> hot(pool, kHotSlots);
+    DBuf<HotPair> top(pool, kHotTop);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_hot_sample, dim3(1), dim3(kHotThreads), 0, ctx.stream(), cur,
-                         heads.get(), L, span, rt, hot.get());
+                         heads.get(), L, span, rt, hot.get(), top.get());
     });
     GFZ_CHECK_LAUNCH();
 
@@ -1574,7 +1678,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint64_t> tile_info(pool, n_tiles);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, span, rt, hot.get(), cand.get(), n_words,
+                         heads.get(), L, span, rt, hot.get(), top.get(), cand.get(), n_words,
                          tile_fn.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -1590,7 +1694,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt,
-                         hot.get(), used.get(), next, dst, heads_next.get(), d_err.get());
+                         hot.get(), top.get(), used.get(), next, dst, heads_next.get(),
+                         d_err.get());
     });
     GFZ_CHECK_LAUNCH();
 
