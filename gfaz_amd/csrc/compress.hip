@@ -28,7 +28,6 @@ namespace {
 
 constexpr uint32_t kInsItems = 8;                   // pair-insert tile: 2048 elements
 constexpr uint32_t kInsTile = kBlock * kInsItems;
-constexpr uint32_t kLdsSlots = 4096;                // LDS pair table (>= 2x tile)
 constexpr uint32_t kRtItems = 16;                   // cand/emit tile: 4096 elements
 constexpr uint32_t kRtTile = kBlock * kRtItems;     // = 64 words of 64 candidates
 constexpr uint32_t kRtWords = kRtTile / 64;
@@ -361,10 +360,11 @@ constexpr int kDict = 4;
 
 // LDS slot = key | kRep once the tile has seen the key twice (kRep is free:
 // compact keys are below 2^62), so the table is 8 bytes per slot.
+template <uint32_t kSlots>
 __device__ inline void lds_add(unsigned long long *skeys, uint64_t k, uint32_t n) {
-  uint32_t s = (uint32_t)key_hash(k) & (kLdsSlots - 1);
+  uint32_t s = (uint32_t)key_hash(k) & (kSlots - 1);
   const unsigned long long v = n >= 2 ? (k | kRep) : k;
-  for (uint32_t p = 0; p < kLdsSlots; ++p) {
+  for (uint32_t p = 0; p < kSlots; ++p) {
     const unsigned long long old = atomicCAS(&skeys[s], (unsigned long long)kFree, v);
     if (old == kFree)
       return;
@@ -373,19 +373,24 @@ __device__ inline void lds_add(unsigned long long *skeys, uint64_t k, uint32_t n
         atomicOr(&skeys[s], (unsigned long long)kRep);
       return;
     }
-    s = (s + 1) & (kLdsSlots - 1);
+    s = (s + 1) & (kSlots - 1);
   }
 }
 
+// A tile holds at most kInsTile pairs, so with kSlots >= kInsTile every key
+// finds a slot (a new key arrives while fewer than kSlots are stored, and a
+// probe visits every slot). Fewer slots keep more blocks resident; more slots
+// keep probes short when most of a tile's keys are distinct.
+template <uint32_t kSlots>
 __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__restrict__ x,
                                                            const uint32_t *__restrict__ heads,
                                                            uint64_t L, uint64_t span,
                                                            unsigned long long *__restrict__ out,
                                                            uint32_t *__restrict__ out_n) {
-  __shared__ unsigned long long skeys[kLdsSlots];
+  __shared__ unsigned long long skeys[kSlots];
   __shared__ uint32_t lds[kBlock / kWave + 1];
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  for (uint32_t s = threadIdx.x; s < kLdsSlots; s += kBlock)
+  for (uint32_t s = threadIdx.x; s < kSlots; s += kBlock)
     skeys[s] = kFree;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__rest
       const uint64_t ends = hmask | ~rem;
       const uint64_t after = lane == kWave - 1 ? 0ull : ends >> (lane + 1);
       const uint32_t len = after ? (uint32_t)__builtin_ctzll(after) + 1u : (uint32_t)(kWave - lane);
-      lds_add(skeys, k, len);
+      lds_add<kSlots>(skeys, k, len);
     }
   }
   if (lane < kDict) {
@@ -443,12 +448,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__rest
         fc = dcnt[d];
       }
     if (fc)
-      lds_add(skeys, fk, fc);
+      lds_add<kSlots>(skeys, fk, fc);
   }
   __syncthreads();
   // The tile's distinct keys (kRep: seen twice in the tile), compacted into the
   // block's region of the staging buffer.
-  constexpr int kPer = kLdsSlots / kBlock;
+  constexpr int kPer = kSlots / kBlock;
   unsigned long long v[kPer];
   uint32_t c = 0;
 #pragma unroll
@@ -593,7 +598,7 @@ __device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__restrict__ x,
                                                            const uint32_t *__restrict__ heads,
                                                            uint64_t L, uint64_t span, Buckets bk) {
-  __shared__ uint32_t hist[kMaxBuckets];
+  extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
   const uint32_t nb = bk.mask + 1;
   for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
     hist[i] = 0;
@@ -652,7 +657,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
     const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
     Buckets bk) {
-  __shared__ uint32_t hist[kMaxBuckets];
+  extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
   const uint32_t nb = bk.mask + 1;
   for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
     hist[i] = 0;
@@ -868,9 +873,14 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     hslot[s] = 0;
   }
   __syncthreads();
+  // Sampled positions are distinct: windows are spread only when they cannot
+  // overlap (window starts at least kHotWin apart); a shorter stream is
+  // sampled whole.
   const uint64_t room = L > kHotWin + 1 ? L - kHotWin - 1 : 0;
+  const bool spread = room >= (uint64_t)kHotWindows * kHotWin;
   for (uint32_t q = threadIdx.x; q < kHotWindows * kHotWin; q += kHotThreads) {
-    const uint64_t i = (uint64_t)(q / kHotWin) * room / kHotWindows + q % kHotWin;
+    const uint64_t i =
+        spread ? (uint64_t)(q / kHotWin) * room / kHotWindows + q % kHotWin : (uint64_t)q;
     if (i + 1 >= L || head_at(heads, i + 1))
       continue;
     const unsigned long long k = pair_key(x[i], x[i + 1], span);
@@ -1532,6 +1542,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     const bool mostly_distinct = round > 0 && 2 * prev_distinct > prev_len;
     DBuf<unsigned long long> tiles;  // aggregated tiles (rounds with hot keys)
     DBuf<uint32_t> tiles_n;
+    // Tiles of few distinct keys (under a quarter of the last round's pairs were
+    // distinct) aggregate in the smaller LDS table.
+    const bool few_keys = round == 0 || 4 * prev_distinct < prev_len;
+    auto aggregate = [&] {
+      tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
+      tiles_n = DBuf<uint32_t>(pool, n_ins);
+      if (few_keys)
+        hipLaunchKernelGGL(k_pair_aggregate<kInsTile>, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+                           ctx.stream(), cur, heads.get(), L, span, tiles.get(), tiles_n.get());
+      else
+        hipLaunchKernelGGL(k_pair_aggregate<2 * kInsTile>, dim3((uint32_t)n_ins), dim3(kBlock),
+                           0, ctx.stream(), cur, heads.get(), L, span, tiles.get(),
+                           tiles_n.get());
+      GFZ_CHECK_LAUNCH();
+    };
     const bool direct = round > 0 && 2 * prev_distinct > prev_len;  // table fallback mode
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
@@ -1552,19 +1577,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                  reinterpret_cast<uint32_t *>(sums.get() + 2)};
       ctx.timed("pair_insert", [&] {
         if (mostly_distinct) {
-          hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), 0,
+          hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), nbk * 4,
                              ctx.stream(), cur, heads.get(), L, span, bk);
         } else {
           // Hot keys: aggregate each tile first, so a key enters a bucket at most
           // once per tile.
-          tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
-          tiles_n = DBuf<uint32_t>(pool, n_ins);
-          hipLaunchKernelGGL(k_pair_aggregate, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                             ctx.stream(), cur, heads.get(), L, span, tiles.get(),
-                             tiles_n.get());
-          GFZ_CHECK_LAUNCH();
-          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                             ctx.stream(), tiles.get(), tiles_n.get(), bk);
+          aggregate();
+          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)n_ins), dim3(kBlock),
+                             nbk * 4, ctx.stream(), tiles.get(), tiles_n.get(), bk);
         }
         GFZ_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
@@ -1597,12 +1617,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                              ctx.stream(), cur, heads.get(), L, span, t);
         } else {
           if (!tiles.get()) {  // the aggregation does not depend on the table size
-            tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
-            tiles_n = DBuf<uint32_t>(pool, n_ins);
-            hipLaunchKernelGGL(k_pair_aggregate, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                               ctx.stream(), cur, heads.get(), L, span, tiles.get(),
-                               tiles_n.get());
-            GFZ_CHECK_LAUNCH();
+            aggregate();
           }
           hipLaunchKernelGGL(k_pair_insert_staged, dim3((uint32_t)n_ins), dim3(kBlock), 0,
                              ctx.stream(), tiles.get(), tiles_n.get(), t);
