@@ -653,42 +653,49 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
     atomicExch(bk.overflow, 1u);
 }
 
-// Same partitioning for aggregated tiles (entries already carry kRep).
+// Same partitioning for aggregated tiles (entries already carry kRep), `per`
+// tiles per block: one reservation per (block, bucket) covers them all, which
+// keeps the atomics on each bucket's fill counter few. A first pass counts per
+// bucket, a second re-reads the entries (cache hits) and places each through an
+// LDS cursor inside the block's reserved range.
+constexpr uint32_t kScatterTilesMax = 4;
+constexpr uint32_t kScatterMinBlocks = 256;  // below this, fewer tiles per block
+
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
     const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
-    Buckets bk) {
+    uint64_t n_tiles, uint32_t per, Buckets bk) {
   extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
   const uint32_t nb = bk.mask + 1;
   for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
     hist[i] = 0;
   __syncthreads();
-  const uint32_t n = tiles_n[blockIdx.x];
-  const unsigned long long *src = tiles + (uint64_t)blockIdx.x * kInsTile;
-  uint64_t kv[kInsItems];
-  uint32_t loc[kInsItems];
-#pragma unroll
-  for (uint32_t j = 0; j < kInsItems; ++j) {
-    const uint32_t e = j * kBlock + threadIdx.x;
-    kv[j] = e < n ? src[e] : kFree;
-    loc[j] = e < n ? atomicAdd(&hist[bucket_of(kv[j] & ~kRep, bk.mask)], 1u) : 0u;
+  const uint64_t t0 = (uint64_t)blockIdx.x * per;
+  const uint64_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint32_t n = tiles_n[t];
+    const unsigned long long *src = tiles + t * kInsTile;
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock)
+      atomicAdd(&hist[bucket_of(src[e] & ~kRep, bk.mask)], 1u);
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
     const uint32_t c = hist[b];
-    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;
+    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;  // this block's start in bucket b
   }
   __syncthreads();
   bool over = false;
-#pragma unroll
-  for (uint32_t j = 0; j < kInsItems; ++j) {
-    if (kv[j] == kFree)
-      continue;
-    const uint32_t b = bucket_of(kv[j] & ~kRep, bk.mask);
-    const uint64_t pos = (uint64_t)hist[b] + loc[j];
-    if (pos < bk.cap)
-      bk.entries[(uint64_t)b * bk.cap + pos] = kv[j];
-    else
-      over = true;
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint32_t n = tiles_n[t];
+    const unsigned long long *src = tiles + t * kInsTile;
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+      const unsigned long long k = src[e];
+      const uint32_t b = bucket_of(k & ~kRep, bk.mask);
+      const uint64_t pos = atomicAdd(&hist[b], 1u);
+      if (pos < bk.cap)
+        bk.entries[(uint64_t)b * bk.cap + pos] = k;
+      else
+        over = true;
+    }
   }
   if (__syncthreads_or(over) && threadIdx.x == 0)
     atomicExch(bk.overflow, 1u);
@@ -1583,8 +1590,11 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
           // Hot keys: aggregate each tile first, so a key enters a bucket at most
           // once per tile.
           aggregate();
-          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)n_ins), dim3(kBlock),
-                             nbk * 4, ctx.stream(), tiles.get(), tiles_n.get(), bk);
+          const uint32_t per = (uint32_t)std::max<uint64_t>(
+              1, std::min<uint64_t>(kScatterTilesMax, n_ins / kScatterMinBlocks));
+          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)ceil_div(n_ins, per)),
+                             dim3(kBlock), nbk * 4, ctx.stream(), tiles.get(), tiles_n.get(),
+                             n_ins, per, bk);
         }
         GFZ_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
