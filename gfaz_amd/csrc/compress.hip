@@ -1568,12 +1568,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
     bool counted = false;
-    // Bucket counting pays one global atomic per (tile, bucket): while tile
-    // aggregation leaves few distinct keys (under 5% of pairs in the last
-    // round), inserting the aggregated tiles into the table is cheaper.
-    const bool use_buckets = round > 0 && 20 * prev_distinct > prev_len;
+    // Every round counts in buckets; the first round's distinct keys are
+    // estimated at L/32 (a bucket that overflows sends the round to the table).
+    const bool use_buckets = true;
     if (use_buckets) {
-      const uint64_t est = std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
+      const uint64_t est =
+          round == 0 ? L / 32 : std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
       const uint64_t nbk = std::min<uint64_t>(
           kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
       const uint64_t bcap = ceil_div(L, nbk) + ceil_div(L, nbk) / 4 + 2048;
