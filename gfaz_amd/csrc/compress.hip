@@ -658,8 +658,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
 // keeps the atomics on each bucket's fill counter few. A first pass counts per
 // bucket, a second re-reads the entries (cache hits) and places each through an
 // LDS cursor inside the block's reserved range.
-constexpr uint32_t kScatterTilesMax = 4;
-constexpr uint32_t kScatterMinBlocks = 256;  // below this, fewer tiles per block
+constexpr uint32_t kScatterTilesMax = 16;
+constexpr uint32_t kScatterMinBlocks = 512;  // below this, fewer tiles per block
 
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
     const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
