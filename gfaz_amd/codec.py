@@ -76,6 +76,7 @@ _SIGNATURES = {
     "gfz_ctx_sync": ([_vp], ctypes.c_int),
     "gfz_ctx_set_memory_limit": ([_vp, _u64], ctypes.c_int),
     "gfz_ctx_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
+    "gfz_debug_sort_u64": ([_vp, _vp, _u64, ctypes.c_uint32], ctypes.c_int),
     "gfz_ctx_set_profile_groups": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "gfz_ctx_reset_stats": ([_vp], ctypes.c_int),
     "gfz_ctx_kernel_stat": ([_vp, ctypes.c_char_p, ctypes.POINTER(_u64),
@@ -342,6 +343,18 @@ class Rulebook:
             self.close()
         except Exception:
             pass
+
+
+def debug_sort_u64(ctx: Context, keys: np.ndarray, local_cap: int = 0) -> np.ndarray:
+    """Sorts distinct uint64 keys on the device with the rule-ordering sort
+    (a test entry; local_cap > 0 sends larger buckets down its in-memory path)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    d = DeviceArray.from_host(ctx, keys)
+    try:
+        _check(lib().gfz_debug_sort_u64(ctx.handle, d.ptr, keys.size, int(local_cap)))
+        return d.to_host()
+    finally:
+        d.free()
 
 
 def decompress_traversals(ctx: Context, rb: Rulebook, encoded, encoded_lengths,

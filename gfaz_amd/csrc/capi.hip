@@ -198,6 +198,18 @@ int gfz_memcpy_d2h(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   });
 }
 
+int gfz_debug_sort_u64(gfz_ctx *ctx, uint64_t *keys, uint64_t n, uint32_t local_cap) {
+  if (!ctx || (n && !keys))
+    return fail(GFZ_E_INVALID, "null argument");
+  return guarded(kRuntimePrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<uint64_t> tmp(c.pool(), std::max<uint64_t>(n, 1));
+    gfz::sort_unique_u64(c, keys, tmp.get(), n, local_cap);
+    c.sync();
+  });
+}
+
 int gfz_compress_traversals(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
                             const uint32_t *lengths, uint64_t n_trav, uint32_t n_paths,
                             int rounds, int threshold, int delta_round, uint32_t flags,

@@ -1340,10 +1340,9 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
                                                          const uint8_t *__restrict__ used,
                                                          uint64_t cap, uint64_t *__restrict__ out,
                                                          unsigned long long *scal) {
-  // scal[1] = keys collected, scal[2] = OR of the keys, scal[3] = AND of the keys.
+  // scal[1] = keys collected.
   __shared__ uint32_t lds[kBlock / kWave + 1];
   __shared__ unsigned long long sbase;
-  __shared__ unsigned long long sor[kBlock / kWave], sand[kBlock / kWave];
   const uint64_t s0 = (uint64_t)blockIdx.x * kCollectTile + (uint64_t)threadIdx.x * kCollectItems;
   uint32_t flags = 0;
   if (s0 + kCollectItems <= cap) {
@@ -1364,35 +1363,10 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
   __syncthreads();
   const uint64_t b = sbase + off;
   uint32_t k = 0;
-  uint64_t kor = 0, kand = ~0ull;
   while (flags) {
     const uint32_t q = (uint32_t)__ffs(flags) - 1;
-    const uint64_t key = rkeys[s0 + q];
-    out[b + k++] = key;
-    kor |= key;
-    kand &= key;
+    out[b + k++] = rkeys[s0 + q];
     flags &= flags - 1;
-  }
-  if (!total)
-    return;
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) {
-    kor |= __shfl_xor(kor, d);
-    kand &= __shfl_xor(kand, d);
-  }
-  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  if (lane == 0) {
-    sor[wid] = kor;
-    sand[wid] = kand;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (uint32_t w = 1; w < kBlock / kWave; ++w) {
-      kor |= sor[w];
-      kand &= sand[w];
-    }
-    atomicOr(&scal[2], (unsigned long long)kor);
-    atomicAnd(&scal[3], (unsigned long long)kand);
   }
 }
 
@@ -1664,15 +1638,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
     const RuleTable rt{rkeys.get(), rvals.get(), rcap / kLine - 1};
-    // Round scalars (read back once, after K4): [0] next length, [1] used keys,
-    // [2] OR and [3] AND of them. One launch clears everything the round needs
-    // cleared: the rule table, the scalars, the used flags and the next round's
-    // head bits (up to L: the next length is at most L).
-    DBuf<unsigned long long> scal(pool, 4);
+    // Round scalars (read back once, after K4): [0] next length, [1] used keys.
+    // One launch clears everything the round needs cleared: the rule table, the
+    // scalars, the used flags and the next round's head bits (up to L: the next
+    // length is at most L).
+    DBuf<unsigned long long> scal(pool, 2);
     DBuf<uint8_t> used(pool, rcap);
     fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
-                    {scal.get(), 3, 0, 8},
-                    {scal.get() + 3, 1, ~0ull, 8},
+                    {scal.get(), 2, 0, 8},
                     {used.get(), rcap, 0, 1},
                     {heads_next.get(), ceil_div(L, 32), 0, 4}});
     ctx.timed("extract_repeated", [&] {
@@ -1735,8 +1708,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     cand.reset();
     tile_fn.reset();
     tile_info.reset();
-    const uint64_t *hs = ctx.fetch({{scal.get(), 4, 0}, {d_err.get(), 1, 4}});
-    const uint64_t L_next = hs[0], R = hs[1], varying = hs[2] & ~hs[3];
+    const uint64_t *hs = ctx.fetch({{scal.get(), 2, 0}, {d_err.get(), 1, 4}});
+    const uint64_t L_next = hs[0], R = hs[1];
     if ((uint32_t)hs[4])
       throw Error(kErrCorrupt, "pair table lost a key (internal error)");
     if (L_next >= L)
@@ -1746,7 +1719,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
-    sort_i64_varying(ctx, ukeys.get(), utmp.get(), R, varying, d_err.get());
+    sort_unique_u64(ctx, ukeys.get(), utmp.get(), R);
     if (out.n_rules + R > rules_cap) {
       // Room for four times the first round's repeated keys up front: later
       // rounds add fewer rules, so the columns rarely have to grow.
@@ -1792,8 +1765,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   ctx.sync();
   if (const uint32_t code = (uint32_t)ctx.scalars()[0])
     throw Error(kErrCorrupt, "grammar induction failed an internal check (code " +
-                                 std::to_string(code) + ": 1 sorted key missing from the rule "
-                                 "table, 3 rule table full, 4 sort look-back timed out)");
+                                 std::to_string(code) + ": 1 picked pair or 2 sorted key "
+                                 "missing from the rule table, 3 rule table full)");
 
   // Encoded lengths from the head positions of the final stream.
   if (!ne_index.empty()) {

@@ -1,5 +1,5 @@
 // Device-wide primitives written for gfx950 (no hipCUB/rocPRIM on this path):
-// exclusive scans, LSD radix sort of signed 64-bit keys, fills, head bitmaps.
+// exclusive scans, a sample sort of distinct 64-bit keys, fills, head bitmaps.
 #pragma once
 
 #include "ctx.hpp"
@@ -53,15 +53,10 @@ __device__ inline T wave_reduce_max(T v) {
 void exclusive_scan(Ctx &ctx, const uint32_t *in, uint64_t *out, uint64_t n);
 void exclusive_scan(Ctx &ctx, const uint64_t *in, uint64_t *out, uint64_t n);
 
-// Sort n signed 64-bit keys ascending (as int64). `keys` is sorted in place
-// (tmp must hold n keys). Keys must be distinct or the order of equal keys is
-// irrelevant to the caller.
-void sort_i64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n);
-// Same, when the OR of (key XOR first key) over all keys is already known:
-// bytes that never vary are skipped.
-// `err` (device) is set non-zero if a look-back wait hit its bound.
-void sort_i64_varying(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying,
-                      uint32_t *err);
+// Ascending sort of n distinct 64-bit keys (sample sort; tmp holds n).
+// local_cap (0 = the default 4096) bounds the buckets sorted in registers;
+// larger ones take the in-memory radix path (a smaller cap lets tests reach it).
+void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t local_cap = 0);
 
 void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);

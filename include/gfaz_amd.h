@@ -84,6 +84,11 @@ int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out);
 int gfz_dev_free(gfz_ctx *ctx, void *ptr);
 int gfz_memcpy_h2d(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int gfz_memcpy_d2h(gfz_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* Sorts n distinct 64-bit keys in device memory ascending with the sort that
+ * orders each round's rules; buckets above local_cap keys (0 = default) take
+ * its in-memory path. No counterpart in the reference: a test and diagnostic
+ * entry for the rule-ordering sort. */
+int gfz_debug_sort_u64(gfz_ctx *ctx, uint64_t *keys, uint64_t n, uint32_t local_cap);
 
 /* ---- compress ------------------------------------------------------------
  * Replaces run_path_compression_gpu (include/gpu/compression/
