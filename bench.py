@@ -219,21 +219,32 @@ def run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
 
 
 # Timed kernel groups and the kernel whose hardware counters stand for them.
-GROUP_KERNEL = {"pair_insert": "pair_insert", "cand_tiles": "cand", "emit": "emit",
-                "collect_used": "collect_used", "remap": "remap", "delta": "delta",
-                "radix_sort": "radix_scatter", "expand": "expand_z"}
+# Kernels of each timed group (short names, as in the counter summaries).
+GROUP_KERNELS = {
+    "pair_insert": ["pair_aggregate", "bucket_scatter_tiles", "bucket_scatter", "bucket_count",
+                    "pair_insert_staged", "pair_insert_direct"],
+    "cand_tiles": ["hot_sample", "cand"], "emit": ["emit"], "collect_used": ["collect_used"],
+    "rule_sort": ["ss_splitters", "ss_count", "ss_place", "ss_local"],
+    "assign_ids": ["assign"], "remap": ["remap"], "delta": ["delta"],
+}
 
 
-def measured_traffic(workload: str, group: str):
+def measured_traffic(workload: str, group: str, launches: int):
     """HBM bytes per launch of a kernel group from the committed counter summary
-    (profiles/traffic_<workload>.md: FETCH_SIZE and WRITE_SIZE scaled by a plain
-    copy), or None when no summary covers it."""
+    (profiles/traffic_<workload>.md: FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by a plain copy, over one compress and one decompress), or None when no summary
+    covers it. `launches` = the group's timed launches per compress + decompress."""
     from gfaz_amd.profile_summary import load_traffic
     t = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{workload}.md"))
-    k = GROUP_KERNEL.get(group)
-    if not t or k not in t["kernels"]:
+    if not t or not launches:
         return None
-    return t["kernels"][k]["bytes_per_dispatch"]
+    if group == "expand":  # the decode expansion is the last expand_z dispatch
+        e = t["kernels"].get("expand_z")
+        return e["bytes"][-1] if e and e.get("bytes") else None
+    ks = [t["kernels"][k] for k in GROUP_KERNELS.get(group, []) if k in t["kernels"]]
+    if not ks or any("bytes" not in k for k in ks):
+        return None
+    return sum(sum(k["bytes"]) for k in ks) / launches
 
 
 def main() -> int:
@@ -312,7 +323,7 @@ def main() -> int:
         rb.close()
         return t1 - t0, t2 - t1, comp
 
-    groups = ["pair_insert", "cand_tiles", "tile_scan", "emit", "collect_used", "radix_sort",
+    groups = ["pair_insert", "cand_tiles", "tile_scan", "emit", "collect_used", "rule_sort",
               "assign_ids", "remap", "delta", "rule_sizes", "build_bodies", "expand",
               "inverse_delta"]
 
@@ -393,7 +404,8 @@ def main() -> int:
             ach = per_launch / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                        "traffic": measured_traffic(cfg.name, dom),
+                        "traffic": measured_traffic(cfg.name, dom,
+                                                    round(ks["launches"] / args.steps)),
                         "avg_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch}
     expand_roof = None
     if exp["launches"]:

@@ -205,7 +205,11 @@ int gfz_debug_sort_u64(gfz_ctx *ctx, uint64_t *keys, uint64_t n, uint32_t local_
     gfz::Ctx &c = *ctx->ctx;
     gfz::DeviceGuard g(c.device());
     gfz::DBuf<uint64_t> tmp(c.pool(), std::max<uint64_t>(n, 1));
-    gfz::sort_unique_u64(c, keys, tmp.get(), n, local_cap);
+    gfz::DBuf<uint32_t> err(c.pool(), 2);
+    gfz::fill_u32(c, err.get(), 0, 2);
+    gfz::sort_unique_u64(c, keys, tmp.get(), n, err.get(), local_cap);
+    if ((uint32_t)c.fetch({{err.get(), 1, 0}})[0])
+      throw gfz::Error(GFZ_E_INVALID, "sort look-back did not complete (internal error)");
     c.sync();
   });
 }
@@ -365,6 +369,8 @@ int gfz_decoded_size(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encode
                      uint32_t flags, uint64_t *n_out) {
   if (!ctx || !rb || !n_out || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
     return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "null argument");
+  if (rb->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kDecompressPrefix) + "rulebook belongs to another ctx");
   return guarded(kDecompressPrefix, [&] {
     gfz::Ctx &c = *ctx->ctx;
     gfz::DeviceGuard g(c.device());

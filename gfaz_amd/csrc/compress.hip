@@ -1542,12 +1542,13 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
     bool counted = false;
-    // Every round counts in buckets; the first round's distinct keys are
-    // estimated at L/32 (a bucket that overflows sends the round to the table).
-    const bool use_buckets = true;
+    // Rounds count in buckets while their distinct keys (the first round's
+    // estimated at L/32) fit kMaxBuckets buckets of kBktSlots / 2; larger rounds,
+    // and any round whose buckets overflow, count in the HBM table.
+    const uint64_t est =
+        round == 0 ? L / 32 : std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
+    const bool use_buckets = est <= (uint64_t)kMaxBuckets * (kBktSlots / 2);
     if (use_buckets) {
-      const uint64_t est =
-          round == 0 ? L / 32 : std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
       const uint64_t nbk = std::min<uint64_t>(
           kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
       const uint64_t bcap = ceil_div(L, nbk) + ceil_div(L, nbk) / 4 + 2048;
@@ -1719,7 +1720,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
-    sort_unique_u64(ctx, ukeys.get(), utmp.get(), R);
+    sort_unique_u64(ctx, ukeys.get(), utmp.get(), R, d_err.get(), 0,
+                    64 - (uint32_t)__builtin_clzll(span * span));  // keys < span^2
     if (out.n_rules + R > rules_cap) {
       // Room for four times the first round's repeated keys up front: later
       // rounds add fewer rules, so the columns rarely have to grow.
@@ -1766,7 +1768,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   if (const uint32_t code = (uint32_t)ctx.scalars()[0])
     throw Error(kErrCorrupt, "grammar induction failed an internal check (code " +
                                  std::to_string(code) + ": 1 picked pair or 2 sorted key "
-                                 "missing from the rule table, 3 rule table full)");
+                                 "missing from the rule table, 3 rule table full, 4 sort "
+                                 "look-back timed out)");
 
   // Encoded lengths from the head positions of the final stream.
   if (!ne_index.empty()) {

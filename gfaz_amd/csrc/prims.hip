@@ -265,6 +265,214 @@ uint64_t bitmap_positions(Ctx &ctx, const uint32_t *bits, uint64_t n_bits, uint6
   return total;
 }
 
+// ---- one-sweep LSD radix sort (inputs too large for the sample sort) ----------
+namespace {
+// One histogram kernel for every digit, then one kernel per digit: each block
+// takes the next tile by ticket, ranks its keys stably, publishes its per-digit
+// counts, and finds the counts of all earlier tiles by decoupled look-back on
+// per-(tile, digit) status words:
+//   bits 63..62 = 1: this tile's own count, 2: inclusive prefix up to it.
+// A block only waits on tiles with smaller tickets, which belong to blocks
+// that already run and publish before they wait, so the waits end; each is
+// still bounded (kLookbackSpin polls) and raises an error flag if exceeded.
+// A status word carries its own count, so relaxed agent-scope atomics suffice;
+// acquire/release at agent scope would add L2 write-back and invalidation
+// (one L2 per XCD) to every publish and poll.
+// Within a tile each wave owns kOsItems * 64 consecutive keys, loaded at once;
+// a key's rank among its wave's keys of the same digit comes from ballots over
+// the digit bits and one LDS add per digit group, so ranking needs no block
+// barrier, and one barrier turns the per-wave counts into wave offsets.
+constexpr uint32_t kOsItems = 16;
+constexpr uint32_t kOsTile = kBlock * kOsItems;
+constexpr uint64_t kStFlagAgg = 1ull << 62, kStFlagPre = 2ull << 62,
+                   kStCount = (1ull << 62) - 1;
+constexpr uint32_t kLookbackSpin = 1u << 22;
+constexpr uint32_t kWindow = 16;
+
+__global__ __launch_bounds__(kBlock) void k_sort_hist_all(const uint64_t *__restrict__ keys,
+                                                          uint64_t n, uint64_t varying,
+                                                          unsigned long long *__restrict__ ghist) {
+  __shared__ uint32_t h[8][256];
+  for (uint32_t i = threadIdx.x; i < 8 * 256; i += kBlock)
+    (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kOsTile;
+  uint64_t key[kOsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+    key[j] = i < n ? keys[i] : 0;
+  }
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    if (base + (uint64_t)j * kBlock + threadIdx.x >= n)
+      break;
+#pragma unroll
+    for (uint32_t p = 0; p < 8; ++p)
+      if ((varying >> (8 * p)) & 255)
+        atomicAdd(&h[p][(key[j] >> (8 * p)) & 255], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 8 * 256; i += kBlock) {
+    const uint32_t c = (&h[0][0])[i];
+    if (c)
+      atomicAdd(&ghist[i], (unsigned long long)c);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t *__restrict__ in,
+                                                     uint64_t *__restrict__ out, uint64_t n,
+                                                     uint32_t shift,
+                                                     const unsigned long long *__restrict__ ghist,
+                                                     unsigned long long *status,
+                                                     unsigned long long *ticket, uint32_t *err) {
+  __shared__ uint32_t wbin[kBlock / kWave][256];  // per-wave digit counts, then wave offsets
+  __shared__ uint64_t base_off[256];
+  __shared__ uint64_t lds[kBlock / kWave + 1];
+  __shared__ unsigned long long s_tile;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  if (tid == 0)
+    s_tile = atomicAdd(ticket, 1ull);
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / kWave; ++w)
+    wbin[w][tid] = 0;
+  __syncthreads();
+  const uint64_t t = s_tile;
+  const uint64_t wbase = t * kOsTile + (uint64_t)wid * (kOsItems * kWave);
+  uint64_t key[kOsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const uint64_t i = wbase + (uint64_t)j * kWave + lane;
+    key[j] = i < n ? in[i] : 0;
+  }
+  // Phase A: rank among the wave's keys of the same digit (item, then lane).
+  const uint64_t below = lanemask_lt();
+  uint32_t rank[kOsItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    const bool valid = wbase + (uint64_t)j * kWave + lane < n;
+    const uint32_t d = (uint32_t)((key[j] >> shift) & 255);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t leader = peers ? (uint32_t)__ffsll((long long)peers) - 1 : lane;
+    uint32_t old = 0;
+    if (valid && lane == leader)
+      old = atomicAdd(&wbin[wid][d], (uint32_t)__popcll(peers));
+    old = __shfl(old, (int)leader);
+    rank[j] = old + (uint32_t)__popcll(peers & below);
+  }
+  __syncthreads();
+  uint32_t r = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+    const uint32_t c = wbin[w][tid];
+    wbin[w][tid] = r;
+    r += c;
+  }
+  // Phase B: digit tid - global start, then this tile's place among earlier tiles.
+  uint64_t total;
+  const uint64_t gstart = block_exclusive_scan<kBlock>((uint64_t)ghist[tid], lds, total);
+  const uint64_t mine = r;
+  unsigned long long *st = status + t * 256 + tid;
+  uint64_t excl = 0;
+  if (t == 0) {
+    __hip_atomic_store(st, kStFlagPre | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(st, kStFlagAgg | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Windowed look-back: kWindow predecessors per round trip, nearest first.
+    int64_t k = (int64_t)t - 1;  // tile 0 always publishes a prefix: k never passes it
+    uint32_t spins = 0;
+    for (;;) {
+      uint64_t v[kWindow];
+#pragma unroll
+      for (uint32_t w = 0; w < kWindow; ++w)
+        v[w] = k - (int64_t)w >= 0
+                   ? __hip_atomic_load(status + (uint64_t)(k - (int64_t)w) * 256 + tid,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : kStFlagPre;  // below tile 0: nothing more to add
+      bool done = false;
+      uint32_t used = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < kWindow; ++w) {
+        if (done || used < w)
+          continue;
+        if (v[w] & kStFlagPre) {
+          excl += v[w] & kStCount;
+          done = true;
+        } else if (v[w] & kStFlagAgg) {
+          excl += v[w] & kStCount;
+          used = w + 1;
+        }
+      }
+      if (done)
+        break;
+      k -= used;  // resume at the first tile that has not published yet
+      if (used == 0) {
+        if (++spins > kLookbackSpin) {
+          atomicExch(err, 4u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __hip_atomic_store(st, kStFlagPre | (excl + mine), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  base_off[tid] = gstart + excl;
+  __syncthreads();
+  // Phase C: scatter.
+#pragma unroll
+  for (uint32_t j = 0; j < kOsItems; ++j) {
+    if (wbase + (uint64_t)j * kWave + lane < n) {
+      const uint32_t d = (uint32_t)((key[j] >> shift) & 255);
+      out[base_off[d] + wbin[wid][d] + rank[j]] = key[j];
+    }
+  }
+}
+
+
+void sort_radix_large(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t varying,
+                      uint32_t *err) {
+  if (n < 2 || !varying)
+    return;
+  uint32_t passes = 0;
+  for (uint32_t p = 0; p < 8; ++p)
+    passes += ((varying >> (8 * p)) & 255) ? 1 : 0;
+  const uint64_t tiles = ceil_div(n, kOsTile);
+  // ghist (8 x 256) | tickets (8) | status (passes x tiles x 256), cleared by one fill.
+  const uint64_t words = 8 * 256 + 8 + (uint64_t)passes * tiles * 256;
+  DBuf<unsigned long long> scratch(ctx.pool(), words);
+  unsigned long long *ghist = scratch.get(), *tickets = ghist + 8 * 256,
+                     *status = tickets + 8;
+  fill_u64(ctx, reinterpret_cast<uint64_t *>(scratch.get()), 0, words);
+  {
+    hipLaunchKernelGGL(k_sort_hist_all, dim3((uint32_t)tiles), dim3(kBlock), 0, ctx.stream(),
+                       keys, n, varying, ghist);
+    GFZ_CHECK_LAUNCH();
+    uint64_t *src = keys, *dst = tmp;
+    uint32_t q = 0;
+    for (uint32_t p = 0; p < 8; ++p) {
+      if (((varying >> (8 * p)) & 255) == 0)
+        continue;
+      hipLaunchKernelGGL(k_onesweep, dim3((uint32_t)tiles), dim3(kBlock), 0, ctx.stream(), src,
+                         dst, n, 8 * p, ghist + 256 * p, status + (uint64_t)q * tiles * 256,
+                         tickets + p, err);
+      GFZ_CHECK_LAUNCH();
+      std::swap(src, dst);
+      ++q;
+    }
+    if (src != keys)
+      GFZ_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+  }
+}
+
+}  // namespace
+
 // ---- sample sort of distinct 64-bit keys ---------------------------------------
 // For keys that are all distinct (stability is moot): one block sorts a sample
 // of 8 B keys (at most 4096) and takes B - 1 evenly spaced splitters;
@@ -568,12 +776,20 @@ __global__ __launch_bounds__(kBlock) void k_ss_local(uint64_t *in,
 }
 }  // namespace
 
-void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t local_cap) {
+void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t *err,
+                     uint32_t local_cap, uint32_t key_bits) {
   if (n < 2)
     return;
   if (local_cap == 0 || local_cap > kSsLocalCap)
     local_cap = kSsLocalCap;
-  ctx.timed("radix_sort", [&] {
+  if (n > (uint64_t)kSsMaxBuckets * kSsAvg) {
+    // More keys than 512 buckets of about kSsAvg hold: radix passes over the
+    // bytes a key of key_bits bits can have.
+    const uint64_t varying = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
+    ctx.timed("rule_sort", [&] { sort_radix_large(ctx, keys, tmp, n, varying, err); });
+    return;
+  }
+  ctx.timed("rule_sort", [&] {
     if (n <= local_cap && n <= kSsSingle) {
       hipLaunchKernelGGL(k_ss_local, dim3(1), dim3(kBlock), 0, ctx.stream(), keys,
                          (const unsigned int *)nullptr, 1u, keys, n, local_cap);

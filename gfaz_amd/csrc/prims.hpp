@@ -53,10 +53,14 @@ __device__ inline T wave_reduce_max(T v) {
 void exclusive_scan(Ctx &ctx, const uint32_t *in, uint64_t *out, uint64_t n);
 void exclusive_scan(Ctx &ctx, const uint64_t *in, uint64_t *out, uint64_t n);
 
-// Ascending sort of n distinct 64-bit keys (sample sort; tmp holds n).
-// local_cap (0 = the default 4096) bounds the buckets sorted in registers;
-// larger ones take the in-memory radix path (a smaller cap lets tests reach it).
-void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t local_cap = 0);
+// Ascending sort of n distinct 64-bit keys below 2^key_bits (tmp holds n): a
+// sample sort up to 512 K keys, an LSD radix sort beyond (over the bytes of
+// key_bits). local_cap (0 = the default 4096) bounds the sample sort's buckets
+// sorted in registers; larger ones take its in-memory radix path (a smaller
+// cap lets tests reach it). err (device) is set to 4 if a radix look-back wait
+// hit its bound.
+void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t *err,
+                     uint32_t local_cap = 0, uint32_t key_bits = 64);
 
 void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);

@@ -92,8 +92,11 @@ def traffic_markdown(fetch_dir: str, write_dir: str, cmd_fetch: str, cmd_write: 
         n = max(len(fv), len(wv))
         fb = sum(fv) * f_scale / max(len(fv), 1)
         wb = sum(wv) * w_scale / max(len(wv), 1)
+        per = [(fv[i] if i < len(fv) else 0.0) * f_scale + (wv[i] if i < len(wv) else 0.0) * w_scale
+               for i in range(n)]
         table[k] = {"dispatches": n, "fetch_bytes_per_dispatch": fb,
-                    "write_bytes_per_dispatch": wb, "bytes_per_dispatch": fb + wb}
+                    "write_bytes_per_dispatch": wb, "bytes_per_dispatch": fb + wb,
+                    "bytes": [round(b) for b in per]}  # each dispatch, in launch order
     meta = {"copy_bytes": copy_bytes, "copy_fetch_counter": f_copy, "copy_write_counter": w_copy,
             "fetch_scale": f_scale, "write_scale": w_scale, "kernels": table}
     lines = [f"FETCH_SIZE run: `{cmd_fetch}`", "", f"WRITE_SIZE run: `{cmd_write}`", "",

@@ -157,6 +157,30 @@ def test_corrupt_rulebooks_fail_cleanly(gpu_ctx):
                                     np.array([5], np.uint32))
 
 
+def test_rulebook_of_another_ctx_is_refused(gpu_ctx):
+    # A rulebook's last kernels run on its own ctx's stream, so every call that decodes
+    # with it must use that ctx.
+    say("rulebook ctx check")
+    other = codec.Context(0)
+    try:
+        rb = codec.Rulebook(other, [1], [2], 10)
+        enc, lens = np.array([10, 3], np.int32), np.array([2], np.uint32)
+        with pytest.raises(RuntimeError, match="another ctx"):
+            codec.decompress_traversals(gpu_ctx, rb, enc, lens)  # plans through gfz_decoded_size
+        with pytest.raises(RuntimeError, match="another ctx"):
+            codec.decompress_traversals(gpu_ctx, rb, enc, lens, np.array([3], np.uint32), 0)
+        with pytest.raises(RuntimeError, match="another ctx"):
+            codec.decompress_stream(gpu_ctx, rb, enc, lens, 0, consumer=lambda *a: None)
+        with pytest.raises(RuntimeError, match="another ctx"):
+            codec.Resident(gpu_ctx, rb, enc, lens, delta_round=0)
+        dec, got_lens, mism = codec.decompress_traversals(other, rb, enc, lens,
+                                                          np.array([3], np.uint32), 0)
+        assert not mism and dec.tolist() == [1, 2, 3] and got_lens.tolist() == [3]
+        rb.close()
+    finally:
+        other.close()
+
+
 def test_bodies_and_descent_paths_agree(gpu_ctx, monkeypatch):
     """Expansion through pre-expanded bodies (default) and by descending the rule
     tree (bodies disabled by a zero budget) both reproduce the input."""

@@ -1,6 +1,7 @@
 """The sort that orders each round's rules (ascending distinct 64-bit keys) against
-numpy's sort: every size boundary of its paths, skewed and presorted inputs, and
-buckets forced past the LDS limit onto the in-memory radix path."""
+numpy's sort: every size boundary of its paths (one block, sample sort, radix sort
+beyond 512 K keys), skewed and presorted inputs, and buckets forced past the
+register limit onto the in-memory radix path."""
 import sys
 
 import numpy as np
@@ -29,7 +30,7 @@ def check(ctx, keys, local_cap=0):
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 100, 1024, 1025, 2487, 4095, 4096, 4097, 8192, 8193,
-                               50_000, 515_000])
+                               50_000, 515_000, 524_288, 524_289, 2_000_000])
 def test_sort_sizes(gpu_ctx, n):
     check(gpu_ctx, distinct_keys(np.random.default_rng(n), n))
 
