@@ -9,8 +9,22 @@ namespace gfz {
 
 Pool::~Pool() { trim(); }
 
-void *Pool::get(size_t bytes) {
+// Requests are rounded to size classes (256-byte multiples up to 4 KB, then
+// quarter steps between powers of two: at most 25% slack), so blocks freed by
+// one call serve the slightly different sizes of the next.
+static size_t size_class(size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
+  if (bytes <= 4096)
+    return bytes;
+  size_t p = 1;
+  while (p <= bytes / 2)
+    p <<= 1;  // largest power of two <= bytes
+  const size_t step = p / 4;
+  return (bytes + step - 1) / step * step;
+}
+
+void *Pool::get(size_t bytes) {
+  bytes = size_class(bytes);
   std::lock_guard<std::mutex> lk(mu_);
   // Smallest cached block that fits and wastes at most half of itself.
   auto it = free_.lower_bound(bytes);
