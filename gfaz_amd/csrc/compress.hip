@@ -869,8 +869,12 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
   __shared__ uint32_t scnt[kHotSampleSlots];
   __shared__ unsigned long long hkey[kHotSlots];
   __shared__ uint32_t hslot[kHotSlots];
-  __shared__ unsigned long long best;
-  __shared__ uint32_t chosen[kHotTop];
+  // The keys seen kHotMin times (at most kHotWindows * kHotWin / kHotMin = 64).
+  __shared__ unsigned long long lkey[kWave];
+  __shared__ uint32_t lcnt[kWave];
+  __shared__ uint32_t ln;
+  __shared__ HotPair stop[kHotTop];
+  static_assert(kHotWindows * kHotWin / kHotMin <= kWave, "one wave ranks the hot keys");
   for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
     skey[s] = kFree;
     scnt[s] = 0;
@@ -879,6 +883,10 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     hkey[s] = kFree;
     hslot[s] = 0;
   }
+  if (threadIdx.x < kHotTop)
+    stop[threadIdx.x] = HotPair{0, 0, 0, 0};
+  if (threadIdx.x == 0)
+    ln = 0;
   __syncthreads();
   // Sampled positions are distinct: windows are spread only when they cannot
   // overlap (window starts at least kHotWin apart); a shorter stream is
@@ -902,54 +910,40 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     }
   }
   __syncthreads();
-  // The kHotTop largest counts (ties to the larger sample slot), one block-wide
-  // maximum at a time.
-  for (uint32_t t = 0; t < kHotTop; ++t) {
-    if (threadIdx.x == 0)
-      best = 0;
-    __syncthreads();
-    unsigned long long mine = 0;
-    for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
-      bool taken = false;
-      for (uint32_t u = 0; u < t; ++u)
-        taken |= chosen[u] == s;
-      const unsigned long long c = ((unsigned long long)scnt[s] << 32) | s;
-      if (scnt[s] >= kHotMin && !taken && c > mine)
-        mine = c;
+  for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads)
+    if (scnt[s] >= kHotMin) {
+      const uint32_t at = atomicAdd(&ln, 1u);
+      lkey[at] = skey[s];
+      lcnt[at] = scnt[s];
     }
-    if (mine)
-      atomicMax(&best, mine);
-    __syncthreads();
-    if (threadIdx.x == 0)
-      chosen[t] = best ? (uint32_t)best : ~0u;
-    __syncthreads();
-  }
-  if (threadIdx.x < kHotTop) {
-    HotPair hp{0, 0, 0, 0};
-    const uint32_t s = chosen[threadIdx.x];
-    if (s != ~0u) {
-      const int64_t slot = rule_find(rt, skey[s]);
-      if (slot >= 0) {
-        const int64_t c = expand_key(skey[s], span);
-        hp = HotPair{key_first(c), key_second(c), (uint32_t)slot, 1};
-      }
+  __syncthreads();
+  // One wave: each lane one listed key; its slot, its rank by count (then key),
+  // the top kHotTop as pairs, all of them into the small table.
+  if (threadIdx.x < kWave) {
+    const uint32_t lane = threadIdx.x, n = ln;
+    const bool have = lane < n;
+    const unsigned long long k = have ? lkey[lane] : kFree;
+    const uint32_t c = have ? lcnt[lane] : 0;
+    const int64_t slot = have ? rule_find(rt, k) : -1;
+    uint32_t rank = 0;
+    for (uint32_t o = 0; o < n; ++o) {
+      const unsigned long long ko = __shfl(k, (int)o);
+      const uint32_t co = __shfl(c, (int)o);
+      rank += co > c || (co == c && ko < k);
     }
-    top[threadIdx.x] = hp;
-  }
-  for (uint32_t s = threadIdx.x; s < kHotSampleSlots; s += kHotThreads) {
-    if (scnt[s] < kHotMin)
-      continue;
-    const unsigned long long k = skey[s];
-    const int64_t slot = rule_find(rt, k);
-    if (slot < 0)
-      continue;
-    uint32_t h = (uint32_t)key_hash(k) & (kHotSlots - 1);
-    for (uint32_t p = 0; p < kHotSlots; ++p) {
-      if (atomicCAS(&hkey[h], (unsigned long long)kFree, k) == kFree) {
-        hslot[h] = (uint32_t)slot;
-        break;
+    if (slot >= 0) {
+      if (rank < kHotTop) {
+        const int64_t cp = expand_key(k, span);
+        stop[rank] = HotPair{key_first(cp), key_second(cp), (uint32_t)slot, 1};
       }
-      h = (h + 1) & (kHotSlots - 1);
+      uint32_t h = (uint32_t)key_hash(k) & (kHotSlots - 1);
+      for (uint32_t p = 0; p < kHotSlots; ++p) {
+        if (atomicCAS(&hkey[h], (unsigned long long)kFree, k) == kFree) {
+          hslot[h] = (uint32_t)slot;
+          break;
+        }
+        h = (h + 1) & (kHotSlots - 1);
+      }
     }
   }
   __syncthreads();
@@ -960,6 +954,8 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     e.pad = 0;
     hot[s] = e;
   }
+  if (threadIdx.x < kHotTop)
+    top[threadIdx.x] = stop[threadIdx.x];
 }
 
 struct HotLds {
