@@ -555,12 +555,16 @@ __device__ inline uint32_t ss_bucket(const uint64_t *spl, uint32_t B, uint64_t k
   return pos;
 }
 
-// Splitters from a sample of 4 * NT keys (eight per bucket), sorted in registers.
+// Splitters from a sample of 4 * NT keys (eight per bucket), sorted in registers;
+// also clears the bucket counters (fill, then cursor: 2 B words).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_ss_splitters(const uint64_t *__restrict__ keys, uint64_t n,
-                                                    uint32_t B, uint64_t *__restrict__ spl) {
+                                                    uint32_t B, uint64_t *__restrict__ spl,
+                                                    unsigned int *__restrict__ counters) {
   constexpr int kPer = 4, kS = kPer * NT;
   __shared__ uint64_t s[kS];
+  for (uint32_t j = threadIdx.x; j < 2 * B; j += NT)
+    counters[j] = 0;
   uint64_t v[kPer];
 #pragma unroll
   for (int r = 0; r < kPer; ++r)  // n > kS: distinct positions
@@ -808,24 +812,27 @@ void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32
     DBuf<uint64_t> scratch(ctx.pool(), 2 * (uint64_t)B);  // splitters | fill, cursor (u32)
     uint64_t *spl = scratch.get();
     unsigned int *fill = reinterpret_cast<unsigned int *>(scratch.get() + B), *cursor = fill + B;
-    fill_runs(ctx, {{fill, 2 * (uint64_t)B, 0, 4}});
     // Eight samples per bucket, four per thread.
     switch (B) {
       case 2: case 4: case 8: case 16: case 32:
-        hipLaunchKernelGGL(k_ss_splitters<64>, dim3(1), dim3(64), 0, ctx.stream(), keys, n, B, spl);
+        hipLaunchKernelGGL(k_ss_splitters<64>, dim3(1), dim3(64), 0, ctx.stream(), keys, n, B, spl,
+                           fill);
         break;
       case 64:
-        hipLaunchKernelGGL(k_ss_splitters<128>, dim3(1), dim3(128), 0, ctx.stream(), keys, n, B, spl);
+        hipLaunchKernelGGL(k_ss_splitters<128>, dim3(1), dim3(128), 0, ctx.stream(), keys, n, B, spl,
+                           fill);
         break;
       case 128:
-        hipLaunchKernelGGL(k_ss_splitters<256>, dim3(1), dim3(256), 0, ctx.stream(), keys, n, B, spl);
+        hipLaunchKernelGGL(k_ss_splitters<256>, dim3(1), dim3(256), 0, ctx.stream(), keys, n, B, spl,
+                           fill);
         break;
       case 256:
-        hipLaunchKernelGGL(k_ss_splitters<512>, dim3(1), dim3(512), 0, ctx.stream(), keys, n, B, spl);
+        hipLaunchKernelGGL(k_ss_splitters<512>, dim3(1), dim3(512), 0, ctx.stream(), keys, n, B, spl,
+                           fill);
         break;
       default:
         hipLaunchKernelGGL(k_ss_splitters<1024>, dim3(1), dim3(1024), 0, ctx.stream(), keys, n, B,
-                           spl);
+                           spl, fill);
         break;
     }
     GFZ_CHECK_LAUNCH();
