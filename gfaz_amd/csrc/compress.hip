@@ -64,6 +64,47 @@ __device__ inline uint64_t pair_key(int32_t a, int32_t b, uint64_t span) {
   return compact_key(canon2(pack2(a, b)), span);
 }
 
+// The pairs starting at elements first + j * stride (j < N, one thread's pass):
+// a[j], b[j] and a bit set of the valid ones (i + 1 < L and no traversal starts
+// at i + 1). Indexes are clamped into [0, L - 2] instead of guarded (callers run
+// with L >= 2), so every load of the pass is in flight before any is used.
+template <uint32_t N>
+__device__ inline uint32_t load_pairs(const int32_t *__restrict__ x,
+                                      const uint32_t *__restrict__ heads, uint64_t L,
+                                      uint64_t first, uint64_t stride, int32_t (&a)[N],
+                                      int32_t (&b)[N]) {
+  uint32_t hw[N];
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) {
+    const uint64_t i = first + (uint64_t)j * stride;
+    const uint64_t ia = i + 1 < L ? i : L - 2;
+    a[j] = x[ia];
+    b[j] = x[ia + 1];
+    hw[j] = heads[(ia + 1) >> 5];
+  }
+  uint32_t valid = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j) {
+    const uint64_t i = first + (uint64_t)j * stride;
+    valid |= (i + 1 < L && !((hw[j] >> ((i + 1) & 31)) & 1)) ? 1u << j : 0u;
+  }
+  return valid;
+}
+
+// Pair keys of the same pass (kFree where no pair starts); returns the valid set.
+template <uint32_t N>
+__device__ inline uint32_t load_pair_keys(const int32_t *__restrict__ x,
+                                          const uint32_t *__restrict__ heads, uint64_t L,
+                                          uint64_t span, uint64_t first, uint64_t stride,
+                                          uint64_t (&kv)[N]) {
+  int32_t a[N], b[N];
+  const uint32_t valid = load_pairs<N>(x, heads, L, first, stride, a, b);
+#pragma unroll
+  for (uint32_t j = 0; j < N; ++j)
+    kv[j] = (valid >> j) & 1 ? pair_key(a[j], b[j], span) : kFree;
+  return valid;
+}
+
 // Counting table (one 8-byte word per slot): key | kRep once the key was seen
 // twice. Its only job is to find the repeated keys of a round.
 // Inserts give up after kMaxProbe slots and raise `overflow` (the host then
@@ -395,13 +436,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__rest
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
   uint64_t kv[kInsItems];
-  bool valid[kInsItems];
-#pragma unroll
-  for (uint32_t j = 0; j < kInsItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    valid[j] = i + 1 < L && !head_at(heads, i + 1);
-    kv[j] = valid[j] ? pair_key(x[i], x[i + 1], span) : kFree;
-  }
+  const uint32_t valid =
+      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
   // A wave-wide dictionary of up to kDict keys with counts (wave-uniform values)
   // absorbs the hot keys without LDS atomics; it is flushed once at the end.
   uint64_t dkey[kDict];
@@ -413,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__rest
   }
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const uint64_t k = kv[j];
-    uint64_t rem = __ballot(valid[j]);
+    uint64_t rem = __ballot((valid >> j) & 1);
     // (1a) keys already in the dictionary; a free entry takes the first other key
 #pragma unroll
     for (int d = 0; d < kDict; ++d) {
@@ -536,16 +572,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
   uint64_t kv[kInsItems];
   uint32_t cnt[kInsItems];
-  uint32_t have = 0;
+  const uint32_t have =
+      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
 #pragma unroll
-  for (uint32_t j = 0; j < kInsItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    const bool v = i + 1 < L && !head_at(heads, i + 1);
-    kv[j] = v ? pair_key(x[i], x[i + 1], span) : kFree;
+  for (uint32_t j = 0; j < kInsItems; ++j)
     cnt[j] = 1;
-    if (v)
-      have |= 1u << j;
-  }
   uint32_t fresh = 0, reps = 0;
   if (have) {
     uint32_t newrep = 0;
@@ -607,11 +638,12 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
   const uint32_t lane = threadIdx.x & (kWave - 1);
   uint64_t kv[kInsItems];
   uint32_t loc[kInsItems];
+  const uint32_t have =
+      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
 #pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j) {
-    const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-    const bool v = i + 1 < L && !head_at(heads, i + 1);
-    uint64_t k = v ? pair_key(x[i], x[i + 1], span) : kFree;
+    const bool v = (have >> j) & 1;
+    uint64_t k = kv[j];
     // The wave's most common key (its first lane's) is written once; if the wave
     // holds it twice it is already known repeated (kRep set in the entry).
     const uint64_t valid = __ballot(v);
@@ -1076,39 +1108,37 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
   hot_load(sh, hot);
   __syncthreads();
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
+    // Validity is a per-lane bit set (bool arrays live as scalar lane masks).
     int32_t xa[kPassItems], xb[kPassItems];
-    bool valid[kPassItems];
-#pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
-      valid[b] = i + 1 < L && !head_at(heads, i + 1);
-      xa[b] = valid[b] ? x[i] : 0;
-      xb[b] = valid[b] ? x[i + 1] : 0;
-    }
-    uint32_t hit = 0, n = 0;
-    uint64_t m[kPassItems];
+    const uint32_t valid = load_pairs<kPassItems>(
+        x, heads, L, base + (uint64_t)j0 * kBlock + threadIdx.x, kBlock, xa, xb);
+    // Listed items keep their list position per lane (no ballot masks held
+    // across the lookups: they would spill scalar registers).
+    uint32_t hit = 0, listed = 0, n = 0;
+    uint32_t at[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       bool rev;
       uint32_t hs;
-      const bool h = valid[b] && hot_pair(hp, xa[b], xb[b], rev, hs) >= 0;
-      const bool c = valid[b] && !h;
+      const bool v = (valid >> b) & 1;
+      const bool h = v && hot_pair(hp, xa[b], xb[b], rev, hs) >= 0;
+      const bool c = v && !h;
       hit |= h ? 1u << b : 0u;
-      m[b] = __ballot(c);
+      const uint64_t m = __ballot(c);
+      at[b] = n + (uint32_t)__popcll(m & below);
+      listed |= c ? 1u << b : 0u;
       if (c)
-        list[wid][n + (uint32_t)__popcll(m[b] & below)] = (unsigned long long)pack2(xa[b], xb[b]);
-      n += (uint32_t)__popcll(m[b]);
+        list[wid][at[b]] = (unsigned long long)pack2(xa[b], xb[b]);
+      n += (uint32_t)__popcll(m);
     }
     __syncthreads();
     resolve_list(rt, sh, span, list[wid], n);
     __syncthreads();
-    uint32_t pre = 0;
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       bool f = (hit >> b) & 1;
-      if ((m[b] >> lane) & 1)
-        f = list[wid][pre + (uint32_t)__popcll(m[b] & below)] != kFree;
-      pre += (uint32_t)__popcll(m[b]);
+      if ((listed >> b) & 1)
+        f = list[wid][at[b]] != kFree;
       const uint64_t word = __ballot(f);
       if (lane == 0) {
         const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
