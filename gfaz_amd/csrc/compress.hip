@@ -604,48 +604,69 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__
 }
 
 // ---- K1 for rounds of mostly distinct pairs: partitioned counting -----------
-// (1) every pair key goes to one of B hash buckets (fixed capacity each); (2)
-// one block per bucket counts its keys in a 128 KB LDS table and compacts the
-// repeated ones in place. Reads and writes are sequential per bucket instead
-// of one random table access per pair. Any bucket or table overflow raises a
-// flag; the host then counts the round with the global table instead.
+// (1) A partition block takes `per` consecutive input tiles and writes their
+// pair keys into a region of its own, grouped by hash bucket (a counting sort
+// in LDS), with one row of bucket starts; no global atomics. (2) One block per
+// bucket gathers that bucket's segment from every region, counts its keys in a
+// 128 KB LDS table and writes the repeated ones to the bucket's list. A table
+// or list overflow raises a flag; the host then counts the round with the
+// global table instead.
 constexpr uint32_t kBktSlots = 16384;   // LDS table of one bucket
 constexpr uint32_t kBktProbe = 512;
 constexpr uint32_t kBktThreads = 1024;
 constexpr uint32_t kMaxBuckets = 4096;
+constexpr uint32_t kMaxParts = 2048;    // partition blocks (segments per bucket)
+constexpr uint32_t kRepStride = kBktSlots / 2;  // repeated keys listed per bucket
 
-struct Buckets {
-  unsigned long long *entries;  // bucket b at [b * cap, b * cap + cap)
-  uint64_t cap;
-  uint32_t mask;                // B - 1
-  unsigned int *fill;           // keys written per bucket (may exceed cap: overflow)
-  uint32_t *overflow;
+struct Parts {
+  unsigned long long *entries;  // block g's region at g * region
+  uint64_t region;              // per * kInsTile
+  uint32_t *starts;             // row g at g * (nb + 1): bucket starts, then the total
+  uint32_t nb;                  // buckets (a power of two)
+  uint32_t count;               // partition blocks
 };
 
 __device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
   return (uint32_t)(key_hash(k) >> 40) & mask;  // independent of the low probe bits
 }
 
-__global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__restrict__ x,
-                                                           const uint32_t *__restrict__ heads,
-                                                           uint64_t L, uint64_t span, Buckets bk) {
-  extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
-  const uint32_t nb = bk.mask + 1;
-  for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
-    hist[i] = 0;
+// Exclusive scan of the block's bucket histogram in place; writes the block's
+// row of starts (nb + 1 words).
+__device__ inline void part_starts(uint32_t *hist, const Parts &pt) {
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  const uint32_t nb = pt.nb;
+  const uint32_t per_t = (nb + kBlock - 1) / kBlock;
+  const uint32_t b0 = threadIdx.x * per_t;
+  uint32_t sum = 0;
+  for (uint32_t j = 0; j < per_t && b0 + j < nb; ++j)
+    sum += hist[b0 + j];
+  uint32_t total;
+  uint32_t run = block_exclusive_scan<kBlock>(sum, lds, total);
+  for (uint32_t j = 0; j < per_t && b0 + j < nb; ++j) {
+    const uint32_t c = hist[b0 + j];
+    hist[b0 + j] = run;
+    run += c;
+  }
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
+  uint32_t *row = pt.starts + (uint64_t)blockIdx.x * (nb + 1);
+  for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
+    row[b] = hist[b];
+  if (threadIdx.x == 0)
+    row[nb] = total;
+}
+
+// Keys of one pair-insert tile straight from the stream. The wave's most common
+// key (its first lane's) is written once; if the wave holds it twice it is
+// already known repeated (kRep set in the entry).
+__device__ inline void tile_keys(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
+                                 uint64_t L, uint64_t span, uint64_t t, uint64_t (&kv)[kInsItems]) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  uint64_t kv[kInsItems];
-  uint32_t loc[kInsItems];
   const uint32_t have =
-      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
+      load_pair_keys<kInsItems>(x, heads, L, span, t * kInsTile + threadIdx.x, kBlock, kv);
 #pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const bool v = (have >> j) & 1;
     uint64_t k = kv[j];
-    // The wave's most common key (its first lane's) is written once; if the wave
-    // holds it twice it is already known repeated (kRep set in the entry).
     const uint64_t valid = __ballot(v);
     if (valid) {
       const uint32_t leader = (uint32_t)__builtin_ctzll(valid);
@@ -659,46 +680,48 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t *__rest
       }
     }
     kv[j] = k;
-    loc[j] = k != kFree ? atomicAdd(&hist[bucket_of(k & ~kRep, bk.mask)], 1u) : 0u;
   }
-  __syncthreads();
-  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
-    const uint32_t c = hist[b];
-    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;  // this block's start in bucket b
-  }
-  __syncthreads();
-  bool over = false;
-#pragma unroll
-  for (uint32_t j = 0; j < kInsItems; ++j) {
-    if (kv[j] == kFree)
-      continue;
-    const uint32_t b = bucket_of(kv[j] & ~kRep, bk.mask);
-    const uint64_t pos = (uint64_t)hist[b] + loc[j];
-    if (pos < bk.cap)
-      bk.entries[(uint64_t)b * bk.cap + pos] = kv[j];
-    else
-      over = true;
-  }
-  // One flag write per block: a full bucket must not turn every later key into
-  // an atomic on the same address.
-  if (__syncthreads_or(over) && threadIdx.x == 0)
-    atomicExch(bk.overflow, 1u);
 }
 
-// Same partitioning for aggregated tiles (entries already carry kRep), `per`
-// tiles per block: one reservation per (block, bucket) covers them all, which
-// keeps the atomics on each bucket's fill counter few. A first pass counts per
-// bucket, a second re-reads the entries (cache hits) and places each through an
-// LDS cursor inside the block's reserved range.
-constexpr uint32_t kScatterTilesMax = 16;
-constexpr uint32_t kScatterMinBlocks = 512;  // below this, fewer tiles per block
-
-__global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
-    const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
-    uint64_t n_tiles, uint32_t per, Buckets bk) {
+__global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restrict__ x,
+                                                        const uint32_t *__restrict__ heads,
+                                                        uint64_t L, uint64_t span, uint64_t n_tiles,
+                                                        uint32_t per, Parts pt) {
   extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
-  const uint32_t nb = bk.mask + 1;
-  for (uint32_t i = threadIdx.x; i < nb; i += kBlock)
+  const uint32_t mask = pt.nb - 1;
+  for (uint32_t i = threadIdx.x; i < pt.nb; i += kBlock)
+    hist[i] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * per;
+  const uint64_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
+  uint64_t kv[kInsItems];
+  for (uint64_t t = t0; t < t1; ++t) {
+    tile_keys(x, heads, L, span, t, kv);
+#pragma unroll
+    for (uint32_t j = 0; j < kInsItems; ++j)
+      if (kv[j] != kFree)
+        atomicAdd(&hist[bucket_of(kv[j] & ~kRep, mask)], 1u);
+  }
+  __syncthreads();
+  part_starts(hist, pt);
+  // Second pass (the same keys, re-read from cache): LDS cursors place them.
+  unsigned long long *dst = pt.entries + (uint64_t)blockIdx.x * pt.region;
+  for (uint64_t t = t0; t < t1; ++t) {
+    tile_keys(x, heads, L, span, t, kv);
+#pragma unroll
+    for (uint32_t j = 0; j < kInsItems; ++j)
+      if (kv[j] != kFree)
+        dst[atomicAdd(&hist[bucket_of(kv[j] & ~kRep, mask)], 1u)] = kv[j];
+  }
+}
+
+// The same partition of aggregated tiles (entries already carry kRep).
+__global__ __launch_bounds__(kBlock) void k_bucket_part_tiles(
+    const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
+    uint64_t n_tiles, uint32_t per, Parts pt) {
+  extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
+  const uint32_t mask = pt.nb - 1;
+  for (uint32_t i = threadIdx.x; i < pt.nb; i += kBlock)
     hist[i] = 0;
   __syncthreads();
   const uint64_t t0 = (uint64_t)blockIdx.x * per;
@@ -707,35 +730,28 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter_tiles(
     const uint32_t n = tiles_n[t];
     const unsigned long long *src = tiles + t * kInsTile;
     for (uint32_t e = threadIdx.x; e < n; e += kBlock)
-      atomicAdd(&hist[bucket_of(src[e] & ~kRep, bk.mask)], 1u);
+      atomicAdd(&hist[bucket_of(src[e] & ~kRep, mask)], 1u);
   }
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
-    const uint32_t c = hist[b];
-    hist[b] = c ? atomicAdd(&bk.fill[b], c) : 0u;  // this block's start in bucket b
-  }
-  __syncthreads();
-  bool over = false;
+  part_starts(hist, pt);
+  unsigned long long *dst = pt.entries + (uint64_t)blockIdx.x * pt.region;
   for (uint64_t t = t0; t < t1; ++t) {
     const uint32_t n = tiles_n[t];
     const unsigned long long *src = tiles + t * kInsTile;
     for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
       const unsigned long long k = src[e];
-      const uint32_t b = bucket_of(k & ~kRep, bk.mask);
-      const uint64_t pos = atomicAdd(&hist[b], 1u);
-      if (pos < bk.cap)
-        bk.entries[(uint64_t)b * bk.cap + pos] = k;
-      else
-        over = true;
+      dst[atomicAdd(&hist[bucket_of(k & ~kRep, mask)], 1u)] = k;
     }
   }
-  if (__syncthreads_or(over) && threadIdx.x == 0)
-    atomicExch(bk.overflow, 1u);
 }
 
-__global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
+// One block per bucket: its segment of every partition region, counted in an
+// LDS table; the repeated keys go to rep[b * kRepStride ...].
+__global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
+                                                              unsigned long long *__restrict__ rep,
                                                               uint2 *__restrict__ block_counts,
-                                                              unsigned long long *__restrict__ sums) {
+                                                              unsigned long long *__restrict__ sums,
+                                                              uint32_t *__restrict__ overflow) {
   __shared__ unsigned long long tab[kBktSlots];
   __shared__ uint32_t lds[kBktThreads / kWave + 1];
   constexpr uint32_t kPer = kBktSlots / kBktThreads;
@@ -743,34 +759,46 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
     tab[i] = kFree;
   __syncthreads();
   const uint32_t b = blockIdx.x;
-  const uint64_t n = bk.fill[b] < bk.cap ? bk.fill[b] : bk.cap;
-  unsigned long long *ent = bk.entries + (uint64_t)b * bk.cap;
   bool fail = false;
-  for (uint64_t e = threadIdx.x; e < n; e += kBktThreads) {
-    const uint64_t ev = ent[e];  // key, with kRep if already known repeated
+  auto insert = [&](uint64_t ev) {  // ev: key, with kRep if already known repeated
     const uint64_t k = ev & ~kRep;
     uint32_t sl = (uint32_t)key_hash(k) & (kBktSlots - 1);
-    bool done = false;
-    for (uint32_t p = 0; p < kBktProbe && !done; ++p) {
+    for (uint32_t p = 0; p < kBktProbe; ++p) {
       const unsigned long long old = atomicCAS(&tab[sl], (unsigned long long)kFree,
                                                (unsigned long long)ev);
-      if (old == kFree) {
-        done = true;
-      } else if ((old & ~kRep) == k) {
+      if (old == kFree)
+        return;
+      if ((old & ~kRep) == k) {
         if (!(old & kRep))
           atomicOr(&tab[sl], (unsigned long long)kRep);
-        done = true;
-      } else {
-        sl = (sl + 1) & (kBktSlots - 1);
+        return;
       }
+      sl = (sl + 1) & (kBktSlots - 1);
     }
-    fail = fail || !done;
+    fail = true;
+  };
+  // Segment g: [starts[g][b], starts[g][b + 1]) of region g; four entries are
+  // loaded together (clamped into the segment), then inserted.
+  constexpr uint32_t kBatch = 4;
+  for (uint32_t g = threadIdx.x; g < pt.count; g += kBktThreads) {
+    const uint32_t *row = pt.starts + (uint64_t)g * (pt.nb + 1);
+    const uint32_t s0 = row[b], s1 = row[b + 1];
+    const unsigned long long *src = pt.entries + (uint64_t)g * pt.region;
+    for (uint32_t i = s0; i < s1; i += kBatch) {
+      uint64_t ev[kBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kBatch; ++j)
+        ev[j] = src[i + j < s1 ? i + j : s1 - 1];
+#pragma unroll
+      for (uint32_t j = 0; j < kBatch; ++j)
+        if (i + j < s1)
+          insert(ev[j]);
+    }
   }
-  if (__syncthreads_or(fail) && threadIdx.x == 0)
-    atomicExch(bk.overflow, 1u);
-  // Counts, and the repeated keys compacted to the front of the bucket.
+  // Counts, and the repeated keys compacted into the bucket's list.
   uint32_t d = 0, r = 0;
   unsigned long long v[kPer];
+  __syncthreads();
 #pragma unroll
   for (uint32_t i = 0; i < kPer; ++i) {
     v[i] = tab[threadIdx.x * kPer + i];
@@ -779,13 +807,21 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Buckets bk,
   }
   uint32_t total_r;
   uint32_t off = block_exclusive_scan<kBktThreads>(r, lds, total_r);
+  unsigned long long *out = rep + (uint64_t)b * kRepStride;
 #pragma unroll
   for (uint32_t i = 0; i < kPer; ++i)
-    if (v[i] != kFree && (v[i] & kRep))
-      ent[off++] = v[i] & ~kRep;
+    if (v[i] != kFree && (v[i] & kRep)) {
+      if (off < kRepStride)
+        out[off] = v[i] & ~kRep;
+      ++off;
+    }
   uint32_t total_d;
   block_exclusive_scan<kBktThreads>(d, lds, total_d);
+  if (__syncthreads_or(fail) && threadIdx.x == 0)
+    atomicExch(overflow, 1u);
   if (threadIdx.x == 0) {
+    if (total_r > kRepStride)
+      atomicExch(overflow, 1u);
     block_counts[b] = make_uint2(total_d, total_r);
     atomicAdd(&sums[0], (unsigned long long)total_d);
     atomicAdd(&sums[1], (unsigned long long)total_r);
@@ -1577,29 +1613,31 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     if (use_buckets) {
       const uint64_t nbk = std::min<uint64_t>(
           kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
-      const uint64_t bcap = ceil_div(L, nbk) + ceil_div(L, nbk) / 4 + 2048;
-      stage = DBuf<unsigned long long>(pool, nbk * bcap);
-      DBuf<unsigned int> fill(pool, nbk);
-      fill_runs(ctx, {{fill.get(), nbk, 0, 4}, {sums.get(), 4, 0, 8}});
-      Buckets bk{stage.get(), bcap, (uint32_t)(nbk - 1), fill.get(),
-                 reinterpret_cast<uint32_t *>(sums.get() + 2)};
+      // At most kMaxParts partition blocks, so a bucket gathers at most that many
+      // segments.
+      const uint32_t per = (uint32_t)std::max<uint64_t>(1, ceil_div(n_ins, kMaxParts));
+      const uint64_t n_parts = ceil_div(n_ins, per);
+      DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
+      DBuf<uint32_t> starts(pool, n_parts * (nbk + 1));
+      stage = DBuf<unsigned long long>(pool, nbk * kRepStride);
+      fill_runs(ctx, {{sums.get(), 4, 0, 8}});
+      const Parts pt{part.get(), (uint64_t)per * kInsTile, starts.get(), (uint32_t)nbk,
+                     (uint32_t)n_parts};
       ctx.timed("pair_insert", [&] {
         if (mostly_distinct) {
-          hipLaunchKernelGGL(k_bucket_scatter, dim3((uint32_t)n_ins), dim3(kBlock), nbk * 4,
-                             ctx.stream(), cur, heads.get(), L, span, bk);
+          hipLaunchKernelGGL(k_bucket_part, dim3((uint32_t)n_parts), dim3(kBlock), nbk * 4,
+                             ctx.stream(), cur, heads.get(), L, span, n_ins, per, pt);
         } else {
           // Hot keys: aggregate each tile first, so a key enters a bucket at most
           // once per tile.
           aggregate();
-          const uint32_t per = (uint32_t)std::max<uint64_t>(
-              1, std::min<uint64_t>(kScatterTilesMax, n_ins / kScatterMinBlocks));
-          hipLaunchKernelGGL(k_bucket_scatter_tiles, dim3((uint32_t)ceil_div(n_ins, per)),
-                             dim3(kBlock), nbk * 4, ctx.stream(), tiles.get(), tiles_n.get(),
-                             n_ins, per, bk);
+          hipLaunchKernelGGL(k_bucket_part_tiles, dim3((uint32_t)n_parts), dim3(kBlock), nbk * 4,
+                             ctx.stream(), tiles.get(), tiles_n.get(), n_ins, per, pt);
         }
         GFZ_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
-                           ctx.stream(), bk, bcounts.get(), sums.get());
+                           ctx.stream(), pt, stage.get(), bcounts.get(), sums.get(),
+                           reinterpret_cast<uint32_t *>(sums.get() + 2));
         GFZ_CHECK_LAUNCH();
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
@@ -1608,7 +1646,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         distinct = hc[0];
         repeated = hc[1];
         counted = true;
-        stage_stride = bcap;
+        stage_stride = kRepStride;
         stage_blocks = nbk;
         cap = nbk * kBktSlots;
       } else {
