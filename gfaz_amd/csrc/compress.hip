@@ -47,6 +47,11 @@ __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
 // all-ones free as the empty slot.
 constexpr uint64_t kFree = ~0ull;
 constexpr uint64_t kRep = 1ull << 63;
+// Slot words of the candidate pass: a rule slot (below 2^30: the rule table has
+// a power-of-two capacity with next + capacity < 2^31), bit 31 for the reverse
+// orientation, all ones for no rule.
+constexpr uint32_t kSlotRev = 1u << 31;
+constexpr uint32_t kNoSlot = ~0u;
 
 __device__ inline uint64_t compact_key(int64_t k, uint64_t span) {
   const int64_t a = key_first(k), b = key_second(k);
@@ -1126,13 +1131,17 @@ __device__ inline void hot_top_load(HotPair (&hp)[kHotTop], const HotPair *__res
     hp[t] = top[t];
 }
 
+// Candidate bits, each tile's pick function, and every position's slot word
+// (the rule slot of the pair starting there, kSlotRev when the pair is the
+// reverse of the rule's canonical key, kNoSlot when it is no rule).
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
                                                  uint64_t span, RuleTable rt,
                                                  const HotEntry *__restrict__ hot,
                                                  const HotPair *__restrict__ top,
                                                  uint64_t *__restrict__ cand,
-                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn) {
+                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn,
+                                                 uint32_t *__restrict__ slots) {
   __shared__ uint64_t wbits[kRtWords];
   __shared__ HotLds sh;
   __shared__ unsigned long long list[kBlock / kWave][kColdCap];
@@ -1150,12 +1159,15 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
         x, heads, L, base + (uint64_t)j0 * kBlock + threadIdx.x, kBlock, xa, xb);
     // Listed items keep their list position per lane (no ballot masks held
     // across the lookups: they would spill scalar registers).
-    uint32_t hit = 0, listed = 0, n = 0;
+    // Hot and invalid positions store their slot word now; listed ones after
+    // the wave's lookups.
+    uint32_t hit = 0, listed = 0, neg = 0, n = 0;
     uint32_t at[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      bool rev;
-      uint32_t hs;
+      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
+      bool rev = false;
+      uint32_t hs = 0;
       const bool v = (valid >> b) & 1;
       const bool h = v && hot_pair(hp, xa[b], xb[b], rev, hs) >= 0;
       const bool c = v && !h;
@@ -1163,8 +1175,13 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
       const uint64_t m = __ballot(c);
       at[b] = n + (uint32_t)__popcll(m & below);
       listed |= c ? 1u << b : 0u;
-      if (c)
-        list[wid][at[b]] = (unsigned long long)pack2(xa[b], xb[b]);
+      if (c) {
+        const int64_t raw = pack2(xa[b], xb[b]);
+        list[wid][at[b]] = (unsigned long long)raw;
+        neg |= raw != canon2(raw) ? 1u << b : 0u;
+      } else if (i < L) {
+        slots[i] = h ? (hs | (rev ? kSlotRev : 0u)) : kNoSlot;
+      }
       n += (uint32_t)__popcll(m);
     }
     __syncthreads();
@@ -1173,8 +1190,12 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       bool f = (hit >> b) & 1;
-      if ((listed >> b) & 1)
-        f = list[wid][at[b]] != kFree;
+      if ((listed >> b) & 1) {
+        const unsigned long long sl = list[wid][at[b]];
+        f = sl != kFree;
+        slots[base + (uint64_t)(j0 + b) * kBlock + threadIdx.x] =
+            f ? ((uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u)) : kNoSlot;
+      }
       const uint64_t word = __ballot(f);
       if (lane == 0) {
         const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
@@ -1262,19 +1283,18 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 
 // ---- K3: emit the next round's stream ---------------------------------------
 // Survivors are elements whose predecessor was not picked; a picked element
-// emits +/-(next + slot) (provisional, remapped after the sort). Heads always
-// survive and carry their bit to the output position.
+// emits +/-(next + slot) (provisional, remapped after the sort), its slot and
+// orientation read from the candidate pass's slot words (no rule lookups here).
+// Heads always survive and carry their bit to the output position.
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    uint64_t span, RuleTable rt, const HotEntry *__restrict__ hot,
-    const HotPair *__restrict__ top, uint8_t *__restrict__ used, uint32_t next,
-    int32_t *__restrict__ y, uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
+    const uint32_t *__restrict__ slots, const HotPair *__restrict__ top,
+    uint8_t *__restrict__ used, uint32_t next, int32_t *__restrict__ y,
+    uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
-  __shared__ HotLds sh;
-  __shared__ unsigned long long list[kBlock / kWave][kColdCap];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t below = lanemask_lt();
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
@@ -1283,7 +1303,6 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   const uint64_t off_tile = info >> 1;
   HotPair hp[kHotTop];
   hot_top_load(hp, top);
-  hot_load(sh, hot);
   if (wid == 0) {
     const uint64_t gw = (base >> 6) + lane;
     const uint64_t c = gw < n_words ? cand[gw] : 0;
@@ -1312,77 +1331,50 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   __syncthreads();
   uint32_t hot_used = 0;  // hot entries this lane picked (their used flags, once per wave)
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
-    // Survivors that are not picked, and picks of the hottest pairs, are written
-    // at once; other picks wait for the wave's list.
-    // Every load of the pass first (the stream is read whole anyway).
-    int32_t xa[kPassItems], xb[kPassItems];
-    uint32_t hw[kPassItems];
+    // Every load of the pass first, from indexes clamped into the stream.
+    int32_t xa[kPassItems];
+    uint32_t sv[kPassItems], hw[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint64_t i = base + (uint64_t)((j0 + b) * (kBlock / kWave) + wid) * 64 + lane;
-      xa[b] = i < L ? x[i] : 0;
-      xb[b] = i + 1 < L ? x[i + 1] : 0;
-      hw[b] = i < L ? heads[i >> 5] : 0;
+      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
+      const uint64_t ia = i < L ? i : L - 1;
+      xa[b] = x[ia];
+      sv[b] = slots[ia];
+      hw[b] = heads[ia >> 5];
     }
-    uint64_t m[kPassItems];
-    uint32_t n = 0, neg = 0;
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
       const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
       const uint64_t i = base + (uint64_t)w * 64 + lane;
       const uint64_t svm = ssurv[w];
-      bool c = false;
-      int64_t raw = 0;
       if (i < L && ((svm >> lane) & 1)) {
         const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(svm & below);
         int32_t o = xa[b];
         if ((spicks[w] >> lane) & 1) {
-          const int32_t o2 = xb[b];
-          bool rev;
-          uint32_t hs;
-          const int t = hot_pair(hp, o, o2, rev, hs);
-          if (t >= 0) {
-            hot_used |= 1u << t;
-            o = (int32_t)(next + hs);
-            if (rev)
-              o = -o;
+          const uint32_t sw = sv[b];
+          if (sw == kNoSlot) {
+            atomicExch(err, 1u);  // a picked pair is always a rule
           } else {
-            c = true;
-            raw = pack2(o, o2);
-            neg |= raw != canon2(raw) ? 1u << b : 0u;
+            const uint32_t sl = sw & ~kSlotRev;
+            int t = -1;
+#pragma unroll
+            for (uint32_t q = 0; q < kHotTop; ++q)
+              if (hp[q].set && hp[q].slot == sl)
+                t = (int)q;
+            if (t >= 0)
+              hot_used |= 1u << t;
+            else
+              used[sl] = 1;
+            o = (int32_t)(next + sl);
+            if (sw & kSlotRev)
+              o = -o;
           }
         }
-        if (!c)
-          y[pos] = o;
+        y[pos] = o;
         if ((hw[b] >> (i & 31)) & 1)
           atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
       }
-      m[b] = __ballot(c);
-      if (c)
-        list[wid][n + (uint32_t)__popcll(m[b] & below)] = (unsigned long long)raw;
-      n += (uint32_t)__popcll(m[b]);
     }
-    __syncthreads();
-    resolve_list(rt, sh, span, list[wid], n);
-    __syncthreads();
-    uint32_t pre = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
-      if ((m[b] >> lane) & 1) {
-        const unsigned long long sl = list[wid][pre + (uint32_t)__popcll(m[b] & below)];
-        if (sl == kFree) {
-          atomicExch(err, 1u);
-        } else {
-          const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
-          const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(ssurv[w] & below);
-          used[sl] = 1;
-          const int32_t o = (int32_t)(next + (uint32_t)sl);
-          y[pos] = (neg >> b) & 1 ? -o : o;
-        }
-      }
-      pre += (uint32_t)__popcll(m[b]);
-    }
-    __syncthreads();  // the list is refilled by the next pass
   }
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t)
@@ -1739,10 +1731,11 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint64_t> cand(pool, n_words);
     DBuf<uint32_t> tile_fn(pool, n_tiles);
     DBuf<uint64_t> tile_info(pool, n_tiles);
+    DBuf<uint32_t> slots(pool, L);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, span, rt, hot.get(), top.get(), cand.get(), n_words,
-                         tile_fn.get());
+                         tile_fn.get(), slots.get());
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("cand_tiles", 4.0 * L);
@@ -1756,9 +1749,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // K3: emit.
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), n_words, tile_info.get(), span, rt,
-                         hot.get(), top.get(), used.get(), next, dst, heads_next.get(),
-                         d_err.get());
+                         heads.get(), L, cand.get(), n_words, tile_info.get(), slots.get(),
+                         top.get(), used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
 
