@@ -658,6 +658,7 @@ __device__ inline void part_starts(uint32_t *hist, const Parts &pt) {
     row[b] = hist[b];
   if (threadIdx.x == 0)
     row[nb] = total;
+  __syncthreads();  // the starts are read before the placing pass moves the cursors
 }
 
 // Keys of one pair-insert tile straight from the stream. The wave's most common

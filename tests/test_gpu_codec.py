@@ -111,7 +111,10 @@ def test_empty_inputs(gpu_ctx):
         np.testing.assert_array_equal(got.encoded_lengths, lengths)
 
 
+# 200 K x 48 (about 11.9 M steps): the first rounds have more than 2048 x 2048
+# pairs, so each bucket-partition block takes several tiles.
 @pytest.mark.parametrize("cfg_name,backbone,haps", [("plain_500k", 100_000, 32),
+                                                    ("plain_500k", 200_000, 48),
                                                     ("walks_2m", 200_000, 100)])
 def test_synthetic_scaled_matches_oracle(gpu_ctx, cfg_name, backbone, haps):
     cfg = synth.CONFIGS[cfg_name].scaled(backbone=backbone, n_haps=haps, len_max=200_000)
