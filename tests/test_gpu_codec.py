@@ -125,6 +125,18 @@ def test_synthetic_scaled_matches_oracle(gpu_ctx, cfg_name, backbone, haps):
     check_decompress(gpu_ctx, tr.nodes, tr.lengths, ref, 1)
 
 
+def test_full_size_plain_500k_matches_oracle(gpu_ctx):
+    """The benchmark workload itself (39.57 M steps, 64 P-lines, rounds=8,
+    threshold=2, delta_round=1): every round's counting uses buckets and the
+    first rounds give each bucket-partition block about ten tiles."""
+    cfg = synth.CONFIGS["plain_500k"]
+    say("full-size plain_500k")
+    tr = synth.generate(cfg)
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=16)
+    check_compress(gpu_ctx, tr.nodes, tr.lengths, 8, 1, tr.n_paths, ref=ref)
+    check_decompress(gpu_ctx, tr.nodes, tr.lengths, ref, 1)
+
+
 def test_device_resident_paths_and_determinism(gpu_ctx):
     say("device resident")
     tr = synth.generate(synth.CONFIGS["plain_20k"])
