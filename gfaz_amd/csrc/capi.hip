@@ -379,7 +379,7 @@ int gfz_decoded_size(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encode
     stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
     gfz::DecodePlan plan;
     gfz::DBuf<uint64_t> sym_off;
-    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, nullptr, n_trav, plan,
+    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, nullptr, n_trav, -1, plan,
                      sym_off);
     *n_out = plan.n_out;
   });
@@ -402,8 +402,9 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
     stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
     gfz::DecodePlan plan;
     gfz::DBuf<uint64_t> sym_off;
-    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, orig_lengths, n_trav, plan,
-                     sym_off);
+    const int dr = delta_round < 0 ? 0 : delta_round;
+    gfz::decode_plan(c, rb->rb, d_enc, n_encoded, encoded_lengths, orig_lengths, n_trav, dr,
+                     plan, sym_off);
     if (n_out)
       *n_out = plan.n_out;
     if (length_mismatch)
@@ -416,7 +417,6 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
                                           " < decoded size " + std::to_string(plan.n_out));
     if (plan.n_out && !out)
       throw gfz::Error(GFZ_E_INVALID, "null output buffer");
-    const int dr = delta_round < 0 ? 0 : delta_round;
     if (flags & GFZ_OUTPUT_DEVICE) {
       gfz::decode_run(c, rb->rb, d_enc, n_encoded, encoded_lengths, n_trav, dr, plan,
                       sym_off.get(), out);

@@ -70,20 +70,30 @@ struct Rulebook {
 void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, uint64_t n_rules,
                     uint32_t min_id, Rulebook &rb);
 
+namespace zx {
+// What one encoded symbol writes at its local position q of an expansion
+// through the rule bodies (device side: zexpand.hpp).
+struct alignas(8) SymInfo {
+  uint32_t base;
+  uint32_t zz;
+};
+}  // namespace zx
+
 struct DecodePlan {
   uint64_t n_out = 0;                 // total decoded symbols
   std::vector<uint64_t> out_offsets;  // per traversal (n_trav + 1), host
-  std::vector<uint64_t> enc_starts;   // encoded start of each non-empty traversal
-  DBuf<int32_t> sym_sum;              // sum of each encoded symbol's expansion
+  DBuf<zx::SymInfo> syms;             // per encoded symbol (rulebooks with bodies, decode)
   bool length_mismatch = false;       // decoded lengths differ from the stored originals
 };
 
 // Expand the encoded traversals and undo delta_round prefix sums.
 // d_enc: device; enc_lengths / orig_lengths: host (orig may be null).
 // d_out must hold the decoded total; call decode_size first to learn it.
+// delta_round < 0: sizes only (no symbol records); otherwise the records for a
+// decode_run with that delta_round.
 void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
                  const uint32_t *enc_lengths, const uint32_t *orig_lengths, uint64_t n_trav,
-                 DecodePlan &plan, DBuf<uint64_t> &sym_off);
+                 int delta_round, DecodePlan &plan, DBuf<uint64_t> &sym_off);
 void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
                 const uint32_t *enc_lengths, uint64_t n_trav, int delta_round,
                 const DecodePlan &plan, const uint64_t *d_sym_off, int32_t *d_out);

@@ -172,49 +172,301 @@ __global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t
     out[i] = in[i] + 1u;
 }
 
-// Per encoded symbol: expanded size, and the signed sum of its expansion.
-__global__ __launch_bounds__(kBlock) void k_sym_plan(const int32_t *__restrict__ enc, uint64_t n,
-                                                     const uint32_t *__restrict__ rsize,
-                                                     const uint32_t *__restrict__ rsum, uint64_t R,
-                                                     uint32_t min_id, uint32_t *__restrict__ sz,
-                                                     int32_t *__restrict__ ssum) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock) {
-    const int32_t v = enc[i];
-    if (is_rule(v, min_id, R)) {
-      const uint32_t r = uabs(v) - min_id;
-      sz[i] = rsize[r];
-      ssum[i] = (int32_t)(v > 0 ? rsum[r] : (uint32_t)0 - rsum[r]);
-    } else {
-      sz[i] = 1u;
-      ssum[i] = v;
-    }
-  }
+// ---- decode planning: symbol offsets and records in three launches ---------
+// Per encoded symbol: its expanded size (1 for a node ID), the wrapping sum of
+// its expansion, and whether a traversal starts there. A reduce pass gives
+// every tile of kPlanTile symbols its aggregate, one block scans the tiles, and
+// an apply pass re-derives the tile's symbols and writes each one's decoded
+// offset (exclusive sum of sizes) and record (SymInfo: the traversal's sum of
+// earlier expansions, i.e. a scan of the sums restarted at traversal starts,
+// plus the Z position). No block waits for another.
+constexpr uint32_t kPlanItems = 16;                  // consecutive symbols per thread
+constexpr uint32_t kPlanTile = kBlock * kPlanItems;  // 4096
+constexpr uint32_t kPlanScanThreads = 1024;
+
+// Aggregate of a run of symbols: total size; whether a traversal starts in it;
+// sum of the expansions after its last traversal start (all of them if none).
+struct PlanAgg {
+  uint64_t size;
+  uint32_t head;
+  uint32_t sum;
+};
+__device__ inline PlanAgg plan_combine(const PlanAgg &a, const PlanAgg &b) {
+  return {a.size + b.size, a.head | b.head, b.head ? b.sum : a.sum + b.sum};
+}
+__device__ inline PlanAgg plan_shfl_up(const PlanAgg &x, int d) {
+  return {__shfl_up(x.size, d), __shfl_up(x.head, d), __shfl_up(x.sum, d)};
 }
 
-// Symbol records of the encoded stream. incl = inclusive segmented scan of the
-// symbol sums (traversal starts reset it), so incl - own sum is the sum of the
-// traversal's earlier symbols.
-__global__ __launch_bounds__(kBlock) void k_sym_info(const int32_t *__restrict__ enc, uint64_t n,
-                                                     const int32_t *__restrict__ incl,
-                                                     const int32_t *__restrict__ ssum, bool delta,
-                                                     const uint32_t *__restrict__ rsize,
-                                                     const uint32_t *__restrict__ rsum,
-                                                     const uint64_t *__restrict__ zoff, uint64_t R,
-                                                     uint32_t min_id, uint32_t zraw,
-                                                     SymInfo *__restrict__ out) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock) {
-    const int32_t v = enc[i];
-    const uint32_t pre = delta ? (uint32_t)incl[i] - (uint32_t)ssum[i] : 0u;
-    SymInfo s;
-    if (is_rule(v, min_id, R)) {
-      s = rule_sym(v, pre, delta, rsize, rsum, zoff, min_id);
-    } else {
-      s.base = pre + (uint32_t)v;
-      s.zz = zraw;
+// Ordered exclusive scan of one PlanAgg per thread over a block of NT threads;
+// `lds` holds NT / kWave + 1 entries. Returns the exclusive prefix, sets total.
+template <uint32_t NT>
+__device__ inline PlanAgg plan_block_scan(PlanAgg x, PlanAgg *lds, PlanAgg &total) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  PlanAgg inc = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const PlanAgg y = plan_shfl_up(inc, d);
+    if (lane >= (uint32_t)d)
+      inc = plan_combine(y, inc);
+  }
+  if (lane == kWave - 1)
+    lds[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    PlanAgg run{0, 0, 0};
+    for (uint32_t w = 0; w < NT / kWave; ++w) {
+      const PlanAgg t = lds[w];
+      lds[w] = run;
+      run = plan_combine(run, t);
     }
-    out[i] = s;
+    lds[NT / kWave] = run;
+  }
+  __syncthreads();
+  PlanAgg ex = plan_shfl_up(inc, 1);
+  if (lane == 0)
+    ex = PlanAgg{0, 0, 0};
+  const PlanAgg res = plan_combine(lds[wid], ex);
+  total = lds[NT / kWave];
+  __syncthreads();
+  return res;
+}
+
+// Entries of a[0, n) below v (a non-decreasing); all 64 lanes of one wave call
+// it and get the count. Each step probes 64 evenly spaced entries of the range
+// the count can still be in, which shrinks it to one gap between probes.
+__device__ inline uint64_t wave_count_below(const uint64_t *__restrict__ a, uint64_t n,
+                                            uint64_t v) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  uint64_t lo = 0, hi = n;  // the count lies in [lo, hi]
+  while (lo < hi) {
+    const uint64_t step = (hi - lo + kWave - 1) / kWave;
+    const uint64_t k = lo + (uint64_t)lane * step;
+    const uint32_t c = (uint32_t)__popcll(__ballot(k < hi && a[k] < v));
+    if (c == 0)
+      break;  // a[lo] >= v: the count is lo
+    const uint64_t nlo = lo + (uint64_t)(c - 1) * step + 1;
+    const uint64_t nhi = lo + (uint64_t)c * step;
+    lo = nlo;
+    hi = nhi < hi ? nhi : hi;
+  }
+  return lo;
+}
+
+// The tile's traversal starts as a bitmap in LDS (starts: encoded start of
+// every traversal, non-decreasing).
+__device__ inline void plan_heads(const uint64_t *__restrict__ starts, uint64_t n_starts,
+                                  uint64_t s0, uint64_t s1, uint32_t *hbits, uint64_t *range) {
+  for (uint32_t w = threadIdx.x; w < kPlanTile / 32; w += kBlock)
+    hbits[w] = 0;
+  if (threadIdx.x < kWave) {
+    const uint64_t lo = wave_count_below(starts, n_starts, s0);
+    const uint64_t hi = wave_count_below(starts, n_starts, s1);
+    if (threadIdx.x == 0) {
+      range[0] = lo;
+      range[1] = hi;
+    }
+  }
+  __syncthreads();
+  for (uint64_t t = range[0] + threadIdx.x; t < range[1]; t += kBlock) {
+    const uint32_t q = (uint32_t)(starts[t] - s0);
+    atomicOr(&hbits[q >> 5], 1u << (q & 31));
+  }
+  __syncthreads();
+}
+
+// A wave's kPlanItems x 64 consecutive symbols, item j of lane l at
+// chunk + j * 64 + l (every load and store is one coalesced run per item);
+// indexes are clamped into the stream (n_enc >= 1 whenever a tile exists) so
+// every load is issued before any is used.
+struct PlanSyms {
+  int32_t v[kPlanItems];
+  uint32_t sz[kPlanItems], ss[kPlanItems];
+  uint32_t head;   // bit j: a traversal starts at this lane's item j
+  uint32_t valid;  // bit j: item j is inside the stream
+};
+__device__ inline uint64_t plan_chunk(uint64_t s0) {
+  return s0 + (uint64_t)(threadIdx.x / kWave) * (kPlanItems * kWave);
+}
+__device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc, uint64_t s0,
+                                 const uint32_t *hbits, const uint32_t *__restrict__ rsize,
+                                 const uint32_t *__restrict__ rsum, uint64_t R, uint32_t min_id,
+                                 PlanSyms &p) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t c0 = plan_chunk(s0);
+  p.head = p.valid = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const uint64_t i = c0 + (uint64_t)j * kWave + lane;
+    p.v[j] = enc[i < n_enc ? i : n_enc - 1];
+    p.valid |= i < n_enc ? 1u << j : 0u;
+    const uint32_t q = (uint32_t)(i - s0);
+    p.head |= ((hbits[q >> 5] >> (q & 31)) & 1) << j;
+  }
+  uint32_t rz[kPlanItems], rs[kPlanItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const bool rule = is_rule(p.v[j], min_id, R);
+    const uint32_t r = rule ? uabs(p.v[j]) - min_id : 0u;
+    rz[j] = rule ? rsize[r] : 1u;
+    rs[j] = rule ? rsum[r] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const bool rule = is_rule(p.v[j], min_id, R);
+    const bool ok = (p.valid >> j) & 1;
+    p.sz[j] = ok ? rz[j] : 0u;
+    p.ss[j] = ok ? (rule ? (p.v[j] > 0 ? rs[j] : 0u - rs[j]) : (uint32_t)p.v[j]) : 0u;
+  }
+  p.head &= p.valid;
+}
+
+// Inclusive ordered scan of one PlanAgg per lane across a wave.
+__device__ inline PlanAgg plan_wave_incl(PlanAgg x) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const PlanAgg y = plan_shfl_up(x, d);
+    if (lane >= (uint32_t)d)
+      x = plan_combine(y, x);
+  }
+  return x;
+}
+__device__ inline PlanAgg plan_lane(const PlanAgg &x, int l) {
+  return {__shfl(x.size, l), __shfl(x.head, l), __shfl(x.sum, l)};
+}
+
+// The wave's aggregate over its chunk, in position order (every lane gets it).
+__device__ inline PlanAgg plan_wave_total(const PlanSyms &p) {
+  PlanAgg t{0, 0, 0};
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const PlanAgg inc = plan_wave_incl(PlanAgg{p.sz[j], (p.head >> j) & 1, p.ss[j]});
+    t = plan_combine(t, plan_lane(inc, kWave - 1));
+  }
+  return t;
+}
+
+// Exclusive prefixes of the block's waves (in order) and the block total.
+__device__ inline PlanAgg plan_waves_prefix(const PlanAgg &wave_total, PlanAgg *lds,
+                                            PlanAgg &total) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0)
+    lds[wid] = wave_total;
+  __syncthreads();
+  PlanAgg run{0, 0, 0}, mine{0, 0, 0};
+  for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+    if (w == wid)
+      mine = run;
+    run = plan_combine(run, lds[w]);
+  }
+  total = run;
+  return mine;
+}
+
+__global__ __launch_bounds__(kBlock) void k_plan_reduce(
+    const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
+    uint64_t n_starts, const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
+    uint64_t R, uint32_t min_id, PlanAgg *__restrict__ tiles) {
+  __shared__ uint32_t hbits[kPlanTile / 32];
+  __shared__ uint64_t range[2];
+  __shared__ PlanAgg lds[kBlock / kWave];
+  const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
+  const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
+  plan_heads(starts, n_starts, s0, s1, hbits, range);
+  PlanSyms p;
+  plan_syms(enc, n_enc, s0, hbits, rsize, rsum, R, min_id, p);
+  PlanAgg total;
+  plan_waves_prefix(plan_wave_total(p), lds, total);
+  if (threadIdx.x == 0)
+    tiles[blockIdx.x] = total;
+}
+
+// One block: tile aggregates -> exclusive tile prefixes (in place); the grand
+// total size goes to *end.
+__global__ __launch_bounds__(kPlanScanThreads) void k_plan_scan(PlanAgg *__restrict__ tiles,
+                                                               uint64_t n_tiles,
+                                                               uint64_t *__restrict__ end) {
+  __shared__ PlanAgg lds[kPlanScanThreads / kWave + 1];
+  const uint64_t chunk = (n_tiles + kPlanScanThreads - 1) / kPlanScanThreads;
+  const uint64_t b = (uint64_t)threadIdx.x * chunk;
+  const uint64_t e = b + chunk < n_tiles ? b + chunk : n_tiles;
+  PlanAgg mine{0, 0, 0};
+  for (uint64_t t = b; t < e; ++t)
+    mine = plan_combine(mine, tiles[t]);
+  PlanAgg total;
+  PlanAgg run = plan_block_scan<kPlanScanThreads>(mine, lds, total);
+  for (uint64_t t = b; t < e; ++t) {
+    const PlanAgg x = tiles[t];
+    tiles[t] = run;
+    run = plan_combine(run, x);
+  }
+  if (threadIdx.x == 0)
+    *end = total.size;
+}
+
+// delta < 0: offsets only; otherwise records too (delta > 0: with the first
+// inverse delta folded in).
+__global__ __launch_bounds__(kBlock) void k_plan_apply(
+    const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
+    uint64_t n_starts, const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
+    const uint64_t *__restrict__ zoff, uint64_t R, uint32_t min_id, uint32_t zraw, int delta,
+    const PlanAgg *__restrict__ tile_prefix, uint64_t *__restrict__ sym_off,
+    SymInfo *__restrict__ syms) {
+  __shared__ uint32_t hbits[kPlanTile / 32];
+  __shared__ uint64_t range[2];
+  __shared__ PlanAgg lds[kBlock / kWave];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
+  const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
+  plan_heads(starts, n_starts, s0, s1, hbits, range);
+  PlanSyms p;
+  plan_syms(enc, n_enc, s0, hbits, rsize, rsum, R, min_id, p);
+  // Z positions of the rule symbols, loaded before any is used.
+  uint64_t zo[kPlanItems];
+  if (delta >= 0) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPlanItems; ++j) {
+      const bool rule = is_rule(p.v[j], min_id, R);
+      zo[j] = rule ? zoff[uabs(p.v[j]) - min_id] : 0u;
+    }
+  }
+  PlanAgg total;
+  PlanAgg carry =
+      plan_combine(tile_prefix[blockIdx.x], plan_waves_prefix(plan_wave_total(p), lds, total));
+  const uint64_t c0 = plan_chunk(s0);
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const bool head = (p.head >> j) & 1;
+    const PlanAgg x{p.sz[j], head ? 1u : 0u, p.ss[j]};
+    const PlanAgg inc = plan_wave_incl(x);
+    PlanAgg ex = plan_shfl_up(inc, 1);
+    if (lane == 0)
+      ex = PlanAgg{0, 0, 0};
+    const PlanAgg at = plan_combine(carry, ex);
+    carry = plan_combine(carry, plan_lane(inc, kWave - 1));
+    const uint64_t i = c0 + (uint64_t)j * kWave + lane;
+    if ((p.valid >> j) & 1) {
+      sym_off[i] = at.size;
+      if (delta >= 0) {
+        const uint32_t pre = delta > 0 && !head ? at.sum : 0u;
+        SymInfo s;
+        const int32_t v = p.v[j];
+        if (is_rule(v, min_id, R)) {  // as rule_sym: ss = -rsum for a reversed rule
+          if (v > 0) {
+            s.base = pre;
+            s.zz = (uint32_t)(zo[j] + 1);
+          } else {
+            s.base = delta > 0 ? pre + p.ss[j] : pre;
+            s.zz = (uint32_t)(zo[j] + p.sz[j] - 1) | kRevBit;
+          }
+        } else {
+          s.base = pre + (uint32_t)v;
+          s.zz = zraw;
+        }
+        syms[i] = s;
+      }
+    }
   }
 }
 
@@ -687,7 +939,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
 
 void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
                  const uint32_t *enc_lengths, const uint32_t *orig_lengths, uint64_t n_trav,
-                 DecodePlan &plan, DBuf<uint64_t> &sym_off) {
+                 int delta_round, DecodePlan &plan, DBuf<uint64_t> &sym_off) {
   plan = DecodePlan();
   uint64_t total_enc = 0;
   std::vector<uint64_t> enc_start(n_trav);
@@ -700,44 +952,53 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
                                  std::to_string(total_enc) + ", block holds " +
                                  std::to_string(n_enc) + ")");
   sym_off = DBuf<uint64_t>(ctx.pool(), n_enc + 1);
-  plan.sym_sum = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
-  {
-    DBuf<uint32_t> sz(ctx.pool(), std::max<uint64_t>(n_enc, 1));
-    if (n_enc) {
-      hipLaunchKernelGGL(k_sym_plan, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0,
-                         ctx.stream(), d_enc, n_enc, rb.size.get(), rb.sum.get(), rb.n_rules,
-                         rb.min_id, sz.get(), plan.sym_sum.get());
+  // Encoded start of every traversal, then n_enc: the traversal starts for the
+  // planning kernels and the gather positions of the decoded offsets. Staged in
+  // pinned memory (an asynchronous copy).
+  uint64_t *h_idx = static_cast<uint64_t *>(ctx.pinned((n_trav + 1) * 8));
+  std::copy(enc_start.begin(), enc_start.end(), h_idx);
+  h_idx[n_trav] = n_enc;
+  DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
+  GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx, (n_trav + 1) * 8, hipMemcpyHostToDevice,
+                         ctx.stream()));
+  // Records only when the expansion goes through rule bodies and a decode follows.
+  const int rec = (delta_round >= 0 && rb.has_bodies) ? (delta_round > 0 ? 1 : 0) : -1;
+  if (rec >= 0)
+    plan.syms = DBuf<SymInfo>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+  if (n_enc) {
+    const uint64_t n_tiles = ceil_div(n_enc, kPlanTile);
+    DBuf<PlanAgg> tiles(ctx.pool(), n_tiles);
+    ctx.timed("decode_plan", [&] {
+      hipLaunchKernelGGL(k_plan_reduce, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, n_enc, idx.get(), n_trav + 1, rb.size.get(), rb.sum.get(),
+                         rb.n_rules, rb.min_id, tiles.get());
       GFZ_CHECK_LAUNCH();
-    }
-    exclusive_scan(ctx, sz.get(), sym_off.get(), n_enc);
+      hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(kPlanScanThreads), 0, ctx.stream(),
+                         tiles.get(), n_tiles, sym_off.get() + n_enc);
+      GFZ_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_plan_apply, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, n_enc, idx.get(), n_trav + 1, rb.size.get(), rb.sum.get(),
+                         rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(),
+                         sym_off.get(), plan.syms.get());
+      GFZ_CHECK_LAUNCH();
+    });
+  } else {
+    fill_u64(ctx, sym_off.get(), 0, 1);
   }
   // Decoded offset of each traversal = symbol offset at its first encoded symbol.
   plan.out_offsets.assign(n_trav + 1, 0);
-  {
-    // Indices staged in pinned memory (an asynchronous copy); the gathered
-    // offsets come back through the mailbox when they fit.
-    uint64_t *h_idx = static_cast<uint64_t *>(ctx.pinned((n_trav + 1) * 8));
-    std::copy(enc_start.begin(), enc_start.end(), h_idx);
-    h_idx[n_trav] = n_enc;
-    DBuf<uint64_t> idx(ctx.pool(), n_trav + 1), val(ctx.pool(), n_trav + 1);
-    GFZ_HIP(hipMemcpyAsync(idx.get(), h_idx, (n_trav + 1) * 8, hipMemcpyHostToDevice,
-                           ctx.stream()));
-    hipLaunchKernelGGL(k_gather_u64, dim3(stride_grid(ctx, n_trav + 1)), dim3(kBlock), 0,
-                       ctx.stream(), sym_off.get(), idx.get(), n_trav + 1, val.get());
-    GFZ_CHECK_LAUNCH();
-    if (n_trav + 1 <= 512) {
-      const uint64_t *m = ctx.fetch({{val.get(), (uint32_t)(n_trav + 1), 0}});
-      std::copy(m, m + n_trav + 1, plan.out_offsets.begin());
-    } else {
-      GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
-                             hipMemcpyDeviceToHost, ctx.stream()));
-      ctx.sync();
-    }
+  hipLaunchKernelGGL(k_gather_u64, dim3(stride_grid(ctx, n_trav + 1)), dim3(kBlock), 0,
+                     ctx.stream(), sym_off.get(), idx.get(), n_trav + 1, val.get());
+  GFZ_CHECK_LAUNCH();
+  if (n_trav + 1 <= 512) {
+    const uint64_t *m = ctx.fetch({{val.get(), (uint32_t)(n_trav + 1), 0}});
+    std::copy(m, m + n_trav + 1, plan.out_offsets.begin());
+  } else {
+    GFZ_HIP(hipMemcpyAsync(plan.out_offsets.data(), val.get(), (n_trav + 1) * 8,
+                           hipMemcpyDeviceToHost, ctx.stream()));
+    ctx.sync();
   }
   plan.n_out = plan.out_offsets[n_trav];
-  for (uint64_t t = 0; t < n_trav; ++t)
-    if (enc_lengths[t])
-      plan.enc_starts.push_back(enc_start[t]);
   if (orig_lengths)
     for (uint64_t t = 0; t < n_trav; ++t)
       if (plan.out_offsets[t + 1] - plan.out_offsets[t] != orig_lengths[t])
@@ -753,30 +1014,14 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
   const bool delta = delta_round > 0;
   int scans_left = delta_round;
   if (rb.has_bodies) {
-    // Symbol records; with an inverse delta to undo, the first one is folded in
-    // through the traversal-level sums of the encoded symbols.
-    DBuf<SymInfo> syms(ctx.pool(), n_enc);
-    DBuf<int32_t> incl(ctx.pool(), std::max<uint64_t>(n_enc, 1));
-    if (delta) {
-      const uint64_t w = ceil_div(n_enc, 32);
-      DBuf<uint32_t> eheads(ctx.pool(), w);
-      fill_u32(ctx, eheads.get(), 0, w);
-      DBuf<uint64_t> d_st(ctx.pool(), std::max<size_t>(plan.enc_starts.size(), 1));
-      if (!plan.enc_starts.empty()) {
-        GFZ_HIP(hipMemcpyAsync(d_st.get(), plan.enc_starts.data(), plan.enc_starts.size() * 8,
-                               hipMemcpyHostToDevice, ctx.stream()));
-        set_head_bits(ctx, d_st.get(), plan.enc_starts.size(), eheads.get(), n_enc);
-      }
-      GFZ_HIP(hipMemcpyAsync(incl.get(), plan.sym_sum.get(), n_enc * 4, hipMemcpyDeviceToDevice,
-                             ctx.stream()));
-      segmented_prefix_sum(ctx, incl.get(), n_enc, eheads.get());
+    // Symbol records from the plan (with an inverse delta to undo, the first one
+    // is folded in through the traversal-level sums of the encoded symbols).
+    if (!plan.syms.get())
+      throw Error(kErrInvalid, "decode plan was made without symbol records");
+    const SymInfo *syms = plan.syms.get();
+    if (delta)
       scans_left -= 1;
-    }
-    hipLaunchKernelGGL(k_sym_info, dim3(stride_grid(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                       d_enc, n_enc, incl.get(), plan.sym_sum.get(), delta, rb.size.get(),
-                       rb.sum.get(), rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, syms.get());
-    GFZ_CHECK_LAUNCH();
-    expand_through_z(ctx, "expand", !delta, syms.get(), d_sym_off, n_enc, 0, n_out, rb.z.get(),
+    expand_through_z(ctx, "expand", !delta, syms, d_sym_off, n_enc, 0, n_out, rb.z.get(),
                      rb.z_raw, d_out);
     ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
   } else {

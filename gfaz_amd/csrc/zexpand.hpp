@@ -20,10 +20,6 @@ __device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
 // points at the two reserved zero entries at the end of Z, so every symbol
 // takes the same path. With kDiff (no inverse delta wanted) the value is
 // Z[j] - Z[j -/+ 1] instead, a body element.
-struct alignas(8) SymInfo {
-  uint32_t base;
-  uint32_t zz;
-};
 constexpr uint32_t kRevBit = 0x80000000u;
 
 __device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint32_t *rsize,
