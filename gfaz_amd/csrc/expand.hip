@@ -180,8 +180,8 @@ __global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t
 // offset (exclusive sum of sizes) and record (SymInfo: the traversal's sum of
 // earlier expansions, i.e. a scan of the sums restarted at traversal starts,
 // plus the Z position). No block waits for another.
-constexpr uint32_t kPlanItems = 16;                  // consecutive symbols per thread
-constexpr uint32_t kPlanTile = kBlock * kPlanItems;  // 4096
+constexpr uint32_t kPlanItems = 8;                  // consecutive symbols per thread
+constexpr uint32_t kPlanTile = kBlock * kPlanItems;  // 2048
 constexpr uint32_t kPlanScanThreads = 1024;
 
 // Aggregate of a run of symbols: total size; whether a traversal starts in it;
@@ -259,13 +259,11 @@ __device__ inline void plan_heads(const uint64_t *__restrict__ starts, uint64_t 
                                   uint64_t s0, uint64_t s1, uint32_t *hbits, uint64_t *range) {
   for (uint32_t w = threadIdx.x; w < kPlanTile / 32; w += kBlock)
     hbits[w] = 0;
-  if (threadIdx.x < kWave) {
-    const uint64_t lo = wave_count_below(starts, n_starts, s0);
-    const uint64_t hi = wave_count_below(starts, n_starts, s1);
-    if (threadIdx.x == 0) {
-      range[0] = lo;
-      range[1] = hi;
-    }
+  if (threadIdx.x < 2 * kWave) {  // wave 0 finds the first start, wave 1 the end
+    const uint32_t w = threadIdx.x / kWave;
+    const uint64_t c = wave_count_below(starts, n_starts, w ? s1 : s0);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+      range[w] = c;
   }
   __syncthreads();
   for (uint64_t t = range[0] + threadIdx.x; t < range[1]; t += kBlock) {
@@ -288,8 +286,10 @@ struct PlanSyms {
 __device__ inline uint64_t plan_chunk(uint64_t s0) {
   return s0 + (uint64_t)(threadIdx.x / kWave) * (kPlanItems * kWave);
 }
+// Symbols and their rules' sizes and sums; issued before the tile's traversal
+// starts are looked up, so the two latencies overlap.
 __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc, uint64_t s0,
-                                 const uint32_t *hbits, const uint32_t *__restrict__ rsize,
+                                 const uint32_t *__restrict__ rsize,
                                  const uint32_t *__restrict__ rsum, uint64_t R, uint32_t min_id,
                                  PlanSyms &p) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -300,23 +300,29 @@ __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc
     const uint64_t i = c0 + (uint64_t)j * kWave + lane;
     p.v[j] = enc[i < n_enc ? i : n_enc - 1];
     p.valid |= i < n_enc ? 1u << j : 0u;
-    const uint32_t q = (uint32_t)(i - s0);
-    p.head |= ((hbits[q >> 5] >> (q & 31)) & 1) << j;
   }
-  uint32_t rz[kPlanItems], rs[kPlanItems];
 #pragma unroll
   for (uint32_t j = 0; j < kPlanItems; ++j) {
     const bool rule = is_rule(p.v[j], min_id, R);
     const uint32_t r = rule ? uabs(p.v[j]) - min_id : 0u;
-    rz[j] = rule ? rsize[r] : 1u;
-    rs[j] = rule ? rsum[r] : 0u;
+    p.sz[j] = rule ? rsize[r] : 1u;
+    p.ss[j] = rule ? rsum[r] : 0u;
   }
+}
+// Then the start flags (from the LDS bitmap) and the sizes and signed sums.
+__device__ inline void plan_finish(uint64_t s0, const uint32_t *hbits, uint64_t R,
+                                   uint32_t min_id, PlanSyms &p) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t c0 = plan_chunk(s0);
 #pragma unroll
   for (uint32_t j = 0; j < kPlanItems; ++j) {
-    const bool rule = is_rule(p.v[j], min_id, R);
+    const uint32_t q = (uint32_t)(c0 + (uint64_t)j * kWave + lane - s0);
+    p.head |= ((hbits[q >> 5] >> (q & 31)) & 1) << j;
     const bool ok = (p.valid >> j) & 1;
-    p.sz[j] = ok ? rz[j] : 0u;
-    p.ss[j] = ok ? (rule ? (p.v[j] > 0 ? rs[j] : 0u - rs[j]) : (uint32_t)p.v[j]) : 0u;
+    const uint32_t rs = p.ss[j];
+    p.ss[j] = !ok ? 0u
+                  : is_rule(p.v[j], min_id, R) ? (p.v[j] > 0 ? rs : 0u - rs) : (uint32_t)p.v[j];
+    p.sz[j] = ok ? p.sz[j] : 0u;
   }
   p.head &= p.valid;
 }
@@ -373,9 +379,10 @@ __global__ __launch_bounds__(kBlock) void k_plan_reduce(
   __shared__ PlanAgg lds[kBlock / kWave];
   const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
-  plan_heads(starts, n_starts, s0, s1, hbits, range);
   PlanSyms p;
-  plan_syms(enc, n_enc, s0, hbits, rsize, rsum, R, min_id, p);
+  plan_syms(enc, n_enc, s0, rsize, rsum, R, min_id, p);
+  plan_heads(starts, n_starts, s0, s1, hbits, range);
+  plan_finish(s0, hbits, R, min_id, p);
   PlanAgg total;
   plan_waves_prefix(plan_wave_total(p), lds, total);
   if (threadIdx.x == 0)
@@ -419,9 +426,8 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
-  plan_heads(starts, n_starts, s0, s1, hbits, range);
   PlanSyms p;
-  plan_syms(enc, n_enc, s0, hbits, rsize, rsum, R, min_id, p);
+  plan_syms(enc, n_enc, s0, rsize, rsum, R, min_id, p);
   // Z positions of the rule symbols, loaded before any is used.
   uint64_t zo[kPlanItems];
   if (delta >= 0) {
@@ -431,6 +437,8 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
       zo[j] = rule ? zoff[uabs(p.v[j]) - min_id] : 0u;
     }
   }
+  plan_heads(starts, n_starts, s0, s1, hbits, range);
+  plan_finish(s0, hbits, R, min_id, p);
   PlanAgg total;
   PlanAgg carry =
       plan_combine(tile_prefix[blockIdx.x], plan_waves_prefix(plan_wave_total(p), lds, total));
