@@ -180,7 +180,7 @@ __global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t
 // offset (exclusive sum of sizes) and record (SymInfo: the traversal's sum of
 // earlier expansions, i.e. a scan of the sums restarted at traversal starts,
 // plus the Z position). No block waits for another.
-constexpr uint32_t kPlanItems = 8;                  // consecutive symbols per thread
+constexpr uint32_t kPlanItems = 8;                  // symbols per lane, 64 apart
 constexpr uint32_t kPlanTile = kBlock * kPlanItems;  // 2048
 constexpr uint32_t kPlanScanThreads = 1024;
 
