@@ -478,34 +478,6 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
   }
 }
 
-// Z-building stream of rules [r0, r1): three symbols per rule - the leading
-// zero, the first child, the second child shifted by the first child's sum.
-// off[] gets the Z position of each symbol and, after the last, the level's end.
-__global__ void k_level_stream(const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
-                               const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff,
-                               uint64_t r0, uint64_t r1, uint32_t min_id, uint64_t R,
-                               uint32_t zraw, SymInfo *__restrict__ syms,
-                               uint64_t *__restrict__ off) {
-  for (uint64_t r = r0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < r1;
-       r += (uint64_t)gridDim.x * blockDim.x) {
-    const RuleRec c = rec[r];
-    const uint64_t k = 3 * (r - r0);
-    const uint64_t z0 = zoff[r];
-    const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
-    syms[k] = SymInfo{0u, zraw};
-    off[k] = z0;
-    const uint32_t suma = child_sum(c.a, ra, rsum, min_id);
-    syms[k + 1] = ra ? SymInfo{(uint32_t)c.a, zraw}
-                     : rule_sym(c.a, 0u, true, rsize, rsum, zoff, min_id);
-    off[k + 1] = z0 + 1;
-    syms[k + 2] = rb ? SymInfo{suma + (uint32_t)c.b, zraw}
-                     : rule_sym(c.b, suma, true, rsize, rsum, zoff, min_id);
-    off[k + 2] = z0 + 1 + c.sa;
-    if (r == r1 - 1)
-      off[k + 3] = zoff[r + 1];
-  }
-}
-
 // ---- tiles ------------------------------------------------------------------------
 // Per output tile: the symbols covering its first and last positions and where
 // the first one starts (relative to base). Symbols are disjoint, non-empty
@@ -552,6 +524,83 @@ __global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__
   const TileDesc td = tdesc[blockIdx.x];
   expand_tile<kItems, kDiff>(sm, syms, sym_off, td.k0, (uint32_t)(td.k_last - td.k0 + 1),
                              base + o0, n_here, Z, zraw, out + base + o0);
+}
+
+// Largest k in [lo, hi) with a[k] <= v, given a non-decreasing and a[lo] <= v.
+// All 64 lanes of one wave call it; every lane gets the result.
+__device__ inline uint64_t wave_last_le_u64(const uint64_t *__restrict__ a, uint64_t lo,
+                                            uint64_t hi, uint64_t v) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + kWave - 1) / kWave;
+    const uint64_t k = lo + (uint64_t)lane * step;
+    const uint64_t m = __ballot(k < hi && a[k] <= v);  // lane 0 always holds
+    const uint32_t top = 63u - (uint32_t)__clzll((long long)m);
+    lo = lo + (uint64_t)top * step;
+    hi = hi < lo + step ? hi : lo + step;
+  }
+  return lo;
+}
+
+// Z bodies of one level, built in place: tile b covers Z[lo + b * kTile, ...),
+// lo = zoff[r0]. The level's rules [r0, r1) have increasing zoff, and a rule's
+// body is three symbols: the leading zero at zoff[r], child a at zoff[r] + 1,
+// child b shifted by a's sum at zoff[r] + 1 + |a|. Each block finds the rules
+// overlapping its tile (wave-wide searches of zoff) and stages their symbols
+// straight into LDS; children are at shallower, already-built levels.
+template <uint32_t kItems>
+__global__ __launch_bounds__(kBlock) void k_expand_level(
+    const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
+    const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff, uint64_t r0,
+    uint64_t r1, uint32_t min_id, uint64_t R, uint64_t lo, uint64_t n_out, int32_t *Z,
+    uint32_t zraw) {
+  constexpr uint32_t kTile = kBlock * kItems;
+  __shared__ ZTileSmem<kItems> sm;
+  __shared__ uint64_t s_rf, s_rl;
+  __shared__ uint32_t s_q0, s_n;
+  const uint64_t o0 = (uint64_t)blockIdx.x * kTile;
+  const uint32_t n_here = (uint32_t)(o0 + kTile < n_out ? kTile : n_out - o0);
+  const uint64_t origin = lo + o0, last = origin + n_here - 1;
+  ztile_begin(sm);
+  if (threadIdx.x < kWave) {
+    const uint64_t rf = wave_last_le_u64(zoff, r0, r1, origin);
+    const uint64_t rl = wave_last_le_u64(zoff, rf, r1, last);
+    if (threadIdx.x == 0) {
+      // The first symbol is the last of rule rf's three that starts at or before
+      // origin; the last symbol likewise for rule rl and the tile's last position.
+      const uint64_t zf = zoff[rf], zl = zoff[rl];
+      const uint32_t saf = rec[rf].sa, sal = rec[rl].sa;
+      const uint32_t q0 = origin >= zf + 1 + saf ? 2u : (origin >= zf + 1 ? 1u : 0u);
+      const uint32_t q1 = last >= zl + 1 + sal ? 2u : (last >= zl + 1 ? 1u : 0u);
+      s_rf = rf;
+      s_rl = rl;
+      s_q0 = q0;
+      s_n = (uint32_t)(3 * (rl - rf) + q1 - q0 + 1);
+    }
+  }
+  __syncthreads();
+  const uint64_t rf = s_rf, rl = s_rl;
+  const uint32_t q0 = s_q0, nsym = s_n;
+  for (uint64_t r = rf + threadIdx.x; r <= rl; r += kBlock) {
+    const RuleRec c = rec[r];
+    const uint64_t z0 = zoff[r];
+    const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
+    const uint32_t suma = child_sum(c.a, ra, rsum, min_id);
+    const SymInfo sa = ra ? SymInfo{(uint32_t)c.a, zraw}
+                          : rule_sym(c.a, 0u, true, rsize, rsum, zoff, min_id);
+    const SymInfo sb = rb ? SymInfo{suma + (uint32_t)c.b, zraw}
+                          : rule_sym(c.b, suma, true, rsize, rsum, zoff, min_id);
+    const uint64_t g0 = 3 * (r - rf);  // symbol q of rule r is g0 + q - q0
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+      if (g0 + q < q0 || g0 + q - q0 >= nsym)
+        continue;
+      const uint64_t st = q == 0 ? z0 : (q == 1 ? z0 + 1 : z0 + 1 + c.sa);
+      const SymInfo si = q == 0 ? SymInfo{0u, zraw} : (q == 1 ? sa : sb);
+      ztile_stage(sm, (uint32_t)(g0 + q - q0), (uint32_t)(st - origin), si);
+    }
+  }
+  ztile_finish<kItems, false>(sm, n_here, Z, zraw, Z + origin);
 }
 
 // Expansion by descending the rule tree (no Z): node IDs, no inverse delta.
@@ -797,6 +846,53 @@ void expand_through_z(Ctx &ctx, const char *group, bool diff, const SymInfo *sym
 
 }  // namespace
 
+template <uint32_t kItems>
+void launch_level(Ctx &ctx, const Rulebook &rb, uint64_t r0, uint64_t r1, uint32_t min_id,
+                  uint64_t R, uint64_t lo, uint64_t n_out) {
+  const uint64_t n_tiles = ceil_div(n_out, (uint64_t)kBlock * kItems);
+  if (n_tiles > UINT32_MAX)
+    throw Error(kErrInvalid, "rule bodies of one level too large for one launch");
+  hipLaunchKernelGGL(k_expand_level<kItems>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
+                     ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), rb.zoff.get(), r0,
+                     r1, min_id, R, lo, n_out, rb.z.get(), rb.z_raw);
+  GFZ_CHECK_LAUNCH();
+}
+
+// Z[lo, lo + n_out) for the rules [r0, r1) of one level, with the tile size
+// autotuned per size class like the decode expansion (every candidate writes
+// the same bodies).
+void build_level(Ctx &ctx, const Rulebook &rb, uint64_t r0, uint64_t r1, uint32_t min_id,
+                 uint64_t R, uint64_t lo, uint64_t n_out) {
+  if (!n_out)
+    return;
+  auto go = [&](uint32_t items) {
+    switch (items) {
+      case 8: launch_level<8>(ctx, rb, r0, r1, min_id, R, lo, n_out); break;
+      case 32: launch_level<32>(ctx, rb, r0, r1, min_id, R, lo, n_out); break;
+      default: launch_level<16>(ctx, rb, r0, r1, min_id, R, lo, n_out); break;
+    }
+  };
+  uint32_t cls = 0;
+  for (uint64_t n = n_out; n >= 4; n >>= 2)
+    ++cls;
+  const std::string key = std::string("level_items/") + std::to_string(cls);
+  int items = ctx.tuned(key);
+  if (items < 0 && ctx.autotune() && n_out >= (uint64_t(1) << 20)) {
+    float best = 0;
+    for (uint32_t c : kZItems) {
+      ctx.time_ms([&] { go(c); });  // warm
+      const float ms = ctx.time_ms([&] { go(c); });
+      if (items < 0 || ms < best) {
+        best = ms;
+        items = (int)c;
+      }
+    }
+    ctx.set_tuned(key, items);
+  }
+  const uint32_t use = items < 0 ? kExItems : (uint32_t)items;
+  ctx.timed("build_bodies", [&] { go(use); });
+}
+
 void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, uint64_t R,
                     uint32_t min_id, Rulebook &rb) {
   rb = Rulebook();
@@ -927,17 +1023,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     if (r0 >= r1)
       continue;
     const uint64_t lo = lvl_off[d], hi = lvl_off[d + 1];
-    const uint64_t n_sym = 3 * (r1 - r0);
-    DBuf<SymInfo> syms(ctx.pool(), n_sym);
-    DBuf<uint64_t> off(ctx.pool(), n_sym + 1);
-    ctx.timed("build_bodies", [&] {
-      hipLaunchKernelGGL(k_level_stream, dim3(stride_grid(ctx, r1 - r0)), dim3(kBlock), 0,
-                         ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), rb.zoff.get(),
-                         r0, r1, min_id, R, rb.z_raw, syms.get(), off.get());
-    });
-    GFZ_CHECK_LAUNCH();
-    expand_through_z(ctx, "build_bodies", false, syms.get(), off.get(), n_sym, lo, hi - lo,
-                     rb.z.get(), rb.z_raw, rb.z.get());
+    build_level(ctx, rb, r0, r1, min_id, R, lo, hi - lo);
   }
   // No synchronisation: the bodies are built on the context stream, which every
   // use of the rulebook runs on, and the rule columns were last read before the

@@ -95,28 +95,33 @@ struct ZTileSmem {
 // symbol's base and the Z index of tile position 0 are staged in LDS; then every
 // thread issues all of its Z gathers before its stores. Block-wide: every thread
 // of the block must call it; it ends with no barrier pending.
-template <uint32_t kItems, bool kDiff>
-__device__ inline void expand_tile(ZTileSmem<kItems> &sm, const SymInfo *__restrict__ syms,
-                                   const uint64_t *__restrict__ sym_off, uint64_t k0,
-                                   uint32_t nsym, uint64_t origin, uint32_t n_here,
-                                   const int32_t *Z, uint32_t zraw, int32_t *out_tile) {
+// The three steps of a tile: clear, stage every covering symbol (index s from
+// 0, its start rel relative to the tile's first position: the first symbol may
+// start before it), then expand. Block-wide except ztile_stage.
+template <uint32_t kItems>
+__device__ inline void ztile_begin(ZTileSmem<kItems> &sm) {
   constexpr uint32_t kTile = kBlock * kItems;
   sm.map.clear();
   for (uint32_t w = threadIdx.x; w < kTile / 32; w += kBlock)
     sm.srev[w] = 0;
   __syncthreads();
-  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock) {
-    // Start of symbol s relative to the tile (<= 0 for the first one).
-    const uint32_t rel = (uint32_t)(sym_off[k0 + s] - origin);
-    const SymInfo si = syms[k0 + s];
-    const bool rev = si.zz & kRevBit;
-    const uint32_t zi = si.zz & ~kRevBit;
-    sm.sbase[s] = si.base;
-    sm.sstart[s] = rev ? zi + rel : zi - rel;
-    if (rev)
-      atomicOr(&sm.srev[s >> 5], 1u << (s & 31));
-    sm.map.mark(s == 0 ? 0u : rel);
-  }
+}
+template <uint32_t kItems>
+__device__ inline void ztile_stage(ZTileSmem<kItems> &sm, uint32_t s, uint32_t rel,
+                                   const SymInfo &si) {
+  const bool rev = si.zz & kRevBit;
+  const uint32_t zi = si.zz & ~kRevBit;
+  sm.sbase[s] = si.base;
+  sm.sstart[s] = rev ? zi + rel : zi - rel;
+  if (rev)
+    atomicOr(&sm.srev[s >> 5], 1u << (s & 31));
+  sm.map.mark(s == 0 ? 0u : rel);
+}
+// Each thread issues all of its Z gathers before its stores; ends with no
+// barrier pending (LDS free for the caller's next tile).
+template <uint32_t kItems, bool kDiff>
+__device__ inline void ztile_finish(ZTileSmem<kItems> &sm, uint32_t n_here, const int32_t *Z,
+                                    uint32_t zraw, int32_t *out_tile) {
   __syncthreads();
   sm.map.rank_words();
   uint32_t jz[kItems], vb[kItems];
@@ -145,7 +150,23 @@ __device__ inline void expand_tile(ZTileSmem<kItems> &sm, const SymInfo *__restr
     if (p < n_here)
       out_tile[p] = (int32_t)(vb[j] + z[j]);
   }
-  __syncthreads();  // LDS is reused by the caller's next tile
+  __syncthreads();
+}
+
+// Expand the n_here (<= tile) output positions starting at decoded position
+// `origin` into out_tile[0..n_here): symbols k0 .. k0 + nsym - 1 cover them,
+// symbol k starting at sym_off[k] (the first may start before origin). Each
+// symbol's base and the Z index of tile position 0 are staged in LDS. Block-wide:
+// every thread of the block must call it.
+template <uint32_t kItems, bool kDiff>
+__device__ inline void expand_tile(ZTileSmem<kItems> &sm, const SymInfo *__restrict__ syms,
+                                   const uint64_t *__restrict__ sym_off, uint64_t k0,
+                                   uint32_t nsym, uint64_t origin, uint32_t n_here,
+                                   const int32_t *Z, uint32_t zraw, int32_t *out_tile) {
+  ztile_begin(sm);
+  for (uint32_t s = threadIdx.x; s < nsym; s += kBlock)
+    ztile_stage(sm, s, (uint32_t)(sym_off[k0 + s] - origin), syms[k0 + s]);
+  ztile_finish<kItems, kDiff>(sm, n_here, Z, zraw, out_tile);
 }
 
 }  // namespace zx
