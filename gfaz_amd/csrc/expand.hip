@@ -327,6 +327,37 @@ __device__ inline void plan_finish(uint64_t s0, const uint32_t *hbits, uint64_t 
   p.head &= p.valid;
 }
 
+// The apply pass's symbols: values, and sizes and signed sums as the reduce pass
+// stored them (no rule lookups here).
+__device__ inline void plan_syms_stored(const int32_t *__restrict__ enc, uint64_t n_enc,
+                                        uint64_t s0, const uint2 *__restrict__ szss,
+                                        PlanSyms &p) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t c0 = plan_chunk(s0);
+  p.head = p.valid = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const uint64_t i = c0 + (uint64_t)j * kWave + lane;
+    const uint64_t ic = i < n_enc ? i : n_enc - 1;
+    p.v[j] = enc[ic];
+    const uint2 w = szss[ic];
+    const bool ok = i < n_enc;
+    p.valid |= ok ? 1u << j : 0u;
+    p.sz[j] = ok ? w.x : 0u;
+    p.ss[j] = ok ? w.y : 0u;
+  }
+}
+__device__ inline void plan_heads_only(uint64_t s0, const uint32_t *hbits, PlanSyms &p) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t c0 = plan_chunk(s0);
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanItems; ++j) {
+    const uint32_t q = (uint32_t)(c0 + (uint64_t)j * kWave + lane - s0);
+    p.head |= ((hbits[q >> 5] >> (q & 31)) & 1) << j;
+  }
+  p.head &= p.valid;
+}
+
 // Inclusive ordered scan of one PlanAgg per lane across a wave.
 __device__ inline PlanAgg plan_wave_incl(PlanAgg x) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -373,7 +404,7 @@ __device__ inline PlanAgg plan_waves_prefix(const PlanAgg &wave_total, PlanAgg *
 __global__ __launch_bounds__(kBlock) void k_plan_reduce(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
     uint64_t n_starts, const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
-    uint64_t R, uint32_t min_id, PlanAgg *__restrict__ tiles) {
+    uint64_t R, uint32_t min_id, PlanAgg *__restrict__ tiles, uint2 *__restrict__ szss) {
   __shared__ uint32_t hbits[kPlanTile / 32];
   __shared__ uint64_t range[2];
   __shared__ PlanAgg lds[kBlock / kWave];
@@ -383,6 +414,14 @@ __global__ __launch_bounds__(kBlock) void k_plan_reduce(
   plan_syms(enc, n_enc, s0, rsize, rsum, R, min_id, p);
   plan_heads(starts, n_starts, s0, s1, hbits, range);
   plan_finish(s0, hbits, R, min_id, p);
+  {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t c0 = plan_chunk(s0);
+#pragma unroll
+    for (uint32_t j = 0; j < kPlanItems; ++j)
+      if ((p.valid >> j) & 1)
+        szss[c0 + (uint64_t)j * kWave + lane] = make_uint2(p.sz[j], p.ss[j]);
+  }
   PlanAgg total;
   plan_waves_prefix(plan_wave_total(p), lds, total);
   if (threadIdx.x == 0)
@@ -416,8 +455,8 @@ __global__ __launch_bounds__(kPlanScanThreads) void k_plan_scan(PlanAgg *__restr
 // inverse delta folded in).
 __global__ __launch_bounds__(kBlock) void k_plan_apply(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
-    uint64_t n_starts, const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
-    const uint64_t *__restrict__ zoff, uint64_t R, uint32_t min_id, uint32_t zraw, int delta,
+    uint64_t n_starts, const uint2 *__restrict__ szss, const uint64_t *__restrict__ zoff,
+    uint64_t R, uint32_t min_id, uint32_t zraw, int delta,
     const PlanAgg *__restrict__ tile_prefix, uint64_t *__restrict__ sym_off,
     SymInfo *__restrict__ syms) {
   __shared__ uint32_t hbits[kPlanTile / 32];
@@ -427,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
   const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
   PlanSyms p;
-  plan_syms(enc, n_enc, s0, rsize, rsum, R, min_id, p);
+  plan_syms_stored(enc, n_enc, s0, szss, p);
   // Z positions of the rule symbols, loaded before any is used.
   uint64_t zo[kPlanItems];
   if (delta >= 0) {
@@ -438,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
     }
   }
   plan_heads(starts, n_starts, s0, s1, hbits, range);
-  plan_finish(s0, hbits, R, min_id, p);
+  plan_heads_only(s0, hbits, p);
   PlanAgg total;
   PlanAgg carry =
       plan_combine(tile_prefix[blockIdx.x], plan_waves_prefix(plan_wave_total(p), lds, total));
@@ -1062,18 +1101,19 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
   if (n_enc) {
     const uint64_t n_tiles = ceil_div(n_enc, kPlanTile);
     DBuf<PlanAgg> tiles(ctx.pool(), n_tiles);
+    DBuf<uint2> szss(ctx.pool(), n_enc);  // each symbol's size and signed sum
     ctx.timed("decode_plan", [&] {
       hipLaunchKernelGGL(k_plan_reduce, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
                          d_enc, n_enc, idx.get(), n_trav + 1, rb.size.get(), rb.sum.get(),
-                         rb.n_rules, rb.min_id, tiles.get());
+                         rb.n_rules, rb.min_id, tiles.get(), szss.get());
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(kPlanScanThreads), 0, ctx.stream(),
                          tiles.get(), n_tiles, sym_off.get() + n_enc);
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_plan_apply, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, idx.get(), n_trav + 1, rb.size.get(), rb.sum.get(),
-                         rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(),
-                         sym_off.get(), plan.syms.get());
+                         d_enc, n_enc, idx.get(), n_trav + 1, szss.get(), rb.zoff.get(),
+                         rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(), sym_off.get(),
+                         plan.syms.get());
       GFZ_CHECK_LAUNCH();
     });
   } else {
