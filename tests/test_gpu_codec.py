@@ -77,6 +77,8 @@ def _random_case(seed, n_trav, max_len, alphabet, zero_frac=0.1):
     (4, 1000, 40, 2, 3, 0),      # many short traversals, empty ones, single steps
     (5, 7, 100000, 4, 8, 1),
     (6, 3, 1, 2, 8, 1),          # nothing to pair
+    (7, 5000, 30, 3, 8, 1),      # thousands of traversal starts: several per decode tile
+    (8, 3000, 60, 2, 8, 2),
 ])
 def test_random_matches_oracle(gpu_ctx, seed, n_trav, max_len, alphabet, rounds, delta):
     say(f"random seed={seed}")
