@@ -139,6 +139,16 @@ def test_full_size_plain_500k_matches_oracle(gpu_ctx):
     check_decompress(gpu_ctx, tr.nodes, tr.lengths, ref, 1)
 
 
+def test_large_walks_match_oracle(gpu_ctx):
+    """W-lines of log-uniform length at 107 M steps (walks_2m with 400 walks):
+    the largest walks case the oracle's 31-bit provisional IDs allow."""
+    cfg = synth.CONFIGS["walks_2m"].scaled(n_haps=400)
+    say("walks 107M")
+    tr = synth.generate(cfg)
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=16)
+    check_compress(gpu_ctx, tr.nodes, tr.lengths, 8, 1, tr.n_paths, ref=ref)
+
+
 def test_device_resident_paths_and_determinism(gpu_ctx):
     say("device resident")
     tr = synth.generate(synth.CONFIGS["plain_20k"])
