@@ -89,12 +89,6 @@ struct ZTileSmem {
   uint32_t srev[kTile / 32];
 };
 
-// Expand the n_here (<= tile) output positions starting at decoded position
-// `origin` into out_tile[0..n_here): symbols k0 .. k0 + nsym - 1 cover them,
-// symbol k starting at sym_off[k] (the first may start before origin). Each
-// symbol's base and the Z index of tile position 0 are staged in LDS; then every
-// thread issues all of its Z gathers before its stores. Block-wide: every thread
-// of the block must call it; it ends with no barrier pending.
 // The three steps of a tile: clear, stage every covering symbol (index s from
 // 0, its start rel relative to the tile's first position: the first symbol may
 // start before it), then expand. Block-wide except ztile_stage.
