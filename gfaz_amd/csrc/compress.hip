@@ -137,12 +137,32 @@ constexpr uint32_t kStageCap = 512;
 // probe chain of single slots at load 1/2 reads 15-20 in the worst of a
 // wave's 512 lookups). The slot index is the key's provisional rule ID within
 // the round; vals[slot] becomes its final ID.
+// The lines form regions of 2^rbits lines, one per counting bucket (nbmask + 1
+// of them; one region when the round was counted in the HBM table): a key's
+// first line is in its bucket's region, and its probe wraps within it, so one
+// block can build a bucket's region alone.
 constexpr uint32_t kLine = 8;
 struct RuleTable {
   unsigned long long *keys;  // kFree = empty; 64-byte aligned
   uint32_t *vals;
-  uint64_t lmask;            // lines - 1 (a power of two minus one)
+  uint32_t rbits;            // log2 of the lines per region
+  uint32_t nbmask;           // regions - 1 (a power of two minus one)
 };
+
+// Bucket of a key among mask + 1 (hash bits above those that pick its line).
+__device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
+  return (uint32_t)(key_hash(k) >> 40) & mask;
+}
+
+__device__ inline uint64_t rule_line(const RuleTable &rt, uint64_t k) {
+  const uint64_t rmask = (1ull << rt.rbits) - 1;
+  return ((uint64_t)bucket_of(k, rt.nbmask) << rt.rbits) | (key_hash(k) & rmask);
+}
+
+__device__ inline uint64_t rule_next_line(const RuleTable &rt, uint64_t l) {
+  const uint64_t rmask = (1ull << rt.rbits) - 1;
+  return (l & ~rmask) | ((l + 1) & rmask);
+}
 
 __device__ inline void load_line(const unsigned long long *keys, uint64_t line,
                                  uint64_t (&w)[kLine]) {
@@ -165,12 +185,12 @@ __device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    l[j] = key_hash(k[j]) & rt.lmask;
+    l[j] = rule_line(rt, k[j]);
     slot[j] = -1;
     if (valid[j])
       pending |= 1u << j;
   }
-  for (uint64_t p = 0; pending && p <= rt.lmask; ++p) {
+  for (uint64_t p = 0; pending && p < (1ull << rt.rbits); ++p) {
     uint64_t w[N][kLine];
 #pragma unroll
     for (int j = 0; j < N; ++j)
@@ -193,7 +213,7 @@ __device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[
       } else if (has_free) {
         pending &= ~(1u << j);
       } else {
-        l[j] = (l[j] + 1) & rt.lmask;
+        l[j] = rule_next_line(rt, l[j]);
       }
     }
   }
@@ -214,7 +234,7 @@ __device__ inline int64_t rule_find(const RuleTable &rt, uint64_t k) {
 // tried), never the reverse, so starting at the first slot seen free keeps
 // every key in the first line that had room.
 __device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
-  uint64_t l = key_hash(k) & rt.lmask;
+  uint64_t l = rule_line(rt, k);
   for (uint32_t p = 0; p < kMaxProbe; ++p) {
     unsigned long long *line = rt.keys + l * kLine;
     uint64_t w[kLine];
@@ -228,7 +248,7 @@ __device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
       if (old == kFree || old == k)
         return true;
     }
-    l = (l + 1) & rt.lmask;
+    l = rule_next_line(rt, l);
   }
   return false;
 }
@@ -631,9 +651,6 @@ struct Parts {
   uint32_t count;               // partition blocks
 };
 
-__device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
-  return (uint32_t)(key_hash(k) >> 40) & mask;  // independent of the low probe bits
-}
 
 // Exclusive scan of the block's bucket histogram in place; writes the block's
 // row of starts (nb + 1 words).
@@ -831,6 +848,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
     block_counts[b] = make_uint2(total_d, total_r);
     atomicAdd(&sums[0], (unsigned long long)total_d);
     atomicAdd(&sums[1], (unsigned long long)total_r);
+    atomicMax(&sums[4], (unsigned long long)total_r);  // sizes the rule-table regions
   }
 }
 
@@ -880,6 +898,40 @@ __global__ __launch_bounds__(kBlock) void k_extract_staged(
 
 // From the counting table itself (when some block staged too many keys).
 constexpr uint32_t kExtractItems = 16;
+
+// Bucket-counted rounds: bucket b's region of the rule table built by one block
+// in LDS (2^rt.rbits lines, at most kBktSlots slots) from the bucket's list,
+// then written out whole, free slots included. A key takes the first line of
+// its probe within the region that has a free slot, as rule_insert would place
+// it; the host sizes regions to at least twice the longest list, so every key
+// finds one.
+__global__ __launch_bounds__(kBktThreads) void k_region_build(
+    const unsigned long long *__restrict__ stage, uint64_t stride,
+    const uint2 *__restrict__ block_counts, RuleTable rt, uint32_t *__restrict__ err) {
+  __shared__ unsigned long long reg[kBktSlots];
+  const uint32_t lines = 1u << rt.rbits, slots = lines * kLine;
+  for (uint32_t i = threadIdx.x; i < slots; i += kBktThreads)
+    reg[i] = kFree;
+  __syncthreads();
+  const uint64_t b = blockIdx.x;
+  const uint64_t n = block_counts[b].y < stride ? block_counts[b].y : stride;
+  for (uint64_t j = threadIdx.x; j < n; j += kBktThreads) {
+    const unsigned long long k = stage[b * stride + j];
+    uint32_t l = (uint32_t)key_hash(k) & (lines - 1);
+    bool placed = false;
+    for (uint32_t p = 0; p < lines && !placed; ++p) {
+      for (uint32_t q = 0; q < kLine && !placed; ++q)
+        placed = atomicCAS(&reg[l * kLine + q], (unsigned long long)kFree, k) == kFree;
+      l = (l + 1) & (lines - 1);
+    }
+    if (!placed)  // a region smaller than its list: never sized so by the host
+      atomicExch(err, 3u);
+  }
+  __syncthreads();
+  unsigned long long *dst = rt.keys + (b << rt.rbits) * kLine;
+  for (uint32_t i = threadIdx.x; i < slots; i += kBktThreads)
+    dst[i] = reg[i];
+}
 
 __global__ __launch_bounds__(kBlock) void k_extract_repeated(
     const unsigned long long *__restrict__ count_slots, uint64_t cap, RuleTable rt,
@@ -1569,7 +1621,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     const uint64_t n_ins = ceil_div(L, kInsTile);
     DBuf<uint2> bcounts(pool, std::max<uint64_t>(n_ins, kMaxBuckets));
     DBuf<unsigned long long> stage;
-    DBuf<unsigned long long> sums(pool, 4);  // distinct, repeated, overflow, stage overflow
+    // distinct, repeated, overflow, stage overflow, longest bucket list
+    DBuf<unsigned long long> sums(pool, 5);
     uint64_t distinct = 0, repeated = 0;
     // Once most pairs of the last round were distinct, count in hash buckets
     // (distinct pairs only grow from round to round; earlier rounds hold keys
@@ -1597,6 +1650,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
     bool counted = false;
+    uint64_t regions = 0, region_max = 0;  // bucket-counted: rule table by bucket regions
     // Rounds count in buckets while their distinct keys (the first round's
     // estimated at L/32) fit kMaxBuckets buckets of kBktSlots / 2; larger rounds,
     // and any round whose buckets overflow, count in the HBM table.
@@ -1613,7 +1667,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
       DBuf<uint32_t> starts(pool, n_parts * (nbk + 1));
       stage = DBuf<unsigned long long>(pool, nbk * kRepStride);
-      fill_runs(ctx, {{sums.get(), 4, 0, 8}});
+      fill_runs(ctx, {{sums.get(), 5, 0, 8}});
       const Parts pt{part.get(), (uint64_t)per * kInsTile, starts.get(), (uint32_t)nbk,
                      (uint32_t)n_parts};
       ctx.timed("pair_insert", [&] {
@@ -1634,10 +1688,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         GFZ_CHECK_LAUNCH();
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
-      const uint64_t *hc = ctx.fetch({{sums.get(), 4, 0}});
+      const uint64_t *hc = ctx.fetch({{sums.get(), 5, 0}});
       if (!hc[2]) {
         distinct = hc[0];
         repeated = hc[1];
+        regions = nbk;
+        region_max = hc[4];
         counted = true;
         stage_stride = kRepStride;
         stage_blocks = nbk;
@@ -1688,26 +1744,45 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     if (repeated == 0)
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
-    // K1b: rule table of the repeated keys (load <= 1/4, whole 64-byte lines).
-    const uint64_t rcap = next_pow2(std::max<uint64_t>(1024, 4 * (uint64_t)repeated));
+    // K1b: rule table of the repeated keys (load <= 1/4, whole 64-byte lines);
+    // a bucket-counted round takes one region per bucket holding four times its
+    // longest list (twice, at most, for lists over kBktSlots / 4).
+    const uint64_t region_slots =
+        regions ? std::min<uint64_t>(kBktSlots,
+                                     next_pow2(std::max<uint64_t>(kLine, 4 * region_max)))
+                : 0;
+    const uint64_t rcap =
+        regions ? regions * region_slots
+                : next_pow2(std::max<uint64_t>(1024, 4 * (uint64_t)repeated));
     if ((uint64_t)next + rcap >= (uint64_t)INT32_MAX)
       throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
                                     ", rule table " + std::to_string(rcap) + ")");
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
-    const RuleTable rt{rkeys.get(), rvals.get(), rcap / kLine - 1};
+    const uint64_t region_lines = regions ? region_slots / kLine : rcap / kLine;
+    const RuleTable rt{rkeys.get(), rvals.get(), (uint32_t)__builtin_ctzll(region_lines),
+                       regions ? (uint32_t)(regions - 1) : 0u};
     // Round scalars (read back once, after K4): [0] next length, [1] used keys.
     // One launch clears everything the round needs cleared: the rule table, the
     // scalars, the used flags and the next round's head bits (up to L: the next
     // length is at most L).
     DBuf<unsigned long long> scal(pool, 2);
     DBuf<uint8_t> used(pool, rcap);
-    fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
-                    {scal.get(), 2, 0, 8},
-                    {used.get(), rcap, 0, 1},
-                    {heads_next.get(), ceil_div(L, 32), 0, 4}});
+    if (regions)  // the regions are written whole: the table needs no clearing
+      fill_runs(ctx, {{scal.get(), 2, 0, 8},
+                      {used.get(), rcap, 0, 1},
+                      {heads_next.get(), ceil_div(L, 32), 0, 4}});
+    else
+      fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
+                      {scal.get(), 2, 0, 8},
+                      {used.get(), rcap, 0, 1},
+                      {heads_next.get(), ceil_div(L, 32), 0, 4}});
     ctx.timed("extract_repeated", [&] {
-      if (stage_overflow)  // some block staged too many: scan the whole table
+      if (regions)
+        hipLaunchKernelGGL(k_region_build, dim3((uint32_t)regions), dim3(kBktThreads), 0,
+                           ctx.stream(), stage.get(), stage_stride, bcounts.get(), rt,
+                           d_err.get());
+      else if (stage_overflow)  // some block staged too many: scan the whole table
         hipLaunchKernelGGL(k_extract_repeated,
                            dim3((uint32_t)ceil_div(cap, (uint64_t)kBlock * kExtractItems)),
                            dim3(kBlock), 0, ctx.stream(), ctab.get(), cap, rt, d_err.get());
