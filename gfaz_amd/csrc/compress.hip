@@ -642,6 +642,8 @@ constexpr uint32_t kBktThreads = 1024;
 constexpr uint32_t kMaxBuckets = 4096;
 constexpr uint32_t kMaxParts = 2048;    // partition blocks (segments per bucket)
 constexpr uint32_t kRepStride = kBktSlots / 2;  // repeated keys listed per bucket
+static_assert((uint64_t)kMaxBuckets * kBktSlots < (1ull << 32),
+              "a round's bucket totals fit half of a 64-bit word");
 
 struct Parts {
   unsigned long long *entries;  // block g's region at g * region
@@ -846,9 +848,11 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
     if (total_r > kRepStride)
       atomicExch(overflow, 1u);
     block_counts[b] = make_uint2(total_d, total_r);
-    atomicAdd(&sums[0], (unsigned long long)total_d);
-    atomicAdd(&sums[1], (unsigned long long)total_r);
-    atomicMax(&sums[4], (unsigned long long)total_r);  // sizes the rule-table regions
+    // Distinct keys (high half) and repeated keys (low half) in one add: each
+    // total is below 2^32 (at most kMaxBuckets * kBktSlots keys). The longest
+    // list, on a line of its own, sizes the rule-table regions.
+    atomicAdd(&sums[0], ((unsigned long long)total_d << 32) | total_r);
+    atomicMax(&sums[8], (unsigned long long)total_r);
   }
 }
 
@@ -1621,8 +1625,9 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     const uint64_t n_ins = ceil_div(L, kInsTile);
     DBuf<uint2> bcounts(pool, std::max<uint64_t>(n_ins, kMaxBuckets));
     DBuf<unsigned long long> stage;
-    // distinct, repeated, overflow, stage overflow, longest bucket list
-    DBuf<unsigned long long> sums(pool, 5);
+    // distinct, repeated, overflow, stage overflow; bucket-counted rounds: distinct
+    // and repeated packed in [0], the longest bucket list in [8]
+    DBuf<unsigned long long> sums(pool, 9);
     uint64_t distinct = 0, repeated = 0;
     // Once most pairs of the last round were distinct, count in hash buckets
     // (distinct pairs only grow from round to round; earlier rounds hold keys
@@ -1667,7 +1672,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
       DBuf<uint32_t> starts(pool, n_parts * (nbk + 1));
       stage = DBuf<unsigned long long>(pool, nbk * kRepStride);
-      fill_runs(ctx, {{sums.get(), 5, 0, 8}});
+      fill_runs(ctx, {{sums.get(), 9, 0, 8}});
       const Parts pt{part.get(), (uint64_t)per * kInsTile, starts.get(), (uint32_t)nbk,
                      (uint32_t)n_parts};
       ctx.timed("pair_insert", [&] {
@@ -1688,12 +1693,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         GFZ_CHECK_LAUNCH();
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
-      const uint64_t *hc = ctx.fetch({{sums.get(), 5, 0}});
+      const uint64_t *hc = ctx.fetch({{sums.get(), 9, 0}});
       if (!hc[2]) {
-        distinct = hc[0];
-        repeated = hc[1];
+        distinct = hc[0] >> 32;
+        repeated = hc[0] & 0xffffffffull;
         regions = nbk;
-        region_max = hc[4];
+        region_max = hc[8];
         counted = true;
         stage_stride = kRepStride;
         stage_blocks = nbk;
