@@ -1750,11 +1750,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
     // K1b: rule table of the repeated keys (load <= 1/4, whole 64-byte lines);
-    // a bucket-counted round takes one region per bucket holding four times its
-    // longest list (twice, at most, for lists over kBktSlots / 4).
+    // a bucket-counted round takes one region per bucket holding twice its
+    // longest list (load <= 1/2 in the fullest region, lower on average): a
+    // smaller table to write and scan outweighs the lookups' extra lines.
     const uint64_t region_slots =
         regions ? std::min<uint64_t>(kBktSlots,
-                                     next_pow2(std::max<uint64_t>(kLine, 4 * region_max)))
+                                     next_pow2(std::max<uint64_t>(kLine, 2 * region_max)))
                 : 0;
     const uint64_t rcap =
         regions ? regions * region_slots
