@@ -2,19 +2,28 @@
 """Benchmark of the gfx950 grammar codec on a seeded synthetic pangenome.
 
 One step = grammar-compress the whole batch of traversals (delta + 8 rounds) and
-decompress it again (rule expansion + inverse delta), with the traversals already
-resident in HBM when the timed region starts. ``value`` = input-GFA bytes moved through
-compress and decompress per second: gfa_bytes / (t_compress + t_decompress).
+decompress it again (rulebook load, rule expansion + inverse delta into HBM), with the
+traversals already resident in HBM when the timed region starts. ``value`` = input-GFA
+bytes moved through compress and decompress per second:
+gfa_bytes / (t_compress + t_decompress).
 
-Default workload: ``plain_500k`` - 500,000-site backbone, 64 P-lines, 2-allele bubbles at
-every 10th site, 2% reversed steps, 1% deletions of 1-5 sites, 0.1% tandem duplications of
-50-500 steps (about 39.6M int32 steps, 368 MB of GFA text); rounds=8, threshold=2,
-delta_round=1.
+Default workload: ``plain_10m`` - 10,000,000-site backbone, 256 P-lines, 2-allele bubbles
+at every 10th site, 2% reversed steps, 1% deletions of 1-5 sites, 0.1% tandem duplications
+of 50-500 steps (3,168,123,299 int32 steps, 12.7 GB in HBM, 30.4 GB of GFA text);
+rounds=8, threshold=2, delta_round=1.
 
 Prints ONE JSON line. ``roofline`` is the dominant kernel group of the timed region
-(HIP events on the codec's own stream); ``cpu_baseline`` is the reference CPU path
-compiled from source (oracle/_ref, when present) or the repo's CPU oracle, timed on this
-host's cores on the same workload.
+(HIP events on the codec's own stream); ``extract`` gives single-request latency
+percentiles of resident extraction at a fixed request rate; ``cpu_baseline`` is the
+reference CPU path compiled from source (oracle/_ref, when present; else the repo's CPU
+oracle), timed on this host's cores on a bounded sample of the same workload, end to end
+and at codec scope (the reference's own stage timers), with the GPU timed on the same
+sample and the traversal columns compared byte for byte.
+
+Other modes: ``--mode batching`` (walks_2m: one persistent launch vs one launch per
+traversal), ``--mode stream`` (plain_10m decoded to pinned host memory in chunks),
+``--mode serve`` (walks_2m resident; Poisson extract/add requests through the serving
+loop, p50/p99 latency and GB/s).
 """
 from __future__ import annotations
 
@@ -48,42 +57,173 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
-def cpu_baseline(cfg, tr, gfa_bytes: int, threads: int) -> dict:
-    """Reference CPU path (oracle/_ref/gfaz_ref) on the same workload when it was built;
-    otherwise the repo's CPU oracle (codec stage only)."""
+# Steps above which the CPU baseline runs on a sample of the workload: the same
+# generator over 1/16 of the backbone with every traversal (about 0.2G steps for
+# plain_10m, 10-30 s of reference CPU work), so the default run stays within minutes.
+CPU_SAMPLE_STEPS = 300_000_000
+CPU_SAMPLE_DIV = 16
+
+
+def _ms_after(text: str, label: str) -> float | None:
+    """The first 'label: X ms' figure in the reference's debug timer output."""
+    import re
+    m = re.search(re.escape(label) + r":\s+([0-9.]+) ms", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_sample(cfg):
+    from gfaz_amd import synth
+    if cfg.n_haps * cfg.backbone * 1.25 <= CPU_SAMPLE_STEPS:
+        return cfg, f"whole workload {cfg.name}"
+    scfg = cfg.scaled(backbone=cfg.backbone // CPU_SAMPLE_DIV)
+    return scfg, (f"{cfg.name} generator over 1/{CPU_SAMPLE_DIV} of the backbone "
+                  f"({scfg.backbone} sites), all {cfg.n_haps} traversals")
+
+
+def _gpu_on_sample(ctx, codec, tr, reps=3):
+    """Device-resident compress and decompress of the sample (best of reps)."""
+    d_nodes = codec.DeviceArray.from_host(ctx, tr.nodes)
+    d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
+    best_c = best_d = None
+    comp = None
+    for _ in range(reps):
+        if comp is not None:
+            comp.close()
+        ctx.sync()
+        t0 = time.perf_counter()
+        comp = codec.DeviceCompressed(ctx, d_nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+        ctx.sync()
+        t1 = time.perf_counter()
+        rb = comp.rulebook()
+        codec.decompress_traversals(ctx, rb, comp.encoded_view(), comp.encoded_lengths,
+                                    tr.lengths, 1, out=d_out)
+        ctx.sync()
+        t2 = time.perf_counter()
+        rb.close()
+        best_c = t1 - t0 if best_c is None else min(best_c, t1 - t0)
+        best_d = t2 - t1 if best_d is None else min(best_d, t2 - t1)
+    ok = bool(np.array_equal(d_out.to_host(), tr.nodes))
+    host = comp.to_host()
+    comp.close()
+    d_nodes.free()
+    d_out.free()
+    return best_c, best_d, ok, host
+
+
+def cpu_baseline(cfg, gfa_bytes_full: int, threads: int, ctx) -> dict:
+    """Reference CPU path (oracle/_ref/gfaz_ref) on a bounded sample of the workload when
+    it was built (otherwise the repo's CPU oracle, codec only), with the GPU timed on the
+    same sample and the traversal columns of both compared."""
     sys.path.insert(0, ROOT)
     from oracle import oracle
-    from gfaz_amd import synth
+    from gfaz_amd import codec, synth
+    scfg, what = cpu_sample(cfg)
+    tr = synth.generate(scfg)
+    sbytes = synth.gfa_bytes(scfg, tr)
+    log(f"[bench] CPU baseline sample: {what}: {tr.n_steps} steps, {sbytes} GFA bytes")
+    gc, gd, gok, ghost = _gpu_on_sample(ctx, codec, tr)
+    gpu = {"compress_s": gc, "decompress_s": gd, "round_trip_ok": gok,
+           "compress_gbps": sbytes / gc / 1e9, "decompress_gbps": sbytes / gd / 1e9,
+           "gbps": sbytes / (gc + gd) / 1e9}
     if oracle.ref_available():
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        import gfaz_py
         with tempfile.TemporaryDirectory(prefix="gfaz_bench_") as tmp:
             gfa = os.path.join(tmp, "in.gfa")
             gz = os.path.join(tmp, "in.gfaz")
-            synth.write_gfa(cfg, tr, gfa)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-            run = lambda *a: subprocess.run([oracle.REF_BIN, *a], check=True, env=env,
-                                            capture_output=True, text=True).stdout
-            c = json.loads(run("time-compress", gfa, "8", "2", "1", str(threads), "1"))
-            run("compress", gfa, gz, "8", "2", "1", str(threads))
-            d = json.loads(run("time-decompress", gz, str(threads), "1"))
+            synth.write_gfa(scfg, tr, gfa)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), GFA_COMPRESSION_DEBUG="1")
+            p = subprocess.run([oracle.REF_BIN, "codec-times", gfa, gz, "8", "2", "1",
+                                str(threads)], check=True, env=env, capture_output=True,
+                               text=True)
+            t = json.loads(p.stdout.strip().splitlines()[-1])
+            err = p.stderr
+            ref_file = gfaz_py.read_gfaz(gz)
             gfaz_bytes = os.path.getsize(gz)
-        total = c["seconds"] + d["seconds"]
-        return {"value": gfa_bytes / total / 1e9, "unit": "GB/s", "cores": threads,
+        # codec scope from the reference's own stage timers
+        cc = (_ms_after(err, "delta_transform (x1)") or 0.0) + \
+            (_ms_after(err, "run_grammar_compression") or 0.0)
+        legacy = err[err.index("[CPU Legacy] ==="):] if "[CPU Legacy] ===" in err else ""
+        cd = sum(_ms_after(legacy, k) or 0.0 for k in
+                 ("delta decode rule columns", "expand path traversals",
+                  "inverse delta traversals", "expand walk traversals"))
+        # the GPU's traversal columns, framed as the reference frames them
+        f, s_ = codec.frame_rule_columns(ghost.rules_first, ghost.rules_second, threads)
+        split = int(ghost.encoded_lengths[:tr.n_paths].sum(dtype=np.uint64))
+        pz = codec.frame_int32(ghost.encoded[:split], threads)
+        wz = codec.frame_int32(ghost.encoded[split:], threads)
+        same = (f[1] == ref_file.rules_first.payload and s_[1] == ref_file.rules_second.payload
+                and pz[1] == ref_file.paths.payload and wz[1] == ref_file.walks.payload
+                and ref_file.ranges[0][1:3] == (ghost.start_id, ghost.end_id))
+        ref_trav = (len(ref_file.rules_first.payload) + len(ref_file.rules_second.payload)
+                    + len(ref_file.paths.payload) + len(ref_file.walks.payload))
+        gpu_trav = len(f[1]) + len(s_[1]) + len(pz[1]) + len(wz[1])
+        gpu_gfaz = gfaz_bytes - ref_trav + gpu_trav
+        total = t["compress_s"] + t["decompress_s"]
+        return {"value": sbytes / total / 1e9, "unit": "GB/s", "cores": threads,
                 "kind": "reference",
-                "sample": f"whole workload {cfg.name}: gfaz compress + decompress end to end "
-                          "(parse, grammar, zstd, GFA write to /dev/null)",
-                "compress_s": c["seconds"], "decompress_s": d["seconds"],
-                "compress_gbps": gfa_bytes / c["seconds"] / 1e9,
-                "decompress_gbps": gfa_bytes / d["seconds"] / 1e9,
-                "gfaz_bytes": gfaz_bytes, "ratio": gfa_bytes / gfaz_bytes}
+                "sample": f"{what}: {tr.n_steps} steps, {sbytes} GFA bytes; gfaz compress "
+                          "+ decompress end to end (parse, grammar, zstd, GFA write to "
+                          "/dev/null)",
+                "compress_s": t["compress_s"], "decompress_s": t["decompress_s"],
+                "compress_gbps": sbytes / t["compress_s"] / 1e9,
+                "decompress_gbps": sbytes / t["decompress_s"] / 1e9,
+                "codec_compress_s": cc / 1e3, "codec_decompress_s": cd / 1e3,
+                "codec_compress_gbps": sbytes / (cc / 1e3) / 1e9 if cc else None,
+                "codec_decompress_gbps": sbytes / (cd / 1e3) / 1e9 if cd else None,
+                "codec_scope": "reference stage timers: delta_transform + "
+                               "run_grammar_compression; delta decode rule columns + expand "
+                               "+ inverse delta (in-memory decoder)",
+                "gfaz_bytes": gfaz_bytes, "ratio": sbytes / gfaz_bytes,
+                "gpu_on_sample": dict(gpu, ratio=sbytes / gpu_gfaz,
+                                      traversal_bytes=gpu_trav,
+                                      traversal_columns_identical=bool(same)),
+                "reference_traversal_bytes": ref_trav}
     t0 = time.perf_counter()
     r = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=threads)
     t1 = time.perf_counter()
     oracle.decompress(r["encoded"], r["encoded_lengths"], r["rules_first"], r["rules_second"],
                       r["start_id"], 1, tr.n_steps, threads=threads)
     t2 = time.perf_counter()
-    return {"value": gfa_bytes / (t2 - t0) / 1e9, "unit": "GB/s", "cores": threads,
-            "kind": "port", "sample": f"whole workload {cfg.name}: traversal codec only",
-            "compress_s": t1 - t0, "decompress_s": t2 - t1}
+    return {"value": sbytes / (t2 - t0) / 1e9, "unit": "GB/s", "cores": threads,
+            "kind": "port", "sample": f"{what}: traversal codec only",
+            "compress_s": t1 - t0, "decompress_s": t2 - t1, "gpu_on_sample": gpu}
+
+
+def extract_latency(ctx, codec, comp, tr, n_req: int, rate: float, seed: int = 11) -> dict:
+    """Single-request extraction from the resident store at a fixed request rate: Poisson
+    arrivals, one random traversal per request, decoded into host memory; latency = result
+    in host memory minus arrival (waiting behind the previous request counts)."""
+    rb = comp.rulebook()
+    res = codec.Resident(ctx, rb, comp.encoded_view(), comp.encoded_lengths, 1)
+    offs = np.concatenate([[0], np.cumsum(tr.lengths, dtype=np.int64)])
+    rng = np.random.default_rng(seed)
+    ids = rng.integers(0, tr.lengths.size, n_req)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    res.extract(ids[:2])  # warm (and autotune)
+    lat, ok, nbytes = [], True, 0
+    t_start = time.perf_counter()
+    arrive = t_start
+    for k in range(n_req):
+        arrive += gaps[k]
+        now = time.perf_counter()
+        if now < arrive:
+            time.sleep(arrive - now)
+        vals, _ = res.extract(ids[k:k + 1])
+        done = time.perf_counter()
+        lat.append((done - arrive) * 1e3)
+        nbytes += vals.nbytes
+        if k < 8:
+            t = int(ids[k])
+            ok = ok and bool(np.array_equal(vals, tr.nodes[offs[t]:offs[t + 1]]))
+    elapsed = time.perf_counter() - t_start
+    res.close()
+    rb.close()
+    lat = np.array(lat)
+    return {"requests": n_req, "rate_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "gbps": nbytes / elapsed / 1e9, "checked_ok": ok,
+            "request": "one random traversal, int32 decoded into host memory (gfz_extract)"}
 
 
 def _compress_timed(ctx, codec, tr, d_nodes, reps=2):
@@ -252,11 +392,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain_500k")
+    ap.add_argument("--workload", default=None,
+                    help="plain_10m for roundtrip and stream, walks_2m for batching and serve")
+    ap.add_argument("--extract-requests", type=int, default=200)
+    ap.add_argument("--extract-rate", type=float, default=50.0,
+                    help="requests per second of the fixed-rate extract latency run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-oracle", action="store_true",
                     help="also compare the encoded grammar with the CPU oracle")
-    ap.add_argument("--mode", default="roundtrip", choices=["roundtrip", "batching", "stream"],
+    ap.add_argument("--mode", default="roundtrip",
+                    choices=["roundtrip", "batching", "stream", "serve"],
                     help="roundtrip (default): compress + decompress steps; batching: one "
                          "persistent launch over all traversals vs one launch per traversal "
                          "(walks_2m); stream: compress, then decompress streamed through two "
@@ -273,8 +418,9 @@ def main() -> int:
     sys.path.insert(0, ROOT)
     from gfaz_amd import codec, synth
 
-    if args.mode != "roundtrip" and args.workload == "plain_500k":
-        args.workload = {"batching": "walks_2m", "stream": "plain_10m"}[args.mode]
+    if args.workload is None:
+        args.workload = {"roundtrip": "plain_10m", "batching": "walks_2m",
+                         "stream": "plain_10m", "serve": "walks_2m"}[args.mode]
     cfg = synth.CONFIGS[args.workload]
     log(f"[bench] generating {cfg.name}")
     tr = synth.generate(cfg)
@@ -282,7 +428,11 @@ def main() -> int:
     log(f"[bench] steps={tr.n_steps} traversals={tr.lengths.size} gfa_bytes={gfa_bytes}")
 
     ctx = codec.Context(0)
+    ctx.sync()
+    t0 = time.perf_counter()
     d_nodes = codec.DeviceArray.from_host(ctx, tr.nodes)
+    ctx.sync()
+    h2d_s = time.perf_counter() - t0
     if args.mode == "batching":
         return run_batching(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     if args.mode == "stream":
@@ -371,10 +521,13 @@ def main() -> int:
         dom = dominant(kstats)
 
     # Correctness of what was timed: exact round trip, and grammar counts.
+    t0 = time.perf_counter()
     got = d_out.to_host()
+    d2h_s = time.perf_counter() - t0
     if not np.array_equal(got, tr.nodes):
         log("[bench] ROUND TRIP MISMATCH")
         return 1
+    del got
     host = last.to_host()
     round_info = {"lengths": last.lengths_per_round, "rules": last.rules_per_round,
                   "distinct": last.distinct_per_round, "slots": last.slots_per_round}
@@ -387,7 +540,14 @@ def main() -> int:
         if not same:
             log("[bench] ORACLE MISMATCH")
             return 1
+    extract = None
+    if args.extract_requests > 0:
+        log(f"[bench] extract latency: {args.extract_requests} requests at "
+            f"{args.extract_rate}/s")
+        extract = extract_latency(ctx, codec, last, tr, args.extract_requests,
+                                  args.extract_rate)
     last.close()
+    d_out.free()
 
     N, E, R = tr.n_steps, host.n_encoded, host.n_rules
     ks = kstats[dom]
@@ -431,8 +591,15 @@ def main() -> int:
         "round_stats": [{"length_out": l, "rules": r, "distinct_pairs": d, "table_slots": t}
                         for l, r, d, t in zip(round_info["lengths"], round_info["rules"],
                                               round_info["distinct"], round_info["slots"])],
+        "grammar_ratio": 4.0 * N / (4.0 * E + 8.0 * R),
         "wall_s_timed_region": t_total,
         "roofline": roofline, "expand_roofline": expand_roof,
+        "extract": extract,
+        "pcie_inclusive": {"h2d_input_s": h2d_s, "d2h_output_s": d2h_s,
+                           "compress_gbps": gfa_bytes / (tc / steps + h2d_s) / 1e9,
+                           "decompress_gbps": gfa_bytes / (td / steps + d2h_s) / 1e9,
+                           "note": "int32 traversals copied host->device before compress and "
+                                   "device->host after decompress (pageable host memory)"},
         "kernel_ms_per_step": {g: profiled[g]["total_ms"] for g in groups
                                if profiled[g]["launches"]},
         "kernel_ms_source": ("one fully profiled warmup step" if args.warmup
@@ -440,7 +607,9 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(cfg, tr, gfa_bytes, cpu_threads())
+        d_nodes.free()
+        del tr
+        out["cpu_baseline"] = cpu_baseline(cfg, gfa_bytes, cpu_threads(), ctx)
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -99,6 +99,8 @@ _SIGNATURES = {
     "gfz_resident_load": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_uint32,
                            ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_resident_free": ([_vp], None),
+    "gfz_resident_append": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, ctypes.POINTER(_u64)],
+                            ctypes.c_int),
     "gfz_resident_info": ([_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_resident_lengths": ([_vp, _vp], ctypes.c_int),
     "gfz_extract": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, _vp,
@@ -556,6 +558,25 @@ class Resident:
         nt, nd = _u64(), _u64()
         _check(lib().gfz_resident_info(self.handle, ctypes.byref(nt), ctypes.byref(nd)))
         self.n_trav, self.n_decoded = int(nt.value), int(nd.value)
+
+    def append(self, encoded, encoded_lengths) -> int:
+        """Append encoded traversals (gfz_resident_append); returns the first new index."""
+        enc_len = np.ascontiguousarray(encoded_lengths, dtype=np.uint32)
+        flags = 0
+        if isinstance(encoded, DeviceArray):
+            flags |= GFZ_INPUT_DEVICE
+            eptr, en = encoded.ptr, encoded.n
+        else:
+            encoded = np.ascontiguousarray(encoded, dtype=np.int32)
+            eptr, en = encoded.ctypes.data, encoded.size
+        first = _u64()
+        _check(lib().gfz_resident_append(self.ctx.handle, self.handle, eptr, en,
+                                         enc_len.ctypes.data, enc_len.size, flags,
+                                         ctypes.byref(first)))
+        nt, nd = _u64(), _u64()
+        _check(lib().gfz_resident_info(self.handle, ctypes.byref(nt), ctypes.byref(nd)))
+        self.n_trav, self.n_decoded = int(nt.value), int(nd.value)
+        return int(first.value)
 
     def lengths(self) -> np.ndarray:
         out = np.empty(max(1, self.n_trav), dtype=np.uint32)

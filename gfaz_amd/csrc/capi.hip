@@ -564,6 +564,25 @@ void gfz_resident_free(gfz_resident *r) {
   delete r;
 }
 
+int gfz_resident_append(gfz_ctx *ctx, gfz_resident *r, const int32_t *encoded,
+                        uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                        uint32_t flags, uint64_t *first_new_id) {
+  if (!ctx || !r || (n_trav && !encoded_lengths) || (n_encoded && !encoded))
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  if (r->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "resident state belongs to another ctx");
+  return guarded(kExtractPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_enc;
+    stage_encoded(c, encoded, n_encoded, flags, staged, d_enc);
+    if (first_new_id)
+      *first_new_id = r->res.n_trav;
+    gfz::resident_append(c, r->res, d_enc, n_encoded, encoded_lengths, n_trav);
+  });
+}
+
 int gfz_resident_info(const gfz_resident *r, uint64_t *n_trav, uint64_t *n_decoded) {
   if (!r)
     return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null resident");

@@ -30,14 +30,18 @@ constexpr uint64_t kGrid = 4096;  // decoded positions per grid cell
 
 // Largest k in [lo, hi) with a[k] <= target, given a non-decreasing and
 // a[lo] <= target. Called by all 64 lanes of one wave; every lane returns it.
+// An empty range (hi <= lo) or a broken precondition returns lo: the search
+// always ends within 64-ary steps, whatever the inputs.
 __device__ inline uint64_t wave_last_le(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi,
                                         uint64_t target) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  while (hi - lo > 1) {
+  while (hi > lo + 1) {
     const uint64_t step = (hi - lo + kWave - 1) / kWave;
     const uint64_t k = lo + lane * step;
     const bool ok = k < hi && a[k] <= target;
-    const uint64_t m = __ballot(ok);  // lane 0 always holds
+    const uint64_t m = __ballot(ok);  // lane 0 holds when the precondition does
+    if (m == 0)
+      break;
     const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
     lo = lo + (uint64_t)last * step;
     hi = hi < lo + step ? hi : lo + step;
@@ -45,13 +49,16 @@ __device__ inline uint64_t wave_last_le(const uint64_t *__restrict__ a, uint64_t
   return lo;
 }
 
-__global__ void k_grid(const uint64_t *__restrict__ sym_off, uint64_t n_sym,
+// grid[c] = index of the symbol covering decoded position c * kGrid, for the
+// symbols base .. base + n_sym - 1 whose decoded starts are sym_off[0 .. n_sym]
+// (sym_off points at the first of them; indices stored are whole-stream ones).
+__global__ void k_grid(const uint64_t *__restrict__ sym_off, uint64_t n_sym, uint64_t base,
                        uint64_t *__restrict__ grid) {
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_sym;
        k += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t lo = sym_off[k], hi = sym_off[k + 1];
     for (uint64_t c = (lo + kGrid - 1) / kGrid; c * kGrid < hi; ++c)
-      grid[c] = k;
+      grid[c] = base + k;
   }
 }
 
@@ -238,7 +245,7 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
   fill_u64(ctx, res.grid.get(), n_enc ? n_enc - 1 : 0, res.n_cells);
   if (n_enc) {
     hipLaunchKernelGGL(k_grid, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                       res.sym_off.get(), n_enc, res.grid.get());
+                       res.sym_off.get(), n_enc, (uint64_t)0, res.grid.get());
     GFZ_CHECK_LAUNCH();
   }
   ctx.sync();
@@ -429,7 +436,7 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
   fill_u64(ctx, res.grid.get() + first, res.n_enc ? res.n_enc - 1 : 0, res.n_cells - first);
   if (n_new) {
     hipLaunchKernelGGL(k_grid, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
-                       res.sym_off.get() + e0, n_new, res.grid.get());
+                       res.sym_off.get() + e0, n_new, e0, res.grid.get());
     GFZ_CHECK_LAUNCH();
   }
   ctx.sync();

@@ -35,6 +35,19 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr const char *kServePrefix = "Serving error: ";
+constexpr uint32_t kDefaultWaitMs = 120000;  // gfz_wait's deadline when the config gives 0
+
+// Where a request is; gfz_wait names it when its deadline passes.
+enum Stage : int { kQueued = 0, kAdding, kDecoding, kCopying, kDone };
+const char *stage_name(int st) {
+  switch (st) {
+    case kQueued: return "queued";
+    case kAdding: return "encoding and appending (add)";
+    case kDecoding: return "admitted, decode being enqueued";
+    case kCopying: return "decode launched, copy to host pending";
+    default: return "done";
+  }
+}
 
 struct Request {
   uint64_t ticket = 0;
@@ -51,6 +64,7 @@ struct Request {
   uint64_t out_elems = 0, slot_off = 0;
   Clock::time_point submitted, finished;
   bool done = false;
+  int stage = kQueued;  // under the server's mu
 };
 using ReqPtr = std::shared_ptr<Request>;
 
@@ -96,6 +110,7 @@ struct gfz_server {
   std::deque<int> inflight;  // slots in launch order
   std::unordered_map<uint64_t, ReqPtr> tickets;
   std::vector<uint64_t> known_len;  // decoded length per stored traversal, under mu
+                                    // (its size is the stored traversal count)
   uint64_t next_ticket = 1;
   bool stop = false;
   std::thread worker, completer;
@@ -147,6 +162,7 @@ struct gfz_server {
     // caller holds mu
     r->finished = t;
     r->done = true;
+    r->stage = kDone;
     ++completed;
     latencies_ms.push_back(std::chrono::duration<double, std::milli>(t - r->submitted).count());
     last_finish = t;
@@ -191,6 +207,7 @@ struct gfz_server {
             r->slot_off = used;
             used += r->out_elems;
           }
+          r->stage = r->add ? kAdding : kDecoding;
           batch.push_back(r);
         }
         if (batch.empty())
@@ -307,9 +324,13 @@ struct gfz_server {
     }
     std::lock_guard<std::mutex> lk(mu);
     ++batches;
-    for (auto it = deferred.rbegin(); it != deferred.rend(); ++it)
+    for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) {
+      (*it)->stage = kQueued;
       queue.push_front(*it);
+    }
     if (launched) {
+      for (const ReqPtr &r : s.reqs)
+        r->stage = kCopying;
       inflight.push_back(si);
     } else {
       for (const ReqPtr &r : s.reqs)  // empty extracts (no output)
@@ -371,15 +392,6 @@ int gfz_server_create(const gfz_server_config *cfg, const int32_t *rules_first,
     return gfz::fail(GFZ_E_INVALID, std::string(kServePrefix) + "null argument");
   if (cfg->hbm_budget_bytes < 2 * 4096)
     return gfz::fail(GFZ_E_INVALID, std::string(kServePrefix) + "HBM budget too small");
-  // The threaded loop (worker + completion thread, compute + copy stream) did
-  // not complete its first GPU test and the cause is not yet shown, so it is off
-  // unless explicitly enabled.
-  const char *en = std::getenv("GFAZ_ENABLE_SERVING");
-  if (!en || std::string(en) != "1")
-    return gfz::fail(GFZ_E_INVALID, std::string(kServePrefix) +
-                                         "the serving loop is disabled in this build (its first "
-                                         "GPU run did not complete; set GFAZ_ENABLE_SERVING=1 "
-                                         "to enable it)");
   *out = nullptr;
   return gfz::guarded(kServePrefix, [&] {
     auto s = std::make_unique<gfz_server>();
@@ -512,7 +524,13 @@ int gfz_wait(gfz_server *s, uint64_t ticket, gfz_result *out) {
     if (it == s->tickets.end())
       return gfz::fail(GFZ_E_INVALID, std::string(kServePrefix) + "unknown ticket");
     r = it->second;
-    s->cv_done.wait(lk, [&] { return r->done; });
+    const uint32_t ms = s->cfg.wait_timeout_ms ? s->cfg.wait_timeout_ms : kDefaultWaitMs;
+    if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(ms), [&] { return r->done; }))
+      // The ticket stays valid: a later gfz_wait may still collect the result.
+      return gfz::fail(GFZ_E_HIP, std::string(kServePrefix) + "request " +
+                                      std::to_string(ticket) + " not finished after " +
+                                      std::to_string(ms) + " ms (stage: " +
+                                      stage_name(r->stage) + ")");
     s->tickets.erase(ticket);  // by key: submissions may have rehashed the map meanwhile
   }
   out->status = r->status;
@@ -565,7 +583,7 @@ int gfz_server_stats(gfz_server *s, gfz_serve_stats *out) {
   out->rejected = s->rejected;
   out->batches = s->batches;
   out->bytes_out = s->bytes_out;
-  out->n_traversals = s->res.n_trav;
+  out->n_traversals = s->known_len.size();  // the worker changes res without mu
   std::vector<double> l = s->latencies_ms;
   if (!l.empty()) {
     auto pct = [&](double q) {

@@ -189,8 +189,18 @@ static std::vector<uint64_t> collect_links(const int32_t *nodes, const uint32_t 
   for (uint64_t t = 0; t < n_trav; ++t)
     off[t + 1] = off[t] + lengths[t];
   const uint64_t pairs = off[n_trav];
+  // Distinct adjacencies are few next to the steps (a node has a handful of
+  // successors), so the table starts at 8 slots per node ID and grows x4 when
+  // more than half full.
+  uint64_t max_id = 0;
+#pragma omp parallel for reduction(max : max_id)
+  for (int64_t i = 0; i < (int64_t)pairs; ++i) {
+    const int64_t v = nodes[i];
+    const uint64_t a = (uint64_t)(v < 0 ? -v : v);
+    max_id = a > max_id ? a : max_id;
+  }
   uint64_t cap = 1024;
-  while (cap < pairs / 4 + 1024)
+  while (cap < std::min(pairs / 4, 8 * max_id) + 1024)
     cap <<= 1;
   for (;;) {
     const uint64_t EMPTY = ~0ull;

@@ -23,7 +23,8 @@ _dbl = ctypes.c_double
 
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", _u64),
-                ("max_batch_requests", ctypes.c_uint32), ("max_queue", ctypes.c_uint32)]
+                ("max_batch_requests", ctypes.c_uint32), ("max_queue", ctypes.c_uint32),
+                ("wait_timeout_ms", ctypes.c_uint32)]
 
 
 class _Result(ctypes.Structure):
@@ -66,7 +67,8 @@ def _bind():
 class Server:
     def __init__(self, rules_first, rules_second, start_id: int, encoded, encoded_lengths,
                  delta_round: int = 1, hbm_budget_bytes: int = 1 << 30, device: int = 0,
-                 layers=None, max_batch_requests: int = 0, max_queue: int = 0):
+                 layers=None, max_batch_requests: int = 0, max_queue: int = 0,
+                 wait_timeout_ms: int = 0):
         L = _bind()
         rf = np.ascontiguousarray(rules_first, dtype=np.int32)
         rs = np.ascontiguousarray(rules_second, dtype=np.int32)
@@ -76,7 +78,8 @@ class Server:
             layers = [(int(start_id), int(start_id) + int(rf.size), 0)] if rf.size else []
         arr = (codec._Layer * max(1, len(layers)))(*[codec._Layer(int(a), int(b), int(c))
                                                     for a, b, c in layers])
-        cfg = _Config(device, int(hbm_budget_bytes), int(max_batch_requests), int(max_queue))
+        cfg = _Config(device, int(hbm_budget_bytes), int(max_batch_requests), int(max_queue),
+                      int(wait_timeout_ms))
         h = _vp()
         _check(L.gfz_server_create(ctypes.byref(cfg), rf.ctypes.data, rs.ctypes.data, rf.size,
                                    int(start_id), arr, len(layers), enc.ctypes.data, enc.size,

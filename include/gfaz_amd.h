@@ -183,6 +183,16 @@ void gfz_resident_free(gfz_resident *r);
 int gfz_resident_info(const gfz_resident *r, uint64_t *n_trav, uint64_t *n_decoded);
 /* Decoded length of every stored traversal (n_trav uint32). */
 int gfz_resident_lengths(const gfz_resident *r, uint32_t *decoded_lengths);
+/* Appends already-encoded traversals (encoded with the same rulebook, e.g. by
+ * gfz_encode_with_rules) to the resident state; they get the next traversal
+ * indices (*first_new_id receives the first). This is the resident half of
+ * add_haplotypes (src/workflows/add_haplotypes_workflow.cpp:467-575), which
+ * appends to the traversal columns and zstd-frames the whole column again; here
+ * only the new symbols' decode state is computed, and nothing is re-framed.
+ * encoded: host, or device with GFZ_INPUT_DEVICE. */
+int gfz_resident_append(gfz_ctx *ctx, gfz_resident *r, const int32_t *encoded,
+                        uint64_t n_encoded, const uint32_t *encoded_lengths, uint64_t n_trav,
+                        uint32_t flags, uint64_t *first_new_id);
 /* Decode traversals ids[0..n_ids) back to back into out (host, or device with
  * GFZ_OUTPUT_DEVICE; out_cap int32). offsets (host, n_ids + 1, may be NULL)
  * receives each one's start; *n_out the total. Repeated ids are decoded again. */
@@ -269,6 +279,9 @@ typedef struct gfz_server_config {
   uint64_t hbm_budget_bytes;    /* device bytes for request outputs in flight */
   uint32_t max_batch_requests;  /* 0: 256 */
   uint32_t max_queue;           /* 0: unbounded */
+  uint32_t wait_timeout_ms;     /* gfz_wait deadline; 0: 120000. On expiry gfz_wait returns
+                                   GFZ_E_HIP naming the request's stage (queued, adding,
+                                   decoding, copying) and the ticket stays waitable */
 } gfz_server_config;
 
 typedef struct gfz_result {

@@ -15,6 +15,14 @@
 //       -> extract_walk_line (src/workflows/extraction_workflow.cpp:266)
 //   add-haplotypes <in.gfaz> <append.gfa> <out.gfaz>
 //       -> add_haplotypes (src/workflows/add_haplotypes_workflow.cpp:579)
+//   codec-times <in.gfa> <out.gfaz> <rounds> <threshold> <delta_round> <threads>
+//       -> one timed compress_gfa (then serialised, untimed), one timed
+//          write_gfa_from_compressed_data to /dev/null (the CLI's decompress) and
+//          one timed decompress_gfa (src/workflows/decompression_workflow.cpp:254,
+//          the in-memory decoder), printed as JSON. With GFA_COMPRESSION_DEBUG=1
+//          in the environment the reference prints its own stage timers to stderr
+//          (compression_workflow.cpp:695-698,791-819; decompression_workflow.cpp:
+//          260-546), which give the codec-only times.
 //   time-compress <in.gfa> <rounds> <threshold> <delta_round> <threads> <reps>
 //   time-decompress <in.gfaz> <threads> <reps>
 //       -> wall-clock timing of the two calls above, printed as JSON; the
@@ -23,6 +31,7 @@
 #include "io/gfa_writer.hpp"
 #include "workflows/add_haplotypes_workflow.hpp"
 #include "workflows/compression_workflow.hpp"
+#include "workflows/decompression_workflow.hpp"
 #include "workflows/extraction_workflow.hpp"
 
 #include <chrono>
@@ -81,6 +90,29 @@ int main(int argc, char **argv) {
       gfaz::CompressedData d = gfaz::deserialize_compressed_data(argv[2]);
       add_haplotypes(d, argv[3], 1);
       gfaz::serialize_compressed_data(d, argv[4]);
+      return 0;
+    }
+    if (cmd == "codec-times" && argc == 8) {
+      const int threads = std::atoi(argv[7]);
+      double t0 = now_s();
+      gfaz::CompressedData d =
+          compress_gfa(argv[2], std::atoi(argv[4]), std::strtoul(argv[5], nullptr, 10),
+                       std::atoi(argv[6]), threads, false);
+      const double tc = now_s() - t0;
+      gfaz::serialize_compressed_data(d, argv[3]);
+      t0 = now_s();
+      write_gfa_from_compressed_data(d, "/dev/null", threads);
+      const double tw = now_s() - t0;
+      double tl;
+      {
+        gfaz::GfaGraph g;
+        t0 = now_s();
+        decompress_gfa(d, g, threads);
+        tl = now_s() - t0;
+      }
+      std::printf("{\"compress_s\": %.6f, \"decompress_s\": %.6f, "
+                  "\"decompress_in_memory_s\": %.6f, \"threads\": %d}\n",
+                  tc, tw, tl, threads);
       return 0;
     }
     if (cmd == "time-compress" && argc == 8) {

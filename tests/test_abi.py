@@ -61,10 +61,15 @@ def test_synthetic_generator_is_deterministic():
     assert a.n_steps > 3000 * 4 * 0.9
 
 
-def test_serving_is_disabled_without_opt_in(monkeypatch):
-    """The serving loop refuses to start unless GFAZ_ENABLE_SERVING=1 (no GPU touched)."""
-    monkeypatch.delenv("GFAZ_ENABLE_SERVING", raising=False)
+def test_serving_argument_checks_need_no_gpu():
+    """gfz_server_create rejects a missing config or a tiny HBM budget before it
+    touches a device."""
     from gfaz_amd import serving
-    with pytest.raises(RuntimeError, match="serving loop is disabled"):
+    L = serving._bind()
+    out = ctypes.c_void_p()
+    assert L.gfz_server_create(None, None, None, 0, 1, None, 0, None, 0, None, 0, 1,
+                               ctypes.byref(out)) == -1
+    assert b"null argument" in L.gfz_last_error()
+    with pytest.raises(RuntimeError, match="HBM budget too small"):
         serving.Server(np.zeros(0, np.int32), np.zeros(0, np.int32), 1,
-                       np.zeros(0, np.int32), np.zeros(0, np.uint32), 1, hbm_budget_bytes=1 << 20)
+                       np.zeros(0, np.int32), np.zeros(0, np.uint32), 1, hbm_budget_bytes=1024)

@@ -108,3 +108,45 @@ def test_extract_without_rules(gpu_ctx):
     _check_subset(res, _split(nodes, lengths), [2, 1, 0])
     res.close()
     rb.close()
+
+
+@pytest.mark.parametrize("delta", [0, 1, 2])
+def test_resident_append_then_extract(gpu_ctx, delta):
+    """Traversals appended to a resident store (gfz_resident_append) decode exactly,
+    and so do the stored ones whose coarse-index cells the append rewrote: the
+    traversals of one oracle compression are loaded in part, the rest appended in
+    batches (one with an empty traversal), then every subset is extracted."""
+    say(f"resident append delta={delta}")
+    tr = synth.generate(synth.CONFIGS["plain_20k"])
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, delta, threads=8)
+    trav = _split(tr.nodes, tr.lengths)
+    enc = _split(ref["encoded"], ref["encoded_lengths"])
+    rb = codec.Rulebook(gpu_ctx, ref["rules_first"], ref["rules_second"], ref["start_id"])
+    base = 7
+    res = codec.Resident(gpu_ctx, rb, np.concatenate(enc[:base]),
+                         ref["encoded_lengths"][:base], delta)
+    order = list(range(base))
+    for batch in ([base], [base + 1, base + 2], [], [base + 3, base + 4, base + 5]):
+        # an empty traversal rides along with the second batch
+        parts = [enc[i] for i in batch]
+        lens = [p.size for p in parts]
+        if len(batch) == 2:
+            parts.append(np.zeros(0, np.int32))
+            lens.append(0)
+        first = res.append(np.concatenate(parts) if parts else np.zeros(0, np.int32),
+                           np.array(lens, np.uint32))
+        assert first == len(order)
+        order += batch + ([None] if len(batch) == 2 else [])
+        stored = [trav[i] if i is not None else np.zeros(0, np.int32) for i in order]
+        assert res.n_trav == len(stored)
+        np.testing.assert_array_equal(res.lengths(), [s.size for s in stored])
+        _check_subset(res, stored, np.arange(len(stored)))
+        _check_subset(res, stored, np.arange(len(stored))[::-1])
+        _check_subset(res, stored, np.random.default_rng(len(order)).integers(0, len(stored), 25))
+    # appended to a store that started empty
+    empty = codec.Resident(gpu_ctx, rb, np.zeros(0, np.int32), np.zeros(0, np.uint32), delta)
+    assert empty.append(np.concatenate(enc[:3]), ref["encoded_lengths"][:3]) == 0
+    _check_subset(empty, trav[:3], [2, 0, 1, 1])
+    empty.close()
+    res.close()
+    rb.close()
