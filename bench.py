@@ -424,7 +424,7 @@ def main() -> int:
     cfg = synth.CONFIGS[args.workload]
     log(f"[bench] generating {cfg.name}")
     tr = synth.generate(cfg)
-    gfa_bytes = synth.gfa_bytes(cfg, tr)
+    gfa_bytes = 0 if args.pmc_pass else synth.gfa_bytes(cfg, tr)
     log(f"[bench] steps={tr.n_steps} traversals={tr.lengths.size} gfa_bytes={gfa_bytes}")
 
     ctx = codec.Context(0)

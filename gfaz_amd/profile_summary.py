@@ -32,7 +32,8 @@ def short_name(kernel: str) -> str:
     return kernel.split("(")[0].split("::")[-1][:40]
 
 
-def stats_markdown(csv_path: str, cmd: str, top: int = 25) -> str:
+def stats_markdown(csv_path: str, cmd: str, top: int = 0) -> str:
+    """Every kernel (top > 0: only the top kernels by total time)."""
     rows = list(csv.DictReader(open(csv_path)))
     agg = collections.OrderedDict()
     for r in rows:
@@ -49,7 +50,8 @@ def stats_markdown(csv_path: str, cmd: str, top: int = 25) -> str:
     lines = [f"Command: `{cmd}`", "",
              "| kernel | calls | total ms | avg us | min us | max us | % |",
              "|---|---:|---:|---:|---:|---:|---:|"]
-    for name, (calls, tot, mn, mx) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+    ranked = sorted(agg.items(), key=lambda kv: -kv[1][1])
+    for name, (calls, tot, mn, mx) in ranked[:top] if top > 0 else ranked:
         lines.append(f"| {name} | {calls} | {tot / 1e6:.3f} | {tot / calls / 1e3:.1f} | "
                      f"{mn / 1e3:.1f} | {mx / 1e3:.1f} | {100 * tot / total:.1f} |")
     return "\n".join(lines) + "\n"
@@ -128,7 +130,7 @@ def main(argv=None) -> int:
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--cmd", required=True)
-    s.add_argument("--top", type=int, default=25)
+    s.add_argument("--top", type=int, default=0, help="0: every kernel")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)

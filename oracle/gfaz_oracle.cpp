@@ -146,15 +146,34 @@ uint64_t grammar_round(Seqs &seqs, uint32_t &next_id, std::vector<int64_t> &rule
     for (size_t i = 0; i + 1 < s.size(); ++i)
       table.insert(canonical_2mer(pack_2mer(s[i], s[i + 1])));
   }
-  bool any = false;
-#pragma omp parallel for schedule(static) reduction(|| : any)
-  for (int64_t i = 0; i < (int64_t)table.cap; ++i)
-    any = any || table.repeated[i].load(std::memory_order_relaxed);
-  if (!any)
+  // Provisional IDs next_id + k, k dense over the repeated keys in table order,
+  // like the hash-order IDs next_id + k of rule_generator.cpp:249-255 (any
+  // order gives the same output: the used rules are renumbered by key below).
+  // prov[slot] = k for a repeated slot; chunk counts first, then offsets.
+  std::unique_ptr<uint32_t[]> prov(new uint32_t[table.cap]);
+  const uint64_t chunk = 1 << 20, n_chunks = (table.cap + chunk - 1) / chunk;
+  std::vector<uint64_t> chunk_rep(n_chunks + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < (int64_t)n_chunks; ++c) {
+    uint64_t n = 0;
+    for (uint64_t i = c * chunk; i < std::min<uint64_t>(table.cap, (c + 1) * chunk); ++i)
+      n += table.repeated[i].load(std::memory_order_relaxed);
+    chunk_rep[c + 1] = n;
+  }
+  for (uint64_t c = 0; c < n_chunks; ++c)
+    chunk_rep[c + 1] += chunk_rep[c];
+  const uint64_t n_rep = chunk_rep[n_chunks];
+  if (n_rep == 0)
     return 0;  // compression_workflow.cpp:576-577
-  // Provisional IDs next_id + slot: distinct per key and above every symbol of
-  // this round, like the hash-order IDs of rule_generator.cpp:249-255.
-  if ((uint64_t)next_id + table.cap >= (uint64_t)INT32_MAX) {
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < (int64_t)n_chunks; ++c) {
+    uint64_t k = chunk_rep[c];
+    for (uint64_t i = c * chunk; i < std::min<uint64_t>(table.cap, (c + 1) * chunk); ++i)
+      if (table.repeated[i].load(std::memory_order_relaxed))
+        prov[i] = (uint32_t)k++;
+  }
+  // The reference's symbols are int32: IDs must stay below 2^31.
+  if ((uint64_t)next_id + n_rep > (uint64_t)INT32_MAX) {
     *err = -2;
     return 0;
   }
@@ -174,11 +193,11 @@ uint64_t grammar_round(Seqs &seqs, uint32_t &next_id, std::vector<int64_t> &rule
       int64_t slot = table.find_repeated(top);
       int32_t sym = 0;
       if (slot >= 0) {
-        sym = (int32_t)(start + (uint64_t)slot);
+        sym = (int32_t)(start + prov[slot]);
       } else {
         slot = table.find_repeated(reverse_2mer(top));
         if (slot >= 0)
-          sym = -(int32_t)(start + (uint64_t)slot);
+          sym = -(int32_t)(start + prov[slot]);
       }
       if (slot >= 0) {
         stack.pop_back();
@@ -197,9 +216,9 @@ uint64_t grammar_round(Seqs &seqs, uint32_t &next_id, std::vector<int64_t> &rule
   if (used.empty())
     return 0;  // compression_workflow.cpp:611-627
   std::sort(used.begin(), used.end());
-  std::vector<uint32_t> id_map(table.cap, 0);
+  std::vector<uint32_t> id_map(n_rep, 0);  // provisional offset -> final ID
   for (size_t r = 0; r < used.size(); ++r)
-    id_map[used[r].second] = start + (uint32_t)r;
+    id_map[prov[used[r].second]] = start + (uint32_t)r;
   // remap_rule_ids (src/workflows/compression_utils.cpp:201-221)
 #pragma omp parallel for schedule(dynamic)
   for (int64_t p = 0; p < (int64_t)seqs.size(); ++p) {
