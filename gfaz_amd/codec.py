@@ -99,6 +99,8 @@ _SIGNATURES = {
     "gfz_resident_load": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_uint32,
                            ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_resident_free": ([_vp], None),
+    "gfz_extract_noalloc": ([_vp, _vp, _vp, _u64, _vp, _u64, _vp, ctypes.POINTER(_u64)],
+                            ctypes.c_int),
     "gfz_resident_append": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, ctypes.POINTER(_u64)],
                             ctypes.c_int),
     "gfz_resident_info": ([_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)], ctypes.c_int),

@@ -6,6 +6,7 @@
 #include "codec.hpp"
 #include "encode.hpp"
 #include "prims.hpp"
+#include "resident.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -66,6 +67,7 @@ int gfz_ctx_create(int device, gfz_ctx **out) {
     gfz::DeviceGuard g(device);
     auto c = std::make_unique<gfz_ctx>();
     c->ctx = std::make_unique<gfz::Ctx>(device);
+    gfz::noalloc_scratch_init(device);  // the fixed scratch of gfz_extract_noalloc
     *out = c.release();
   });
 }
@@ -658,6 +660,42 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
       GFZ_HIP(hipMemcpyAsync(out, d_out, total * 4, hipMemcpyDeviceToHost, c.stream()));
       c.sync();
     }
+  });
+}
+
+static_assert(gfz::kNoAllocScratchBytes == GFZ_NOALLOC_SCRATCH_BYTES,
+              "the header states the scratch array's size");
+
+int gfz_extract_noalloc(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids,
+                        uint64_t n_ids, int32_t *out, uint64_t out_cap, uint64_t *offsets,
+                        uint64_t *n_out) {
+  if (!ctx || !r || (n_ids && !ids))
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  if (r->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "resident state belongs to another ctx");
+  return guarded(kExtractPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n_ids; ++i) {
+      if (ids[i] >= r->res.n_trav)
+        throw gfz::Error(GFZ_E_INVALID, "traversal index " + std::to_string(ids[i]) +
+                                            " out of range (" + std::to_string(r->res.n_trav) +
+                                            " stored)");
+      total += r->res.dec_len[ids[i]];
+    }
+    if (n_out)
+      *n_out = total;
+    if (total && !out)
+      throw gfz::Error(GFZ_E_INVALID, "null output buffer");
+    // The tile size the autotuner chose for gfz_extract in this size class, if any;
+    // no timing runs here.
+    uint32_t cls = 0;
+    for (uint64_t n = total; n >= 4; n >>= 2)
+      ++cls;
+    const int items = c.tuned("extract_items/" + std::to_string(cls));
+    gfz::resident_extract_noalloc(c, r->res, ids, n_ids, out, out_cap, offsets,
+                                  items < 0 ? 16u : (uint32_t)items);
   });
 }
 

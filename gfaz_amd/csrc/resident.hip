@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace gfz {
 
@@ -340,6 +341,70 @@ uint64_t resident_extract(Ctx &ctx, const Resident &res, const uint64_t *ids, ui
   std::vector<uint8_t> stage(bytes);
   extract_enqueue(ctx, res, plan, stage.data(), meta.get(), d_out);
   ctx.sync();  // the pageable staging vector goes out of scope
+  return total;
+}
+
+// ---- extraction without allocation ------------------------------------------
+
+namespace {
+
+// One per device; the host must never pass this name to a kernel or a copy
+// (in host code it is only a stand-in): noalloc_scratch() gives the address.
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    throw Error(kErrInvalid, "device index " + std::to_string(device) + " beyond " +
+                                 std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device])
+    return;
+  DeviceGuard g(device);
+  void *ptr = nullptr;
+  GFZ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr)
+    throw Error(kErrHip, "no device address for the extraction scratch array");
+  g_scratch_ptr[device] = static_cast<uint8_t *>(ptr);
+}
+
+uint8_t *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  uint8_t *p = device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+  if (!p)
+    throw Error(kErrInvalid, "extraction scratch of device " + std::to_string(device) +
+                                 " is not set up (create a context on it first)");
+  return p;
+}
+
+uint64_t resident_extract_noalloc(Ctx &ctx, const Resident &res, const uint64_t *ids,
+                                  uint64_t n_ids, int32_t *d_out, uint64_t out_cap,
+                                  uint64_t *offsets, uint32_t items) {
+  if (res.delta_round >= 2)
+    throw Error(kErrInvalid, "extraction without allocation supports delta_round 0 and 1 "
+                             "(this store has " + std::to_string(res.delta_round) + ")");
+  ExtractPlan plan;
+  const uint64_t total = extract_plan(res, ids, n_ids, items, plan, offsets);
+  if (total > out_cap)
+    throw Error(kErrInvalid, "output capacity " + std::to_string(out_cap) + " < extracted size " +
+                                 std::to_string(total));
+  if (!total)
+    return 0;
+  const size_t bytes = extract_meta_bytes(plan);
+  if (bytes > kNoAllocScratchBytes)
+    throw Error(kErrInvalid, "request plan needs " + std::to_string(bytes) +
+                                 " bytes of scratch, more than the fixed " +
+                                 std::to_string(kNoAllocScratchBytes) + " (" +
+                                 std::to_string(plan.segs.size()) + " non-empty traversals)");
+  uint8_t *d_meta = noalloc_scratch(ctx.device());
+  std::vector<uint8_t> stage(bytes);
+  extract_enqueue(ctx, res, plan, stage.data(), d_meta, d_out);  // copies the zeroed counter too
+  ctx.sync();  // the host staging vector goes out of scope
   return total;
 }
 

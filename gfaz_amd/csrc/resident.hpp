@@ -58,6 +58,24 @@ size_t extract_meta_bytes(const ExtractPlan &plan);
 void extract_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, uint8_t *h_stage,
                      uint8_t *d_meta, int32_t *d_out);
 
+// Fixed device scratch for extraction without allocation: the plan of one
+// request (segment table, tile prefix, work-stealing counter, segment starts)
+// goes into a file-scope device array of kNoAllocScratchBytes per device,
+// held until the process ends.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+// Resolves (once per device, with that device current) the array's device
+// address; throws on a HIP error. Called at context creation.
+void noalloc_scratch_init(int device);
+// The device address of the array on `device` (noalloc_scratch_init first).
+uint8_t *noalloc_scratch(int device);
+// Extraction into caller-provided device memory with the plan in the fixed
+// scratch array: no device allocation. Rejects plans above the array's size
+// and stores with delta_round >= 2 (their scans need more scratch); calls on
+// one device share the array and must not overlap. Synchronises the stream.
+uint64_t resident_extract_noalloc(Ctx &ctx, const Resident &res, const uint64_t *ids,
+                                  uint64_t n_ids, int32_t *d_out, uint64_t out_cap,
+                                  uint64_t *offsets, uint32_t items);
+
 // Appends encoded traversals (d_enc device, enc_lengths host) to the resident
 // state; they get the next traversal indices. Synchronises the ctx stream.
 void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_new,

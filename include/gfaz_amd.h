@@ -200,6 +200,20 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
                 int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *offsets,
                 uint64_t *n_out);
 
+/* No counterpart in the reference: gfz_extract for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). out is device memory (out_cap int32); the request's plan
+ * (segment table, tile prefix, work-stealing counter) goes into a fixed device
+ * array of GFZ_NOALLOC_SCRATCH_BYTES that the library holds per device until the
+ * process ends; a request whose plan needs more (about 18 K non-empty traversals)
+ * fails with GFZ_E_INVALID, and so does a store with delta_round >= 2. Calls on
+ * one device share that array and keep its last contents: calls that overlap on
+ * one GPU race, so serialise them. Returns after the decode finished. */
+#define GFZ_NOALLOC_SCRATCH_BYTES (1u << 20)
+int gfz_extract_noalloc(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids,
+                        uint64_t n_ids, int32_t *out, uint64_t out_cap, uint64_t *offsets,
+                        uint64_t *n_out);
+
 /* ---- streaming decompress ---------------------------------------------------
  * Replaces decompress_gpu_traversal_rolling_direct_writer / the rolling stream
  * (src/gpu/decompression/traversal_decode_gpu.cu:122-156,
