@@ -214,3 +214,29 @@ def test_bits_cpp():
         assert "bad=0" in out
     finally:
         os.unlink(exe)
+
+
+def test_invariant_checker_catches_planted_mistakes():
+    """tests/invariants.check_grammar (used on full-size GPU outputs) accepts the
+    oracle's grammar and rejects plausible mistakes planted in it."""
+    from invariants import check_grammar
+    g = goldens.load("plain_20k_r8_t2_d1")
+    nodes, lengths, _ = goldens.inputs(g)
+    r = oracle.compress(nodes, lengths, 8, 2, 1, threads=4)
+    args = [r["encoded"], r["rules_first"], r["rules_second"], r["start_id"], r["end_id"],
+            list(r["rules_per_round"])]
+    check_grammar(*args)
+    swapped = list(args)
+    swapped[1], swapped[2] = args[2], args[1]            # transposed rule columns
+    shifted = list(args)
+    rp = list(args[5])
+    rp[0], rp[1] = rp[0] - 1, rp[1] + 1                  # a rule counted in the wrong round
+    shifted[5] = rp
+    negated = list(args)
+    negated[1] = -args[1]                                # a dropped sign
+    beyond = list(args)
+    beyond[0] = args[0].copy()
+    beyond[0][7] = r["end_id"]                           # an ID past the rule window
+    for bad in (swapped, shifted, negated, beyond):
+        with pytest.raises(AssertionError):
+            check_grammar(*bad)
