@@ -358,6 +358,116 @@ def run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
     return 0 if state["ok"] else 1
 
 
+def run_serve(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
+    """The serving form: the grammar of the workload (walks_2m by default) stays resident
+    in HBM behind the serving loop; a Poisson stream at a fixed rate sends 80% extract
+    requests (one random stored traversal each, so lengths follow the stored 1k-2M mix)
+    and 20% add requests (1-16 new synthetic traversals of the same graph, encoded with
+    the stored rules and appended). Latency = submission to result in host memory."""
+    import threading
+    from gfaz_amd import codec, serving, synth
+    log("[bench] compress")
+    comp = codec.compress_traversals(ctx, d_nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+    d_nodes.free()
+    rng = np.random.default_rng(args.serve_seed)
+    # Pool of new traversals for add requests: the same generator with other seeds;
+    # traversals whose largest delta reaches the rule IDs would be refused (the
+    # reference's collision check), so the pool keeps the others.
+    pool = []
+    for k in range(4):
+        new = synth.generate(cfg.scaled(n_haps=16, seed=cfg.seed + 7919 * (k + 1)))
+        offs = np.concatenate([[0], np.cumsum(new.lengths, dtype=np.int64)])
+        for i in range(new.lengths.size):
+            t = new.nodes[offs[i]:offs[i + 1]].astype(np.int64)
+            d = np.abs(np.diff(t)) if t.size > 1 else np.zeros(0, np.int64)
+            if (d.max() if d.size else 0) < comp.start_id and abs(int(t[0])) < comp.start_id:
+                pool.append(new.nodes[offs[i]:offs[i + 1]].copy())
+    log(f"[bench] serving: {len(pool)} traversals for adds; starting the server")
+    srv = serving.Server(comp.rules_first, comp.rules_second, comp.start_id, comp.encoded,
+                         comp.encoded_lengths, 1, hbm_budget_bytes=args.serve_budget_mb << 20,
+                         wait_timeout_ms=60000)
+    offs = np.concatenate([[0], np.cumsum(tr.lengths, dtype=np.int64)])
+    n = args.serve_requests
+    kinds = rng.random(n) < 0.2
+    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    plan = []
+    for k in range(n):
+        if kinds[k]:
+            m = int(rng.integers(1, 17))
+            pick = [pool[int(j)] for j in rng.integers(0, len(pool), m)]
+            plan.append(("add", np.concatenate(pick),
+                         np.array([p.size for p in pick], np.uint32)))
+        else:
+            plan.append(("extract", int(rng.integers(0, tr.lengths.size)), None))
+    # warm: one of each kind
+    srv.wait(srv.submit_extract([0]))
+    tickets = [None] * n
+    results = [None] * n
+    errors = []
+
+    def collector():
+        for k in range(n):
+            while tickets[k] is None:
+                time.sleep(0.0002)
+            try:
+                results[k] = srv.wait(tickets[k])
+            except Exception as e:  # reported below
+                errors.append((k, str(e)))
+
+    th = threading.Thread(target=collector)
+    srv.reset_stats()
+    th.start()
+    t0 = time.perf_counter()
+    arrive = t0
+    for k, (kind, a, b) in enumerate(plan):
+        arrive += gaps[k]
+        now = time.perf_counter()
+        if now < arrive:
+            time.sleep(arrive - now)
+        tickets[k] = srv.submit_extract([a]) if kind == "extract" else srv.submit_add(a, b)
+    th.join()
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    ok = not errors
+    lat = {"extract": [], "add": []}
+    checked = 0
+    for (kind, a, b), r in zip(plan, results):
+        if r is None:
+            continue
+        lat[kind].append(r["latency_ms"])
+        if kind == "extract" and checked < 50:
+            ok = ok and bool(np.array_equal(r["values"], tr.nodes[offs[a]:offs[a + 1]]))
+            checked += 1
+
+    def pct(xs, q):
+        return float(np.percentile(xs, q)) if xs else None
+
+    ex_bytes = sum(int(tr.lengths[a]) * 4 for (kind, a, _), r in zip(plan, results)
+                   if kind == "extract" and r is not None)
+    out = {
+        "metric": "serving: extract / add-haplotypes latency p50/p99 ms at a fixed request rate",
+        "value": pct(lat["extract"], 99), "unit": "ms (extract p99)", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg.name, "traversals": int(tr.lengths.size),
+                   "steps": int(tr.n_steps), "rounds": 8, "threshold": 2, "delta_round": 1,
+                   "requests": n, "rate_per_s": args.serve_rate, "extract_share": 0.8,
+                   "add_traversals": "1-16", "hbm_budget_bytes": args.serve_budget_mb << 20},
+        "extract_p50_ms": pct(lat["extract"], 50), "extract_p99_ms": pct(lat["extract"], 99),
+        "add_p50_ms": pct(lat["add"], 50), "add_p99_ms": pct(lat["add"], 99),
+        "all_p50_ms": st["p50_ms"], "all_p99_ms": st["p99_ms"],
+        "extracts": len(lat["extract"]), "adds": len(lat["add"]), "batches": st["batches"],
+        "rejected": st["rejected"], "errors": errors[:5],
+        "extract_gbps": ex_bytes / elapsed / 1e9, "server_gbps": st["gbps"],
+        "elapsed_s": elapsed, "results_checked_ok": ok, "stored_after": st["n_traversals"],
+        "note": "extract lengths are those of the stored traversals (1k-2M steps); the "
+                "1k-10M range of the serving config exceeds what walks_2m stores",
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok and st["rejected"] == 0 else 1
+
+
 # Timed kernel groups and the kernel whose hardware counters stand for them.
 # Kernels of each timed group (short names, as in the counter summaries).
 GROUP_KERNELS = {
@@ -394,6 +504,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=None,
                     help="plain_10m for roundtrip and stream, walks_2m for batching and serve")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--serve-budget-mb", type=int, default=8192,
+                    help="HBM budget of the serving loop's output slots")
+    ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--extract-requests", type=int, default=200)
     ap.add_argument("--extract-rate", type=float, default=50.0,
                     help="requests per second of the fixed-rate extract latency run")
@@ -437,6 +552,8 @@ def main() -> int:
         return run_batching(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     if args.mode == "stream":
         return run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes)
+    if args.mode == "serve":
+        return run_serve(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
 
     if args.pmc_pass:

@@ -229,10 +229,63 @@ void upload_heads(Ctx &ctx, const std::vector<uint64_t> &starts, uint64_t n, DBu
 
 }  // namespace
 
+void encode_tables_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second,
+                         uint64_t n_rules, const EncodeLayer *layers, uint64_t n_layers,
+                         EncodeTables &out) {
+  out = EncodeTables();
+  for (uint64_t l = 0; l < n_layers; ++l) {
+    const uint64_t count = (uint64_t)layers[l].end_id - layers[l].start_id;
+    if (layers[l].end_id < layers[l].start_id || layers[l].row_offset + count > n_rules)
+      throw Error(kErrCorrupt, "rule column is truncated");
+  }
+  Pool &pool = ctx.pool();
+  DBuf<uint32_t> err(pool, 1);
+  fill_u32(ctx, err.get(), 0, 1);
+  for (uint64_t l = 0; l < n_layers; ++l) {
+    out.layers.push_back(layers[l]);
+    const uint32_t count = layers[l].end_id - layers[l].start_id;
+    if (!count) {
+      out.slots.emplace_back();
+      out.caps.push_back(0);
+      continue;
+    }
+    const uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)count));
+    DBuf<uint8_t> buf(pool, cap * sizeof(EncSlot));
+    EncSlot *slots = reinterpret_cast<EncSlot *>(buf.get());
+    hipLaunchKernelGGL(k_enc_fill, dim3(grid_for(ctx, cap)), dim3(kBlock), 0, ctx.stream(),
+                       slots, cap);
+    GFZ_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_enc_insert, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx.stream(),
+                       d_first, d_second, layers[l].row_offset, count, layers[l].start_id,
+                       slots, cap - 1, err.get());
+    GFZ_CHECK_LAUNCH();
+    out.slots.push_back(std::move(buf));
+    out.caps.push_back(cap);
+  }
+  ctx.stage_scalars(err.get(), 1);
+  ctx.sync();
+  if (reinterpret_cast<const uint32_t *>(ctx.scalars())[0])
+    throw Error(kErrCorrupt, "rule table probe bound exceeded (internal error)");
+}
+
 void encode_with_rules(Ctx &ctx, const int32_t *d_nodes, uint64_t n, const uint32_t *lengths,
                        uint64_t n_trav, const int32_t *d_first, const int32_t *d_second,
                        uint64_t n_rules, const EncodeLayer *layers, uint64_t n_layers,
                        int delta_round, EncodeResult &out) {
+  uint64_t total = 0;
+  for (uint64_t t = 0; t < n_trav; ++t)
+    total += lengths[t];
+  if (total != n)
+    throw Error(kErrInvalid, "sum of traversal lengths (" + std::to_string(total) +
+                                 ") != number of nodes (" + std::to_string(n) + ")");
+  EncodeTables tables;
+  encode_tables_build(ctx, d_first, d_second, n_rules, layers, n_layers, tables);
+  encode_with_tables(ctx, d_nodes, n, lengths, n_trav, tables, delta_round, out);
+}
+
+void encode_with_tables(Ctx &ctx, const int32_t *d_nodes, uint64_t n, const uint32_t *lengths,
+                        uint64_t n_trav, const EncodeTables &tables, int delta_round,
+                        EncodeResult &out) {
   out = EncodeResult();
   out.lengths.assign(lengths, lengths + n_trav);
   std::vector<uint64_t> starts;
@@ -245,11 +298,8 @@ void encode_with_rules(Ctx &ctx, const int32_t *d_nodes, uint64_t n, const uint3
   if (total != n)
     throw Error(kErrInvalid, "sum of traversal lengths (" + std::to_string(total) +
                                  ") != number of nodes (" + std::to_string(n) + ")");
-  for (uint64_t l = 0; l < n_layers; ++l) {
-    const uint64_t count = (uint64_t)layers[l].end_id - layers[l].start_id;
-    if (layers[l].end_id < layers[l].start_id || layers[l].row_offset + count > n_rules)
-      throw Error(kErrCorrupt, "rule column is truncated");
-  }
+  const uint64_t n_layers = tables.layers.size();
+  const EncodeLayer *layers = tables.layers.data();
   Pool &pool = ctx.pool();
   DBuf<int32_t> cur(pool, std::max<uint64_t>(n, 1));
   if (n)
@@ -280,27 +330,18 @@ void encode_with_rules(Ctx &ctx, const int32_t *d_nodes, uint64_t n, const uint3
 
   uint64_t L = n;
   std::vector<uint32_t> cur_len(lengths, lengths + n_trav);
+  DBuf<uint32_t> flags(pool, 1);  // changed
   for (uint64_t l = 0; l < n_layers && L; ++l) {
-    const uint32_t count = layers[l].end_id - layers[l].start_id;
-    if (!count)
+    const uint64_t cap = tables.caps[l];
+    if (!cap)
       continue;
-    const uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)count));
-    DBuf<EncSlot> slots(pool, cap);
-    DBuf<uint32_t> flags(pool, 2);  // err, changed
-    fill_u32(ctx, flags.get(), 0, 2);
-    hipLaunchKernelGGL(k_enc_fill, dim3(grid_for(ctx, cap)), dim3(kBlock), 0, ctx.stream(),
-                       slots.get(), cap);
-    GFZ_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_enc_insert, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx.stream(),
-                       d_first, d_second, layers[l].row_offset, count, layers[l].start_id,
-                       slots.get(), cap - 1, flags.get());
-    GFZ_CHECK_LAUNCH();
+    const EncSlot *slots = reinterpret_cast<const EncSlot *>(tables.slots[l].get());
     const uint64_t n_chunks = ceil_div(L, kChunk);
     DBuf<int32_t> st(pool, L), ends(pool, n_chunks);
     DBuf<uint8_t> red(pool, L);
     ctx.timed("encode_rules", [&] {
       hipLaunchKernelGGL(k_enc_spec, dim3(grid_for(ctx, n_chunks, 64)), dim3(64), 0, ctx.stream(),
-                         cur.get(), L, heads.get(), slots.get(), cap - 1, st.get(), red.get());
+                         cur.get(), L, heads.get(), slots, cap - 1, st.get(), red.get());
     });
     GFZ_CHECK_LAUNCH();
     // Fix-up passes; each pass makes at least one more chunk exact, so at most
@@ -308,22 +349,19 @@ void encode_with_rules(Ctx &ctx, const int32_t *d_nodes, uint64_t n, const uint3
     for (uint64_t pass = 0;; ++pass) {
       if (pass > n_chunks)
         throw Error(kErrCorrupt, "add-haplotypes encoding did not settle (internal error)");
-      fill_u32(ctx, flags.get() + 1, 0, 1);
+      fill_u32(ctx, flags.get(), 0, 1);
       hipLaunchKernelGGL(k_enc_ends, dim3(grid_for(ctx, n_chunks)), dim3(kBlock), 0,
                          ctx.stream(), st.get(), L, ends.get());
       GFZ_CHECK_LAUNCH();
       ctx.timed("encode_rules", [&] {
         hipLaunchKernelGGL(k_enc_fix, dim3(grid_for(ctx, n_chunks, 64)), dim3(64), 0,
-                           ctx.stream(), cur.get(), L, heads.get(), slots.get(), cap - 1,
-                           ends.get(), st.get(), red.get(), flags.get() + 1);
+                           ctx.stream(), cur.get(), L, heads.get(), slots, cap - 1,
+                           ends.get(), st.get(), red.get(), flags.get());
       });
       GFZ_CHECK_LAUNCH();
       ctx.stage_scalars(flags.get(), 1);
       ctx.sync();
-      const uint32_t *h = reinterpret_cast<const uint32_t *>(ctx.scalars());
-      if (h[0])
-        throw Error(kErrCorrupt, "rule table probe bound exceeded (internal error)");
-      if (!h[1])
+      if (!reinterpret_cast<const uint32_t *>(ctx.scalars())[0])
         break;
     }
     // Emission: compaction of the stack tops that survive.

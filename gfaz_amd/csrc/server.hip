@@ -102,6 +102,7 @@ struct gfz_server {
   gfz::Resident res;
   gfz::DBuf<int32_t> d_first, d_second;
   std::vector<gfz::EncodeLayer> layers;
+  gfz::EncodeTables enc_tables;  // add-haplotypes lookup tables, built once
   Slot slots[2];
 
   std::mutex mu;
@@ -218,49 +219,32 @@ struct gfz_server {
     }
   }
 
+  // One batch: its extract requests are planned and launched first (decode on the
+  // compute stream, copy to pinned host memory on the copy stream), then its add
+  // requests are encoded and appended in arrival order, stream-ordered after that
+  // decode, so an extract never waits for the adds of its own batch. An extract
+  // naming traversals that are not stored yet is put back in the queue when the
+  // batch holds adds (a later batch sees them), and refused otherwise.
   void run_batch(int si, std::vector<ReqPtr> &batch) {
     Slot &s = slots[si];
     gfz::Ctx &c = *ctx;
-    // Adds first, in arrival order: encode with the stored rules and append.
-    for (const ReqPtr &r : batch) {
-      if (!r->add)
-        continue;
-      try {
-        const uint64_t n = r->nodes.size();
-        gfz::DBuf<int32_t> d_nodes(c.pool(), std::max<uint64_t>(n, 1));
-        if (n)
-          GFZ_HIP(hipMemcpyAsync(d_nodes.get(), r->nodes.data(), n * 4, hipMemcpyHostToDevice,
-                                 c.stream()));
-        gfz::EncodeResult er;
-        gfz::encode_with_rules(c, d_nodes.get(), n, r->lengths.data(), r->lengths.size(),
-                               d_first.get(), d_second.get(), rb.n_rules, layers.data(),
-                               layers.size(), res.delta_round, er);
-        const uint64_t first = res.n_trav;
-        gfz::resident_append(c, res, er.encoded.get(), er.n_encoded, er.lengths.data(),
-                             er.lengths.size());
-        std::lock_guard<std::mutex> lk(mu);
-        known_len.insert(known_len.end(), res.dec_len.begin() + first, res.dec_len.end());
-        r->first_new_id = first;
-        r->enc_lengths = er.lengths;
-        finish(r, Clock::now());
-      } catch (const gfz::Error &e) {
-        std::lock_guard<std::mutex> lk(mu);
-        r->status = e.code;
-        r->error = std::string(kServePrefix) + e.what();
-        ++rejected;
-        finish(r, Clock::now());
-      }
-    }
+    bool has_add = false;
+    for (const ReqPtr &r : batch)
+      has_add = has_add || r->add;
     // Extracts: validate per request, then one plan for all of them.
     std::vector<ReqPtr> ex, deferred;
     std::vector<uint64_t> ids;
     uint64_t off = 0;
     for (const ReqPtr &r : batch) {
-      if (r->add || r->done)
+      if (r->add)
         continue;
       bool ok = true;
       for (uint64_t id : r->ids)
         ok = ok && id < res.n_trav;
+      if (!ok && has_add) {
+        deferred.push_back(r);
+        continue;
+      }
       if (!ok) {
         std::lock_guard<std::mutex> lk(mu);
         r->status = GFZ_E_INVALID;
@@ -322,17 +306,53 @@ struct gfz_server {
         s.reqs.clear();
       }
     }
+    {
+      // The completion thread may take this slot's results while the adds run.
+      std::lock_guard<std::mutex> lk(mu);
+      if (launched) {
+        for (const ReqPtr &r : s.reqs)
+          r->stage = kCopying;
+        inflight.push_back(si);
+        cv_work.notify_all();
+      }
+    }
+    // Adds, in arrival order: encode with the stored rules (tables built once) and
+    // append; stream-ordered after the decode launched above.
+    for (const ReqPtr &r : batch) {
+      if (!r->add)
+        continue;
+      try {
+        const uint64_t n = r->nodes.size();
+        gfz::DBuf<int32_t> d_nodes(c.pool(), std::max<uint64_t>(n, 1));
+        if (n)
+          GFZ_HIP(hipMemcpyAsync(d_nodes.get(), r->nodes.data(), n * 4, hipMemcpyHostToDevice,
+                                 c.stream()));
+        gfz::EncodeResult er;
+        gfz::encode_with_tables(c, d_nodes.get(), n, r->lengths.data(), r->lengths.size(),
+                                enc_tables, res.delta_round, er);
+        const uint64_t first = res.n_trav;
+        gfz::resident_append(c, res, er.encoded.get(), er.n_encoded, er.lengths.data(),
+                             er.lengths.size());
+        std::lock_guard<std::mutex> lk(mu);
+        known_len.insert(known_len.end(), res.dec_len.begin() + first, res.dec_len.end());
+        r->first_new_id = first;
+        r->enc_lengths = er.lengths;
+        finish(r, Clock::now());
+      } catch (const gfz::Error &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        r->status = e.code;
+        r->error = std::string(kServePrefix) + e.what();
+        ++rejected;
+        finish(r, Clock::now());
+      }
+    }
     std::lock_guard<std::mutex> lk(mu);
     ++batches;
     for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) {
       (*it)->stage = kQueued;
       queue.push_front(*it);
     }
-    if (launched) {
-      for (const ReqPtr &r : s.reqs)
-        r->stage = kCopying;
-      inflight.push_back(si);
-    } else {
+    if (!launched) {
       for (const ReqPtr &r : s.reqs)  // empty extracts (no output)
         finish(r, Clock::now());
       s.reqs.clear();
@@ -413,6 +433,8 @@ int gfz_server_create(const gfz_server_config *cfg, const int32_t *rules_first,
     for (uint64_t l = 0; l < n_layers; ++l)
       s->layers.push_back({layers[l].start_id, layers[l].end_id, layers[l].row_offset});
     gfz::rulebook_build(c, s->d_first.get(), s->d_second.get(), n_rules, min_rule_id, s->rb);
+    gfz::encode_tables_build(c, s->d_first.get(), s->d_second.get(), n_rules,
+                             s->layers.data(), s->layers.size(), s->enc_tables);
     gfz::DBuf<int32_t> d_enc(c.pool(), std::max<uint64_t>(n_encoded, 1));
     if (n_encoded)
       GFZ_HIP(hipMemcpyAsync(d_enc.get(), encoded, n_encoded * 4, hipMemcpyHostToDevice,
