@@ -639,19 +639,39 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__
 constexpr uint32_t kBktSlots = 16384;   // LDS table of one bucket
 constexpr uint32_t kBktProbe = 512;
 constexpr uint32_t kBktThreads = 1024;
-constexpr uint32_t kMaxBuckets = 4096;
+constexpr uint32_t kMaxBuckets1 = 4096;  // buckets of a one-level partition
+constexpr uint32_t kFineBuckets = 128;   // second-level buckets per coarse bucket
+constexpr uint32_t kFineBits = 7;
+constexpr uint32_t kMaxBuckets = 32768;  // buckets of a two-level partition
 constexpr uint32_t kMaxParts = 2048;    // partition blocks (segments per bucket)
+constexpr uint32_t kL2Parts = 16;        // second-level blocks per coarse bucket
 constexpr uint32_t kRepStride = kBktSlots / 2;  // repeated keys listed per bucket
+constexpr uint32_t kDedupSlots = 2048;   // per-block set of keys already known repeated
 static_assert((uint64_t)kMaxBuckets * kBktSlots < (1ull << 32),
               "a round's bucket totals fit half of a 64-bit word");
+static_assert(kFineBuckets == 1u << kFineBits, "fine buckets are a power of two");
 
+// Partitioned keys: regions, each with a row of nb + 1 bucket starts. A key's
+// row bucket is (bucket_of(k, tmask) >> pshift) & (nb - 1). One-level rounds
+// have one set of regions whose rows split all tmask + 1 buckets; two-level
+// rounds first split by coarse bucket (the high bits), then each coarse bucket
+// c is split by fine bucket into regions c * count .. c * count + count - 1.
+// Counting bucket b then reads row bucket b & (nb - 1) of the regions of coarse
+// bucket b / nb.
 struct Parts {
-  unsigned long long *entries;  // block g's region at g * region
-  uint64_t region;              // per * kInsTile
+  unsigned long long *entries;  // region g at base[g] if base, else at g * region
+  uint64_t region;
+  const uint64_t *base;
   uint32_t *starts;             // row g at g * (nb + 1): bucket starts, then the total
-  uint32_t nb;                  // buckets (a power of two)
-  uint32_t count;               // partition blocks
+  uint32_t nb;                  // buckets per row (a power of two)
+  uint32_t count;               // regions per coarse bucket (all of them when one-level)
+  uint32_t tmask;               // the round's buckets - 1
+  uint32_t pshift;
 };
+
+__device__ inline uint32_t row_bucket(uint64_t k, const Parts &pt) {
+  return (bucket_of(k, pt.tmask) >> pt.pshift) & (pt.nb - 1);
+}
 
 
 // Exclusive scan of the block's bucket histogram in place; writes the block's
@@ -713,7 +733,6 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
                                                         uint64_t L, uint64_t span, uint64_t n_tiles,
                                                         uint32_t per, Parts pt) {
   extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
-  const uint32_t mask = pt.nb - 1;
   for (uint32_t i = threadIdx.x; i < pt.nb; i += kBlock)
     hist[i] = 0;
   __syncthreads();
@@ -725,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
 #pragma unroll
     for (uint32_t j = 0; j < kInsItems; ++j)
       if (kv[j] != kFree)
-        atomicAdd(&hist[bucket_of(kv[j] & ~kRep, mask)], 1u);
+        atomicAdd(&hist[row_bucket(kv[j] & ~kRep, pt)], 1u);
   }
   __syncthreads();
   part_starts(hist, pt);
@@ -736,38 +755,188 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
 #pragma unroll
     for (uint32_t j = 0; j < kInsItems; ++j)
       if (kv[j] != kFree)
-        dst[atomicAdd(&hist[bucket_of(kv[j] & ~kRep, mask)], 1u)] = kv[j];
+        dst[atomicAdd(&hist[row_bucket(kv[j] & ~kRep, pt)], 1u)] = kv[j];
   }
 }
 
-// The same partition of aggregated tiles (entries already carry kRep).
+// A block's entries spread over segments (aggregated tiles, or one coarse
+// bucket's segment of several regions), visited as one flat sequence: the
+// segments' lengths are scanned in LDS, kFlatSegs at a time, and every thread
+// takes entries kBlock apart, kFlatItems loads in flight, walking the prefix
+// forward to find each entry's segment (entries of a wave are consecutive, so
+// loads coalesce; segments are tens to hundreds of entries, so the walk is a
+// step or two). fn(entry, segment index, offset in the segment).
+constexpr uint32_t kFlatSegs = 1024;
+constexpr uint32_t kFlatItems = 4;
+struct FlatLds {
+  uint32_t pre[kFlatSegs + 1];
+  const unsigned long long *src[kFlatSegs];
+  uint32_t scan[kBlock / kWave + 1];
+};
+
+template <class Seg, class Fn>
+__device__ inline void for_flat_entries(FlatLds &fl, uint32_t nseg, Seg seg, Fn fn) {
+  constexpr uint32_t kPerThread = kFlatSegs / kBlock;
+  for (uint32_t c0 = 0; c0 < nseg; c0 += kFlatSegs) {
+    const uint32_t nt = nseg - c0 < kFlatSegs ? nseg - c0 : kFlatSegs;
+    uint32_t len[kPerThread], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPerThread; ++k) {
+      const uint32_t q = threadIdx.x * kPerThread + k;
+      len[k] = 0;
+      if (q < nt)
+        seg(c0 + q, fl.src[q], len[k]);
+      sum += len[k];
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan<kBlock>(sum, fl.scan, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kPerThread; ++k) {
+      fl.pre[threadIdx.x * kPerThread + k] = off;
+      off += len[k];
+    }
+    if (threadIdx.x == 0)
+      fl.pre[kFlatSegs] = total;
+    __syncthreads();
+    uint32_t cur = 0;  // this thread's segment, non-decreasing
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kBlock * kFlatItems) {
+      unsigned long long v[kFlatItems];
+      uint32_t sg[kFlatItems], at[kFlatItems];
+#pragma unroll
+      for (uint32_t j = 0; j < kFlatItems; ++j) {
+        const uint32_t i = i0 + j * kBlock < total ? i0 + j * kBlock : total - 1;
+        while (fl.pre[cur + 1] <= i)
+          ++cur;
+        sg[j] = cur;
+        at[j] = i - fl.pre[cur];
+        v[j] = fl.src[cur][at[j]];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kFlatItems; ++j)
+        if (i0 + j * kBlock < total)
+          fn(v[j], c0 + sg[j], at[j]);
+    }
+    __syncthreads();  // the prefix is rewritten for the next segments
+  }
+}
+
+// The same partition of aggregated tiles (entries already carry kRep). A key
+// already known repeated adds nothing after its first copy, so each block keeps
+// one copy of it: the first pass claims it in an LDS set (the claiming element's
+// index recorded), the second places only the claimant's copy. The hottest
+// keys, repeated in every tile, then reach their bucket once per block instead
+// of once per tile. A key the full set cannot take is kept in every copy; both
+// passes see the same set, so their counts agree.
+__device__ inline uint32_t dedup_slot(uint64_t k) {
+  return (uint32_t)(key_hash(k) >> 13) & (kDedupSlots - 1);
+}
+
 __global__ __launch_bounds__(kBlock) void k_bucket_part_tiles(
     const unsigned long long *__restrict__ tiles, const uint32_t *__restrict__ tiles_n,
     uint64_t n_tiles, uint32_t per, Parts pt) {
   extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
-  const uint32_t mask = pt.nb - 1;
+  __shared__ unsigned long long dkey[kDedupSlots];
+  __shared__ uint32_t didx[kDedupSlots];
+  __shared__ FlatLds fl;
   for (uint32_t i = threadIdx.x; i < pt.nb; i += kBlock)
     hist[i] = 0;
+  for (uint32_t i = threadIdx.x; i < kDedupSlots; i += kBlock)
+    dkey[i] = kFree;
   __syncthreads();
   const uint64_t t0 = (uint64_t)blockIdx.x * per;
   const uint64_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
-  for (uint64_t t = t0; t < t1; ++t) {
-    const uint32_t n = tiles_n[t];
-    const unsigned long long *src = tiles + t * kInsTile;
-    for (uint32_t e = threadIdx.x; e < n; e += kBlock)
-      atomicAdd(&hist[bucket_of(src[e] & ~kRep, mask)], 1u);
-  }
-  __syncthreads();
+  const uint32_t nseg = (uint32_t)(t1 - t0);
+  auto seg = [&](uint32_t q, const unsigned long long *&src, uint32_t &len) {
+    src = tiles + (t0 + q) * kInsTile;
+    len = tiles_n[t0 + q];
+  };
+  for_flat_entries(fl, nseg, seg, [&](unsigned long long k, uint32_t q, uint32_t at) {
+    if (k & kRep) {
+      const uint32_t me = q * kInsTile + at;
+      uint32_t sl = dedup_slot(k);
+      for (uint32_t p = 0; p < 16; ++p) {
+        const unsigned long long old = atomicCAS(&dkey[sl], (unsigned long long)kFree, k);
+        if (old == kFree) {
+          didx[sl] = me;
+          break;
+        }
+        if (old == k)
+          return;  // a later copy of a known-repeated key
+        sl = (sl + 1) & (kDedupSlots - 1);
+      }
+    }
+    atomicAdd(&hist[row_bucket(k & ~kRep, pt)], 1u);
+  });
   part_starts(hist, pt);
   unsigned long long *dst = pt.entries + (uint64_t)blockIdx.x * pt.region;
-  for (uint64_t t = t0; t < t1; ++t) {
-    const uint32_t n = tiles_n[t];
-    const unsigned long long *src = tiles + t * kInsTile;
-    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-      const unsigned long long k = src[e];
-      dst[atomicAdd(&hist[bucket_of(k & ~kRep, mask)], 1u)] = k;
+  for_flat_entries(fl, nseg, seg, [&](unsigned long long k, uint32_t q, uint32_t at) {
+    if (k & kRep) {
+      const uint32_t me = q * kInsTile + at;
+      uint32_t sl = dedup_slot(k);
+      for (uint32_t p = 0; p < 16; ++p) {
+        const unsigned long long cur = dkey[sl];
+        if (cur == k) {
+          if (didx[sl] != me)
+            return;
+          break;
+        }
+        if (cur == kFree)
+          break;
+        sl = (sl + 1) & (kDedupSlots - 1);
+      }
     }
+    dst[atomicAdd(&hist[row_bucket(k & ~kRep, pt)], 1u)] = k;
+  });
+}
+
+// Entries of a coarse bucket's level-one segments, per second-level block: block
+// g = c * p2count + j takes coarse bucket c of level-one regions
+// [j * P / p2count, (j + 1) * P / p2count).
+__global__ void k_part2_sizes(Parts p1, uint32_t p2count, uint32_t n_coarse,
+                              uint32_t *__restrict__ sizes) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (uint64_t)n_coarse * p2count)
+    return;
+  const uint32_t c = (uint32_t)(g / p2count), j = (uint32_t)(g % p2count);
+  const uint32_t P = p1.count;
+  const uint32_t r0 = (uint32_t)((uint64_t)j * P / p2count);
+  const uint32_t r1 = (uint32_t)((uint64_t)(j + 1) * P / p2count);
+  uint32_t n = 0;
+  for (uint32_t r = r0; r < r1; ++r) {
+    const uint32_t *row = p1.starts + (uint64_t)r * (p1.nb + 1);
+    n += row[c + 1] - row[c];
   }
+  sizes[g] = n;
+}
+
+// Second level of a two-level partition: block c * p2.count + j splits coarse
+// bucket c of its level-one regions by fine bucket into its own region (at
+// p2.base[block]), with a row of fine-bucket starts.
+__global__ __launch_bounds__(kBlock) void k_bucket_part2(Parts p1, Parts p2) {
+  extern __shared__ uint32_t hist[];  // one counter per fine bucket (dynamic LDS)
+  __shared__ FlatLds fl;
+  for (uint32_t i = threadIdx.x; i < p2.nb; i += kBlock)
+    hist[i] = 0;
+  __syncthreads();
+  const uint32_t c = blockIdx.x / p2.count, j = blockIdx.x % p2.count;
+  const uint32_t P = p1.count;
+  const uint32_t r0 = (uint32_t)((uint64_t)j * P / p2.count);
+  const uint32_t r1 = (uint32_t)((uint64_t)(j + 1) * P / p2.count);
+  auto seg = [&](uint32_t q, const unsigned long long *&src, uint32_t &len) {
+    const uint64_t r = r0 + q;
+    const uint32_t *row = p1.starts + r * (p1.nb + 1);
+    const uint32_t s0 = row[c];
+    len = row[c + 1] - s0;
+    src = p1.entries + r * p1.region + s0;
+  };
+  for_flat_entries(fl, r1 - r0, seg, [&](unsigned long long k, uint32_t, uint32_t) {
+    atomicAdd(&hist[row_bucket(k & ~kRep, p2)], 1u);
+  });
+  part_starts(hist, p2);
+  unsigned long long *dst = p2.entries + p2.base[blockIdx.x];
+  for_flat_entries(fl, r1 - r0, seg, [&](unsigned long long k, uint32_t, uint32_t) {
+    dst[atomicAdd(&hist[row_bucket(k & ~kRep, p2)], 1u)] = k;
+  });
 }
 
 // One block per bucket: its segment of every partition region, counted in an
@@ -776,7 +945,8 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
                                                               unsigned long long *__restrict__ rep,
                                                               uint2 *__restrict__ block_counts,
                                                               unsigned long long *__restrict__ sums,
-                                                              uint32_t *__restrict__ overflow) {
+                                                              uint32_t *__restrict__ overflow,
+                                                              uint32_t rep_cap) {
   __shared__ unsigned long long tab[kBktSlots];
   __shared__ uint32_t lds[kBktThreads / kWave + 1];
   constexpr uint32_t kPer = kBktSlots / kBktThreads;
@@ -802,22 +972,77 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
     }
     fail = true;
   };
-  // Segment g: [starts[g][b], starts[g][b + 1]) of region g; four entries are
-  // loaded together (clamped into the segment), then inserted.
+  // Segment g of the bucket: [starts[r][lb], starts[r][lb + 1]) of region
+  // r = coarse * count + g.
+  const uint32_t lb = b & (pt.nb - 1);
+  const uint64_t r0 = (uint64_t)(b / pt.nb) * pt.count;
   constexpr uint32_t kBatch = 4;
-  for (uint32_t g = threadIdx.x; g < pt.count; g += kBktThreads) {
-    const uint32_t *row = pt.starts + (uint64_t)g * (pt.nb + 1);
-    const uint32_t s0 = row[b], s1 = row[b + 1];
-    const unsigned long long *src = pt.entries + (uint64_t)g * pt.region;
-    for (uint32_t i = s0; i < s1; i += kBatch) {
+  if (pt.count <= kWave) {
+    // Few long segments (two-level rounds): every thread takes entries of the
+    // bucket's concatenated segments, found by a search of their prefix in LDS.
+    __shared__ uint32_t pre[kWave + 1];
+    __shared__ const unsigned long long *src_of[kWave];
+    if (threadIdx.x < kWave) {
+      const uint32_t g = threadIdx.x;
+      uint32_t len = 0;
+      if (g < pt.count) {
+        const uint64_t r = r0 + g;
+        const uint32_t *row = pt.starts + r * (pt.nb + 1);
+        const uint32_t s0 = row[lb];
+        len = row[lb + 1] - s0;
+        src_of[g] = pt.entries + (pt.base ? pt.base[r] : r * pt.region) + s0;
+      }
+      uint32_t x = len;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (g >= (uint32_t)d)
+          x += y;
+      }
+      pre[g + 1] = x;
+      if (g == 0)
+        pre[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = pre[pt.count];
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kBktThreads * kBatch) {
       uint64_t ev[kBatch];
 #pragma unroll
-      for (uint32_t j = 0; j < kBatch; ++j)
-        ev[j] = src[i + j < s1 ? i + j : s1 - 1];
+      for (uint32_t j = 0; j < kBatch; ++j) {
+        const uint32_t i = i0 + j * kBktThreads < total ? i0 + j * kBktThreads : total - 1;
+        uint32_t lo = 0, hi = pt.count;  // last g with pre[g] <= i
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (pre[mid] <= i)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        ev[j] = src_of[lo][i - pre[lo]];
+      }
 #pragma unroll
       for (uint32_t j = 0; j < kBatch; ++j)
-        if (i + j < s1)
+        if (i0 + j * kBktThreads < total)
           insert(ev[j]);
+    }
+  } else {
+    // Many short segments (one-level rounds): a thread per segment, four entries
+    // loaded together (clamped into the segment), then inserted.
+    for (uint32_t g = threadIdx.x; g < pt.count; g += kBktThreads) {
+      const uint64_t r = r0 + g;
+      const uint32_t *row = pt.starts + r * (pt.nb + 1);
+      const uint32_t s0 = row[lb], s1 = row[lb + 1];
+      const unsigned long long *src = pt.entries + (pt.base ? pt.base[r] : r * pt.region);
+      for (uint32_t i = s0; i < s1; i += kBatch) {
+        uint64_t ev[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j)
+          ev[j] = src[i + j < s1 ? i + j : s1 - 1];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j)
+          if (i + j < s1)
+            insert(ev[j]);
+      }
     }
   }
   // Counts, and the repeated keys compacted into the bucket's list.
@@ -845,7 +1070,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket_count(Parts pt,
   if (__syncthreads_or(fail) && threadIdx.x == 0)
     atomicExch(overflow, 1u);
   if (threadIdx.x == 0) {
-    if (total_r > kRepStride)
+    if (total_r > rep_cap)
       atomicExch(overflow, 1u);
     block_counts[b] = make_uint2(total_d, total_r);
     // Distinct keys (high half) and repeated keys (low half) in one add: each
@@ -1657,39 +1882,86 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     bool counted = false;
     uint64_t regions = 0, region_max = 0;  // bucket-counted: rule table by bucket regions
     // Rounds count in buckets while their distinct keys (the first round's
-    // estimated at L/32) fit kMaxBuckets buckets of kBktSlots / 2; larger rounds,
-    // and any round whose buckets overflow, count in the HBM table.
+    // estimated at L/32) fit kMaxBuckets buckets of kBktSlots / 2 (two-level
+    // partitions beyond kMaxBuckets1); larger rounds, and any round whose buckets
+    // overflow, count in the HBM table.
     const uint64_t est =
         round == 0 ? L / 32 : std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
-    const bool use_buckets = est <= (uint64_t)kMaxBuckets * (kBktSlots / 2);
+    // Tuning and test knobs (ctx tuned values; every setting gives the same
+    // grammar): "count_buckets" fixes the bucket count (a power of two, 64 ..
+    // kMaxBuckets), "count_rep_cap" lowers the repeated keys a bucket may list
+    // (a bucket above it sends the round to the HBM table).
+    const int forced_buckets = ctx.tuned("count_buckets");
+    const int forced_rep_cap = ctx.tuned("count_rep_cap");
+    const uint32_t rep_cap =
+        forced_rep_cap > 0 ? std::min<uint32_t>((uint32_t)forced_rep_cap, kRepStride) : kRepStride;
+    const bool use_buckets =
+        forced_buckets > 0 || est <= (uint64_t)kMaxBuckets * (kBktSlots / 2);
     if (use_buckets) {
-      const uint64_t nbk = std::min<uint64_t>(
-          kMaxBuckets, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
+      const uint64_t nbk =
+          forced_buckets > 0
+              ? std::min<uint64_t>(kMaxBuckets, next_pow2(std::max<uint64_t>(64, forced_buckets)))
+              : std::min<uint64_t>(kMaxBuckets,
+                                   next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
       // At most kMaxParts partition blocks, so a bucket gathers at most that many
-      // segments.
+      // segments. Beyond kMaxBuckets1 buckets the keys are split twice: by coarse
+      // bucket (the bucket's high bits) into the partition blocks' regions, then
+      // each coarse bucket by fine bucket into kL2Parts regions, so a bucket
+      // gathers kL2Parts long segments instead of kMaxParts short ones.
       const uint32_t per = (uint32_t)std::max<uint64_t>(1, ceil_div(n_ins, kMaxParts));
       const uint64_t n_parts = ceil_div(n_ins, per);
+      const bool two_level = nbk > kMaxBuckets1;
+      const uint64_t nb1 = two_level ? nbk >> kFineBits : nbk;
       DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
-      DBuf<uint32_t> starts(pool, n_parts * (nbk + 1));
+      DBuf<uint32_t> starts(pool, n_parts * (nb1 + 1));
       stage = DBuf<unsigned long long>(pool, nbk * kRepStride);
       fill_runs(ctx, {{sums.get(), 9, 0, 8}});
-      const Parts pt{part.get(), (uint64_t)per * kInsTile, starts.get(), (uint32_t)nbk,
-                     (uint32_t)n_parts};
+      const Parts p1{part.get(), (uint64_t)per * kInsTile, nullptr, starts.get(),
+                     (uint32_t)nb1, (uint32_t)n_parts, (uint32_t)(nbk - 1),
+                     two_level ? kFineBits : 0u};
+      const uint64_t n_l2 = two_level ? nb1 * kL2Parts : 0;
+      DBuf<uint32_t> l2_sizes, l2_starts;
+      DBuf<uint64_t> l2_base;
+      DBuf<unsigned long long> part2;
+      Parts pt = p1;
+      if (two_level) {
+        l2_sizes = DBuf<uint32_t>(pool, n_l2);
+        l2_base = DBuf<uint64_t>(pool, n_l2 + 1);
+        l2_starts = DBuf<uint32_t>(pool, n_l2 * (kFineBuckets + 1));
+      }
       ctx.timed("pair_insert", [&] {
         if (mostly_distinct) {
-          hipLaunchKernelGGL(k_bucket_part, dim3((uint32_t)n_parts), dim3(kBlock), nbk * 4,
-                             ctx.stream(), cur, heads.get(), L, span, n_ins, per, pt);
+          hipLaunchKernelGGL(k_bucket_part, dim3((uint32_t)n_parts), dim3(kBlock), nb1 * 4,
+                             ctx.stream(), cur, heads.get(), L, span, n_ins, per, p1);
         } else {
           // Hot keys: aggregate each tile first, so a key enters a bucket at most
           // once per tile.
           aggregate();
-          hipLaunchKernelGGL(k_bucket_part_tiles, dim3((uint32_t)n_parts), dim3(kBlock), nbk * 4,
-                             ctx.stream(), tiles.get(), tiles_n.get(), n_ins, per, pt);
+          hipLaunchKernelGGL(k_bucket_part_tiles, dim3((uint32_t)n_parts), dim3(kBlock), nb1 * 4,
+                             ctx.stream(), tiles.get(), tiles_n.get(), n_ins, per, p1);
         }
         GFZ_CHECK_LAUNCH();
+        if (two_level) {
+          hipLaunchKernelGGL(k_part2_sizes, dim3((uint32_t)ceil_div(n_l2, kBlock)), dim3(kBlock),
+                             0, ctx.stream(), p1, kL2Parts, (uint32_t)nb1, l2_sizes.get());
+          GFZ_CHECK_LAUNCH();
+          exclusive_scan(ctx, l2_sizes.get(), l2_base.get(), n_l2);
+          // The second level's regions hold the first level's entries: the
+          // aggregated tiles (read by then) have room for them.
+          unsigned long long *e2 = tiles.get();
+          if (!e2) {
+            part2 = DBuf<unsigned long long>(pool, n_parts * per * kInsTile);
+            e2 = part2.get();
+          }
+          pt = Parts{e2, 0, l2_base.get(), l2_starts.get(), kFineBuckets, kL2Parts,
+                     (uint32_t)(nbk - 1), 0u};
+          hipLaunchKernelGGL(k_bucket_part2, dim3((uint32_t)n_l2), dim3(kBlock),
+                             kFineBuckets * 4, ctx.stream(), p1, pt);
+          GFZ_CHECK_LAUNCH();
+        }
         hipLaunchKernelGGL(k_bucket_count, dim3((uint32_t)nbk), dim3(kBktThreads), 0,
                            ctx.stream(), pt, stage.get(), bcounts.get(), sums.get(),
-                           reinterpret_cast<uint32_t *>(sums.get() + 2));
+                           reinterpret_cast<uint32_t *>(sums.get() + 2), rep_cap);
         GFZ_CHECK_LAUNCH();
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
@@ -1705,6 +1977,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         cap = nbk * kBktSlots;
       } else {
         info.table_retries += 1;  // a bucket overflowed: count with the table instead
+        if (two_level && !mostly_distinct)
+          tiles = DBuf<unsigned long long>();  // its room held the second level: aggregate again
       }
     }
     if (!counted)

@@ -1,0 +1,73 @@
+"""Pair counting layouts on the GPU: every bucket count (one-level up to 4096
+buckets, two-level beyond), and the fall back to the HBM counting table when a
+bucket overflows (forced with a tiny repeated-key cap, also after a two-level
+partition has reused the aggregated tiles' memory), give the oracle's grammar bit
+for bit. The knobs are the ctx tuned values "count_buckets" and "count_rep_cap"."""
+import sys
+
+import numpy as np
+import pytest
+
+from gfaz_amd import codec, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def case():
+    cfg = synth.CONFIGS["plain_500k"].scaled(backbone=120_000, n_haps=24)
+    tr = synth.generate(cfg)
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=16)
+    return tr, ref
+
+
+def _check(ctx, tr, ref):
+    got = codec.compress_traversals(ctx, tr.nodes, tr.lengths, tr.n_paths, 8, 2, 1)
+    assert (got.start_id, got.end_id) == (ref["start_id"], ref["end_id"])
+    np.testing.assert_array_equal(got.encoded_lengths, ref["encoded_lengths"])
+    np.testing.assert_array_equal(got.encoded, ref["encoded"])
+    np.testing.assert_array_equal(got.rules_first, ref["rules_first"])
+    np.testing.assert_array_equal(got.rules_second, ref["rules_second"])
+
+
+@pytest.mark.parametrize("buckets", [64, 1024, 4096, 8192, 32768])
+def test_every_bucket_count_gives_the_oracle_grammar(gpu_ctx, case, buckets):
+    tr, ref = case
+    say(f"count buckets={buckets} ({tr.n_steps} steps)")
+    gpu_ctx.set_tuned("count_buckets", buckets)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("count_buckets", 0)
+
+
+@pytest.mark.parametrize("buckets", [1024, 16384])
+def test_bucket_overflow_falls_back_to_the_table(gpu_ctx, case, buckets):
+    tr, ref = case
+    say(f"count fallback buckets={buckets}")
+    gpu_ctx.set_tuned("count_buckets", buckets)
+    gpu_ctx.set_tuned("count_rep_cap", 1)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("count_buckets", 0)
+        gpu_ctx.set_tuned("count_rep_cap", 0)
+
+
+def test_walks_two_level(gpu_ctx):
+    """Uneven walks (rounds of mostly distinct pairs partition the raw stream)."""
+    cfg = synth.CONFIGS["walks_2m"].scaled(n_haps=60, len_max=200_000)
+    tr = synth.generate(cfg)
+    say(f"count walks two-level ({tr.n_steps} steps)")
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=16)
+    gpu_ctx.set_tuned("count_buckets", 16384)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("count_buckets", 0)
