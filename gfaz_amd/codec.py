@@ -99,6 +99,10 @@ _SIGNATURES = {
     "gfz_resident_load": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_uint32,
                            ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_resident_free": ([_vp], None),
+    "gfz_format_steps": ([_vp, ctypes.c_int, _vp, _u64, _vp, _u64, _vp, _vp, _vp, _vp,
+                          ctypes.c_uint32, _vp, _u64, ctypes.POINTER(_u64)], ctypes.c_int),
+    "gfz_parse_steps": ([_vp, ctypes.c_int, _vp, _u64, _vp, _u64, ctypes.c_uint32, _vp, _u64,
+                         _vp, ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_extract_noalloc": ([_vp, _vp, _vp, _u64, _vp, _u64, _vp, ctypes.POINTER(_u64)],
                             ctypes.c_int),
     "gfz_resident_append": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, ctypes.POINTER(_u64)],
@@ -690,3 +694,77 @@ def decompress_stream(ctx: Context, rb: "Rulebook", encoded, encoded_lengths,
     if errors:
         raise errors[0]
     _check(rc)
+
+
+# ---- P/W line step text --------------------------------------------------------------
+
+GFZ_TEXT_P, GFZ_TEXT_W = 0, 1
+
+
+def _concat(parts) -> tuple[bytes, np.ndarray]:
+    parts = [bytes(p) for p in parts]
+    off = np.zeros(len(parts) + 1, dtype=np.uint64)
+    if parts:
+        np.cumsum([len(p) for p in parts], dtype=np.uint64, out=off[1:])
+    return b"".join(parts), off
+
+
+def format_steps(ctx: Context, kind: int, nodes, lengths, heads, tails, out=None):
+    """Lines head_t + steps_t + tail_t (gfz_format_steps). nodes: host array or
+    DeviceArray; heads/tails: one bytes object per traversal. Returns the text as bytes,
+    or (with ``out`` a DeviceArray of uint8) the text size, written there."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(nodes, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        nptr, nn = nodes.ptr, nodes.n
+    else:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        nptr, nn = nodes.ctypes.data, nodes.size
+    hb, ho = _concat(heads)
+    tb, tof = _concat(tails)
+    hbuf = ctypes.create_string_buffer(hb, max(1, len(hb)))
+    tbuf = ctypes.create_string_buffer(tb, max(1, len(tb)))
+    n = _u64()
+    L = lib()
+    args = (ctx.handle, int(kind), nptr, nn, lengths.ctypes.data, lengths.size, hbuf,
+            ho.ctypes.data, tbuf, tof.ctypes.data)
+    if out is not None:
+        _check(L.gfz_format_steps(*args, flags | GFZ_OUTPUT_DEVICE, out.ptr, out.n,
+                                  ctypes.byref(n)))
+        return int(n.value)
+    _check(L.gfz_format_steps(*args, flags, None, 0, ctypes.byref(n)))
+    res = np.empty(max(1, int(n.value)), dtype=np.uint8)
+    _check(L.gfz_format_steps(*args, flags, res.ctypes.data, res.size, ctypes.byref(n)))
+    return res[: int(n.value)].tobytes()
+
+
+def parse_steps(ctx: Context, kind: int, text, line_off, out=None):
+    """Step lists text[line_off[l]:line_off[l + 1]] -> (nodes, lengths)
+    (gfz_parse_steps). text: bytes or a DeviceArray of uint8; with ``out`` (a
+    DeviceArray of int32) the nodes stay on the device and (n_nodes, lengths) is
+    returned."""
+    line_off = np.ascontiguousarray(line_off, dtype=np.uint64)
+    n_lines = line_off.size - 1
+    flags = 0
+    if isinstance(text, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        tptr, tn = text.ptr, text.n
+    else:
+        tbuf = np.frombuffer(bytes(text), dtype=np.uint8) if len(text) else np.zeros(1, np.uint8)
+        tptr, tn = tbuf.ctypes.data, len(text)
+    lengths = np.zeros(max(1, n_lines), dtype=np.uint32)
+    n = _u64()
+    L = lib()
+    if out is not None:
+        _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
+                                 flags | GFZ_OUTPUT_DEVICE, out.ptr, out.n, lengths.ctypes.data,
+                                 ctypes.byref(n)))
+        return int(n.value), lengths[:n_lines]
+    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
+                             flags, None, 0, lengths.ctypes.data, ctypes.byref(n)))
+    nodes = np.empty(max(1, int(n.value)), dtype=np.int32)
+    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
+                             flags, nodes.ctypes.data, nodes.size, lengths.ctypes.data,
+                             ctypes.byref(n)))
+    return nodes[: int(n.value)], lengths[:n_lines]

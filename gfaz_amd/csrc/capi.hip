@@ -7,6 +7,7 @@
 #include "encode.hpp"
 #include "prims.hpp"
 #include "resident.hpp"
+#include "text.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -658,6 +659,86 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
                           items < 0 ? 16u : (uint32_t)items);
     if (!(flags & GFZ_OUTPUT_DEVICE) && total) {
       GFZ_HIP(hipMemcpyAsync(out, d_out, total * 4, hipMemcpyDeviceToHost, c.stream()));
+      c.sync();
+    }
+  });
+}
+
+// ---- P/W line step text ------------------------------------------------------
+
+static_assert(gfz::kTextP == GFZ_TEXT_P && gfz::kTextW == GFZ_TEXT_W, "line kinds");
+
+int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_nodes,
+                     const uint32_t *lengths, uint64_t n_trav, const uint8_t *heads,
+                     const uint64_t *head_off, const uint8_t *tails, const uint64_t *tail_off,
+                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out) {
+  constexpr const char *kPrefix = "GFA writer error: ";
+  if (!ctx || (n_nodes && !nodes) || (n_trav && !lengths) || !head_off || !tail_off ||
+      (head_off[n_trav] && !heads) || (tail_off[n_trav] && !tails))
+    return fail(GFZ_E_INVALID, std::string(kPrefix) + "null argument");
+  return guarded(kPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_nodes;
+    stage_encoded(c, nodes, n_nodes, flags, staged, d_nodes);
+    gfz::FormatPlan plan;
+    gfz::format_plan(c, kind, d_nodes, n_nodes, lengths, n_trav, head_off, tail_off, plan);
+    const uint64_t total = plan.total;
+    if (n_out)
+      *n_out = total;
+    if (!out)
+      return;
+    if (total > out_cap)
+      throw gfz::Error(GFZ_E_INVALID, "output capacity " + std::to_string(out_cap) +
+                                          " < text size " + std::to_string(total));
+    if (flags & GFZ_OUTPUT_DEVICE) {
+      gfz::format_write(c, plan, heads, head_off, tails, tail_off, out, out_cap);
+    } else {
+      gfz::DBuf<uint8_t> d_out(c.pool(), std::max<uint64_t>(total, 1));
+      gfz::format_write(c, plan, heads, head_off, tails, tail_off, d_out.get(), total);
+      if (total)
+        GFZ_HIP(hipMemcpyAsync(out, d_out.get(), total, hipMemcpyDeviceToHost, c.stream()));
+      c.sync();
+    }
+  });
+}
+
+int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_bytes,
+                    const uint64_t *line_off, uint64_t n_lines, uint32_t flags, int32_t *nodes,
+                    uint64_t nodes_cap, uint32_t *lengths, uint64_t *n_nodes) {
+  constexpr const char *kPrefix = "GFA parser error: ";
+  if (!ctx || (n_bytes && !text) || !line_off)
+    return fail(GFZ_E_INVALID, std::string(kPrefix) + "null argument");
+  return guarded(kPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<uint8_t> staged;
+    const uint8_t *d_txt = text;
+    if (!(flags & GFZ_INPUT_DEVICE)) {
+      staged = gfz::DBuf<uint8_t>(c.pool(), std::max<uint64_t>(n_bytes, 1));
+      if (n_bytes)
+        GFZ_HIP(hipMemcpyAsync(staged.get(), text, n_bytes, hipMemcpyHostToDevice, c.stream()));
+      d_txt = staged.get();
+    }
+    gfz::ParsePlan plan;
+    gfz::parse_plan(c, kind, d_txt, n_bytes, line_off, n_lines, lengths, plan);
+    const uint64_t total = plan.total;
+    if (n_nodes)
+      *n_nodes = total;
+    if (!nodes)
+      return;
+    if (total > nodes_cap)
+      throw gfz::Error(GFZ_E_INVALID, "node capacity " + std::to_string(nodes_cap) + " < steps " +
+                                          std::to_string(total));
+    if (flags & GFZ_OUTPUT_DEVICE) {
+      gfz::parse_write(c, plan, nodes, nodes_cap);
+    } else {
+      gfz::DBuf<int32_t> d_nodes(c.pool(), std::max<uint64_t>(total, 1));
+      gfz::parse_write(c, plan, d_nodes.get(), total);
+      if (total)
+        GFZ_HIP(hipMemcpyAsync(nodes, d_nodes.get(), total * 4, hipMemcpyDeviceToHost,
+                               c.stream()));
       c.sync();
     }
   });

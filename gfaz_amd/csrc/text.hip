@@ -1,0 +1,566 @@
+// P- and W-line step text on gfx950: formatting decoded traversals as the step
+// lists of GFA P/W lines, and parsing those step lists back into signed node IDs.
+//
+// Formatting restates src/io/gfa_write_utils.cpp:238-290
+// (format_path_line_numeric: "P\t" name "\t" steps "\t" overlap "\n", steps
+// "12+,13-" with |id| in decimal and the orientation after it;
+// format_walk_line_numeric: "W\t" six fields "\t" steps "\n", steps ">12<13").
+// The caller supplies each line's bytes before and after the steps (its head and
+// tail), as the reference builds them from names and fields; the GPU writes the
+// steps and places the heads and tails. Lines are back to back.
+//
+// Parsing restates the numeric-name fast path of src/io/gfa_parser.cpp:676-790
+// with resolve_node_id -> parse_uint32 (:69-91, :1210-1219): a P token is the
+// text between commas, its last byte the orientation ('-' = reverse, anything
+// else forward) and the rest the name; a W token starts at '<' or '>' and runs to
+// the next one (bytes before the first are skipped). A name's value is its
+// leading decimal digits; no leading digit or a value past 2^32 - 1 is an error
+// (the reference throws), an empty name or the value 0 drops the step.
+//
+// Both directions work in tiles: one pass sizes every tile (bytes of its steps,
+// or its kept tokens), a scan places the tiles, and a second pass writes. Output
+// bytes are staged per tile in LDS and stored as whole 32-bit words where the
+// destination allows.
+#include "codec.hpp"
+#include "prims.hpp"
+#include "text.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace gfz {
+
+namespace {
+
+constexpr uint32_t kFmtItems = 8;                    // steps per thread
+constexpr uint32_t kFmtTile = kBlock * kFmtItems;    // steps per tile
+constexpr uint32_t kMaxStepBytes = 12;               // ',' + 10 digits + orientation
+constexpr uint32_t kParseItems = 16;                 // text bytes per thread
+constexpr uint32_t kParseTile = kBlock * kParseItems;
+
+__device__ inline uint32_t ndigits(uint32_t v) {
+  uint32_t d = 1;
+  uint32_t p = 10;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    d += v >= p ? 1u : 0u;
+    p = p <= 0xFFFFFFFFu / 10 ? p * 10 : 0xFFFFFFFFu;
+  }
+  return d;
+}
+
+// Bytes of step i: P "[,]digits(+|-)", W "(<|>)digits".
+__device__ inline uint32_t step_bytes(int kind, int32_t v, bool first) {
+  const uint32_t d = ndigits(uabs(v));
+  return kind == kTextP ? d + 1 + (first ? 0u : 1u) : d + 1;
+}
+
+// Writes step text at dst (dst[0 .. step_bytes)).
+__device__ inline void put_step(int kind, int32_t v, bool first, uint8_t *dst) {
+  const uint32_t a = uabs(v);
+  const uint32_t d = ndigits(a);
+  uint32_t o = 0;
+  if (kind == kTextP) {
+    if (!first)
+      dst[o++] = ',';
+  } else {
+    dst[o++] = v < 0 ? '<' : '>';
+  }
+  uint32_t x = a;
+  for (uint32_t k = d; k > 0; --k) {
+    dst[o + k - 1] = (uint8_t)('0' + x % 10u);
+    x /= 10u;
+  }
+  o += d;
+  if (kind == kTextP)
+    dst[o] = v < 0 ? '-' : '+';
+}
+
+__device__ inline bool bit_at(const uint32_t *heads, uint64_t i) {
+  return (heads[i >> 5] >> (i & 31)) & 1u;
+}
+
+// Sum of the step bytes of each tile.
+__global__ __launch_bounds__(kBlock) void k_fmt_tile_bytes(int kind, const int32_t *__restrict__ x,
+                                                           uint64_t n,
+                                                           const uint32_t *__restrict__ heads,
+                                                           uint32_t *__restrict__ tile_bytes) {
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  const uint64_t base = (uint64_t)blockIdx.x * kFmtTile + (uint64_t)threadIdx.x * kFmtItems;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kFmtItems; ++j) {
+    const uint64_t i = base + j;
+    if (i < n)
+      sum += step_bytes(kind, x[i], bit_at(heads, i));
+  }
+  uint32_t total;
+  block_exclusive_scan<kBlock>(sum, lds, total);
+  if (threadIdx.x == 0)
+    tile_bytes[blockIdx.x] = total;
+}
+
+// tpos[t] = byte position (in the steps-only stream) of traversal t's first step;
+// tpos[n_trav] = the total. One thread per traversal, summing from its tile start.
+__global__ void k_fmt_trav_pos(int kind, const int32_t *__restrict__ x, uint64_t n,
+                               const uint32_t *__restrict__ heads,
+                               const uint64_t *__restrict__ tstart, uint64_t n_trav,
+                               const uint64_t *__restrict__ tile_off,
+                               uint64_t *__restrict__ tpos) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_trav)
+    return;
+  const uint64_t i = tstart[t];
+  const uint64_t tile = i / kFmtTile;
+  uint64_t p = tile_off[tile];
+  for (uint64_t k = tile * kFmtTile; k < i && k < n; ++k)
+    p += step_bytes(kind, x[k], bit_at(heads, k));
+  tpos[t] = p;
+}
+
+// Steps of one tile formatted into LDS, then copied to their lines: a tile's
+// steps are contiguous in the output except where a traversal ends inside it
+// (its tail and the next line's head come in between).
+__global__ __launch_bounds__(kBlock) void k_fmt_write(
+    int kind, const int32_t *__restrict__ x, uint64_t n, const uint32_t *__restrict__ heads,
+    const uint64_t *__restrict__ tstart, uint64_t n_trav, const uint64_t *__restrict__ tile_off,
+    const uint64_t *__restrict__ tpos, const uint64_t *__restrict__ line_off,
+    const uint64_t *__restrict__ head_len, uint8_t *__restrict__ out) {
+  __shared__ uint8_t buf[kFmtTile * kMaxStepBytes];
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  __shared__ uint32_t s_total;
+  const uint64_t tile0 = (uint64_t)blockIdx.x * kFmtTile;
+  const uint64_t base = tile0 + (uint64_t)threadIdx.x * kFmtItems;
+  int32_t v[kFmtItems];
+  uint32_t first = 0, sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kFmtItems; ++j) {
+    const uint64_t i = base + j;
+    v[j] = i < n ? x[i] : 0;
+    const bool f = i < n && bit_at(heads, i);
+    first |= f ? 1u << j : 0u;
+    sum += i < n ? step_bytes(kind, v[j], f) : 0u;
+  }
+  uint32_t total;
+  uint32_t o = block_exclusive_scan<kBlock>(sum, lds, total);
+#pragma unroll
+  for (uint32_t j = 0; j < kFmtItems; ++j) {
+    if (base + j < n) {
+      const bool f = (first >> j) & 1;
+      put_step(kind, v[j], f, buf + o);
+      o += step_bytes(kind, v[j], f);
+    }
+  }
+  if (threadIdx.x == 0)
+    s_total = total;
+  __syncthreads();
+  // Copy out, one traversal piece at a time: steps [a, b) of the tile belong to
+  // traversal t, whose steps start at output byte line_off[t] + head_len[t].
+  const uint64_t tile_end = tile0 + kFmtTile < n ? tile0 + kFmtTile : n;
+  // First traversal with a step in the tile: last t with tstart[t] <= tile0 that
+  // has steps at or after tile0 (empty traversals are skipped by the search).
+  uint64_t lo = 0, hi = n_trav;  // tstart[n_trav] = n > tile0
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (tstart[mid] <= tile0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  uint64_t t = lo;
+  const uint64_t tile_pos = tile_off[blockIdx.x];
+  uint64_t a = tile0;
+  while (a < tile_end) {
+    while (tstart[t + 1] <= a)
+      ++t;
+    const uint64_t b = tstart[t + 1] < tile_end ? tstart[t + 1] : tile_end;
+    // Stream positions (bytes of all earlier steps) of the piece's ends: a piece
+    // starts at the tile or at a traversal start, and ends at the tile's end or at
+    // the next traversal start; LDS holds the tile from stream position tile_pos.
+    const uint64_t sa = a == tile0 ? tile_pos : tpos[t];
+    const uint64_t sb = b == tile_end ? tile_pos + s_total : tpos[t + 1];
+    const uint32_t la = (uint32_t)(sa - tile_pos), lb = (uint32_t)(sb - tile_pos);
+    // Output bytes [d, d + (lb - la)) where d = line start + head + (sa - tpos[t]).
+    const uint64_t d = line_off[t] + head_len[t] + (sa - tpos[t]);
+    const uint32_t len = lb - la;
+    // Byte edges up to 4-byte alignment of the destination, whole words between.
+    const uint32_t head_b = (uint32_t)((4 - (d & 3)) & 3) < len ? (uint32_t)((4 - (d & 3)) & 3) : len;
+    const uint32_t words = (len - head_b) / 4;
+    const uint32_t tail_b = len - head_b - words * 4;
+    for (uint32_t k = threadIdx.x; k < head_b; k += kBlock)
+      out[d + k] = buf[la + k];
+    uint32_t *dw = reinterpret_cast<uint32_t *>(out + d + head_b);
+    for (uint32_t w = threadIdx.x; w < words; w += kBlock) {
+      const uint32_t s = la + head_b + 4 * w;
+      dw[w] = (uint32_t)buf[s] | ((uint32_t)buf[s + 1] << 8) | ((uint32_t)buf[s + 2] << 16) |
+              ((uint32_t)buf[s + 3] << 24);
+    }
+    for (uint32_t k = threadIdx.x; k < tail_b; k += kBlock) {
+      const uint32_t s = la + head_b + 4 * words + k;
+      out[d + head_b + 4 * words + k] = buf[s];
+    }
+    a = b;
+  }
+}
+
+// Heads and tails of every line: one block per traversal.
+__global__ void k_fmt_heads(const uint8_t *__restrict__ heads_txt,
+                            const uint64_t *__restrict__ head_off,
+                            const uint8_t *__restrict__ tails_txt,
+                            const uint64_t *__restrict__ tail_off,
+                            const uint64_t *__restrict__ line_off,
+                            const uint64_t *__restrict__ tpos, uint8_t *__restrict__ out) {
+  const uint64_t t = blockIdx.x;
+  const uint64_t hl = head_off[t + 1] - head_off[t];
+  const uint64_t tl = tail_off[t + 1] - tail_off[t];
+  const uint64_t d = line_off[t];
+  for (uint64_t k = threadIdx.x; k < hl; k += blockDim.x)
+    out[d + k] = heads_txt[head_off[t] + k];
+  const uint64_t d2 = d + hl + (tpos[t + 1] - tpos[t]);
+  for (uint64_t k = threadIdx.x; k < tl; k += blockDim.x)
+    out[d2 + k] = tails_txt[tail_off[t] + k];
+}
+
+// ---- parsing ---------------------------------------------------------------
+
+// Line of byte p: last l with line_off[l] <= p (line_off non-decreasing, n_lines + 1).
+__device__ inline uint64_t line_of(const uint64_t *line_off, uint64_t n_lines, uint64_t p) {
+  uint64_t lo = 0, hi = n_lines;
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (line_off[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// The token starting at byte p of a line ending at e (p < e), if one starts there.
+// Returns 0 (none), 1 (a step: *id signed) or 2 (a dropped step: empty name or
+// value 0); sets *err to 1 (no digits) or 2 (overflow) on a malformed name.
+__device__ inline int parse_token(int kind, const uint8_t *txt, uint64_t s, uint64_t p, uint64_t e,
+                                  int32_t *id, uint32_t *err) {
+  uint64_t name_b, name_e;
+  bool rev;
+  if (kind == kTextP) {
+    if (txt[p] == ',' || (p > s && txt[p - 1] != ','))
+      return 0;
+    uint64_t q = p;
+    while (q < e && txt[q] != ',')
+      ++q;
+    // token [p, q), q > p: orientation txt[q - 1], name [p, q - 1)
+    rev = txt[q - 1] == '-';
+    name_b = p;
+    name_e = q - 1;
+  } else {
+    const uint8_t c = txt[p];
+    if (c != '<' && c != '>')
+      return 0;
+    uint64_t q = p + 1;
+    while (q < e && txt[q] != '<' && txt[q] != '>')
+      ++q;
+    rev = c == '<';
+    name_b = p + 1;
+    name_e = q;
+  }
+  if (name_e <= name_b)
+    return 2;
+  uint32_t val = 0;
+  bool any = false;
+  for (uint64_t q = name_b; q < name_e; ++q) {
+    const uint8_t c = txt[q];
+    if (c < '0' || c > '9')
+      break;
+    any = true;
+    const uint32_t dg = c - '0';
+    if (val > 0xFFFFFFFFu / 10 || (val == 0xFFFFFFFFu / 10 && dg > 0xFFFFFFFFu % 10)) {
+      atomicOr(err, 2u);
+      return 2;
+    }
+    val = val * 10 + dg;
+  }
+  if (!any) {
+    atomicOr(err, 1u);
+    return 2;
+  }
+  if (val == 0)
+    return 2;
+  *id = (int32_t)(rev ? 0u - val : val);  // the reference's uint32 -> NodeId conversion
+  return 1;
+}
+
+// Kept steps per tile of text bytes.
+__global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t *__restrict__ txt,
+                                                        uint64_t n_bytes,
+                                                        const uint64_t *__restrict__ line_off,
+                                                        uint64_t n_lines,
+                                                        uint32_t *__restrict__ tile_count,
+                                                        uint32_t *__restrict__ err) {
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  const uint64_t p0 = (uint64_t)blockIdx.x * kParseTile + (uint64_t)threadIdx.x * kParseItems;
+  uint32_t c = 0;
+  if (p0 < n_bytes) {
+    uint64_t l = line_of(line_off, n_lines, p0);
+    for (uint32_t j = 0; j < kParseItems; ++j) {
+      const uint64_t p = p0 + j;
+      if (p >= n_bytes)
+        break;
+      while (line_off[l + 1] <= p)
+        ++l;
+      int32_t id;
+      c += parse_token(kind, txt, line_off[l], p, line_off[l + 1], &id, err) == 1 ? 1u : 0u;
+    }
+  }
+  uint32_t total;
+  block_exclusive_scan<kBlock>(c, lds, total);
+  if (threadIdx.x == 0)
+    tile_count[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t *__restrict__ txt,
+                                                        uint64_t n_bytes,
+                                                        const uint64_t *__restrict__ line_off,
+                                                        uint64_t n_lines,
+                                                        const uint64_t *__restrict__ tile_pos,
+                                                        int32_t *__restrict__ nodes,
+                                                        uint32_t *__restrict__ err) {
+  __shared__ uint32_t lds[kBlock / kWave + 1];
+  const uint64_t p0 = (uint64_t)blockIdx.x * kParseTile + (uint64_t)threadIdx.x * kParseItems;
+  int32_t ids[kParseItems];
+  uint32_t keep = 0, c = 0;
+  if (p0 < n_bytes) {
+    uint64_t l = line_of(line_off, n_lines, p0);
+    for (uint32_t j = 0; j < kParseItems; ++j) {
+      const uint64_t p = p0 + j;
+      if (p >= n_bytes)
+        break;
+      while (line_off[l + 1] <= p)
+        ++l;
+      if (parse_token(kind, txt, line_off[l], p, line_off[l + 1], &ids[j], err) == 1) {
+        keep |= 1u << j;
+        ++c;
+      }
+    }
+  }
+  uint32_t total;
+  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>(c, lds, total);
+  for (uint32_t j = 0; j < kParseItems; ++j)
+    if ((keep >> j) & 1)
+      nodes[o++] = ids[j];
+}
+
+// Kept steps before each line start: one thread per line, counting from its tile.
+__global__ void k_parse_line_pos(int kind, const uint8_t *__restrict__ txt, uint64_t n_bytes,
+                                 const uint64_t *__restrict__ line_off, uint64_t n_lines,
+                                 const uint64_t *__restrict__ tile_pos, uint64_t *__restrict__ lpos,
+                                 uint32_t *__restrict__ err) {
+  const uint64_t li = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li > n_lines)
+    return;
+  const uint64_t p = line_off[li];
+  const uint64_t tile = p / kParseTile;
+  uint64_t c = tile_pos[tile];
+  uint64_t q = tile * kParseTile;
+  if (q < p) {
+    uint64_t l = line_of(line_off, n_lines, q);
+    for (; q < p; ++q) {
+      while (line_off[l + 1] <= q)
+        ++l;
+      int32_t id;
+      c += parse_token(kind, txt, line_off[l], q, line_off[l + 1], &id, err) == 1 ? 1u : 0u;
+    }
+  }
+  lpos[li] = c;
+}
+
+uint32_t grid1(uint64_t n, uint32_t per) {
+  return (uint32_t)std::max<uint64_t>(1, ceil_div(n, (uint64_t)per));
+}
+
+}  // namespace
+
+void format_plan(Ctx &ctx, int kind, const int32_t *d_nodes, uint64_t n,
+                 const uint32_t *lengths, uint64_t n_trav, const uint64_t *head_off,
+                 const uint64_t *tail_off, FormatPlan &plan) {
+  if (kind != kTextP && kind != kTextW)
+    throw Error(kErrInvalid, "line kind must be P or W");
+  plan = FormatPlan();
+  plan.kind = kind;
+  plan.nodes = d_nodes;
+  plan.n = n;
+  plan.n_trav = n_trav;
+  std::vector<uint64_t> tstart(n_trav + 1, 0), starts;
+  for (uint64_t t = 0; t < n_trav; ++t) {
+    if (lengths[t])
+      starts.push_back(tstart[t]);
+    tstart[t + 1] = tstart[t] + lengths[t];
+  }
+  if (tstart[n_trav] != n)
+    throw Error(kErrInvalid, "sum of traversal lengths (" + std::to_string(tstart[n_trav]) +
+                                 ") != number of steps (" + std::to_string(n) + ")");
+  if (head_off[0] != 0 || tail_off[0] != 0)
+    throw Error(kErrInvalid, "head and tail offsets must start at 0");
+  for (uint64_t t = 0; t < n_trav; ++t)
+    if (head_off[t + 1] < head_off[t] || tail_off[t + 1] < tail_off[t])
+      throw Error(kErrInvalid, "head and tail offsets must be non-decreasing");
+  Pool &pool = ctx.pool();
+  plan.n_tiles = ceil_div(std::max<uint64_t>(n, 1), kFmtTile);
+  const uint64_t words = ceil_div(std::max<uint64_t>(n, 1), 32);
+  plan.heads = DBuf<uint32_t>(pool, words);
+  plan.tstart = DBuf<uint64_t>(pool, n_trav + 1);
+  plan.tile_off = DBuf<uint64_t>(pool, plan.n_tiles + 1);
+  plan.tpos = DBuf<uint64_t>(pool, n_trav + 1);
+  DBuf<uint64_t> d_starts(pool, std::max<size_t>(starts.size(), 1));
+  DBuf<uint32_t> tile_bytes(pool, plan.n_tiles);
+  fill_u32(ctx, plan.heads.get(), 0, words);
+  GFZ_HIP(hipMemcpyAsync(plan.tstart.get(), tstart.data(), (n_trav + 1) * 8,
+                         hipMemcpyHostToDevice, ctx.stream()));
+  if (!starts.empty()) {
+    GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
+                           hipMemcpyHostToDevice, ctx.stream()));
+    set_head_bits(ctx, d_starts.get(), starts.size(), plan.heads.get(), n);
+  }
+  if (n) {
+    hipLaunchKernelGGL(k_fmt_tile_bytes, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0,
+                       ctx.stream(), kind, d_nodes, n, plan.heads.get(), tile_bytes.get());
+    GFZ_CHECK_LAUNCH();
+  }
+  exclusive_scan(ctx, tile_bytes.get(), plan.tile_off.get(), n ? plan.n_tiles : 0);
+  hipLaunchKernelGGL(k_fmt_trav_pos, dim3(grid1(n_trav + 1, kBlock)), dim3(kBlock), 0,
+                     ctx.stream(), kind, d_nodes, n, plan.heads.get(), plan.tstart.get(), n_trav,
+                     plan.tile_off.get(), plan.tpos.get());
+  GFZ_CHECK_LAUNCH();
+  std::vector<uint64_t> h_tpos(n_trav + 1);
+  GFZ_HIP(hipMemcpyAsync(h_tpos.data(), plan.tpos.get(), (n_trav + 1) * 8, hipMemcpyDeviceToHost,
+                         ctx.stream()));
+  ctx.sync();  // also: the pageable tstart/starts copies are done
+  plan.line_off.assign(n_trav + 1, 0);
+  plan.head_len.resize(n_trav);
+  for (uint64_t t = 0; t < n_trav; ++t) {
+    plan.head_len[t] = head_off[t + 1] - head_off[t];
+    plan.line_off[t + 1] = plan.line_off[t] + plan.head_len[t] + (h_tpos[t + 1] - h_tpos[t]) +
+                           (tail_off[t + 1] - tail_off[t]);
+  }
+  plan.total = plan.line_off[n_trav];
+}
+
+void format_write(Ctx &ctx, const FormatPlan &plan, const uint8_t *heads_txt,
+                  const uint64_t *head_off, const uint8_t *tails_txt, const uint64_t *tail_off,
+                  uint8_t *d_out, uint64_t out_cap) {
+  const uint64_t n_trav = plan.n_trav, n = plan.n;
+  if (plan.total > out_cap)
+    throw Error(kErrInvalid, "output capacity " + std::to_string(out_cap) + " < text size " +
+                                 std::to_string(plan.total));
+  if (!n_trav)
+    return;
+  Pool &pool = ctx.pool();
+  DBuf<uint64_t> d_line(pool, n_trav + 1), d_hlen(pool, n_trav), d_hoff(pool, n_trav + 1),
+      d_toff(pool, n_trav + 1);
+  const uint64_t hbytes = head_off[n_trav], tbytes = tail_off[n_trav];
+  DBuf<uint8_t> d_htxt(pool, std::max<uint64_t>(hbytes, 1)),
+      d_ttxt(pool, std::max<uint64_t>(tbytes, 1));
+  GFZ_HIP(hipMemcpyAsync(d_line.get(), plan.line_off.data(), (n_trav + 1) * 8,
+                         hipMemcpyHostToDevice, ctx.stream()));
+  GFZ_HIP(hipMemcpyAsync(d_hlen.get(), plan.head_len.data(), n_trav * 8, hipMemcpyHostToDevice,
+                         ctx.stream()));
+  GFZ_HIP(hipMemcpyAsync(d_hoff.get(), head_off, (n_trav + 1) * 8, hipMemcpyHostToDevice,
+                         ctx.stream()));
+  GFZ_HIP(hipMemcpyAsync(d_toff.get(), tail_off, (n_trav + 1) * 8, hipMemcpyHostToDevice,
+                         ctx.stream()));
+  if (hbytes)
+    GFZ_HIP(hipMemcpyAsync(d_htxt.get(), heads_txt, hbytes, hipMemcpyHostToDevice, ctx.stream()));
+  if (tbytes)
+    GFZ_HIP(hipMemcpyAsync(d_ttxt.get(), tails_txt, tbytes, hipMemcpyHostToDevice, ctx.stream()));
+  ctx.timed("format", [&] {
+    if (n) {
+      hipLaunchKernelGGL(k_fmt_write, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0, ctx.stream(),
+                         plan.kind, plan.nodes, n, plan.heads.get(), plan.tstart.get(), n_trav,
+                         plan.tile_off.get(), plan.tpos.get(), d_line.get(), d_hlen.get(), d_out);
+      GFZ_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_fmt_heads, dim3((uint32_t)n_trav), dim3(64), 0, ctx.stream(),
+                       d_htxt.get(), d_hoff.get(), d_ttxt.get(), d_toff.get(), d_line.get(),
+                       plan.tpos.get(), d_out);
+    GFZ_CHECK_LAUNCH();
+  });
+  ctx.add_bytes("format", 4.0 * n + (double)plan.total);
+  ctx.sync();  // the pageable line tables go out of scope
+}
+
+void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
+                const uint64_t *line_off, uint64_t n_lines, uint32_t *lengths, ParsePlan &plan) {
+  if (kind != kTextP && kind != kTextW)
+    throw Error(kErrInvalid, "line kind must be P or W");
+  if (line_off[0] != 0 || line_off[n_lines] != n_bytes)
+    throw Error(kErrInvalid, "line offsets must run from 0 to the text size");
+  for (uint64_t l = 0; l < n_lines; ++l)
+    if (line_off[l + 1] < line_off[l])
+      throw Error(kErrInvalid, "line offsets must be non-decreasing");
+  plan = ParsePlan();
+  plan.kind = kind;
+  plan.txt = d_txt;
+  plan.n_bytes = n_bytes;
+  plan.n_lines = n_lines;
+  Pool &pool = ctx.pool();
+  plan.n_tiles = ceil_div(std::max<uint64_t>(n_bytes, 1), kParseTile);
+  plan.line_off = DBuf<uint64_t>(pool, n_lines + 1);
+  plan.tile_pos = DBuf<uint64_t>(pool, plan.n_tiles + 1);
+  DBuf<uint64_t> lpos(pool, n_lines + 1);
+  DBuf<uint32_t> tile_cnt(pool, plan.n_tiles), err(pool, 1);
+  fill_u32(ctx, err.get(), 0, 1);
+  GFZ_HIP(hipMemcpyAsync(plan.line_off.get(), line_off, (n_lines + 1) * 8, hipMemcpyHostToDevice,
+                         ctx.stream()));
+  const bool any = n_bytes && n_lines;
+  if (any) {
+    hipLaunchKernelGGL(k_parse_count, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0, ctx.stream(),
+                       kind, d_txt, n_bytes, plan.line_off.get(), n_lines, tile_cnt.get(),
+                       err.get());
+    GFZ_CHECK_LAUNCH();
+  }
+  exclusive_scan(ctx, tile_cnt.get(), plan.tile_pos.get(), any ? plan.n_tiles : 0);
+  if (n_lines) {
+    hipLaunchKernelGGL(k_parse_line_pos, dim3(grid1(n_lines + 1, kBlock)), dim3(kBlock), 0,
+                       ctx.stream(), kind, d_txt, n_bytes, plan.line_off.get(), n_lines,
+                       plan.tile_pos.get(), lpos.get(), err.get());
+    GFZ_CHECK_LAUNCH();
+  }
+  std::vector<uint64_t> h_lpos(n_lines + 1, 0);
+  if (n_lines)
+    GFZ_HIP(hipMemcpyAsync(h_lpos.data(), lpos.get(), (n_lines + 1) * 8, hipMemcpyDeviceToHost,
+                           ctx.stream()));
+  ctx.stage_scalars(err.get(), 1);
+  ctx.sync();
+  const uint32_t e = (uint32_t)ctx.scalars()[0];
+  if (e & 2u)
+    throw Error(kErrInvalid, "parse_uint32: overflow");
+  if (e & 1u)
+    throw Error(kErrInvalid, "parse_uint32: no digits");
+  plan.total = n_lines ? h_lpos[n_lines] : 0;
+  for (uint64_t l = 0; l < n_lines; ++l) {
+    const uint64_t c = h_lpos[l + 1] - h_lpos[l];
+    if (c > 0xFFFFFFFFull)
+      throw Error(kErrInvalid, "a line holds more than 2^32 - 1 steps");
+    if (lengths)
+      lengths[l] = (uint32_t)c;
+  }
+}
+
+void parse_write(Ctx &ctx, const ParsePlan &plan, int32_t *d_nodes, uint64_t nodes_cap) {
+  if (plan.total > nodes_cap)
+    throw Error(kErrInvalid, "node capacity " + std::to_string(nodes_cap) + " < steps " +
+                                 std::to_string(plan.total));
+  if (plan.n_bytes && plan.n_lines) {
+    DBuf<uint32_t> err(ctx.pool(), 1);  // every malformed name was reported by the plan
+    ctx.timed("parse", [&] {
+      hipLaunchKernelGGL(k_parse_write, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0,
+                         ctx.stream(), plan.kind, plan.txt, plan.n_bytes, plan.line_off.get(),
+                         plan.n_lines, plan.tile_pos.get(), d_nodes, err.get());
+    });
+    GFZ_CHECK_LAUNCH();
+  }
+  ctx.add_bytes("parse", (double)plan.n_bytes + 4.0 * plan.total);
+  ctx.sync();
+}
+
+}  // namespace gfz

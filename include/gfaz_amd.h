@@ -335,6 +335,33 @@ void gfz_result_free(gfz_result *r);
 int gfz_server_stats(gfz_server *s, gfz_serve_stats *out);
 int gfz_server_reset_stats(gfz_server *s);
 
+/* ---- P/W line step text ------------------------------------------------------
+ * gfz_format_steps writes traversals as GFA P or W lines, line t = head_t +
+ * steps_t + tail_t, lines back to back: the step lists of
+ * format_path_line_numeric (kind GFZ_TEXT_P: "12+,13-", src/io/
+ * gfa_write_utils.cpp:238-260) and format_walk_line_numeric (GFZ_TEXT_W:
+ * ">12<13", :262-290), formatted on the GPU; head_t / tail_t are the caller's
+ * bytes around them ("P\tname\t" / "\t*\n", "W\tsample\thap\tseq\tstart\tend\t" /
+ * "\n"), given as concatenated bytes with n_trav + 1 offsets each. nodes: host
+ * or device (GFZ_INPUT_DEVICE); out: host or device (GFZ_OUTPUT_DEVICE) of
+ * out_cap bytes, or NULL to only size the text (*n_out).
+ * gfz_parse_steps is the inverse for the reference parser's numeric-name path
+ * (parse_p_line / parse_w_line with parse_uint32, src/io/gfa_parser.cpp:69-91,
+ * 676-790, 1210-1219): the step-list field of each line,
+ * text[line_off[l] .. line_off[l + 1]), into signed node IDs (steps with an empty
+ * name or the value 0 are dropped, as there; a name without leading digits or
+ * past 2^32 - 1 fails with the reference's "parse_uint32: ..." message).
+ * text: host or device (GFZ_INPUT_DEVICE); nodes: host or device
+ * (GFZ_OUTPUT_DEVICE), or NULL to only count; lengths (host, n_lines) may be NULL. */
+enum { GFZ_TEXT_P = 0, GFZ_TEXT_W = 1 };
+int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_nodes,
+                     const uint32_t *lengths, uint64_t n_trav, const uint8_t *heads,
+                     const uint64_t *head_off, const uint8_t *tails, const uint64_t *tail_off,
+                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out);
+int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_bytes,
+                    const uint64_t *line_off, uint64_t n_lines, uint32_t flags, int32_t *nodes,
+                    uint64_t nodes_cap, uint32_t *lengths, uint64_t *n_nodes);
+
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before
  * serialisation, producing the same bytes, so a caller can fill CompressedData

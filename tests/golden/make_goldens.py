@@ -210,6 +210,74 @@ def add_synth_case(tmp: str) -> None:
         "appended_sha256": sha(new.nodes)})
 
 
+# Step lists the reference parser must read (numeric segment names, so the
+# parse_uint32 fast path, src/io/gfa_parser.cpp:1210-1219): empty tokens, a lone
+# orientation, the value 0, names with trailing non-digits, a missing orientation,
+# leading junk before a W step, and an ID past 2^31.
+EDGE_TEXT = (
+    "H\tVN:Z:1.1\n"
+    "S\t1\tA\n"
+    "S\t2\tC\n"
+    "S\t3\tG\n"
+    "S\t4\tT\n"
+    "P\tp1\t1+,,2-,+,0+,3x-,4\t*\n"
+    "P\tp2\t12+,2+,1-\t3M,2M\n"
+    "P\tp3\t4+,3+,4+,3+\t*\n"
+    "W\ts1\t0\tc1\t*\t*\tjunk>1<2>0<3y>4>\n"
+    "W\ts2\t1\tc2\t5\t9\t>4>3>4>3<2\n"
+    "W\ts3\t2\tc3\t*\t*\t>3000000000<2>1\n"
+)
+EDGE_TEXT_ERRORS = {
+    "no_digits": "H\tVN:Z:1.1\nS\t1\tA\nP\tp\t1+,x+\t*\n",
+    "overflow": "H\tVN:Z:1.1\nS\t1\tA\nW\ts\t0\tc\t*\t*\t>1>4294967296\n",
+}
+
+
+def step_lines(gfa_text: str) -> str:
+    return "".join(l for l in gfa_text.splitlines(keepends=True) if l[:2] in ("P\t", "W\t"))
+
+
+def lines_cases(tmp: str) -> None:
+    """P/W line text written by the reference's direct writer
+    (write_gfa_from_compressed_data), for the step formatter and parser: the
+    example and CLI fixtures, a small synthetic pangenome, and edge-case step lists
+    (their input lines and the reference's lines after a round trip), plus the
+    parser's error messages."""
+    def decoded_lines(gfaz: str) -> str:
+        out = os.path.join(tmp, "decoded.gfa")
+        run("decompress", gfaz, out, "1")
+        with open(out) as f:
+            return step_lines(f.read())
+    for name in ("example_r8_t2_d1", "cli_fixture_r8_t2_d1"):
+        with open(os.path.join(HERE, name.split("_r8")[0] + "_lines.txt"), "w") as f:
+            f.write(decoded_lines(os.path.join(HERE, name + ".gfaz")))
+    cfg = synth.CONFIGS["plain_20k"].scaled(backbone=3000, n_haps=6)
+    tr = synth.generate(cfg)
+    gfa = os.path.join(tmp, "small.gfa")
+    synth.write_gfa(cfg, tr, gfa)
+    gz = os.path.join(tmp, "small.gfaz")
+    compress_case(gfa, gz, 8, 2, 1)
+    with open(os.path.join(HERE, "plain_3k_lines.txt"), "w") as f:
+        f.write(decoded_lines(gz))
+    edge = os.path.join(tmp, "edge_text.gfa")
+    with open(edge, "w") as f:
+        f.write(EDGE_TEXT)
+    gz = os.path.join(tmp, "edge_text.gfaz")
+    compress_case(edge, gz, 8, 2, 1)
+    errors = {}
+    for label, text in EDGE_TEXT_ERRORS.items():
+        bad = os.path.join(tmp, f"bad_{label}.gfa")
+        with open(bad, "w") as f:
+            f.write(text)
+        res = run("compress", bad, os.path.join(tmp, "bad.gfaz"), "8", "2", "1", "1",
+                  check=False)
+        errors[label] = {"rc": res.returncode, "stderr": res.stderr.strip().splitlines()[-1:]}
+    with open(os.path.join(HERE, "edge_text_lines.json"), "w") as f:
+        json.dump({"input": step_lines(EDGE_TEXT), "reference_output": decoded_lines(gz),
+                   "errors": errors}, f, indent=1)
+    print("wrote *_lines.txt, edge_text_lines.json", flush=True)
+
+
 def main() -> None:
     oracle.build_reference()
     tmp = tempfile.mkdtemp(prefix="gfaz_goldens_")
@@ -225,6 +293,7 @@ def main() -> None:
         walks = synth.CONFIGS["walks_2m"].scaled(backbone=30_000, n_haps=40, len_max=20_000)
         synth_case("walks_30k", walks, tmp, [(8, 2, 1)])
         add_synth_case(tmp)
+        lines_cases(tmp)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
