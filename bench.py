@@ -144,6 +144,11 @@ def cpu_baseline(cfg, gfa_bytes_full: int, threads: int, ctx) -> dict:
         cc = (_ms_after(err, "delta_transform (x1)") or 0.0) + \
             (_ms_after(err, "run_grammar_compression") or 0.0)
         legacy = err[err.index("[CPU Legacy] ==="):] if "[CPU Legacy] ===" in err else ""
+        writer = err[err.index("[CPU Direct-Writer] ==="):] if "[CPU Direct-Writer] ===" in err \
+            else ""
+        fmt_ms = sum(_ms_after(writer, k) or 0.0
+                     for k in ("expand and stream paths", "expand and stream walks"))
+        parse_ms = _ms_after(err, "[GfaParser] parse P/W-lines")
         cd = sum(_ms_after(legacy, k) or 0.0 for k in
                  ("delta decode rule columns", "expand path traversals",
                   "inverse delta traversals", "expand walk traversals"))
@@ -174,6 +179,11 @@ def cpu_baseline(cfg, gfa_bytes_full: int, threads: int, ctx) -> dict:
                 "codec_scope": "reference stage timers: delta_transform + "
                                "run_grammar_compression; delta decode rule columns + expand "
                                "+ inverse delta (in-memory decoder)",
+                "text_stages": {"expand_and_format_paths_s": fmt_ms / 1e3,
+                                "parse_pw_lines_s": (parse_ms or 0.0) / 1e3,
+                                "scope": "reference stage timers: direct writer's expand and "
+                                         "stream paths/walks (decode + P/W text to /dev/null); "
+                                         "parser's parse P/W-lines"},
                 "gfaz_bytes": gfaz_bytes, "ratio": sbytes / gfaz_bytes,
                 "gpu_on_sample": dict(gpu, ratio=sbytes / gpu_gfaz,
                                       traversal_bytes=gpu_trav,
@@ -188,6 +198,42 @@ def cpu_baseline(cfg, gfa_bytes_full: int, threads: int, ctx) -> dict:
     return {"value": sbytes / (t2 - t0) / 1e9, "unit": "GB/s", "cores": threads,
             "kind": "port", "sample": f"{what}: traversal codec only",
             "compress_s": t1 - t0, "decompress_s": t2 - t1, "gpu_on_sample": gpu}
+
+
+def text_throughput(ctx, codec, d_nodes, tr, reps: int = 3) -> dict:
+    """The decoded traversals (in HBM) written as P-lines (gfz_format_steps, device in and
+    out), then the step fields parsed back in place (gfz_parse_steps); best of reps,
+    each call whole (sizing pass included). Checks the first traversal of the round trip."""
+    n = tr.lengths.size
+    heads = [f"P\thap{h}\t".encode() for h in range(n)]
+    tails = [b"\t*\n"] * n
+    size, line_off = codec.format_size(ctx, 0, d_nodes, tr.lengths, heads, tails)
+    d_txt = codec.DeviceArray(ctx, size, np.uint8)
+    hl = np.array([len(h) for h in heads], np.uint64)
+    tl = np.array([len(t) for t in tails], np.uint64)
+    fields = (line_off[:-1] + hl, line_off[1:] - tl)
+    d_back = codec.DeviceArray(ctx, tr.n_steps, np.int32)
+
+    def best(fn):
+        b = None
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            b = dt if b is None else min(b, dt)
+        return b
+
+    tf = best(lambda: codec.format_steps(ctx, 0, d_nodes, tr.lengths, heads, tails, out=d_txt))
+    tp = best(lambda: codec.parse_steps(ctx, 0, d_txt, out=d_back, fields=fields))
+    first = int(tr.lengths[0])
+    ok = bool(np.array_equal(d_back.to_host(first), tr.nodes[:first]))
+    d_txt.free()
+    d_back.free()
+    return {"text_bytes": size, "format_s": tf, "format_gbps": size / tf / 1e9,
+            "parse_s": tp, "parse_gbps": size / tp / 1e9, "first_traversal_ok": ok,
+            "scope": "P-lines of every traversal, device-resident in and out, whole calls"}
 
 
 def extract_latency(ctx, codec, comp, tr, n_req: int, rate: float, seed: int = 11) -> dict:
@@ -657,6 +703,8 @@ def main() -> int:
         if not same:
             log("[bench] ORACLE MISMATCH")
             return 1
+    log("[bench] P-line text")
+    text = text_throughput(ctx, codec, d_out, tr)
     extract = None
     if args.extract_requests > 0:
         log(f"[bench] extract latency: {args.extract_requests} requests at "
@@ -712,6 +760,7 @@ def main() -> int:
         "wall_s_timed_region": t_total,
         "roofline": roofline, "expand_roofline": expand_roof,
         "extract": extract,
+        "text": text,
         "pcie_inclusive": {"h2d_input_s": h2d_s, "d2h_output_s": d2h_s,
                            "compress_gbps": gfa_bytes / (tc / steps + h2d_s) / 1e9,
                            "decompress_gbps": gfa_bytes / (td / steps + d2h_s) / 1e9,

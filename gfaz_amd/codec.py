@@ -86,6 +86,11 @@ _SIGNATURES = {
     "gfz_ctx_set_autotune": ([_vp, ctypes.c_int], ctypes.c_int),
     "gfz_ctx_tuned": ([_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "gfz_ctx_set_tuned": ([_vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "gfz_autotune_compress": ([_vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)],
+                              ctypes.c_int),
+    "gfz_ctx_save_tuning": ([_vp, ctypes.c_char_p], ctypes.c_int),
+    "gfz_ctx_load_tuning": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "gfz_dev_alloc": ([_vp, _u64, ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_dev_free": ([_vp, _vp], ctypes.c_int),
     "gfz_memcpy_h2d": ([_vp, _vp, _vp, _u64], ctypes.c_int),
@@ -100,9 +105,9 @@ _SIGNATURES = {
                            ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_resident_free": ([_vp], None),
     "gfz_format_steps": ([_vp, ctypes.c_int, _vp, _u64, _vp, _u64, _vp, _vp, _vp, _vp,
-                          ctypes.c_uint32, _vp, _u64, ctypes.POINTER(_u64)], ctypes.c_int),
-    "gfz_parse_steps": ([_vp, ctypes.c_int, _vp, _u64, _vp, _u64, ctypes.c_uint32, _vp, _u64,
-                         _vp, ctypes.POINTER(_u64)], ctypes.c_int),
+                          ctypes.c_uint32, _vp, _u64, ctypes.POINTER(_u64), _vp], ctypes.c_int),
+    "gfz_parse_steps": ([_vp, ctypes.c_int, _vp, _u64, _vp, _vp, _u64, ctypes.c_uint32, _vp,
+                         _u64, _vp, ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_extract_noalloc": ([_vp, _vp, _vp, _u64, _vp, _u64, _vp, ctypes.POINTER(_u64)],
                             ctypes.c_int),
     "gfz_resident_append": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_uint32, ctypes.POINTER(_u64)],
@@ -248,6 +253,30 @@ class Context:
 
     def set_tuned(self, key: str, value: int) -> None:
         _check(lib().gfz_ctx_set_tuned(self.handle, key.encode(), int(value)))
+
+    def autotune_compress(self, nodes, lengths, rounds: int = 8, threshold: int = 2,
+                          delta_round: int = 1) -> float:
+        """Tune the pair-counting knobs on this input (gfz_autotune_compress); returns
+        the fastest setting's compress time in ms."""
+        lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+        flags = 0
+        if isinstance(nodes, DeviceArray):
+            flags |= GFZ_INPUT_DEVICE
+            nptr, nn = nodes.ptr, nodes.n
+        else:
+            nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+            nptr, nn = nodes.ctypes.data, nodes.size
+        ms = ctypes.c_double()
+        _check(lib().gfz_autotune_compress(self.handle, nptr, nn, lengths.ctypes.data,
+                                           lengths.size, rounds, threshold, delta_round, flags,
+                                           ctypes.byref(ms)))
+        return float(ms.value)
+
+    def save_tuning(self, path: str) -> None:
+        _check(lib().gfz_ctx_save_tuning(self.handle, path.encode()))
+
+    def load_tuning(self, path: str) -> None:
+        _check(lib().gfz_ctx_load_tuning(self.handle, path.encode()))
 
     def calibration_copy(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int) -> float:
         """Plain device copy of nbytes; returns its duration in ms."""
@@ -709,10 +738,12 @@ def _concat(parts) -> tuple[bytes, np.ndarray]:
     return b"".join(parts), off
 
 
-def format_steps(ctx: Context, kind: int, nodes, lengths, heads, tails, out=None):
+def format_steps(ctx: Context, kind: int, nodes, lengths, heads, tails, out=None,
+                 with_offsets: bool = False):
     """Lines head_t + steps_t + tail_t (gfz_format_steps). nodes: host array or
     DeviceArray; heads/tails: one bytes object per traversal. Returns the text as bytes,
-    or (with ``out`` a DeviceArray of uint8) the text size, written there."""
+    or (with ``out`` a DeviceArray of uint8) the text size, written there; with
+    ``with_offsets`` also the n_trav + 1 line starts."""
     lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
     flags = 0
     if isinstance(nodes, DeviceArray):
@@ -726,26 +757,60 @@ def format_steps(ctx: Context, kind: int, nodes, lengths, heads, tails, out=None
     hbuf = ctypes.create_string_buffer(hb, max(1, len(hb)))
     tbuf = ctypes.create_string_buffer(tb, max(1, len(tb)))
     n = _u64()
+    line_off = np.zeros(lengths.size + 1, dtype=np.uint64)
     L = lib()
     args = (ctx.handle, int(kind), nptr, nn, lengths.ctypes.data, lengths.size, hbuf,
             ho.ctypes.data, tbuf, tof.ctypes.data)
     if out is not None:
         _check(L.gfz_format_steps(*args, flags | GFZ_OUTPUT_DEVICE, out.ptr, out.n,
-                                  ctypes.byref(n)))
-        return int(n.value)
-    _check(L.gfz_format_steps(*args, flags, None, 0, ctypes.byref(n)))
+                                  ctypes.byref(n), line_off.ctypes.data))
+        return (int(n.value), line_off) if with_offsets else int(n.value)
+    _check(L.gfz_format_steps(*args, flags, None, 0, ctypes.byref(n), None))
     res = np.empty(max(1, int(n.value)), dtype=np.uint8)
-    _check(L.gfz_format_steps(*args, flags, res.ctypes.data, res.size, ctypes.byref(n)))
-    return res[: int(n.value)].tobytes()
+    _check(L.gfz_format_steps(*args, flags, res.ctypes.data, res.size, ctypes.byref(n),
+                              line_off.ctypes.data))
+    text = res[: int(n.value)].tobytes()
+    return (text, line_off) if with_offsets else text
 
 
-def parse_steps(ctx: Context, kind: int, text, line_off, out=None):
-    """Step lists text[line_off[l]:line_off[l + 1]] -> (nodes, lengths)
-    (gfz_parse_steps). text: bytes or a DeviceArray of uint8; with ``out`` (a
+def format_size(ctx: Context, kind: int, nodes, lengths, heads, tails):
+    """Text size and line starts of gfz_format_steps, without writing the text."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(nodes, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        nptr, nn = nodes.ptr, nodes.n
+    else:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        nptr, nn = nodes.ctypes.data, nodes.size
+    hb, ho = _concat(heads)
+    tb, tof = _concat(tails)
+    hbuf = ctypes.create_string_buffer(hb, max(1, len(hb)))
+    tbuf = ctypes.create_string_buffer(tb, max(1, len(tb)))
+    n = _u64()
+    line_off = np.zeros(lengths.size + 1, dtype=np.uint64)
+    _check(lib().gfz_format_steps(ctx.handle, int(kind), nptr, nn, lengths.ctypes.data,
+                                  lengths.size, hbuf, ho.ctypes.data, tbuf, tof.ctypes.data,
+                                  flags, None, 0, ctypes.byref(n), line_off.ctypes.data))
+    return int(n.value), line_off
+
+
+def parse_steps(ctx: Context, kind: int, text, line_off=None, out=None, fields=None):
+    """Step fields -> (nodes, lengths) (gfz_parse_steps). The fields are either
+    text[line_off[l]:line_off[l + 1]] (contiguous) or, with ``fields=(begin, end)``,
+    text[begin[l]:end[l]]. text: bytes or a DeviceArray of uint8; with ``out`` (a
     DeviceArray of int32) the nodes stay on the device and (n_nodes, lengths) is
     returned."""
-    line_off = np.ascontiguousarray(line_off, dtype=np.uint64)
-    n_lines = line_off.size - 1
+    if fields is None:
+        line_off = np.ascontiguousarray(line_off, dtype=np.uint64)
+        fb = np.ascontiguousarray(line_off[:-1])
+        fe = np.ascontiguousarray(line_off[1:])
+    else:
+        fb = np.ascontiguousarray(fields[0], dtype=np.uint64)
+        fe = np.ascontiguousarray(fields[1], dtype=np.uint64)
+    n_lines = fb.size
+    fbp = fb.ctypes.data if n_lines else None
+    fep = fe.ctypes.data if n_lines else None
     flags = 0
     if isinstance(text, DeviceArray):
         flags |= GFZ_INPUT_DEVICE
@@ -757,14 +822,14 @@ def parse_steps(ctx: Context, kind: int, text, line_off, out=None):
     n = _u64()
     L = lib()
     if out is not None:
-        _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
+        _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, fbp, fep, n_lines,
                                  flags | GFZ_OUTPUT_DEVICE, out.ptr, out.n, lengths.ctypes.data,
                                  ctypes.byref(n)))
         return int(n.value), lengths[:n_lines]
-    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
-                             flags, None, 0, lengths.ctypes.data, ctypes.byref(n)))
+    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, fbp, fep, n_lines, flags, None,
+                             0, lengths.ctypes.data, ctypes.byref(n)))
     nodes = np.empty(max(1, int(n.value)), dtype=np.int32)
-    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, line_off.ctypes.data, n_lines,
-                             flags, nodes.ctypes.data, nodes.size, lengths.ctypes.data,
+    _check(L.gfz_parse_steps(ctx.handle, int(kind), tptr, tn, fbp, fep, n_lines, flags,
+                             nodes.ctypes.data, nodes.size, lengths.ctypes.data,
                              ctypes.byref(n)))
     return nodes[: int(n.value)], lengths[:n_lines]

@@ -9,6 +9,9 @@
 #include "resident.hpp"
 #include "text.hpp"
 
+#include <cctype>
+#include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -161,6 +164,170 @@ int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value) {
     return fail(GFZ_E_INVALID, "null argument");
   ctx->ctx->set_tuned(key, value);
   return GFZ_OK;
+}
+
+
+int gfz_autotune_compress(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                          const uint32_t *lengths, uint64_t n_trav, int rounds, int threshold,
+                          int delta_round, uint32_t flags, double *best_ms) {
+  if (!ctx || (n_nodes && !nodes) || (n_trav && !lengths))
+    return fail(GFZ_E_INVALID, std::string(kCompressPrefix) + "null argument");
+  return guarded(kCompressPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_nodes = nodes;
+    if (!(flags & GFZ_INPUT_DEVICE)) {
+      staged = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(n_nodes, 1));
+      if (n_nodes)
+        GFZ_HIP(hipMemcpyAsync(staged.get(), nodes, n_nodes * 4, hipMemcpyHostToDevice,
+                               c.stream()));
+      d_nodes = staged.get();
+    }
+    gfz::CompressParams p;
+    p.rounds = rounds;
+    p.threshold = threshold;
+    p.delta_round = delta_round;
+    std::vector<gfz::TuneTrial> trials;
+    const double ms = gfz::autotune_compress(c, d_nodes, n_nodes, lengths, n_trav, p, trials);
+    if (best_ms)
+      *best_ms = ms;
+  });
+}
+
+int gfz_ctx_save_tuning(gfz_ctx *ctx, const char *path) {
+  if (!ctx || !path)
+    return fail(GFZ_E_INVALID, std::string(kRuntimePrefix) + "null argument");
+  return guarded(kRuntimePrefix, [&] {
+    const gfz::Ctx &c = *ctx->ctx;
+    std::string js = "{\"arch\": \"" + c.arch() + "\", \"cus\": " + std::to_string(c.num_cus()) +
+                     ", \"tuned\": {";
+    bool first = true;
+    for (const auto &kv : c.tuned_all()) {
+      js += (first ? "" : ", ") + std::string("\"") + kv.first + "\": " + std::to_string(kv.second);
+      first = false;
+    }
+    js += "}}\n";
+    FILE *f = std::fopen(path, "wb");
+    if (!f)
+      throw gfz::Error(GFZ_E_INVALID, std::string("cannot write ") + path);
+    const bool ok = std::fwrite(js.data(), 1, js.size(), f) == js.size();
+    if (std::fclose(f) != 0 || !ok)
+      throw gfz::Error(GFZ_E_INVALID, std::string("cannot write ") + path);
+  });
+}
+
+namespace {
+// Reader of the flat JSON that gfz_ctx_save_tuning writes: "arch", "cus" and a
+// "tuned" object of integer values whose keys hold no quotes or backslashes.
+struct TuningFile {
+  std::string arch;
+  long long cus = -1;
+  std::vector<std::pair<std::string, int>> tuned;
+};
+
+bool parse_tuning(const std::string &s, TuningFile &out) {
+  size_t i = 0;
+  auto ws = [&] {
+    while (i < s.size() && std::isspace((unsigned char)s[i]))
+      ++i;
+  };
+  auto lit = [&](char ch) {
+    ws();
+    if (i < s.size() && s[i] == ch) {
+      ++i;
+      return true;
+    }
+    return false;
+  };
+  auto str = [&](std::string &v) {
+    if (!lit('"'))
+      return false;
+    const size_t e = s.find('"', i);
+    if (e == std::string::npos || s.find('\\', i) < e)
+      return false;
+    v = s.substr(i, e - i);
+    i = e + 1;
+    return true;
+  };
+  auto num = [&](long long &v) {
+    ws();
+    size_t used = 0;
+    try {
+      v = std::stoll(s.substr(i, 24), &used);
+    } catch (...) {
+      return false;
+    }
+    i += used;
+    return used > 0;
+  };
+  if (!lit('{'))
+    return false;
+  bool more = !lit('}');
+  while (more) {
+    std::string key;
+    if (!str(key) || !lit(':'))
+      return false;
+    if (key == "arch") {
+      if (!str(out.arch))
+        return false;
+    } else if (key == "cus") {
+      if (!num(out.cus))
+        return false;
+    } else if (key == "tuned") {
+      if (!lit('{'))
+        return false;
+      bool m = !lit('}');
+      while (m) {
+        std::string k;
+        long long v;
+        if (!str(k) || !lit(':') || !num(v) || v < INT32_MIN || v > INT32_MAX)
+          return false;
+        out.tuned.emplace_back(k, (int)v);
+        if (lit(','))
+          continue;
+        if (!lit('}'))
+          return false;
+        m = false;
+      }
+    } else {
+      return false;
+    }
+    if (lit(','))
+      continue;
+    if (!lit('}'))
+      return false;
+    more = false;
+  }
+  ws();
+  return i == s.size();
+}
+}  // namespace
+
+int gfz_ctx_load_tuning(gfz_ctx *ctx, const char *path) {
+  if (!ctx || !path)
+    return fail(GFZ_E_INVALID, std::string(kRuntimePrefix) + "null argument");
+  return guarded(kRuntimePrefix, [&] {
+    FILE *f = std::fopen(path, "rb");
+    if (!f)
+      throw gfz::Error(GFZ_E_INVALID, std::string("cannot read ") + path);
+    std::string s;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0)
+      s.append(buf, n);
+    std::fclose(f);
+    TuningFile tf;
+    if (!parse_tuning(s, tf))
+      throw gfz::Error(GFZ_E_INVALID, std::string("malformed tuning file ") + path);
+    gfz::Ctx &c = *ctx->ctx;
+    if (tf.arch != c.arch() || tf.cus != c.num_cus())
+      throw gfz::Error(GFZ_E_INVALID, "tuning file is for " + tf.arch + " with " +
+                                          std::to_string(tf.cus) + " CUs, this device is " +
+                                          c.arch() + " with " + std::to_string(c.num_cus()));
+    for (const auto &kv : tf.tuned)
+      c.set_tuned(kv.first, kv.second);
+  });
 }
 
 int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out) {
@@ -671,7 +838,8 @@ static_assert(gfz::kTextP == GFZ_TEXT_P && gfz::kTextW == GFZ_TEXT_W, "line kind
 int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_nodes,
                      const uint32_t *lengths, uint64_t n_trav, const uint8_t *heads,
                      const uint64_t *head_off, const uint8_t *tails, const uint64_t *tail_off,
-                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out) {
+                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out,
+                     uint64_t *line_off) {
   constexpr const char *kPrefix = "GFA writer error: ";
   if (!ctx || (n_nodes && !nodes) || (n_trav && !lengths) || !head_off || !tail_off ||
       (head_off[n_trav] && !heads) || (tail_off[n_trav] && !tails))
@@ -687,6 +855,8 @@ int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_no
     const uint64_t total = plan.total;
     if (n_out)
       *n_out = total;
+    if (line_off)
+      std::memcpy(line_off, plan.line_off.data(), (n_trav + 1) * 8);
     if (!out)
       return;
     if (total > out_cap)
@@ -705,10 +875,11 @@ int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_no
 }
 
 int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_bytes,
-                    const uint64_t *line_off, uint64_t n_lines, uint32_t flags, int32_t *nodes,
-                    uint64_t nodes_cap, uint32_t *lengths, uint64_t *n_nodes) {
+                    const uint64_t *field_begin, const uint64_t *field_end, uint64_t n_lines,
+                    uint32_t flags, int32_t *nodes, uint64_t nodes_cap, uint32_t *lengths,
+                    uint64_t *n_nodes) {
   constexpr const char *kPrefix = "GFA parser error: ";
-  if (!ctx || (n_bytes && !text) || !line_off)
+  if (!ctx || (n_bytes && !text) || (n_lines && (!field_begin || !field_end)))
     return fail(GFZ_E_INVALID, std::string(kPrefix) + "null argument");
   return guarded(kPrefix, [&] {
     gfz::Ctx &c = *ctx->ctx;
@@ -722,7 +893,7 @@ int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_byte
       d_txt = staged.get();
     }
     gfz::ParsePlan plan;
-    gfz::parse_plan(c, kind, d_txt, n_bytes, line_off, n_lines, lengths, plan);
+    gfz::parse_plan(c, kind, d_txt, n_bytes, field_begin, field_end, n_lines, lengths, plan);
     const uint64_t total = plan.total;
     if (n_nodes)
       *n_nodes = total;

@@ -42,6 +42,19 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t n_nodes,
                      const uint32_t *lengths, uint64_t n_trav, const CompressParams &p,
                      CompressResult &out);
 
+// Autotuning of the pair-counting knobs on one input: every combination of
+// "count_bucket_scale" (1, 2, 4) and "count_agg_slots" (2048, 4096) for the
+// input's size class is run twice; each must give the first one's grammar
+// (else kErrCorrupt), and the fastest is kept in the ctx's tuned table. Returns
+// the fastest compress time (ms) and fills `trials` with (scale, slots, ms).
+struct TuneTrial {
+  int scale, slots;
+  double ms;
+};
+double autotune_compress(Ctx &ctx, const int32_t *d_nodes, uint64_t n_nodes,
+                         const uint32_t *lengths, uint64_t n_trav, const CompressParams &p,
+                         std::vector<TuneTrial> &trials);
+
 // Device-resident rulebook for expansion: per rule {first, second, |first|, |second|}
 // (expanded sizes of the two children) plus the rule-ID window.
 struct RuleRec {

@@ -21,6 +21,7 @@
 
 
 #include <algorithm>
+#include <chrono>
 
 namespace gfz {
 
@@ -1863,7 +1864,11 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint32_t> tiles_n;
     // Tiles of few distinct keys (under a quarter of the last round's pairs were
     // distinct) aggregate in the smaller LDS table.
-    const bool few_keys = round == 0 || 4 * prev_distinct < prev_len;
+    // The aggregation table (2048 or 4096 slots) follows the last round's share of
+    // distinct pairs unless the tuned "count_agg_slots/<size class>" fixes it.
+    const int tuned_agg = ctx.tuned("count_agg_slots/" + std::to_string(Ctx::size_class(N)));
+    const bool few_keys = tuned_agg > 0 ? tuned_agg <= (int)kInsTile
+                                        : round == 0 || 4 * prev_distinct < prev_len;
     auto aggregate = [&] {
       tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
       tiles_n = DBuf<uint32_t>(pool, n_ins);
@@ -1892,6 +1897,9 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // kMaxBuckets), "count_rep_cap" lowers the repeated keys a bucket may list
     // (a bucket above it sends the round to the HBM table).
     const int forced_buckets = ctx.tuned("count_buckets");
+    // "count_bucket_scale/<size class>": 1, 2 or 4 halves, keeps or doubles the
+    // estimated bucket count (the autotuner's knob).
+    const int bucket_scale = ctx.tuned("count_bucket_scale/" + std::to_string(Ctx::size_class(N)));
     const int forced_rep_cap = ctx.tuned("count_rep_cap");
     const uint32_t rep_cap =
         forced_rep_cap > 0 ? std::min<uint32_t>((uint32_t)forced_rep_cap, kRepStride) : kRepStride;
@@ -1901,8 +1909,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       const uint64_t nbk =
           forced_buckets > 0
               ? std::min<uint64_t>(kMaxBuckets, next_pow2(std::max<uint64_t>(64, forced_buckets)))
-              : std::min<uint64_t>(kMaxBuckets,
-                                   next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))));
+              : std::min<uint64_t>(
+                    kMaxBuckets,
+                    std::max<uint64_t>(
+                        64, next_pow2(std::max<uint64_t>(64, ceil_div(est, kBktSlots / 2))) *
+                                (bucket_scale == 1 || bucket_scale == 4 ? bucket_scale : 2) /
+                                2));
       // At most kMaxParts partition blocks, so a bucket gathers at most that many
       // segments. Beyond kMaxBuckets1 buckets the keys are split twice: by coarse
       // bucket (the bucket's high bits) into the partition blocks' regions, then
@@ -2211,6 +2223,70 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   out.rules_first = std::move(rf);
   out.rules_second = std::move(rs);
   ctx.sync();
+}
+
+double autotune_compress(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_t *lengths,
+                         uint64_t T, const CompressParams &p, std::vector<TuneTrial> &trials) {
+  const std::string cls = std::to_string(Ctx::size_class(N));
+  const std::string k_scale = "count_bucket_scale/" + cls, k_slots = "count_agg_slots/" + cls;
+  const int old_scale = ctx.tuned(k_scale), old_slots = ctx.tuned(k_slots);
+  trials.clear();
+  CompressResult first;
+  bool have_first = false;
+  double best = -1;
+  int best_scale = 2, best_slots = (int)kInsTile;
+  try {
+    for (int scale : {2, 1, 4}) {
+      for (int slots : {(int)kInsTile, 2 * (int)kInsTile}) {
+        ctx.set_tuned(k_scale, scale);
+        ctx.set_tuned(k_slots, slots);
+        double ms = -1;
+        for (int rep = 0; rep < 2; ++rep) {
+          CompressResult r;
+          ctx.sync();
+          const auto t0 = std::chrono::steady_clock::now();
+          compress_device(ctx, d_nodes, N, lengths, T, p, r);
+          ctx.sync();
+          const double dt =
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                  .count();
+          ms = ms < 0 ? dt : std::min(ms, dt);
+          if (!have_first) {
+            first = std::move(r);
+            have_first = true;
+          } else if (rep == 0) {
+            const bool same =
+                r.n_encoded == first.n_encoded && r.n_rules == first.n_rules &&
+                r.start_id == first.start_id && r.end_id == first.end_id &&
+                r.encoded_lengths == first.encoded_lengths &&
+                count_mismatch_i32(ctx, r.encoded.get(), first.encoded.get(), r.n_encoded) == 0 &&
+                count_mismatch_i32(ctx, r.rules_first.get(), first.rules_first.get(),
+                                   r.n_rules) == 0 &&
+                count_mismatch_i32(ctx, r.rules_second.get(), first.rules_second.get(),
+                                   r.n_rules) == 0;
+            if (!same)
+              throw Error(kErrCorrupt, "autotune: counting setting (scale " +
+                                           std::to_string(scale) + ", slots " +
+                                           std::to_string(slots) +
+                                           ") gave a different grammar (internal error)");
+          }
+        }
+        trials.push_back({scale, slots, ms});
+        if (best < 0 || ms < best) {
+          best = ms;
+          best_scale = scale;
+          best_slots = slots;
+        }
+      }
+    }
+  } catch (...) {
+    ctx.set_tuned(k_scale, old_scale);
+    ctx.set_tuned(k_slots, old_slots);
+    throw;
+  }
+  ctx.set_tuned(k_scale, best_scale);
+  ctx.set_tuned(k_slots, best_slots);
+  return best;
 }
 
 }  // namespace gfz

@@ -89,8 +89,10 @@ Ctx::Ctx(int device) : device_(device) {
   DeviceGuard g(device);
   GFZ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     num_cus_ = std::max(1, prop.multiProcessorCount);
+    arch_ = prop.gcnArchName;
+  }
 }
 
 Ctx::~Ctx() {

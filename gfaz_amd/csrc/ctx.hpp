@@ -90,6 +90,7 @@ class Ctx {
   explicit Ctx(int device);
   ~Ctx();
   int device() const { return device_; }
+  const std::string &arch() const { return arch_; }  // e.g. "gfx950:sramecc+:xnack-"
   hipStream_t stream() const { return stream_; }
   Pool &pool() { return pool_; }
   int num_cus() const { return num_cus_; }
@@ -155,6 +156,13 @@ class Ctx {
     return it == tuned_.end() ? -1 : it->second;
   }
   void set_tuned(const std::string &key, int v) { tuned_[key] = v; }
+  // Size class of n elements (log4), the suffix of size-dependent tuned keys.
+  static uint32_t size_class(uint64_t n) {
+    uint32_t c = 0;
+    for (; n >= 4; n >>= 2)
+      ++c;
+    return c;
+  }
   const std::map<std::string, int> &tuned_all() const { return tuned_; }
   // Elapsed milliseconds of `launch` on the context stream (synchronises).
   template <class F>
@@ -178,6 +186,7 @@ class Ctx {
     hipEvent_t a, b;
   };
   int device_;
+  std::string arch_;
   int num_cus_ = 256;
   hipStream_t stream_ = nullptr;
   Pool pool_;

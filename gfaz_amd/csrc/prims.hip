@@ -186,6 +186,34 @@ uint32_t grid_for(Ctx &ctx, uint64_t n, uint32_t per_block) {
 
 }  // namespace
 
+namespace {
+__global__ void k_count_mismatch(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+                                 uint64_t n, unsigned long long *__restrict__ out) {
+  unsigned long long c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c += a[i] != b[i] ? 1ull : 0ull;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1)
+    c += __shfl_xor(c, d);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c)
+    atomicAdd(out, c);
+}
+}  // namespace
+
+uint64_t count_mismatch_i32(Ctx &ctx, const int32_t *a, const int32_t *b, uint64_t n) {
+  if (!n)
+    return 0;
+  DBuf<unsigned long long> cnt(ctx.pool(), 1);
+  fill_u64(ctx, reinterpret_cast<uint64_t *>(cnt.get()), 0, 1);
+  hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(ctx, n, kBlock * 8)), dim3(kBlock), 0,
+                     ctx.stream(), a, b, n, cnt.get());
+  GFZ_CHECK_LAUNCH();
+  ctx.stage_scalars(cnt.get(), 1);
+  ctx.sync();
+  return ctx.scalars()[0];
+}
+
 void exclusive_scan(Ctx &ctx, const uint32_t *in, uint64_t *out, uint64_t n) {
   exclusive_scan_impl<uint32_t>(ctx, in, out, n);
 }

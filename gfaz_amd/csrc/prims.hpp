@@ -62,6 +62,9 @@ void exclusive_scan(Ctx &ctx, const uint64_t *in, uint64_t *out, uint64_t n);
 void sort_unique_u64(Ctx &ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, uint32_t *err,
                      uint32_t local_cap = 0, uint32_t key_bits = 64);
 
+// Positions where a[i] != b[i] over n int32 (synchronises).
+uint64_t count_mismatch_i32(Ctx &ctx, const int32_t *a, const int32_t *b, uint64_t n);
+
 void fill_u32(Ctx &ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(Ctx &ctx, uint64_t *p, uint64_t v, uint64_t n);
 void fill_u8(Ctx &ctx, uint8_t *p, uint8_t v, uint64_t n);

@@ -223,12 +223,13 @@ __global__ void k_fmt_heads(const uint8_t *__restrict__ heads_txt,
 
 // ---- parsing ---------------------------------------------------------------
 
-// Line of byte p: last l with line_off[l] <= p (line_off non-decreasing, n_lines + 1).
-__device__ inline uint64_t line_of(const uint64_t *line_off, uint64_t n_lines, uint64_t p) {
+// Field of byte p: last l with fb[l] <= p (fb non-decreasing, n_lines entries,
+// fb[0] <= p); the byte is in it when p < fe[l].
+__device__ inline uint64_t line_of(const uint64_t *fb, uint64_t n_lines, uint64_t p) {
   uint64_t lo = 0, hi = n_lines;
   while (hi - lo > 1) {
     const uint64_t mid = (lo + hi) / 2;
-    if (line_off[mid] <= p)
+    if (fb[mid] <= p)
       lo = mid;
     else
       hi = mid;
@@ -236,29 +237,38 @@ __device__ inline uint64_t line_of(const uint64_t *line_off, uint64_t n_lines, u
   return lo;
 }
 
-// The token starting at byte p of a line ending at e (p < e), if one starts there.
+// Bytes of the text as a kernel reads them: from an LDS window [w0, w0 + wn) of
+// the tile being parsed where it holds them, else from global memory.
+struct TextReader {
+  const uint8_t *g;
+  const uint8_t *win;
+  uint64_t w0, wn;
+  __device__ uint8_t operator()(uint64_t p) const { return p - w0 < wn ? win[p - w0] : g[p]; }
+};
+
+// The token starting at byte p of a field [s, e) (p < e), if one starts there.
 // Returns 0 (none), 1 (a step: *id signed) or 2 (a dropped step: empty name or
 // value 0); sets *err to 1 (no digits) or 2 (overflow) on a malformed name.
-__device__ inline int parse_token(int kind, const uint8_t *txt, uint64_t s, uint64_t p, uint64_t e,
-                                  int32_t *id, uint32_t *err) {
+__device__ inline int parse_token(int kind, const TextReader &txt, uint64_t s, uint64_t p,
+                                  uint64_t e, int32_t *id, uint32_t *err) {
   uint64_t name_b, name_e;
   bool rev;
   if (kind == kTextP) {
-    if (txt[p] == ',' || (p > s && txt[p - 1] != ','))
+    if (txt(p) == ',' || (p > s && txt(p - 1) != ','))
       return 0;
     uint64_t q = p;
-    while (q < e && txt[q] != ',')
+    while (q < e && txt(q) != ',')
       ++q;
     // token [p, q), q > p: orientation txt[q - 1], name [p, q - 1)
-    rev = txt[q - 1] == '-';
+    rev = txt(q - 1) == '-';
     name_b = p;
     name_e = q - 1;
   } else {
-    const uint8_t c = txt[p];
+    const uint8_t c = txt(p);
     if (c != '<' && c != '>')
       return 0;
     uint64_t q = p + 1;
-    while (q < e && txt[q] != '<' && txt[q] != '>')
+    while (q < e && txt(q) != '<' && txt(q) != '>')
       ++q;
     rev = c == '<';
     name_b = p + 1;
@@ -269,7 +279,7 @@ __device__ inline int parse_token(int kind, const uint8_t *txt, uint64_t s, uint
   uint32_t val = 0;
   bool any = false;
   for (uint64_t q = name_b; q < name_e; ++q) {
-    const uint8_t c = txt[q];
+    const uint8_t c = txt(q);
     if (c < '0' || c > '9')
       break;
     any = true;
@@ -290,27 +300,76 @@ __device__ inline int parse_token(int kind, const uint8_t *txt, uint64_t s, uint
   return 1;
 }
 
+// Field cursor of one thread: the field holding (or next after) its position.
+struct FieldCursor {
+  const uint64_t *fb, *fe;
+  uint64_t n, l, b, e, nb;  // field l = [b, e); nb = the next field's begin
+  __device__ void seek(uint64_t p) {
+    l = line_of(fb, n, p);
+    b = fb[l];
+    e = fe[l];
+    nb = l + 1 < n ? fb[l + 1] : ~0ull;
+  }
+  __device__ void advance(uint64_t p) {
+    while (nb <= p) {
+      ++l;
+      b = nb;
+      e = fe[l];
+      nb = l + 1 < n ? fb[l + 1] : ~0ull;
+    }
+  }
+};
+
+// Stages text bytes [w0, w0 + kParseTile + kParseHalo) of the block's tile into LDS
+// (16-byte loads where the text allows).
+constexpr uint32_t kParseHalo = 64;
+constexpr uint32_t kParseWin = kParseTile + kParseHalo;
+__device__ inline uint64_t stage_window(const uint8_t *txt, uint64_t n_bytes, uint64_t w0,
+                                        uint8_t *win) {
+  const uint64_t wn = w0 + kParseWin < n_bytes ? kParseWin : n_bytes - w0;
+  const bool aligned = ((uintptr_t)(txt + w0) & 15) == 0;
+  if (aligned) {
+    const uint64_t nv = wn / 16;
+    const uint4 *src = reinterpret_cast<const uint4 *>(txt + w0);
+    uint4 *dst = reinterpret_cast<uint4 *>(win);
+    for (uint64_t i = threadIdx.x; i < nv; i += kBlock)
+      dst[i] = src[i];
+    for (uint64_t i = nv * 16 + threadIdx.x; i < wn; i += kBlock)
+      win[i] = txt[w0 + i];
+  } else {
+    for (uint64_t i = threadIdx.x; i < wn; i += kBlock)
+      win[i] = txt[w0 + i];
+  }
+  __syncthreads();
+  return wn;
+}
+
 // Kept steps per tile of text bytes.
 __global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
-                                                        const uint64_t *__restrict__ line_off,
+                                                        const uint64_t *__restrict__ fb,
+                                                        const uint64_t *__restrict__ fe,
                                                         uint64_t n_lines,
                                                         uint32_t *__restrict__ tile_count,
                                                         uint32_t *__restrict__ err) {
+  __shared__ __align__(16) uint8_t win[kParseWin];
   __shared__ uint32_t lds[kBlock / kWave + 1];
-  const uint64_t p0 = (uint64_t)blockIdx.x * kParseTile + (uint64_t)threadIdx.x * kParseItems;
+  const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
+  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};
+  // Thread t takes bytes t, t + kBlock, ... of the tile: a wave reads 64
+  // consecutive LDS bytes per step.
   uint32_t c = 0;
-  if (p0 < n_bytes) {
-    uint64_t l = line_of(line_off, n_lines, p0);
-    for (uint32_t j = 0; j < kParseItems; ++j) {
-      const uint64_t p = p0 + j;
-      if (p >= n_bytes)
-        break;
-      while (line_off[l + 1] <= p)
-        ++l;
-      int32_t id;
-      c += parse_token(kind, txt, line_off[l], p, line_off[l + 1], &id, err) == 1 ? 1u : 0u;
-    }
+  FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
+  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0);
+  for (uint32_t j = 0; j < kParseItems; ++j) {
+    const uint64_t p = w0 + (uint64_t)j * kBlock + threadIdx.x;
+    if (p >= n_bytes)
+      break;
+    fc.advance(p);
+    if (p < fc.b || p >= fc.e)
+      continue;
+    int32_t id;
+    c += parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1 ? 1u : 0u;
   }
   uint32_t total;
   block_exclusive_scan<kBlock>(c, lds, total);
@@ -318,57 +377,78 @@ __global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t 
     tile_count[blockIdx.x] = total;
 }
 
+// Steps in byte order: each thread's kept tokens are ranked inside the tile by a
+// block scan over the byte-strided ownership (bytes j * kBlock + t belong to
+// thread t), so the write pass orders them by byte with an LDS list.
 __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
-                                                        const uint64_t *__restrict__ line_off,
+                                                        const uint64_t *__restrict__ fb,
+                                                        const uint64_t *__restrict__ fe,
                                                         uint64_t n_lines,
                                                         const uint64_t *__restrict__ tile_pos,
                                                         int32_t *__restrict__ nodes,
                                                         uint32_t *__restrict__ err) {
+  __shared__ __align__(16) uint8_t win[kParseWin];
   __shared__ uint32_t lds[kBlock / kWave + 1];
-  const uint64_t p0 = (uint64_t)blockIdx.x * kParseTile + (uint64_t)threadIdx.x * kParseItems;
-  int32_t ids[kParseItems];
-  uint32_t keep = 0, c = 0;
-  if (p0 < n_bytes) {
-    uint64_t l = line_of(line_off, n_lines, p0);
-    for (uint32_t j = 0; j < kParseItems; ++j) {
-      const uint64_t p = p0 + j;
-      if (p >= n_bytes)
-        break;
-      while (line_off[l + 1] <= p)
-        ++l;
-      if (parse_token(kind, txt, line_off[l], p, line_off[l + 1], &ids[j], err) == 1) {
-        keep |= 1u << j;
-        ++c;
-      }
+  // Kept-token flags of the tile's bytes (one bit per byte) and the IDs by byte.
+  __shared__ uint32_t keep_bits[kParseTile / 32];
+  __shared__ int32_t ids[kParseTile];
+  const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
+  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};
+  for (uint32_t i = threadIdx.x; i < kParseTile / 32; i += kBlock)
+    keep_bits[i] = 0;
+  __syncthreads();
+  FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
+  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0);
+  for (uint32_t j = 0; j < kParseItems; ++j) {
+    const uint32_t r = j * kBlock + threadIdx.x;
+    const uint64_t p = w0 + r;
+    if (p >= n_bytes)
+      break;
+    fc.advance(p);
+    if (p < fc.b || p >= fc.e)
+      continue;
+    int32_t id;
+    if (parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1) {
+      ids[r] = id;
+      atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
     }
   }
+  __syncthreads();
+  // Compaction in byte order: thread t owns bytes [t * kParseItems, +kParseItems).
+  const uint32_t r0 = threadIdx.x * kParseItems;
+  const uint32_t bits = (keep_bits[r0 >> 5] >> (r0 & 31)) & ((1u << kParseItems) - 1);
   uint32_t total;
-  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>(c, lds, total);
+  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>((uint32_t)__popc(bits), lds,
+                                                                   total);
   for (uint32_t j = 0; j < kParseItems; ++j)
-    if ((keep >> j) & 1)
-      nodes[o++] = ids[j];
+    if ((bits >> j) & 1)
+      nodes[o++] = ids[r0 + j];
 }
 
-// Kept steps before each line start: one thread per line, counting from its tile.
+// Kept steps before each field start (lpos[n_lines]: before byte n_bytes): one
+// thread per field, counting from its tile start.
 __global__ void k_parse_line_pos(int kind, const uint8_t *__restrict__ txt, uint64_t n_bytes,
-                                 const uint64_t *__restrict__ line_off, uint64_t n_lines,
-                                 const uint64_t *__restrict__ tile_pos, uint64_t *__restrict__ lpos,
-                                 uint32_t *__restrict__ err) {
+                                 const uint64_t *__restrict__ fb, const uint64_t *__restrict__ fe,
+                                 uint64_t n_lines, const uint64_t *__restrict__ tile_pos,
+                                 uint64_t *__restrict__ lpos, uint32_t *__restrict__ err) {
   const uint64_t li = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (li > n_lines)
     return;
-  const uint64_t p = line_off[li];
+  const uint64_t p = li < n_lines ? fb[li] : n_bytes;
   const uint64_t tile = p / kParseTile;
   uint64_t c = tile_pos[tile];
   uint64_t q = tile * kParseTile;
   if (q < p) {
-    uint64_t l = line_of(line_off, n_lines, q);
+    const TextReader rd{txt, nullptr, 0, 0};
+    FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
+    fc.seek(q);
     for (; q < p; ++q) {
-      while (line_off[l + 1] <= q)
-        ++l;
+      fc.advance(q);
+      if (q < fc.b || q >= fc.e)
+        continue;
       int32_t id;
-      c += parse_token(kind, txt, line_off[l], q, line_off[l + 1], &id, err) == 1 ? 1u : 0u;
+      c += parse_token(kind, rd, fc.b, q, fc.e, &id, err) == 1 ? 1u : 0u;
     }
   }
   lpos[li] = c;
@@ -489,14 +569,14 @@ void format_write(Ctx &ctx, const FormatPlan &plan, const uint8_t *heads_txt,
 }
 
 void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
-                const uint64_t *line_off, uint64_t n_lines, uint32_t *lengths, ParsePlan &plan) {
+                const uint64_t *field_begin, const uint64_t *field_end, uint64_t n_lines,
+                uint32_t *lengths, ParsePlan &plan) {
   if (kind != kTextP && kind != kTextW)
     throw Error(kErrInvalid, "line kind must be P or W");
-  if (line_off[0] != 0 || line_off[n_lines] != n_bytes)
-    throw Error(kErrInvalid, "line offsets must run from 0 to the text size");
   for (uint64_t l = 0; l < n_lines; ++l)
-    if (line_off[l + 1] < line_off[l])
-      throw Error(kErrInvalid, "line offsets must be non-decreasing");
+    if (field_end[l] < field_begin[l] || field_end[l] > n_bytes ||
+        (l + 1 < n_lines && field_begin[l + 1] < field_end[l]))
+      throw Error(kErrInvalid, "step fields must be ordered, disjoint and inside the text");
   plan = ParsePlan();
   plan.kind = kind;
   plan.txt = d_txt;
@@ -504,29 +584,34 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
   plan.n_lines = n_lines;
   Pool &pool = ctx.pool();
   plan.n_tiles = ceil_div(std::max<uint64_t>(n_bytes, 1), kParseTile);
-  plan.line_off = DBuf<uint64_t>(pool, n_lines + 1);
+  plan.fb = DBuf<uint64_t>(pool, std::max<uint64_t>(n_lines, 1));
+  plan.fe = DBuf<uint64_t>(pool, std::max<uint64_t>(n_lines, 1));
   plan.tile_pos = DBuf<uint64_t>(pool, plan.n_tiles + 1);
   DBuf<uint64_t> lpos(pool, n_lines + 1);
   DBuf<uint32_t> tile_cnt(pool, plan.n_tiles), err(pool, 1);
   fill_u32(ctx, err.get(), 0, 1);
-  GFZ_HIP(hipMemcpyAsync(plan.line_off.get(), line_off, (n_lines + 1) * 8, hipMemcpyHostToDevice,
-                         ctx.stream()));
+  if (n_lines) {
+    GFZ_HIP(hipMemcpyAsync(plan.fb.get(), field_begin, n_lines * 8, hipMemcpyHostToDevice,
+                           ctx.stream()));
+    GFZ_HIP(hipMemcpyAsync(plan.fe.get(), field_end, n_lines * 8, hipMemcpyHostToDevice,
+                           ctx.stream()));
+  }
   const bool any = n_bytes && n_lines;
   if (any) {
     hipLaunchKernelGGL(k_parse_count, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0, ctx.stream(),
-                       kind, d_txt, n_bytes, plan.line_off.get(), n_lines, tile_cnt.get(),
-                       err.get());
+                       kind, d_txt, n_bytes, plan.fb.get(), plan.fe.get(), n_lines,
+                       tile_cnt.get(), err.get());
     GFZ_CHECK_LAUNCH();
   }
   exclusive_scan(ctx, tile_cnt.get(), plan.tile_pos.get(), any ? plan.n_tiles : 0);
-  if (n_lines) {
+  if (any) {
     hipLaunchKernelGGL(k_parse_line_pos, dim3(grid1(n_lines + 1, kBlock)), dim3(kBlock), 0,
-                       ctx.stream(), kind, d_txt, n_bytes, plan.line_off.get(), n_lines,
+                       ctx.stream(), kind, d_txt, n_bytes, plan.fb.get(), plan.fe.get(), n_lines,
                        plan.tile_pos.get(), lpos.get(), err.get());
     GFZ_CHECK_LAUNCH();
   }
   std::vector<uint64_t> h_lpos(n_lines + 1, 0);
-  if (n_lines)
+  if (any)
     GFZ_HIP(hipMemcpyAsync(h_lpos.data(), lpos.get(), (n_lines + 1) * 8, hipMemcpyDeviceToHost,
                            ctx.stream()));
   ctx.stage_scalars(err.get(), 1);
@@ -536,7 +621,7 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
     throw Error(kErrInvalid, "parse_uint32: overflow");
   if (e & 1u)
     throw Error(kErrInvalid, "parse_uint32: no digits");
-  plan.total = n_lines ? h_lpos[n_lines] : 0;
+  plan.total = h_lpos[n_lines];
   for (uint64_t l = 0; l < n_lines; ++l) {
     const uint64_t c = h_lpos[l + 1] - h_lpos[l];
     if (c > 0xFFFFFFFFull)
@@ -554,8 +639,8 @@ void parse_write(Ctx &ctx, const ParsePlan &plan, int32_t *d_nodes, uint64_t nod
     DBuf<uint32_t> err(ctx.pool(), 1);  // every malformed name was reported by the plan
     ctx.timed("parse", [&] {
       hipLaunchKernelGGL(k_parse_write, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0,
-                         ctx.stream(), plan.kind, plan.txt, plan.n_bytes, plan.line_off.get(),
-                         plan.n_lines, plan.tile_pos.get(), d_nodes, err.get());
+                         ctx.stream(), plan.kind, plan.txt, plan.n_bytes, plan.fb.get(),
+                         plan.fe.get(), plan.n_lines, plan.tile_pos.get(), d_nodes, err.get());
     });
     GFZ_CHECK_LAUNCH();
   }

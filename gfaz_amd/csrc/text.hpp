@@ -38,16 +38,18 @@ struct ParsePlan {
   int kind = 0;
   const uint8_t *txt = nullptr;
   uint64_t n_bytes = 0, n_lines = 0, n_tiles = 0, total = 0;
-  DBuf<uint64_t> line_off;   // n_lines + 1
+  DBuf<uint64_t> fb, fe;     // field begins and ends (n_lines each)
   DBuf<uint64_t> tile_pos;   // kept steps before each tile (n_tiles + 1)
 };
 
-// Counts the steps of the step lists d_txt[line_off[l] .. line_off[l + 1])
-// (device text; host offsets, n_lines + 1): plan.total, and per line in lengths
-// (host, n_lines, may be null). Throws the reference's parse_uint32 messages on
-// a malformed name. d_txt must stay valid for parse_write.
+// Counts the steps of the step fields d_txt[field_begin[l] .. field_end[l]) (device
+// text; host ranges, ordered and disjoint; bytes outside them are ignored):
+// plan.total, and per line in lengths (host, n_lines, may be null). Throws the
+// reference's parse_uint32 messages on a malformed name. d_txt must stay valid for
+// parse_write.
 void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
-                const uint64_t *line_off, uint64_t n_lines, uint32_t *lengths, ParsePlan &plan);
+                const uint64_t *field_begin, const uint64_t *field_end, uint64_t n_lines,
+                uint32_t *lengths, ParsePlan &plan);
 // Writes the planned steps into d_nodes (device, nodes_cap). Synchronises.
 void parse_write(Ctx &ctx, const ParsePlan &plan, int32_t *d_nodes, uint64_t nodes_cap);
 

@@ -78,6 +78,20 @@ int gfz_ctx_num_cus(gfz_ctx *ctx);
 int gfz_ctx_set_autotune(gfz_ctx *ctx, int on);
 int gfz_ctx_tuned(gfz_ctx *ctx, const char *key, int *value);
 int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value);
+/* Tunes the pair-counting knobs of compression on a representative input: each
+ * setting of "count_bucket_scale/<size class>" (1, 2, 4: half, the estimate,
+ * twice the estimated bucket count) and "count_agg_slots/<size class>" (2048,
+ * 4096 LDS slots per aggregated tile) compresses the input twice; every setting
+ * must give the same grammar (else GFZ_E_CORRUPT) and the fastest is kept in the
+ * ctx. *best_ms (may be NULL) receives its time. Inputs as gfz_compress_traversals. */
+int gfz_autotune_compress(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                          const uint32_t *lengths, uint64_t n_trav, int rounds, int threshold,
+                          int delta_round, uint32_t flags, double *best_ms);
+/* The ctx's tuned table as JSON {"arch": ..., "cus": N, "tuned": {key: value}},
+ * and back: a file written for another GPU architecture or CU count is refused
+ * (GFZ_E_INVALID) and leaves the table as it was. */
+int gfz_ctx_save_tuning(gfz_ctx *ctx, const char *path);
+int gfz_ctx_load_tuning(gfz_ctx *ctx, const char *path);
 
 /* Device memory helpers (pool-backed) for callers without their own allocator. */
 int gfz_dev_alloc(gfz_ctx *ctx, uint64_t bytes, void **out);
@@ -344,11 +358,13 @@ int gfz_server_reset_stats(gfz_server *s);
  * bytes around them ("P\tname\t" / "\t*\n", "W\tsample\thap\tseq\tstart\tend\t" /
  * "\n"), given as concatenated bytes with n_trav + 1 offsets each. nodes: host
  * or device (GFZ_INPUT_DEVICE); out: host or device (GFZ_OUTPUT_DEVICE) of
- * out_cap bytes, or NULL to only size the text (*n_out).
+ * out_cap bytes, or NULL to only size the text (*n_out); line_off (host,
+ * n_trav + 1, may be NULL) receives where each line starts.
  * gfz_parse_steps is the inverse for the reference parser's numeric-name path
  * (parse_p_line / parse_w_line with parse_uint32, src/io/gfa_parser.cpp:69-91,
  * 676-790, 1210-1219): the step-list field of each line,
- * text[line_off[l] .. line_off[l + 1]), into signed node IDs (steps with an empty
+ * text[field_begin[l] .. field_end[l]) (ordered, disjoint; other bytes, such as
+ * the rest of each line, are ignored), into signed node IDs (steps with an empty
  * name or the value 0 are dropped, as there; a name without leading digits or
  * past 2^32 - 1 fails with the reference's "parse_uint32: ..." message).
  * text: host or device (GFZ_INPUT_DEVICE); nodes: host or device
@@ -357,10 +373,12 @@ enum { GFZ_TEXT_P = 0, GFZ_TEXT_W = 1 };
 int gfz_format_steps(gfz_ctx *ctx, int kind, const int32_t *nodes, uint64_t n_nodes,
                      const uint32_t *lengths, uint64_t n_trav, const uint8_t *heads,
                      const uint64_t *head_off, const uint8_t *tails, const uint64_t *tail_off,
-                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out);
+                     uint32_t flags, uint8_t *out, uint64_t out_cap, uint64_t *n_out,
+                     uint64_t *line_off);
 int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_bytes,
-                    const uint64_t *line_off, uint64_t n_lines, uint32_t flags, int32_t *nodes,
-                    uint64_t nodes_cap, uint32_t *lengths, uint64_t *n_nodes);
+                    const uint64_t *field_begin, const uint64_t *field_end, uint64_t n_lines,
+                    uint32_t flags, int32_t *nodes, uint64_t nodes_cap, uint32_t *lengths,
+                    uint64_t *n_nodes);
 
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before

@@ -149,3 +149,21 @@ def test_full_pangenome_p_lines(gpu_ctx):
     nodes, lens = codec.parse_steps(gpu_ctx, to.P, b"".join(l[2] for l in lines), loff)
     np.testing.assert_array_equal(lens, tr.lengths)
     assert np.array_equal(nodes, tr.nodes)
+
+
+def test_parse_fields_inside_whole_lines(gpu_ctx):
+    """Step fields parsed in place inside whole P-lines (heads and tails skipped), with
+    the line starts that formatting reports."""
+    tr = synth.generate(synth.CONFIGS["plain_20k"].scaled(backbone=4000, n_haps=5))
+    lens = np.concatenate([tr.lengths, [0]]).astype(np.uint32)  # and an empty line
+    heads = [f"P\thap{h}\t".encode() for h in range(lens.size)]
+    tails = [b"\t3M,2M\n" if h % 2 else b"\t*\n" for h in range(lens.size)]
+    text, line_off = codec.format_steps(gpu_ctx, to.P, tr.nodes, lens, heads, tails,
+                                        with_offsets=True)
+    assert text == to.format_steps(to.P, tr.nodes, lens, heads, tails)
+    hl = np.array([len(h) for h in heads], np.uint64)
+    tl = np.array([len(t) for t in tails], np.uint64)
+    begin, end = line_off[:-1] + hl, line_off[1:] - tl
+    nodes, got_lens = codec.parse_steps(gpu_ctx, to.P, text, fields=(begin, end))
+    np.testing.assert_array_equal(got_lens, lens)
+    np.testing.assert_array_equal(nodes, tr.nodes)
