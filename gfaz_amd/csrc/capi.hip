@@ -8,6 +8,7 @@
 #include "prims.hpp"
 #include "resident.hpp"
 #include "text.hpp"
+#include "coverage.hpp"
 
 #include <cctype>
 #include <climits>
@@ -912,6 +913,72 @@ int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_byte
                                c.stream()));
       c.sync();
     }
+  });
+}
+
+// ---- node coverage ---------------------------------------------------------------
+
+}  // extern "C"
+
+struct gfz_coverage {
+  gfz_ctx *owner = nullptr;
+  gfz::Coverage cov;
+};
+
+extern "C" {
+
+int gfz_coverage_build(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                       const uint32_t *lengths, uint64_t n_trav, const uint32_t *group_of,
+                       uint32_t n_groups, uint32_t num_nodes, uint32_t flags,
+                       gfz_coverage **out) {
+  constexpr const char *kPrefix = "Coverage error: ";
+  if (!ctx || !out || (n_nodes && !nodes) || (n_trav && (!lengths || !group_of)))
+    return fail(GFZ_E_INVALID, std::string(kPrefix) + "null argument");
+  *out = nullptr;
+  return guarded(kPrefix, [&] {
+    gfz::Ctx &c = *ctx->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::DBuf<int32_t> staged;
+    const int32_t *d_nodes;
+    stage_encoded(c, nodes, n_nodes, flags, staged, d_nodes);
+    auto cv = std::make_unique<gfz_coverage>();
+    cv->owner = ctx;
+    gfz::coverage_build(c, d_nodes, n_nodes, lengths, n_trav, group_of, n_groups, num_nodes,
+                        cv->cov);
+    *out = cv.release();
+  });
+}
+
+void gfz_coverage_free(gfz_coverage *cov) {
+  if (!cov)
+    return;
+  gfz::DeviceGuard g(cov->owner->ctx->device());
+  (void)hipStreamSynchronize(cov->owner->ctx->stream());
+  delete cov;
+}
+
+int gfz_coverage_hist(gfz_coverage *cov, uint64_t *hist) {
+  if (!cov || !hist)
+    return fail(GFZ_E_INVALID, "Growth workflow error: null argument");
+  return guarded("Growth workflow error: ", [&] {
+    gfz::Ctx &c = *cov->owner->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::coverage_hist(c, cov->cov, hist);
+  });
+}
+
+int gfz_coverage_windows(gfz_coverage *cov, const uint64_t *window_off,
+                         const uint32_t *window_nodes, uint64_t n_windows,
+                         const uint32_t *segment_lengths, uint64_t *denominators,
+                         uint64_t *numerators) {
+  if (!cov || !window_off || !numerators || (n_windows && window_off[n_windows] &&
+                                             (!window_nodes || !segment_lengths)))
+    return fail(GFZ_E_INVALID, "PAV workflow error: null argument");
+  return guarded("PAV workflow error: ", [&] {
+    gfz::Ctx &c = *cov->owner->ctx;
+    gfz::DeviceGuard g(c.device());
+    gfz::coverage_windows(c, cov->cov, window_off, window_nodes, n_windows, segment_lengths,
+                          denominators, numerators);
   });
 }
 

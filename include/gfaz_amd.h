@@ -380,6 +380,41 @@ int gfz_parse_steps(gfz_ctx *ctx, int kind, const uint8_t *text, uint64_t n_byte
                     uint32_t flags, int32_t *nodes, uint64_t nodes_cap, uint32_t *lengths,
                     uint64_t *n_nodes);
 
+/* ---- node coverage: pangenome growth and presence/absence (PAV) -------------
+ * The GPU part of compute_growth (src/workflows/growth_workflow.cpp:261-503) and
+ * compute_pav (src/workflows/pav_workflow.cpp:475-760), on traversals decoded by
+ * gfz_decompress_traversals: gfz_coverage_build marks which of n_groups groups
+ * (the caller's grouping of the traversals, group_of[t] per traversal, e.g. the
+ * reference's PanSN sample / haplotype keys; up to 65535 groups) visit which
+ * nodes 1 .. num_nodes (|step|; 0 and larger IDs are skipped as there). nodes:
+ * host or device (GFZ_INPUT_DEVICE).
+ * gfz_coverage_hist: hist[c] (host, n_groups + 1) = nodes visited by exactly c
+ * groups; gfz_growth_curve (host) turns it into growth[k] = expected nodes covered
+ * by k random groups, with the reference's closed form (k = 1 .. n_groups;
+ * growth[0] = 0).
+ * gfz_pav_window_nodes (host) finds, for BED ranges [starts[r], ends[r]) on one
+ * reference traversal (ref_nodes, with segment lengths 1 .. num_nodes), each
+ * range's distinct overlapping nodes as the reference does: window_off
+ * (n_ranges + 1) always, window_nodes when not NULL (nodes_cap entries).
+ * gfz_coverage_windows then gives denominators[w] (the windows' node lengths) and
+ * numerators[w * n_groups + g] (the lengths of those nodes group g visits). */
+typedef struct gfz_coverage gfz_coverage;
+int gfz_coverage_build(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                       const uint32_t *lengths, uint64_t n_trav, const uint32_t *group_of,
+                       uint32_t n_groups, uint32_t num_nodes, uint32_t flags,
+                       gfz_coverage **out);
+void gfz_coverage_free(gfz_coverage *cov);
+int gfz_coverage_hist(gfz_coverage *cov, uint64_t *hist);
+int gfz_coverage_windows(gfz_coverage *cov, const uint64_t *window_off,
+                         const uint32_t *window_nodes, uint64_t n_windows,
+                         const uint32_t *segment_lengths, uint64_t *denominators,
+                         uint64_t *numerators);
+int gfz_growth_curve(const uint64_t *hist, uint32_t n_groups, double *growth);
+int gfz_pav_window_nodes(const int32_t *ref_nodes, uint64_t n_ref,
+                         const uint32_t *segment_lengths, uint32_t num_nodes,
+                         const uint64_t *starts, const uint64_t *ends, uint64_t n_ranges,
+                         uint64_t *window_off, uint32_t *window_nodes, uint64_t nodes_cap);
+
 /* ---- traversal-column framing (host) -----------------------------------
  * The steps the reference CPU workflow applies around the grammar before
  * serialisation, producing the same bytes, so a caller can fill CompressedData

@@ -23,6 +23,13 @@
 //          in the environment the reference prints its own stage timers to stderr
 //          (compression_workflow.cpp:695-698,791-819; decompression_workflow.cpp:
 //          260-546), which give the codec-only times.
+//   growth <in.gfaz> <mode> <threads>
+//       -> compute_growth (src/workflows/growth_workflow.cpp:261), printed as JSON
+//          (hist, growth with 17 significant digits); mode 0 PerPathWalk,
+//          1 SampleHapSeq, 2 SampleHap, 3 Sample
+//   pav <in.gfaz> <in.bed> <mode> <threads>
+//       -> compute_pav (src/workflows/pav_workflow.cpp:475), printed as JSON
+//          (group names, denominators, numerators)
 //   time-compress <in.gfa> <rounds> <threshold> <delta_round> <threads> <reps>
 //   time-decompress <in.gfaz> <threads> <reps>
 //       -> wall-clock timing of the two calls above, printed as JSON; the
@@ -32,6 +39,8 @@
 #include "workflows/add_haplotypes_workflow.hpp"
 #include "workflows/compression_workflow.hpp"
 #include "workflows/decompression_workflow.hpp"
+#include "workflows/growth_workflow.hpp"
+#include "workflows/pav_workflow.hpp"
 #include "workflows/extraction_workflow.hpp"
 
 #include <chrono>
@@ -113,6 +122,49 @@ int main(int argc, char **argv) {
       std::printf("{\"compress_s\": %.6f, \"decompress_s\": %.6f, "
                   "\"decompress_in_memory_s\": %.6f, \"threads\": %d}\n",
                   tc, tw, tl, threads);
+      return 0;
+    }
+    if ((cmd == "growth" && argc == 5) || (cmd == "pav" && argc == 6)) {
+      static const gfaz::GroupingMode modes[] = {
+          gfaz::GroupingMode::PerPathWalk, gfaz::GroupingMode::SampleHapSeq,
+          gfaz::GroupingMode::SampleHap, gfaz::GroupingMode::Sample};
+      gfaz::CompressedData d = gfaz::deserialize_compressed_data(argv[2]);
+      const int m = std::atoi(argv[cmd == "growth" ? 3 : 4]);
+      const int threads = std::atoi(argv[cmd == "growth" ? 4 : 5]);
+      if (m < 0 || m > 3)
+        throw std::runtime_error("mode must be 0..3");
+      if (cmd == "growth") {
+        gfaz::GrowthResult r = gfaz::compute_growth(d, threads, modes[m]);
+        std::printf("{\"num_haplotypes\": %u, \"num_nodes\": %u, \"hist\": [", r.num_haplotypes,
+                    r.num_nodes);
+        for (size_t i = 0; i < r.hist.size(); ++i)
+          std::printf("%s%llu", i ? ", " : "", (unsigned long long)r.hist[i]);
+        std::printf("], \"growth\": [");
+        for (size_t i = 0; i < r.growth.size(); ++i)
+          std::printf("%s%.17g", i ? ", " : "", r.growth[i]);
+        std::printf("]}\n");
+      } else {
+        gfaz::PavOptions o;
+        o.bed_path = argv[3];
+        o.num_threads = threads;
+        o.grouping = modes[m];
+        gfaz::PavResult r = gfaz::compute_pav(d, o);
+        std::printf("{\"groups\": [");
+        for (size_t i = 0; i < r.group_names.size(); ++i)
+          std::printf("%s\"%s\"", i ? ", " : "", r.group_names[i].c_str());
+        std::printf("], \"ranges\": [");
+        for (size_t i = 0; i < r.ranges.size(); ++i)
+          std::printf("%s[\"%s\", %llu, %llu, \"%s\"]", i ? ", " : "", r.ranges[i].chrom.c_str(),
+                      (unsigned long long)r.ranges[i].start, (unsigned long long)r.ranges[i].end,
+                      r.ranges[i].name.c_str());
+        std::printf("], \"denominators\": [");
+        for (size_t i = 0; i < r.denominators.size(); ++i)
+          std::printf("%s%llu", i ? ", " : "", (unsigned long long)r.denominators[i]);
+        std::printf("], \"numerators\": [");
+        for (size_t i = 0; i < r.numerators.size(); ++i)
+          std::printf("%s%llu", i ? ", " : "", (unsigned long long)r.numerators[i]);
+        std::printf("]}\n");
+      }
       return 0;
     }
     if (cmd == "time-compress" && argc == 8) {

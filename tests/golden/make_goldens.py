@@ -278,6 +278,61 @@ def lines_cases(tmp: str) -> None:
     print("wrote *_lines.txt, edge_text_lines.json", flush=True)
 
 
+PAV_BEDS = ("pav_fixture.bed", "pav_path_only_fixture.bed", "pav_walk_fixture.bed")
+PANSN_NAMES = ["s0#0#c1", "s0#1#c1", "s1#0#c1", "s1#0#c2:0-100", "s2#0#c1", "s2"]
+
+
+def analytics_cases(tmp: str) -> None:
+    """compute_growth and compute_pav as the reference computes them (through
+    oracle/_ref): the CLI fixture with the reference's own BED fixtures, and a small
+    synthetic pangenome whose P-lines carry PanSN names, with a BED of windows on one
+    of them; every grouping mode."""
+    out = {}
+    beds_dir = "/root/reference/tests/fixtures"
+    for bed in PAV_BEDS:
+        shutil.copy(os.path.join(beds_dir, bed), os.path.join(HERE, bed))
+    cli = os.path.join(HERE, "cli_fixture_r8_t2_d1.gfaz")
+    cfg = synth.CONFIGS["plain_20k"].scaled(backbone=3000, n_haps=6)
+    tr = synth.generate(cfg)
+    gfa = os.path.join(tmp, "pansn.gfa")
+    synth.write_gfa(cfg, tr, gfa)
+    with open(gfa) as f:
+        lines = f.read().splitlines(keepends=True)
+    h = 0
+    for i, l in enumerate(lines):
+        if l.startswith("P\t"):
+            f_ = l.split("\t")
+            f_[1] = PANSN_NAMES[h]
+            lines[i] = "\t".join(f_)
+            h += 1
+    with open(gfa, "w") as f:
+        f.writelines(lines)
+    seg = [len(l.split("\t")[2].rstrip("\n")) for l in lines if l.startswith("S\t")]
+    gz = os.path.join(tmp, "pansn.gfaz")
+    compress_case(gfa, gz, 8, 2, 1)
+    bed = os.path.join(HERE, "pansn_windows.bed")
+    with open(bed, "w") as f:
+        for k in range(40):
+            f.write(f"s0#0#c1\t{k * 997}\t{k * 997 + 1500}\tw{k}\n")
+        f.write("s2\t0\t50\n")
+        f.write("s1#0#c2:0-100\t100\t100\tempty\n")
+    np.savez_compressed(os.path.join(HERE, "analytics_pansn_3k.npz"), nodes=tr.nodes, lengths=tr.lengths,
+                        segment_lengths=np.array(seg, np.uint32),
+                        names=np.array(json.dumps(PANSN_NAMES)))
+    for label, gfaz, beds in (("cli_fixture", cli, PAV_BEDS), ("pansn_3k", gz,
+                                                               ("pansn_windows.bed",))):
+        case = {"growth": {}, "pav": {}}
+        for mode in range(4):
+            case["growth"][str(mode)] = json.loads(run("growth", gfaz, str(mode), "2").stdout)
+            for b in beds:
+                case["pav"][f"{b}/{mode}"] = json.loads(
+                    run("pav", gfaz, os.path.join(HERE, b), str(mode), "2").stdout)
+        out[label] = case
+    with open(os.path.join(HERE, "analytics.json"), "w") as f:
+        json.dump(out, f)
+    print("wrote analytics.json, analytics_pansn_3k.npz, BED fixtures", flush=True)
+
+
 def main() -> None:
     oracle.build_reference()
     tmp = tempfile.mkdtemp(prefix="gfaz_goldens_")
@@ -294,6 +349,7 @@ def main() -> None:
         synth_case("walks_30k", walks, tmp, [(8, 2, 1)])
         add_synth_case(tmp)
         lines_cases(tmp)
+        analytics_cases(tmp)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

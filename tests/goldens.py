@@ -16,7 +16,7 @@ def names(pattern: str = "*") -> list[str]:
     out = []
     for p in sorted(glob.glob(os.path.join(GOLDEN_DIR, pattern + ".npz"))):
         n = os.path.basename(p)[:-4]
-        if not n.endswith("_add6"):
+        if not n.endswith("_add6") and not n.startswith("analytics_"):
             out.append(n)
     return out
 
