@@ -404,6 +404,64 @@ def run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
     return 0 if state["ok"] else 1
 
 
+def run_framed(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
+    """The codec with the traversal columns' zstd framing, device-resident traversals
+    to .gfaz traversal fields and back to host traversals: overlapped
+    (gfz_compress_framed / gfz_decompress_framed) against step by step (codec, then
+    each column framed in turn; each column unframed in turn, then upload and
+    decode), the order the reference runs them in. One timed pass of each."""
+    from gfaz_amd import codec, container
+    threads = cpu_threads()
+    log("[bench] framed: warmup on the first traversal")
+    n0 = int(tr.lengths[0])
+    container.compress_traversals_gpu(ctx, tr.nodes[:n0], tr.lengths[:1], 1, threads=threads)
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = fn()
+        ctx.sync()
+        return time.perf_counter() - t0, r
+
+    log("[bench] framed: compress, overlapped")
+    tco, over = timed(lambda: container.compress_traversals_gpu(
+        ctx, d_nodes, tr.lengths, tr.n_paths, threads=threads))
+    log(f"[bench] {tco:.2f} s; compress, step by step")
+    tcs, step = timed(lambda: container.compress_traversals_gpu(
+        ctx, d_nodes, tr.lengths, tr.n_paths, threads=threads, overlapped=False))
+    fields = ("rules_first_zstd", "rules_second_zstd", "paths_zstd", "walks_zstd")
+    same_bytes = all(getattr(over, f).pair() == getattr(step, f).pair() for f in fields)
+    payload = sum(len(getattr(over, f).payload) for f in fields)
+    del step
+    d_nodes.free()
+    log(f"[bench] {tcs:.2f} s; decompress, overlapped")
+    tdo, got = timed(lambda: container.decompress_traversals_gpu(ctx, over))
+    ok = bool(np.array_equal(got[0], tr.nodes))
+    del got
+    log(f"[bench] {tdo:.2f} s; decompress, step by step")
+    tds, got = timed(lambda: container.decompress_traversals_gpu(ctx, over, overlapped=False))
+    ok = ok and bool(np.array_equal(got[0], tr.nodes))
+    del got
+    out = {
+        "metric": "compress+frame and unframe+decompress GFA GB/s (.gfaz traversal fields)",
+        "value": gfa_bytes / (tco + tdo) / 1e9, "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": cfg.name, "traversals": int(tr.lengths.size),
+                   "steps": int(tr.n_steps), "gfa_bytes": gfa_bytes, "rounds": 8,
+                   "threshold": 2, "delta_round": 1, "zstd_level": 9,
+                   "host_threads": threads},
+        "round_trip_ok": ok, "same_bytes": same_bytes, "payload_bytes": payload,
+        "ratio_vs_gfa": gfa_bytes / payload if payload else None,
+        "compress_s": {"overlapped": tco, "step_by_step": tcs},
+        "decompress_s": {"overlapped": tdo, "step_by_step": tds},
+        "column_bytes": {f: getattr(over, f).original_size for f in fields},
+        "column_payload_bytes": {f: len(getattr(over, f).payload) for f in fields},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0 if ok and same_bytes else 1
+
+
 def run_serve(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
     """The serving form: the grammar of the workload (walks_2m by default) stays resident
     in HBM behind the serving loop; a Poisson stream at a fixed rate sends 80% extract
@@ -562,11 +620,12 @@ def main() -> int:
     ap.add_argument("--verify-oracle", action="store_true",
                     help="also compare the encoded grammar with the CPU oracle")
     ap.add_argument("--mode", default="roundtrip",
-                    choices=["roundtrip", "batching", "stream", "serve"],
+                    choices=["roundtrip", "batching", "stream", "serve", "framed"],
                     help="roundtrip (default): compress + decompress steps; batching: one "
                          "persistent launch over all traversals vs one launch per traversal "
                          "(walks_2m); stream: compress, then decompress streamed through two "
-                         "pinned host buffers (plain_10m)")
+                         "pinned host buffers (plain_10m); framed: codec plus zstd framing "
+                         "of the columns, overlapped vs step by step (plain_10m)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for hardware-counter runs: one 1 GiB calibration copy, then one "
                          "compress and one decompress of the workload; prints a JSON summary")
@@ -581,7 +640,8 @@ def main() -> int:
 
     if args.workload is None:
         args.workload = {"roundtrip": "plain_10m", "batching": "walks_2m",
-                         "stream": "plain_10m", "serve": "walks_2m"}[args.mode]
+                         "stream": "plain_10m", "serve": "walks_2m",
+                         "framed": "plain_10m"}[args.mode]
     cfg = synth.CONFIGS[args.workload]
     log(f"[bench] generating {cfg.name}")
     tr = synth.generate(cfg)
@@ -600,6 +660,8 @@ def main() -> int:
         return run_stream(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     if args.mode == "serve":
         return run_serve(args, ctx, cfg, tr, gfa_bytes, d_nodes)
+    if args.mode == "framed":
+        return run_framed(args, ctx, cfg, tr, gfa_bytes, d_nodes)
     d_out = codec.DeviceArray(ctx, tr.n_steps, np.int32)
 
     if args.pmc_pass:

@@ -36,6 +36,15 @@ class _Block(ctypes.Structure):
                 ("payload_size", _u64)]
 
 
+class _Framed(ctypes.Structure):
+    _fields_ = [("start_id", _u32), ("end_id", _u32), ("n_rules", _u64),
+                ("rounds_run", ctypes.c_int32), ("delta_round", ctypes.c_int32),
+                ("n_trav", _u64), ("n_paths", _u32),
+                ("encoded_lengths", ctypes.POINTER(ctypes.c_uint32)),
+                ("rules_first", _Block), ("rules_second", _Block), ("paths", _Block),
+                ("walks", _Block), ("gpu_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 class _Layer(ctypes.Structure):
     _fields_ = [("start_id", ctypes.c_uint32), ("end_id", ctypes.c_uint32),
                 ("row_offset", _u64)]
@@ -101,6 +110,17 @@ _SIGNATURES = {
     "gfz_unframe_rule_columns": ([_vp, _u64, _u64, _vp, _u64, _u64, _vp, _vp, _u64],
                                  ctypes.c_int),
     "gfz_block_free": ([_vp], None),
+    "gfz_framer_begin": ([_vp, _u64, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_framer_ready": ([_vp, _u64], ctypes.c_int),
+    "gfz_framer_end": ([_vp, _vp], ctypes.c_int),
+    "gfz_framer_free": ([_vp], None),
+    "gfz_compress_framed": ([_vp, _vp, _u64, _vp, _u64, _u32, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, _u32, ctypes.c_int, ctypes.POINTER(_Framed)],
+                            ctypes.c_int),
+    "gfz_framed_free": ([ctypes.POINTER(_Framed)], None),
+    "gfz_decompress_framed": ([_vp, ctypes.POINTER(_Framed), _vp, _vp, _u64, _u32,
+                               ctypes.POINTER(_u64), _vp, ctypes.POINTER(ctypes.c_int), _vp],
+                              ctypes.c_int),
     "gfz_resident_load": ([_vp, _vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_uint32,
                            ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_resident_free": ([_vp], None),
@@ -567,6 +587,129 @@ def unframe_rule_columns(first: tuple[int, bytes], second: tuple[int, bytes], n_
                                           int(second[0]), f.ctypes.data, s_.ctypes.data,
                                           int(n_rules)))
     return f[:n_rules], s_[:n_rules]
+
+
+class Framer:
+    """Piecewise zstd framing of one column (gfz_framer_*): ``ready(n)`` declares
+    bytes [0, n) of the column final; ``end()`` returns (original_size, payload),
+    the bytes gfz_frame_int32 gives for the same data."""
+
+    def __init__(self, column: np.ndarray, threads: int = 0):
+        self.column = column  # the library reads it until end()
+        h = _vp()
+        _check(lib().gfz_framer_begin(column.ctypes.data if column.nbytes else None,
+                                      column.nbytes, int(threads), ctypes.byref(h)))
+        self.handle = h.value
+
+    def ready(self, nbytes: int) -> None:
+        _check(lib().gfz_framer_ready(self.handle, int(nbytes)))
+
+    def end(self) -> tuple[int, bytes]:
+        b = _Block()
+        try:
+            _check(lib().gfz_framer_end(self.handle, ctypes.byref(b)))
+            return _take_block(b)
+        finally:
+            lib().gfz_framer_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().gfz_framer_free(self.handle)
+            self.handle = None
+
+
+@dataclass
+class Framed:
+    """The traversal fields of CompressedData, framed (gfz_compress_framed)."""
+
+    start_id: int
+    end_id: int
+    n_rules: int
+    rounds_run: int
+    delta_round: int
+    n_paths: int
+    encoded_lengths: np.ndarray            # uint32 per traversal
+    rules_first: tuple[int, bytes]         # (original_size, zstd payload)
+    rules_second: tuple[int, bytes]
+    paths: tuple[int, bytes]
+    walks: tuple[int, bytes]
+    gpu_ms: float = 0.0
+    total_ms: float = 0.0
+
+
+def compress_framed(ctx: Context, nodes, lengths, n_paths: int = 0, rounds: int = 8,
+                    threshold: int = 2, delta_round: int = 1, threads: int = 0) -> Framed:
+    """Grammar-compress and frame the columns, the framing overlapped with the
+    copies out of HBM (gfz_compress_framed)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    flags = 0
+    if isinstance(nodes, DeviceArray):
+        flags |= GFZ_INPUT_DEVICE
+        nptr, nn = nodes.ptr, nodes.n
+    else:
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        nptr, nn = nodes.ctypes.data, nodes.size
+    out = _Framed()
+    _check(lib().gfz_compress_framed(ctx.handle, nptr, nn, lengths.ctypes.data, lengths.size,
+                                     int(n_paths), rounds, threshold, delta_round, flags,
+                                     int(threads), ctypes.byref(out)))
+    try:
+        def blk(b: _Block):
+            data = ctypes.string_at(b.payload, b.payload_size) if b.payload_size else b""
+            return int(b.original_size), data
+
+        el = (np.ctypeslib.as_array(out.encoded_lengths, shape=(lengths.size,)).copy()
+              if lengths.size else np.zeros(0, dtype=np.uint32))
+        return Framed(int(out.start_id), int(out.end_id), int(out.n_rules),
+                      int(out.rounds_run), int(out.delta_round), int(out.n_paths), el,
+                      blk(out.rules_first), blk(out.rules_second), blk(out.paths),
+                      blk(out.walks), float(out.gpu_ms), float(out.total_ms))
+    finally:
+        lib().gfz_framed_free(ctypes.byref(out))
+
+
+def decompress_framed(ctx: Context, f: Framed, orig_lengths=None):
+    """Unframe and decode (gfz_decompress_framed). Returns (nodes, lengths,
+    length_mismatch, timings_ms) with timings = (rulebook built, payloads decoded,
+    done) on the host clock."""
+    keep = []
+
+    def blk(pair):
+        b = _Block()
+        b.original_size = int(pair[0])
+        if pair[1]:
+            buf = ctypes.create_string_buffer(pair[1], len(pair[1]))
+            keep.append(buf)
+            b.payload = ctypes.cast(buf, ctypes.POINTER(ctypes.c_uint8))
+        b.payload_size = len(pair[1])
+        return b
+
+    el = np.ascontiguousarray(f.encoded_lengths, dtype=np.uint32)
+    s = _Framed()
+    s.start_id, s.end_id, s.n_rules = f.start_id, f.end_id, f.n_rules
+    s.rounds_run, s.delta_round = f.rounds_run, f.delta_round
+    s.n_trav, s.n_paths = el.size, f.n_paths
+    s.encoded_lengths = el.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    s.rules_first, s.rules_second = blk(f.rules_first), blk(f.rules_second)
+    s.paths, s.walks = blk(f.paths), blk(f.walks)
+    ol = None if orig_lengths is None else np.ascontiguousarray(orig_lengths, dtype=np.uint32)
+    n_out, mism = _u64(), ctypes.c_int()
+    lens = np.zeros(max(1, el.size), dtype=np.uint32)
+    tms = (ctypes.c_double * 3)()
+    # Size first (out_cap 0 fails with the decoded size), then decode.
+    cap = int(ol.sum()) if ol is not None else None
+    if cap is None:
+        rc = lib().gfz_decompress_framed(ctx.handle, ctypes.byref(s), None, None, 0, 0,
+                                         ctypes.byref(n_out), None, None, None)
+        if rc != 0 and n_out.value == 0:
+            _check(rc)
+        cap = int(n_out.value)
+    out = np.empty(max(1, cap), dtype=np.int32)
+    _check(lib().gfz_decompress_framed(ctx.handle, ctypes.byref(s), _ptr(ol), out.ctypes.data,
+                                       cap, 0, ctypes.byref(n_out), lens.ctypes.data,
+                                       ctypes.byref(mism), tms))
+    return (out[: n_out.value], lens[: el.size], bool(mism.value), tuple(tms))
 
 
 # ---- resident decoding / extract ----------------------------------------------

@@ -73,44 +73,69 @@ class TraversalColumns:
 
 
 def compress_traversals_gpu(ctx: codec.Context, nodes, lengths, n_paths: int, rounds: int = 8,
-                            threshold: int = 2, delta_round: int = 1,
-                            threads: int = 0) -> TraversalColumns:
+                            threshold: int = 2, delta_round: int = 1, threads: int = 0,
+                            overlapped: bool = True) -> TraversalColumns:
     """Grammar-compress paths (the first n_paths traversals) and walks on the GPU and
     frame the columns as the reference CPU workflow does
-    (src/workflows/compression_workflow.cpp:548-700, compression_utils.cpp:38-54,165-193)."""
+    (src/workflows/compression_workflow.cpp:548-700, compression_utils.cpp:38-54,165-193).
+    overlapped: one gfz_compress_framed call (columns framed on host threads as their
+    pieces leave HBM); else the grammar comes back whole and is framed column after
+    column, as the reference does. Both give the same bytes."""
     lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
     if not 0 <= n_paths <= lengths.size:
         raise ValueError(f"n_paths {n_paths} outside 0..{lengths.size}")
-    got = codec.compress_traversals(ctx, nodes, lengths, n_paths, rounds, threshold, delta_round)
-    enc_len = got.encoded_lengths.astype(np.uint32)
-    split = int(enc_len[:n_paths].sum(dtype=np.uint64))
     out = TraversalColumns()
-    n_rules = int(got.rules_first.size)
-    out.layer_rule_ranges = [LayerRuleRange(2, got.start_id, got.end_id, 0, 2 * n_rules)]
-    f, s = codec.frame_rule_columns(got.rules_first, got.rules_second, threads)
-    out.rules_first_zstd, out.rules_second_zstd = ZstdBlock.of(f), ZstdBlock.of(s)
     out.delta_round = max(0, int(delta_round))
+    if overlapped:
+        f = codec.compress_framed(ctx, nodes, lengths, n_paths, rounds, threshold, delta_round,
+                                  threads)
+        enc_len = f.encoded_lengths.astype(np.uint32)
+        out.layer_rule_ranges = [LayerRuleRange(2, f.start_id, f.end_id, 0, 2 * f.n_rules)]
+        out.rules_first_zstd = ZstdBlock.of(f.rules_first)
+        out.rules_second_zstd = ZstdBlock.of(f.rules_second)
+        out.paths_zstd = ZstdBlock.of(f.paths)
+        out.walks_zstd = ZstdBlock.of(f.walks)
+    else:
+        got = codec.compress_traversals(ctx, nodes, lengths, n_paths, rounds, threshold,
+                                        delta_round)
+        enc_len = got.encoded_lengths.astype(np.uint32)
+        split = int(enc_len[:n_paths].sum(dtype=np.uint64))
+        n_rules = int(got.rules_first.size)
+        out.layer_rule_ranges = [LayerRuleRange(2, got.start_id, got.end_id, 0, 2 * n_rules)]
+        f, s = codec.frame_rule_columns(got.rules_first, got.rules_second, threads)
+        out.rules_first_zstd, out.rules_second_zstd = ZstdBlock.of(f), ZstdBlock.of(s)
+        out.paths_zstd = ZstdBlock.of(codec.frame_int32(got.encoded[:split], threads))
+        out.walks_zstd = ZstdBlock.of(codec.frame_int32(got.encoded[split:], threads))
     out.sequence_lengths = enc_len[:n_paths].copy()
     out.original_path_lengths = lengths[:n_paths].copy()
-    out.paths_zstd = ZstdBlock.of(codec.frame_int32(got.encoded[:split], threads))
     out.walk_lengths = enc_len[n_paths:].copy()
     out.original_walk_lengths = lengths[n_paths:].copy()
-    out.walks_zstd = ZstdBlock.of(codec.frame_int32(got.encoded[split:], threads))
     return out
 
 
-def decompress_traversals_gpu(ctx: codec.Context, cols: TraversalColumns):
+def decompress_traversals_gpu(ctx: codec.Context, cols: TraversalColumns,
+                              overlapped: bool = True):
     """Decode every path and walk on the GPU: (nodes, lengths, n_paths), paths first
     (decompression_workflow.cpp:110-174 semantics; lengths that disagree with the stored
-    originals only warn there, and are returned as decoded here)."""
+    originals only warn there, and are returned as decoded here).
+    overlapped: one gfz_decompress_framed call (payloads decoded on host threads,
+    rulebook built meanwhile, columns uploaded in pieces); else unframe everything,
+    then upload and decode, as the reference does."""
+    enc_len = np.concatenate([cols.sequence_lengths, cols.walk_lengths]).astype(np.uint32)
+    orig = np.concatenate([cols.original_path_lengths,
+                           cols.original_walk_lengths]).astype(np.uint32)
+    if overlapped:
+        f = codec.Framed(cols.min_rule_id, cols.min_rule_id + cols.n_rules, cols.n_rules, 0,
+                         cols.delta_round, int(cols.sequence_lengths.size), enc_len,
+                         cols.rules_first_zstd.pair(), cols.rules_second_zstd.pair(),
+                         cols.paths_zstd.pair(), cols.walks_zstd.pair())
+        dec, lens, _, _ = codec.decompress_framed(ctx, f, orig)
+        return dec, lens, int(cols.sequence_lengths.size)
     rf, rs = codec.unframe_rule_columns(cols.rules_first_zstd.pair(),
                                         cols.rules_second_zstd.pair(), cols.n_rules)
     paths = codec.unframe_int32(cols.paths_zstd.payload, cols.paths_zstd.original_size)
     walks = codec.unframe_int32(cols.walks_zstd.payload, cols.walks_zstd.original_size)
     enc = np.concatenate([paths, walks]).astype(np.int32)
-    enc_len = np.concatenate([cols.sequence_lengths, cols.walk_lengths]).astype(np.uint32)
-    orig = np.concatenate([cols.original_path_lengths,
-                           cols.original_walk_lengths]).astype(np.uint32)
     rb = codec.Rulebook(ctx, rf, rs, cols.min_rule_id)
     try:
         dec, lens, _ = codec.decompress_traversals(ctx, rb, enc, enc_len, orig, cols.delta_round)

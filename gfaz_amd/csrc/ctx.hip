@@ -107,6 +107,8 @@ Ctx::~Ctx() {
     (void)hipEventDestroy(e);
   if (pinned_)
     (void)hipHostFree(pinned_);
+  if (staging_)
+    (void)hipHostFree(staging_);
   if (scal_)
     (void)hipHostFree(scal_);
   if (mail_)
@@ -234,6 +236,21 @@ void *Ctx::pinned(size_t bytes) {
   GFZ_HIP(hipHostMalloc(&pinned_, bytes, hipHostMallocDefault));
   pinned_bytes_ = bytes;
   return pinned_;
+}
+
+void *Ctx::column_staging(size_t bytes) {
+  if (bytes <= staging_bytes_)
+    return staging_;
+  if (staging_) {
+    // Calls that copy through it drain their copy streams before returning.
+    GFZ_HIP(hipStreamSynchronize(stream_));
+    GFZ_HIP(hipHostFree(staging_));
+    staging_ = nullptr;
+    staging_bytes_ = 0;
+  }
+  GFZ_HIP(hipHostMalloc(&staging_, bytes, hipHostMallocDefault));
+  staging_bytes_ = bytes;
+  return staging_;
 }
 
 }  // namespace gfz

@@ -138,6 +138,9 @@ class Ctx {
 
   // Pinned host staging buffer of at least `bytes`, reused across calls.
   void *pinned(size_t bytes);
+  // A second one, for columns handed between the host and the device in pieces
+  // (framing); separate so the codec's own staging never lands in it.
+  void *column_staging(size_t bytes);
   // Small pinned area (512 x 8 bytes) for device-to-host scalar readbacks.
   uint64_t *scalars();
   // Copy n 64-bit device words into scalars()[at..) (no sync).
@@ -203,6 +206,8 @@ class Ctx {
   void *pinned_ = nullptr;
   uint64_t *scal_ = nullptr;
   size_t pinned_bytes_ = 0;
+  void *staging_ = nullptr;
+  size_t staging_bytes_ = 0;
 };
 
 // RAII: make `ctx`'s device current for the duration of a call.

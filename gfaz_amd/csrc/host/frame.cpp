@@ -10,8 +10,13 @@
 //   and the inverses (src/codec/codec.cpp:88-95,224-244).
 // zstd is the system libzstd the reference links by default (CMakeLists.txt:14).
 #include "abi_error.hpp"
+#include "frame.hpp"
 
+// The block-level (buffer-less) API used for piecewise framing.
+#define ZSTD_STATIC_LINKING_ONLY
 #include <zstd.h>
+
+#include <algorithm>
 
 #include <cstdlib>
 #include <cstring>
@@ -51,32 +56,8 @@ int zstd_workers(int threads) {
 }
 
 void frame_bytes(const void *data, size_t bytes, int threads, gfz_block *out) {
-  out->original_size = bytes;
-  out->payload = nullptr;
-  out->payload_size = 0;
-  if (!bytes)
-    return;
-  ZSTD_CCtx *cctx = ZSTD_createCCtx();
-  if (!cctx)
-    throw gfz::Error(GFZ_E_NOMEM, "unable to create a zstd compression context");
-  size_t r = ZSTD_CCtx_setParameter(cctx, ZSTD_c_compressionLevel, zstd_level());
-  if (!ZSTD_isError(r))
-    (void)ZSTD_CCtx_setParameter(cctx, ZSTD_c_nbWorkers, zstd_workers(threads));
-  const size_t bound = ZSTD_compressBound(bytes);
-  uint8_t *buf = static_cast<uint8_t *>(std::malloc(bound));
-  if (!buf) {
-    ZSTD_freeCCtx(cctx);
-    throw std::bad_alloc();
-  }
-  const size_t n = ZSTD_isError(r) ? r : ZSTD_compress2(cctx, buf, bound, data, bytes);
-  ZSTD_freeCCtx(cctx);
-  if (ZSTD_isError(n)) {
-    std::free(buf);
-    throw gfz::Error(GFZ_E_INVALID, std::string("zstd compression failed: ") +
-                                        ZSTD_getErrorName(n));
-  }
-  out->payload = buf;
-  out->payload_size = n;
+  gfz::ColumnFramer f(data, bytes, threads);
+  f.finish(out);
 }
 
 void unframe_bytes(const uint8_t *payload, uint64_t payload_size, uint64_t original_size,
@@ -86,15 +67,8 @@ void unframe_bytes(const uint8_t *payload, uint64_t payload_size, uint64_t origi
   if (original_size > cap_bytes)
     throw gfz::Error(GFZ_E_INVALID, "output holds " + std::to_string(cap_bytes) + " < " +
                                         std::to_string(original_size) + " bytes");
-  if (!payload)
-    throw gfz::Error(GFZ_E_CORRUPT, "empty zstd payload for a non-empty column");
-  const size_t n = ZSTD_decompress(dst, original_size, payload, payload_size);
-  if (ZSTD_isError(n))
-    throw gfz::Error(GFZ_E_CORRUPT, std::string("zstd decompression failed: ") +
-                                        ZSTD_getErrorName(n));
-  if (n != original_size)
-    throw gfz::Error(GFZ_E_CORRUPT, "zstd payload decodes to " + std::to_string(n) +
-                                        " bytes, block says " + std::to_string(original_size));
+  gfz::ColumnUnframer u(payload, payload_size, dst, original_size);
+  u.next(original_size);
 }
 
 void delta_encode(std::vector<int32_t> &v) {
@@ -108,6 +82,202 @@ void delta_decode(int32_t *v, uint64_t n) {
 }
 
 }  // namespace
+
+namespace gfz {
+
+// ---- ColumnFramer -------------------------------------------------------------
+
+struct ColumnFramer::Impl {
+  const uint8_t *base = nullptr;
+  uint64_t total = 0;
+  uint64_t fed = 0;          // bytes handed to zstd so far
+  bool blockwise = false;    // block-level API (else one ZSTD_compress2 at the end)
+  bool ended = false;
+  ZSTD_CCtx *cctx = nullptr;
+  uint8_t *buf = nullptr;    // frame being written
+  uint64_t cap = 0, pos = 0;
+
+  ~Impl() {
+    if (cctx)
+      ZSTD_freeCCtx(cctx);
+    std::free(buf);
+  }
+
+  void fail_zstd(size_t r, const char *what) {
+    throw Error(GFZ_E_INVALID, std::string(what) + ZSTD_getErrorName(r));
+  }
+};
+
+ColumnFramer::ColumnFramer(const void *base, uint64_t bytes, int threads)
+    : impl_(std::make_unique<Impl>()) {
+  Impl &m = *impl_;
+  m.base = static_cast<const uint8_t *>(base);
+  m.total = bytes;
+  if (!bytes)
+    return;
+  m.cctx = ZSTD_createCCtx();
+  if (!m.cctx)
+    throw Error(GFZ_E_NOMEM, "unable to create a zstd compression context");
+  const int level = zstd_level();
+  const size_t r = ZSTD_CCtx_setParameter(m.cctx, ZSTD_c_compressionLevel, level);
+  if (ZSTD_isError(r))
+    m.fail_zstd(r, "zstd compression failed: ");
+  // The worker count is refused by a libzstd built without threads; ZSTD_compress2
+  // then compresses on the calling thread, which the block-level API reproduces.
+  m.blockwise = ZSTD_isError(ZSTD_CCtx_setParameter(m.cctx, ZSTD_c_nbWorkers,
+                                                    zstd_workers(threads)));
+  m.cap = ZSTD_compressBound(bytes);
+  m.buf = static_cast<uint8_t *>(std::malloc(m.cap));
+  if (!m.buf)
+    throw std::bad_alloc();
+  if (m.blockwise) {
+    // ZSTD_compress2 derives its parameters from the level and the known size
+    // and writes the content size into the frame header; so do these.
+    const ZSTD_parameters p = ZSTD_getParams(level, bytes, 0);
+    const size_t b = ZSTD_compressBegin_advanced(m.cctx, nullptr, 0, p, bytes);
+    if (ZSTD_isError(b))
+      m.fail_zstd(b, "zstd compression failed: ");
+  }
+}
+
+ColumnFramer::~ColumnFramer() = default;
+
+void ColumnFramer::ready(uint64_t bytes_ready) {
+  Impl &m = *impl_;
+  if (!m.total || m.ended)
+    return;
+  bytes_ready = std::min(bytes_ready, m.total);
+  if (!m.blockwise) {
+    if (bytes_ready < m.total)
+      return;
+    const size_t n = ZSTD_compress2(m.cctx, m.buf, m.cap, m.base, m.total);
+    if (ZSTD_isError(n))
+      m.fail_zstd(n, "zstd compression failed: ");
+    m.pos = n;
+    m.fed = m.total;
+    m.ended = true;
+    return;
+  }
+  const uint64_t upto =
+      bytes_ready == m.total ? m.total : bytes_ready / kFrameBlockBytes * kFrameBlockBytes;
+  if (upto <= m.fed)
+    return;
+  const bool last = upto == m.total;
+  const size_t n =
+      last ? ZSTD_compressEnd(m.cctx, m.buf + m.pos, m.cap - m.pos, m.base + m.fed, upto - m.fed)
+           : ZSTD_compressContinue(m.cctx, m.buf + m.pos, m.cap - m.pos, m.base + m.fed,
+                                   upto - m.fed);
+  if (ZSTD_isError(n))
+    m.fail_zstd(n, "zstd compression failed: ");
+  m.pos += n;
+  m.fed = upto;
+  m.ended = last;
+}
+
+void ColumnFramer::finish(gfz_block *out) {
+  Impl &m = *impl_;
+  ready(m.total);
+  out->original_size = m.total;
+  out->payload = nullptr;
+  out->payload_size = 0;
+  if (!m.total)
+    return;
+  out->payload = m.buf;
+  out->payload_size = m.pos;
+  m.buf = nullptr;
+}
+
+// ---- ColumnUnframer -----------------------------------------------------------
+
+struct ColumnUnframer::Impl {
+  const uint8_t *src = nullptr;
+  uint64_t src_size = 0, ip = 0;
+  uint8_t *dst = nullptr;
+  uint64_t total = 0, op = 0;
+  bool single_frame = false;
+  bool finished = false;
+  ZSTD_DCtx *dctx = nullptr;
+
+  ~Impl() {
+    if (dctx)
+      ZSTD_freeDCtx(dctx);
+  }
+};
+
+ColumnUnframer::ColumnUnframer(const uint8_t *payload, uint64_t payload_size, void *dst,
+                               uint64_t original_size)
+    : impl_(std::make_unique<Impl>()) {
+  Impl &m = *impl_;
+  m.src = payload;
+  m.src_size = payload_size;
+  m.dst = static_cast<uint8_t *>(dst);
+  m.total = original_size;
+  if (!original_size) {
+    m.finished = true;
+    return;
+  }
+  if (!payload)
+    throw Error(GFZ_E_CORRUPT, "empty zstd payload for a non-empty column");
+  // A payload of exactly one frame is decoded block by block straight into dst;
+  // anything else (several frames) in one ZSTD_decompress call, as the reference.
+  const size_t fsize = ZSTD_findFrameCompressedSize(payload, payload_size);
+  m.single_frame = !ZSTD_isError(fsize) && fsize == payload_size;
+  m.dctx = ZSTD_createDCtx();
+  if (!m.dctx)
+    throw Error(GFZ_E_NOMEM, "unable to create a zstd decompression context");
+  if (m.single_frame) {
+    const size_t r = ZSTD_decompressBegin(m.dctx);
+    if (ZSTD_isError(r))
+      throw Error(GFZ_E_CORRUPT, std::string("zstd decompression failed: ") +
+                                     ZSTD_getErrorName(r));
+  }
+}
+
+ColumnUnframer::~ColumnUnframer() = default;
+
+bool ColumnUnframer::done() const { return impl_->finished; }
+
+uint64_t ColumnUnframer::next(uint64_t min_bytes) {
+  Impl &m = *impl_;
+  if (m.finished)
+    return m.op;
+  auto bad = [](size_t r) {
+    throw Error(GFZ_E_CORRUPT, std::string("zstd decompression failed: ") + ZSTD_getErrorName(r));
+  };
+  if (!m.single_frame) {
+    const size_t n = ZSTD_decompressDCtx(m.dctx, m.dst, m.total, m.src, m.src_size);
+    if (ZSTD_isError(n))
+      bad(n);
+    if (n != m.total)
+      throw Error(GFZ_E_CORRUPT, "zstd payload decodes to " + std::to_string(n) +
+                                     " bytes, block says " + std::to_string(m.total));
+    m.op = m.total;
+    m.finished = true;
+    return m.op;
+  }
+  const uint64_t goal = m.op + std::max<uint64_t>(min_bytes, 1);
+  size_t want = ZSTD_nextSrcSizeToDecompress(m.dctx);
+  while (want && m.op < goal) {
+    if (want > m.src_size - m.ip)
+      throw Error(GFZ_E_CORRUPT, "zstd payload is truncated");
+    const size_t n =
+        ZSTD_decompressContinue(m.dctx, m.dst + m.op, m.total - m.op, m.src + m.ip, want);
+    if (ZSTD_isError(n))
+      bad(n);
+    m.ip += want;
+    m.op += n;
+    want = ZSTD_nextSrcSizeToDecompress(m.dctx);
+  }
+  if (!want) {
+    if (m.op != m.total)
+      throw Error(GFZ_E_CORRUPT, "zstd payload decodes to " + std::to_string(m.op) +
+                                     " bytes, block says " + std::to_string(m.total));
+    m.finished = true;
+  }
+  return m.op;
+}
+
+}  // namespace gfz
 
 extern "C" {
 
@@ -166,6 +336,35 @@ int gfz_unframe_rule_columns(const uint8_t *first_payload, uint64_t first_payloa
     delta_decode(second, n_rules);
   });
 }
+
+struct gfz_framer {
+  gfz::ColumnFramer f;
+  uint64_t bytes;
+  gfz_framer(const void *base, uint64_t n, int threads) : f(base, n, threads), bytes(n) {}
+};
+
+int gfz_framer_begin(const void *column, uint64_t bytes, int threads, gfz_framer **out) {
+  if (!out || (bytes && !column))
+    return gfz::fail(GFZ_E_INVALID, std::string(kFramePrefix) + "null argument");
+  *out = nullptr;
+  return gfz::guarded(kFramePrefix, [&] { *out = new gfz_framer(column, bytes, threads); });
+}
+
+int gfz_framer_ready(gfz_framer *f, uint64_t bytes_ready) {
+  if (!f)
+    return gfz::fail(GFZ_E_INVALID, std::string(kFramePrefix) + "null framer");
+  if (bytes_ready > f->bytes)
+    return gfz::fail(GFZ_E_INVALID, std::string(kFramePrefix) + "bytes_ready exceeds the column");
+  return gfz::guarded(kFramePrefix, [&] { f->f.ready(bytes_ready); });
+}
+
+int gfz_framer_end(gfz_framer *f, gfz_block *out) {
+  if (!f || !out)
+    return gfz::fail(GFZ_E_INVALID, std::string(kFramePrefix) + "null argument");
+  return gfz::guarded(kFramePrefix, [&] { f->f.finish(out); });
+}
+
+void gfz_framer_free(gfz_framer *f) { delete f; }
 
 void gfz_block_free(gfz_block *b) {
   if (!b)

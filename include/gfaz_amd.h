@@ -448,6 +448,67 @@ int gfz_unframe_rule_columns(const uint8_t *first_payload, uint64_t first_payloa
                              int32_t *first, int32_t *second, uint64_t n_rules);
 void gfz_block_free(gfz_block *b);
 
+/* Piecewise framing of one column in a contiguous host buffer: the chunked
+ * handoff the pipelined calls below use, open to callers that produce a column in
+ * pieces. begin() takes the column's address and full size; ready(n) says bytes
+ * [0, n) are final (they must then stay unchanged until end()); end() completes
+ * the frame. The payload has the bytes of gfz_frame_int32 over the same data, i.e.
+ * of zstd_compress_bytes (src/codec/codec.cpp:180-216): zstd is fed whole 128 KiB
+ * blocks of the one buffer, which keeps every block's window and entropy state as
+ * in a single ZSTD_compress2 call. With a libzstd that accepts the worker count
+ * (and so compresses with threads), the frame depends on the worker split and
+ * end() makes that single call instead. free() releases a framer (ended or not). */
+typedef struct gfz_framer gfz_framer;
+int gfz_framer_begin(const void *column, uint64_t bytes, int threads, gfz_framer **out);
+int gfz_framer_ready(gfz_framer *f, uint64_t bytes_ready);
+int gfz_framer_end(gfz_framer *f, gfz_block *out);
+void gfz_framer_free(gfz_framer *f);
+
+/* ---- traversal codec with its framing overlapped -------------------------------
+ * The GPU codec and the traversal-column framing in one call each way. They
+ * replace, on the compress side, run_path_compression_gpu followed by
+ * finalize_traversal_columns (src/gpu/compression/path_compression_gpu_rolling.cu:29-56,
+ * 596; CPU: compress_rule_columns, compress_path_fields, compress_walk_fields,
+ * src/workflows/compression_workflow.cpp:281-383), and on the decompress side
+ * prepare_gpu_traversal_rulebook + prepare_gpu_traversal_payload + the decode
+ * (src/gpu/decompression/decompression_primitives_gpu.cu:18-56). The reference
+ * runs these steps one after the other; here:
+ *   compress: the four columns (rule first/second, paths, walks) are copied to
+ *     pinned host memory in 4 MiB pieces, interleaved, and each is framed on its
+ *     own host thread as its pieces land (rule columns delta-encoded on the way);
+ *   decompress: four host threads decode the four payloads at once; the rule
+ *     columns go up first and the rulebook builds on the GPU while the traversal
+ *     payloads are still being decoded, each handed to the device in 4 MiB pieces
+ *     on its own copy stream as they come out.
+ * Payload bytes equal gfz_frame_rule_columns / gfz_frame_int32 (so the reference's
+ * .gfaz fields), and the decoded traversals equal gfz_decompress_traversals. Input
+ * flags as gfz_compress_traversals (GFZ_INPUT_DEVICE) and gfz_decompress_traversals
+ * (GFZ_OUTPUT_DEVICE). threads: zstd worker count as gfz_frame_int32. */
+typedef struct gfz_framed {
+  uint32_t start_id, end_id; /* the one LayerRuleRange of the rule columns */
+  uint64_t n_rules;
+  int32_t rounds_run;
+  int32_t delta_round;       /* as given (negative stored as 0) */
+  uint64_t n_trav;
+  uint32_t n_paths;          /* traversals [0, n_paths) are paths, the rest walks */
+  uint32_t *encoded_lengths; /* n_trav; library-owned */
+  gfz_block rules_first, rules_second, paths, walks;
+  double gpu_ms;             /* host clock: the codec, input upload included */
+  double total_ms;           /* host clock: the whole call */
+} gfz_framed;
+
+int gfz_compress_framed(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
+                        const uint32_t *lengths, uint64_t n_trav, uint32_t n_paths, int rounds,
+                        int threshold, int delta_round, uint32_t flags, int threads,
+                        gfz_framed *out);
+void gfz_framed_free(gfz_framed *f);
+/* orig_lengths (optional) and out_lengths (optional) as gfz_decompress_traversals.
+ * timings_ms (optional, 3 values, host clock from the call's start): rulebook
+ * built, payloads all decoded on the host, traversals decoded. */
+int gfz_decompress_framed(gfz_ctx *ctx, const gfz_framed *in, const uint32_t *orig_lengths,
+                          int32_t *out, uint64_t out_cap, uint32_t flags, uint64_t *n_out,
+                          uint32_t *out_lengths, int *length_mismatch, double *timings_ms);
+
 /* ---- measurement support -------------------------------------------------
  * No counterpart in the reference: a plain copy of `bytes` (a multiple of 16)
  * between device buffers, timed with HIP events on the ctx stream. Profiles use
