@@ -45,6 +45,21 @@ class _Framed(ctypes.Structure):
                 ("walks", _Block), ("gpu_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
+class _Strings(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_char_p), ("off", ctypes.POINTER(_u64)), ("n", _u64)]
+
+
+class _WalkKeys(ctypes.Structure):
+    _fields_ = [("sample_ids", _Strings), ("hap_indices", ctypes.POINTER(ctypes.c_uint32)),
+                ("seq_ids", _Strings), ("seq_starts", ctypes.POINTER(ctypes.c_int64)),
+                ("seq_ends", ctypes.POINTER(ctypes.c_int64))]
+
+
+class _Lines(ctypes.Structure):
+    _fields_ = [("text", ctypes.POINTER(ctypes.c_uint8)), ("n_bytes", _u64),
+                ("line_off", ctypes.POINTER(_u64)), ("n_lines", _u64)]
+
+
 class _Layer(ctypes.Structure):
     _fields_ = [("start_id", ctypes.c_uint32), ("end_id", ctypes.c_uint32),
                 ("row_offset", _u64)]
@@ -110,6 +125,22 @@ _SIGNATURES = {
     "gfz_unframe_rule_columns": ([_vp, _u64, _u64, _vp, _u64, _u64, _vp, _vp, _u64],
                                  ctypes.c_int),
     "gfz_block_free": ([_vp], None),
+    "gfz_catalog_create": ([ctypes.POINTER(_vp)], ctypes.c_int),
+    "gfz_catalog_free": ([_vp], None),
+    "gfz_catalog_add_paths": ([_vp, ctypes.POINTER(_Strings), ctypes.POINTER(_Strings), _u64],
+                              ctypes.c_int),
+    "gfz_catalog_add_walks": ([_vp, ctypes.POINTER(_WalkKeys), _u64], ctypes.c_int),
+    "gfz_catalog_info": ([_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)], ctypes.c_int),
+    "gfz_catalog_find_paths": ([_vp, ctypes.POINTER(_Strings), _vp], ctypes.c_int),
+    "gfz_catalog_find_walks": ([_vp, ctypes.POINTER(_WalkKeys), _vp], ctypes.c_int),
+    "gfz_catalog_find_walks_by_name": ([_vp, ctypes.POINTER(_Strings), _vp], ctypes.c_int),
+    "gfz_extract_path_lines": ([_vp, _vp, _vp, ctypes.POINTER(_Strings), ctypes.POINTER(_Lines)],
+                               ctypes.c_int),
+    "gfz_extract_walk_lines": ([_vp, _vp, _vp, ctypes.POINTER(_WalkKeys), ctypes.POINTER(_Lines)],
+                               ctypes.c_int),
+    "gfz_extract_walk_lines_by_name": ([_vp, _vp, _vp, ctypes.POINTER(_Strings),
+                                        ctypes.POINTER(_Lines)], ctypes.c_int),
+    "gfz_lines_free": ([ctypes.POINTER(_Lines)], None),
     "gfz_framer_begin": ([_vp, _u64, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "gfz_framer_ready": ([_vp, _u64], ctypes.c_int),
     "gfz_framer_end": ([_vp, _vp], ctypes.c_int),
@@ -976,3 +1007,127 @@ def parse_steps(ctx: Context, kind: int, text, line_off=None, out=None, fields=N
                              nodes.ctypes.data, nodes.size, lengths.ctypes.data,
                              ctypes.byref(n)))
     return nodes[: int(n.value)], lengths[:n_lines]
+
+
+# ---- extraction front-end: catalog + lines (gfz_catalog_*, gfz_extract_*_lines) ----
+
+def _as_bytes(v) -> bytes:
+    return v if isinstance(v, bytes) else str(v).encode()
+
+
+class _StringList:
+    """A list of strings as gfz_strings (keeps the buffers alive)."""
+
+    def __init__(self, items):
+        bs = [_as_bytes(v) for v in items]
+        self.blob = b"".join(bs)
+        off = np.zeros(len(bs) + 1, dtype=np.uint64)
+        if bs:
+            off[1:] = np.cumsum([len(b) for b in bs])
+        self.off = off
+        self.s = _Strings(self.blob, off.ctypes.data_as(ctypes.POINTER(_u64)), len(bs))
+
+
+class _WalkKeyList:
+    def __init__(self, keys):
+        keys = list(keys)
+        self.samples = _StringList([k[0] for k in keys])
+        self.seqs = _StringList([k[2] for k in keys])
+        self.hap = np.array([int(k[1]) for k in keys] or [0], dtype=np.uint32)
+        self.start = np.array([int(k[3]) for k in keys] or [0], dtype=np.int64)
+        self.end = np.array([int(k[4]) for k in keys] or [0], dtype=np.int64)
+        self.s = _WalkKeys(self.samples.s, self.hap.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                           self.seqs.s, self.start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                           self.end.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.n = len(keys)
+
+
+def _take_lines(l: _Lines) -> list[bytes]:
+    try:
+        text = ctypes.string_at(l.text, l.n_bytes) if l.n_bytes else b""
+        off = [int(l.line_off[i]) for i in range(l.n_lines + 1)] if l.n_lines else [0]
+        return [text[off[i]:off[i + 1]] for i in range(l.n_lines)]
+    finally:
+        lib().gfz_lines_free(ctypes.byref(l))
+
+
+class Catalog:
+    """Path names and walk identifiers of resident traversals (gfz_catalog_*).
+    Walk keys are (sample_id, hap_index, seq_id, seq_start, seq_end) tuples, -1 = '*'."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().gfz_catalog_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gfz_catalog_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_paths(self, names, first_trav_id: int, overlaps=None) -> None:
+        n = _StringList(names)
+        o = _StringList(overlaps) if overlaps is not None else None
+        _check(lib().gfz_catalog_add_paths(self.handle, ctypes.byref(n.s),
+                                           ctypes.byref(o.s) if o else None, int(first_trav_id)))
+
+    def add_walks(self, keys, first_trav_id: int) -> None:
+        k = _WalkKeyList(keys)
+        _check(lib().gfz_catalog_add_walks(self.handle, ctypes.byref(k.s), int(first_trav_id)))
+
+    def info(self) -> tuple[int, int]:
+        a, b = _u64(), _u64()
+        _check(lib().gfz_catalog_info(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
+
+    def find_paths(self, names) -> np.ndarray:
+        n = _StringList(names)
+        out = np.zeros(max(1, n.s.n), dtype=np.uint64)
+        _check(lib().gfz_catalog_find_paths(self.handle, ctypes.byref(n.s), out.ctypes.data))
+        return out[: n.s.n]
+
+    def find_walks(self, keys) -> np.ndarray:
+        k = _WalkKeyList(keys)
+        out = np.zeros(max(1, k.n), dtype=np.uint64)
+        _check(lib().gfz_catalog_find_walks(self.handle, ctypes.byref(k.s), out.ctypes.data))
+        return out[: k.n]
+
+    def find_walks_by_name(self, names) -> np.ndarray:
+        n = _StringList(names)
+        out = np.zeros(max(1, n.s.n), dtype=np.uint64)
+        _check(lib().gfz_catalog_find_walks_by_name(self.handle, ctypes.byref(n.s),
+                                                    out.ctypes.data))
+        return out[: n.s.n]
+
+
+def extract_path_lines(ctx: Context, res: "Resident", cat: Catalog, names) -> list[bytes]:
+    """extract_path_lines_by_name on a resident store: P lines, one per name."""
+    n = _StringList(names)
+    out = _Lines()
+    _check(lib().gfz_extract_path_lines(ctx.handle, res.handle, cat.handle, ctypes.byref(n.s),
+                                        ctypes.byref(out)))
+    return _take_lines(out)
+
+
+def extract_walk_lines(ctx: Context, res: "Resident", cat: Catalog, keys) -> list[bytes]:
+    """extract_walk_lines: W lines for (sample, hap, seq, start, end) keys."""
+    k = _WalkKeyList(keys)
+    out = _Lines()
+    _check(lib().gfz_extract_walk_lines(ctx.handle, res.handle, cat.handle, ctypes.byref(k.s),
+                                        ctypes.byref(out)))
+    return _take_lines(out)
+
+
+def extract_walk_lines_by_name(ctx: Context, res: "Resident", cat: Catalog, names) -> list[bytes]:
+    """extract_walk_lines_by_name: W lines for sample IDs."""
+    n = _StringList(names)
+    out = _Lines()
+    _check(lib().gfz_extract_walk_lines_by_name(ctx.handle, res.handle, cat.handle,
+                                                ctypes.byref(n.s), ctypes.byref(out)))
+    return _take_lines(out)

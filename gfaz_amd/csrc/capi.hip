@@ -9,6 +9,7 @@
 #include "resident.hpp"
 #include "text.hpp"
 #include "coverage.hpp"
+#include "host/catalog.hpp"
 
 #include <cctype>
 #include <climits>
@@ -52,6 +53,45 @@ T *host_copy(gfz::Ctx &ctx, const T *d, uint64_t n) {
   return h;
 }
 
+// Decoded size of traversals ids[0..n) of a resident store (range-checked).
+uint64_t extracted_size(const gfz_resident *r, const uint64_t *ids, uint64_t n_ids) {
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n_ids; ++i) {
+    if (ids[i] >= r->res.n_trav)
+      throw gfz::Error(GFZ_E_INVALID, "traversal index " + std::to_string(ids[i]) +
+                                          " out of range (" + std::to_string(r->res.n_trav) +
+                                          " stored)");
+    total += r->res.dec_len[ids[i]];
+  }
+  return total;
+}
+
+// Resident decode of ids into d_out (device); the tile size is autotuned per
+// size class, like the batch decoder's.
+void extract_device(gfz::Ctx &c, const gfz_resident *r, const uint64_t *ids, uint64_t n_ids,
+                    int32_t *d_out, uint64_t out_cap, uint64_t *offsets, uint64_t total) {
+  uint32_t cls = 0;
+  for (uint64_t n = total; n >= 4; n >>= 2)
+    ++cls;
+  const std::string key = "extract_items/" + std::to_string(cls);
+  int items = c.tuned(key);
+  if (items < 0 && c.autotune() && total >= (uint64_t(1) << 20)) {
+    float best = 0;
+    for (uint32_t cand : {8u, 16u, 32u}) {
+      gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);  // warm
+      const float ms = c.time_ms([&] {
+        gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);
+      });
+      if (items < 0 || ms < best) {
+        best = ms;
+        items = (int)cand;
+      }
+    }
+    c.set_tuned(key, items);
+  }
+  gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets,
+                        items < 0 ? 16u : (uint32_t)items);
+}
 }  // namespace
 
 extern "C" {
@@ -782,14 +822,7 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
   return guarded(kExtractPrefix, [&] {
     gfz::Ctx &c = *ctx->ctx;
     gfz::DeviceGuard g(c.device());
-    uint64_t total = 0;
-    for (uint64_t i = 0; i < n_ids; ++i) {
-      if (ids[i] >= r->res.n_trav)
-        throw gfz::Error(GFZ_E_INVALID, "traversal index " + std::to_string(ids[i]) +
-                                            " out of range (" + std::to_string(r->res.n_trav) +
-                                            " stored)");
-      total += r->res.dec_len[ids[i]];
-    }
+    const uint64_t total = extracted_size(r, ids, n_ids);
     if (n_out)
       *n_out = total;
     if (total > out_cap)
@@ -803,33 +836,122 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
       tmp = gfz::DBuf<int32_t>(c.pool(), std::max<uint64_t>(total, 1));
       d_out = tmp.get();
     }
-    // Tile size: autotuned per size class, like the batch decoder.
-    uint32_t cls = 0;
-    for (uint64_t n = total; n >= 4; n >>= 2)
-      ++cls;
-    const std::string key = "extract_items/" + std::to_string(cls);
-    int items = c.tuned(key);
-    if (items < 0 && c.autotune() && total >= (uint64_t(1) << 20)) {
-      float best = 0;
-      for (uint32_t cand : {8u, 16u, 32u}) {
-        gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);  // warm
-        const float ms = c.time_ms([&] {
-          gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets, cand);
-        });
-        if (items < 0 || ms < best) {
-          best = ms;
-          items = (int)cand;
-        }
-      }
-      c.set_tuned(key, items);
-    }
-    gfz::resident_extract(c, r->res, ids, n_ids, d_out, out_cap, offsets,
-                          items < 0 ? 16u : (uint32_t)items);
+    extract_device(c, r, ids, n_ids, d_out, out_cap, offsets, total);
     if (!(flags & GFZ_OUTPUT_DEVICE) && total) {
       GFZ_HIP(hipMemcpyAsync(out, d_out, total * 4, hipMemcpyDeviceToHost, c.stream()));
       c.sync();
     }
   });
+}
+
+// ---- extraction front-end: catalog lookups + resident decode + line text ------
+
+}  // extern "C"
+
+namespace {
+
+// Decodes the specs' traversals and formats their lines on the GPU.
+void extract_lines(gfz_ctx *ctx, const gfz_resident *r, int kind,
+                   const std::vector<gfz::LineSpec> &specs, gfz_lines *out) {
+  gfz::Ctx &c = *ctx->ctx;
+  gfz::DeviceGuard g(c.device());
+  const uint64_t n = specs.size();
+  if (!n) {
+    out->text = static_cast<uint8_t *>(std::malloc(1));
+    out->line_off = static_cast<uint64_t *>(std::calloc(1, 8));
+    if (!out->text || !out->line_off)
+      throw std::bad_alloc();
+    return;
+  }
+  std::vector<uint64_t> ids(n), head_off(n + 1, 0), tail_off(n + 1, 0);
+  std::vector<uint32_t> lens(n);
+  std::string heads, tails;
+  for (uint64_t i = 0; i < n; ++i) {
+    ids[i] = specs[i].trav_id;
+    heads += specs[i].head;
+    tails += specs[i].tail;
+    head_off[i + 1] = heads.size();
+    tail_off[i + 1] = tails.size();
+  }
+  const uint64_t total = extracted_size(r, ids.data(), n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t len = r->res.dec_len[ids[i]];
+    if (len > UINT32_MAX)
+      throw gfz::Error(GFZ_E_INVALID, "traversal " + std::to_string(ids[i]) +
+                                          " is too long for a line (" + std::to_string(len) +
+                                          " steps)");
+    lens[i] = (uint32_t)len;
+  }
+  gfz::DBuf<int32_t> d_nodes(c.pool(), std::max<uint64_t>(total, 1));
+  extract_device(c, r, ids.data(), n, d_nodes.get(), total, nullptr, total);
+  gfz::FormatPlan plan;
+  gfz::format_plan(c, kind, d_nodes.get(), total, lens.data(), n, head_off.data(),
+                   tail_off.data(), plan);
+  gfz::DBuf<uint8_t> d_txt(c.pool(), std::max<uint64_t>(plan.total, 1));
+  gfz::format_write(c, plan, reinterpret_cast<const uint8_t *>(heads.data()), head_off.data(),
+                    reinterpret_cast<const uint8_t *>(tails.data()), tail_off.data(),
+                    d_txt.get(), plan.total);
+  out->text = static_cast<uint8_t *>(std::malloc(std::max<uint64_t>(plan.total, 1)));
+  out->line_off = static_cast<uint64_t *>(std::malloc((n + 1) * 8));
+  if (!out->text || !out->line_off)
+    throw std::bad_alloc();
+  if (plan.total)
+    GFZ_HIP(hipMemcpyAsync(out->text, d_txt.get(), plan.total, hipMemcpyDeviceToHost,
+                           c.stream()));
+  std::memcpy(out->line_off, plan.line_off.data(), (n + 1) * 8);
+  out->n_bytes = plan.total;
+  out->n_lines = n;
+  c.sync();
+}
+
+template <class F>
+int lines_entry(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *cat, gfz_lines *out,
+                int kind, F specs) {
+  if (!ctx || !r || !cat || !out)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "null argument");
+  if (r->owner != ctx)
+    return fail(GFZ_E_INVALID, std::string(kExtractPrefix) + "resident state belongs to another ctx");
+  std::memset(out, 0, sizeof(*out));
+  return guarded(kExtractPrefix, [&] {
+    try {
+      extract_lines(ctx, r, kind, specs(), out);
+    } catch (...) {
+      gfz_lines_free(out);
+      throw;
+    }
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfz_extract_path_lines(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                           const gfz_strings *names, gfz_lines *out) {
+  return lines_entry(ctx, r, c, out, gfz::kTextP, [&] {
+    return c->cat.path_lines(gfz::strings_from_abi(names, "path names"));
+  });
+}
+
+int gfz_extract_walk_lines(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                           const gfz_walk_keys *keys, gfz_lines *out) {
+  return lines_entry(ctx, r, c, out, gfz::kTextW,
+                     [&] { return c->cat.walk_lines(gfz::walk_keys_from_abi(keys)); });
+}
+
+int gfz_extract_walk_lines_by_name(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                                   const gfz_strings *names, gfz_lines *out) {
+  return lines_entry(ctx, r, c, out, gfz::kTextW, [&] {
+    return c->cat.walk_lines_by_name(gfz::strings_from_abi(names, "walk names"));
+  });
+}
+
+void gfz_lines_free(gfz_lines *l) {
+  if (!l)
+    return;
+  std::free(l->text);
+  std::free(l->line_off);
+  std::memset(l, 0, sizeof(*l));
 }
 
 // ---- P/W line step text ------------------------------------------------------

@@ -228,6 +228,71 @@ int gfz_extract_noalloc(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids
                         uint64_t n_ids, int32_t *out, uint64_t out_cap, uint64_t *offsets,
                         uint64_t *n_out);
 
+/* ---- extraction front-end: a resident catalog and P/W lines ------------------
+ * The reference's extract_path_lines_by_name, extract_walk_lines and
+ * extract_walk_lines_by_name (src/workflows/extraction_workflow.cpp:206-424)
+ * zstd-decode the name columns, build a name map (or scan the walk tuples
+ * linearly) and zstd-decode the whole paths or walks column on every call. Here
+ * a gfz_catalog indexes the names once (hash maps; appends extend them) and the
+ * traversals stay decodable in a gfz_resident, so a request costs hash probes,
+ * one resident decode of just the requested traversals and GPU formatting of
+ * their lines (gfz_format_steps). Lookup semantics and messages are the
+ * reference's: any repeated stored path name (walk sample ID) fails every lookup
+ * by name with "path name is ambiguous: <first repeat>" ("walk name is
+ * ambiguous: ..."); a missing one fails with "path not found: <name>" ("walk not
+ * found: ..."); a walk tuple takes the first stored walk equal in all five
+ * fields ("walk not found for the provided identifier: s\th\tseq\tstart\tend").
+ * Lines: "P\t<name>\t<steps>\t<overlap or *>\n" and "W\t<sample>\t<hap>\t<seq>\t
+ * <start or *>\t<end or *>\t<steps>\n" (format_path_line / format_walk_line,
+ * :150-193), the requested fields, or for a walk by name the stored ones.
+ * Catalog lookups and appends are thread-safe (lookups shared, appends
+ * exclusive). Strings cross the ABI as concatenated bytes with n + 1 offsets. */
+typedef struct gfz_strings {
+  const char *bytes;
+  const uint64_t *off; /* n + 1 */
+  uint64_t n;
+} gfz_strings;
+
+typedef struct gfz_walk_keys {  /* WalkLookupKey (extraction_workflow.hpp) per walk */
+  gfz_strings sample_ids;       /* n walks */
+  const uint32_t *hap_indices;
+  gfz_strings seq_ids;
+  const int64_t *seq_starts;    /* -1 = "*" */
+  const int64_t *seq_ends;
+} gfz_walk_keys;
+
+typedef struct gfz_catalog gfz_catalog;
+int gfz_catalog_create(gfz_catalog **out);
+void gfz_catalog_free(gfz_catalog *c);
+/* Paths (walks) get traversal indices first_trav_id, first_trav_id + 1, ... of
+ * the resident store: 0 for the paths of a loaded .gfaz, n_paths for its walks,
+ * gfz_resident_append's first_new_id for appended ones. overlaps: NULL or one
+ * per path (empty = "*"). */
+int gfz_catalog_add_paths(gfz_catalog *c, const gfz_strings *names, const gfz_strings *overlaps,
+                          uint64_t first_trav_id);
+int gfz_catalog_add_walks(gfz_catalog *c, const gfz_walk_keys *walks, uint64_t first_trav_id);
+int gfz_catalog_info(const gfz_catalog *c, uint64_t *n_paths, uint64_t *n_walks);
+/* Resolve requests to traversal indices (trav_ids: one per request). */
+int gfz_catalog_find_paths(const gfz_catalog *c, const gfz_strings *names, uint64_t *trav_ids);
+int gfz_catalog_find_walks(const gfz_catalog *c, const gfz_walk_keys *keys, uint64_t *trav_ids);
+int gfz_catalog_find_walks_by_name(const gfz_catalog *c, const gfz_strings *names,
+                                   uint64_t *trav_ids);
+
+/* Text of the requested lines, back to back (library-owned; gfz_lines_free). */
+typedef struct gfz_lines {
+  uint8_t *text;
+  uint64_t n_bytes;
+  uint64_t *line_off;  /* n_lines + 1 */
+  uint64_t n_lines;
+} gfz_lines;
+int gfz_extract_path_lines(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                           const gfz_strings *names, gfz_lines *out);
+int gfz_extract_walk_lines(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                           const gfz_walk_keys *keys, gfz_lines *out);
+int gfz_extract_walk_lines_by_name(gfz_ctx *ctx, const gfz_resident *r, const gfz_catalog *c,
+                                   const gfz_strings *names, gfz_lines *out);
+void gfz_lines_free(gfz_lines *l);
+
 /* ---- streaming decompress ---------------------------------------------------
  * Replaces decompress_gpu_traversal_rolling_direct_writer / the rolling stream
  * (src/gpu/decompression/traversal_decode_gpu.cu:122-156,
