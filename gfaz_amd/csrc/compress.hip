@@ -416,6 +416,86 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
     atomicMax(max_abs, m);
 }
 
+// ---- wide symbols -----------------------------------------------------------------
+// Compact pair keys need every |symbol| below 2^30. Node symbols (IDs or deltas)
+// can reach 2^31, and rule IDs start above the largest. The grammar reads a
+// symbol only through equality, sign and the signed order of packed pairs, and
+// all three survive a renumbering that keeps magnitudes below kWideFloor, maps
+// the distinct magnitudes from kWideFloor up to kWideFloor + their rank, and
+// commutes with negation. So a stream whose rule-ID floor is kWideTrigger or more
+// is renumbered that way, compressed with the floor kWideFloor + (distinct large
+// magnitudes), and mapped back: node magnitudes through the table, rule IDs by
+// the shift between the two floors.
+constexpr uint32_t kWideFloor = 1u << 28;
+constexpr uint32_t kWideTrigger = 1u << 29;
+constexpr uint64_t kWideCap = uint64_t(1) << 27;  // steps with a large magnitude
+
+__global__ __launch_bounds__(kBlock) void k_wide_gather(const int32_t *__restrict__ x,
+                                                        uint64_t n, uint32_t *__restrict__ vals,
+                                                        uint64_t cap,
+                                                        unsigned long long *__restrict__ count) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < n;
+       base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const uint32_t a = i < n ? uabs(x[i]) : 0u;
+    const bool big = a >= kWideFloor;
+    const uint64_t mask = __ballot(big);
+    if (!mask)
+      continue;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+    unsigned long long at = 0;
+    if (lane == leader)
+      at = atomicAdd(count, (unsigned long long)__popcll(mask));
+    at = __shfl(at, (int)leader);
+    if (big) {
+      const uint64_t k = at + __popcll(mask & ((1ull << lane) - 1));
+      if (k < cap)
+        vals[k] = a;
+    }
+  }
+}
+
+// Renumbers (in -> out) with the sorted distinct large magnitudes u[0..n_u).
+__global__ __launch_bounds__(kBlock) void k_wide_map(const int32_t *__restrict__ in,
+                                                     int32_t *__restrict__ out, uint64_t n,
+                                                     const uint32_t *__restrict__ u, uint32_t n_u) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    int32_t v = in[i];
+    const uint32_t a = uabs(v);
+    if (a >= kWideFloor) {
+      uint32_t lo = 0, hi = n_u;  // lower_bound; a is in the table
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (u[mid] < a)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const uint32_t r = kWideFloor + lo;
+      v = v < 0 ? -(int32_t)r : (int32_t)r;
+    }
+    out[i] = v;
+  }
+}
+
+// Maps renumbered symbols back in place: node magnitudes through the table, rule
+// IDs (from next_wide up) to next_orig + their offset.
+__global__ __launch_bounds__(kBlock) void k_wide_unmap(int32_t *__restrict__ x, uint64_t n,
+                                                       const uint32_t *__restrict__ u,
+                                                       uint32_t next_wide, uint32_t next_orig) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const int32_t v = x[i];
+    const uint32_t a = uabs(v);
+    if (a >= kWideFloor) {
+      const uint32_t o = a < next_wide ? u[a - kWideFloor] : a - next_wide + next_orig;
+      x[i] = v < 0 ? (int32_t)(0u - o) : (int32_t)o;
+    }
+  }
+}
+
 // ---- K1: repeated-pair set ----------------------------------------------------
 // Three levels of aggregation before HBM: (1) within a wave, the few most common
 // keys (the backbone's (+1,+1) dominates early rounds) are matched across all
@@ -1821,6 +1901,45 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   uint32_t next = max_abs + 1;
   out.start_id = next;
 
+  // Wide symbols: renumber large magnitudes (see k_wide_map).
+  const uint32_t next_orig = next;
+  uint32_t next_wide = 0;  // nonzero: the stream is renumbered
+  DBuf<uint32_t> d_wide;
+  if (N && next >= kWideTrigger) {
+    const uint64_t cap = std::min<uint64_t>(N, kWideCap);
+    DBuf<uint32_t> vals(pool, cap);
+    DBuf<unsigned long long> cnt(pool, 1);
+    fill_u32(ctx, reinterpret_cast<uint32_t *>(cnt.get()), 0, 2);
+    hipLaunchKernelGGL(k_wide_gather, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(),
+                       cur, N, vals.get(), cap, cnt.get());
+    GFZ_CHECK_LAUNCH();
+    const uint64_t n_big = ctx.fetch({{cnt.get(), 1, 0}})[0];
+    if (n_big > cap)
+      throw Error(kErrOverflow, std::to_string(n_big) + " steps carry symbols of magnitude >= " +
+                                    std::to_string(kWideFloor) + " (at most " +
+                                    std::to_string(kWideCap) + " supported)");
+    std::vector<uint32_t> u(n_big);
+    if (n_big)
+      GFZ_HIP(hipMemcpyAsync(u.data(), vals.get(), n_big * 4, hipMemcpyDeviceToHost,
+                             ctx.stream()));
+    ctx.sync();
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    next_wide = kWideFloor + (uint32_t)u.size();
+    d_wide = DBuf<uint32_t>(pool, std::max<size_t>(u.size(), 1));
+    if (!u.empty())
+      GFZ_HIP(hipMemcpyAsync(d_wide.get(), u.data(), u.size() * 4, hipMemcpyHostToDevice,
+                             ctx.stream()));
+    int32_t *dst = (cur == xa.get()) ? xb.get() : xa.get();
+    hipLaunchKernelGGL(k_wide_map, dim3(stride_grid(ctx, N)), dim3(kBlock), 0, ctx.stream(), cur,
+                       dst, N, d_wide.get(), (uint32_t)u.size());
+    GFZ_CHECK_LAUNCH();
+    ctx.sync();  // u is released at scope end
+    cur = dst;
+    cur_buf = dst;
+    next = next_wide;
+  }
+
   uint64_t L = N;
   uint64_t rules_cap = 0;
   DBuf<int32_t> rf, rs;
@@ -2187,6 +2306,24 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     heads.swap(heads_next);
   }
   out.end_id = next;
+  if (next_wide) {
+    const uint64_t end = (uint64_t)next_orig + (next - next_wide);
+    if (end > (uint64_t)INT32_MAX + 1)
+      throw Error(kErrOverflow, "rule IDs would pass INT32_MAX (first " +
+                                    std::to_string(next_orig) + ", " +
+                                    std::to_string(next - next_wide) + " rules)");
+    if (L)
+      hipLaunchKernelGGL(k_wide_unmap, dim3(stride_grid(ctx, L)), dim3(kBlock), 0, ctx.stream(),
+                         cur_buf, L, d_wide.get(), next_wide, next_orig);
+    if (out.n_rules) {
+      hipLaunchKernelGGL(k_wide_unmap, dim3(stride_grid(ctx, out.n_rules)), dim3(kBlock), 0,
+                         ctx.stream(), rf.get(), out.n_rules, d_wide.get(), next_wide, next_orig);
+      hipLaunchKernelGGL(k_wide_unmap, dim3(stride_grid(ctx, out.n_rules)), dim3(kBlock), 0,
+                         ctx.stream(), rs.get(), out.n_rules, d_wide.get(), next_wide, next_orig);
+    }
+    GFZ_CHECK_LAUNCH();
+    out.end_id = (uint32_t)end;
+  }
   ctx.stage_scalars(d_err.get(), 1);
   ctx.sync();
   if (const uint32_t code = (uint32_t)ctx.scalars()[0])

@@ -151,6 +151,31 @@ def test_oracle_matches_bruteforce_scan_formulation(seed):
     assert dec == nodes.tolist()
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_matches_bruteforce_large_symbols(seed):
+    """Node symbols of magnitude 2^30 .. 2^31 - 1 (rule IDs start above them): the
+    oracle's packed keys, signed order and dense IDs agree with the scan formulation.
+    Deltas are kept inside int32 so that the exact arithmetic here needs no wrap."""
+    rng = random.Random(1000 + seed)
+    top = (1 << 31) - 5000
+    big = [top - rng.randint(0, 3000) for _ in range(3)] + [(1 << 30) + rng.randint(0, 9)]
+    seqs = []
+    delta = seed % 2
+    for _ in range(rng.randint(1, 4)):
+        n = rng.randint(2, 16)
+        if delta:
+            seqs.append([rng.choice(big[-1:] + [1, 2]) for _ in range(n)])
+        else:
+            seqs.append([rng.choice([1, -1]) * rng.choice(big + [1, 2]) for _ in range(n)])
+    bs, brules, bstart, bend = brute_compress(seqs, 8, delta)
+    nodes = np.array([v for s in seqs for v in s], dtype=np.int32)
+    lengths = np.array([len(s) for s in seqs], dtype=np.uint32)
+    r = oracle.compress(nodes, lengths, 8, 2, delta, threads=2)
+    assert (r["start_id"], r["end_id"]) == (bstart, bend)
+    assert r["encoded"].tolist() == [v for s in bs for v in s]
+    assert list(zip(r["rules_first"].tolist(), r["rules_second"].tolist())) == brules
+
+
 def test_bruteforce_exhaustive_tiny():
     """All length-<=6 single traversals over {+-1, +-2}: oracle == scan formulation."""
     syms = [1, -1, 2, -2]
