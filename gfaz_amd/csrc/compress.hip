@@ -1390,8 +1390,15 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
     e.pad = 0;
     hot[s] = e;
   }
-  if (threadIdx.x < kHotTop)
-    top[threadIdx.x] = stop[threadIdx.x];
+  // Unset entries repeat the first set one, so a reader compares every entry
+  // without testing `set` (entry 0 is set iff any is).
+  if (threadIdx.x < kHotTop) {
+    HotPair e = stop[threadIdx.x];
+    for (uint32_t t = 0; t < kHotTop && !e.set; ++t)
+      if (stop[t].set)
+        e = stop[t];
+    top[threadIdx.x] = e;
+  }
 }
 
 struct HotLds {
@@ -1488,6 +1495,79 @@ __device__ inline void resolve_list(const RuleTable &rt, const HotLds &sh, uint6
   }
 }
 
+// The hot canonical pairs as packed 64-bit words (x in the low half) in both
+// orientations: a raw pair is tested with two 64-bit compares per entry.
+struct HotKeys {
+  uint64_t fwd[kHotTop], bwd[kHotTop];
+  uint32_t slot[kHotTop];
+  bool any;
+};
+
+__device__ inline uint64_t pack_raw(int32_t x, int32_t y) {
+  return (uint64_t)(uint32_t)x | ((uint64_t)(uint32_t)y << 32);
+}
+
+__device__ inline void hot_keys_load(HotKeys &h, const HotPair *__restrict__ top) {
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t) {
+    const HotPair e = top[t];
+    h.fwd[t] = pack_raw(e.a, e.b);
+    h.bwd[t] = pack_raw((int32_t)(0u - (uint32_t)e.b), (int32_t)(0u - (uint32_t)e.a));
+    h.slot[t] = e.slot;
+  }
+  h.any = top[0].set != 0;
+}
+
+// hot_pair on packed words: true when (x, y) is a hot pair, with its slot and
+// whether it is the reverse of the canonical key (a pair equal to its own
+// reverse is not).
+__device__ inline bool hot_match(const HotKeys &h, int32_t x, int32_t y, bool &rev,
+                                 uint32_t &slot) {
+  const uint64_t p = pack_raw(x, y);
+  bool f = false, r = false;
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t) {
+    const bool ft = p == h.fwd[t], rt = p == h.bwd[t];
+    s = (ft || rt) ? h.slot[t] : s;
+    f |= ft;
+    r |= rt;
+  }
+  rev = r && !f;
+  slot = s;
+  return h.any && (f || r);
+}
+
+// Pairs of one pass of a tile (item j of this lane at tile offset
+// first + j * stride) and the valid set. Interior tiles (every pair's second
+// element in the stream) are addressed by 32-bit offsets from the tile's base
+// with no clamping; the last tile takes load_pairs.
+template <uint32_t N, bool kInterior>
+__device__ inline uint32_t load_tile_pairs(const int32_t *__restrict__ x,
+                                           const uint32_t *__restrict__ heads, uint64_t L,
+                                           uint64_t base, uint32_t first, uint32_t stride,
+                                           int32_t (&a)[N], int32_t (&b)[N]) {
+  if constexpr (!kInterior) {
+    return load_pairs<N>(x, heads, L, base + first, stride, a, b);
+  } else {
+    const int32_t *__restrict__ xt = x + base;
+    const uint32_t *__restrict__ ht = heads + (base >> 5);  // base is a multiple of 32
+    uint32_t hw[N];
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j) {
+      const uint32_t r = first + j * stride;
+      a[j] = xt[r];
+      b[j] = xt[r + 1];
+      hw[j] = ht[(r + 1) >> 5];
+    }
+    uint32_t valid = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < N; ++j)
+      valid |= ((hw[j] >> ((first + j * stride + 1) & 31)) & 1) ? 0u : 1u << j;
+    return valid;
+  }
+}
+
 __device__ inline void hot_top_load(HotPair (&hp)[kHotTop], const HotPair *__restrict__ top) {
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t)
@@ -1497,29 +1577,22 @@ __device__ inline void hot_top_load(HotPair (&hp)[kHotTop], const HotPair *__res
 // Candidate bits, each tile's pick function, and every position's slot word
 // (the rule slot of the pair starting there, kSlotRev when the pair is the
 // reverse of the rule's canonical key, kNoSlot when it is no rule).
-__global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
-                                                 const uint32_t *__restrict__ heads, uint64_t L,
-                                                 uint64_t span, RuleTable rt,
-                                                 const HotEntry *__restrict__ hot,
-                                                 const HotPair *__restrict__ top,
-                                                 uint64_t *__restrict__ cand,
-                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn,
-                                                 uint32_t *__restrict__ slots) {
-  __shared__ uint64_t wbits[kRtWords];
-  __shared__ HotLds sh;
-  __shared__ unsigned long long list[kBlock / kWave][kColdCap];
+template <bool kInterior>
+__device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
+                                 uint64_t L, uint64_t span, const RuleTable &rt,
+                                 const HotKeys &hk, HotLds &sh,
+                                 unsigned long long (*list)[kColdCap], uint64_t *wbits,
+                                 uint64_t *__restrict__ cand, uint64_t n_words,
+                                 uint32_t *__restrict__ slots) {
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
+  uint32_t *__restrict__ st = slots + base;  // this tile's slot words
   const uint64_t below = lanemask_lt();
-  HotPair hp[kHotTop];
-  hot_top_load(hp, top);
-  hot_load(sh, hot);
-  __syncthreads();
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
     // Validity is a per-lane bit set (bool arrays live as scalar lane masks).
     int32_t xa[kPassItems], xb[kPassItems];
-    const uint32_t valid = load_pairs<kPassItems>(
-        x, heads, L, base + (uint64_t)j0 * kBlock + threadIdx.x, kBlock, xa, xb);
+    const uint32_t valid = load_tile_pairs<kPassItems, kInterior>(
+        x, heads, L, base, j0 * kBlock + threadIdx.x, kBlock, xa, xb);
     // Listed items keep their list position per lane (no ballot masks held
     // across the lookups: they would spill scalar registers).
     // Hot and invalid positions store their slot word now; listed ones after
@@ -1528,11 +1601,11 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
     uint32_t at[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
+      const uint32_t r = (j0 + b) * kBlock + threadIdx.x;
       bool rev = false;
       uint32_t hs = 0;
       const bool v = (valid >> b) & 1;
-      const bool h = v && hot_pair(hp, xa[b], xb[b], rev, hs) >= 0;
+      const bool h = v && hot_match(hk, xa[b], xb[b], rev, hs);
       const bool c = v && !h;
       hit |= h ? 1u << b : 0u;
       const uint64_t m = __ballot(c);
@@ -1542,8 +1615,8 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
         const int64_t raw = pack2(xa[b], xb[b]);
         list[wid][at[b]] = (unsigned long long)raw;
         neg |= raw != canon2(raw) ? 1u << b : 0u;
-      } else if (i < L) {
-        slots[i] = h ? (hs | (rev ? kSlotRev : 0u)) : kNoSlot;
+      } else if (kInterior || base + r < L) {
+        st[r] = h ? (hs | (rev ? kSlotRev : 0u)) : kNoSlot;
       }
       n += (uint32_t)__popcll(m);
     }
@@ -1556,7 +1629,7 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
       if ((listed >> b) & 1) {
         const unsigned long long sl = list[wid][at[b]];
         f = sl != kFree;
-        slots[base + (uint64_t)(j0 + b) * kBlock + threadIdx.x] =
+        st[(j0 + b) * kBlock + threadIdx.x] =
             f ? ((uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u)) : kNoSlot;
       }
       const uint64_t word = __ballot(f);
@@ -1570,6 +1643,29 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
     }
     __syncthreads();  // the list is refilled by the next pass
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
+                                                 const uint32_t *__restrict__ heads, uint64_t L,
+                                                 uint64_t span, RuleTable rt,
+                                                 const HotEntry *__restrict__ hot,
+                                                 const HotPair *__restrict__ top,
+                                                 uint64_t *__restrict__ cand,
+                                                 uint64_t n_words, uint32_t *__restrict__ tile_fn,
+                                                 uint32_t *__restrict__ slots) {
+  __shared__ uint64_t wbits[kRtWords];
+  __shared__ HotLds sh;
+  __shared__ unsigned long long list[kBlock / kWave][kColdCap];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  HotKeys hk;
+  hot_keys_load(hk, top);
+  hot_load(sh, hot);
+  __syncthreads();
+  // Every tile but the last is interior (its last pair ends inside the stream).
+  if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
+    cand_tile<true>(x, heads, L, span, rt, hk, sh, list, wbits, cand, n_words, slots);
+  else
+    cand_tile<false>(x, heads, L, span, rt, hk, sh, list, wbits, cand, n_words, slots);
   if (wid == 0) {
     const uint32_t f = wave_fn_inclusive(word_fn(wbits[lane]));
     if (lane == kWave - 1)
@@ -1974,11 +2070,14 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // and repeated packed in [0], the longest bucket list in [8]
     DBuf<unsigned long long> sums(pool, 9);
     uint64_t distinct = 0, repeated = 0;
-    // Once most pairs of the last round were distinct, count in hash buckets
-    // (distinct pairs only grow from round to round; earlier rounds hold keys
-    // hot enough to fill a bucket); the table counts the round if a bucket
-    // overflows anyway.
-    const bool mostly_distinct = round > 0 && 2 * prev_distinct > prev_len;
+    // Tile aggregation pays while keys recur within a tile. Once half of the last
+    // round's pairs were distinct, tiles hold few repeats (a key's copies lie in
+    // different traversals): the partition then reads the pairs straight from the
+    // stream. "count_direct_share" (tuned or fixed) replaces the 2 (on plain_10m
+    // and plain_500k, 2, 4 and 8 measured the same within 1 %).
+    const int tuned_share = ctx.tuned("count_direct_share");
+    const uint64_t direct_share = tuned_share > 0 ? (uint64_t)tuned_share : 2;
+    const bool mostly_distinct = round > 0 && direct_share * prev_distinct > prev_len;
     DBuf<unsigned long long> tiles;  // aggregated tiles (rounds with hot keys)
     DBuf<uint32_t> tiles_n;
     // Tiles of few distinct keys (under a quarter of the last round's pairs were
