@@ -148,7 +148,26 @@ struct RuleTable {
   uint32_t *vals;
   uint32_t rbits;            // log2 of the lines per region
   uint32_t nbmask;           // regions - 1 (a power of two minus one)
+  // Rounds where most pairs are no rule: one bit per hashed key of the table
+  // (2^fbits bits, set as keys go in). A clear bit proves a key absent without
+  // reading its line; nullptr in rounds where most pairs are rules.
+  uint32_t *filter;
+  uint32_t fbits;
 };
+
+__device__ inline uint32_t filter_bit(const RuleTable &rt, uint64_t k) {
+  return (uint32_t)(key_hash(k) >> 13) & ((1u << rt.fbits) - 1);
+}
+__device__ inline void filter_add(const RuleTable &rt, uint64_t k) {
+  if (rt.filter) {
+    const uint32_t b = filter_bit(rt, k);
+    atomicOr(&rt.filter[b >> 5], 1u << (b & 31));
+  }
+}
+__device__ inline bool filter_has(const RuleTable &rt, uint64_t k) {
+  const uint32_t b = filter_bit(rt, k);
+  return (rt.filter[b >> 5] >> (b & 31)) & 1u;
+}
 
 // Bucket of a key among mask + 1 (hash bits above those that pick its line).
 __device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
@@ -1201,9 +1220,12 @@ __global__ __launch_bounds__(kBlock) void k_extract_staged(
     const uint2 *__restrict__ block_counts, RuleTable rt, uint32_t *__restrict__ err) {
   const uint64_t b = blockIdx.x;
   const uint64_t n = block_counts[b].y < stride ? block_counts[b].y : stride;
-  for (uint64_t j = threadIdx.x; j < n; j += kBlock)
-    if (!rule_insert(rt, stage[b * stride + j]))
+  for (uint64_t j = threadIdx.x; j < n; j += kBlock) {
+    const unsigned long long k = stage[b * stride + j];
+    if (!rule_insert(rt, k))
       atomicExch(err, 3u);
+    filter_add(rt, k);
+  }
 }
 
 // From the counting table itself (when some block staged too many keys).
@@ -1236,6 +1258,7 @@ __global__ __launch_bounds__(kBktThreads) void k_region_build(
     }
     if (!placed)  // a region smaller than its list: never sized so by the host
       atomicExch(err, 3u);
+    filter_add(rt, k);
   }
   __syncthreads();
   unsigned long long *dst = rt.keys + (b << rt.rbits) * kLine;
@@ -1249,8 +1272,11 @@ __global__ __launch_bounds__(kBlock) void k_extract_repeated(
   const uint64_t s0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kExtractItems;
   for (uint32_t i = 0; i < kExtractItems && s0 + i < cap; ++i) {
     const uint64_t v = count_slots[s0 + i];
-    if (v != kFree && (v & kRep) && !rule_insert(rt, v & ~kRep))
-      atomicExch(err, 3u);
+    if (v != kFree && (v & kRep)) {
+      if (!rule_insert(rt, v & ~kRep))
+        atomicExch(err, 3u);
+      filter_add(rt, v & ~kRep);
+    }
   }
 }
 
@@ -1480,7 +1506,7 @@ __device__ inline void resolve_list(const RuleTable &rt, const HotLds &sh, uint6
         kk[q] = pair_key(key_first(raw), key_second(raw), span);
         hs[q] = hot_find(sh, kk[q]);
       }
-      cold[q] = e < n && hs[q] < 0;
+      cold[q] = e < n && hs[q] < 0 && (!rt.filter || filter_has(rt, kk[q]));
     }
     int64_t sl[kColdBatch];
     rule_find_batch<kColdBatch>(rt, kk, cold, sl);
@@ -2270,23 +2296,38 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
     const uint64_t region_lines = regions ? region_slots / kLine : rcap / kLine;
+    // The absence filter pays once a quarter of the pairs are no rule (each of
+    // the distinct - repeated keys is one such pair): about 16 bits per repeated
+    // key, at most 2^27 bits (16 MB, cache-resident).
+    const bool use_filter = 4 * (distinct - std::min(distinct, repeated)) > L;
+    const uint32_t fbits =
+        use_filter ? (uint32_t)std::min<uint64_t>(
+                         27, std::max<uint64_t>(12, 64 - __builtin_clzll(
+                                                             std::max<uint64_t>(repeated, 1)) + 4))
+                   : 0u;
+    DBuf<uint32_t> filter;
+    if (use_filter)
+      filter = DBuf<uint32_t>(pool, (uint64_t(1) << fbits) / 32);
     const RuleTable rt{rkeys.get(), rvals.get(), (uint32_t)__builtin_ctzll(region_lines),
-                       regions ? (uint32_t)(regions - 1) : 0u};
+                       regions ? (uint32_t)(regions - 1) : 0u, filter.get(), fbits};
     // Round scalars (read back once, after K4): [0] next length, [1] used keys.
     // One launch clears everything the round needs cleared: the rule table, the
     // scalars, the used flags and the next round's head bits (up to L: the next
     // length is at most L).
     DBuf<unsigned long long> scal(pool, 2);
     DBuf<uint8_t> used(pool, rcap);
+    const uint64_t fwords = use_filter ? (uint64_t(1) << fbits) / 32 : 0;
     if (regions)  // the regions are written whole: the table needs no clearing
       fill_runs(ctx, {{scal.get(), 2, 0, 8},
                       {used.get(), rcap, 0, 1},
-                      {heads_next.get(), ceil_div(L, 32), 0, 4}});
+                      {heads_next.get(), ceil_div(L, 32), 0, 4},
+                      {filter.get(), fwords, 0, 4}});
     else
       fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
                       {scal.get(), 2, 0, 8},
                       {used.get(), rcap, 0, 1},
-                      {heads_next.get(), ceil_div(L, 32), 0, 4}});
+                      {heads_next.get(), ceil_div(L, 32), 0, 4},
+                      {filter.get(), fwords, 0, 4}});
     ctx.timed("extract_repeated", [&] {
       if (regions)
         hipLaunchKernelGGL(k_region_build, dim3((uint32_t)regions), dim3(kBktThreads), 0,
