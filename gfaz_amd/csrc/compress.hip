@@ -48,11 +48,10 @@ __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
 // all-ones free as the empty slot.
 constexpr uint64_t kFree = ~0ull;
 constexpr uint64_t kRep = 1ull << 63;
-// Slot words of the candidate pass: a rule slot (below 2^30: the rule table has
-// a power-of-two capacity with next + capacity < 2^31), bit 31 for the reverse
-// orientation, all ones for no rule.
+// Slot words of the candidate pass (cold candidates only): a rule slot (below
+// 2^30: the rule table has a power-of-two capacity with next + capacity < 2^31),
+// bit 31 for the reverse orientation.
 constexpr uint32_t kSlotRev = 1u << 31;
-constexpr uint32_t kNoSlot = ~0u;
 
 __device__ inline uint64_t compact_key(int64_t k, uint64_t span) {
   const int64_t a = key_first(k), b = key_second(k);
@@ -1467,26 +1466,6 @@ constexpr uint32_t kPassItems = 8;
 constexpr uint32_t kColdCap = kPassItems * kWave;   // every item of a pass listed
 constexpr int kColdBatch = 2;
 
-// Raw pair (x, y) against the hot canonical pairs: the matching entry's index
-// and slot, or -1; rev = the pair appears as (-b, -a) of its canonical (a, b).
-__device__ inline int hot_pair(const HotPair (&hp)[kHotTop], int32_t x, int32_t y, bool &rev,
-                               uint32_t &slot) {
-  int hit = -1;
-  rev = false;
-  slot = 0;
-#pragma unroll
-  for (uint32_t t = 0; t < kHotTop; ++t) {
-    const bool fwd = x == hp[t].a && y == hp[t].b;
-    const bool bwd = x == -hp[t].b && y == -hp[t].a;
-    if (hp[t].set && (fwd || bwd)) {
-      hit = (int)t;
-      slot = hp[t].slot;
-      rev = !fwd;
-    }
-  }
-  return hit;
-}
-
 // Resolves the wave's n listed raw pairs: each entry becomes its key's rule
 // slot, or kFree when the key is not in the table.
 __device__ inline void resolve_list(const RuleTable &rt, const HotLds &sh, uint64_t span,
@@ -1544,9 +1523,8 @@ __device__ inline void hot_keys_load(HotKeys &h, const HotPair *__restrict__ top
   h.any = top[0].set != 0;
 }
 
-// hot_pair on packed words: true when (x, y) is a hot pair, with its slot and
-// whether it is the reverse of the canonical key (a pair equal to its own
-// reverse is not).
+// True when the raw pair (x, y) is a hot pair, with its slot and whether it is
+// the reverse of the canonical key (a pair equal to its own reverse is not).
 __device__ inline bool hot_match(const HotKeys &h, int32_t x, int32_t y, bool &rev,
                                  uint32_t &slot) {
   const uint64_t p = pack_raw(x, y);
@@ -1594,15 +1572,11 @@ __device__ inline uint32_t load_tile_pairs(const int32_t *__restrict__ x,
   }
 }
 
-__device__ inline void hot_top_load(HotPair (&hp)[kHotTop], const HotPair *__restrict__ top) {
-#pragma unroll
-  for (uint32_t t = 0; t < kHotTop; ++t)
-    hp[t] = top[t];
-}
 
-// Candidate bits, each tile's pick function, and every position's slot word
-// (the rule slot of the pair starting there, kSlotRev when the pair is the
-// reverse of the rule's canonical key, kNoSlot when it is no rule).
+// Candidate bits, each tile's pick function, and the slot words of the cold
+// candidates (the rule slot of the pair starting there, kSlotRev when the pair
+// is the reverse of the rule's canonical key). Hot pairs need none: emit tests
+// its picks against the hot pairs again, from the raw pair.
 template <bool kInterior>
 __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
                                  uint64_t L, uint64_t span, const RuleTable &rt,
@@ -1621,13 +1595,12 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
         x, heads, L, base, j0 * kBlock + threadIdx.x, kBlock, xa, xb);
     // Listed items keep their list position per lane (no ballot masks held
     // across the lookups: they would spill scalar registers).
-    // Hot and invalid positions store their slot word now; listed ones after
-    // the wave's lookups.
+    // Listed (cold) pairs store their slot word after the wave's lookups, and
+    // only when they are rules.
     uint32_t hit = 0, listed = 0, neg = 0, n = 0;
     uint32_t at[kPassItems];
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint32_t r = (j0 + b) * kBlock + threadIdx.x;
       bool rev = false;
       uint32_t hs = 0;
       const bool v = (valid >> b) & 1;
@@ -1641,8 +1614,6 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
         const int64_t raw = pack2(xa[b], xb[b]);
         list[wid][at[b]] = (unsigned long long)raw;
         neg |= raw != canon2(raw) ? 1u << b : 0u;
-      } else if (kInterior || base + r < L) {
-        st[r] = h ? (hs | (rev ? kSlotRev : 0u)) : kNoSlot;
       }
       n += (uint32_t)__popcll(m);
     }
@@ -1655,8 +1626,8 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
       if ((listed >> b) & 1) {
         const unsigned long long sl = list[wid][at[b]];
         f = sl != kFree;
-        st[(j0 + b) * kBlock + threadIdx.x] =
-            f ? ((uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u)) : kNoSlot;
+        if (f)
+          st[(j0 + b) * kBlock + threadIdx.x] = (uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u);
       }
       const uint64_t word = __ballot(f);
       if (lane == 0) {
@@ -1768,26 +1739,101 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
 
 // ---- K3: emit the next round's stream ---------------------------------------
 // Survivors are elements whose predecessor was not picked; a picked element
-// emits +/-(next + slot) (provisional, remapped after the sort), its slot and
-// orientation read from the candidate pass's slot words (no rule lookups here).
-// Heads always survive and carry their bit to the output position.
+// emits +/-(next + slot) (provisional, remapped after the sort). A pick's slot
+// and orientation come from the hot pairs (tested raw, as in K2) or, for a cold
+// pick, from its slot word. Heads always survive and carry their bit to the
+// output position.
+template <bool kInterior>
+__device__ inline void emit_tile(const int32_t *__restrict__ x,
+                                 const uint32_t *__restrict__ heads, uint64_t L,
+                                 const uint32_t *__restrict__ slots, uint64_t rcap,
+                                 const HotKeys &hk, const uint64_t *spicks,
+                                 const uint64_t *ssurv, const uint32_t *soff, uint64_t off_tile,
+                                 uint8_t *__restrict__ used, uint32_t next,
+                                 int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
+                                 uint32_t *__restrict__ err, uint32_t &hot_used) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t below = lanemask_lt();
+  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
+  const int32_t *__restrict__ xt = x + base;
+  const uint32_t *__restrict__ ht = heads + (base >> 5);
+  const uint32_t *__restrict__ st = slots + base;
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
+    // Every load of the pass first (interior tiles: every pair inside the
+    // stream; the last tile clamps its indexes into it).
+    int32_t xa[kPassItems], xb[kPassItems];
+    uint32_t hw[kPassItems];
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      const uint32_t r = (j0 + b) * kBlock + threadIdx.x;
+      if constexpr (kInterior) {
+        xa[b] = xt[r];
+        xb[b] = xt[r + 1];
+        hw[b] = ht[r >> 5];
+      } else {
+        const uint64_t i = base + r;
+        const uint64_t ia = i < L ? i : L - 1;
+        const uint64_t ib = ia + 1 < L ? ia + 1 : L - 1;
+        xa[b] = x[ia];
+        xb[b] = x[ib];
+        hw[b] = heads[ia >> 5];
+      }
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < kPassItems; ++b) {
+      const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+      const uint32_t r = w * 64 + lane;
+      const uint64_t svm = ssurv[w];
+      if ((kInterior || base + r < L) && ((svm >> lane) & 1)) {
+        const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(svm & below);
+        int32_t o = xa[b];
+        if ((spicks[w] >> lane) & 1) {
+          bool rev = false;
+          uint32_t sl = 0;
+          if (hot_match(hk, xa[b], xb[b], rev, sl)) {
+#pragma unroll
+            for (uint32_t q = 0; q < kHotTop; ++q)
+              if (hk.slot[q] == sl)
+                hot_used |= 1u << q;
+          } else {
+            const uint32_t sw = st[r];
+            sl = sw & ~kSlotRev;
+            rev = (sw & kSlotRev) != 0;
+            if (sl < rcap) {
+              used[sl] = 1;
+            } else {
+              atomicExch(err, 1u);  // a picked cold pair always has its slot word
+              sl = 0;
+            }
+          }
+          o = (int32_t)(next + sl);
+          if (rev)
+            o = -o;
+        }
+        y[pos] = o;
+        if ((hw[b] >> (r & 31)) & 1)
+          atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
     const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
-    const uint32_t *__restrict__ slots, const HotPair *__restrict__ top,
+    const uint32_t *__restrict__ slots, uint64_t rcap, const HotPair *__restrict__ top,
     uint8_t *__restrict__ used, uint32_t next, int32_t *__restrict__ y,
     uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const uint64_t below = lanemask_lt();
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const uint64_t info = tile_info[blockIdx.x];
   const uint32_t p_tile = (uint32_t)(info & 1);
   const uint64_t off_tile = info >> 1;
-  HotPair hp[kHotTop];
-  hot_top_load(hp, top);
+  HotKeys hk;
+  hot_keys_load(hk, top);
   if (wid == 0) {
     const uint64_t gw = (base >> 6) + lane;
     const uint64_t c = gw < n_words ? cand[gw] : 0;
@@ -1815,56 +1861,16 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   }
   __syncthreads();
   uint32_t hot_used = 0;  // hot entries this lane picked (their used flags, once per wave)
-  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
-    // Every load of the pass first, from indexes clamped into the stream.
-    int32_t xa[kPassItems];
-    uint32_t sv[kPassItems], hw[kPassItems];
-#pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint64_t i = base + (uint64_t)(j0 + b) * kBlock + threadIdx.x;
-      const uint64_t ia = i < L ? i : L - 1;
-      xa[b] = x[ia];
-      sv[b] = slots[ia];
-      hw[b] = heads[ia >> 5];
-    }
-#pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
-      const uint64_t i = base + (uint64_t)w * 64 + lane;
-      const uint64_t svm = ssurv[w];
-      if (i < L && ((svm >> lane) & 1)) {
-        const uint64_t pos = off_tile + soff[w] + (uint64_t)__popcll(svm & below);
-        int32_t o = xa[b];
-        if ((spicks[w] >> lane) & 1) {
-          const uint32_t sw = sv[b];
-          if (sw == kNoSlot) {
-            atomicExch(err, 1u);  // a picked pair is always a rule
-          } else {
-            const uint32_t sl = sw & ~kSlotRev;
-            int t = -1;
-#pragma unroll
-            for (uint32_t q = 0; q < kHotTop; ++q)
-              if (hp[q].set && hp[q].slot == sl)
-                t = (int)q;
-            if (t >= 0)
-              hot_used |= 1u << t;
-            else
-              used[sl] = 1;
-            o = (int32_t)(next + sl);
-            if (sw & kSlotRev)
-              o = -o;
-          }
-        }
-        y[pos] = o;
-        if ((hw[b] >> (i & 31)) & 1)
-          atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
-      }
-    }
-  }
+  if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
+    emit_tile<true>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, off_tile, used, next, y,
+                    heads_out, err, hot_used);
+  else
+    emit_tile<false>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, off_tile, used, next, y,
+                     heads_out, err, hot_used);
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t)
     if (__ballot((hot_used >> t) & 1) && lane == 0)
-      used[hp[t].slot] = 1;
+      used[hk.slot[t]] = 1;
 }
 
 // ---- K4: keys of the used rules (order irrelevant: they are sorted next) -----
@@ -2377,7 +2383,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, cand.get(), n_words, tile_info.get(), slots.get(),
-                         top.get(), used.get(), next, dst, heads_next.get(), d_err.get());
+                         rcap, top.get(), used.get(), next, dst, heads_next.get(), d_err.get());
     });
     GFZ_CHECK_LAUNCH();
 
