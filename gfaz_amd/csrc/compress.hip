@@ -1937,16 +1937,50 @@ __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ 
 }
 
 // ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
+// Provisional IDs (next + slot) to final ones (vals[slot]), in place, four
+// symbols per 16-byte access. The round's hot rules (most picks of the early
+// rounds) take their final IDs from registers instead of a gather each.
 __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
-                                                  const uint32_t *__restrict__ vals, uint32_t next) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock) {
-    const int32_t v = y[i];
+                                                  const uint32_t *__restrict__ vals, uint32_t next,
+                                                  const HotPair *__restrict__ top) {
+  uint32_t hs[kHotTop], hf[kHotTop];
+#pragma unroll
+  for (uint32_t t = 0; t < kHotTop; ++t) {
+    hs[t] = top[t].slot;
+    hf[t] = vals[hs[t]];  // a slot of the table (unset entries copy a set one or hold 0)
+  }
+  const bool any = top[0].set != 0;
+  auto map = [&](int32_t v) -> int32_t {
     const uint32_t a = uabs(v);
-    if (a >= next) {
-      const int32_t f = (int32_t)vals[a - next];
-      y[i] = v > 0 ? f : -f;
+    if (a < next)
+      return v;
+    const uint32_t sl = a - next;
+    bool hit = false;
+    uint32_t f = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < kHotTop; ++t) {
+      const bool m = any && sl == hs[t];
+      f = m ? hf[t] : f;
+      hit |= m;
     }
+    if (!hit)
+      f = vals[sl];
+    return v > 0 ? (int32_t)f : -(int32_t)f;
+  };
+  const uint64_t n4 = n / 4;
+  int4 *__restrict__ y4 = reinterpret_cast<int4 *>(y);  // y: a pool buffer, 256-byte aligned
+  for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < n4;
+       c += (uint64_t)gridDim.x * kBlock) {
+    int4 v = y4[c];
+    v.x = map(v.x);
+    v.y = map(v.y);
+    v.z = map(v.z);
+    v.w = map(v.w);
+    y4[c] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - n4 * 4) {
+    const uint64_t i = n4 * 4 + threadIdx.x;
+    y[i] = map(y[i]);
   }
 }
 
@@ -2435,8 +2469,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
     // K7: remap provisional IDs (errors of K6 surface at the next readback).
     ctx.timed("remap", [&] {
-      hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, L_next)), dim3(kBlock), 0, ctx.stream(),
-                         dst, L_next, rvals.get(), next);
+      hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, ceil_div(L_next, 4))), dim3(kBlock), 0,
+                         ctx.stream(), dst, L_next, rvals.get(), next, top.get());
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);
