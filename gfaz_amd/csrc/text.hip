@@ -344,37 +344,87 @@ __device__ inline uint64_t stage_window(const uint8_t *txt, uint64_t n_bytes, ui
   return wn;
 }
 
-// Kept steps per tile of text bytes.
+// Kept steps per tile of text bytes, and for every field that starts inside the
+// tile the kept steps of the tile before its first byte (partial[l]).
 __global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
                                                         const uint64_t *__restrict__ fb,
                                                         const uint64_t *__restrict__ fe,
                                                         uint64_t n_lines,
                                                         uint32_t *__restrict__ tile_count,
+                                                        uint32_t *__restrict__ partial,
                                                         uint32_t *__restrict__ err) {
   __shared__ __align__(16) uint8_t win[kParseWin];
   __shared__ uint32_t lds[kBlock / kWave + 1];
+  __shared__ uint32_t keep_bits[kParseTile / 32];  // kept-token starts, one bit per byte
+  __shared__ uint64_t lrange[2];                   // fields starting in the tile
   const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
-  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};
+  for (uint32_t i = threadIdx.x; i < kParseTile / 32; i += kBlock)
+    keep_bits[i] = 0;
+  if (threadIdx.x < 2) {
+    // first field starting at or after w0 (thread 0) or w0 + kParseTile (thread 1)
+    const uint64_t at = w0 + (uint64_t)threadIdx.x * kParseTile;
+    uint64_t lo = 0, hi = n_lines;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (fb[mid] < at)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    lrange[threadIdx.x] = lo;
+  }
+  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};  // syncs
   // Thread t takes bytes t, t + kBlock, ... of the tile: a wave reads 64
   // consecutive LDS bytes per step.
   uint32_t c = 0;
   FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
   fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0);
   for (uint32_t j = 0; j < kParseItems; ++j) {
-    const uint64_t p = w0 + (uint64_t)j * kBlock + threadIdx.x;
+    const uint32_t r = j * kBlock + threadIdx.x;
+    const uint64_t p = w0 + r;
     if (p >= n_bytes)
       break;
     fc.advance(p);
     if (p < fc.b || p >= fc.e)
       continue;
     int32_t id;
-    c += parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1 ? 1u : 0u;
+    if (parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1) {
+      ++c;
+      atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
+    }
   }
   uint32_t total;
-  block_exclusive_scan<kBlock>(c, lds, total);
+  block_exclusive_scan<kBlock>(c, lds, total);  // its barriers publish keep_bits
   if (threadIdx.x == 0)
     tile_count[blockIdx.x] = total;
+  for (uint64_t l = lrange[0] + threadIdx.x; l < lrange[1]; l += kBlock) {
+    const uint32_t off = (uint32_t)(fb[l] - w0);  // < kParseTile
+    uint32_t n = 0;
+    for (uint32_t w = 0; w < off / 32; ++w)
+      n += (uint32_t)__popc(keep_bits[w]);
+    if (off % 32)
+      n += (uint32_t)__popc(keep_bits[off / 32] & ((1u << (off % 32)) - 1));
+    partial[l] = n;
+  }
+}
+
+// Kept steps before each field start (lpos[n_lines]: all of them): the tile's
+// prefix plus the field's partial count; a field starting at the text's end past
+// every tile has them all.
+__global__ void k_parse_line_pos(const uint64_t *__restrict__ fb, uint64_t n_lines,
+                                 uint64_t n_tiles, const uint64_t *__restrict__ tile_pos,
+                                 const uint32_t *__restrict__ partial,
+                                 uint64_t *__restrict__ lpos) {
+  const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > n_lines)
+    return;
+  if (l == n_lines) {
+    lpos[l] = tile_pos[n_tiles];
+    return;
+  }
+  const uint64_t t = fb[l] / kParseTile;
+  lpos[l] = t < n_tiles ? tile_pos[t] + partial[l] : tile_pos[n_tiles];
 }
 
 // Steps in byte order: each thread's kept tokens are ranked inside the tile by a
@@ -424,34 +474,6 @@ __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t 
   for (uint32_t j = 0; j < kParseItems; ++j)
     if ((bits >> j) & 1)
       nodes[o++] = ids[r0 + j];
-}
-
-// Kept steps before each field start (lpos[n_lines]: before byte n_bytes): one
-// thread per field, counting from its tile start.
-__global__ void k_parse_line_pos(int kind, const uint8_t *__restrict__ txt, uint64_t n_bytes,
-                                 const uint64_t *__restrict__ fb, const uint64_t *__restrict__ fe,
-                                 uint64_t n_lines, const uint64_t *__restrict__ tile_pos,
-                                 uint64_t *__restrict__ lpos, uint32_t *__restrict__ err) {
-  const uint64_t li = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (li > n_lines)
-    return;
-  const uint64_t p = li < n_lines ? fb[li] : n_bytes;
-  const uint64_t tile = p / kParseTile;
-  uint64_t c = tile_pos[tile];
-  uint64_t q = tile * kParseTile;
-  if (q < p) {
-    const TextReader rd{txt, nullptr, 0, 0};
-    FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
-    fc.seek(q);
-    for (; q < p; ++q) {
-      fc.advance(q);
-      if (q < fc.b || q >= fc.e)
-        continue;
-      int32_t id;
-      c += parse_token(kind, rd, fc.b, q, fc.e, &id, err) == 1 ? 1u : 0u;
-    }
-  }
-  lpos[li] = c;
 }
 
 uint32_t grid1(uint64_t n, uint32_t per) {
@@ -589,6 +611,7 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
   plan.tile_pos = DBuf<uint64_t>(pool, plan.n_tiles + 1);
   DBuf<uint64_t> lpos(pool, n_lines + 1);
   DBuf<uint32_t> tile_cnt(pool, plan.n_tiles), err(pool, 1);
+  DBuf<uint32_t> partial(pool, std::max<uint64_t>(n_lines, 1));
   fill_u32(ctx, err.get(), 0, 1);
   if (n_lines) {
     GFZ_HIP(hipMemcpyAsync(plan.fb.get(), field_begin, n_lines * 8, hipMemcpyHostToDevice,
@@ -600,14 +623,14 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
   if (any) {
     hipLaunchKernelGGL(k_parse_count, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0, ctx.stream(),
                        kind, d_txt, n_bytes, plan.fb.get(), plan.fe.get(), n_lines,
-                       tile_cnt.get(), err.get());
+                       tile_cnt.get(), partial.get(), err.get());
     GFZ_CHECK_LAUNCH();
   }
   exclusive_scan(ctx, tile_cnt.get(), plan.tile_pos.get(), any ? plan.n_tiles : 0);
   if (any) {
     hipLaunchKernelGGL(k_parse_line_pos, dim3(grid1(n_lines + 1, kBlock)), dim3(kBlock), 0,
-                       ctx.stream(), kind, d_txt, n_bytes, plan.fb.get(), plan.fe.get(), n_lines,
-                       plan.tile_pos.get(), lpos.get(), err.get());
+                       ctx.stream(), plan.fb.get(), n_lines, plan.n_tiles, plan.tile_pos.get(),
+                       partial.get(), lpos.get());
     GFZ_CHECK_LAUNCH();
   }
   std::vector<uint64_t> h_lpos(n_lines + 1, 0);
