@@ -34,6 +34,7 @@ constexpr uint32_t kRtTile = kBlock * kRtItems;     // = 64 words of 64 candidat
 constexpr uint32_t kRtWords = kRtTile / 64;
 static_assert(kRtWords == kWave, "one candidate word per lane of one wave");
 constexpr uint32_t kTileScanThreads = 1024;
+constexpr uint64_t kTileScanSingle = 32768;  // tiles one block scans alone
 
 __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
   return (heads[i >> 5] >> (i & 31)) & 1u;
@@ -1672,30 +1673,27 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
 
 // ---- K2b: carry bit and output offset of every tile --------------------------
 // tile_info[t] = (survivors before tile t) << 1 | pick[-1] of tile t;
-// *total = survivors overall (the next round's length).
-__global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
-    const uint32_t *__restrict__ tile_fn, uint64_t n_tiles, uint64_t L,
-    uint64_t *__restrict__ tile_info, uint64_t *__restrict__ total) {
-  __shared__ FnWide wsum[kTileScanThreads / kWave];
-  __shared__ FnWide carry_in[kTileScanThreads / kWave];
+// *total = survivors overall (the next round's length). A scan of the tiles'
+// transfer functions in three launches: each block of kTileScanThreads threads
+// composes its tiles (one per thread while there are at most kTileScanThreads
+// blocks), one block scans the blocks' compositions, and each block then
+// replays its tiles from its carry-in.
+__device__ inline FnWide tile_widen(const uint32_t *__restrict__ tile_fn, uint64_t t, uint64_t L) {
+  const uint32_t f = tile_fn[t];
+  const uint64_t len = (t + 1) * kRtTile <= L ? kRtTile : L - t * kRtTile;
+  FnWide w;
+  w.o0 = fn_out(f, 0);
+  w.o1 = fn_out(f, 1);
+  w.s0 = len - fn_cnt(f, 0) + w.o0;
+  w.s1 = len - 1 - fn_cnt(f, 1) + w.o1;
+  return w;
+}
+
+// Exclusive scan of one FnWide per thread across the block (kTileScanThreads);
+// returns this thread's prefix, *block_total the block's composition.
+__device__ inline FnWide fnw_block_exclusive(FnWide mine, FnWide *wsum, FnWide *carry_in,
+                                             FnWide *block_total) {
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const uint64_t chunk = (n_tiles + kTileScanThreads - 1) / kTileScanThreads;
-  const uint64_t begin = (uint64_t)threadIdx.x * chunk;
-  const uint64_t end = begin + chunk < n_tiles ? begin + chunk : n_tiles;
-  auto widen = [&](uint64_t t) {
-    const uint32_t f = tile_fn[t];
-    const uint64_t len = (t + 1) * kRtTile <= L ? kRtTile : L - t * kRtTile;
-    FnWide w;
-    w.o0 = fn_out(f, 0);
-    w.o1 = fn_out(f, 1);
-    w.s0 = len - fn_cnt(f, 0) + w.o0;
-    w.s1 = len - 1 - fn_cnt(f, 1) + w.o1;
-    return w;
-  };
-  FnWide mine = fnw_identity();
-  for (uint64_t t = begin; t < end; ++t)
-    mine = fnw_then(mine, widen(t));
-  // Inclusive scan across the wave, then across waves.
   FnWide inc = mine;
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
@@ -1716,7 +1714,7 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
       carry_in[w] = run;
       run = fnw_then(run, wsum[w]);
     }
-    *total = run.s0;
+    *block_total = run;
   }
   __syncthreads();
   FnWide ex;
@@ -1726,12 +1724,76 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(
   ex.o1 = __shfl_up(inc.o1, 1);
   if (lane == 0)
     ex = fnw_identity();
-  const FnWide pre = fnw_then(carry_in[wid], ex);
+  return fnw_then(carry_in[wid], ex);
+}
+
+// Tiles [b * per_block, (b + 1) * per_block) of block b, chunk per thread.
+struct TileRange {
+  uint64_t begin, end;
+};
+__device__ inline TileRange tile_range(uint64_t n_tiles, uint64_t per_block) {
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
+  const uint64_t b1 = b0 + per_block < n_tiles ? b0 + per_block : n_tiles;
+  const uint64_t chunk = (per_block + kTileScanThreads - 1) / kTileScanThreads;
+  const uint64_t begin = b0 + (uint64_t)threadIdx.x * chunk;
+  TileRange r;
+  r.begin = begin < b1 ? begin : b1;
+  r.end = begin + chunk < b1 ? begin + chunk : b1;
+  return r;
+}
+
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_reduce(
+    const uint32_t *__restrict__ tile_fn, uint64_t n_tiles, uint64_t per_block, uint64_t L,
+    FnWide *__restrict__ block_fn) {
+  __shared__ FnWide wsum[kTileScanThreads / kWave];
+  __shared__ FnWide carry_in[kTileScanThreads / kWave];
+  __shared__ FnWide tot;
+  const TileRange r = tile_range(n_tiles, per_block);
+  FnWide mine = fnw_identity();
+  for (uint64_t t = r.begin; t < r.end; ++t)
+    mine = fnw_then(mine, tile_widen(tile_fn, t, L));
+  fnw_block_exclusive(mine, wsum, carry_in, &tot);
+  if (threadIdx.x == 0)
+    block_fn[blockIdx.x] = tot;
+}
+
+// One block: exclusive prefixes of the blocks' compositions (in place; at most
+// kTileScanThreads of them) and the survivors overall.
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_blocks(FnWide *__restrict__ block_fn,
+                                                                  uint64_t n_blocks,
+                                                                  uint64_t *__restrict__ total) {
+  __shared__ FnWide wsum[kTileScanThreads / kWave];
+  __shared__ FnWide carry_in[kTileScanThreads / kWave];
+  __shared__ FnWide tot;
+  const FnWide mine = threadIdx.x < n_blocks ? block_fn[threadIdx.x] : fnw_identity();
+  const FnWide pre = fnw_block_exclusive(mine, wsum, carry_in, &tot);
+  if (threadIdx.x < n_blocks)
+    block_fn[threadIdx.x] = pre;
+  if (threadIdx.x == 0)
+    *total = tot.s0;
+}
+
+// block_pre null: a single block with no carry-in, which also writes *total.
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_apply(
+    const uint32_t *__restrict__ tile_fn, uint64_t n_tiles, uint64_t per_block, uint64_t L,
+    const FnWide *__restrict__ block_pre, uint64_t *__restrict__ tile_info,
+    uint64_t *__restrict__ total) {
+  __shared__ FnWide wsum[kTileScanThreads / kWave];
+  __shared__ FnWide carry_in[kTileScanThreads / kWave];
+  __shared__ FnWide tot;
+  const TileRange r = tile_range(n_tiles, per_block);
+  FnWide mine = fnw_identity();
+  for (uint64_t t = r.begin; t < r.end; ++t)
+    mine = fnw_then(mine, tile_widen(tile_fn, t, L));
+  const FnWide in = block_pre ? block_pre[blockIdx.x] : fnw_identity();
+  const FnWide pre = fnw_then(in, fnw_block_exclusive(mine, wsum, carry_in, &tot));
+  if (!block_pre && threadIdx.x == 0)
+    *total = tot.s0;
   uint32_t p = pre.o0;
   uint64_t off = pre.s0;
-  for (uint64_t t = begin; t < end; ++t) {
+  for (uint64_t t = r.begin; t < r.end; ++t) {
     tile_info[t] = (off << 1) | p;
-    const FnWide w = widen(t);
+    const FnWide w = tile_widen(tile_fn, t, L);
     off += p ? w.s1 : w.s0;
     p = p ? w.o1 : w.o0;
   }
@@ -2406,10 +2468,29 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     });
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("cand_tiles", 4.0 * L);
+    // Up to kTileScanSingle tiles one block scans them all (one launch); beyond,
+    // blocks of kTileScanThreads tiles while that makes at most kTileScanThreads
+    // blocks (then each thread takes a chunk), in three launches.
+    const bool single = n_tiles <= kTileScanSingle;
+    const uint64_t n_blocks =
+        single ? 1 : std::min<uint64_t>(ceil_div(n_tiles, kTileScanThreads), kTileScanThreads);
+    const uint64_t per_block = ceil_div(n_tiles, n_blocks);
+    DBuf<FnWide> block_fn(pool, n_blocks);
+    uint64_t *d_total = reinterpret_cast<uint64_t *>(scal.get());
     ctx.timed("tile_scan", [&] {
-      hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
-                         tile_fn.get(), n_tiles, L, tile_info.get(),
-                         reinterpret_cast<uint64_t *>(scal.get()));
+      if (single) {
+        hipLaunchKernelGGL(k_tile_apply, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
+                           tile_fn.get(), n_tiles, per_block, L, (const FnWide *)nullptr,
+                           tile_info.get(), d_total);
+      } else {
+        hipLaunchKernelGGL(k_tile_reduce, dim3((uint32_t)n_blocks), dim3(kTileScanThreads), 0,
+                           ctx.stream(), tile_fn.get(), n_tiles, per_block, L, block_fn.get());
+        hipLaunchKernelGGL(k_tile_blocks, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
+                           block_fn.get(), n_blocks, d_total);
+        hipLaunchKernelGGL(k_tile_apply, dim3((uint32_t)n_blocks), dim3(kTileScanThreads), 0,
+                           ctx.stream(), tile_fn.get(), n_tiles, per_block, L, block_fn.get(),
+                           tile_info.get(), (uint64_t *)nullptr);
+      }
     });
     GFZ_CHECK_LAUNCH();
 
