@@ -54,20 +54,33 @@ constexpr uint64_t kRep = 1ull << 63;
 // bit 31 for the reverse orientation.
 constexpr uint32_t kSlotRev = 1u << 31;
 
-__device__ inline uint64_t compact_key(int64_t k, uint64_t span) {
-  const int64_t a = key_first(k), b = key_second(k);
-  const int64_t half = (int64_t)(span / 2);
-  return (uint64_t)(a + half) * span + (uint64_t)(b >= 0 ? b : b + (int64_t)span);
-}
+// The quotient by span (< 2^31) from a double estimate, which is within one of
+// it: c < span^2 is exact in a double below 2^53, and above that its rounding
+// moves c / span by under 2^-21.
 __device__ inline int64_t expand_key(uint64_t c, uint64_t span) {
-  const int64_t half = (int64_t)(span / 2);
-  const int64_t a = (int64_t)(c / span) - half;
-  const int64_t bq = (int64_t)(c % span);
-  const int64_t b = bq < half ? bq : bq - (int64_t)span;
-  return pack2((int32_t)a, (int32_t)b);
+  const uint32_t sp = (uint32_t)span, half = sp / 2;
+  uint32_t q = (uint32_t)((double)c / (double)sp);
+  int64_t r = (int64_t)c - (int64_t)((uint64_t)q * sp);
+  if (r < 0) {
+    --q;
+    r += sp;
+  } else if (r >= (int64_t)sp) {
+    ++q;
+    r -= sp;
+  }
+  const int32_t a = (int32_t)(q - half);
+  const int32_t b = (uint32_t)r < half ? (int32_t)r : (int32_t)((uint32_t)r - sp);
+  return pack2(a, b);
 }
+// compact_key(canon2(pack2(a, b))) in 32-bit arithmetic: the canonical pair is
+// (a, b) or its reverse (-b, -a), whichever packs smaller (a signed, then b as
+// an unsigned 32-bit word), and both factors of the product fit 31 bits.
 __device__ inline uint64_t pair_key(int32_t a, int32_t b, uint64_t span) {
-  return compact_key(canon2(pack2(a, b)), span);
+  const int32_t ra = (int32_t)(0u - (uint32_t)b), rb = (int32_t)(0u - (uint32_t)a);
+  const bool rev = ra < a || (ra == a && (uint32_t)rb < (uint32_t)b);
+  const int32_t ca = rev ? ra : a, cb = rev ? rb : b;
+  const uint32_t sp = (uint32_t)span, half = sp / 2;
+  return (uint64_t)((uint32_t)ca + half) * sp + (uint32_t)(cb >= 0 ? cb : (int32_t)((uint32_t)cb + sp));
 }
 
 // The pairs starting at elements first + j * stride (j < N, one thread's pass):
