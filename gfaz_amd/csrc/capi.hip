@@ -545,7 +545,7 @@ int gfz_rulebook_rule_size(gfz_rulebook *rb, uint64_t rule_index, uint32_t *size
   return guarded(kDecompressPrefix, [&] {
     gfz::Ctx &c = *rb->owner->ctx;
     gfz::DeviceGuard g(c.device());
-    GFZ_HIP(hipMemcpyAsync(size, rb->rb.size.get() + rule_index, 4, hipMemcpyDeviceToHost,
+    GFZ_HIP(hipMemcpyAsync(size, &rb->rb.ss.get()[rule_index].x, 4, hipMemcpyDeviceToHost,
                            c.stream()));
     c.sync();
   });

@@ -64,8 +64,9 @@ struct RuleRec {
 
 struct Rulebook {
   DBuf<RuleRec> rec;
-  DBuf<uint32_t> size;      // expanded size of each rule
-  DBuf<uint32_t> sum;       // sum of each rule's expansion (mod 2^32)
+  // Per rule: x = expanded size, y = sum of the expansion (mod 2^32); one
+  // 8-byte gather gives both.
+  DBuf<uint2> ss;
   uint64_t n_rules = 0;
   uint32_t min_id = 0;
   uint32_t depth = 0;       // deepest rule

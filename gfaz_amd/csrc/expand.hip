@@ -49,8 +49,7 @@ constexpr uint32_t kSsScanThreads = 1024;
 __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict__ f,
                                                       const int32_t *__restrict__ s, uint64_t R,
                                                       uint32_t min_id, RuleRec *__restrict__ rec,
-                                                      uint32_t *__restrict__ size,
-                                                      uint32_t *__restrict__ rsum,
+                                                      uint2 *__restrict__ rss,
                                                       uint32_t *__restrict__ depth) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
        r += (uint64_t)gridDim.x * kBlock) {
@@ -62,17 +61,16 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
     q.sb = is_rule(b, min_id, R) ? 0u : 1u;
     rec[r] = q;
     const bool leaf = q.sa && q.sb;
-    size[r] = leaf ? 2u : 0u;
-    rsum[r] = leaf ? (uint32_t)a + (uint32_t)b : 0u;
+    rss[r] = leaf ? make_uint2(2u, (uint32_t)a + (uint32_t)b) : make_uint2(0u, 0u);
     depth[r] = leaf ? 1u : 0u;
   }
 }
 
 // Signed sum (mod 2^32) of a child symbol's expansion.
-__device__ inline uint32_t child_sum(int32_t c, bool raw, const uint32_t *rsum, uint32_t min_id) {
+__device__ inline uint32_t child_sum(int32_t c, bool raw, const uint2 *rss, uint32_t min_id) {
   if (raw)
     return (uint32_t)c;
-  const uint32_t v = rsum[uabs(c) - min_id];
+  const uint32_t v = rss[uabs(c) - min_id].y;
   return c > 0 ? v : (uint32_t)0 - v;
 }
 
@@ -82,8 +80,7 @@ __device__ inline uint32_t child_sum(int32_t c, bool raw, const uint32_t *rsum, 
 // every depth is exact whatever the thread order. `changed` is raised once per
 // wave.
 __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
-                                                      uint32_t *__restrict__ size,
-                                                      uint32_t *__restrict__ rsum,
+                                                      uint2 *__restrict__ rss,
                                                       uint32_t *__restrict__ depth, uint64_t R,
                                                       uint32_t min_id, uint32_t pass,
                                                       uint32_t *changed, uint32_t *err) {
@@ -95,29 +92,36 @@ __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
       RuleRec q = rec[r];
       const bool ra = q.sa != 0, rb = q.sb != 0;  // node-ID children
       bool ok = true;
-      uint32_t sa = q.sa, sb = q.sb;
+      uint32_t sa = q.sa, sb = q.sb, suma = (uint32_t)q.a, sumb = (uint32_t)q.b;
       if (!ra) {
-        const uint32_t dc = depth[uabs(q.a) - min_id];
+        const uint32_t c = uabs(q.a) - min_id;
+        const uint32_t dc = depth[c];
         ok = ok && dc != 0 && dc <= pass + 1;
-        if (ok)
-          sa = size[uabs(q.a) - min_id];
+        if (ok) {
+          const uint2 w = rss[c];
+          sa = w.x;
+          suma = q.a > 0 ? w.y : 0u - w.y;
+        }
       }
       if (!rb) {
-        const uint32_t dc = depth[uabs(q.b) - min_id];
+        const uint32_t c = uabs(q.b) - min_id;
+        const uint32_t dc = depth[c];
         ok = ok && dc != 0 && dc <= pass + 1;
-        if (ok)
-          sb = size[uabs(q.b) - min_id];
+        if (ok) {
+          const uint2 w = rss[c];
+          sb = w.x;
+          sumb = q.b > 0 ? w.y : 0u - w.y;
+        }
       }
       if (ok) {
         const uint32_t total = sa + sb;
         if (total < sa) {
           atomicExch(err, 1u);
         } else {
-          rsum[r] = child_sum(q.a, ra, rsum, min_id) + child_sum(q.b, rb, rsum, min_id);
+          rss[r] = make_uint2(total, suma + sumb);
           q.sa = sa;
           q.sb = sb;
           rec[r] = q;
-          size[r] = total;
           depth[r] = pass + 2;
           resolved = true;
         }
@@ -166,10 +170,11 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
     dst[i] = src[idx[i]];
 }
 
-__global__ void k_plus_one(const uint32_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+// Expanded size + 1 (the rule's Z entries) of every rule.
+__global__ void k_plus_one(const uint2 *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = in[i] + 1u;
+    out[i] = in[i].x + 1u;
 }
 
 // ---- decode planning: symbol offsets and records in three launches ---------
@@ -289,8 +294,7 @@ __device__ inline uint64_t plan_chunk(uint64_t s0) {
 // Symbols and their rules' sizes and sums; issued before the tile's traversal
 // starts are looked up, so the two latencies overlap.
 __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc, uint64_t s0,
-                                 const uint32_t *__restrict__ rsize,
-                                 const uint32_t *__restrict__ rsum, uint64_t R, uint32_t min_id,
+                                 const uint2 *__restrict__ rss, uint64_t R, uint32_t min_id,
                                  PlanSyms &p) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t c0 = plan_chunk(s0);
@@ -304,9 +308,9 @@ __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc
 #pragma unroll
   for (uint32_t j = 0; j < kPlanItems; ++j) {
     const bool rule = is_rule(p.v[j], min_id, R);
-    const uint32_t r = rule ? uabs(p.v[j]) - min_id : 0u;
-    p.sz[j] = rule ? rsize[r] : 1u;
-    p.ss[j] = rule ? rsum[r] : 0u;
+    const uint2 w = rule ? rss[uabs(p.v[j]) - min_id] : make_uint2(1u, 0u);
+    p.sz[j] = w.x;
+    p.ss[j] = w.y;
   }
 }
 // Then the start flags (from the LDS bitmap) and the sizes and signed sums.
@@ -403,15 +407,19 @@ __device__ inline PlanAgg plan_waves_prefix(const PlanAgg &wave_total, PlanAgg *
 
 __global__ __launch_bounds__(kBlock) void k_plan_reduce(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
-    uint64_t n_starts, const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
-    uint64_t R, uint32_t min_id, PlanAgg *__restrict__ tiles, uint2 *__restrict__ szss) {
+    uint64_t n_starts, const uint2 *__restrict__ rss, uint64_t R, uint32_t min_id,
+    uint64_t n_tiles, uint32_t rows, uint64_t cols,
+    PlanAgg *__restrict__ tiles, uint2 *__restrict__ szss) {
   __shared__ uint32_t hbits[kPlanTile / 32];
   __shared__ uint64_t range[2];
   __shared__ PlanAgg lds[kBlock / kWave];
-  const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
+  const uint64_t t = block_tile(rows, cols);
+  if (t >= n_tiles)
+    return;  // uniform per block, before any barrier
+  const uint64_t s0 = t * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
   PlanSyms p;
-  plan_syms(enc, n_enc, s0, rsize, rsum, R, min_id, p);
+  plan_syms(enc, n_enc, s0, rss, R, min_id, p);
   plan_heads(starts, n_starts, s0, s1, hbits, range);
   plan_finish(s0, hbits, R, min_id, p);
   {
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_reduce(
   PlanAgg total;
   plan_waves_prefix(plan_wave_total(p), lds, total);
   if (threadIdx.x == 0)
-    tiles[blockIdx.x] = total;
+    tiles[t] = total;
 }
 
 // One block: tile aggregates -> exclusive tile prefixes (in place); the grand
@@ -457,13 +465,16 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
     uint64_t n_starts, const uint2 *__restrict__ szss, const uint64_t *__restrict__ zoff,
     uint64_t R, uint32_t min_id, uint32_t zraw, int delta,
-    const PlanAgg *__restrict__ tile_prefix, uint64_t *__restrict__ sym_off,
-    SymInfo *__restrict__ syms) {
+    const PlanAgg *__restrict__ tile_prefix, uint64_t n_tiles, uint32_t rows, uint64_t cols,
+    uint64_t *__restrict__ sym_off, SymInfo *__restrict__ syms) {
   __shared__ uint32_t hbits[kPlanTile / 32];
   __shared__ uint64_t range[2];
   __shared__ PlanAgg lds[kBlock / kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint64_t s0 = (uint64_t)blockIdx.x * kPlanTile;
+  const uint64_t t = block_tile(rows, cols);
+  if (t >= n_tiles)
+    return;  // uniform per block, before any barrier
+  const uint64_t s0 = t * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
   PlanSyms p;
   plan_syms_stored(enc, n_enc, s0, szss, p);
@@ -480,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
   plan_heads_only(s0, hbits, p);
   PlanAgg total;
   PlanAgg carry =
-      plan_combine(tile_prefix[blockIdx.x], plan_waves_prefix(plan_wave_total(p), lds, total));
+      plan_combine(tile_prefix[t], plan_waves_prefix(plan_wave_total(p), lds, total));
   const uint64_t c0 = plan_chunk(s0);
 #pragma unroll
   for (uint32_t j = 0; j < kPlanItems; ++j) {
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
         const uint32_t pre = delta > 0 && !head ? at.sum : 0u;
         SymInfo s;
         const int32_t v = p.v[j];
-        if (is_rule(v, min_id, R)) {  // as rule_sym: ss = -rsum for a reversed rule
+        if (is_rule(v, min_id, R)) {  // as rule_sym: ss = -sum for a reversed rule
           if (v > 0) {
             s.base = pre;
             s.zz = (uint32_t)(zo[j] + 1);
@@ -548,19 +559,24 @@ __global__ void k_tile_desc(const uint64_t *__restrict__ sym_off, uint64_t n_sym
 
 // Expansion through Z (see the file comment): out[base + o] for o in [0, n_out),
 // one tile per block, symbols found through the tile descriptors. Z and out may
-// be one buffer when the ranges read and written are disjoint.
+// be one buffer when the ranges read and written are disjoint. Blocks take tiles
+// in block_tile order.
 template <uint32_t kItems, bool kDiff>
 __global__ __launch_bounds__(kBlock) void k_expand_z(const SymInfo *__restrict__ syms,
                                                      const uint64_t *__restrict__ sym_off,
                                                      uint64_t base, uint64_t n_out,
                                                      const TileDesc *__restrict__ tdesc,
-                                                     const int32_t *Z, uint32_t zraw,
-                                                     int32_t *out) {
+                                                     uint64_t n_tiles, uint32_t rows,
+                                                     uint64_t cols, const int32_t *Z,
+                                                     uint32_t zraw, int32_t *out) {
   constexpr uint32_t kTile = kBlock * kItems;
   __shared__ ZTileSmem<kItems> sm;
-  const uint64_t o0 = (uint64_t)blockIdx.x * kTile;
+  const uint64_t t = block_tile(rows, cols);
+  if (t >= n_tiles)
+    return;  // uniform per block, before any barrier
+  const uint64_t o0 = t * kTile;
   const uint32_t n_here = (uint32_t)(o0 + kTile < n_out ? kTile : n_out - o0);
-  const TileDesc td = tdesc[blockIdx.x];
+  const TileDesc td = tdesc[t];
   expand_tile<kItems, kDiff>(sm, syms, sym_off, td.k0, (uint32_t)(td.k_last - td.k0 + 1),
                              base + o0, n_here, Z, zraw, out + base + o0);
 }
@@ -589,8 +605,8 @@ __device__ inline uint64_t wave_last_le_u64(const uint64_t *__restrict__ a, uint
 // straight into LDS; children are at shallower, already-built levels.
 template <uint32_t kItems>
 __global__ __launch_bounds__(kBlock) void k_expand_level(
-    const RuleRec *__restrict__ rec, const uint32_t *__restrict__ rsize,
-    const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff, uint64_t r0,
+    const RuleRec *__restrict__ rec, const uint2 *__restrict__ rss,
+    const uint64_t *__restrict__ zoff, uint64_t r0,
     uint64_t r1, uint32_t min_id, uint64_t R, uint64_t lo, uint64_t n_out, int32_t *Z,
     uint32_t zraw) {
   constexpr uint32_t kTile = kBlock * kItems;
@@ -624,11 +640,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_level(
     const RuleRec c = rec[r];
     const uint64_t z0 = zoff[r];
     const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
-    const uint32_t suma = child_sum(c.a, ra, rsum, min_id);
+    const uint32_t suma = child_sum(c.a, ra, rss, min_id);
     const SymInfo sa = ra ? SymInfo{(uint32_t)c.a, zraw}
-                          : rule_sym(c.a, 0u, true, rsize, rsum, zoff, min_id);
+                          : rule_sym(c.a, 0u, true, rss, zoff, min_id);
     const SymInfo sb = rb ? SymInfo{suma + (uint32_t)c.b, zraw}
-                          : rule_sym(c.b, suma, true, rsize, rsum, zoff, min_id);
+                          : rule_sym(c.b, suma, true, rss, zoff, min_id);
     const uint64_t g0 = 3 * (r - rf);  // symbol q of rule r is g0 + q - q0
 #pragma unroll
     for (uint32_t q = 0; q < 3; ++q) {
@@ -823,22 +839,39 @@ constexpr uint32_t kZItems[] = {8, 16, 32};
 
 template <uint32_t kItems>
 void launch_z(Ctx &ctx, bool diff, const SymInfo *syms, const uint64_t *off, uint64_t base,
-              uint64_t n_out, const TileDesc *td, uint64_t n_tiles, const int32_t *Z,
-              uint32_t zraw, int32_t *out) {
+              uint64_t n_out, const TileDesc *td, uint64_t n_tiles, uint32_t rows,
+              const int32_t *Z, uint32_t zraw, int32_t *out) {
+  uint64_t cols = 0;
+  const uint64_t grid = row_grid(n_tiles, rows, cols);
+  if (grid > UINT32_MAX)
+    throw Error(kErrInvalid, "expansion too large for one launch");
   if (diff)
-    hipLaunchKernelGGL((k_expand_z<kItems, true>), dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                       ctx.stream(), syms, off, base, n_out, td, Z, zraw, out);
+    hipLaunchKernelGGL((k_expand_z<kItems, true>), dim3((uint32_t)grid), dim3(kBlock), 0,
+                       ctx.stream(), syms, off, base, n_out, td, n_tiles, rows, cols, Z, zraw,
+                       out);
   else
-    hipLaunchKernelGGL((k_expand_z<kItems, false>), dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                       ctx.stream(), syms, off, base, n_out, td, Z, zraw, out);
+    hipLaunchKernelGGL((k_expand_z<kItems, false>), dim3((uint32_t)grid), dim3(kBlock), 0,
+                       ctx.stream(), syms, off, base, n_out, td, n_tiles, rows, cols, Z, zraw,
+                       out);
   GFZ_CHECK_LAUNCH();
 }
 
+// Rows of the cross-traversal tile order for a batch of n_trav traversals over
+// n_tiles tiles: one per traversal while every row keeps a few columns, else the
+// plain order. The tuned key "expand_rows" = 0 pins the plain order.
+uint32_t expansion_rows(Ctx &ctx, uint64_t n_trav, uint64_t n_tiles) {
+  if (ctx.tuned("expand_rows") == 0 || n_trav < 2 || n_trav > (1u << 20) ||
+      n_tiles < 4 * kXcds * n_trav)
+    return 1;
+  return (uint32_t)n_trav;
+}
+
 // out[base + o] for o in [0, n_out) from the symbols syms[0..n_sym) whose
-// starts (absolute, offset by base) are off[0..n_sym] - through Z.
+// starts (absolute, offset by base) are off[0..n_sym] - through Z. The output
+// holds n_trav traversals (used for the tile order only).
 void expand_through_z(Ctx &ctx, const char *group, bool diff, const SymInfo *syms,
                       const uint64_t *off, uint64_t n_sym, uint64_t base, uint64_t n_out,
-                      const int32_t *Z, uint32_t zraw, int32_t *out) {
+                      uint64_t n_trav, const int32_t *Z, uint32_t zraw, int32_t *out) {
   if (!n_out)
     return;
   auto run = [&](uint32_t items, bool timed) {
@@ -846,15 +879,16 @@ void expand_through_z(Ctx &ctx, const char *group, bool diff, const SymInfo *sym
     const uint64_t n_tiles = ceil_div(n_out, tile);
     if (n_tiles > UINT32_MAX)
       throw Error(kErrInvalid, "expansion too large for one launch");
+    const uint32_t rows = expansion_rows(ctx, n_trav, n_tiles);
     DBuf<TileDesc> td(ctx.pool(), n_tiles);
     hipLaunchKernelGGL(k_tile_desc, dim3(stride_grid(ctx, n_sym)), dim3(kBlock), 0, ctx.stream(),
                        off, n_sym, base, n_out, (uint32_t)tile, n_tiles, td.get());
     GFZ_CHECK_LAUNCH();
     auto go = [&] {
       switch (items) {
-        case 8: launch_z<8>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
-        case 32: launch_z<32>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
-        default: launch_z<16>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, Z, zraw, out); break;
+        case 8: launch_z<8>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, rows, Z, zraw, out); break;
+        case 32: launch_z<32>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, rows, Z, zraw, out); break;
+        default: launch_z<16>(ctx, diff, syms, off, base, n_out, td.get(), n_tiles, rows, Z, zraw, out); break;
       }
     };
     if (timed)
@@ -892,7 +926,7 @@ void launch_level(Ctx &ctx, const Rulebook &rb, uint64_t r0, uint64_t r1, uint32
   if (n_tiles > UINT32_MAX)
     throw Error(kErrInvalid, "rule bodies of one level too large for one launch");
   hipLaunchKernelGGL(k_expand_level<kItems>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                     ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), rb.zoff.get(), r0,
+                     ctx.stream(), rb.rec.get(), rb.ss.get(), rb.zoff.get(), r0,
                      r1, min_id, R, lo, n_out, rb.z.get(), rb.z_raw);
   GFZ_CHECK_LAUNCH();
 }
@@ -940,8 +974,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   if ((uint64_t)min_id + R > (uint64_t)INT32_MAX + 1)
     throw Error(kErrCorrupt, "rule-ID window exceeds the int32 symbol range");
   rb.rec = DBuf<RuleRec>(ctx.pool(), std::max<uint64_t>(R, 1));
-  rb.size = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
-  rb.sum = DBuf<uint32_t>(ctx.pool(), std::max<uint64_t>(R, 1));
+  rb.ss = DBuf<uint2>(ctx.pool(), std::max<uint64_t>(R, 1));
   if (!R) {  // no rules: Z is just the two zeros node IDs point at
     rb.zoff = DBuf<uint64_t>(ctx.pool(), 1);
     fill_u64(ctx, rb.zoff.get(), 0, 1);
@@ -959,8 +992,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   DBuf<uint32_t> flags(ctx.pool(), kPassBatch + 4);  // changed per pass, err, bad, pad
   uint32_t *err = flags.get() + kPassBatch, *bad = flags.get() + kPassBatch + 1;
   hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     d_first, d_second, R, min_id, rb.rec.get(), rb.size.get(), rb.sum.get(),
-                     depth.get());
+                     d_first, d_second, R, min_id, rb.rec.get(), rb.ss.get(), depth.get());
   GFZ_CHECK_LAUNCH();
   uint32_t max_depth = 1;
   bool converged = false;
@@ -969,7 +1001,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     ctx.timed("rule_sizes", [&] {
       for (uint32_t k = 0; k < kPassBatch; ++k)
         hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0,
-                           ctx.stream(), rb.rec.get(), rb.size.get(), rb.sum.get(), depth.get(),
+                           ctx.stream(), rb.rec.get(), rb.ss.get(), depth.get(),
                            R, min_id, (uint32_t)(pass0 + k), flags.get() + k, err);
     });
     GFZ_CHECK_LAUNCH();
@@ -1003,7 +1035,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     GFZ_CHECK_LAUNCH();
     DBuf<uint32_t> szp1(ctx.pool(), R);
     hipLaunchKernelGGL(k_plus_one, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                       rb.size.get(), R, szp1.get());
+                       rb.ss.get(), R, szp1.get());
     GFZ_CHECK_LAUNCH();
     exclusive_scan(ctx, szp1.get(), rb.zoff.get(), R);
     hipLaunchKernelGGL(k_gather_u64, dim3(1), dim3(kBlock), 0, ctx.stream(), rb.zoff.get(),
@@ -1102,18 +1134,24 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
     const uint64_t n_tiles = ceil_div(n_enc, kPlanTile);
     DBuf<PlanAgg> tiles(ctx.pool(), n_tiles);
     DBuf<uint2> szss(ctx.pool(), n_enc);  // each symbol's size and signed sum
+    // Same stretch of every traversal together (block_tile): its symbols gather
+    // the same rules' sizes, sums and Z offsets.
+    const uint32_t rows = expansion_rows(ctx, n_trav, n_tiles);
+    uint64_t cols = 0;
+    const uint64_t grid = row_grid(n_tiles, rows, cols);
+    if (grid > UINT32_MAX)
+      throw Error(kErrInvalid, "encoded block too large for one launch");
     ctx.timed("decode_plan", [&] {
-      hipLaunchKernelGGL(k_plan_reduce, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, idx.get(), n_trav + 1, rb.size.get(), rb.sum.get(),
-                         rb.n_rules, rb.min_id, tiles.get(), szss.get());
+      hipLaunchKernelGGL(k_plan_reduce, dim3((uint32_t)grid), dim3(kBlock), 0, ctx.stream(),
+                         d_enc, n_enc, idx.get(), n_trav + 1, rb.ss.get(), rb.n_rules, rb.min_id, n_tiles, rows, cols, tiles.get(), szss.get());
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(kPlanScanThreads), 0, ctx.stream(),
                          tiles.get(), n_tiles, sym_off.get() + n_enc);
       GFZ_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_plan_apply, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(),
+      hipLaunchKernelGGL(k_plan_apply, dim3((uint32_t)grid), dim3(kBlock), 0, ctx.stream(),
                          d_enc, n_enc, idx.get(), n_trav + 1, szss.get(), rb.zoff.get(),
-                         rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(), sym_off.get(),
-                         plan.syms.get());
+                         rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(), n_tiles, rows, cols,
+                         sym_off.get(), plan.syms.get());
       GFZ_CHECK_LAUNCH();
     });
   } else {
@@ -1155,8 +1193,8 @@ void decode_run(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_e
     const SymInfo *syms = plan.syms.get();
     if (delta)
       scans_left -= 1;
-    expand_through_z(ctx, "expand", !delta, syms, d_sym_off, n_enc, 0, n_out, rb.z.get(),
-                     rb.z_raw, d_out);
+    expand_through_z(ctx, "expand", !delta, syms, d_sym_off, n_enc, 0, n_out, n_trav,
+                     rb.z.get(), rb.z_raw, d_out);
     ctx.add_bytes("expand", 4.0 * n_enc + 4.0 * n_out + 8.0 * rb.n_rules);
   } else {
     const uint64_t n_tiles = ceil_div(n_out, kDescTile);

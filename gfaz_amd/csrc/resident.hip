@@ -109,16 +109,16 @@ __global__ __launch_bounds__(kBlock) void k_expand_batch(
 }
 
 __global__ void k_sym_plan_r(const int32_t *__restrict__ enc, uint64_t n,
-                             const uint32_t *__restrict__ rsize, const uint32_t *__restrict__ rsum,
+                             const uint2 *__restrict__ rss,
                              uint64_t R, uint32_t min_id, uint32_t *__restrict__ sz,
                              int32_t *__restrict__ ssum) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const int32_t v = enc[i];
     if (is_rule(v, min_id, R)) {
-      const uint32_t r = uabs(v) - min_id;
-      sz[i] = rsize[r];
-      ssum[i] = (int32_t)(v > 0 ? rsum[r] : (uint32_t)0 - rsum[r]);
+      const uint2 w = rss[uabs(v) - min_id];
+      sz[i] = w.x;
+      ssum[i] = (int32_t)(v > 0 ? w.y : (uint32_t)0 - w.y);
     } else {
       sz[i] = 1u;
       ssum[i] = v;
@@ -128,15 +128,15 @@ __global__ void k_sym_plan_r(const int32_t *__restrict__ enc, uint64_t n,
 
 __global__ void k_sym_info_r(const int32_t *__restrict__ enc, uint64_t n,
                              const int32_t *__restrict__ incl, const int32_t *__restrict__ ssum,
-                             bool delta, const uint32_t *__restrict__ rsize,
-                             const uint32_t *__restrict__ rsum, const uint64_t *__restrict__ zoff,
+                             bool delta, const uint2 *__restrict__ rss,
+                             const uint64_t *__restrict__ zoff,
                              uint64_t R, uint32_t min_id, uint32_t zraw,
                              SymInfo *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const int32_t v = enc[i];
     const uint32_t pre = delta ? (uint32_t)incl[i] - (uint32_t)ssum[i] : 0u;
-    out[i] = is_rule(v, min_id, R) ? rule_sym(v, pre, delta, rsize, rsum, zoff, min_id)
+    out[i] = is_rule(v, min_id, R) ? rule_sym(v, pre, delta, rss, zoff, min_id)
                                    : SymInfo{pre + (uint32_t)v, zraw};
   }
 }
@@ -200,7 +200,7 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
     DBuf<int32_t> incl(ctx.pool(), std::max<uint64_t>(n_enc, 1));
     if (n_enc) {
       hipLaunchKernelGGL(k_sym_plan_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, rb.size.get(), rb.sum.get(), rb.n_rules, rb.min_id,
+                         d_enc, n_enc, rb.ss.get(), rb.n_rules, rb.min_id,
                          sz.get(), ssum.get());
       GFZ_CHECK_LAUNCH();
     }
@@ -222,7 +222,7 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
     }
     if (n_enc) {
       hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, incl.get(), ssum.get(), delta, rb.size.get(), rb.sum.get(),
+                         d_enc, n_enc, incl.get(), ssum.get(), delta, rb.ss.get(),
                          rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get());
       GFZ_CHECK_LAUNCH();
     }
@@ -452,7 +452,7 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
     DBuf<int32_t> ssum(ctx.pool(), n_new), incl(ctx.pool(), n_new);
     DBuf<uint64_t> off(ctx.pool(), n_new + 1);
     hipLaunchKernelGGL(k_sym_plan_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
-                       d_enc, n_new, rb.size.get(), rb.sum.get(), rb.n_rules, rb.min_id, sz.get(),
+                       d_enc, n_new, rb.ss.get(), rb.n_rules, rb.min_id, sz.get(),
                        ssum.get());
     GFZ_CHECK_LAUNCH();
     exclusive_scan(ctx, sz.get(), off.get(), n_new);
@@ -474,7 +474,7 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
       ctx.sync();  // `starts` is pageable
     }
     hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
-                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.size.get(), rb.sum.get(),
+                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.ss.get(),
                        rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
     GFZ_CHECK_LAUNCH();
     GFZ_HIP(hipMemcpyAsync(h_off.data(), off.get(), (n_new + 1) * 8, hipMemcpyDeviceToHost,

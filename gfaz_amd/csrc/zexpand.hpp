@@ -13,6 +13,36 @@ __device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
   return a >= min_id && (uint64_t)(a - min_id) < R;
 }
 
+// ---- block order ------------------------------------------------------------------
+// Which tile a block takes. The tiles [0, rows * cols) are read as `rows` rows of
+// `cols` (row r = tiles [r * cols, (r + 1) * cols)); blocks take them column by
+// column, and whole columns are dealt to the XCDs in turn (the dispatcher hands
+// consecutive blocks to the eight XCDs round-robin), so one XCD's L2 holds what
+// the same column of every row gathers. With one traversal per row that column
+// is the same stretch of every haplotype, which gathers the same rule bodies.
+// Returns UINT64_MAX for the padding blocks of a short last column group;
+// rows <= 1 keeps the launch order.
+constexpr uint32_t kXcds = 8;
+
+__device__ inline uint64_t block_tile(uint32_t rows, uint64_t cols) {
+  const uint64_t b = blockIdx.x;
+  if (rows <= 1)
+    return b;
+  const uint64_t i = b / kXcds;
+  const uint64_t c = (i / rows) * kXcds + b % kXcds;
+  return c < cols ? (i % rows) * cols + c : UINT64_MAX;
+}
+
+// Grid of a launch ordered by block_tile over n_tiles tiles (cols set for it).
+inline uint64_t row_grid(uint64_t n_tiles, uint32_t rows, uint64_t &cols) {
+  if (rows <= 1) {
+    cols = n_tiles;
+    return n_tiles;
+  }
+  cols = (n_tiles + rows - 1) / rows;
+  return (uint64_t)kXcds * rows * ((cols + kXcds - 1) / kXcds);
+}
+
 // ---- symbols of an expansion ------------------------------------------------------
 // What one symbol writes at its local position q: base + Z[zidx0 + q], or
 // base + Z[zidx0 - q] when bit 31 of zz (reverse orientation) is set, with
@@ -22,16 +52,18 @@ __device__ inline bool is_rule(int32_t v, uint32_t min_id, uint64_t R) {
 // Z[j] - Z[j -/+ 1] instead, a body element.
 constexpr uint32_t kRevBit = 0x80000000u;
 
-__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint32_t *rsize,
-                                   const uint32_t *rsum, const uint64_t *zoff, uint32_t min_id) {
+// rss[r] = (expanded size, sum) of rule r.
+__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint2 *rss,
+                                   const uint64_t *zoff, uint32_t min_id) {
   const uint32_t r = uabs(v) - min_id;
   SymInfo s;
   if (v > 0) {
     s.base = pre;
     s.zz = (uint32_t)(zoff[r] + 1);
   } else {
-    s.base = delta ? pre - rsum[r] : pre;
-    s.zz = (uint32_t)(zoff[r] + rsize[r] - 1) | kRevBit;
+    const uint2 w = rss[r];
+    s.base = delta ? pre - w.y : pre;
+    s.zz = (uint32_t)(zoff[r] + w.x - 1) | kRevBit;
   }
   return s;
 }
