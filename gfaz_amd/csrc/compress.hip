@@ -83,15 +83,32 @@ __device__ inline uint64_t pair_key(int32_t a, int32_t b, uint64_t span) {
   return (uint64_t)((uint32_t)ca + half) * sp + (uint32_t)(cb >= 0 ? cb : (int32_t)((uint32_t)cb + sp));
 }
 
+// Fresh pairs. The reference encoder is a left-to-right stack greedy
+// (src/grammar/path_encoder.cpp:29-83): two symbols that stay adjacent and
+// unreplaced through a round did not form a rule of that round, so their
+// canonical key occurred at most once in it (rule_generator.cpp:148-260), and
+// it occurs at most once in the next stream too (each of its occurrences there
+// is one of the round before). So from the second round on, only pairs with a
+// symbol of the last round's rules (|v| >= fresh, the floor of those IDs) can
+// repeat or be rules; the others are skipped by counting and candidate marking
+// alike, which leaves the grammar unchanged. (Equivalently: a rule made in
+// round r > 1 has depth r, so one of its children is a rule of round r - 1.)
+// fresh = 0 keeps every pair.
+__device__ inline bool fresh_pair(int32_t a, int32_t b, uint32_t fresh) {
+  const uint32_t ua = uabs(a), ub = uabs(b);
+  return (ua > ub ? ua : ub) >= fresh;
+}
+
 // The pairs starting at elements first + j * stride (j < N, one thread's pass):
-// a[j], b[j] and a bit set of the valid ones (i + 1 < L and no traversal starts
-// at i + 1). Indexes are clamped into [0, L - 2] instead of guarded (callers run
-// with L >= 2), so every load of the pass is in flight before any is used.
+// a[j], b[j] and a bit set of the valid ones (i + 1 < L, no traversal starts
+// at i + 1, and the pair is fresh). Indexes are clamped into [0, L - 2] instead
+// of guarded (callers run with L >= 2), so every load of the pass is in flight
+// before any is used.
 template <uint32_t N>
 __device__ inline uint32_t load_pairs(const int32_t *__restrict__ x,
                                       const uint32_t *__restrict__ heads, uint64_t L,
                                       uint64_t first, uint64_t stride, int32_t (&a)[N],
-                                      int32_t (&b)[N]) {
+                                      int32_t (&b)[N], uint32_t fresh) {
   uint32_t hw[N];
 #pragma unroll
   for (uint32_t j = 0; j < N; ++j) {
@@ -105,7 +122,9 @@ __device__ inline uint32_t load_pairs(const int32_t *__restrict__ x,
 #pragma unroll
   for (uint32_t j = 0; j < N; ++j) {
     const uint64_t i = first + (uint64_t)j * stride;
-    valid |= (i + 1 < L && !((hw[j] >> ((i + 1) & 31)) & 1)) ? 1u << j : 0u;
+    valid |= (i + 1 < L && !((hw[j] >> ((i + 1) & 31)) & 1) && fresh_pair(a[j], b[j], fresh))
+                 ? 1u << j
+                 : 0u;
   }
   return valid;
 }
@@ -115,9 +134,9 @@ template <uint32_t N>
 __device__ inline uint32_t load_pair_keys(const int32_t *__restrict__ x,
                                           const uint32_t *__restrict__ heads, uint64_t L,
                                           uint64_t span, uint64_t first, uint64_t stride,
-                                          uint64_t (&kv)[N]) {
+                                          uint64_t (&kv)[N], uint32_t fresh) {
   int32_t a[N], b[N];
-  const uint32_t valid = load_pairs<N>(x, heads, L, first, stride, a, b);
+  const uint32_t valid = load_pairs<N>(x, heads, L, first, stride, a, b, fresh);
 #pragma unroll
   for (uint32_t j = 0; j < N; ++j)
     kv[j] = (valid >> j) & 1 ? pair_key(a[j], b[j], span) : kFree;
@@ -564,6 +583,7 @@ template <uint32_t kSlots>
 __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__restrict__ x,
                                                            const uint32_t *__restrict__ heads,
                                                            uint64_t L, uint64_t span,
+                                                           uint32_t fresh,
                                                            unsigned long long *__restrict__ out,
                                                            uint32_t *__restrict__ out_n) {
   __shared__ unsigned long long skeys[kSlots];
@@ -575,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate(const int32_t *__rest
   const uint64_t base = (uint64_t)blockIdx.x * kInsTile;
   uint64_t kv[kInsItems];
   const uint32_t valid =
-      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
+      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv, fresh);
   // A wave-wide dictionary of up to kDict keys with counts (wave-uniform values)
   // absorbs the hot keys without LDS atomics; it is flushed once at the end.
   uint64_t dkey[kDict];
@@ -700,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_staged(
 __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__restrict__ x,
                                                                const uint32_t *__restrict__ heads,
                                                                uint64_t L, uint64_t span,
-                                                               CountTable t) {
+                                                               uint32_t fresh, CountTable t) {
   __shared__ uint32_t sfresh[kBlock / kWave], sreps[kBlock / kWave];
   __shared__ uint32_t s_stage_n;
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -711,23 +731,23 @@ __global__ __launch_bounds__(kBlock) void k_pair_insert_direct(const int32_t *__
   uint64_t kv[kInsItems];
   uint32_t cnt[kInsItems];
   const uint32_t have =
-      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv);
+      load_pair_keys<kInsItems>(x, heads, L, span, base + threadIdx.x, kBlock, kv, fresh);
 #pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j)
     cnt[j] = 1;
-  uint32_t fresh = 0, reps = 0;
+  uint32_t newk = 0, reps = 0;
   if (have) {
     uint32_t newrep = 0;
-    count_insert_batch<(int)kInsItems>(t, kv, cnt, have, fresh, reps, newrep);
+    count_insert_batch<(int)kInsItems>(t, kv, cnt, have, newk, reps, newrep);
     stage_keys<(int)kInsItems>(t, &s_stage_n, kv, newrep);
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
-    fresh += __shfl_xor(fresh, d);
+    newk += __shfl_xor(newk, d);
     reps += __shfl_xor(reps, d);
   }
   if (lane == 0) {
-    sfresh[wid] = fresh;
+    sfresh[wid] = newk;
     sreps[wid] = reps;
   }
   __syncthreads();
@@ -817,10 +837,11 @@ __device__ inline void part_starts(uint32_t *hist, const Parts &pt) {
 // key (its first lane's) is written once; if the wave holds it twice it is
 // already known repeated (kRep set in the entry).
 __device__ inline void tile_keys(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
-                                 uint64_t L, uint64_t span, uint64_t t, uint64_t (&kv)[kInsItems]) {
+                                 uint64_t L, uint64_t span, uint32_t fresh, uint64_t t,
+                                 uint64_t (&kv)[kInsItems]) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t have =
-      load_pair_keys<kInsItems>(x, heads, L, span, t * kInsTile + threadIdx.x, kBlock, kv);
+  const uint32_t have = load_pair_keys<kInsItems>(x, heads, L, span, t * kInsTile + threadIdx.x,
+                                                  kBlock, kv, fresh);
 #pragma unroll
   for (uint32_t j = 0; j < kInsItems; ++j) {
     const bool v = (have >> j) & 1;
@@ -843,8 +864,8 @@ __device__ inline void tile_keys(const int32_t *__restrict__ x, const uint32_t *
 
 __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restrict__ x,
                                                         const uint32_t *__restrict__ heads,
-                                                        uint64_t L, uint64_t span, uint64_t n_tiles,
-                                                        uint32_t per, Parts pt) {
+                                                        uint64_t L, uint64_t span, uint32_t fresh,
+                                                        uint64_t n_tiles, uint32_t per, Parts pt) {
   extern __shared__ uint32_t hist[];  // one counter per bucket (dynamic LDS)
   for (uint32_t i = threadIdx.x; i < pt.nb; i += kBlock)
     hist[i] = 0;
@@ -853,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
   const uint64_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
   uint64_t kv[kInsItems];
   for (uint64_t t = t0; t < t1; ++t) {
-    tile_keys(x, heads, L, span, t, kv);
+    tile_keys(x, heads, L, span, fresh, t, kv);
 #pragma unroll
     for (uint32_t j = 0; j < kInsItems; ++j)
       if (kv[j] != kFree)
@@ -864,7 +885,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
   // Second pass (the same keys, re-read from cache): LDS cursors place them.
   unsigned long long *dst = pt.entries + (uint64_t)blockIdx.x * pt.region;
   for (uint64_t t = t0; t < t1; ++t) {
-    tile_keys(x, heads, L, span, t, kv);
+    tile_keys(x, heads, L, span, fresh, t, kv);
 #pragma unroll
     for (uint32_t j = 0; j < kInsItems; ++j)
       if (kv[j] != kFree)
@@ -1337,7 +1358,8 @@ struct alignas(16) HotPair {
 
 __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__restrict__ x,
                                                           const uint32_t *__restrict__ heads,
-                                                          uint64_t L, uint64_t span, RuleTable rt,
+                                                          uint64_t L, uint64_t span,
+                                                          uint32_t fresh, RuleTable rt,
                                                           HotEntry *__restrict__ hot,
                                                           HotPair *__restrict__ top) {
   __shared__ unsigned long long skey[kHotSampleSlots];
@@ -1371,7 +1393,7 @@ __global__ __launch_bounds__(kHotThreads) void k_hot_sample(const int32_t *__res
   for (uint32_t q = threadIdx.x; q < kHotWindows * kHotWin; q += kHotThreads) {
     const uint64_t i =
         spread ? (uint64_t)(q / kHotWin) * room / kHotWindows + q % kHotWin : (uint64_t)q;
-    if (i + 1 >= L || head_at(heads, i + 1))
+    if (i + 1 >= L || head_at(heads, i + 1) || !fresh_pair(x[i], x[i + 1], fresh))
       continue;
     const unsigned long long k = pair_key(x[i], x[i + 1], span);
     uint32_t sl = (uint32_t)key_hash(k) & (kHotSampleSlots - 1);
@@ -1564,9 +1586,9 @@ template <uint32_t N, bool kInterior>
 __device__ inline uint32_t load_tile_pairs(const int32_t *__restrict__ x,
                                            const uint32_t *__restrict__ heads, uint64_t L,
                                            uint64_t base, uint32_t first, uint32_t stride,
-                                           int32_t (&a)[N], int32_t (&b)[N]) {
+                                           int32_t (&a)[N], int32_t (&b)[N], uint32_t fresh) {
   if constexpr (!kInterior) {
-    return load_pairs<N>(x, heads, L, base + first, stride, a, b);
+    return load_pairs<N>(x, heads, L, base + first, stride, a, b, fresh);
   } else {
     const int32_t *__restrict__ xt = x + base;
     const uint32_t *__restrict__ ht = heads + (base >> 5);  // base is a multiple of 32
@@ -1581,7 +1603,9 @@ __device__ inline uint32_t load_tile_pairs(const int32_t *__restrict__ x,
     uint32_t valid = 0;
 #pragma unroll
     for (uint32_t j = 0; j < N; ++j)
-      valid |= ((hw[j] >> ((first + j * stride + 1) & 31)) & 1) ? 0u : 1u << j;
+      valid |= (((hw[j] >> ((first + j * stride + 1) & 31)) & 1) || !fresh_pair(a[j], b[j], fresh))
+                   ? 0u
+                   : 1u << j;
     return valid;
   }
 }
@@ -1593,7 +1617,7 @@ __device__ inline uint32_t load_tile_pairs(const int32_t *__restrict__ x,
 // its picks against the hot pairs again, from the raw pair.
 template <bool kInterior>
 __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
-                                 uint64_t L, uint64_t span, const RuleTable &rt,
+                                 uint64_t L, uint64_t span, uint32_t fresh, const RuleTable &rt,
                                  const HotKeys &hk, HotLds &sh,
                                  unsigned long long (*list)[kColdCap], uint64_t *wbits,
                                  uint64_t *__restrict__ cand, uint64_t n_words,
@@ -1606,7 +1630,7 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
     // Validity is a per-lane bit set (bool arrays live as scalar lane masks).
     int32_t xa[kPassItems], xb[kPassItems];
     const uint32_t valid = load_tile_pairs<kPassItems, kInterior>(
-        x, heads, L, base, j0 * kBlock + threadIdx.x, kBlock, xa, xb);
+        x, heads, L, base, j0 * kBlock + threadIdx.x, kBlock, xa, xb, fresh);
     // Listed items keep their list position per lane (no ballot masks held
     // across the lookups: they would spill scalar registers).
     // Listed (cold) pairs store their slot word after the wave's lookups, and
@@ -1658,7 +1682,7 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
 
 __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const uint32_t *__restrict__ heads, uint64_t L,
-                                                 uint64_t span, RuleTable rt,
+                                                 uint64_t span, uint32_t fresh, RuleTable rt,
                                                  const HotEntry *__restrict__ hot,
                                                  const HotPair *__restrict__ top,
                                                  uint64_t *__restrict__ cand,
@@ -1674,9 +1698,9 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
   __syncthreads();
   // Every tile but the last is interior (its last pair ends inside the stream).
   if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
-    cand_tile<true>(x, heads, L, span, rt, hk, sh, list, wbits, cand, n_words, slots);
+    cand_tile<true>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, n_words, slots);
   else
-    cand_tile<false>(x, heads, L, span, rt, hk, sh, list, wbits, cand, n_words, slots);
+    cand_tile<false>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, n_words, slots);
   if (wid == 0) {
     const uint32_t f = wave_fn_inclusive(word_fn(wbits[lane]));
     if (lane == kWave - 1)
@@ -2182,11 +2206,17 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   DBuf<int32_t> rf, rs;
   DBuf<uint32_t> d_err(pool, 2);
   fill_u32(ctx, d_err.get(), 0, 2);
-  uint64_t prev_distinct = 0, prev_len = 0;
+  uint64_t prev_distinct = 0, prev_len = 0, prev_elig = 0;
+  // The floor of the last round's rule IDs: only pairs with a symbol at or
+  // above it can repeat (fresh_pair); 0 in the first round.
+  uint32_t fresh = 0;
 
   for (int round = 0; round < p.rounds && L >= 2; ++round) {
     RoundInfo info;
     info.length_in = L;
+    // Pairs that can be fresh: each of the last round's picks left one new
+    // symbol, and a symbol starts at most two pairs (all pairs in the first round).
+    const uint64_t elig = round == 0 ? L : std::min<uint64_t>(L, 2 * (prev_len - L));
     int32_t *dst = (cur_buf == xa.get()) ? xb.get() : xa.get();
     const uint64_t n_tiles = ceil_div(L, kRtTile);
     const uint64_t n_words = ceil_div(L, 64);
@@ -2218,7 +2248,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // and plain_500k, 2, 4 and 8 measured the same within 1 %).
     const int tuned_share = ctx.tuned("count_direct_share");
     const uint64_t direct_share = tuned_share > 0 ? (uint64_t)tuned_share : 2;
-    const bool mostly_distinct = round > 0 && direct_share * prev_distinct > prev_len;
+    // Rounds where most pairs are not fresh aggregate too: the tiles then hold
+    // few keys, and the stream is read once instead of twice.
+    const bool mostly_distinct =
+        round > 0 && direct_share * prev_distinct > prev_elig && 2 * elig > L;
     DBuf<unsigned long long> tiles;  // aggregated tiles (rounds with hot keys)
     DBuf<uint32_t> tiles_n;
     // Tiles of few distinct keys (under a quarter of the last round's pairs were
@@ -2227,20 +2260,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // distinct pairs unless the tuned "count_agg_slots/<size class>" fixes it.
     const int tuned_agg = ctx.tuned("count_agg_slots/" + std::to_string(Ctx::size_class(N)));
     const bool few_keys = tuned_agg > 0 ? tuned_agg <= (int)kInsTile
-                                        : round == 0 || 4 * prev_distinct < prev_len;
+                                        : round == 0 || 4 * prev_distinct < prev_elig;
     auto aggregate = [&] {
       tiles = DBuf<unsigned long long>(pool, n_ins * kInsTile);
       tiles_n = DBuf<uint32_t>(pool, n_ins);
       if (few_keys)
         hipLaunchKernelGGL(k_pair_aggregate<kInsTile>, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                           ctx.stream(), cur, heads.get(), L, span, tiles.get(), tiles_n.get());
+                           ctx.stream(), cur, heads.get(), L, span, fresh, tiles.get(),
+                           tiles_n.get());
       else
         hipLaunchKernelGGL(k_pair_aggregate<2 * kInsTile>, dim3((uint32_t)n_ins), dim3(kBlock),
-                           0, ctx.stream(), cur, heads.get(), L, span, tiles.get(),
+                           0, ctx.stream(), cur, heads.get(), L, span, fresh, tiles.get(),
                            tiles_n.get());
       GFZ_CHECK_LAUNCH();
     };
-    const bool direct = round > 0 && 2 * prev_distinct > prev_len;  // table fallback mode
+    const bool direct = round > 0 && 2 * prev_distinct > prev_elig;  // table fallback mode
     bool stage_overflow = false;
     uint64_t stage_stride = kStageCap, stage_blocks = n_ins;
     bool counted = false;
@@ -2250,7 +2284,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // partitions beyond kMaxBuckets1); larger rounds, and any round whose buckets
     // overflow, count in the HBM table.
     const uint64_t est =
-        round == 0 ? L / 32 : std::min<uint64_t>(L, prev_distinct + prev_distinct / 4);
+        round == 0 ? L / 32 : std::min<uint64_t>(elig, prev_distinct + prev_distinct / 4);
     // Tuning and test knobs (ctx tuned values; every setting gives the same
     // grammar): "count_buckets" fixes the bucket count (a power of two, 64 ..
     // kMaxBuckets), "count_rep_cap" lowers the repeated keys a bucket may list
@@ -2303,7 +2337,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       ctx.timed("pair_insert", [&] {
         if (mostly_distinct) {
           hipLaunchKernelGGL(k_bucket_part, dim3((uint32_t)n_parts), dim3(kBlock), nb1 * 4,
-                             ctx.stream(), cur, heads.get(), L, span, n_ins, per, p1);
+                             ctx.stream(), cur, heads.get(), L, span, fresh, n_ins, per, p1);
         } else {
           // Hot keys: aggregate each tile first, so a key enters a bucket at most
           // once per tile.
@@ -2362,7 +2396,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       ctx.timed("pair_insert", [&] {
         if (direct) {
           hipLaunchKernelGGL(k_pair_insert_direct, dim3((uint32_t)n_ins), dim3(kBlock), 0,
-                             ctx.stream(), cur, heads.get(), L, span, t);
+                             ctx.stream(), cur, heads.get(), L, span, fresh, t);
         } else {
           if (!tiles.get()) {  // the aggregation does not depend on the table size
             aggregate();
@@ -2391,6 +2425,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     info.table_slots = cap;
     prev_distinct = distinct;
     prev_len = L;
+    prev_elig = elig;
     if (repeated == 0)
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
@@ -2411,10 +2446,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
     const uint64_t region_lines = regions ? region_slots / kLine : rcap / kLine;
-    // The absence filter pays once a quarter of the pairs are no rule (each of
-    // the distinct - repeated keys is one such pair): about 16 bits per repeated
+    // The absence filter pays once a quarter of the fresh pairs are no rule (each
+    // of the distinct - repeated keys is one such pair): about 16 bits per repeated
     // key, at most 2^27 bits (16 MB, cache-resident).
-    const bool use_filter = 4 * (distinct - std::min(distinct, repeated)) > L;
+    const bool use_filter = 4 * (distinct - std::min(distinct, repeated)) > elig;
     const uint32_t fbits =
         use_filter ? (uint32_t)std::min<uint64_t>(
                          27, std::max<uint64_t>(12, 64 - __builtin_clzll(
@@ -2465,7 +2500,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<HotPair> top(pool, kHotTop);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_hot_sample, dim3(1), dim3(kHotThreads), 0, ctx.stream(), cur,
-                         heads.get(), L, span, rt, hot.get(), top.get());
+                         heads.get(), L, span, fresh, rt, hot.get(), top.get());
     });
     GFZ_CHECK_LAUNCH();
 
@@ -2476,7 +2511,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint32_t> slots(pool, L);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, span, rt, hot.get(), top.get(), cand.get(), n_words,
+                         heads.get(), L, span, fresh, rt, hot.get(), top.get(), cand.get(), n_words,
                          tile_fn.get(), slots.get());
     });
     GFZ_CHECK_LAUNCH();
@@ -2569,6 +2604,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
     ctx.add_bytes("remap", 8.0 * L_next);
 
+    fresh = next;
     next += (uint32_t)R;
     out.n_rules += R;
     info.length_out = L_next;
