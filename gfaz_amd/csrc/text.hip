@@ -35,7 +35,7 @@ namespace {
 constexpr uint32_t kFmtItems = 8;                    // steps per thread
 constexpr uint32_t kFmtTile = kBlock * kFmtItems;    // steps per tile
 constexpr uint32_t kMaxStepBytes = 12;               // ',' + 10 digits + orientation
-constexpr uint32_t kParseItems = 16;                 // text bytes per thread
+constexpr uint32_t kParseItems = 8;                  // text bytes per thread
 constexpr uint32_t kParseTile = kBlock * kParseItems;
 
 __device__ inline uint32_t ndigits(uint32_t v) {
@@ -223,20 +223,6 @@ __global__ void k_fmt_heads(const uint8_t *__restrict__ heads_txt,
 
 // ---- parsing ---------------------------------------------------------------
 
-// Field of byte p: last l with fb[l] <= p (fb non-decreasing, n_lines entries,
-// fb[0] <= p); the byte is in it when p < fe[l].
-__device__ inline uint64_t line_of(const uint64_t *fb, uint64_t n_lines, uint64_t p) {
-  uint64_t lo = 0, hi = n_lines;
-  while (hi - lo > 1) {
-    const uint64_t mid = (lo + hi) / 2;
-    if (fb[mid] <= p)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // Bytes of the text as a kernel reads them: from an LDS window [w0, w0 + wn) of
 // the tile being parsed where it holds them, else from global memory.
 struct TextReader {
@@ -246,56 +232,106 @@ struct TextReader {
   __device__ uint8_t operator()(uint64_t p) const { return p - w0 < wn ? win[p - w0] : g[p]; }
 };
 
-// The token starting at byte p of a field [s, e) (p < e), if one starts there.
-// Returns 0 (none), 1 (a step: *id signed) or 2 (a dropped step: empty name or
-// value 0); sets *err to 1 (no digits) or 2 (overflow) on a malformed name.
-__device__ inline int parse_token(int kind, const TextReader &txt, uint64_t s, uint64_t p,
-                                  uint64_t e, int32_t *id, uint32_t *err) {
-  uint64_t name_b, name_e;
-  bool rev;
-  if (kind == kTextP) {
-    if (txt(p) == ',' || (p > s && txt(p - 1) != ','))
-      return 0;
-    uint64_t q = p;
-    while (q < e && txt(q) != ',')
-      ++q;
-    // token [p, q), q > p: orientation txt[q - 1], name [p, q - 1)
-    rev = txt(q - 1) == '-';
-    name_b = p;
-    name_e = q - 1;
-  } else {
-    const uint8_t c = txt(p);
-    if (c != '<' && c != '>')
-      return 0;
-    uint64_t q = p + 1;
-    while (q < e && txt(q) != '<' && txt(q) != '>')
-      ++q;
-    rev = c == '<';
-    name_b = p + 1;
-    name_e = q;
+// Where a token's field ends: inside the tile's window [w0, w0 + wn] a bit per
+// byte marks the ends of the fields with a byte in the tile (no field's end
+// lies strictly inside another field; a field may end where the next begins),
+// past it the field holding p is searched for.
+struct FieldEnds {
+  const uint32_t *bits;
+  uint64_t w0, wn;
+  const uint64_t *fb, *fe;
+  uint64_t n;
+  __device__ bool at(uint64_t q) const {
+    const uint64_t o = q - w0;
+    return (bits[o >> 5] >> (o & 31)) & 1u;
   }
-  if (name_e <= name_b)
-    return 2;
-  uint32_t val = 0;
-  bool any = false;
-  for (uint64_t q = name_b; q < name_e; ++q) {
-    const uint8_t c = txt(q);
-    if (c < '0' || c > '9')
-      break;
-    any = true;
-    const uint32_t dg = c - '0';
-    if (val > 0xFFFFFFFFu / 10 || (val == 0xFFFFFFFFu / 10 && dg > 0xFFFFFFFFu % 10)) {
-      atomicOr(err, 2u);
-      return 2;
+  __device__ uint64_t search(uint64_t p) const {  // fe of the last field with fb <= p
+    uint64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (fb[mid] <= p)
+        lo = mid;
+      else
+        hi = mid;
     }
-    val = val * 10 + dg;
+    return fe[lo];
   }
-  if (!any) {
+};
+
+// The token starting at byte p (a token start inside a field). For P the token
+// runs to the next ',' or the field's end, its last byte is the orientation and
+// the bytes before it the name; for W it runs from the '<' / '>' at p to the
+// next one or the field's end, the name after p. One forward pass keeps the
+// state of the name's leading digit run (value, digits, overflow); for P the
+// state before the last byte is the name's. Returns 0 (a dropped step: empty
+// name or value 0) or 1 (a step: *id signed); sets *err to 1 (no digits) or 2
+// (overflow) on a malformed name, as parse_uint32 throws
+// (src/io/gfa_parser.cpp:69-91).
+__device__ inline int parse_token(int kind, const TextReader &txt, const FieldEnds &fends,
+                                  uint64_t p, int32_t *id, uint32_t *err) {
+  const bool pk = kind == kTextP;
+  const uint8_t c0 = txt(p);
+  uint32_t val = 0, ndig = 0;
+  bool ovf = false, run = true;
+  uint32_t bval = 0, bdig = 0;  // P: the state before the byte just consumed
+  bool bovf = false;
+  uint64_t q = pk ? p : p + 1;
+  uint64_t e = ~0ull;  // the field's end once the token leaves the window
+  for (;; ++q) {
+    if (q - fends.w0 <= fends.wn) {
+      if (q > p && fends.at(q))  // at p itself: the end of an adjacent field before
+        break;
+    } else {
+      if (e == ~0ull)
+        e = fends.search(p);
+      if (q >= e)
+        break;
+    }
+    const uint8_t c = txt(q);
+    if (pk ? c == ',' : (c == '<' || c == '>'))
+      break;
+    bval = val;
+    bdig = ndig;
+    bovf = ovf;
+    if (run) {
+      if (c >= '0' && c <= '9') {
+        const uint32_t dg = c - '0';
+        if (!ovf) {
+          if (val > 0xFFFFFFFFu / 10 || (val == 0xFFFFFFFFu / 10 && dg > 0xFFFFFFFFu % 10))
+            ovf = true;
+          else
+            val = val * 10 + dg;
+        }
+        ++ndig;
+      } else {
+        run = false;
+      }
+    }
+  }
+  bool rev;
+  uint64_t name_len;
+  if (pk) {  // token [p, q), q > p; orientation txt[q - 1], name [p, q - 1)
+    rev = txt(q - 1) == '-';
+    name_len = q - 1 - p;
+    val = bval;
+    ndig = bdig;
+    ovf = bovf;
+  } else {
+    rev = c0 == '<';
+    name_len = q - p - 1;
+  }
+  if (!name_len)
+    return 0;
+  if (ovf) {
+    atomicOr(err, 2u);
+    return 0;
+  }
+  if (!ndig) {
     atomicOr(err, 1u);
-    return 2;
+    return 0;
   }
   if (val == 0)
-    return 2;
+    return 0;
   *id = (int32_t)(rev ? 0u - val : val);  // the reference's uint32 -> NodeId conversion
   return 1;
 }
@@ -304,8 +340,17 @@ __device__ inline int parse_token(int kind, const TextReader &txt, uint64_t s, u
 struct FieldCursor {
   const uint64_t *fb, *fe;
   uint64_t n, l, b, e, nb;  // field l = [b, e); nb = the next field's begin
-  __device__ void seek(uint64_t p) {
-    l = line_of(fb, n, p);
+  // Last field in [lo, hi) beginning at or before p (lo if none does): the
+  // fields that can hold a byte of a tile are few, found per tile beforehand.
+  __device__ void seek(uint64_t p, uint64_t lo, uint64_t hi) {
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (fb[mid] <= p)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    l = lo;
     b = fb[l];
     e = fe[l];
     nb = l + 1 < n ? fb[l + 1] : ~0ull;
@@ -344,26 +389,27 @@ __device__ inline uint64_t stage_window(const uint8_t *txt, uint64_t n_bytes, ui
   return wn;
 }
 
-// Kept steps per tile of text bytes, and for every field that starts inside the
-// tile the kept steps of the tile before its first byte (partial[l]).
-__global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t *__restrict__ txt,
-                                                        uint64_t n_bytes,
-                                                        const uint64_t *__restrict__ fb,
-                                                        const uint64_t *__restrict__ fe,
-                                                        uint64_t n_lines,
-                                                        uint32_t *__restrict__ tile_count,
-                                                        uint32_t *__restrict__ partial,
-                                                        uint32_t *__restrict__ err) {
-  __shared__ __align__(16) uint8_t win[kParseWin];
-  __shared__ uint32_t lds[kBlock / kWave + 1];
-  __shared__ uint32_t keep_bits[kParseTile / 32];  // kept-token starts, one bit per byte
-  __shared__ uint64_t lrange[2];                   // fields starting in the tile
-  const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
-  for (uint32_t i = threadIdx.x; i < kParseTile / 32; i += kBlock)
-    keep_bits[i] = 0;
-  if (threadIdx.x < 2) {
-    // first field starting at or after w0 (thread 0) or w0 + kParseTile (thread 1)
-    const uint64_t at = w0 + (uint64_t)threadIdx.x * kParseTile;
+// The tile's tokens, in byte order. Classification: thread t looks at bytes
+// t, t + kBlock, ... (a wave at 64 consecutive bytes per step, so its starts are
+// one ballot, stored as two bitmap words), recording the field of each start;
+// then the bitmap's set bits are listed by a block scan. Parsing then gives
+// every lane a token (a token is several bytes, most bytes start none).
+struct TokenLds {
+  __align__(16) uint8_t win[kParseWin];
+  uint32_t start_bits[kParseTile / 32];
+  uint32_t end_bits[kParseWin / 32 + 1];  // field ends at window offsets 0 .. kParseWin
+  uint16_t list[kParseTile];              // token starts (tile offsets), byte order
+  uint32_t scan[kBlock / kWave + 1];
+};
+
+// first[t] = the first field beginning at or after tile t's first byte
+// (t = 0 .. n_tiles): the fields that can hold a byte of tile t are
+// first[t] - 1 .. first[t + 1] - 1.
+__global__ void k_parse_first(const uint64_t *__restrict__ fb, uint64_t n_lines,
+                              uint64_t n_tiles, uint64_t *__restrict__ first) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t at = t * kParseTile;
     uint64_t lo = 0, hi = n_lines;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) / 2;
@@ -372,33 +418,96 @@ __global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t 
       else
         hi = mid;
     }
-    lrange[threadIdx.x] = lo;
+    first[t] = lo;
   }
-  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};  // syncs
-  // Thread t takes bytes t, t + kBlock, ... of the tile: a wave reads 64
-  // consecutive LDS bytes per step.
-  uint32_t c = 0;
+}
+
+__device__ inline uint32_t tile_tokens(int kind, const uint8_t *__restrict__ txt,
+                                       uint64_t n_bytes, const uint64_t *__restrict__ fb,
+                                       const uint64_t *__restrict__ fe, uint64_t n_lines,
+                                       const uint64_t *__restrict__ first, uint64_t w0,
+                                       TokenLds &sm, TextReader &rd, FieldEnds &fends) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t f0 = first[blockIdx.x], f1 = first[blockIdx.x + 1];
+  for (uint32_t i = threadIdx.x; i < kParseWin / 32 + 1; i += kBlock)
+    sm.end_bits[i] = 0;
+  rd = TextReader{txt, sm.win, w0, stage_window(txt, n_bytes, w0, sm.win)};  // syncs
+  fends = FieldEnds{sm.end_bits, w0, rd.wn, fb, fe, n_lines};
   FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
-  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0);
+  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0, f0 ? f0 - 1 : 0,
+          f1 > f0 ? f1 : f0 + 1 < n_lines ? f0 + 1 : n_lines);
+  uint64_t marked = ~0ull;  // the last field whose end this thread marked
   for (uint32_t j = 0; j < kParseItems; ++j) {
     const uint32_t r = j * kBlock + threadIdx.x;
     const uint64_t p = w0 + r;
-    if (p >= n_bytes)
-      break;
-    fc.advance(p);
-    if (p < fc.b || p >= fc.e)
-      continue;
+    bool st = false;
+    if (p < n_bytes) {
+      fc.advance(p);
+      if (p >= fc.b && p < fc.e) {
+        const uint8_t c = rd(p);
+        st = kind == kTextP ? c != ',' && (p == fc.b || rd(p - 1) == ',')
+                            : c == '<' || c == '>';
+        if (fc.l != marked && fc.e - w0 <= rd.wn) {
+          const uint64_t o = fc.e - w0;
+          atomicOr(&sm.end_bits[o >> 5], 1u << (o & 31));
+          marked = fc.l;
+        }
+      }
+    }
+    const uint64_t m = __ballot(st);
+    if (lane == 0) {
+      const uint32_t w = (j * kBlock + wid * kWave) / 32;
+      sm.start_bits[w] = (uint32_t)m;
+      sm.start_bits[w + 1] = (uint32_t)(m >> 32);
+    }
+  }
+  __syncthreads();
+  constexpr uint32_t kWords = kParseTile / 32;
+  static_assert(kWords <= kBlock, "one bitmap word per thread");
+  const uint32_t word = threadIdx.x < kWords ? sm.start_bits[threadIdx.x] : 0u;
+  uint32_t total;
+  uint32_t at = block_exclusive_scan<kBlock>((uint32_t)__popc(word), sm.scan, total);
+  for (uint32_t bits = word; bits; bits &= bits - 1)
+    sm.list[at++] = (uint16_t)(threadIdx.x * 32 + __builtin_ctz(bits));
+  __syncthreads();
+  return total;
+}
+
+// Kept steps per tile of text bytes, and for every field that starts inside the
+// tile the kept steps of the tile before its first byte (partial[l]).
+__global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t *__restrict__ txt,
+                                                        uint64_t n_bytes,
+                                                        const uint64_t *__restrict__ fb,
+                                                        const uint64_t *__restrict__ fe,
+                                                        uint64_t n_lines,
+                                                        const uint64_t *__restrict__ first,
+                                                        uint32_t *__restrict__ tile_count,
+                                                        uint32_t *__restrict__ partial,
+                                                        uint32_t *__restrict__ err) {
+  __shared__ TokenLds sm;
+  __shared__ uint32_t keep_bits[kParseTile / 32];  // kept-token starts, one bit per byte
+  const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
+  for (uint32_t i = threadIdx.x; i < kParseTile / 32; i += kBlock)
+    keep_bits[i] = 0;
+  TextReader rd;
+  FieldEnds fends;
+  const uint32_t n_tok =
+      tile_tokens(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
+    const uint32_t r = sm.list[i];
     int32_t id;
-    if (parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1) {
+    if (parse_token(kind, rd, fends, w0 + r, &id, err)) {
       ++c;
       atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
     }
   }
   uint32_t total;
-  block_exclusive_scan<kBlock>(c, lds, total);  // its barriers publish keep_bits
+  block_exclusive_scan<kBlock>(c, sm.scan, total);  // its barriers publish keep_bits
   if (threadIdx.x == 0)
     tile_count[blockIdx.x] = total;
-  for (uint64_t l = lrange[0] + threadIdx.x; l < lrange[1]; l += kBlock) {
+  // fields starting in the tile: first[t] .. first[t + 1] - 1
+  for (uint64_t l = first[blockIdx.x] + threadIdx.x; l < first[blockIdx.x + 1]; l += kBlock) {
     const uint32_t off = (uint32_t)(fb[l] - w0);  // < kParseTile
     uint32_t n = 0;
     for (uint32_t w = 0; w < off / 32; ++w)
@@ -427,53 +536,43 @@ __global__ void k_parse_line_pos(const uint64_t *__restrict__ fb, uint64_t n_lin
   lpos[l] = t < n_tiles ? tile_pos[t] + partial[l] : tile_pos[n_tiles];
 }
 
-// Steps in byte order: each thread's kept tokens are ranked inside the tile by a
-// block scan over the byte-strided ownership (bytes j * kBlock + t belong to
-// thread t), so the write pass orders them by byte with an LDS list.
+// Steps in byte order: tokens are parsed a lane each into LDS (the ID, or a
+// dropped mark, per list entry), then thread t writes the kept IDs of entries
+// [t * kParseItems, +kParseItems) at their rank in the tile (a block scan).
+constexpr int32_t kDropped = 0;  // no kept step has ID 0
 __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
                                                         const uint64_t *__restrict__ fb,
                                                         const uint64_t *__restrict__ fe,
                                                         uint64_t n_lines,
+                                                        const uint64_t *__restrict__ first,
                                                         const uint64_t *__restrict__ tile_pos,
                                                         int32_t *__restrict__ nodes,
                                                         uint32_t *__restrict__ err) {
-  __shared__ __align__(16) uint8_t win[kParseWin];
-  __shared__ uint32_t lds[kBlock / kWave + 1];
-  // Kept-token flags of the tile's bytes (one bit per byte) and the IDs by byte.
-  __shared__ uint32_t keep_bits[kParseTile / 32];
+  __shared__ TokenLds sm;
   __shared__ int32_t ids[kParseTile];
   const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
-  const TextReader rd{txt, win, w0, stage_window(txt, n_bytes, w0, win)};
-  for (uint32_t i = threadIdx.x; i < kParseTile / 32; i += kBlock)
-    keep_bits[i] = 0;
-  __syncthreads();
-  FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
-  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0);
-  for (uint32_t j = 0; j < kParseItems; ++j) {
-    const uint32_t r = j * kBlock + threadIdx.x;
-    const uint64_t p = w0 + r;
-    if (p >= n_bytes)
-      break;
-    fc.advance(p);
-    if (p < fc.b || p >= fc.e)
-      continue;
-    int32_t id;
-    if (parse_token(kind, rd, fc.b, p, fc.e, &id, err) == 1) {
-      ids[r] = id;
-      atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
-    }
+  TextReader rd;
+  FieldEnds fends;
+  const uint32_t n_tok =
+      tile_tokens(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
+  for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
+    const uint32_t r = sm.list[i];
+    int32_t id = kDropped;
+    if (!parse_token(kind, rd, fends, w0 + r, &id, err))
+      id = kDropped;
+    ids[i] = id;
   }
   __syncthreads();
-  // Compaction in byte order: thread t owns bytes [t * kParseItems, +kParseItems).
-  const uint32_t r0 = threadIdx.x * kParseItems;
-  const uint32_t bits = (keep_bits[r0 >> 5] >> (r0 & 31)) & ((1u << kParseItems) - 1);
+  const uint32_t e0 = threadIdx.x * kParseItems;
+  uint32_t mine = 0;
+  for (uint32_t j = 0; j < kParseItems && e0 + j < n_tok; ++j)
+    mine += ids[e0 + j] != kDropped;
   uint32_t total;
-  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>((uint32_t)__popc(bits), lds,
-                                                                   total);
-  for (uint32_t j = 0; j < kParseItems; ++j)
-    if ((bits >> j) & 1)
-      nodes[o++] = ids[r0 + j];
+  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>(mine, sm.scan, total);
+  for (uint32_t j = 0; j < kParseItems && e0 + j < n_tok; ++j)
+    if (ids[e0 + j] != kDropped)
+      nodes[o++] = ids[e0 + j];
 }
 
 uint32_t grid1(uint64_t n, uint32_t per) {
@@ -609,6 +708,7 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
   plan.fb = DBuf<uint64_t>(pool, std::max<uint64_t>(n_lines, 1));
   plan.fe = DBuf<uint64_t>(pool, std::max<uint64_t>(n_lines, 1));
   plan.tile_pos = DBuf<uint64_t>(pool, plan.n_tiles + 1);
+  plan.first = DBuf<uint64_t>(pool, plan.n_tiles + 1);
   DBuf<uint64_t> lpos(pool, n_lines + 1);
   DBuf<uint32_t> tile_cnt(pool, plan.n_tiles), err(pool, 1);
   DBuf<uint32_t> partial(pool, std::max<uint64_t>(n_lines, 1));
@@ -621,9 +721,12 @@ void parse_plan(Ctx &ctx, int kind, const uint8_t *d_txt, uint64_t n_bytes,
   }
   const bool any = n_bytes && n_lines;
   if (any) {
+    hipLaunchKernelGGL(k_parse_first, dim3(grid1(plan.n_tiles + 1, kBlock)), dim3(kBlock), 0,
+                       ctx.stream(), plan.fb.get(), n_lines, plan.n_tiles, plan.first.get());
+    GFZ_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_parse_count, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0, ctx.stream(),
                        kind, d_txt, n_bytes, plan.fb.get(), plan.fe.get(), n_lines,
-                       tile_cnt.get(), partial.get(), err.get());
+                       plan.first.get(), tile_cnt.get(), partial.get(), err.get());
     GFZ_CHECK_LAUNCH();
   }
   exclusive_scan(ctx, tile_cnt.get(), plan.tile_pos.get(), any ? plan.n_tiles : 0);
@@ -663,7 +766,8 @@ void parse_write(Ctx &ctx, const ParsePlan &plan, int32_t *d_nodes, uint64_t nod
     ctx.timed("parse", [&] {
       hipLaunchKernelGGL(k_parse_write, dim3((uint32_t)plan.n_tiles), dim3(kBlock), 0,
                          ctx.stream(), plan.kind, plan.txt, plan.n_bytes, plan.fb.get(),
-                         plan.fe.get(), plan.n_lines, plan.tile_pos.get(), d_nodes, err.get());
+                         plan.fe.get(), plan.n_lines, plan.first.get(), plan.tile_pos.get(),
+                         d_nodes, err.get());
     });
     GFZ_CHECK_LAUNCH();
   }

@@ -40,6 +40,7 @@ struct ParsePlan {
   uint64_t n_bytes = 0, n_lines = 0, n_tiles = 0, total = 0;
   DBuf<uint64_t> fb, fe;     // field begins and ends (n_lines each)
   DBuf<uint64_t> tile_pos;   // kept steps before each tile (n_tiles + 1)
+  DBuf<uint64_t> first;      // first field beginning in or after each tile (n_tiles + 1)
 };
 
 // Counts the steps of the step fields d_txt[field_begin[l] .. field_end[l]) (device
