@@ -575,10 +575,12 @@ def run_serve(args, ctx, cfg, tr, gfa_bytes, d_nodes) -> int:
 # Timed kernel groups and the kernel whose hardware counters stand for them.
 # Kernels of each timed group (short names, as in the counter summaries).
 GROUP_KERNELS = {
-    "pair_insert": ["pair_aggregate", "bucket_part_tiles", "bucket_part", "part2_sizes",
-                    "bucket_part2", "bucket_count", "pair_insert_staged", "pair_insert_direct"],
+    "pair_insert": ["count_hot", "pair_aggregate", "pair_aggregate_cold", "bucket_part_tiles",
+                    "bucket_part", "part2_sizes", "bucket_part2", "bucket_count",
+                    "pair_insert_staged", "pair_insert_direct"],
     "cand_tiles": ["hot_sample", "cand"], "emit": ["emit"], "collect_used": ["collect_used"],
-    "rule_sort": ["ss_splitters", "ss_count", "ss_place", "ss_local"],
+    "rule_sort": ["ss_splitters", "ss_count", "ss_place", "ss_local", "sort_hist_all",
+                  "onesweep"],
     "assign_ids": ["assign"], "remap": ["remap"], "delta": ["delta"],
 }
 

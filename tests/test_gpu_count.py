@@ -2,7 +2,10 @@
 buckets, two-level beyond), and the fall back to the HBM counting table when a
 bucket overflows (forced with a tiny repeated-key cap, also after a two-level
 partition has reused the aggregated tiles' memory), give the oracle's grammar bit
-for bit. The knobs are the ctx tuned values "count_buckets" and "count_rep_cap"."""
+for bit. So does tile aggregation in rounds of few keys with and without the known
+repeated pairs dropped first (their keys then enter as an extra tile), also when
+the round falls back to the table. The knobs are the ctx tuned values
+"count_buckets", "count_rep_cap" and "count_hot_drop"."""
 import sys
 
 import numpy as np
@@ -71,3 +74,17 @@ def test_walks_two_level(gpu_ctx):
         _check(gpu_ctx, tr, ref)
     finally:
         gpu_ctx.set_tuned("count_buckets", 0)
+
+
+@pytest.mark.parametrize("hot_drop", [0, 1, -1])
+@pytest.mark.parametrize("rep_cap", [0, 1])
+def test_known_hot_drop_gives_the_oracle_grammar(gpu_ctx, case, hot_drop, rep_cap):
+    tr, ref = case
+    say(f"count hot_drop={hot_drop} rep_cap={rep_cap}")
+    gpu_ctx.set_tuned("count_hot_drop", hot_drop)
+    gpu_ctx.set_tuned("count_rep_cap", rep_cap)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("count_hot_drop", -1)
+        gpu_ctx.set_tuned("count_rep_cap", 0)
