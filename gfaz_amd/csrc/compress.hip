@@ -170,15 +170,16 @@ constexpr uint32_t kStageCap = 512;
 // probe chain of single slots at load 1/2 reads 15-20 in the worst of a
 // wave's 512 lookups). The slot index is the key's provisional rule ID within
 // the round; vals[slot] becomes its final ID.
-// The lines form regions of 2^rbits lines, one per counting bucket (nbmask + 1
+// The lines form regions of rlines lines, one per counting bucket (nbmask + 1
 // of them; one region when the round was counted in the HBM table): a key's
-// first line is in its bucket's region, and its probe wraps within it, so one
-// block can build a bucket's region alone.
+// first line is in its bucket's region (picked by a multiply-shift of its hash,
+// so a region may hold any number of lines), and its probe wraps within it, so
+// one block can build a bucket's region alone.
 constexpr uint32_t kLine = 8;
 struct RuleTable {
   unsigned long long *keys;  // kFree = empty; 64-byte aligned
   uint32_t *vals;
-  uint32_t rbits;            // log2 of the lines per region
+  uint32_t rlines;           // lines per region
   uint32_t nbmask;           // regions - 1 (a power of two minus one)
   // Rounds where most pairs are no rule: one bit per hashed key of the table
   // (2^fbits bits, set as keys go in). A clear bit proves a key absent without
@@ -206,14 +207,23 @@ __device__ inline uint32_t bucket_of(uint64_t k, uint32_t mask) {
   return (uint32_t)(key_hash(k) >> 40) & mask;
 }
 
-__device__ inline uint64_t rule_line(const RuleTable &rt, uint64_t k) {
-  const uint64_t rmask = (1ull << rt.rbits) - 1;
-  return ((uint64_t)bucket_of(k, rt.nbmask) << rt.rbits) | (key_hash(k) & rmask);
+// A key's first line within its region (of `lines` lines).
+__device__ inline uint32_t region_line(uint64_t k, uint32_t lines) {
+  return (uint32_t)(((key_hash(k) & 0xffffffffull) * lines) >> 32);
 }
 
-__device__ inline uint64_t rule_next_line(const RuleTable &rt, uint64_t l) {
-  const uint64_t rmask = (1ull << rt.rbits) - 1;
-  return (l & ~rmask) | ((l + 1) & rmask);
+// A probe position: the region's first line and the line within the region.
+struct RuleProbe {
+  uint64_t base;
+  uint32_t off;
+};
+
+__device__ inline RuleProbe rule_line(const RuleTable &rt, uint64_t k) {
+  return RuleProbe{(uint64_t)bucket_of(k, rt.nbmask) * rt.rlines, region_line(k, rt.rlines)};
+}
+
+__device__ inline void rule_next_line(const RuleTable &rt, RuleProbe &p) {
+  p.off = p.off + 1 == rt.rlines ? 0u : p.off + 1;
 }
 
 __device__ inline void load_line(const unsigned long long *keys, uint64_t line,
@@ -233,7 +243,7 @@ __device__ inline void load_line(const unsigned long long *keys, uint64_t line,
 template <int N>
 __device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[N],
                                        const bool (&valid)[N], int64_t (&slot)[N]) {
-  uint64_t l[N];
+  RuleProbe l[N];
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
@@ -242,12 +252,12 @@ __device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[
     if (valid[j])
       pending |= 1u << j;
   }
-  for (uint64_t p = 0; pending && p < (1ull << rt.rbits); ++p) {
+  for (uint32_t p = 0; pending && p < rt.rlines; ++p) {
     uint64_t w[N][kLine];
 #pragma unroll
     for (int j = 0; j < N; ++j)
       if ((pending >> j) & 1)
-        load_line(rt.keys, l[j], w[j]);
+        load_line(rt.keys, l[j].base + l[j].off, w[j]);
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (!((pending >> j) & 1))
@@ -260,12 +270,12 @@ __device__ inline void rule_find_batch(const RuleTable &rt, const uint64_t (&k)[
         has_free |= w[j][q] == kFree;
       }
       if (hit >= 0) {
-        slot[j] = (int64_t)(l[j] * kLine + (uint32_t)hit);
+        slot[j] = (int64_t)((l[j].base + l[j].off) * kLine + (uint32_t)hit);
         pending &= ~(1u << j);
       } else if (has_free) {
         pending &= ~(1u << j);
       } else {
-        l[j] = rule_next_line(rt, l[j]);
+        rule_next_line(rt, l[j]);
       }
     }
   }
@@ -286,11 +296,11 @@ __device__ inline int64_t rule_find(const RuleTable &rt, uint64_t k) {
 // tried), never the reverse, so starting at the first slot seen free keeps
 // every key in the first line that had room.
 __device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
-  uint64_t l = rule_line(rt, k);
+  RuleProbe l = rule_line(rt, k);
   for (uint32_t p = 0; p < kMaxProbe; ++p) {
-    unsigned long long *line = rt.keys + l * kLine;
+    unsigned long long *line = rt.keys + (l.base + l.off) * kLine;
     uint64_t w[kLine];
-    load_line(rt.keys, l, w);
+    load_line(rt.keys, l.base + l.off, w);
     uint32_t j = 0;
     while (j < kLine && w[j] != kFree)
       ++j;
@@ -300,7 +310,7 @@ __device__ inline bool rule_insert(const RuleTable &rt, uint64_t k) {
       if (old == kFree || old == k)
         return true;
     }
-    l = rule_next_line(rt, l);
+    rule_next_line(rt, l);
   }
   return false;
 }
@@ -784,7 +794,8 @@ constexpr uint32_t kFineBuckets = 128;   // second-level buckets per coarse buck
 constexpr uint32_t kFineBits = 7;
 constexpr uint32_t kMaxBuckets = 32768;  // buckets of a two-level partition
 constexpr uint32_t kMaxParts = 2048;    // partition blocks (segments per bucket)
-constexpr uint32_t kL2Parts = 16;        // second-level blocks per coarse bucket
+constexpr uint32_t kMaxParts2 = 16384;  // partition blocks of a two-level round
+constexpr uint32_t kL2Parts = 64;        // second-level blocks per coarse bucket
 constexpr uint32_t kRepStride = kBktSlots / 2;  // repeated keys listed per bucket
 constexpr uint32_t kDedupSlots = 2048;   // per-block set of keys already known repeated
 static_assert((uint64_t)kMaxBuckets * kBktSlots < (1ull << 32),
@@ -1273,16 +1284,15 @@ __global__ __launch_bounds__(kBlock) void k_extract_staged(
 constexpr uint32_t kExtractItems = 16;
 
 // Bucket-counted rounds: bucket b's region of the rule table built by one block
-// in LDS (2^rt.rbits lines, at most kBktSlots slots) from the bucket's list,
+// in LDS (rt.rlines lines, at most kBktSlots slots) from the bucket's list,
 // then written out whole, free slots included. A key takes the first line of
 // its probe within the region that has a free slot, as rule_insert would place
-// it; the host sizes regions to at least twice the longest list, so every key
-// finds one.
+// it; the host sizes regions above the longest list, so every key finds one.
 __global__ __launch_bounds__(kBktThreads) void k_region_build(
     const unsigned long long *__restrict__ stage, uint64_t stride,
     const uint2 *__restrict__ block_counts, RuleTable rt, uint32_t *__restrict__ err) {
   __shared__ unsigned long long reg[kBktSlots];
-  const uint32_t lines = 1u << rt.rbits, slots = lines * kLine;
+  const uint32_t lines = rt.rlines, slots = lines * kLine;
   for (uint32_t i = threadIdx.x; i < slots; i += kBktThreads)
     reg[i] = kFree;
   __syncthreads();
@@ -1290,19 +1300,19 @@ __global__ __launch_bounds__(kBktThreads) void k_region_build(
   const uint64_t n = block_counts[b].y < stride ? block_counts[b].y : stride;
   for (uint64_t j = threadIdx.x; j < n; j += kBktThreads) {
     const unsigned long long k = stage[b * stride + j];
-    uint32_t l = (uint32_t)key_hash(k) & (lines - 1);
+    uint32_t l = region_line(k, lines);
     bool placed = false;
     for (uint32_t p = 0; p < lines && !placed; ++p) {
       for (uint32_t q = 0; q < kLine && !placed; ++q)
         placed = atomicCAS(&reg[l * kLine + q], (unsigned long long)kFree, k) == kFree;
-      l = (l + 1) & (lines - 1);
+      l = l + 1 == lines ? 0u : l + 1;
     }
     if (!placed)  // a region smaller than its list: never sized so by the host
       atomicExch(err, 3u);
     filter_add(rt, k);
   }
   __syncthreads();
-  unsigned long long *dst = rt.keys + (b << rt.rbits) * kLine;
+  unsigned long long *dst = rt.keys + b * rt.rlines * kLine;
   for (uint32_t i = threadIdx.x; i < slots; i += kBktThreads)
     dst[i] = reg[i];
 }
@@ -2522,9 +2532,16 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       // gathers kL2Parts long segments instead of kMaxParts short ones.
       // (the partition of aggregated tiles takes the known keys' extra tile too)
       const uint64_t n_src = mostly_distinct ? n_ins : n_agg;
-      const uint32_t per = (uint32_t)std::max<uint64_t>(1, ceil_div(n_src, kMaxParts));
-      const uint64_t n_parts = ceil_div(n_src, per);
       const bool two_level = nbk > kMaxBuckets1;
+      // Two-level rounds gather second-level regions, not the partition blocks'
+      // segments, so they use more partition blocks (kMaxParts2, or the knob
+      // "count_parts2"): on plain_10m 2048 -> 16384 blocks with 16 -> 64 regions
+      // per coarse bucket took the counting from 50.1 to 45.9 ms per compress.
+      const int tuned_parts = ctx.tuned("count_parts2");
+      const uint64_t max_parts =
+          !two_level ? kMaxParts : tuned_parts > 0 ? (uint64_t)tuned_parts : kMaxParts2;
+      const uint32_t per = (uint32_t)std::max<uint64_t>(1, ceil_div(n_src, max_parts));
+      const uint64_t n_parts = ceil_div(n_src, per);
       const uint64_t nb1 = two_level ? nbk >> kFineBits : nbk;
       DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
       DBuf<uint32_t> starts(pool, n_parts * (nb1 + 1));
@@ -2533,7 +2550,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       const Parts p1{part.get(), (uint64_t)per * kInsTile, nullptr, starts.get(),
                      (uint32_t)nb1, (uint32_t)n_parts, (uint32_t)(nbk - 1),
                      two_level ? kFineBits : 0u};
-      const uint64_t n_l2 = two_level ? nb1 * kL2Parts : 0;
+      // Second-level regions per coarse bucket ("count_l2parts", at most kWave:
+      // a counting block takes that many segments in its few-segment path).
+      const int tuned_l2 = ctx.tuned("count_l2parts");
+      const uint32_t l2parts =
+          tuned_l2 > 0 ? std::min<uint32_t>((uint32_t)tuned_l2, kWave) : kL2Parts;
+      const uint64_t n_l2 = two_level ? nb1 * l2parts : 0;
       DBuf<uint32_t> l2_sizes, l2_starts;
       DBuf<uint64_t> l2_base;
       DBuf<unsigned long long> part2;
@@ -2557,7 +2579,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         GFZ_CHECK_LAUNCH();
         if (two_level) {
           hipLaunchKernelGGL(k_part2_sizes, dim3((uint32_t)ceil_div(n_l2, kBlock)), dim3(kBlock),
-                             0, ctx.stream(), p1, kL2Parts, (uint32_t)nb1, l2_sizes.get());
+                             0, ctx.stream(), p1, l2parts, (uint32_t)nb1, l2_sizes.get());
           GFZ_CHECK_LAUNCH();
           exclusive_scan(ctx, l2_sizes.get(), l2_base.get(), n_l2);
           // The second level's regions hold the first level's entries: the
@@ -2567,7 +2589,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
             part2 = DBuf<unsigned long long>(pool, n_parts * per * kInsTile);
             e2 = part2.get();
           }
-          pt = Parts{e2, 0, l2_base.get(), l2_starts.get(), kFineBuckets, kL2Parts,
+          pt = Parts{e2, 0, l2_base.get(), l2_starts.get(), kFineBuckets, l2parts,
                      (uint32_t)(nbk - 1), 0u};
           hipLaunchKernelGGL(k_bucket_part2, dim3((uint32_t)n_l2), dim3(kBlock),
                              kFineBuckets * 4, ctx.stream(), p1, pt);
@@ -2639,22 +2661,29 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       break;  // no pair occurs twice: the reference stops here (compression_workflow.cpp:576)
 
     // K1b: rule table of the repeated keys (load <= 1/4, whole 64-byte lines);
-    // a bucket-counted round takes one region per bucket holding twice its
-    // longest list (load <= 1/2 in the fullest region, lower on average): a
-    // smaller table to write and scan outweighs the lookups' extra lines.
-    const uint64_t region_slots =
-        regions ? std::min<uint64_t>(kBktSlots,
-                                     next_pow2(std::max<uint64_t>(kLine, 2 * region_max)))
+    // a bucket-counted round takes one region per bucket of whole lines, 9/4 of
+    // its longest list ("rule_region_pct" gives another percentage above 100;
+    // load <= 4/9 in the fullest region, lower on average). Lookups of absent
+    // keys (candidate marking) want a sparse table, the arrays indexed by slot
+    // (values, used flags; emit and remap) a small one: on plain_10m 150 %,
+    // 200 %, 250 % and 400 % took 139.0, 134.2, 133.7 and 135.6 ms of kernels
+    // per compress, twice the longest list rounded up to a power of two 137.
+    const int region_pct = ctx.tuned("rule_region_pct");
+    const uint64_t region_lines =
+        regions ? std::min<uint64_t>(
+                      kBktSlots / kLine,
+                      std::max<uint64_t>(1, ceil_div(region_max * (region_pct > 100 ? region_pct
+                                                                                    : 225),
+                                                     100 * (uint64_t)kLine)))
                 : 0;
     const uint64_t rcap =
-        regions ? regions * region_slots
+        regions ? regions * region_lines * kLine
                 : next_pow2(std::max<uint64_t>(1024, 4 * (uint64_t)repeated));
     if ((uint64_t)next + rcap >= (uint64_t)INT32_MAX)
       throw Error(kErrOverflow, "rule-ID space exhausted (next id " + std::to_string(next) +
                                     ", rule table " + std::to_string(rcap) + ")");
     DBuf<unsigned long long> rkeys(pool, rcap);  // pool blocks are whole hipMalloc blocks
     DBuf<uint32_t> rvals(pool, rcap);
-    const uint64_t region_lines = regions ? region_slots / kLine : rcap / kLine;
     // The absence filter pays once a quarter of the fresh pairs are no rule (each
     // of the distinct - repeated keys is one such pair): about 16 bits per repeated
     // key, at most 2^27 bits (16 MB, cache-resident).
@@ -2667,7 +2696,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<uint32_t> filter;
     if (use_filter)
       filter = DBuf<uint32_t>(pool, (uint64_t(1) << fbits) / 32);
-    const RuleTable rt{rkeys.get(), rvals.get(), (uint32_t)__builtin_ctzll(region_lines),
+    const RuleTable rt{rkeys.get(), rvals.get(),
+                       (uint32_t)(regions ? region_lines : rcap / kLine),
                        regions ? (uint32_t)(regions - 1) : 0u, filter.get(), fbits};
     // Round scalars (read back once, after K4): [0] next length, [1] used keys.
     // One launch clears everything the round needs cleared: the rule table, the

@@ -4,8 +4,10 @@ bucket overflows (forced with a tiny repeated-key cap, also after a two-level
 partition has reused the aggregated tiles' memory), give the oracle's grammar bit
 for bit. So does tile aggregation in rounds of few keys with and without the known
 repeated pairs dropped first (their keys then enter as an extra tile), also when
-the round falls back to the table. The knobs are the ctx tuned values
-"count_buckets", "count_rep_cap" and "count_hot_drop"."""
+the round falls back to the table, and with rule-table regions from barely above
+their longest list (long probe chains that wrap within a region) to four times it.
+The knobs are the ctx tuned values "count_buckets", "count_rep_cap",
+"count_hot_drop" and "rule_region_pct"."""
 import sys
 
 import numpy as np
@@ -88,3 +90,14 @@ def test_known_hot_drop_gives_the_oracle_grammar(gpu_ctx, case, hot_drop, rep_ca
     finally:
         gpu_ctx.set_tuned("count_hot_drop", -1)
         gpu_ctx.set_tuned("count_rep_cap", 0)
+
+
+@pytest.mark.parametrize("pct", [101, 400])
+def test_rule_region_sizes_give_the_oracle_grammar(gpu_ctx, case, pct):
+    tr, ref = case
+    say(f"rule regions {pct}% of the longest list")
+    gpu_ctx.set_tuned("rule_region_pct", pct)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("rule_region_pct", -1)
