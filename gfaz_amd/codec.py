@@ -110,6 +110,7 @@ _SIGNATURES = {
     "gfz_ctx_set_autotune": ([_vp, ctypes.c_int], ctypes.c_int),
     "gfz_ctx_tuned": ([_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "gfz_ctx_set_tuned": ([_vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "gfz_ctx_graph_stats": ([_vp, ctypes.POINTER(_u64)], ctypes.c_int),
     "gfz_autotune_compress": ([_vp, _vp, _u64, _vp, _u64, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)],
                               ctypes.c_int),
@@ -304,6 +305,13 @@ class Context:
 
     def set_tuned(self, key: str, value: int) -> None:
         _check(lib().gfz_ctx_set_tuned(self.handle, key.encode(), int(value)))
+
+    def graph_stats(self) -> dict:
+        """Round graphs of compression (gfz_ctx_graph_stats): launches, graphs
+        instantiated and graphs updated in place over the ctx's life."""
+        out = (_u64 * 3)()
+        _check(lib().gfz_ctx_graph_stats(self.handle, out))
+        return {"launches": int(out[0]), "instantiated": int(out[1]), "updated": int(out[2])}
 
     def autotune_compress(self, nodes, lengths, rounds: int = 8, threshold: int = 2,
                           delta_round: int = 1) -> float:

@@ -207,6 +207,16 @@ int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value) {
   return GFZ_OK;
 }
 
+int gfz_ctx_graph_stats(gfz_ctx *ctx, uint64_t out[3]) {
+  if (!ctx || !out)
+    return fail(GFZ_E_INVALID, "null argument");
+  const gfz::Ctx::GraphStats &g = ctx->ctx->graph_stats();
+  out[0] = g.launches;
+  out[1] = g.instantiated;
+  out[2] = g.updated;
+  return GFZ_OK;
+}
+
 
 int gfz_autotune_compress(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
                           const uint32_t *lengths, uint64_t n_trav, int rounds, int threshold,

@@ -2410,9 +2410,22 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   // above it can repeat (fresh_pair); 0 in the first round.
   uint32_t fresh = 0;
 
+  // Graph mode (tuned "round_graphs" = 1): each round's three stretches of
+  // launches between host steps (counting; rule table to used keys; sort to
+  // remap) run as HIP graphs, kept per round and updated in place on later
+  // calls (Ctx::segment).
+  struct GraphMode {
+    Ctx &c;
+    bool prev;
+    GraphMode(Ctx &ctx_, bool on) : c(ctx_), prev(ctx_.graphs()) { c.set_graphs(on); }
+    ~GraphMode() { c.set_graphs(prev); }
+  } graph_mode(ctx, ctx.tuned("round_graphs") > 0);
   for (int round = 0; round < p.rounds && L >= 2; ++round) {
     RoundInfo info;
     info.length_in = L;
+    auto seg_key = [&](const char *part) {
+      return std::string("compress/") + part + "/" + std::to_string(round);
+    };
     // Pairs that can be fresh: each of the last round's picks left one new
     // symbol, and a symbol starts at most two pairs (all pairs in the first round).
     const uint64_t elig = round == 0 ? L : std::min<uint64_t>(L, 2 * (prev_len - L));
@@ -2546,7 +2559,6 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       DBuf<unsigned long long> part(pool, n_parts * per * kInsTile);
       DBuf<uint32_t> starts(pool, n_parts * (nb1 + 1));
       stage = DBuf<unsigned long long>(pool, nbk * kRepStride);
-      fill_runs(ctx, {{sums.get(), 9, 0, 8}});
       const Parts p1{part.get(), (uint64_t)per * kInsTile, nullptr, starts.get(),
                      (uint32_t)nb1, (uint32_t)n_parts, (uint32_t)(nbk - 1),
                      two_level ? kFineBits : 0u};
@@ -2565,6 +2577,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
         l2_base = DBuf<uint64_t>(pool, n_l2 + 1);
         l2_starts = DBuf<uint32_t>(pool, n_l2 * (kFineBuckets + 1));
       }
+      ctx.segment(seg_key("count"), [&] {
+      fill_runs(ctx, {{sums.get(), 9, 0, 8}});
       ctx.timed("pair_insert", [&] {
         if (mostly_distinct) {
           hipLaunchKernelGGL(k_bucket_part, dim3((uint32_t)n_parts), dim3(kBlock), nb1 * 4,
@@ -2599,6 +2613,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                            ctx.stream(), pt, stage.get(), bcounts.get(), sums.get(),
                            reinterpret_cast<uint32_t *>(sums.get() + 2), rep_cap);
         GFZ_CHECK_LAUNCH();
+      });
       });
       ctx.add_bytes("pair_insert", 4.0 * L);
       const uint64_t *hc = ctx.fetch({{sums.get(), 9, 0}});
@@ -2706,6 +2721,26 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<unsigned long long> scal(pool, 2);
     DBuf<uint8_t> used(pool, rcap);
     const uint64_t fwords = use_filter ? (uint64_t(1) << fbits) / 32 : 0;
+    DBuf<HotEntry> hot(pool, kHotSlots);
+    DBuf<HotPair> top(pool, kHotTop);
+    // K2 + K2b: candidates, tile carries, next length.
+    DBuf<uint64_t> cand(pool, n_words);
+    DBuf<uint32_t> tile_fn(pool, n_tiles);
+    DBuf<uint64_t> tile_info(pool, n_tiles);
+    DBuf<uint32_t> slots(pool, L);
+    // Up to kTileScanSingle tiles one block scans them all (one launch); beyond,
+    // blocks of kTileScanThreads tiles while that makes at most kTileScanThreads
+    // blocks (then each thread takes a chunk), in three launches.
+    const bool single = n_tiles <= kTileScanSingle;
+    const uint64_t n_blocks =
+        single ? 1 : std::min<uint64_t>(ceil_div(n_tiles, kTileScanThreads), kTileScanThreads);
+    const uint64_t per_block = ceil_div(n_tiles, n_blocks);
+    DBuf<FnWide> block_fn(pool, n_blocks);
+    uint64_t *d_total = reinterpret_cast<uint64_t *>(scal.get());
+    DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);  // K4's used keys
+    // Everything from the rule table to the used keys runs without a host step
+    // in between: one graph segment.
+    ctx.segment(seg_key("tail"), [&] {
     if (regions)  // the regions are written whole: the table needs no clearing
       fill_runs(ctx, {{scal.get(), 2, 0, 8},
                       {used.get(), rcap, 0, 1},
@@ -2732,38 +2767,17 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                            d_err.get());
     });
     GFZ_CHECK_LAUNCH();
-    stage.reset();
-    ctab.reset();
-
-    DBuf<HotEntry> hot(pool, kHotSlots);
-    DBuf<HotPair> top(pool, kHotTop);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_hot_sample, dim3(1), dim3(kHotThreads), 0, ctx.stream(), cur,
                          heads.get(), L, span, fresh, rt, hot.get(), top.get());
     });
     GFZ_CHECK_LAUNCH();
-
-    // K2 + K2b: candidates, tile carries, next length.
-    DBuf<uint64_t> cand(pool, n_words);
-    DBuf<uint32_t> tile_fn(pool, n_tiles);
-    DBuf<uint64_t> tile_info(pool, n_tiles);
-    DBuf<uint32_t> slots(pool, L);
     ctx.timed("cand_tiles", [&] {
       hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
                          heads.get(), L, span, fresh, rt, hot.get(), top.get(), cand.get(), n_words,
                          tile_fn.get(), slots.get());
     });
     GFZ_CHECK_LAUNCH();
-    ctx.add_bytes("cand_tiles", 4.0 * L);
-    // Up to kTileScanSingle tiles one block scans them all (one launch); beyond,
-    // blocks of kTileScanThreads tiles while that makes at most kTileScanThreads
-    // blocks (then each thread takes a chunk), in three launches.
-    const bool single = n_tiles <= kTileScanSingle;
-    const uint64_t n_blocks =
-        single ? 1 : std::min<uint64_t>(ceil_div(n_tiles, kTileScanThreads), kTileScanThreads);
-    const uint64_t per_block = ceil_div(n_tiles, n_blocks);
-    DBuf<FnWide> block_fn(pool, n_blocks);
-    uint64_t *d_total = reinterpret_cast<uint64_t *>(scal.get());
     ctx.timed("tile_scan", [&] {
       if (single) {
         hipLaunchKernelGGL(k_tile_apply, dim3(1), dim3(kTileScanThreads), 0, ctx.stream(),
@@ -2790,13 +2804,16 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
 
     // K4: used keys.
-    DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);
     ctx.timed("collect_used", [&] {
       hipLaunchKernelGGL(k_collect_used, dim3((uint32_t)ceil_div(rcap, kCollectTile)),
                          dim3(kBlock), 0, ctx.stream(), rkeys.get(), used.get(), rcap, ukeys.get(),
                          scal.get());
     });
     GFZ_CHECK_LAUNCH();
+    });
+    ctx.add_bytes("cand_tiles", 4.0 * L);
+    stage.reset();
+    ctab.reset();
     cand.reset();
     tile_fn.reset();
     tile_info.reset();
@@ -2810,9 +2827,6 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     if (R == 0)
       break;  // cannot happen when some pair repeats (compression_workflow.cpp:611)
 
-    // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
-    sort_unique_u64(ctx, ukeys.get(), utmp.get(), R, d_err.get(), 0,
-                    64 - (uint32_t)__builtin_clzll(span * span));  // keys < span^2
     if (out.n_rules + R > rules_cap) {
       // Room for four times the first round's repeated keys up front: later
       // rounds add fewer rules, so the columns rarely have to grow.
@@ -2829,6 +2843,10 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       rs = std::move(ns);
       rules_cap = ncap;
     }
+    ctx.segment(seg_key("ids"), [&] {
+    // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
+    sort_unique_u64(ctx, ukeys.get(), utmp.get(), R, d_err.get(), 0,
+                    64 - (uint32_t)__builtin_clzll(span * span));  // keys < span^2
     ctx.timed("assign_ids", [&] {
       hipLaunchKernelGGL(k_assign, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
                          ukeys.get(), R, span, rt, next, rf.get() + out.n_rules,
@@ -2841,6 +2859,7 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                          ctx.stream(), dst, L_next, rvals.get(), next, top.get());
     });
     GFZ_CHECK_LAUNCH();
+    });
     ctx.add_bytes("remap", 8.0 * L_next);
 
     fresh = next;

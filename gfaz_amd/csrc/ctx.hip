@@ -36,10 +36,9 @@ void *Pool::get(size_t bytes) {
     return p;
   }
   if (limit_ && live_ + cached_ + bytes > limit_) {
-    for (auto &kv : free_)
-      (void)hipFree(kv.second), size_of_.erase(kv.second);
-    free_.clear();
-    cached_ = 0;
+    if (keep_cache_)
+      throw Error(kErrOutOfMemory, "device pool limit reached during graph capture");
+    release_cache();
     if (live_ + bytes > limit_)
       throw Error(kErrOutOfMemory, "device pool limit reached (" + std::to_string(live_ + bytes) +
                                        " > " + std::to_string(limit_) + " bytes)");
@@ -48,10 +47,10 @@ void *Pool::get(size_t bytes) {
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    for (auto &kv : free_)
-      (void)hipFree(kv.second), size_of_.erase(kv.second);
-    free_.clear();
-    cached_ = 0;
+    if (keep_cache_)
+      throw Error(kErrOutOfMemory,
+                  "hipMalloc of " + std::to_string(bytes) + " bytes failed during graph capture");
+    release_cache();
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -75,14 +74,18 @@ void Pool::put(void *p) {
   free_.emplace(it->second, p);
 }
 
-void Pool::trim() {
-  std::lock_guard<std::mutex> lk(mu_);
+void Pool::release_cache() {
   for (auto &kv : free_) {
     (void)hipFree(kv.second);
     size_of_.erase(kv.second);
   }
   free_.clear();
   cached_ = 0;
+}
+
+void Pool::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  release_cache();
 }
 
 Ctx::Ctx(int device) : device_(device) {
@@ -99,6 +102,8 @@ Ctx::~Ctx() {
   (void)hipSetDevice(device_);
   if (stream_)
     (void)hipStreamSynchronize(stream_);
+  for (auto &kv : execs_)
+    (void)hipGraphExecDestroy(kv.second);
   for (auto &p : pending_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -116,6 +121,62 @@ Ctx::~Ctx() {
   pool_.trim();
   if (stream_)
     (void)hipStreamDestroy(stream_);
+}
+
+// Stream capture in relaxed mode: the pool may still allocate (hipMalloc is not
+// stream work), but keeps its cached blocks, which captured work may name.
+void Ctx::capture_begin() {
+  GFZ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+  capturing_ = true;
+  pool_.set_keep_cache(true);
+}
+
+void Ctx::capture_abort() {
+  hipGraph_t g = nullptr;
+  (void)hipStreamEndCapture(stream_, &g);
+  if (g)
+    (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  capturing_ = false;
+  pool_.set_keep_cache(false);
+}
+
+void Ctx::capture_end_launch(const std::string &key) {
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream_, &g);
+  capturing_ = false;
+  pool_.set_keep_cache(false);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    throw Error(kErrHip, std::string("graph capture failed: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t &x = execs_[key];
+  bool ok = false;
+  if (x) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    ok = hipGraphExecUpdate(x, g, &err_node, &res) == hipSuccess &&
+         res == hipGraphExecUpdateSuccess;
+    if (ok) {
+      ++graph_stats_.updated;
+    } else {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+  }
+  if (!ok) {
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      x = nullptr;
+      (void)hipGraphDestroy(g);
+      throw Error(kErrHip, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+    }
+    ++graph_stats_.instantiated;
+  }
+  (void)hipGraphDestroy(g);
+  GFZ_HIP(hipGraphLaunch(x, stream_));
+  ++graph_stats_.launches;
 }
 
 hipEvent_t Ctx::event() {
@@ -170,6 +231,8 @@ __global__ __launch_bounds__(kWave) void k_fetch(FetchArgs a, uint64_t *mail, ui
 }  // namespace
 
 const uint64_t *Ctx::fetch(std::initializer_list<FetchRun> runs) {
+  if (capturing_)  // the captured work has not run: its mailbox would never fill
+    throw Error(kErrInvalid, "host readback inside a graph segment (internal error)");
   if (!mail_) {
     void *p = nullptr;
     GFZ_HIP(hipHostMalloc(&p, (kMailWords + 1) * 8, hipHostMallocMapped | hipHostMallocCoherent));

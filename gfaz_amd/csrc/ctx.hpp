@@ -27,11 +27,17 @@ class Pool {
   // Optional cap on live + cached bytes (0 = none); get() throws kErrOutOfMemory
   // past it after trimming the cache.
   void set_limit(size_t bytes) { limit_ = bytes; }
+  // While set, get() never frees cached blocks to make room (it throws
+  // kErrOutOfMemory instead): during stream capture a cached block may still be
+  // named by captured work that has not run yet.
+  void set_keep_cache(bool on) { keep_cache_ = on; }
 
  private:
+  void release_cache();  // caller holds mu_
   std::multimap<size_t, void *> free_;
   std::unordered_map<void *, size_t> size_of_;
   size_t cached_ = 0, live_ = 0, limit_ = 0;
+  bool keep_cache_ = false;
   std::mutex mu_;
 };
 
@@ -105,7 +111,7 @@ class Ctx {
   void set_profile_groups(std::set<std::string> groups) { only_ = std::move(groups); }
   template <class F>
   void timed(const char *name, F &&launch) {
-    if (!profiling_ || (!only_.empty() && !only_.count(name))) {
+    if (!profiling_ || capturing_ || (!only_.empty() && !only_.count(name))) {
       launch();
       return;
     }
@@ -167,6 +173,33 @@ class Ctx {
     return c;
   }
   const std::map<std::string, int> &tuned_all() const { return tuned_; }
+  // HIP graphs of launch sequences. With graphs on, segment(key, fn) captures
+  // the work fn puts on the context stream (no host synchronisation may happen
+  // inside fn), updates the executable graph kept under `key` to it (or
+  // instantiates one when the sequence's shape changed) and launches it; with
+  // graphs off it just runs fn. Timed groups are not timed while capturing.
+  void set_graphs(bool on) { graphs_ = on; }
+  bool graphs() const { return graphs_; }
+  template <class F>
+  void segment(const std::string &key, F &&fn) {
+    if (!graphs_) {
+      fn();
+      return;
+    }
+    capture_begin();
+    try {
+      fn();
+    } catch (...) {
+      capture_abort();
+      throw;
+    }
+    capture_end_launch(key);
+  }
+  struct GraphStats {
+    uint64_t launches = 0, instantiated = 0, updated = 0;
+  };
+  const GraphStats &graph_stats() const { return graph_stats_; }
+
   // Elapsed milliseconds of `launch` on the context stream (synchronises).
   template <class F>
   float time_ms(F &&launch) {
@@ -184,6 +217,9 @@ class Ctx {
 
  private:
   hipEvent_t event();
+  void capture_begin();
+  void capture_abort();
+  void capture_end_launch(const std::string &key);
   struct Pending {
     std::string name;
     hipEvent_t a, b;
@@ -208,6 +244,9 @@ class Ctx {
   size_t pinned_bytes_ = 0;
   void *staging_ = nullptr;
   size_t staging_bytes_ = 0;
+  bool graphs_ = false, capturing_ = false;
+  std::map<std::string, hipGraphExec_t> execs_;
+  GraphStats graph_stats_;
 };
 
 // RAII: make `ctx`'s device current for the duration of a call.

@@ -87,6 +87,15 @@ int gfz_ctx_set_tuned(gfz_ctx *ctx, const char *key, int value);
 int gfz_autotune_compress(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
                           const uint32_t *lengths, uint64_t n_trav, int rounds, int threshold,
                           int delta_round, uint32_t flags, double *best_ms);
+/* HIP graphs of the induction rounds (no counterpart in the reference, whose
+ * rolling scheduler launches every kernel from the host:
+ * src/gpu/compression/path_compression_gpu_rolling.cu:325-625). With the tuned
+ * key "round_graphs" = 1 each round's three stretches of launches between host
+ * readbacks (counting; rule table to used keys; sort to remap) are captured and
+ * launched as graphs, kept per ctx and round and updated in place by later
+ * calls. out[0] = graph launches, out[1] = graphs instantiated, out[2] = graphs
+ * updated in place, over the ctx's life. */
+int gfz_ctx_graph_stats(gfz_ctx *ctx, uint64_t out[3]);
 /* The ctx's tuned table as JSON {"arch": ..., "cus": N, "tuned": {key: value}},
  * and back: a file written for another GPU architecture or CU count is refused
  * (GFZ_E_INVALID) and leaves the table as it was. */
