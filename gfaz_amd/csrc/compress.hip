@@ -40,6 +40,58 @@ __device__ inline bool head_at(const uint32_t *heads, uint64_t i) {
   return (heads[i >> 5] >> (i & 31)) & 1u;
 }
 
+// ---- cross-traversal tile order ------------------------------------------------
+// Which kRtTile tile of the flat stream a block takes. Traversal h owns the tiles
+// whose first element lies in [start[h], start[h + 1]) (start[n] = L), so every
+// tile has exactly one owner. Blocks take column c (the c-th tile of every
+// traversal) for all traversals together, and the columns are dealt to the XCDs
+// in turn (the dispatcher hands consecutive blocks to the eight XCDs
+// round-robin): one XCD then marks the same stretch of every haplotype at once,
+// so the rule-table lines of the pairs those haplotypes share can still be in its
+// L2 when the next haplotype looks them up. Most keys of the later rounds occur
+// two or three times, so the gain is small (on plain_10m about 1 ms of the
+// 29 ms of candidate marking per compress, in the first two rounds).
+// start == nullptr keeps the launch order.
+constexpr uint32_t kXcdCount = 8;
+constexpr uint64_t kNoTile = ~0ull;
+struct TileOrder {
+  const uint64_t *start;  // first element of each non-empty traversal
+  uint64_t L;
+  uint32_t n;             // traversals
+  uint32_t cols;          // most tiles any traversal owns
+};
+
+__device__ inline uint64_t order_tile(const TileOrder &o) {
+  const uint64_t b = blockIdx.x;
+  if (!o.start)
+    return b;
+  const uint64_t i = b / kXcdCount;
+  const uint64_t c = (i / o.n) * kXcdCount + b % kXcdCount;
+  if (c >= o.cols)
+    return kNoTile;
+  const uint32_t h = (uint32_t)(i % o.n);
+  const uint64_t e = h + 1 < o.n ? o.start[h + 1] : o.L;
+  const uint64_t t = (o.start[h] + kRtTile - 1) / kRtTile + c;
+  return t * kRtTile < e ? t : kNoTile;
+}
+
+// Blocks of a launch in that order over traversals of at most max_len elements.
+inline uint64_t order_grid(uint32_t n, uint64_t max_len, uint32_t &cols) {
+  cols = (uint32_t)((max_len + kRtTile - 1) / kRtTile);
+  return (uint64_t)kXcdCount * n * ((cols + kXcdCount - 1) / kXcdCount);
+}
+
+// *longest = the longest traversal of the next stream (length *L_next).
+__global__ void k_trav_longest(const uint64_t *__restrict__ start, uint32_t n,
+                               const uint64_t *__restrict__ L_next,
+                               unsigned long long *__restrict__ longest) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < n) {
+    const uint64_t e = h + 1 < n ? start[h + 1] : *L_next;
+    atomicMax(longest, (unsigned long long)(e - start[h]));
+  }
+}
+
 // Keys are compacted before they reach any table: every symbol of a round has
 // |v| < next = span / 2, so ck = (a + next) * span + (b >= 0 ? b : b + span)
 // keeps the signed packed-key order of (a, b) (a signed, then b as unsigned 32
@@ -1815,19 +1867,28 @@ __global__ __launch_bounds__(kBlock) void k_pair_aggregate_cold(
 
 // Candidate bits, each tile's pick function, and the slot words of the cold
 // candidates (the rule slot of the pair starting there, kSlotRev when the pair
-// is the reverse of the rule's canonical key). Hot pairs need none: emit tests
-// its picks against the hot pairs again, from the raw pair.
+// is the reverse of the rule's canonical key), with a bit per cold candidate.
+// Each wave's slot words are packed, in position order, at the front of its
+// kWavePos entries of the tile's kRtTile (a wave takes candidate words w with
+// w % kWaves == wave), so the k-th cold candidate of a wave (counted on its
+// cold bits) has word k there: emit then reads the few words of its cold picks
+// from a short dense run instead of one line each, and no wave waits for
+// another to place its words. Hot pairs need no word: emit tests its other
+// picks against the hot pairs again, from the raw pair.
+constexpr uint32_t kWaves = kBlock / kWave;
+constexpr uint32_t kWavePos = kRtItems * kWave;  // tile positions of one wave
 template <bool kInterior>
 __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *__restrict__ heads,
                                  uint64_t L, uint64_t span, uint32_t fresh, const RuleTable &rt,
                                  const HotKeys &hk, HotLds &sh,
                                  unsigned long long (*list)[kColdCap], uint64_t *wbits,
-                                 uint64_t *__restrict__ cand, uint64_t n_words,
-                                 uint32_t *__restrict__ slots) {
+                                 uint64_t *__restrict__ cand, uint64_t *__restrict__ coldw,
+                                 uint64_t n_words, uint32_t *__restrict__ slots, uint64_t tile) {
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
-  uint32_t *__restrict__ st = slots + base;  // this tile's slot words
+  const uint64_t base = tile * kRtTile;
+  uint32_t *__restrict__ st = slots + base + wid * kWavePos;  // this wave's slot words
   const uint64_t below = lanemask_lt();
+  uint32_t cold_run = 0;  // this wave's cold candidates so far
   for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
     // Validity is a per-lane bit set (bool arrays live as scalar lane masks).
     int32_t xa[kPassItems], xb[kPassItems];
@@ -1862,20 +1923,26 @@ __device__ inline void cand_tile(const int32_t *__restrict__ x, const uint32_t *
     __syncthreads();
 #pragma unroll
     for (uint32_t b = 0; b < kPassItems; ++b) {
-      bool f = (hit >> b) & 1;
+      bool f = (hit >> b) & 1, c = false;
+      unsigned long long sl = kFree;
       if ((listed >> b) & 1) {
-        const unsigned long long sl = list[wid][at[b]];
-        f = sl != kFree;
-        if (f)
-          st[(j0 + b) * kBlock + threadIdx.x] = (uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u);
+        sl = list[wid][at[b]];
+        f = c = sl != kFree;
       }
       const uint64_t word = __ballot(f);
+      const uint64_t cword = __ballot(c);
+      if (c)
+        st[cold_run + (uint32_t)__popcll(cword & below)] =
+            (uint32_t)sl | ((neg >> b) & 1 ? kSlotRev : 0u);
+      cold_run += (uint32_t)__popcll(cword);
       if (lane == 0) {
-        const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+        const uint32_t w = (j0 + b) * kWaves + wid;
         wbits[w] = word;
         const uint64_t gw = (base >> 6) + w;
-        if (gw < n_words)
+        if (gw < n_words) {
           cand[gw] = word;
+          coldw[gw] = cword;
+        }
       }
     }
     __syncthreads();  // the list is refilled by the next pass
@@ -1888,8 +1955,12 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
                                                  const HotEntry *__restrict__ hot,
                                                  const HotPair *__restrict__ top,
                                                  uint64_t *__restrict__ cand,
+                                                 uint64_t *__restrict__ coldw,
                                                  uint64_t n_words, uint32_t *__restrict__ tile_fn,
-                                                 uint32_t *__restrict__ slots) {
+                                                 uint32_t *__restrict__ slots, TileOrder order) {
+  const uint64_t tile = order_tile(order);
+  if (tile == kNoTile)
+    return;
   __shared__ uint64_t wbits[kRtWords];
   __shared__ HotLds sh;
   __shared__ unsigned long long list[kBlock / kWave][kColdCap];
@@ -1899,14 +1970,16 @@ __global__ __launch_bounds__(kBlock) void k_cand(const int32_t *__restrict__ x,
   hot_load(sh, hot);
   __syncthreads();
   // Every tile but the last is interior (its last pair ends inside the stream).
-  if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
-    cand_tile<true>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, n_words, slots);
+  if ((tile + 1) * kRtTile < L)
+    cand_tile<true>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, coldw, n_words,
+                    slots, tile);
   else
-    cand_tile<false>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, n_words, slots);
+    cand_tile<false>(x, heads, L, span, fresh, rt, hk, sh, list, wbits, cand, coldw, n_words,
+                     slots, tile);
   if (wid == 0) {
     const uint32_t f = wave_fn_inclusive(word_fn(wbits[lane]));
     if (lane == kWave - 1)
-      tile_fn[blockIdx.x] = f;
+      tile_fn[tile] = f;
   }
 }
 
@@ -2039,17 +2112,64 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_apply(
 }
 
 // ---- K3: emit the next round's stream ---------------------------------------
+// One wave, lane l: the picks and survivors of candidate word l of tile t
+// (p_tile = the pick carried into the tile). Survivors are the valid elements
+// whose predecessor was not picked.
+__device__ inline void word_survivors(const uint64_t *__restrict__ cand, uint64_t n_words,
+                                      uint64_t L, uint64_t t, uint32_t p_tile, uint64_t &picks,
+                                      uint64_t &surv) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t gw = t * kRtWords + lane;
+  const uint64_t c = gw < n_words ? cand[gw] : 0;
+  const uint32_t inc = wave_fn_inclusive(word_fn(c));
+  uint32_t ex = __shfl_up(inc, 1);
+  if (lane == 0)
+    ex = fn_identity();
+  const uint32_t p_in = fn_out(ex, p_tile);
+  picks = word_picks(c, p_in);
+  const uint64_t lo = t * kRtTile + (uint64_t)lane * 64;
+  const uint64_t nvalid = lo >= L ? 0 : (L - lo >= 64 ? 64 : L - lo);
+  const uint64_t vmask = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
+  surv = ~((picks << 1) | (uint64_t)p_in) & vmask;
+}
+
+// Each traversal's first element in the next stream: one wave per traversal
+// counts the survivors of its start's tile before it (heads always survive).
+__global__ __launch_bounds__(kBlock) void k_starts_next(
+    const uint64_t *__restrict__ start, uint32_t n, const uint64_t *__restrict__ cand,
+    uint64_t n_words, const uint64_t *__restrict__ tile_info, uint64_t L,
+    uint64_t *__restrict__ start_next) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (h >= n)
+    return;
+  const uint64_t p = start[h], t = p / kRtTile;
+  const uint64_t info = tile_info[t];
+  uint64_t picks, surv;
+  word_survivors(cand, n_words, L, t, (uint32_t)(info & 1), picks, surv);
+  const uint64_t o = p - t * kRtTile, lo = (uint64_t)lane * 64;
+  const uint64_t before = o <= lo ? 0 : (o - lo >= 64 ? ~0ull : ((1ull << (o - lo)) - 1));
+  uint32_t cnt = (uint32_t)__popcll(surv & before);
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1)
+    cnt += __shfl_xor(cnt, d);
+  if (lane == 0)
+    start_next[h] = (info >> 1) + cnt;
+}
+
 // Survivors are elements whose predecessor was not picked; a picked element
 // emits +/-(next + slot) (provisional, remapped after the sort). A pick's slot
 // and orientation come from the hot pairs (tested raw, as in K2) or, for a cold
-// pick, from its slot word. Heads always survive and carry their bit to the
-// output position.
+// pick, from its slot word (its wave's k-th for the wave's k-th cold
+// candidate). Heads always survive and carry their bit to the output position.
+constexpr uint32_t kEmitPass = 8;  // items per pass of emit (their loads in flight together)
 template <bool kInterior>
 __device__ inline void emit_tile(const int32_t *__restrict__ x,
                                  const uint32_t *__restrict__ heads, uint64_t L,
                                  const uint32_t *__restrict__ slots, uint64_t rcap,
                                  const HotKeys &hk, const uint64_t *spicks,
-                                 const uint64_t *ssurv, const uint32_t *soff, uint64_t off_tile,
+                                 const uint64_t *ssurv, const uint32_t *soff,
+                                 const uint64_t *scold, uint64_t off_tile,
                                  uint8_t *__restrict__ used, uint32_t next,
                                  int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
                                  uint32_t *__restrict__ err, uint32_t &hot_used) {
@@ -2059,13 +2179,18 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
   const int32_t *__restrict__ xt = x + base;
   const uint32_t *__restrict__ ht = heads + (base >> 5);
   const uint32_t *__restrict__ st = slots + base;
-  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kPassItems) {
-    // Every load of the pass first (interior tiles: every pair inside the
-    // stream; the last tile clamps its indexes into it).
-    int32_t xa[kPassItems], xb[kPassItems];
-    uint32_t hw[kPassItems];
+  // This wave's cold candidates so far, from its slot words' region on: the
+  // next cold candidate's slot word.
+  uint32_t cold_run = wid * kWavePos;
+  for (uint32_t j0 = 0; j0 < kRtItems; j0 += kEmitPass) {
+    // Every load of the pass first, the slot words of its cold picks included
+    // (interior tiles: every pair inside the stream; the last tile clamps its
+    // indexes into it): one wait per pass, not one per cold pick.
+    int32_t xa[kEmitPass], xb[kEmitPass];
+    uint32_t hw[kEmitPass], sw[kEmitPass];
+    uint32_t coldpick = 0;
 #pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
+    for (uint32_t b = 0; b < kEmitPass; ++b) {
       const uint32_t r = (j0 + b) * kBlock + threadIdx.x;
       if constexpr (kInterior) {
         xa[b] = xt[r];
@@ -2079,10 +2204,16 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
         xb[b] = x[ib];
         hw[b] = heads[ia >> 5];
       }
+      const uint32_t w = (j0 + b) * kWaves + wid;
+      const uint64_t cw = scold[w];
+      const bool cp = ((spicks[w] & cw) >> lane) & 1;
+      sw[b] = cp ? st[cold_run + (uint32_t)__popcll(cw & below)] : 0u;
+      coldpick |= cp ? 1u << b : 0u;
+      cold_run += (uint32_t)__popcll(cw);
     }
 #pragma unroll
-    for (uint32_t b = 0; b < kPassItems; ++b) {
-      const uint32_t w = (j0 + b) * (kBlock / kWave) + wid;
+    for (uint32_t b = 0; b < kEmitPass; ++b) {
+      const uint32_t w = (j0 + b) * kWaves + wid;
       const uint32_t r = w * 64 + lane;
       const uint64_t svm = ssurv[w];
       if ((kInterior || base + r < L) && ((svm >> lane) & 1)) {
@@ -2091,21 +2222,23 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
         if ((spicks[w] >> lane) & 1) {
           bool rev = false;
           uint32_t sl = 0;
-          if (hot_match(hk, xa[b], xb[b], rev, sl)) {
+          if ((coldpick >> b) & 1) {
+            sl = sw[b] & ~kSlotRev;
+            rev = (sw[b] & kSlotRev) != 0;
+            if (sl < rcap) {
+              used[sl] = 1;
+            } else {
+              atomicExch(err, 1u);  // a slot word always holds a table slot
+              sl = 0;
+            }
+          } else if (hot_match(hk, xa[b], xb[b], rev, sl)) {
 #pragma unroll
             for (uint32_t q = 0; q < kHotTop; ++q)
               if (hk.slot[q] == sl)
                 hot_used |= 1u << q;
           } else {
-            const uint32_t sw = st[r];
-            sl = sw & ~kSlotRev;
-            rev = (sw & kSlotRev) != 0;
-            if (sl < rcap) {
-              used[sl] = 1;
-            } else {
-              atomicExch(err, 1u);  // a picked cold pair always has its slot word
-              sl = 0;
-            }
+            atomicExch(err, 1u);  // a pick is a hot or a cold candidate
+            sl = 0;
           }
           o = (int32_t)(next + sl);
           if (rev)
@@ -2121,13 +2254,15 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
 
 __global__ __launch_bounds__(kBlock) void k_emit(
     const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
-    const uint64_t *__restrict__ cand, uint64_t n_words, const uint64_t *__restrict__ tile_info,
+    const uint64_t *__restrict__ cand, const uint64_t *__restrict__ coldw, uint64_t n_words,
+    const uint64_t *__restrict__ tile_info,
     const uint32_t *__restrict__ slots, uint64_t rcap, const HotPair *__restrict__ top,
     uint8_t *__restrict__ used, uint32_t next, int32_t *__restrict__ y,
     uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
+  __shared__ uint64_t scold[kRtWords];
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const uint64_t info = tile_info[blockIdx.x];
@@ -2137,17 +2272,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   hot_keys_load(hk, top);
   if (wid == 0) {
     const uint64_t gw = (base >> 6) + lane;
-    const uint64_t c = gw < n_words ? cand[gw] : 0;
-    const uint32_t inc = wave_fn_inclusive(word_fn(c));
-    uint32_t ex = __shfl_up(inc, 1);
-    if (lane == 0)
-      ex = fn_identity();
-    const uint32_t p_in = fn_out(ex, p_tile);
-    const uint64_t picks = word_picks(c, p_in);
-    const uint64_t lo = base + (uint64_t)lane * 64;
-    const uint64_t nvalid = lo >= L ? 0 : (L - lo >= 64 ? 64 : L - lo);
-    const uint64_t vmask = nvalid == 64 ? ~0ull : ((1ull << nvalid) - 1);
-    const uint64_t surv = ~((picks << 1) | (uint64_t)p_in) & vmask;
+    uint64_t picks, surv;
+    word_survivors(cand, n_words, L, blockIdx.x, p_tile, picks, surv);
     const uint32_t sc = (uint32_t)__popcll(surv);
     uint32_t sx = sc;
 #pragma unroll
@@ -2159,15 +2285,16 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     spicks[lane] = picks;
     ssurv[lane] = surv;
     soff[lane] = sx - sc;
+    scold[lane] = gw < n_words ? coldw[gw] : 0;
   }
   __syncthreads();
   uint32_t hot_used = 0;  // hot entries this lane picked (their used flags, once per wave)
   if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
-    emit_tile<true>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, off_tile, used, next, y,
-                    heads_out, err, hot_used);
+    emit_tile<true>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile,
+                    used, next, y, heads_out, err, hot_used);
   else
-    emit_tile<false>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, off_tile, used, next, y,
-                     heads_out, err, hot_used);
+    emit_tile<false>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile,
+                     used, next, y, heads_out, err, hot_used);
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t)
     if (__ballot((hot_used >> t) & 1) && lane == 0)
@@ -2317,12 +2444,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
   const uint64_t words32 = ceil_div(std::max<uint64_t>(N, 1), 32);
   DBuf<uint32_t> heads(pool, words32), heads_next(pool, words32);
   fill_u32(ctx, heads.get(), 0, words32);
-  if (!starts.empty()) {
-    DBuf<uint64_t> d_starts(pool, starts.size());
-    GFZ_HIP(hipMemcpyAsync(d_starts.get(), starts.data(), starts.size() * 8,
+  // Each non-empty traversal's first element in the current stream (kept by
+  // emit from round to round), and the longest traversal's length.
+  if (starts.size() >= (1ull << 31))
+    throw Error(kErrInvalid, "too many traversals (" + std::to_string(starts.size()) + ")");
+  const uint32_t n_start = (uint32_t)starts.size();
+  DBuf<uint64_t> tstart(pool, std::max<uint64_t>(n_start, 1)),
+      tstart_next(pool, std::max<uint64_t>(n_start, 1));
+  uint64_t longest = 0;
+  for (uint64_t t = 0; t < T; ++t)
+    longest = std::max<uint64_t>(longest, lengths[t]);
+  if (n_start) {
+    GFZ_HIP(hipMemcpyAsync(tstart.get(), starts.data(), starts.size() * 8,
                            hipMemcpyHostToDevice, ctx.stream()));
-    set_head_bits(ctx, d_starts.get(), starts.size(), heads.get(), N);
-    ctx.sync();  // d_starts is released at scope end
+    set_head_bits(ctx, tstart.get(), n_start, heads.get(), N);
+    ctx.sync();  // starts is host memory of this call
   }
 
   // Delta transform and the rule-ID floor.
@@ -2717,20 +2853,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     const RuleTable rt{rkeys.get(), rvals.get(),
                        (uint32_t)(regions ? region_lines : rcap / kLine),
                        regions ? (uint32_t)(regions - 1) : 0u, filter.get(), fbits};
-    // Round scalars (read back once, after K4): [0] next length, [1] used keys.
+    // Round scalars (read back once, after K4): [0] next length, [1] used keys,
+    // [2] the longest traversal of the next stream.
     // One launch clears everything the round needs cleared: the rule table, the
     // scalars, the used flags and the next round's head bits (up to L: the next
     // length is at most L).
-    DBuf<unsigned long long> scal(pool, 2);
+    DBuf<unsigned long long> scal(pool, 3);
     DBuf<uint8_t> used(pool, rcap);
     const uint64_t fwords = use_filter ? (uint64_t(1) << fbits) / 32 : 0;
     DBuf<HotEntry> hot(pool, kHotSlots);
     DBuf<HotPair> top(pool, kHotTop);
     // K2 + K2b: candidates, tile carries, next length.
-    DBuf<uint64_t> cand(pool, n_words);
+    DBuf<uint64_t> cand(pool, n_words), coldw(pool, n_words);
     DBuf<uint32_t> tile_fn(pool, n_tiles);
     DBuf<uint64_t> tile_info(pool, n_tiles);
-    DBuf<uint32_t> slots(pool, L);
+    DBuf<uint32_t> slots(pool, n_tiles * kRtTile);  // whole tiles: each wave's words at wave * kWavePos
     // Up to kTileScanSingle tiles one block scans them all (one launch); beyond,
     // blocks of kTileScanThreads tiles while that makes at most kTileScanThreads
     // blocks (then each thread takes a chunk), in three launches.
@@ -2741,17 +2878,30 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     DBuf<FnWide> block_fn(pool, n_blocks);
     uint64_t *d_total = reinterpret_cast<uint64_t *>(scal.get());
     DBuf<uint64_t> ukeys(pool, repeated), utmp(pool, repeated);  // K4's used keys
+    // Candidate marking in the cross-traversal tile order while its grid pads the
+    // tiles by at most a quarter (traversals of similar length; "cand_order" = 0
+    // keeps the launch order).
+    TileOrder order{nullptr, L, 0, 0};
+    uint64_t cand_grid = n_tiles;
+    if (n_start >= 2 && ctx.tuned("cand_order") != 0) {
+      uint32_t cols = 0;
+      const uint64_t g = order_grid(n_start, longest, cols);
+      if (4 * g <= 5 * n_tiles && g <= UINT32_MAX) {
+        order = TileOrder{tstart.get(), L, n_start, cols};
+        cand_grid = g;
+      }
+    }
     // Everything from the rule table to the used keys runs without a host step
     // in between: one graph segment.
     ctx.segment(seg_key("tail"), [&] {
     if (regions)  // the regions are written whole: the table needs no clearing
-      fill_runs(ctx, {{scal.get(), 2, 0, 8},
+      fill_runs(ctx, {{scal.get(), 3, 0, 8},
                       {used.get(), rcap, 0, 1},
                       {heads_next.get(), ceil_div(L, 32), 0, 4},
                       {filter.get(), fwords, 0, 4}});
     else
       fill_runs(ctx, {{rkeys.get(), rcap, kFree, 8},
-                      {scal.get(), 2, 0, 8},
+                      {scal.get(), 3, 0, 8},
                       {used.get(), rcap, 0, 1},
                       {heads_next.get(), ceil_div(L, 32), 0, 4},
                       {filter.get(), fwords, 0, 4}});
@@ -2776,9 +2926,9 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     });
     GFZ_CHECK_LAUNCH();
     ctx.timed("cand_tiles", [&] {
-      hipLaunchKernelGGL(k_cand, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, span, fresh, rt, hot.get(), top.get(), cand.get(), n_words,
-                         tile_fn.get(), slots.get());
+      hipLaunchKernelGGL(k_cand, dim3((uint32_t)cand_grid), dim3(kBlock), 0, ctx.stream(), cur,
+                         heads.get(), L, span, fresh, rt, hot.get(), top.get(), cand.get(),
+                         coldw.get(), n_words, tile_fn.get(), slots.get(), order);
     });
     GFZ_CHECK_LAUNCH();
     ctx.timed("tile_scan", [&] {
@@ -2801,10 +2951,19 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // K3: emit.
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), n_words, tile_info.get(), slots.get(),
-                         rcap, top.get(), used.get(), next, dst, heads_next.get(), d_err.get());
+                         heads.get(), L, cand.get(), coldw.get(), n_words, tile_info.get(),
+                         slots.get(), rcap, top.get(), used.get(), next, dst, heads_next.get(),
+                         d_err.get());
     });
     GFZ_CHECK_LAUNCH();
+    if (n_start) {
+      hipLaunchKernelGGL(k_starts_next, dim3((uint32_t)ceil_div(n_start, kBlock / kWave)),
+                         dim3(kBlock), 0, ctx.stream(), tstart.get(), n_start, cand.get(),
+                         n_words, tile_info.get(), L, tstart_next.get());
+      hipLaunchKernelGGL(k_trav_longest, dim3((n_start + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                         ctx.stream(), tstart_next.get(), n_start, d_total, scal.get() + 2);
+      GFZ_CHECK_LAUNCH();
+    }
 
     // K4: used keys.
     ctx.timed("collect_used", [&] {
@@ -2818,10 +2977,12 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     stage.reset();
     ctab.reset();
     cand.reset();
+    coldw.reset();
     tile_fn.reset();
     tile_info.reset();
-    const uint64_t *hs = ctx.fetch({{scal.get(), 2, 0}, {d_err.get(), 1, 4}});
+    const uint64_t *hs = ctx.fetch({{scal.get(), 3, 0}, {d_err.get(), 1, 4}});
     const uint64_t L_next = hs[0], R = hs[1];
+    const uint64_t longest_next = hs[2];
     if ((uint32_t)hs[4])
       throw Error(kErrCorrupt, "pair table lost a key (internal error)");
     if (L_next >= L)
@@ -2875,6 +3036,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     cur = dst;
     cur_buf = dst;
     heads.swap(heads_next);
+    tstart.swap(tstart_next);
+    longest = longest_next;
   }
   out.end_id = next;
   if (next_wide) {
@@ -2903,17 +3066,16 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
                                  "missing from the rule table, 3 rule table full, 4 sort "
                                  "look-back timed out)");
 
-  // Encoded lengths from the head positions of the final stream.
-  if (!ne_index.empty()) {
-    DBuf<uint64_t> pos(pool, ne_index.size());
-    const uint64_t cnt = bitmap_positions(ctx, heads.get(), L, pos.get(), ne_index.size());
-    if (cnt != ne_index.size())
-      throw Error(kErrCorrupt, "traversal heads lost during encoding (internal error)");
-    std::vector<uint64_t> h(cnt);
-    GFZ_HIP(hipMemcpyAsync(h.data(), pos.get(), cnt * 8, hipMemcpyDeviceToHost, ctx.stream()));
+  // Encoded lengths from the traversal starts of the final stream.
+  if (n_start) {
+    std::vector<uint64_t> h(n_start);
+    GFZ_HIP(hipMemcpyAsync(h.data(), tstart.get(), h.size() * 8, hipMemcpyDeviceToHost,
+                           ctx.stream()));
     ctx.sync();
-    for (uint64_t k = 0; k < cnt; ++k) {
-      const uint64_t end = k + 1 < cnt ? h[k + 1] : L;
+    for (uint64_t k = 0; k < n_start; ++k) {
+      const uint64_t end = k + 1 < n_start ? h[k + 1] : L;
+      if (h[k] >= end)
+        throw Error(kErrCorrupt, "traversal starts lost during encoding (internal error)");
       out.encoded_lengths[ne_index[k]] = (uint32_t)(end - h[k]);
     }
   }

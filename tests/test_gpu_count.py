@@ -6,8 +6,11 @@ for bit. So does tile aggregation in rounds of few keys with and without the kno
 repeated pairs dropped first (their keys then enter as an extra tile), also when
 the round falls back to the table, and with rule-table regions from barely above
 their longest list (long probe chains that wrap within a region) to four times it.
+Candidate marking in the cross-traversal tile order (24 traversals of similar
+length: the order's grid pads the tiles by under a quarter, so it is taken) and in
+the launch order gives the same grammar too.
 The knobs are the ctx tuned values "count_buckets", "count_rep_cap",
-"count_hot_drop" and "rule_region_pct"."""
+"count_hot_drop", "rule_region_pct" and "cand_order"."""
 import sys
 
 import numpy as np
@@ -101,3 +104,14 @@ def test_rule_region_sizes_give_the_oracle_grammar(gpu_ctx, case, pct):
         _check(gpu_ctx, tr, ref)
     finally:
         gpu_ctx.set_tuned("rule_region_pct", -1)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_candidate_tile_orders_give_the_oracle_grammar(gpu_ctx, case, order):
+    tr, ref = case
+    say(f"candidate tile order {order}")
+    gpu_ctx.set_tuned("cand_order", order)
+    try:
+        _check(gpu_ctx, tr, ref)
+    finally:
+        gpu_ctx.set_tuned("cand_order", -1)
