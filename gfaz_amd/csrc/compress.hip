@@ -2344,23 +2344,41 @@ __global__ __launch_bounds__(kBlock) void k_collect_used(const unsigned long lon
 }
 
 // ---- K6: final IDs next + rank, rule columns ----------------------------------
+// Each thread looks up kAssignBatch sorted keys at once (their key loads, then
+// their line reads, in flight together).
+constexpr int kAssignBatch = 4;
 __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ sorted, uint64_t R,
                                                    uint64_t span, RuleTable rt, uint32_t next,
                                                    int32_t *__restrict__ rf,
                                                    int32_t *__restrict__ rs,
                                                    uint32_t *__restrict__ err) {
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
-       r += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t ck = sorted[r];
-    const int64_t s = rule_find(rt, ck);
-    if (s < 0) {
-      atomicExch(err, 2u);
-      continue;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r0 < R;
+       r0 += stride * kAssignBatch) {
+    uint64_t ck[kAssignBatch];
+    bool v[kAssignBatch];
+    int64_t s[kAssignBatch];
+#pragma unroll
+    for (int q = 0; q < kAssignBatch; ++q) {
+      const uint64_t r = r0 + (uint64_t)q * stride;
+      v[q] = r < R;
+      ck[q] = v[q] ? sorted[r] : 0;
     }
-    rt.vals[s] = next + (uint32_t)r;
-    const int64_t k = expand_key(ck, span);
-    rf[r] = key_first(k);
-    rs[r] = key_second(k);
+    rule_find_batch<kAssignBatch>(rt, ck, v, s);
+#pragma unroll
+    for (int q = 0; q < kAssignBatch; ++q) {
+      const uint64_t r = r0 + (uint64_t)q * stride;
+      if (!v[q])
+        continue;
+      if (s[q] < 0) {
+        atomicExch(err, 2u);
+        continue;
+      }
+      rt.vals[s[q]] = next + (uint32_t)r;
+      const int64_t k = expand_key(ck[q], span);
+      rf[r] = key_first(k);
+      rs[r] = key_second(k);
+    }
   }
 }
 
