@@ -75,6 +75,9 @@ struct Rulebook {
   // ID and the total fits the budget; otherwise expansion descends the tree.
   bool has_bodies = false;
   DBuf<uint64_t> zoff;
+  // With the bodies: per rule {expanded size, sum, zoff (below 2^31), 0}, so a
+  // rule symbol's size, sum and body position come in one 16-byte gather.
+  DBuf<uint4> szo;
   DBuf<int32_t> z;          // + two trailing zeros that node-ID symbols point at
   uint64_t z_elems = 0;
   uint32_t z_raw = 0;       // index of the second trailing zero

@@ -971,7 +971,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_part(const int32_t *__restric
 // loads coalesce; segments are tens to hundreds of entries, so the walk is a
 // step or two). fn(entry, segment index, offset in the segment).
 constexpr uint32_t kFlatSegs = 1024;
-constexpr uint32_t kFlatItems = 4;
+constexpr uint32_t kFlatItems = 8;
 struct FlatLds {
   uint32_t pre[kFlatSegs + 1];
   const unsigned long long *src[kFlatSegs];

@@ -66,12 +66,16 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
   }
 }
 
-// Signed sum (mod 2^32) of a child symbol's expansion.
-__device__ inline uint32_t child_sum(int32_t c, bool raw, const uint2 *rss, uint32_t min_id) {
-  if (raw)
-    return (uint32_t)c;
-  const uint32_t v = rss[uabs(c) - min_id].y;
-  return c > 0 ? v : (uint32_t)0 - v;
+// Size, sum and Z offset of every rule in one record (the Z offsets are below
+// 2^31 whenever bodies are built).
+__global__ __launch_bounds__(kBlock) void k_rule_szo(const uint2 *__restrict__ rss,
+                                                     const uint64_t *__restrict__ zoff,
+                                                     uint64_t R, uint4 *__restrict__ szo) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * kBlock) {
+    const uint2 w = rss[r];
+    szo[r] = make_uint4(w.x, w.y, (uint32_t)zoff[r], 0u);
+  }
 }
 
 // One bottom-up pass. In pass k a rule resolves when each child is a node ID or a
@@ -285,17 +289,19 @@ __device__ inline void plan_heads(const uint64_t *__restrict__ starts, uint64_t 
 struct PlanSyms {
   int32_t v[kPlanItems];
   uint32_t sz[kPlanItems], ss[kPlanItems];
+  uint32_t zo[kPlanItems];  // a rule symbol's Z offset (plans that decode through Z)
   uint32_t head;   // bit j: a traversal starts at this lane's item j
   uint32_t valid;  // bit j: item j is inside the stream
 };
 __device__ inline uint64_t plan_chunk(uint64_t s0) {
   return s0 + (uint64_t)(threadIdx.x / kWave) * (kPlanItems * kWave);
 }
-// Symbols and their rules' sizes and sums; issued before the tile's traversal
+// Symbols and their rules' sizes and sums (and Z offsets, from szo when it is
+// given, else sizes and sums from rss); issued before the tile's traversal
 // starts are looked up, so the two latencies overlap.
 __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc, uint64_t s0,
-                                 const uint2 *__restrict__ rss, uint64_t R, uint32_t min_id,
-                                 PlanSyms &p) {
+                                 const uint2 *__restrict__ rss, const uint4 *__restrict__ szo,
+                                 uint64_t R, uint32_t min_id, PlanSyms &p) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t c0 = plan_chunk(s0);
   p.head = p.valid = 0;
@@ -305,12 +311,24 @@ __device__ inline void plan_syms(const int32_t *__restrict__ enc, uint64_t n_enc
     p.v[j] = enc[i < n_enc ? i : n_enc - 1];
     p.valid |= i < n_enc ? 1u << j : 0u;
   }
+  if (szo) {
 #pragma unroll
-  for (uint32_t j = 0; j < kPlanItems; ++j) {
-    const bool rule = is_rule(p.v[j], min_id, R);
-    const uint2 w = rule ? rss[uabs(p.v[j]) - min_id] : make_uint2(1u, 0u);
-    p.sz[j] = w.x;
-    p.ss[j] = w.y;
+    for (uint32_t j = 0; j < kPlanItems; ++j) {
+      const bool rule = is_rule(p.v[j], min_id, R);
+      const uint4 w = rule ? szo[uabs(p.v[j]) - min_id] : make_uint4(1u, 0u, 0u, 0u);
+      p.sz[j] = w.x;
+      p.ss[j] = w.y;
+      p.zo[j] = w.z;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kPlanItems; ++j) {
+      const bool rule = is_rule(p.v[j], min_id, R);
+      const uint2 w = rule ? rss[uabs(p.v[j]) - min_id] : make_uint2(1u, 0u);
+      p.sz[j] = w.x;
+      p.ss[j] = w.y;
+      p.zo[j] = 0u;
+    }
   }
 }
 // Then the start flags (from the LDS bitmap) and the sizes and signed sums.
@@ -331,11 +349,11 @@ __device__ inline void plan_finish(uint64_t s0, const uint32_t *hbits, uint64_t 
   p.head &= p.valid;
 }
 
-// The apply pass's symbols: values, and sizes and signed sums as the reduce pass
-// stored them (no rule lookups here).
+// The apply pass's symbols: values, and sizes, signed sums and Z offsets as the
+// reduce pass stored them (no rule lookups here).
 __device__ inline void plan_syms_stored(const int32_t *__restrict__ enc, uint64_t n_enc,
                                         uint64_t s0, const uint2 *__restrict__ szss,
-                                        PlanSyms &p) {
+                                        const uint32_t *__restrict__ zos, PlanSyms &p) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t c0 = plan_chunk(s0);
   p.head = p.valid = 0;
@@ -345,6 +363,7 @@ __device__ inline void plan_syms_stored(const int32_t *__restrict__ enc, uint64_
     const uint64_t ic = i < n_enc ? i : n_enc - 1;
     p.v[j] = enc[ic];
     const uint2 w = szss[ic];
+    p.zo[j] = zos ? zos[ic] : 0u;
     const bool ok = i < n_enc;
     p.valid |= ok ? 1u << j : 0u;
     p.sz[j] = ok ? w.x : 0u;
@@ -407,9 +426,9 @@ __device__ inline PlanAgg plan_waves_prefix(const PlanAgg &wave_total, PlanAgg *
 
 __global__ __launch_bounds__(kBlock) void k_plan_reduce(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
-    uint64_t n_starts, const uint2 *__restrict__ rss, uint64_t R, uint32_t min_id,
-    uint64_t n_tiles, uint32_t rows, uint64_t cols,
-    PlanAgg *__restrict__ tiles, uint2 *__restrict__ szss) {
+    uint64_t n_starts, const uint2 *__restrict__ rss, const uint4 *__restrict__ szo,
+    uint64_t R, uint32_t min_id, uint64_t n_tiles, uint32_t rows, uint64_t cols,
+    PlanAgg *__restrict__ tiles, uint2 *__restrict__ szss, uint32_t *__restrict__ zos) {
   __shared__ uint32_t hbits[kPlanTile / 32];
   __shared__ uint64_t range[2];
   __shared__ PlanAgg lds[kBlock / kWave];
@@ -419,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_reduce(
   const uint64_t s0 = t * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
   PlanSyms p;
-  plan_syms(enc, n_enc, s0, rss, R, min_id, p);
+  plan_syms(enc, n_enc, s0, rss, szo, R, min_id, p);
   plan_heads(starts, n_starts, s0, s1, hbits, range);
   plan_finish(s0, hbits, R, min_id, p);
   {
@@ -427,8 +446,11 @@ __global__ __launch_bounds__(kBlock) void k_plan_reduce(
     const uint64_t c0 = plan_chunk(s0);
 #pragma unroll
     for (uint32_t j = 0; j < kPlanItems; ++j)
-      if ((p.valid >> j) & 1)
+      if ((p.valid >> j) & 1) {
         szss[c0 + (uint64_t)j * kWave + lane] = make_uint2(p.sz[j], p.ss[j]);
+        if (zos)
+          zos[c0 + (uint64_t)j * kWave + lane] = p.zo[j];
+      }
   }
   PlanAgg total;
   plan_waves_prefix(plan_wave_total(p), lds, total);
@@ -463,7 +485,7 @@ __global__ __launch_bounds__(kPlanScanThreads) void k_plan_scan(PlanAgg *__restr
 // inverse delta folded in).
 __global__ __launch_bounds__(kBlock) void k_plan_apply(
     const int32_t *__restrict__ enc, uint64_t n_enc, const uint64_t *__restrict__ starts,
-    uint64_t n_starts, const uint2 *__restrict__ szss, const uint64_t *__restrict__ zoff,
+    uint64_t n_starts, const uint2 *__restrict__ szss, const uint32_t *__restrict__ zos,
     uint64_t R, uint32_t min_id, uint32_t zraw, int delta,
     const PlanAgg *__restrict__ tile_prefix, uint64_t n_tiles, uint32_t rows, uint64_t cols,
     uint64_t *__restrict__ sym_off, SymInfo *__restrict__ syms) {
@@ -477,16 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
   const uint64_t s0 = t * kPlanTile;
   const uint64_t s1 = s0 + kPlanTile < n_enc ? s0 + kPlanTile : n_enc;
   PlanSyms p;
-  plan_syms_stored(enc, n_enc, s0, szss, p);
-  // Z positions of the rule symbols, loaded before any is used.
-  uint64_t zo[kPlanItems];
-  if (delta >= 0) {
-#pragma unroll
-    for (uint32_t j = 0; j < kPlanItems; ++j) {
-      const bool rule = is_rule(p.v[j], min_id, R);
-      zo[j] = rule ? zoff[uabs(p.v[j]) - min_id] : 0u;
-    }
-  }
+  plan_syms_stored(enc, n_enc, s0, szss, delta >= 0 ? zos : nullptr, p);
   plan_heads(starts, n_starts, s0, s1, hbits, range);
   plan_heads_only(s0, hbits, p);
   PlanAgg total;
@@ -513,10 +526,10 @@ __global__ __launch_bounds__(kBlock) void k_plan_apply(
         if (is_rule(v, min_id, R)) {  // as rule_sym: ss = -sum for a reversed rule
           if (v > 0) {
             s.base = pre;
-            s.zz = (uint32_t)(zo[j] + 1);
+            s.zz = p.zo[j] + 1;
           } else {
             s.base = delta > 0 ? pre + p.ss[j] : pre;
-            s.zz = (uint32_t)(zo[j] + p.sz[j] - 1) | kRevBit;
+            s.zz = (p.zo[j] + p.sz[j] - 1) | kRevBit;
           }
         } else {
           s.base = pre + (uint32_t)v;
@@ -605,7 +618,7 @@ __device__ inline uint64_t wave_last_le_u64(const uint64_t *__restrict__ a, uint
 // straight into LDS; children are at shallower, already-built levels.
 template <uint32_t kItems>
 __global__ __launch_bounds__(kBlock) void k_expand_level(
-    const RuleRec *__restrict__ rec, const uint2 *__restrict__ rss,
+    const RuleRec *__restrict__ rec, const uint4 *__restrict__ szo,
     const uint64_t *__restrict__ zoff, uint64_t r0,
     uint64_t r1, uint32_t min_id, uint64_t R, uint64_t lo, uint64_t n_out, int32_t *Z,
     uint32_t zraw) {
@@ -640,11 +653,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_level(
     const RuleRec c = rec[r];
     const uint64_t z0 = zoff[r];
     const bool ra = !is_rule(c.a, min_id, R), rb = !is_rule(c.b, min_id, R);
-    const uint32_t suma = child_sum(c.a, ra, rss, min_id);
+    // A rule child's record gives its body position and, for child a, the sum
+    // that shifts child b (one gather per rule child).
+    const uint4 wa = ra ? make_uint4(0u, 0u, 0u, 0u) : szo[uabs(c.a) - min_id];
+    const uint32_t suma = ra ? (uint32_t)c.a : (c.a > 0 ? wa.y : 0u - wa.y);
     const SymInfo sa = ra ? SymInfo{(uint32_t)c.a, zraw}
-                          : rule_sym(c.a, 0u, true, rss, zoff, min_id);
+                          : (c.a > 0 ? SymInfo{0u, wa.z + 1}
+                                     : SymInfo{0u - wa.y, (wa.z + wa.x - 1) | kRevBit});
     const SymInfo sb = rb ? SymInfo{suma + (uint32_t)c.b, zraw}
-                          : rule_sym(c.b, suma, true, rss, zoff, min_id);
+                          : rule_sym(c.b, suma, true, szo, min_id);
     const uint64_t g0 = 3 * (r - rf);  // symbol q of rule r is g0 + q - q0
 #pragma unroll
     for (uint32_t q = 0; q < 3; ++q) {
@@ -926,7 +943,7 @@ void launch_level(Ctx &ctx, const Rulebook &rb, uint64_t r0, uint64_t r1, uint32
   if (n_tiles > UINT32_MAX)
     throw Error(kErrInvalid, "rule bodies of one level too large for one launch");
   hipLaunchKernelGGL(k_expand_level<kItems>, dim3((uint32_t)n_tiles), dim3(kBlock), 0,
-                     ctx.stream(), rb.rec.get(), rb.ss.get(), rb.zoff.get(), r0,
+                     ctx.stream(), rb.rec.get(), rb.szo.get(), rb.zoff.get(), r0,
                      r1, min_id, R, lo, n_out, rb.z.get(), rb.z_raw);
   GFZ_CHECK_LAUNCH();
 }
@@ -981,6 +998,7 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
     rb.z = DBuf<int32_t>(ctx.pool(), 2);
     fill_u32(ctx, reinterpret_cast<uint32_t *>(rb.z.get()), 0, 2);
     rb.z_raw = 1;
+    rb.szo = DBuf<uint4>(ctx.pool(), 1);
     rb.has_bodies = true;
     ctx.sync();
     return;
@@ -1088,6 +1106,10 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   rb.z_elems = total;
   rb.z_raw = (uint32_t)(total + 1);
   fill_u32(ctx, reinterpret_cast<uint32_t *>(rb.z.get()) + total, 0, 2);
+  rb.szo = DBuf<uint4>(ctx.pool(), R);
+  hipLaunchKernelGGL(k_rule_szo, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
+                     rb.ss.get(), rb.zoff.get(), R, rb.szo.get());
+  GFZ_CHECK_LAUNCH();
   // One level at a time: children are always at shallower, already-built levels.
   for (uint32_t d = 1; d <= max_depth; ++d) {
     const uint64_t r0 = bound[d], r1 = bound[d + 1];
@@ -1134,6 +1156,9 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
     const uint64_t n_tiles = ceil_div(n_enc, kPlanTile);
     DBuf<PlanAgg> tiles(ctx.pool(), n_tiles);
     DBuf<uint2> szss(ctx.pool(), n_enc);  // each symbol's size and signed sum
+    DBuf<uint32_t> zos;                   // and its rule's Z offset (decodes through Z)
+    if (rec >= 0)
+      zos = DBuf<uint32_t>(ctx.pool(), n_enc);
     // Same stretch of every traversal together (block_tile): its symbols gather
     // the same rules' sizes, sums and Z offsets.
     const uint32_t rows = expansion_rows(ctx, n_trav, n_tiles);
@@ -1143,13 +1168,15 @@ void decode_plan(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_
       throw Error(kErrInvalid, "encoded block too large for one launch");
     ctx.timed("decode_plan", [&] {
       hipLaunchKernelGGL(k_plan_reduce, dim3((uint32_t)grid), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, idx.get(), n_trav + 1, rb.ss.get(), rb.n_rules, rb.min_id, n_tiles, rows, cols, tiles.get(), szss.get());
+                         d_enc, n_enc, idx.get(), n_trav + 1, rb.ss.get(),
+                         rec >= 0 ? rb.szo.get() : nullptr, rb.n_rules, rb.min_id, n_tiles,
+                         rows, cols, tiles.get(), szss.get(), zos.get());
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(kPlanScanThreads), 0, ctx.stream(),
                          tiles.get(), n_tiles, sym_off.get() + n_enc);
       GFZ_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_plan_apply, dim3((uint32_t)grid), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, idx.get(), n_trav + 1, szss.get(), rb.zoff.get(),
+                         d_enc, n_enc, idx.get(), n_trav + 1, szss.get(), zos.get(),
                          rb.n_rules, rb.min_id, rb.z_raw, rec, tiles.get(), n_tiles, rows, cols,
                          sym_off.get(), plan.syms.get());
       GFZ_CHECK_LAUNCH();

@@ -128,15 +128,14 @@ __global__ void k_sym_plan_r(const int32_t *__restrict__ enc, uint64_t n,
 
 __global__ void k_sym_info_r(const int32_t *__restrict__ enc, uint64_t n,
                              const int32_t *__restrict__ incl, const int32_t *__restrict__ ssum,
-                             bool delta, const uint2 *__restrict__ rss,
-                             const uint64_t *__restrict__ zoff,
+                             bool delta, const uint4 *__restrict__ szo,
                              uint64_t R, uint32_t min_id, uint32_t zraw,
                              SymInfo *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const int32_t v = enc[i];
     const uint32_t pre = delta ? (uint32_t)incl[i] - (uint32_t)ssum[i] : 0u;
-    out[i] = is_rule(v, min_id, R) ? rule_sym(v, pre, delta, rss, zoff, min_id)
+    out[i] = is_rule(v, min_id, R) ? rule_sym(v, pre, delta, szo, min_id)
                                    : SymInfo{pre + (uint32_t)v, zraw};
   }
 }
@@ -222,8 +221,8 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
     }
     if (n_enc) {
       hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
-                         d_enc, n_enc, incl.get(), ssum.get(), delta, rb.ss.get(),
-                         rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get());
+                         d_enc, n_enc, incl.get(), ssum.get(), delta, rb.szo.get(),
+                         rb.n_rules, rb.min_id, rb.z_raw, res.syms.get());
       GFZ_CHECK_LAUNCH();
     }
     ctx.sync();
@@ -474,8 +473,8 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
       ctx.sync();  // `starts` is pageable
     }
     hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
-                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.ss.get(),
-                       rb.zoff.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
+                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.szo.get(),
+                       rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
     GFZ_CHECK_LAUNCH();
     GFZ_HIP(hipMemcpyAsync(h_off.data(), off.get(), (n_new + 1) * 8, hipMemcpyDeviceToHost,
                            ctx.stream()));

@@ -52,18 +52,17 @@ inline uint64_t row_grid(uint64_t n_tiles, uint32_t rows, uint64_t &cols) {
 // Z[j] - Z[j -/+ 1] instead, a body element.
 constexpr uint32_t kRevBit = 0x80000000u;
 
-// rss[r] = (expanded size, sum) of rule r.
-__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint2 *rss,
-                                   const uint64_t *zoff, uint32_t min_id) {
-  const uint32_t r = uabs(v) - min_id;
+// szo[r] = (expanded size, sum, Z offset) of rule r.
+__device__ inline SymInfo rule_sym(int32_t v, uint32_t pre, bool delta, const uint4 *szo,
+                                   uint32_t min_id) {
+  const uint4 w = szo[uabs(v) - min_id];
   SymInfo s;
   if (v > 0) {
     s.base = pre;
-    s.zz = (uint32_t)(zoff[r] + 1);
+    s.zz = w.z + 1;
   } else {
-    const uint2 w = rss[r];
     s.base = delta ? pre - w.y : pre;
-    s.zz = (uint32_t)(zoff[r] + w.x - 1) | kRevBit;
+    s.zz = (w.z + w.x - 1) | kRevBit;
   }
   return s;
 }
