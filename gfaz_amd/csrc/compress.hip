@@ -486,30 +486,51 @@ __global__ __launch_bounds__(kBlock) void k_delta(const int32_t *__restrict__ in
                                                   int transform, uint32_t *max_abs) {
   uint32_t m = 0;
   const uint64_t n8 = n / 8;
-  for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < n8;
-       c += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t i0 = c * 8;
-    const int4 a = *reinterpret_cast<const int4 *>(in + i0);
-    const int4 b = *reinterpret_cast<const int4 *>(in + i0 + 4);
-    int32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    if (transform) {
-      const uint32_t hb = (heads[i0 >> 5] >> (i0 & 31)) & 0xFFu;  // i0 % 8 == 0
-      const int32_t prev = (hb & 1u) ? 0 : in[i0 - 1];
-      int32_t d[8];
-      d[0] = (int32_t)((uint32_t)v[0] - (uint32_t)prev);
+  // kDeltaChunks chunks of 8 per thread and step, every load of the step
+  // (chunks, head words, the element before each chunk) issued before any use.
+  constexpr uint32_t kDeltaChunks = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c0 < n8;
+       c0 += stride * kDeltaChunks) {
+    int4 a[kDeltaChunks], b[kDeltaChunks];
+    uint32_t hw[kDeltaChunks];
+    int32_t pv[kDeltaChunks];
 #pragma unroll
-      for (int j = 1; j < 8; ++j)
-        d[j] = ((hb >> j) & 1u) ? v[j] : (int32_t)((uint32_t)v[j] - (uint32_t)v[j - 1]);
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        v[j] = d[j];
-      *reinterpret_cast<int4 *>(out + i0) = make_int4(v[0], v[1], v[2], v[3]);
-      *reinterpret_cast<int4 *>(out + i0 + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    for (uint32_t u = 0; u < kDeltaChunks; ++u) {
+      const uint64_t c = c0 + u * stride;
+      const uint64_t i0 = (c < n8 ? c : n8 - 1) * 8;
+      a[u] = *reinterpret_cast<const int4 *>(in + i0);
+      b[u] = *reinterpret_cast<const int4 *>(in + i0 + 4);
+      if (transform) {
+        hw[u] = heads[i0 >> 5];
+        pv[u] = in[i0 ? i0 - 1 : 0];  // unused at a head (the stream starts with one)
+      }
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t u = uabs(v[j]);
-      m = u > m ? u : m;
+    for (uint32_t u = 0; u < kDeltaChunks; ++u) {
+      const uint64_t c = c0 + u * stride;
+      if (c >= n8)
+        break;
+      const uint64_t i0 = c * 8;
+      int32_t v[8] = {a[u].x, a[u].y, a[u].z, a[u].w, b[u].x, b[u].y, b[u].z, b[u].w};
+      if (transform) {
+        const uint32_t hb = (hw[u] >> (i0 & 31)) & 0xFFu;  // i0 % 8 == 0
+        int32_t d[8];
+        d[0] = (hb & 1u) ? v[0] : (int32_t)((uint32_t)v[0] - (uint32_t)pv[u]);
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+          d[j] = ((hb >> j) & 1u) ? v[j] : (int32_t)((uint32_t)v[j] - (uint32_t)v[j - 1]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          v[j] = d[j];
+        *reinterpret_cast<int4 *>(out + i0) = make_int4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<int4 *>(out + i0 + 4) = make_int4(v[4], v[5], v[6], v[7]);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t w = uabs(v[j]);
+        m = w > m ? w : m;
+      }
     }
   }
   // Tail (n % 8 elements), one block.
