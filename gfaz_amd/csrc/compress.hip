@@ -2404,12 +2404,21 @@ __global__ __launch_bounds__(kBlock) void k_assign(const uint64_t *__restrict__ 
 }
 
 // ---- K7: provisional -> final IDs (compression_utils.cpp:201-221) -------------
-// Provisional IDs (next + slot) to final ones (vals[slot]), in place, four
-// symbols per 16-byte access. The round's hot rules (most picks of the early
-// rounds) take their final IDs from registers instead of a gather each.
+// Provisional IDs (next + slot) to final ones (vals[slot]), in place, one
+// kRtTile tile per block, four symbols per 16-byte access with every load of the
+// tile issued first. The round's hot rules (most picks of the early rounds) take
+// their final IDs from registers instead of a gather each. In the
+// cross-traversal tile order (TileOrder over the next stream's traversals) the
+// same stretch of every haplotype is remapped together on one XCD, so the
+// final-ID lines of the rules they share are gathered into its L2 once.
+constexpr uint32_t kRemapQuads = kRtTile / 4 / kBlock;
 __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint64_t n,
                                                   const uint32_t *__restrict__ vals, uint32_t next,
-                                                  const HotPair *__restrict__ top) {
+                                                  const HotPair *__restrict__ top,
+                                                  TileOrder order) {
+  const uint64_t tile = order_tile(order);
+  if (tile == kNoTile)
+    return;
   uint32_t hs[kHotTop], hf[kHotTop];
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t) {
@@ -2434,18 +2443,29 @@ __global__ __launch_bounds__(kBlock) void k_remap(int32_t *__restrict__ y, uint6
       f = vals[sl];
     return v > 0 ? (int32_t)f : -(int32_t)f;
   };
+  const uint64_t q0 = tile * (kRtTile / 4);  // first 16-byte quad of the tile
   const uint64_t n4 = n / 4;
   int4 *__restrict__ y4 = reinterpret_cast<int4 *>(y);  // y: a pool buffer, 256-byte aligned
-  for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < n4;
-       c += (uint64_t)gridDim.x * kBlock) {
-    int4 v = y4[c];
-    v.x = map(v.x);
-    v.y = map(v.y);
-    v.z = map(v.z);
-    v.w = map(v.w);
-    y4[c] = v;
+  int4 v[kRemapQuads];
+#pragma unroll
+  for (uint32_t q = 0; q < kRemapQuads; ++q) {
+    const uint64_t c = q0 + (uint64_t)q * kBlock + threadIdx.x;
+    v[q] = c < n4 ? y4[c] : make_int4(0, 0, 0, 0);
   }
-  if (blockIdx.x == 0 && threadIdx.x < n - n4 * 4) {
+#pragma unroll
+  for (uint32_t q = 0; q < kRemapQuads; ++q) {
+    const uint64_t c = q0 + (uint64_t)q * kBlock + threadIdx.x;
+    if (c < n4) {
+      int4 w = v[q];
+      w.x = map(w.x);
+      w.y = map(w.y);
+      w.z = map(w.z);
+      w.w = map(w.w);
+      y4[c] = w;
+    }
+  }
+  // The stream's last n % 4 symbols (inside the last tile).
+  if ((tile + 1) * kRtTile >= n && threadIdx.x < n - n4 * 4) {
     const uint64_t i = n4 * 4 + threadIdx.x;
     y[i] = map(y[i]);
   }
@@ -3046,6 +3066,21 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
       rs = std::move(ns);
       rules_cap = ncap;
     }
+    // Remap in the cross-traversal tile order of the next stream (its traversal
+    // starts are tstart_next), on the same terms as candidate marking.
+    const uint64_t next_tiles = ceil_div(L_next, kRtTile);
+    TileOrder remap_order{nullptr, L_next, 0, 0};
+    uint64_t remap_grid = std::max<uint64_t>(next_tiles, 1);
+    if (n_start >= 2 && ctx.tuned("cand_order") != 0) {
+      uint32_t cols = 0;
+      const uint64_t g = order_grid(n_start, longest_next, cols);
+      if (4 * g <= 5 * next_tiles && g <= UINT32_MAX) {
+        remap_order = TileOrder{tstart_next.get(), L_next, n_start, cols};
+        remap_grid = g;
+      }
+    }
+    if (remap_grid > UINT32_MAX)
+      throw Error(kErrInvalid, "stream too long for one remap launch");
     ctx.segment(seg_key("ids"), [&] {
     // K5: sort by signed key (compacted keys sort as unsigned); K6: IDs, columns.
     sort_unique_u64(ctx, ukeys.get(), utmp.get(), R, d_err.get(), 0,
@@ -3058,8 +3093,8 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     GFZ_CHECK_LAUNCH();
     // K7: remap provisional IDs (errors of K6 surface at the next readback).
     ctx.timed("remap", [&] {
-      hipLaunchKernelGGL(k_remap, dim3(stride_grid(ctx, ceil_div(L_next, 4))), dim3(kBlock), 0,
-                         ctx.stream(), dst, L_next, rvals.get(), next, top.get());
+      hipLaunchKernelGGL(k_remap, dim3((uint32_t)remap_grid), dim3(kBlock), 0, ctx.stream(),
+                         dst, L_next, rvals.get(), next, top.get(), remap_order);
     });
     GFZ_CHECK_LAUNCH();
     });
