@@ -2182,23 +2182,22 @@ __global__ __launch_bounds__(kBlock) void k_starts_next(
 // emits +/-(next + slot) (provisional, remapped after the sort). A pick's slot
 // and orientation come from the hot pairs (tested raw, as in K2) or, for a cold
 // pick, from its slot word (its wave's k-th for the wave's k-th cold
-// candidate). Heads always survive and carry their bit to the output position.
+// candidate). Heads always survive; their bits in the next stream are set from
+// the next traversal starts (k_starts_next).
 constexpr uint32_t kEmitPass = 8;  // items per pass of emit (their loads in flight together)
 template <bool kInterior>
-__device__ inline void emit_tile(const int32_t *__restrict__ x,
-                                 const uint32_t *__restrict__ heads, uint64_t L,
+__device__ inline void emit_tile(const int32_t *__restrict__ x, uint64_t L,
                                  const uint32_t *__restrict__ slots, uint64_t rcap,
                                  const HotKeys &hk, const uint64_t *spicks,
                                  const uint64_t *ssurv, const uint32_t *soff,
                                  const uint64_t *scold, uint64_t off_tile,
                                  uint8_t *__restrict__ used, uint32_t next,
-                                 int32_t *__restrict__ y, uint32_t *__restrict__ heads_out,
-                                 uint32_t *__restrict__ err, uint32_t &hot_used) {
+                                 int32_t *__restrict__ y, uint32_t *__restrict__ err,
+                                 uint32_t &hot_used) {
   const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t below = lanemask_lt();
   const uint64_t base = (uint64_t)blockIdx.x * kRtTile;
   const int32_t *__restrict__ xt = x + base;
-  const uint32_t *__restrict__ ht = heads + (base >> 5);
   const uint32_t *__restrict__ st = slots + base;
   // This wave's cold candidates so far, from its slot words' region on: the
   // next cold candidate's slot word.
@@ -2208,7 +2207,7 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
     // (interior tiles: every pair inside the stream; the last tile clamps its
     // indexes into it): one wait per pass, not one per cold pick.
     int32_t xa[kEmitPass], xb[kEmitPass];
-    uint32_t hw[kEmitPass], sw[kEmitPass];
+    uint32_t sw[kEmitPass];
     uint32_t coldpick = 0;
 #pragma unroll
     for (uint32_t b = 0; b < kEmitPass; ++b) {
@@ -2216,14 +2215,12 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
       if constexpr (kInterior) {
         xa[b] = xt[r];
         xb[b] = xt[r + 1];
-        hw[b] = ht[r >> 5];
       } else {
         const uint64_t i = base + r;
         const uint64_t ia = i < L ? i : L - 1;
         const uint64_t ib = ia + 1 < L ? ia + 1 : L - 1;
         xa[b] = x[ia];
         xb[b] = x[ib];
-        hw[b] = heads[ia >> 5];
       }
       const uint32_t w = (j0 + b) * kWaves + wid;
       const uint64_t cw = scold[w];
@@ -2266,20 +2263,18 @@ __device__ inline void emit_tile(const int32_t *__restrict__ x,
             o = -o;
         }
         y[pos] = o;
-        if ((hw[b] >> (r & 31)) & 1)
-          atomicOr(&heads_out[pos >> 5], 1u << (pos & 31));
       }
     }
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_emit(
-    const int32_t *__restrict__ x, const uint32_t *__restrict__ heads, uint64_t L,
+    const int32_t *__restrict__ x, uint64_t L,
     const uint64_t *__restrict__ cand, const uint64_t *__restrict__ coldw, uint64_t n_words,
     const uint64_t *__restrict__ tile_info,
     const uint32_t *__restrict__ slots, uint64_t rcap, const HotPair *__restrict__ top,
     uint8_t *__restrict__ used, uint32_t next, int32_t *__restrict__ y,
-    uint32_t *__restrict__ heads_out, uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ err) {
   __shared__ uint64_t spicks[kRtWords];
   __shared__ uint64_t ssurv[kRtWords];
   __shared__ uint32_t soff[kRtWords];
@@ -2311,11 +2306,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   __syncthreads();
   uint32_t hot_used = 0;  // hot entries this lane picked (their used flags, once per wave)
   if ((uint64_t)(blockIdx.x + 1) * kRtTile < L)
-    emit_tile<true>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile,
-                    used, next, y, heads_out, err, hot_used);
+    emit_tile<true>(x, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile, used, next,
+                    y, err, hot_used);
   else
-    emit_tile<false>(x, heads, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile,
-                     used, next, y, heads_out, err, hot_used);
+    emit_tile<false>(x, L, slots, rcap, hk, spicks, ssurv, soff, scold, off_tile, used, next,
+                     y, err, hot_used);
 #pragma unroll
   for (uint32_t t = 0; t < kHotTop; ++t)
     if (__ballot((hot_used >> t) & 1) && lane == 0)
@@ -3010,15 +3005,16 @@ void compress_device(Ctx &ctx, const int32_t *d_nodes, uint64_t N, const uint32_
     // K3: emit.
     ctx.timed("emit", [&] {
       hipLaunchKernelGGL(k_emit, dim3((uint32_t)n_tiles), dim3(kBlock), 0, ctx.stream(), cur,
-                         heads.get(), L, cand.get(), coldw.get(), n_words, tile_info.get(),
-                         slots.get(), rcap, top.get(), used.get(), next, dst, heads_next.get(),
-                         d_err.get());
+                         L, cand.get(), coldw.get(), n_words, tile_info.get(), slots.get(), rcap,
+                         top.get(), used.get(), next, dst, d_err.get());
     });
     GFZ_CHECK_LAUNCH();
     if (n_start) {
       hipLaunchKernelGGL(k_starts_next, dim3((uint32_t)ceil_div(n_start, kBlock / kWave)),
                          dim3(kBlock), 0, ctx.stream(), tstart.get(), n_start, cand.get(),
                          n_words, tile_info.get(), L, tstart_next.get());
+      // The next stream's head bits, at its traversal starts (all below L).
+      set_head_bits(ctx, tstart_next.get(), n_start, heads_next.get(), L);
       hipLaunchKernelGGL(k_trav_longest, dim3((n_start + kBlock - 1) / kBlock), dim3(kBlock), 0,
                          ctx.stream(), tstart_next.get(), n_start, d_total, scal.get() + 2);
       GFZ_CHECK_LAUNCH();
