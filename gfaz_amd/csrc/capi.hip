@@ -11,6 +11,7 @@
 #include "coverage.hpp"
 #include "host/catalog.hpp"
 
+#include <optional>
 #include <cctype>
 #include <climits>
 #include <cstdio>
@@ -780,8 +781,14 @@ int gfz_resident_load(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_t *encod
 void gfz_resident_free(gfz_resident *r) {
   if (!r)
     return;
-  gfz::DeviceGuard g(r->owner->ctx->device());
-  (void)hipStreamSynchronize(r->owner->ctx->stream());
+  // Never throws: at process exit the runtime may be gone (the device can no
+  // longer be made current); then only the host side is released.
+  std::optional<gfz::DeviceGuard> g;
+  try {
+    g.emplace(r->owner->ctx->device());
+    (void)hipStreamSynchronize(r->owner->ctx->stream());
+  } catch (const gfz::Error &) {
+  }
   delete r;
 }
 
@@ -1084,8 +1091,14 @@ int gfz_coverage_build(gfz_ctx *ctx, const int32_t *nodes, uint64_t n_nodes,
 void gfz_coverage_free(gfz_coverage *cov) {
   if (!cov)
     return;
-  gfz::DeviceGuard g(cov->owner->ctx->device());
-  (void)hipStreamSynchronize(cov->owner->ctx->stream());
+  // Never throws: at process exit the runtime may be gone (the device can no
+  // longer be made current); then only the host side is released.
+  std::optional<gfz::DeviceGuard> g;
+  try {
+    g.emplace(cov->owner->ctx->device());
+    (void)hipStreamSynchronize(cov->owner->ctx->stream());
+  } catch (const gfz::Error &) {
+  }
   delete cov;
 }
 

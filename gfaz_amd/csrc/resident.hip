@@ -167,12 +167,9 @@ void launch_batch(Ctx &ctx, bool diff, const Resident &res, const SegDesc *segs,
 
 void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t n_enc,
                    const uint32_t *enc_lengths, uint64_t n_trav, int delta_round, Resident &res) {
-  if (!rb.has_bodies && rb.n_rules)
-    throw Error(kErrInvalid,
-                "resident decoding needs the rule bodies in HBM (this rulebook is expanded by "
-                "descent: its depths are not monotone in rule ID or its bodies exceed the budget)");
   res = Resident();
   res.rb = &rb;
+  res.descent = !rb.has_bodies && rb.n_rules;
   res.delta_round = delta_round < 0 ? 0 : delta_round;
   res.n_trav = n_trav;
   res.n_enc = n_enc;
@@ -191,7 +188,14 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
                                  std::to_string(total) + ", block holds " +
                                  std::to_string(n_enc) + ")");
   res.sym_off = DBuf<uint64_t>(ctx.pool(), n_enc + 1);
-  res.syms = DBuf<SymInfo>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+  if (res.descent) {
+    res.enc = DBuf<int32_t>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+    if (n_enc)
+      GFZ_HIP(hipMemcpyAsync(res.enc.get(), d_enc, n_enc * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+  } else {
+    res.syms = DBuf<SymInfo>(ctx.pool(), std::max<uint64_t>(n_enc, 1));
+  }
   const bool delta = res.delta_round > 0;
   {
     DBuf<uint32_t> sz(ctx.pool(), std::max<uint64_t>(n_enc, 1));
@@ -204,7 +208,7 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
       GFZ_CHECK_LAUNCH();
     }
     exclusive_scan(ctx, sz.get(), res.sym_off.get(), n_enc);
-    if (delta && n_enc) {
+    if (delta && n_enc && !res.descent) {
       const uint64_t w = ceil_div(n_enc, 32);
       DBuf<uint32_t> heads(ctx.pool(), w);
       fill_u32(ctx, heads.get(), 0, w);
@@ -219,7 +223,7 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
       segmented_prefix_sum(ctx, incl.get(), n_enc, heads.get());
       ctx.sync();  // `starts` (pageable source) and the scratch buffers go out of scope
     }
-    if (n_enc) {
+    if (n_enc && !res.descent) {
       hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_enc)), dim3(kBlock), 0, ctx.stream(),
                          d_enc, n_enc, incl.get(), ssum.get(), delta, rb.szo.get(),
                          rb.n_rules, rb.min_id, rb.z_raw, res.syms.get());
@@ -239,6 +243,8 @@ void resident_load(Ctx &ctx, const Rulebook &rb, const int32_t *d_enc, uint64_t 
     res.dec_start[t] = h_off[res.enc_start[t]];
     res.dec_len[t] = h_off[res.enc_start[t] + res.enc_len[t]] - res.dec_start[t];
   }
+  if (res.descent)
+    return;  // no coarse index: extraction decodes whole traversals by descent
   // Coarse index; the last entry is a sentinel for ranges that end the stream.
   res.n_cells = ceil_div(std::max<uint64_t>(res.n_out, 1), kGrid) + 1;
   res.grid = DBuf<uint64_t>(ctx.pool(), res.n_cells);
@@ -285,10 +291,55 @@ size_t extract_meta_bytes(const ExtractPlan &plan) {
          plan.seg_dst.size() * 8;
 }
 
+namespace {
+
+// Extraction from a store without rule bodies: the planned traversals' encoded
+// symbols are gathered in output order (one copy per run of consecutive stored
+// traversals) and decoded by the descent expansion, inverse deltas included.
+void descent_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, int32_t *d_out) {
+  std::vector<uint32_t> lens(plan.segs.size());
+  uint64_t n_sub = 0;
+  for (size_t i = 0; i < plan.segs.size(); ++i) {
+    lens[i] = (uint32_t)(plan.segs[i].k_hi - plan.segs[i].k_lo);
+    n_sub += lens[i];
+  }
+  DBuf<int32_t> sub(ctx.pool(), std::max<uint64_t>(n_sub, 1));
+  uint64_t at = 0;
+  for (size_t i = 0; i < plan.segs.size();) {
+    const uint64_t lo = plan.segs[i].k_lo;
+    uint64_t hi = plan.segs[i].k_hi;
+    size_t j = i + 1;
+    while (j < plan.segs.size() && plan.segs[j].k_lo == hi)
+      hi = plan.segs[j++].k_hi;
+    if (hi > lo)
+      GFZ_HIP(hipMemcpyAsync(sub.get() + at, res.enc.get() + lo, (hi - lo) * 4,
+                             hipMemcpyDeviceToDevice, ctx.stream()));
+    at += hi - lo;
+    i = j;
+  }
+  DecodePlan dp;
+  DBuf<uint64_t> sym_off;
+  decode_plan(ctx, *res.rb, sub.get(), n_sub, lens.data(), nullptr, lens.size(), res.delta_round,
+              dp, sym_off);
+  if (dp.n_out != plan.total)
+    throw Error(kErrCorrupt, "descent extraction decoded " + std::to_string(dp.n_out) +
+                                 " symbols where the store records " +
+                                 std::to_string(plan.total));
+  decode_run(ctx, *res.rb, sub.get(), n_sub, lens.data(), lens.size(), res.delta_round, dp,
+             sym_off.get(), d_out);
+  ctx.add_bytes("extract", 4.0 * plan.total);
+}
+
+}  // namespace
+
 void extract_enqueue(Ctx &ctx, const Resident &res, const ExtractPlan &plan, uint8_t *h_stage,
                      uint8_t *d_meta, int32_t *d_out) {
   if (!plan.total)
     return;
+  if (res.descent) {
+    descent_enqueue(ctx, res, plan, d_out);
+    return;
+  }
   const uint64_t n_segs = plan.segs.size(), total_tiles = plan.tile_prefix.back();
   const size_t seg_bytes = n_segs * sizeof(SegDesc), pre_bytes = (n_segs + 1) * 8;
   const size_t dst_off = seg_bytes + pre_bytes + 8;
@@ -384,6 +435,9 @@ uint8_t *noalloc_scratch(int device) {
 uint64_t resident_extract_noalloc(Ctx &ctx, const Resident &res, const uint64_t *ids,
                                   uint64_t n_ids, int32_t *d_out, uint64_t out_cap,
                                   uint64_t *offsets, uint32_t items) {
+  if (res.descent)
+    throw Error(kErrInvalid, "extraction without allocation needs the rule bodies in HBM (this "
+                             "store's rulebook is expanded by descent)");
   if (res.delta_round >= 2)
     throw Error(kErrInvalid, "extraction without allocation supports delta_round 0 and 1 "
                              "(this store has " + std::to_string(res.delta_round) + ")");
@@ -443,7 +497,14 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
     throw Error(kErrInvalid, "appended lengths sum " + std::to_string(total) + " != " +
                                  std::to_string(n_new) + " symbols");
   const uint64_t e0 = res.n_enc, out0 = res.n_out;
-  grow(ctx, res.syms, e0, e0 + n_new);
+  if (res.descent) {
+    grow(ctx, res.enc, e0, e0 + n_new);
+    if (n_new)
+      GFZ_HIP(hipMemcpyAsync(res.enc.get() + e0, d_enc, n_new * 4, hipMemcpyDeviceToDevice,
+                             ctx.stream()));
+  } else {
+    grow(ctx, res.syms, e0, e0 + n_new);
+  }
   grow(ctx, res.sym_off, e0 + 1, e0 + n_new + 1);
   std::vector<uint64_t> h_off(n_new + 1, 0);
   if (n_new) {
@@ -459,7 +520,7 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
                        ctx.stream(), off.get(), n_new + 1, out0, res.sym_off.get() + e0);
     GFZ_CHECK_LAUNCH();
     const bool delta = res.delta_round > 0;
-    if (delta) {
+    if (delta && !res.descent) {
       const uint64_t w = ceil_div(n_new, 32);
       DBuf<uint32_t> heads(ctx.pool(), w);
       fill_u32(ctx, heads.get(), 0, w);
@@ -472,10 +533,12 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
       segmented_prefix_sum(ctx, incl.get(), n_new, heads.get());
       ctx.sync();  // `starts` is pageable
     }
-    hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0, ctx.stream(),
-                       d_enc, n_new, incl.get(), ssum.get(), delta, rb.szo.get(),
-                       rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
-    GFZ_CHECK_LAUNCH();
+    if (!res.descent) {
+      hipLaunchKernelGGL(k_sym_info_r, dim3(grid_for(ctx, n_new)), dim3(kBlock), 0,
+                         ctx.stream(), d_enc, n_new, incl.get(), ssum.get(), delta,
+                         rb.szo.get(), rb.n_rules, rb.min_id, rb.z_raw, res.syms.get() + e0);
+      GFZ_CHECK_LAUNCH();
+    }
     GFZ_HIP(hipMemcpyAsync(h_off.data(), off.get(), (n_new + 1) * 8, hipMemcpyDeviceToHost,
                            ctx.stream()));
     ctx.sync();
@@ -492,6 +555,8 @@ void resident_append(Ctx &ctx, Resident &res, const int32_t *d_enc, uint64_t n_n
   res.n_trav += n_trav_new;
   res.n_enc = e0 + n_new;
   res.n_out = out0 + h_off[n_new];
+  if (res.descent)
+    return;
   // Grid: cells from the old sentinel on are set again (sentinel, then the new symbols).
   const uint64_t old_cells = res.n_cells;
   res.n_cells = ceil_div(std::max<uint64_t>(res.n_out, 1), kGrid) + 1;

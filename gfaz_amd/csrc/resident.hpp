@@ -24,6 +24,12 @@ struct Resident {
   const Rulebook *rb = nullptr;  // not owned; must outlive the resident state
   int delta_round = 1;
   uint64_t n_trav = 0, n_enc = 0, n_out = 0;
+  // Rulebooks without bodies (depth not monotone in rule ID, or bodies over the
+  // budget): the encoded symbols stay in HBM and extraction gathers the
+  // requested traversals' symbols and expands them by descent; syms and grid
+  // stay empty.
+  bool descent = false;
+  DBuf<int32_t> enc;         // encoded symbols (descent only)
   DBuf<zx::SymInfo> syms;    // per encoded symbol
   DBuf<uint64_t> sym_off;    // n_enc + 1 decoded starts
   DBuf<uint64_t> grid;       // symbol covering each kGrid-th decoded position

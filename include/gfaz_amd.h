@@ -194,8 +194,9 @@ int gfz_decompress_traversals(gfz_ctx *ctx, const gfz_rulebook *rb, const int32_
  * selected traversals after a zstd decode of the whole column on every call); the
  * name or tuple -> index lookup and the P/W line text stay with the caller.
  * Traversal indices count paths first, then walks (the flattened order). The
- * rulebook must outlive the resident state and needs its bodies in HBM
- * (gfz_rulebook_info has_bodies; otherwise GFZ_E_INVALID). Errors carry the
+ * rulebook must outlive the resident state. Without its bodies in HBM
+ * (gfz_rulebook_info has_bodies = 0) the store keeps the encoded symbols and
+ * extracts by descending the rule tree (slower, same values). Errors carry the
  * reference's "Extraction workflow error: " prefix (extraction_workflow.cpp:15). */
 typedef struct gfz_resident gfz_resident;
 
@@ -229,7 +230,8 @@ int gfz_extract(gfz_ctx *ctx, const gfz_resident *r, const uint64_t *ids, uint64
  * (segment table, tile prefix, work-stealing counter) goes into a fixed device
  * array of GFZ_NOALLOC_SCRATCH_BYTES that the library holds per device until the
  * process ends; a request whose plan needs more (about 18 K non-empty traversals)
- * fails with GFZ_E_INVALID, and so does a store with delta_round >= 2. Calls on
+ * fails with GFZ_E_INVALID, and so does a store with delta_round >= 2 or one whose
+ * rulebook has no bodies in HBM (its descent plan allocates). Calls on
  * one device share that array and keep its last contents: calls that overlap on
  * one GPU race, so serialise them. Returns after the decode finished. */
 #define GFZ_NOALLOC_SCRATCH_BYTES (1u << 20)
@@ -311,7 +313,8 @@ void gfz_lines_free(gfz_lines *l);
  * the reference's kDefaultRollingOutputChunkBytes) into two alternating pinned
  * host buffers; the consumer is called in order on the calling thread, and runs
  * for chunk k while the GPU decodes chunk k + 1. A nonzero return from the
- * consumer stops the stream (GFZ_E_INVALID). Needs the rulebook bodies in HBM. */
+ * consumer stops the stream (GFZ_E_INVALID). Rulebooks without bodies in HBM
+ * decode each chunk by descent. */
 typedef struct gfz_chunk {
   const int32_t *values;      /* pinned host memory, valid during the call */
   uint64_t n_values;

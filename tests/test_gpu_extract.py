@@ -150,3 +150,52 @@ def test_resident_append_then_extract(gpu_ctx, delta):
     empty.close()
     res.close()
     rb.close()
+
+
+@pytest.mark.parametrize("delta", [0, 1, 2])
+def test_extract_append_stream_without_bodies(gpu_ctx, monkeypatch, delta):
+    """A rulebook without bodies in HBM (budget 0 here; in use, bodies past the body
+    budget or depths not monotone in rule ID) still backs a resident store: extraction
+    gathers the requested traversals' symbols and expands them by descent, appends
+    and streaming go through the same path, and every result equals the input
+    traversals. The no-alloc entry point refuses such a store cleanly."""
+    say(f"extract without bodies delta={delta}")
+    tr = synth.generate(synth.CONFIGS["plain_20k"])
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, delta, threads=8)
+    trav = _split(tr.nodes, tr.lengths)
+    enc = _split(ref["encoded"], ref["encoded_lengths"])
+    monkeypatch.setenv("GFAZ_BODY_BUDGET_MB", "0")
+    rb = codec.Rulebook(gpu_ctx, ref["rules_first"], ref["rules_second"], ref["start_id"])
+    assert not rb.info()["has_bodies"]
+    base = 9
+    res = codec.Resident(gpu_ctx, rb, np.concatenate(enc[:base]),
+                         ref["encoded_lengths"][:base], delta)
+    assert res.n_decoded == sum(t.size for t in trav[:base])
+    rng = np.random.default_rng(40 + delta)
+    _check_subset(res, trav[:base], np.arange(base))
+    _check_subset(res, trav[:base], rng.integers(0, base, 30))
+    _check_subset(res, trav[:base], [])
+    res.append(np.concatenate(enc[base:]), ref["encoded_lengths"][base:])
+    np.testing.assert_array_equal(res.lengths(), tr.lengths)
+    _check_subset(res, trav, np.arange(len(trav))[::-1])
+    _check_subset(res, trav, rng.integers(0, len(trav), 30))
+    out = codec.DeviceArray(gpu_ctx, 16, np.int32)
+    ids = np.array([0], np.uint64)
+    offs = np.zeros(2, np.uint64)
+    n = codec._u64()
+    rc = codec.lib().gfz_extract_noalloc(gpu_ctx.handle, res.handle, ids.ctypes.data, 1, out.ptr,
+                                         out.n, offs.ctypes.data, codec.ctypes.byref(n))
+    assert rc != 0 and b"descent" in codec.lib().gfz_last_error()
+    res.close()
+    parts, meta = [], []
+
+    def consume(values, b, e, expanded_begin, lengths):
+        parts.append(values.copy())
+        meta.append((b, e))
+        return False
+
+    codec.decompress_stream(gpu_ctx, rb, ref["encoded"], ref["encoded_lengths"], delta,
+                            256 << 10, consume)
+    assert len(parts) > 1 and meta[0][0] == 0 and meta[-1][1] == len(trav)
+    np.testing.assert_array_equal(np.concatenate(parts), tr.nodes)
+    rb.close()
