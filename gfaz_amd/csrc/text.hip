@@ -35,8 +35,10 @@ namespace {
 constexpr uint32_t kFmtItems = 8;                    // steps per thread
 constexpr uint32_t kFmtTile = kBlock * kFmtItems;    // steps per tile
 constexpr uint32_t kMaxStepBytes = 12;               // ',' + 10 digits + orientation
-constexpr uint32_t kParseItems = 8;                  // text bytes per thread
-constexpr uint32_t kParseTile = kBlock * kParseItems;
+constexpr uint32_t kParseItems = 32;                 // text bytes per thread
+constexpr uint32_t kParseTile = kBlock * kParseItems;  // 8 KB
+constexpr uint32_t kParseSub = 2048;                   // bytes whose tokens are listed at once
+constexpr uint32_t kParseSubs = kParseTile / kParseSub;
 
 __device__ inline uint32_t ndigits(uint32_t v) {
   uint32_t d = 1;
@@ -392,13 +394,16 @@ __device__ inline uint64_t stage_window(const uint8_t *txt, uint64_t n_bytes, ui
 // The tile's tokens, in byte order. Classification: thread t looks at bytes
 // t, t + kBlock, ... (a wave at 64 consecutive bytes per step, so its starts are
 // one ballot, stored as two bitmap words), recording the field of each start;
-// then the bitmap's set bits are listed by a block scan. Parsing then gives
-// every lane a token (a token is several bytes, most bytes start none).
+// then, a kParseSub stretch at a time, the bitmap's set bits are listed by a
+// block scan and parsed a lane per token (a token is several bytes, most bytes
+// start none). A block takes an 8 KB tile so that the dependent global reads
+// that open it (its first field, the field search, the window) are paid once
+// per 8 KB; the token list and the parsed IDs stay at one stretch's size.
 struct TokenLds {
   __align__(16) uint8_t win[kParseWin];
   uint32_t start_bits[kParseTile / 32];
   uint32_t end_bits[kParseWin / 32 + 1];  // field ends at window offsets 0 .. kParseWin
-  uint16_t list[kParseTile];              // token starts (tile offsets), byte order
+  uint16_t list[kParseSub];               // one stretch's token starts (tile offsets)
   uint32_t scan[kBlock / kWave + 1];
 };
 
@@ -422,7 +427,7 @@ __global__ void k_parse_first(const uint64_t *__restrict__ fb, uint64_t n_lines,
   }
 }
 
-__device__ inline uint32_t tile_tokens(int kind, const uint8_t *__restrict__ txt,
+__device__ inline void tile_classify(int kind, const uint8_t *__restrict__ txt,
                                        uint64_t n_bytes, const uint64_t *__restrict__ fb,
                                        const uint64_t *__restrict__ fe, uint64_t n_lines,
                                        const uint64_t *__restrict__ first, uint64_t w0,
@@ -462,13 +467,18 @@ __device__ inline uint32_t tile_tokens(int kind, const uint8_t *__restrict__ txt
     }
   }
   __syncthreads();
-  constexpr uint32_t kWords = kParseTile / 32;
+}
+
+// Lists the token starts of stretch s of the tile (tile offsets, byte order);
+// returns their count. The caller syncs before the next stretch refills the list.
+__device__ inline uint32_t sub_tokens(TokenLds &sm, uint32_t s) {
+  constexpr uint32_t kWords = kParseSub / 32;
   static_assert(kWords <= kBlock, "one bitmap word per thread");
-  const uint32_t word = threadIdx.x < kWords ? sm.start_bits[threadIdx.x] : 0u;
+  const uint32_t word = threadIdx.x < kWords ? sm.start_bits[s * kWords + threadIdx.x] : 0u;
   uint32_t total;
   uint32_t at = block_exclusive_scan<kBlock>((uint32_t)__popc(word), sm.scan, total);
   for (uint32_t bits = word; bits; bits &= bits - 1)
-    sm.list[at++] = (uint16_t)(threadIdx.x * 32 + __builtin_ctz(bits));
+    sm.list[at++] = (uint16_t)(s * kParseSub + threadIdx.x * 32 + __builtin_ctz(bits));
   __syncthreads();
   return total;
 }
@@ -491,16 +501,19 @@ __global__ __launch_bounds__(kBlock) void k_parse_count(int kind, const uint8_t 
     keep_bits[i] = 0;
   TextReader rd;
   FieldEnds fends;
-  const uint32_t n_tok =
-      tile_tokens(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
+  tile_classify(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
   uint32_t c = 0;
-  for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
-    const uint32_t r = sm.list[i];
-    int32_t id;
-    if (parse_token(kind, rd, fends, w0 + r, &id, err)) {
-      ++c;
-      atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
+  for (uint32_t s = 0; s < kParseSubs; ++s) {
+    const uint32_t n_tok = sub_tokens(sm, s);  // syncs
+    for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
+      const uint32_t r = sm.list[i];
+      int32_t id;
+      if (parse_token(kind, rd, fends, w0 + r, &id, err)) {
+        ++c;
+        atomicOr(&keep_bits[r >> 5], 1u << (r & 31));
+      }
     }
+    __syncthreads();  // the next stretch refills the list
   }
   uint32_t total;
   block_exclusive_scan<kBlock>(c, sm.scan, total);  // its barriers publish keep_bits
@@ -538,7 +551,7 @@ __global__ void k_parse_line_pos(const uint64_t *__restrict__ fb, uint64_t n_lin
 
 // Steps in byte order: tokens are parsed a lane each into LDS (the ID, or a
 // dropped mark, per list entry), then thread t writes the kept IDs of entries
-// [t * kParseItems, +kParseItems) at their rank in the tile (a block scan).
+// [t * kPer, +kPer) of each stretch at their rank in the tile (a block scan).
 constexpr int32_t kDropped = 0;  // no kept step has ID 0
 __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
@@ -550,29 +563,35 @@ __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t 
                                                         int32_t *__restrict__ nodes,
                                                         uint32_t *__restrict__ err) {
   __shared__ TokenLds sm;
-  __shared__ int32_t ids[kParseTile];
+  __shared__ int32_t ids[kParseSub];
   const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
   TextReader rd;
   FieldEnds fends;
-  const uint32_t n_tok =
-      tile_tokens(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
-  for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
-    const uint32_t r = sm.list[i];
-    int32_t id = kDropped;
-    if (!parse_token(kind, rd, fends, w0 + r, &id, err))
-      id = kDropped;
-    ids[i] = id;
+  tile_classify(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
+  constexpr uint32_t kPer = kParseSub / kBlock;  // list entries written per thread
+  uint64_t base = tile_pos[blockIdx.x];
+  for (uint32_t s = 0; s < kParseSubs; ++s) {
+    const uint32_t n_tok = sub_tokens(sm, s);  // syncs
+    for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
+      const uint32_t r = sm.list[i];
+      int32_t id = kDropped;
+      if (!parse_token(kind, rd, fends, w0 + r, &id, err))
+        id = kDropped;
+      ids[i] = id;
+    }
+    __syncthreads();
+    const uint32_t e0 = threadIdx.x * kPer;
+    uint32_t mine = 0;
+    for (uint32_t j = 0; j < kPer && e0 + j < n_tok; ++j)
+      mine += ids[e0 + j] != kDropped;
+    uint32_t total;
+    uint64_t o = base + block_exclusive_scan<kBlock>(mine, sm.scan, total);  // syncs
+    for (uint32_t j = 0; j < kPer && e0 + j < n_tok; ++j)
+      if (ids[e0 + j] != kDropped)
+        nodes[o++] = ids[e0 + j];
+    base += total;
+    __syncthreads();  // the next stretch refills the list and the IDs
   }
-  __syncthreads();
-  const uint32_t e0 = threadIdx.x * kParseItems;
-  uint32_t mine = 0;
-  for (uint32_t j = 0; j < kParseItems && e0 + j < n_tok; ++j)
-    mine += ids[e0 + j] != kDropped;
-  uint32_t total;
-  uint64_t o = tile_pos[blockIdx.x] + block_exclusive_scan<kBlock>(mine, sm.scan, total);
-  for (uint32_t j = 0; j < kParseItems && e0 + j < n_tok; ++j)
-    if (ids[e0 + j] != kDropped)
-      nodes[o++] = ids[e0 + j];
 }
 
 uint32_t grid1(uint64_t n, uint32_t per) {
