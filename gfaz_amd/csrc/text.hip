@@ -225,6 +225,20 @@ __global__ void k_fmt_heads(const uint8_t *__restrict__ heads_txt,
 
 // ---- parsing ---------------------------------------------------------------
 
+// Byte-wise tests on a 32-bit word of text, four bits out (bit k = byte k).
+// zero_bytes is exact (no borrow crosses bytes): 0x80 in each zero byte.
+__device__ inline uint32_t zero_bytes(uint32_t x) {
+  return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+__device__ inline uint32_t byte_bits(uint32_t z) {  // 0x80 per byte -> 4 bits
+  return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+// Token separators of a word: ',' for P; '<' or '>' (0x3C, 0x3E) for W.
+__device__ inline uint32_t sep_bits(int kind, uint32_t v) {
+  return kind == kTextP ? byte_bits(zero_bytes(v ^ 0x2C2C2C2Cu))
+                        : byte_bits(zero_bytes((v | 0x02020202u) ^ 0x3E3E3E3Eu));
+}
+
 // Bytes of the text as a kernel reads them: from an LDS window [w0, w0 + wn) of
 // the tile being parsed where it holds them, else from global memory.
 struct TextReader {
@@ -272,6 +286,63 @@ struct FieldEnds {
 __device__ inline int parse_token(int kind, const TextReader &txt, const FieldEnds &fends,
                                   uint64_t p, int32_t *id, uint32_t *err) {
   const bool pk = kind == kTextP;
+  // Fast path: a token ending within 12 bytes inside the window. Its 16 bytes
+  // come from five LDS words (byte-aligned by alignbyte), its end is the first
+  // separator or field end after p, and the name's leading digit run is summed
+  // in a fixed loop (at most 11 digits: exact in 64 bits, so overflow is a
+  // compare); every other token takes the byte loop below, with the same result.
+  const uint64_t o = p - fends.w0;
+  if (o + 16 <= fends.wn) {
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(txt.win) + (o >> 2);
+    const uint32_t sh = (uint32_t)(o & 3);
+    uint32_t raw[5];
+#pragma unroll
+    for (uint32_t k = 0; k < 5; ++k)
+      raw[k] = w32[k];
+    uint32_t d[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      d[k] = __builtin_amdgcn_alignbyte(raw[k + 1], raw[k], sh);
+    uint32_t sep = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      sep |= sep_bits(kind, d[k]) << (4 * k);
+    const uint64_t eb2 = (uint64_t)fends.bits[(o >> 5) + 1] << 32 | fends.bits[o >> 5];
+    const uint32_t ends = (uint32_t)(eb2 >> (o & 31)) & 0xFFFFu;
+    const uint32_t stop = (sep | ends) & ~1u;  // the token's own first byte never ends it
+    const uint32_t len = stop ? (uint32_t)__builtin_ctz(stop) : 32u;
+    if (len <= 12) {
+      const uint32_t name0 = pk ? 0u : 1u;   // the name: bytes [name0, name0 + name_len)
+      const uint32_t name_len = len - 1;
+      if (!name_len)
+        return 0;
+      uint64_t val = 0;
+      uint32_t ndig = 0;
+      bool run = true;
+#pragma unroll
+      for (uint32_t i = 0; i < 11; ++i) {
+        const uint32_t j = name0 + i;
+        const uint32_t dg = ((d[j >> 2] >> (8 * (j & 3))) & 0xFFu) - '0';
+        run = run && i < name_len && dg < 10u;
+        val = run ? val * 10 + dg : val;
+        ndig += run ? 1u : 0u;
+      }
+      if (val > 0xFFFFFFFFull) {
+        atomicOr(err, 2u);
+        return 0;
+      }
+      if (!ndig) {
+        atomicOr(err, 1u);
+        return 0;
+      }
+      if (val == 0)
+        return 0;
+      const uint32_t rb = pk ? len - 1 : 0u;  // the orientation byte
+      const bool rev = ((d[rb >> 2] >> (8 * (rb & 3))) & 0xFFu) == (pk ? '-' : '<');
+      *id = (int32_t)(rev ? 0u - (uint32_t)val : (uint32_t)val);
+      return 1;
+    }
+  }
   const uint8_t c0 = txt(p);
   uint32_t val = 0, ndig = 0;
   bool ovf = false, run = true;
@@ -427,45 +498,65 @@ __global__ void k_parse_first(const uint64_t *__restrict__ fb, uint64_t n_lines,
   }
 }
 
+// Token starts of the tile: thread t classifies the 32 bytes [32 t, 32 t + 32)
+// from two 16-byte LDS reads (start_bits word t). The fields meeting the chunk
+// (usually one) give its in-field and field-begin masks; a P start is an in-field
+// byte other than ',' that begins its field or follows a ','; a W start is an
+// in-field '<' or '>'. Every field with a byte in the tile has its end marked in
+// end_bits when the window holds it (by the thread holding its last byte, or by
+// the tile's last thread for a field that runs past the tile).
 __device__ inline void tile_classify(int kind, const uint8_t *__restrict__ txt,
-                                       uint64_t n_bytes, const uint64_t *__restrict__ fb,
-                                       const uint64_t *__restrict__ fe, uint64_t n_lines,
-                                       const uint64_t *__restrict__ first, uint64_t w0,
-                                       TokenLds &sm, TextReader &rd, FieldEnds &fends) {
-  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+                                     uint64_t n_bytes, const uint64_t *__restrict__ fb,
+                                     const uint64_t *__restrict__ fe, uint64_t n_lines,
+                                     const uint64_t *__restrict__ first, uint64_t w0,
+                                     TokenLds &sm, TextReader &rd, FieldEnds &fends) {
   const uint64_t f0 = first[blockIdx.x], f1 = first[blockIdx.x + 1];
   for (uint32_t i = threadIdx.x; i < kParseWin / 32 + 1; i += kBlock)
     sm.end_bits[i] = 0;
   rd = TextReader{txt, sm.win, w0, stage_window(txt, n_bytes, w0, sm.win)};  // syncs
   fends = FieldEnds{sm.end_bits, w0, rd.wn, fb, fe, n_lines};
-  FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
-  fc.seek(w0 + threadIdx.x < n_bytes ? w0 + threadIdx.x : w0, f0 ? f0 - 1 : 0,
-          f1 > f0 ? f1 : f0 + 1 < n_lines ? f0 + 1 : n_lines);
-  uint64_t marked = ~0ull;  // the last field whose end this thread marked
-  for (uint32_t j = 0; j < kParseItems; ++j) {
-    const uint32_t r = j * kBlock + threadIdx.x;
-    const uint64_t p = w0 + r;
-    bool st = false;
-    if (p < n_bytes) {
-      fc.advance(p);
-      if (p >= fc.b && p < fc.e) {
-        const uint8_t c = rd(p);
-        st = kind == kTextP ? c != ',' && (p == fc.b || rd(p - 1) == ',')
-                            : c == '<' || c == '>';
-        if (fc.l != marked && fc.e - w0 <= rd.wn) {
+  static_assert(kParseTile == 32 * kBlock, "one 32-byte chunk per thread");
+  const uint32_t r0 = threadIdx.x * 32;
+  const uint64_t p0 = w0 + r0, p1 = p0 + 32;
+  const bool last_chunk = threadIdx.x == kBlock - 1;
+  uint32_t in = 0, beg = 0;
+  if (p0 < n_bytes) {
+    FieldCursor fc{fb, fe, n_lines, 0, 0, 0, 0};
+    fc.seek(p0, f0 ? f0 - 1 : 0, f1 > f0 ? f1 : f0 + 1 < n_lines ? f0 + 1 : n_lines);
+    for (;;) {
+      if (fc.b < p1 && fc.e > p0 && fc.e > fc.b) {
+        const uint32_t lo = fc.b > p0 ? (uint32_t)(fc.b - p0) : 0u;
+        const uint32_t hi = fc.e < p1 ? (uint32_t)(fc.e - p0) : 32u;
+        in |= (hi == 32 ? ~0u : (1u << hi) - 1u) & ~((1u << lo) - 1u);
+        if (fc.b >= p0)
+          beg |= 1u << lo;
+        if ((fc.e <= p1 || last_chunk) && fc.e - w0 <= rd.wn) {
           const uint64_t o = fc.e - w0;
           atomicOr(&sm.end_bits[o >> 5], 1u << (o & 31));
-          marked = fc.l;
         }
       }
-    }
-    const uint64_t m = __ballot(st);
-    if (lane == 0) {
-      const uint32_t w = (j * kBlock + wid * kWave) / 32;
-      sm.start_bits[w] = (uint32_t)m;
-      sm.start_bits[w + 1] = (uint32_t)(m >> 32);
+      if (fc.nb >= p1)
+        break;
+      fc.advance(fc.nb);
     }
   }
+  uint32_t word = 0;
+  if (in) {
+    const uint4 *w4 = reinterpret_cast<const uint4 *>(sm.win + r0);  // 16-byte aligned
+    const uint4 a = w4[0], b = w4[1];
+    const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t sep = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+      sep |= sep_bits(kind, v[k]) << (4 * k);
+    if (kind == kTextP) {
+      const uint32_t prev = (p0 > 0 && rd(p0 - 1) == ',') ? 1u : 0u;
+      word = in & ~sep & ((sep << 1) | prev | beg);
+    } else {
+      word = in & sep;
+    }
+  }
+  sm.start_bits[threadIdx.x] = word;
   __syncthreads();
 }
 

@@ -167,3 +167,38 @@ def test_parse_fields_inside_whole_lines(gpu_ctx):
     nodes, got_lens = codec.parse_steps(gpu_ctx, to.P, text, fields=(begin, end))
     np.testing.assert_array_equal(got_lens, lens)
     np.testing.assert_array_equal(nodes, tr.nodes)
+
+
+def test_edge_tokens_inside_long_fields(gpu_ctx):
+    """Edge tokens (10 and 11 digits, leading zeros, trailing letters, empty names,
+    zero, the largest name) placed among thousands of ordinary ones, so that tokens
+    of every length from 1 to 14 bytes sit at every offset of the parse tiles' LDS
+    windows: the GPU parse equals the CPU restatement, and malformed names deep in a
+    field raise the reference's errors."""
+    say("text edge tokens in long fields")
+    rng = np.random.default_rng(11)
+    p_toks = [b"1+", b"9-", b"4294967295+", b"4294967295-", b"0+", b"+", b"12ab+", b"007-",
+              b"2147483648-", b"1234567890+", b"01234567890-", b"001234567890+",
+              b"0001234567890-", b"55x9+", b"3"]
+    w_toks = [b">1", b"<9", b">4294967295", b"<4294967295", b"<0", b">", b"<12ab", b">007",
+              b"<2147483648", b">1234567890", b"<01234567890", b">001234567890",
+              b"<0001234567890", b">55x9"]
+    for kind, toks, sep in ((to.P, p_toks, b","), (to.W, w_toks, b"")):
+        fields = []
+        for n in (3000, 1, 0, 777, 5000):
+            pick = rng.integers(0, len(toks), n)
+            fields.append(sep.join(toks[i] for i in pick))
+        loff = np.concatenate([[0], np.cumsum([len(f) for f in fields])])
+        want_nodes, want_len = to.parse_steps(kind, b"".join(fields), loff)
+        nodes, lens = codec.parse_steps(gpu_ctx, kind, b"".join(fields), loff)
+        np.testing.assert_array_equal(lens, want_len)
+        np.testing.assert_array_equal(nodes, want_nodes)
+    for kind, bad, msg in ((to.P, b"x+", "no digits"), (to.P, b"4294967296+", "overflow"),
+                           (to.P, b"99999999999-", "overflow"), (to.W, b">x", "no digits"),
+                           (to.W, b"<4294967296", "overflow"), (to.W, b">99999999999", "overflow")):
+        ok = b"12+," * 700 if kind == to.P else b">12" * 700
+        text = ok + bad + (b"," if kind == to.P else b"") + ok
+        with pytest.raises(to.ParseError, match=msg):
+            to.parse_steps(kind, text, [0, len(text)])
+        with pytest.raises(RuntimeError, match=msg):
+            codec.parse_steps(gpu_ctx, kind, text, [0, len(text)])
