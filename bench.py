@@ -801,6 +801,13 @@ def main() -> int:
         avg = exp["total_ms"] / exp["launches"]
         a = exp_bytes / (avg * 1e-3) / 1e9
         expand_roof = {"kernel": "expand", "achieved": a, "frac": a / HBM_PEAK_GBS, "avg_ms": avg}
+        # The HBM bytes the expansion really moves (counter summary: the Z reads of
+        # every decoded symbol on top of the algorithmic bytes) over its event time.
+        xt = measured_traffic(cfg.name, "expand", 1)
+        if xt:
+            expand_roof["traffic"] = xt
+            expand_roof["traffic_gbps"] = xt / (avg * 1e-3) / 1e9
+            expand_roof["traffic_frac"] = expand_roof["traffic_gbps"] / HBM_PEAK_GBS
 
     steps = args.steps
     ms_step = t_total / steps * 1e3

@@ -50,9 +50,14 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
                                                       const int32_t *__restrict__ s, uint64_t R,
                                                       uint32_t min_id, RuleRec *__restrict__ rec,
                                                       uint2 *__restrict__ rss,
-                                                      uint32_t *__restrict__ depth) {
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < R;
-       r += (uint64_t)gridDim.x * kBlock) {
+                                                      uint32_t *__restrict__ depth,
+                                                      uint32_t *lo_out) {
+  // The first non-leaf rule goes to *lo_out (one atomic per wave at most).
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kBlock; r0 < R; r0 += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t r = r0 + threadIdx.x;
+    if (r >= R)
+      continue;  // a wave's lane 0 holds its first rule, so lane 0 stays active below
     const int32_t a = f[r], b = s[r];
     RuleRec q;
     q.a = a;
@@ -63,6 +68,12 @@ __global__ __launch_bounds__(kBlock) void k_rule_init(const int32_t *__restrict_
     const bool leaf = q.sa && q.sb;
     rss[r] = leaf ? make_uint2(2u, (uint32_t)a + (uint32_t)b) : make_uint2(0u, 0u);
     depth[r] = leaf ? 1u : 0u;
+    const uint64_t inner = __ballot(!leaf);
+    if (inner && lane == 0) {
+      const uint32_t first = (uint32_t)(r + (uint32_t)__ffsll((unsigned long long)inner) - 1);
+      if (first < __atomic_load_n(lo_out, __ATOMIC_RELAXED))
+        atomicMin(lo_out, first);
+    }
   }
 }
 
@@ -82,17 +93,25 @@ __global__ __launch_bounds__(kBlock) void k_rule_szo(const uint2 *__restrict__ r
 // rule resolved before this pass (depth <= k + 1); it then gets depth k + 2, its
 // size and its sum. Children resolved during the same pass are not used, so
 // every depth is exact whatever the thread order. `changed` is raised once per
-// wave.
+// wave. The pass starts at *lo_in, the first rule left unresolved by the pass
+// before (every rule below it has its depth), and leaves in *lo_out the first
+// rule it left unresolved: rule IDs grow with depth, so the passes scan the
+// shrinking suffix of deeper rules instead of the whole rulebook.
 __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
                                                       uint2 *__restrict__ rss,
                                                       uint32_t *__restrict__ depth, uint64_t R,
                                                       uint32_t min_id, uint32_t pass,
-                                                      uint32_t *changed, uint32_t *err) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  for (uint64_t r0 = (uint64_t)blockIdx.x * kBlock; r0 < R; r0 += (uint64_t)gridDim.x * kBlock) {
+                                                      uint32_t *changed, uint32_t *err,
+                                                      const uint32_t *__restrict__ lo_in,
+                                                      uint32_t *lo_out) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const uint64_t start = (uint64_t)(*lo_in & ~(kBlock - 1));
+  for (uint64_t r0 = start + (uint64_t)blockIdx.x * kBlock; r0 < R;
+       r0 += (uint64_t)gridDim.x * kBlock) {
     const uint64_t r = r0 + threadIdx.x;
     bool resolved = false;
-    if (r < R && !depth[r]) {
+    const bool pending = r < R && !depth[r];
+    if (pending) {
       RuleRec q = rec[r];
       const bool ra = q.sa != 0, rb = q.sb != 0;  // node-ID children
       bool ok = true;
@@ -134,6 +153,12 @@ __global__ __launch_bounds__(kBlock) void k_rule_pass(RuleRec *__restrict__ rec,
     const uint64_t m = __ballot(resolved);
     if (m && lane == (uint32_t)__ffsll((unsigned long long)m) - 1)
       *changed = 1u;
+    const uint64_t left = __ballot(pending && !resolved);
+    if (left && lane == 0) {
+      const uint32_t first = (uint32_t)(r0 + wid * kWave + (uint32_t)__ffsll((unsigned long long)left) - 1);
+      if (first < __atomic_load_n(lo_out, __ATOMIC_RELAXED))
+        atomicMin(lo_out, first);
+    }
   }
 }
 
@@ -1009,20 +1034,29 @@ void rulebook_build(Ctx &ctx, const int32_t *d_first, const int32_t *d_second, u
   constexpr uint32_t kPassBatch = 8;
   DBuf<uint32_t> flags(ctx.pool(), kPassBatch + 4);  // changed per pass, err, bad, pad
   uint32_t *err = flags.get() + kPassBatch, *bad = flags.get() + kPassBatch + 1;
+  // First unresolved rule before each pass of a batch (lows[0] carries the last
+  // batch's lows[kPassBatch]); the first pass starts at the first non-leaf rule.
+  DBuf<uint32_t> lows(ctx.pool(), kPassBatch + 1);
+  fill_u32(ctx, lows.get(), ~0u, 1);
   hipLaunchKernelGGL(k_rule_init, dim3(stride_grid(ctx, R)), dim3(kBlock), 0, ctx.stream(),
-                     d_first, d_second, R, min_id, rb.rec.get(), rb.ss.get(), depth.get());
+                     d_first, d_second, R, min_id, rb.rec.get(), rb.ss.get(), depth.get(),
+                     lows.get());
   GFZ_CHECK_LAUNCH();
   uint32_t max_depth = 1;
   bool converged = false;
   for (uint64_t pass0 = 0; !converged && pass0 <= R; pass0 += kPassBatch) {
     fill_u32(ctx, flags.get(), 0, kPassBatch + 4);
+    fill_u32(ctx, lows.get() + 1, ~0u, kPassBatch);
     ctx.timed("rule_sizes", [&] {
       for (uint32_t k = 0; k < kPassBatch; ++k)
         hipLaunchKernelGGL(k_rule_pass, dim3(stride_grid(ctx, R)), dim3(kBlock), 0,
                            ctx.stream(), rb.rec.get(), rb.ss.get(), depth.get(),
-                           R, min_id, (uint32_t)(pass0 + k), flags.get() + k, err);
+                           R, min_id, (uint32_t)(pass0 + k), flags.get() + k, err,
+                           lows.get() + k, lows.get() + k + 1);
     });
     GFZ_CHECK_LAUNCH();
+    GFZ_HIP(hipMemcpyAsync(lows.get(), lows.get() + kPassBatch, 4, hipMemcpyDeviceToDevice,
+                           ctx.stream()));
     const uint32_t *h =
         reinterpret_cast<const uint32_t *>(ctx.fetch({{flags.get(), (kPassBatch + 4) / 2, 0}}));
     if (h[kPassBatch])
