@@ -640,9 +640,11 @@ __global__ void k_parse_line_pos(const uint64_t *__restrict__ fb, uint64_t n_lin
   lpos[l] = t < n_tiles ? tile_pos[t] + partial[l] : tile_pos[n_tiles];
 }
 
-// Steps in byte order: tokens are parsed a lane each into LDS (the ID, or a
-// dropped mark, per list entry), then thread t writes the kept IDs of entries
-// [t * kPer, +kPer) of each stretch at their rank in the tile (a block scan).
+// Steps in byte order: 2 KB of the tile at a time, tokens are parsed a lane each
+// (token i of the stretch by thread i % kBlock), a ballot per wave records the
+// kept ones as bits in list order, one wave turns the 64 bitmap words into
+// prefix counts, and each lane stores its kept ID at the tile's position plus its
+// rank (consecutive lanes hold consecutive tokens, so the stores are contiguous).
 constexpr int32_t kDropped = 0;  // no kept step has ID 0
 __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t *__restrict__ txt,
                                                         uint64_t n_bytes,
@@ -653,35 +655,58 @@ __global__ __launch_bounds__(kBlock) void k_parse_write(int kind, const uint8_t 
                                                         const uint64_t *__restrict__ tile_pos,
                                                         int32_t *__restrict__ nodes,
                                                         uint32_t *__restrict__ err) {
+  constexpr uint32_t kWords = kParseSub / 32;      // kept bits of a stretch
+  constexpr uint32_t kPer = kParseSub / kBlock;    // tokens per thread and stretch
+  static_assert(kWords == kWave, "one wave scans a stretch's kept words");
   __shared__ TokenLds sm;
-  __shared__ int32_t ids[kParseSub];
+  __shared__ uint32_t kbits[kWords];
+  __shared__ uint32_t kpre[kWords + 1];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const uint64_t w0 = (uint64_t)blockIdx.x * kParseTile;
   TextReader rd;
   FieldEnds fends;
   tile_classify(kind, txt, n_bytes, fb, fe, n_lines, first, w0, sm, rd, fends);  // syncs
-  constexpr uint32_t kPer = kParseSub / kBlock;  // list entries written per thread
   uint64_t base = tile_pos[blockIdx.x];
   for (uint32_t s = 0; s < kParseSubs; ++s) {
     const uint32_t n_tok = sub_tokens(sm, s);  // syncs
-    for (uint32_t i = threadIdx.x; i < n_tok; i += kBlock) {
-      const uint32_t r = sm.list[i];
-      int32_t id = kDropped;
-      if (!parse_token(kind, rd, fends, w0 + r, &id, err))
-        id = kDropped;
-      ids[i] = id;
+    int32_t id[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = k * kBlock + threadIdx.x;
+      id[k] = kDropped;
+      if (i < n_tok && !parse_token(kind, rd, fends, w0 + sm.list[i], &id[k], err))
+        id[k] = kDropped;
+      const uint64_t m = __ballot(id[k] != kDropped);
+      if (lane == 0) {
+        const uint32_t w = (k * kBlock + wid * kWave) / 32;
+        kbits[w] = (uint32_t)m;
+        kbits[w + 1] = (uint32_t)(m >> 32);
+      }
     }
     __syncthreads();
-    const uint32_t e0 = threadIdx.x * kPer;
-    uint32_t mine = 0;
-    for (uint32_t j = 0; j < kPer && e0 + j < n_tok; ++j)
-      mine += ids[e0 + j] != kDropped;
-    uint32_t total;
-    uint64_t o = base + block_exclusive_scan<kBlock>(mine, sm.scan, total);  // syncs
-    for (uint32_t j = 0; j < kPer && e0 + j < n_tok; ++j)
-      if (ids[e0 + j] != kDropped)
-        nodes[o++] = ids[e0 + j];
-    base += total;
-    __syncthreads();  // the next stretch refills the list and the IDs
+    if (wid == 0) {
+      const uint32_t c = (uint32_t)__popc(kbits[lane]);
+      uint32_t incl = c;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d);
+        if (lane >= (uint32_t)d)
+          incl += y;
+      }
+      kpre[lane] = incl - c;
+      if (lane == kWave - 1)
+        kpre[kWave] = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = k * kBlock + threadIdx.x;
+      if (id[k] != kDropped)
+        nodes[base + kpre[i >> 5] + (uint32_t)__popc(kbits[i >> 5] & ((1u << (i & 31)) - 1u))] =
+            id[k];
+    }
+    base += kpre[kWords];
+    __syncthreads();  // the next stretch refills the list and the kept bits
   }
 }
 
