@@ -305,3 +305,45 @@ def test_deep_chain_rulebook(gpu_ctx):
             exp = np.cumsum(exp)
         np.testing.assert_array_equal(dec.astype(np.int64), (exp + 2**31) % 2**32 - 2**31)
     rb.close()
+
+
+def test_shuffled_rule_ids_decode_by_descent(gpu_ctx):
+    """A grammar whose rule IDs are shuffled (depth no longer grows with ID, legal for
+    the recursive CPU decoder) over tens of thousands of rules: the rule-size passes,
+    which start at the first rule the previous pass left unresolved, still resolve every
+    rule across many blocks, the rulebook falls back to descent, and the traversals
+    decode to the input and to the oracle's decode of the same rulebook."""
+    cfg = synth.CONFIGS["plain_500k"].scaled(backbone=60000, n_haps=24)
+    tr = synth.generate(cfg)
+    ref = oracle.compress(tr.nodes, tr.lengths, 8, 2, 1, threads=8)
+    R, lo = ref["rules_first"].size, ref["start_id"]
+    say(f"shuffled rule IDs: {R} rules")
+    assert R > 20000
+    perm = np.random.default_rng(5).permutation(R).astype(np.int64)
+
+    def remap(x):
+        x = x.astype(np.int64)
+        a = np.abs(x)
+        rule = (a >= lo) & (a < lo + R)
+        out = x.copy()
+        out[rule] = np.sign(x[rule]) * (lo + perm[a[rule] - lo])
+        return out.astype(np.int32)
+
+    rf = np.empty(R, np.int32)
+    rs = np.empty(R, np.int32)
+    rf[perm] = remap(ref["rules_first"])
+    rs[perm] = remap(ref["rules_second"])
+    enc = remap(ref["encoded"])
+    rb0 = codec.Rulebook(gpu_ctx, ref["rules_first"], ref["rules_second"], lo)
+    depth = rb0.info()["max_depth"]
+    rb0.close()
+    rb = codec.Rulebook(gpu_ctx, rf, rs, lo)
+    info = rb.info()
+    assert not info["has_bodies"] and info["max_depth"] == depth >= 4
+    dec, _, mism = codec.decompress_traversals(gpu_ctx, rb, enc, ref["encoded_lengths"],
+                                               tr.lengths, 1)
+    rb.close()
+    assert not mism
+    np.testing.assert_array_equal(dec, tr.nodes)
+    want = oracle.decompress(enc, ref["encoded_lengths"], rf, rs, lo, 1, tr.n_steps, threads=8)
+    np.testing.assert_array_equal(dec, want)
